@@ -1,0 +1,100 @@
+/*
+ * ysx.h -- C-ABI of the MI355X (gfx950) yspec spectral path, libysx.so.
+ *
+ * The reference has no plugin or FFI layer: the path sits behind Fortran
+ * subroutines and module globals called from its driver program.  These
+ * entry points replace those call sites one for one; a Fortran host binds
+ * them with iso_c_binding (interface in yspec_amd/fortran/ysx_bind.f90).
+ *
+ * Conventions
+ *  - every function returns 0 on success and a nonzero code otherwise; the
+ *    message is in ysx_last_error(), and carries the reference's own stop
+ *    text where the reference has one;
+ *  - all arrays are caller-owned host memory, column-major as the reference
+ *    stores them, complex numbers interleaved (re, im) as Fortran
+ *    complex(dpc);
+ *  - device memory, the HIP stream and every intermediate are owned by the
+ *    context; calls are synchronous; a context is not thread-safe;
+ *  - units in: those of the parameter file and deck (SI, mHz, minutes, s,
+ *    km, degrees, N m); units out: SI (src/yspec.f90:447-460).
+ */
+#ifndef YSX_H
+#define YSX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ysx_ctx ysx_ctx;
+
+#define YSX_OK 0
+#define YSX_ERR_ARG 1      /* bad argument or call order */
+#define YSX_ERR_INPUT 2    /* input rejected where the reference stops */
+#define YSX_ERR_HIP 3      /* HIP runtime error */
+#define YSX_ERR_UNSUPPORTED 4
+
+/* Create a context on HIP device `device`: stream, device buffers.
+ * Replaces the reference's module-global state (src/module_model.f90:118-160,
+ * src/module_int.f90:93-117, src/module_solver.f90:11). */
+int ysx_create(ysx_ctx **ctx, int device);
+void ysx_destroy(ysx_ctx *ctx);
+const char *ysx_last_error(const ysx_ctx *ctx);
+
+/* Load a deck model: nknot rows of 9 raw columns (r, rho, vpv, vsv, qkappa,
+ * qmu, vph, vsh, eta) as read from the deck file; ians = 0 uses the first 6
+ * (src/module_model.f90:214-221).  tref_period is the header's reference
+ * period in s.  Normalises, builds the per-layer splines and gravity and
+ * uploads the tables.  Replaces read_model's numerics and set_parameters
+ * (src/module_model.f90:175-383, 474-492). */
+int ysx_set_model(ysx_ctx *ctx, int nknot, int nic, int noc, int ians,
+                  double tref_period, const double *deck);
+
+/* Set the run: ip = lmin lmax ats grav out cor nr; fp = f1 f2 tout dt f11
+ * f12 f21 f22 src_depth src_lat src_lon Mrr Mrt Mrp Mtt Mtp Mpp rec_depth;
+ * lat/lon of the nr receivers.  Receiver geometry, Legendre tables, fcal,
+ * source/receiver layers; rejects what the reference stops on.
+ * Replaces src/yspec.f90:141-215 and src/module_fourier.f90:362-397 (fcal),
+ * src/module_solver.f90:1855-1861 (set_ats_switch). */
+int ysx_set_run(ysx_ctx *ctx, const int *ip, const double *fp,
+                const double *lat, const double *lon);
+
+/* iout = nw nt i1 i2 is ir nout npairs ; fout = rs rr dt tout ep df wmin dw
+ * (normalised units) */
+int ysx_run_info(const ysx_ctx *ctx, int *iout8, double *fout8);
+
+/* Spectra: every (l, w) pair of the run on the GPU, then the Legendre
+ * receiver sum.  ur/ut/up are (nw, nr) complex, or NULL to keep the result
+ * on the device only.  Replaces the l x w loop and its solver calls,
+ * src/yspec.f90:250-426 (tor_solver, sph_solver, rad_solver_wg,
+ * source_vector_*: src/module_solver.f90:25-215, 446-627, 640-823,
+ * 1747-1847). */
+int ysx_spectra(ysx_ctx *ctx, double *ur, double *ut, double *up);
+
+/* Seismograms from the device-resident spectra: cosine filters, Hermitian
+ * fill, inverse FFT, exp(eps t), units, rotation to Z/N/E.  seis is
+ * (nt, 3, nr) real (component order Z, N, E), or NULL.  Replaces
+ * ifft (src/module_fourier.f90:55-145) and src/yspec.f90:432-473. */
+int ysx_seismograms(ysx_ctx *ctx, double *seis);
+
+/* ysx_spectra followed by ysx_seismograms without host copies of the
+ * spectra. */
+int ysx_forward(ysx_ctx *ctx, double *seis);
+
+/* Per-pair records for n pairs (l[i], j[i]) (j 1-based), 89 complex each:
+ * svt(5,2) ytor(2,2) svs(5,6) ysph(6,4) sr0(2) yrad(2,2) uu(5) vv(5) ww(5),
+ * the values the reference's loop body holds after its source-vector and
+ * solver calls (src/yspec.f90:270-400); the pairs must have been computed
+ * by the last ysx_spectra. */
+int ysx_pair_records(ysx_ctx *ctx, int n, const int *l, const int *j,
+                     double *rec);
+
+/* Timings of the last ysx_spectra / ysx_seismograms, milliseconds, from
+ * HIP events on the context's stream: [0] source vectors + start levels,
+ * [1] toroidal, [2] spheroidal minors, [3] b-vector, [4] radial,
+ * [5] coefficient assembly, [6] Legendre sum, [7] IFFT + post. */
+int ysx_timings(const ysx_ctx *ctx, double *ms8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -226,7 +226,7 @@ contains
        if (l > 1) then
           call source_vector_tor(l, w, is, rs, mm, svt)
           call tor_solver(l, w, rr, rs, ir, is, s2, ytor)
-          call tor_start_level(l, real(w), is, 1, itor)
+          call tor_start_level(l, real(w), merge(is, ir, rr >= rs), 1, itor)
           ytor_raw = ytor
           if (out_switch == 1) ytor = ii*w*ytor
           if (out_switch == 2) ytor = -w*w*ytor
@@ -244,7 +244,7 @@ contains
        call source_vector_sph(l, w, is, rs, mm, svs)
        if (fic .or. grav_switch /= 2) stop 'ref_dump: only grav=2 with a solid inner core is dumped'
        call sph_solver(l, w, rr, rs, ir, is, s6, ysph)
-       call sph_start_level(l, real(w), is, 1, isph)
+       call sph_start_level(l, real(w), merge(is, ir, rr >= rs), 1, isph)
        ysph_raw = ysph
        if (out_switch == 1) then
           ysph(1:3,:) = ii*w*ysph(1:3,:)

@@ -1,0 +1,123 @@
+"""Write the golden fixtures of tests/golden/ from the reference itself.
+
+Runs oracle/_ref/ref_dump (the unmodified reference modules, built from the
+reference tree by ``make -C oracle ref``) on small seeded cases and stores
+FP64 results as compressed .npz files:
+
+    case_<name>.npz : deck columns + parameter block (the inputs), model
+                      arrays, geometry tables, a spread subset of the per-pair
+                      records, the full spectra and seismograms.
+
+Cases cover: attenuation off/on, shallow and deep sources, a receiver below
+the source, an ocean layer, a transversely isotropic (ians=1) deck, and the
+three output types with the potential/tilt correction.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [ROOT, HERE]
+
+from yspec_amd.inputs import Deck  # noqa: E402
+from yspec_amd.synth import synth_deck, workload  # noqa: E402
+from refdump import read_dump, MODEL_NAMES  # noqa: E402
+
+REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+
+
+def ocean_deck(seed=0):
+    """synthetic deck with a 3 km ocean (2 knots, vs = 0) on top"""
+    d = synth_deck(seed)
+    cols = d.cols.copy()
+    top = cols[-1].copy()
+    cols[-1, 0] = 6368000.0  # crust now ends 3 km lower
+    oc1 = np.array([6368000.0, 1020.0, 1450.0, 0.0, 57823.0, 0.0, 1450.0, 0.0, 1.0])
+    oc2 = oc1.copy()
+    oc2[0] = 6371000.0
+    cols[-6:-1, 0] = np.linspace(6347000.0, 6368000.0, 6)[:-1]
+    cols = np.vstack([cols, oc1, oc2])
+    return Deck("synth.ocean", 0, 1.0, len(cols), d.nic, d.noc, cols)
+
+
+def aniso_deck(seed=0):
+    """ians = 1 variant: vph, vsh 1% above vpv, vsv in the mantle, eta 0.95"""
+    d = synth_deck(seed)
+    cols = d.cols.copy()
+    m = (cols[:, 0] > 3480000.0) & (cols[:, 3] > 0)
+    cols[m, 6] *= 1.01
+    cols[m, 7] *= 1.01
+    cols[m, 8] = 0.95
+    cols = np.array([[float(f"{v:.13g}") for v in row] for row in cols])
+    return Deck("synth.aniso", 1, 1.0, d.nknot, d.nic, d.noc, cols)
+
+
+def cases():
+    d0 = synth_deck(0)
+    yield "cfg1_small", d0, workload("cfg1", lmax=40, nr=3)
+    yield "att_deep", d0, workload("cfg1", lmax=30, nr=3, ats=1, src_depth=600.0, f2=6.0,
+                                   tout=40.0, dt=20.0, f21=4.0, f22=5.5)
+    yield "rec_below", d0, workload("cfg1", lmax=24, nr=2, ats=1, src_depth=20.0,
+                                    rec_depth=150.0, f2=5.0, tout=40.0, dt=20.0,
+                                    f21=3.0, f22=4.5)
+    yield "ocean_vel", ocean_deck(0), workload("cfg1", lmax=24, nr=2, ats=1, out=1,
+                                               src_depth=50.0, rec_depth=4.0, f2=5.0,
+                                               tout=40.0, dt=20.0, f21=3.0, f22=4.5)
+    yield "aniso_acc_cor", aniso_deck(0), workload("cfg1", lmax=24, nr=2, ats=1, out=2,
+                                                   cor=1, src_depth=300.0, f2=5.0,
+                                                   tout=40.0, dt=20.0, f21=3.0, f22=4.5)
+
+
+def run_case(name, deck, p, outdir, keep_pairs=240):
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "model.deck"), "w") as f:
+            f.write(deck.to_text())
+        p.model_file = "model.deck"
+        with open(os.path.join(td, "in.txt"), "w") as f:
+            f.write(p.to_text())
+        dd = os.path.join(td, "dump")
+        os.makedirs(dd)
+        subprocess.run([REF_DUMP, "in.txt", dd], cwd=td, check=True,
+                       stdout=subprocess.DEVNULL)
+        D = read_dump(dd)
+    P = D["pairs"]
+    n = len(P["l"])
+    sel = np.unique(np.linspace(0, n - 1, min(keep_pairs, n)).astype(int))
+    out = dict(
+        deck_cols=deck.cols, deck_head=np.array([deck.nknot, deck.nic, deck.noc, deck.ians]),
+        deck_tref=np.array(deck.tref), ip=p.int_block(), fp=p.float_block(),
+        lat=p.lat, lon=p.lon,
+        run_i=np.array([D[k] for k in ("nw", "nt", "i1", "i2", "is", "ir", "nsl")]),
+        run_f=np.array([D[k] for k in ("rs", "rr", "dt", "tout", "ep", "df", "wmin", "dw",
+                                       "f11", "f12", "f21", "f22")]),
+        model=D["model_stack"], delta=D["delta"], azep=D["azep"], azst=D["azst"],
+        xa=D["xa"], xpa=D["xpa"], xca=D["xca"],
+        pair_l=P["l"][sel], pair_j=P["j"][sel], tor_start=P["tor_start"][sel],
+        sph_start=P["sph_start"][sel],
+        ur=D["spec"][0], ut=D["spec"][1], up=D["spec"][2],
+        seis=np.stack([D["seis"][c].real for c in range(3)], axis=1), nout=np.array(D["nout"]))
+    for key in ("svt", "ytor", "svs", "ysph", "sr0", "yrad", "uu", "vv", "ww"):
+        out["pair_" + key] = P[key][sel]
+    path = os.path.join(outdir, f"case_{name}.npz")
+    np.savez_compressed(path, **out)
+    return path, os.path.getsize(path)
+
+
+def main():
+    if not os.path.exists(REF_DUMP):
+        sys.exit("build the reference first: make -C oracle ref")
+    for name, deck, p in cases():
+        path, size = run_case(name, deck, p, HERE)
+        print(f"{os.path.basename(path)}: {size / 1024:.1f} KiB")
+
+
+if __name__ == "__main__":
+    main()

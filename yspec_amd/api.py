@@ -1,0 +1,164 @@
+"""Python host over the C-ABI of libysx.so (include/ysx.h).
+
+Mirrors the reference driver's flow (src/yspec.f90): a model is loaded,
+a run is set, the (l, w) spectra are computed and summed over receivers,
+then the filtered inverse FFT gives the seismograms.  All compute is on
+the GPU; there is no CPU fallback -- loading fails loudly when the HIP
+library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libysx.so")
+_LIB = None
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+
+EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_set_run",
+           "ysx_run_info", "ysx_spectra", "ysx_seismograms", "ysx_forward",
+           "ysx_pair_records", "ysx_timings")
+
+PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
+               ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
+               ("uu", (5,)), ("vv", (5,)), ("ww", (5,)))
+
+
+class YSpecError(RuntimeError):
+    """A nonzero return code of the C-ABI, with its message."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+        self.msg = msg
+
+
+def lib():
+    """Load libysx.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run python -m yspec_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.ysx_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.ysx_destroy.argtypes = [vp]
+        L.ysx_destroy.restype = None
+        L.ysx_last_error.argtypes = [vp]
+        L.ysx_last_error.restype = C.c_char_p
+        L.ysx_set_model.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D]
+        L.ysx_set_run.argtypes = [vp, I, D, D, D]
+        L.ysx_run_info.argtypes = [vp, I, D]
+        L.ysx_spectra.argtypes = [vp, D, D, D]
+        L.ysx_seismograms.argtypes = [vp, D]
+        L.ysx_forward.argtypes = [vp, D]
+        L.ysx_pair_records.argtypes = [vp, C.c_int, I, I, D]
+        L.ysx_timings.argtypes = [vp, D]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(I)
+
+
+class YSpec:
+    """One GPU context: a deck model and a run (parameter block)."""
+
+    def __init__(self, deck, params=None, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        rc = self.L.ysx_create(C.byref(self.h), device)
+        if rc:
+            msg = self.L.ysx_last_error(self.h).decode() if self.h else ""
+            raise YSpecError(rc, msg or "ysx_create failed")
+        cols = np.ascontiguousarray(deck.cols, dtype=np.float64)
+        self._check(self.L.ysx_set_model(self.h, deck.nknot, deck.nic, deck.noc, deck.ians,
+                                         deck.tref, _dp(cols)))
+        self.deck = deck
+        if params is not None:
+            self.set_run(params)
+
+    def _check(self, rc):
+        if rc:
+            raise YSpecError(rc, self.L.ysx_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.ysx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_run(self, p):
+        ip = p.int_block()
+        fp = p.float_block()
+        lat = np.ascontiguousarray(p.lat, dtype=np.float64)
+        lon = np.ascontiguousarray(p.lon, dtype=np.float64)
+        self._check(self.L.ysx_set_run(self.h, _ip(ip), _dp(fp), _dp(lat), _dp(lon)))
+        self.p = p
+        io = np.zeros(8, np.int32)
+        fo = np.zeros(8)
+        self._check(self.L.ysx_run_info(self.h, _ip(io), _dp(fo)))
+        self.nw, self.nt, self.i1, self.i2, self.is_, self.ir, self.nout, self.npair = map(int, io)
+        (self.rs, self.rr, self.dt, self.tout, self.ep, self.df, self.wmin,
+         self.dw) = map(float, fo)
+
+    def spectra(self, copy=True):
+        """ur, ut, up: (nw, nr) complex (Fortran order), or None if copy=False."""
+        if not copy:
+            self._check(self.L.ysx_spectra(self.h, None, None, None))
+            return None
+        shp = (self.nw, self.p.nr)
+        out = [np.zeros(shp, np.complex128, order="F") for _ in range(3)]
+        self._check(self.L.ysx_spectra(self.h, *[a.ctypes.data_as(D) for a in out]))
+        return tuple(out)
+
+    def seismograms(self, copy=True):
+        """(nt, 3, nr) real: Z, N, E in SI units, rows t = i*dt."""
+        if not copy:
+            self._check(self.L.ysx_seismograms(self.h, None))
+            return None
+        seis = np.zeros((self.nt, 3, self.p.nr), order="F")
+        self._check(self.L.ysx_seismograms(self.h, _dp(seis)))
+        return seis
+
+    def forward(self, copy=True):
+        if not copy:
+            self._check(self.L.ysx_forward(self.h, None))
+            return None
+        seis = np.zeros((self.nt, 3, self.p.nr), order="F")
+        self._check(self.L.ysx_forward(self.h, _dp(seis)))
+        return seis
+
+    def pair_records(self, ls, js):
+        ls = np.ascontiguousarray(ls, dtype=np.int32)
+        js = np.ascontiguousarray(js, dtype=np.int32)
+        rec = np.zeros((len(ls), 89), np.complex128)
+        self._check(self.L.ysx_pair_records(self.h, len(ls), _ip(ls), _ip(js),
+                                            rec.ctypes.data_as(D)))
+        out, n = {}, 0
+        for name, shape in PAIR_FIELDS:
+            cnt = int(np.prod(shape))
+            out[name] = rec[:, n:n + cnt].reshape((len(ls),) + shape, order="F")
+            n += cnt
+        return out
+
+    def timings(self):
+        """ms: source, toroidal, minors, b-vector, radial, assemble, l-sum, IFFT"""
+        t = np.zeros(8)
+        self._check(self.L.ysx_timings(self.h, _dp(t)))
+        return t

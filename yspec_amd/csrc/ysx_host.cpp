@@ -1,0 +1,362 @@
+// ysx_host.cpp -- host set-up: deck -> normalised model tables, run
+// parameters, receiver geometry and Legendre tables.
+#include "ysx_host.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace ysx {
+
+namespace {
+
+constexpr double R_NORM = 6.371e+6;    // src/module_model.f90:18
+constexpr double RHO_NORM = 5.515e+3;  // src/module_model.f90:19
+
+// Natural cubic spline second derivatives on one layer (spline_r with
+// yp1 = ypn = 1e40 and tridag_ser_r, src/module_spline.f90:40-73,
+// 426-449).  x, y, y2 point at the layer's first knot; n knots.
+bool natural_spline(const double* x, const double* y, double* y2, int n) {
+  std::vector<double> sub(n + 1, 0.0), dia(n + 1, 0.0), sup(n + 1, 0.0),
+      rhs(n + 1, 0.0), gam(n + 1, 0.0);
+  // 1-based working arrays
+  for (int i = 1; i <= n - 1; i++) sup[i] = x[i] - x[i - 1];
+  for (int i = 1; i <= n - 1; i++) rhs[i] = 6.0 * ((y[i] - y[i - 1]) / sup[i]);
+  for (int i = n - 1; i >= 2; i--) rhs[i] -= rhs[i - 1];
+  for (int i = 2; i <= n - 1; i++) {
+    sub[i] = sup[i - 1];
+    dia[i] = 2.0 * (sup[i] + sub[i]);
+  }
+  dia[1] = dia[n] = 1.0;
+  rhs[1] = rhs[n] = 0.0;
+  sup[1] = 0.0;
+  sub[n] = 0.0;
+  // forward elimination / back substitution
+  double bet = dia[1];
+  if (bet == 0.0) return false;
+  y2[0] = rhs[1] / bet;
+  for (int j = 2; j <= n; j++) {
+    gam[j] = sup[j - 1] / bet;
+    bet = dia[j] - sub[j] * gam[j];
+    if (bet == 0.0) return false;
+    y2[j - 1] = (rhs[j] - sub[j] * y2[j - 2]) / bet;
+  }
+  for (int j = n - 1; j >= 1; j--) y2[j - 1] -= gam[j + 1] * y2[j];
+  return true;
+}
+
+// splint_dis_r, src/module_spline.f90:262-283 (0-based knot index klo)
+double splint(const double* xa, const double* ya, const double* y2a, double x, int klo) {
+  double h = xa[klo + 1] - xa[klo];
+  if (h == 0.0) return ya[klo];
+  double a = (xa[klo + 1] - x) / h, b = (x - xa[klo]) / h;
+  return a * ya[klo] + b * ya[klo + 1] +
+         ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[klo + 1]) * (h * h) / 6.0;
+}
+
+}  // namespace
+
+Norms make_norms() {
+  Norms N;
+  N.r_norm = R_NORM;
+  N.rho_norm = RHO_NORM;
+  N.vel_norm = R_NORM * std::sqrt(PI_D * BIGG * RHO_NORM);
+  N.acl_norm = PI_D * BIGG * RHO_NORM * R_NORM;
+  N.fre_norm = N.vel_norm / R_NORM;
+  N.t_norm = 1 / N.fre_norm;
+  N.grav_norm = PI_D * BIGG;
+  N.moment_norm = R_NORM * R_NORM * R_NORM * R_NORM * R_NORM * RHO_NORM * RHO_NORM * N.grav_norm;
+  N.pibigg = PI_D * BIGG / N.grav_norm;
+  return N;
+}
+
+// read_model numerics, src/module_model.f90:196-378
+std::string build_model(const Norms& N, int nknot, int nic, int noc, int ians,
+                        double tref_period, const double* deck, HostModel& M) {
+  if (nknot < 3 || nic < 1 || noc <= nic || noc >= nknot) return "bad deck header";
+  M = HostModel();
+  M.nknot = nknot;
+  M.nic = nic;
+  M.noc = noc;
+  M.ifanis = (ians == 1);
+  M.tref = TWOPI_D / tref_period / N.fre_norm;
+  M.tab.assign((size_t)F_COUNT * nknot, 0.0);
+  auto col = [&](int f) { return M.tab.data() + (size_t)f * nknot; };
+  double *r = col(F_R), *rho = col(F_RHO), *A = col(F_ACON), *Cc = col(F_CCON),
+         *F = col(F_FCON), *L = col(F_LCON), *Nn = col(F_NCON), *kap = col(F_KAPPA),
+         *mu = col(F_MU), *qk = col(F_QKAPPA), *qm = col(F_QMU), *xlam = col(F_XLAM),
+         *xa2 = col(F_XA2), *grav = col(F_GRAV), *alpha = col(F_ALPHA), *beta = col(F_BETA);
+  for (int i = 0; i < nknot; i++) {
+    const double* d = deck + 9 * (size_t)i;
+    r[i] = d[0];
+    rho[i] = d[1];
+    Cc[i] = d[2];
+    L[i] = d[3];
+    qk[i] = d[4];
+    qm[i] = d[5];
+    A[i] = M.ifanis ? d[6] : d[2];
+    Nn[i] = M.ifanis ? d[7] : d[3];
+    F[i] = M.ifanis ? d[8] : 1.0;
+  }
+  M.ocean = (L[nknot - 1] == 0.0);
+  M.nsl = nknot;
+  if (M.ocean)
+    for (int i = nknot - 1; i >= 1; i--)
+      if (L[i - 1] > 0.0) { M.nsl = i; break; }
+  M.fic = (L[0] == 0.0);
+  std::vector<double> rat(nknot);
+  for (int i = 0; i < nknot; i++) {
+    r[i] /= R_NORM;
+    rho[i] /= RHO_NORM;
+    double vA = A[i] / N.vel_norm, vC = Cc[i] / N.vel_norm, vL = L[i] / N.vel_norm,
+           vN = Nn[i] / N.vel_norm;
+    A[i] = rho[i] * (vA * vA);
+    Cc[i] = rho[i] * (vC * vC);
+    L[i] = rho[i] * (vL * vL);
+    Nn[i] = rho[i] * (vN * vN);
+    F[i] = (A[i] - 2.0 * L[i]) * F[i];
+    kap[i] = (Cc[i] + 4.0 * A[i] - 4.0 * Nn[i] + 4.0 * F[i]) / 9.0;
+    mu[i] = (Cc[i] + A[i] + 6.0 * L[i] + 5.0 * Nn[i] - 2.0 * F[i]) / 15.0;
+    double lam = kap[i] - 2.0 * mu[i] / 3.0;
+    beta[i] = std::sqrt(mu[i] / rho[i]);
+    alpha[i] = std::sqrt((kap[i] + 1.3333333333 * mu[i]) / rho[i]);
+    if (qk[i] > 0.0) qk[i] = 1.0 / qk[i];
+    if (qm[i] > 0.0) qm[i] = 1.0 / qm[i];
+    rat[i] = 4.0 * mu[i] / (3.0 * (lam + 2.0 * mu[i]));
+    xlam[i] = ((1.0 - rat[i]) * qk[i] - 0.5 * rat[i] * qm[i]) / (1.0 - 1.5 * rat[i]);
+    xa2[i] = (1.0 - rat[i]) * qk[i] + rat[i] * qm[i];
+  }
+  // layers split at duplicated radii (src/module_model.f90:294-308)
+  M.layer_lo = {1};
+  for (int i = 2; i <= nknot; i++)
+    if (r[i - 1] == r[i - 2]) {
+      M.layer_hi.push_back(i - 1);
+      M.layer_lo.push_back(i);
+    }
+  M.layer_hi.push_back(nknot);
+  M.nlayer = (int)M.layer_lo.size();
+  static const int fields[] = {F_RHO, F_ACON, F_CCON, F_FCON, F_LCON, F_NCON, F_KAPPA,
+                               F_MU, F_QKAPPA, F_QMU, F_XLAM, F_XA2};
+  for (int k = 0; k < M.nlayer; k++) {
+    int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
+    for (int f : fields)
+      if (!natural_spline(r + lo, col(f) + lo, col(f + 1) + lo, n))
+        return "tridag_ser: error at code stage 2";
+  }
+  // gravity: 5-point Gauss-Legendre on the spline of r^2 rho
+  // (gauslv, src/module_model.f90:356-371, 507-535)
+  std::vector<double> intg(nknot), qint(nknot);
+  for (int i = 0; i < nknot; i++) intg[i] = r[i] * r[i] * rho[i];
+  for (int k = 0; k < M.nlayer; k++) {
+    int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
+    natural_spline(r + lo, intg.data() + lo, qint.data() + lo, n);
+  }
+  static const double gw[2] = {.478628670499366, .236926885056189};
+  static const double gx[2] = {.538469310105683, .906179845938664};
+  grav[0] = 0.0;
+  double fint = 0.0;
+  for (int i = 0; i < nknot - 1; i++) {
+    double y1 = 0.5 * (r[i + 1] + r[i]), y2 = 0.5 * (r[i + 1] - r[i]);
+    double sum = .568888888888889 * splint(r, intg.data(), qint.data(), y1, i);
+    for (int g = 0; g < 2; g++) {
+      double t1 = gx[g] * y2;
+      sum += gw[g] * (splint(r, intg.data(), qint.data(), y1 + t1, i) +
+                      splint(r, intg.data(), qint.data(), y1 - t1, i));
+    }
+    fint += y2 * sum;
+    grav[i + 1] = fint;
+  }
+  for (int i = 0; i < nknot; i++)
+    if (r[i] > 0.0) grav[i] = (4.0 * grav[i]) / (r[i] * r[i]);
+  for (int k = 0; k < M.nlayer; k++) {
+    int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
+    natural_spline(r + lo, grav + lo, col(F_GRAV2) + lo, n);
+  }
+  return "";
+}
+
+// legendre, src/module_function.f90:6-132.  X_lm, dX_lm/dtheta and
+// X_lm/sin(theta) for orders 0..m; the reference takes sfl3, f1, f2 and
+// the second normalising square root in single precision.
+void legendre_m2(double theta, int l, int m, double* x, double* xp, double* xc) {
+  double ct = std::cos(theta), st = std::sin(theta);
+  double fct = std::sqrt((double)(float)(2 * l + 1) / FOURPI_D);
+  double sfl3 = (double)std::sqrt((float)(l * (l + 1)));
+  double compar = (double)(float)(2 * l + 1) / FOURPI_D;
+  double small = 1.0e-16 * compar;
+  for (int i = 0; i <= m; i++) x[i] = xp[i] = xc[i] = 0.0;
+  if (l <= 1 || std::fabs(theta) <= 1.0e-5) {
+    x[0] = fct;
+    if (l == 0) return;
+    x[0] = ct * fct;
+    x[1] = -st * fct / sfl3;
+    xp[0] = -st * fct;
+    xp[1] = -0.5 * ct * fct * sfl3;
+    xc[1] = (std::fabs(theta) < 1.0e-5) ? xp[1] : x[1] / st;
+    return;
+  }
+  double p0 = 1.0, p1 = ct;  // Legendre recurrence P_{i-2}, P_{i-1}
+  for (int i = 2; i <= l; i++) {
+    double p2 = ((double)(float)(2 * i - 1) * ct * p1 - (double)(float)(i - 1) * p0) /
+                (double)(float)i;
+    p0 = p1;
+    p1 = p2;
+  }
+  double cot = ct / st, cosec = 1.0 / st;
+  double x0 = p1 * fct;
+  double x1 = (double)(float)l * (p0 - ct * p1) * fct / st;
+  x[0] = x0;
+  xp[0] = -x1;
+  double xp1 = (double)(float)(l * (l + 1)) * x0 - cot * x1;
+  x[1] = -x1 / sfl3;
+  xc[1] = x[1] * cosec;
+  xp[1] = -xp1 / sfl3;
+  double sum = x0 * x0 + 2.0 * x[1] * x[1];
+  if (sum - compar > small) return;
+  double xa = x0, xb = -x1 / (double)std::sqrt((float)(l * (l + 1)));
+  for (int i = 3; i <= m + 1; i++) {
+    double f1 = (double)std::sqrt((float)(l + i - 1) * (float)(l - i + 2));
+    double f2 = (double)std::sqrt((float)(l + i - 2) * (float)(l - i + 3));
+    double xm = i - 1;
+    double x3 = -(2.0 * cot * (xm - 1.0) * xb + f2 * xa) / f1;
+    sum += 2.0 * x3 * x3;
+    if (sum - compar > small && i != l + 1) return;
+    x[i - 1] = x3;
+    xc[i - 1] = x3 * cosec;
+    xa = xb;
+    xp[i - 1] = -(f1 * xb + xm * cot * x3);
+    xb = x3;
+  }
+}
+
+// delaz, src/module_function.f90:317-390 (geocentric latitude, the
+// reference's rounded degree-to-radian factor)
+void delaz(double eplati, double eplong, double stlati, double stlong, double* delta,
+           double* azep, double* azst) {
+  const double geoco = 0.993277, hpi = 0.5 * PI_D, rad = 0.0174533;
+  double eplat = (eplati == -90.0) ? -89.98 : eplati;
+  double stlat = (stlati == -90.0) ? -89.98 : stlati;
+  if (eplat == stlat && eplong == stlong) {
+    *delta = *azep = *azst = 0.0;
+    return;
+  }
+  double el = hpi - std::atan(geoco * std::tan(eplat * rad));
+  double stl = hpi - std::atan(geoco * std::tan(stlat * rad));
+  double elon = eplong * rad, slon = stlong * rad;
+  double as = std::cos(stl), bs = std::sin(stl), cs = std::cos(slon), ds = std::sin(slon);
+  double a = std::cos(el), b = std::sin(el), c = std::cos(elon), d = std::sin(elon);
+  double cdel = a * as + b * bs * (c * cs + d * ds);
+  if (std::fabs(cdel) > 1.0) cdel = std::copysign(1.0, cdel);
+  double delt = std::acos(cdel), sdel = std::sin(delt);
+  double caze = (as - a * cdel) / (sdel * b);
+  if (std::fabs(caze) > 1.0) caze = std::copysign(1.0, caze);
+  double aze = std::acos(caze);
+  double cazs = 0.0;
+  if (bs > 0.0) cazs = (a - as * cdel) / (bs * sdel);
+  if (bs == 0.0) cazs = std::copysign(1.0, cazs);
+  if (std::fabs(cazs) > 1.0) cazs = std::copysign(1.0, cazs);
+  double azs = std::acos(cazs);
+  double dif = ds * c - cs * d;
+  if (dif < 0.0) aze = TWOPI_D - aze;
+  if (dif > 0.0) azs = TWOPI_D - azs;
+  *delta = delt;
+  *azep = aze;
+  *azst = azs;
+}
+
+// run set-up: src/yspec.f90:141-215 and fcal, src/module_fourier.f90:362-397
+std::string build_run(const Norms& N, const HostModel& M, const int* ip, const double* fp,
+                      const double* lat, const double* lon, HostRun& R) {
+  R = HostRun();
+  R.lmin = ip[0];
+  R.lmax = ip[1];
+  R.ats = ip[2];
+  R.grav = ip[3];
+  R.out = ip[4];
+  R.cor = ip[5];
+  R.nr = ip[6];
+  if (R.lmin < 0 || R.lmax < R.lmin || R.nr < 1) return "bad lmin/lmax/receiver count";
+  if (R.out < 0 || R.out > 2) return "output switch must be 0, 1 or 2";
+  if (R.grav != 2 || M.fic)
+    return "only full self-gravitation (2) with a solid inner core is implemented";
+  double f1 = fp[0], f2 = fp[1], tout = fp[2], dt = fp[3];
+  double sdep = fp[8], lats = fp[9], lons = fp[10], rdep = fp[17];
+  int nr = R.nr, L1 = R.lmax + 1;
+  R.delta.resize(nr);
+  R.azep.resize(nr);
+  R.azst.resize(nr);
+  R.eia.resize(nr);
+  R.xa.assign((size_t)L1 * 3 * nr, 0.0);
+  R.xpa.assign((size_t)L1 * 3 * nr, 0.0);
+  R.xca.assign((size_t)L1 * 3 * nr, 0.0);
+  if (lats == 90.0) lats = 89.98;
+  for (int k = 0; k < nr; k++) {
+    double dl, ae, as;
+    delaz(lat[k], lon[k], lats, lons, &dl, &ae, &as);
+    R.delta[k] = dl;
+    R.azep[k] = TWOPI_D - ae;
+    R.azst[k] = PI_D - as;
+    R.eia[k] = std::exp(std::complex<double>(0.0, R.azst[k]));
+    for (int l = 0; l <= R.lmax; l++) {
+      int m = l < 2 ? l : 2;
+      double x[3], xp[3], xc[3];
+      legendre_m2(R.delta[k], l, m, x, xp, xc);
+      for (int q = 0; q <= m; q++) {
+        size_t idx = (size_t)l + (size_t)L1 * (q + 3 * (size_t)k);
+        R.xa[idx] = x[q];
+        R.xpa[idx] = xp[q];
+        R.xca[idx] = xc[q];
+      }
+    }
+  }
+  // normalisation (src/yspec.f90:179-189)
+  double rtop = M.at(F_R, M.nknot);
+  f1 = f1 * N.t_norm / 1000.0;
+  f2 = f2 * N.t_norm / 1000.0;
+  R.dt = dt / N.t_norm;
+  R.rs = rtop - sdep * 1000.0 / R_NORM;
+  R.rr = rtop - rdep * 1000.0 / R_NORM;
+  R.tout = tout * 60.0 / N.t_norm;
+  for (int i = 0; i < 6; i++) R.mm[i] = fp[11 + i] / N.moment_norm;
+  R.f11 = fp[4] / 1000.0 * N.t_norm;
+  R.f12 = fp[5] / 1000.0 * N.t_norm;
+  R.f21 = fp[6] / 1000.0 * N.t_norm;
+  R.f22 = fp[7] / 1000.0 * N.t_norm;
+  // fcal with mex = 5, qex = 1; single-precision log as in the reference
+  if (0.5 / R.dt < f2) return "fmax above the Nyquist frequency";
+  R.ep = 5.0 / R.tout;
+  R.df = R.ep / TWOPI_D;
+  int nt = (int)(1.0 / (R.df * R.dt));
+  int ne = (int)(std::log((float)nt) / std::log(2.0f) + 1.0f);
+  if (ne > 24) return "time series too long";
+  R.nt = 1 << ne;
+  R.df = 1.0 / (R.nt * R.dt);
+  R.i1 = std::max((int)std::floor(f1 / R.df), 2);
+  R.i2 = (int)std::floor(f2 / R.df) + 2;
+  R.dw = TWOPI_D * R.df;
+  if (R.i1 == R.i2) return "too few frequency steps";
+  R.wmin = (R.i1 - 1) * R.dw;
+  R.nw = R.i2 - R.i1 + 1;
+  // find_radius_index(.., -1), src/module_model.f90:537-576
+  auto fri = [&](double x) {
+    if (x < 0.0 || x > 1.0) return 0;
+    for (int i = 1; i <= M.nknot - 1; i++) {
+      double a = M.at(F_R, i), b = M.at(F_R, i + 1);
+      if (x == a && x == b) return i;
+      if (x > a && x <= b) return i;
+    }
+    return 0;
+  };
+  R.is = fri(R.rs);
+  if (R.is == 0) return " source not in model!";
+  if (R.is > M.nsl) return " source in the ocean!";
+  R.ir = fri(R.rr);
+  if (R.ir == 0) return " receiver not in model!";
+  if (R.ir > M.nsl) return " receiver in the ocean!";
+  // nout: first sample with t > tout (src/yspec.f90:479-486)
+  R.nout = R.nt;
+  for (int i = 1; i <= R.nt; i++)
+    if ((i - 1) * R.dt > R.tout) { R.nout = i; break; }
+  return "";
+}
+
+}  // namespace ysx

@@ -180,7 +180,8 @@ class Params:
         """A parameter file the reference reads back to these values."""
         def blk(comment, value):
             return f"\n# {comment}\n  {value}\n"
-        s = "# yspec parameter file\n# \n" + f"# prefix for output files\n  {self.prefix}\n"
+        s = ("# This file contains the parameters needed to run the\n# program yspec\n\n"
+             f"# prefix for output files\n  {self.prefix}\n")
         s += blk("Earth model", f"'{self.model_file}'")
         s += blk("attenuation switch: 1 = on, 0 = off", self.ats)
         s += blk("gravitation: 0 = none, 1 = cowling, 2 = self", self.grav)
