@@ -1,0 +1,153 @@
+"""Benchmark of the spectral path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (the metric's configuration): seeded 200-knot deck with attenuation
+and full self-gravitation, lmax = 600, fmin = 0.1 mHz, fmax = 40 mHz, 360 min
+series at dt = 1 s (nt = 32768), filters 0.2/0.5/35/38 mHz, one source at
+600 km depth with a seeded moment tensor, 200 seeded receivers at the
+surface: 787,310 (l, w) pairs.
+
+One step = one full forward: every (l, w) solve, the Legendre receiver sum
+and the filtered inverse FFT to seismograms, with the model, run tables and
+all buffers resident in HBM before timing starts.  Every library call is
+synchronous (the context's HIP stream is synchronised before it returns), so
+the wall clock around K steps brackets exactly K steps.
+
+Prints one JSON line.  value = (l, w) solves/s of the full forward (pairs per
+step / seconds per step); seismograms/s = receivers per step / seconds per
+step.  roofline: the dominant kernel's algorithmic FP64 FLOPs (exact step
+counts x the per-step constants of yspec_amd/flops.py) over its HIP-event
+time.  cpu_baseline: the reference Fortran (compiled from the reference's
+own sources into oracle/_ref, run through its FP64 driver) timed on this
+host on a stride sample of degrees, all frequencies and receivers.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from yspec_amd import api  # noqa: E402
+from yspec_amd.flops import ode_flops, sum_flops, fft_bytes  # noqa: E402
+from yspec_amd.synth import synth_deck, workload  # noqa: E402
+
+METRIC = "(l,ω) solves/s + seismograms/s, batched vs plain GPU; gradient/forward time"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+PHASES = ("source", "tor", "minors", "bvec", "rad", "assemble", "lsum", "ifft")
+
+
+def cpu_baseline(deck, p, sample):
+    """Reference yspec (oracle/_ref/ref_dump --time) on degrees `sample`,
+    every frequency and receiver, one core.  Returns (pairs/s, pairs, s)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "synth.200"), "w") as f:
+            f.write(deck.to_text())
+        q = workload("cfg2", model_file="synth.200")
+        with open(os.path.join(td, "in.txt"), "w") as f:
+            f.write(q.to_text())
+        t0 = time.perf_counter()
+        out = subprocess.run([exe, "in.txt", "--time", *map(str, sample)], cwd=td, check=True,
+                             capture_output=True, text=True).stdout
+        wall = time.perf_counter() - t0
+    secs = [float(line.split()[-1]) for line in out.splitlines() if line.startswith("l ")]
+    nws = [int(line.split()[3]) for line in out.splitlines() if line.startswith("l ")]
+    pairs = int(sum(nws))
+    return pairs / sum(secs), pairs, sum(secs), wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--lmax", type=int, default=None, help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the work would shard by frequency band; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    deck = synth_deck(0)
+    p = workload("cfg2") if args.lmax is None else workload("cfg2", lmax=args.lmax)
+    g = api.YSpec(deck, p, device=0)
+    P, nr = g.npair, p.nr
+    print(f"# workload cfg2: {P} (l,w) pairs, {nr} receivers, nt={g.nt}", file=sys.stderr,
+          flush=True)
+    for _ in range(args.warmup):
+        g.forward(copy=False)
+    phase = np.zeros(8)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        g.forward(copy=False)
+        phase += g.timings()
+        print(f"# step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    sec = (t1 - t0) / args.steps
+    phase /= args.steps
+    counts = g.step_counts()
+    fl = ode_flops(counts)
+    work = {"tor": fl["tor"], "minors": fl["minors"], "bvec": fl["bvec"], "rad": fl["rad"],
+            "lsum": sum_flops(g.nw, p.lmax, nr)}
+    ms = dict(zip(PHASES, phase))
+    dom = max(("tor", "minors", "bvec", "rad", "lsum"), key=lambda k: ms[k])
+    achieved = work[dom] / (ms[dom] * 1e-3) / 1e12
+    ode_ms = ms["tor"] + ms["minors"] + ms["bvec"] + ms["rad"]
+    ode_flop = fl["tor"] + fl["minors"] + fl["bvec"] + fl["rad"]
+    line = {
+        "metric": METRIC,
+        "value": P / sec,
+        "unit": "(l,ω) solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded 200-knot deck, seeded moment tensor and receivers",
+        "config": {"workload": "cfg2 plain path: lmax=600, fmax=40 mHz, 360 min, dt=1 s, "
+                               "600 km source, 200 receivers, attenuation + self-gravitation",
+                   "pairs": P, "receivers": nr, "nt": g.nt, "nw": g.nw},
+        "seismograms_per_s": nr / sec,
+        "ode_solves_per_s": P / (ode_ms * 1e-3),
+        "phase_ms": {k: round(float(v), 4) for k, v in ms.items()},
+        "ode_tflops": ode_flop / (ode_ms * 1e-3) / 1e12,
+        "step_counts": {k: int(v) for k, v in zip(("tor", "solid", "fluid", "bvec_col", "rad"),
+                                                  counts)},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "FP64 vector work; MI355X FP64 vector and matrix peaks are both "
+                             "78.6 TFLOP/s"},
+    }
+    if not args.no_cpu:
+        sample = list(range(0, p.lmax + 1, 75))
+        cb = cpu_baseline(deck, p, sample)
+        if cb:
+            rate, pairs, secs, wall = cb
+            line["cpu_baseline"] = {
+                "value": rate, "unit": "(l,ω) solves/s", "cores": 1, "kind": "reference",
+                "sample": f"reference yspec (amdflang -O2) on l in {sample}, all {g.nw} w, "
+                          f"{nr} receivers: {pairs} pairs in {secs:.1f} s",
+            }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -22,7 +22,7 @@ I = C.POINTER(C.c_int)
 
 EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_set_run",
            "ysx_run_info", "ysx_spectra", "ysx_seismograms", "ysx_forward",
-           "ysx_pair_records", "ysx_timings")
+           "ysx_pair_records", "ysx_timings", "ysx_step_counts")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -59,6 +59,7 @@ def lib():
         L.ysx_forward.argtypes = [vp, D]
         L.ysx_pair_records.argtypes = [vp, C.c_int, I, I, D]
         L.ysx_timings.argtypes = [vp, D]
+        L.ysx_step_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
         _LIB = L
     return _LIB
 
@@ -162,3 +163,10 @@ class YSpec:
         t = np.zeros(8)
         self._check(self.L.ysx_timings(self.h, _dp(t)))
         return t
+
+    def step_counts(self):
+        """exact Cash-Karp step totals of the last spectra: toroidal, solid
+        minors, fluid minors, b-vector (per column), radial"""
+        out = np.zeros(5, np.int64)
+        self._check(self.L.ysx_step_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return out

@@ -380,7 +380,7 @@ __device__ inline double step_len(const Tab& T, const PS& s, int i, int nknot) {
 // yint, src/module_int.f90:1847-1968: fixed equal steps per knot interval
 template <class F>
 __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, double r1, double r2,
-                     cd* y, const F& f, int* scale) {
+                     cd* y, const F& f, int* scale, int& nsteps) {
   if (r2 > r1) {
     for (int i = i1; i <= i2; i++) {
       double rr1 = (i == i1) ? r1 : T(F_R, i);
@@ -389,6 +389,7 @@ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, doubl
       double drr = step_len(T, s, i, nknot);
       int n = 2 + (int)floor((rr2 - rr1) / drr);
       drr = (rr2 - rr1) / (double)(n - 1);
+      nsteps += n - 1;
       for (int j = 1; j <= n - 1; j++) {
         double rr = rr1 + (double)(j - 1) * drr;
         rk_step(T, s, i, rr, drr, y, f);
@@ -403,6 +404,7 @@ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, doubl
       double drr = step_len(T, s, i, nknot);
       int n = 2 + (int)floor(fabs(rr2 - rr1) / drr);
       drr = (rr2 - rr1) / (double)(n - 1);
+      nsteps += n - 1;
       for (int j = 1; j <= n - 1; j++) {
         double rr = rr1 + (double)(j - 1) * drr;
         rk_step(T, s, i, rr, drr, y, f);

@@ -71,7 +71,7 @@ constexpr int O_SVT = 0, O_YTOR = 10, O_SVS = 14, O_YSPH = 44, O_SR0 = 68, O_YRA
 // 566-627, 1747-1847.  source_vector_sph keeps only the real part of its
 // dispersion factors (they are declared real(dp) there, :1771-1772).
 __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab, RunK R,
-                                                cd* __restrict__ rec) {
+                                                cd* __restrict__ rec, int* __restrict__ cnt) {
   Tab T = load_tab(gtab, R.nknot);
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= R.npair) return;
@@ -81,6 +81,7 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
   pair_state(R, l, j, s);
   cd* o = rec + (size_t)p * REC;
   for (int i = 0; i < REC; i++) o[i] = C(0.0);
+  for (int i = 0; i < 5; i++) cnt[5 * p + i] = 0;
   SplW w = spl_w(T, R.is, R.rs);
   double rlcon = spl(T, w, F_LCON), rccon = spl(T, w, F_CCON), rfcon = spl(T, w, F_FCON),
          rqmu = spl(T, w, F_QMU), rxlam = spl(T, w, F_XLAM), rxa2 = spl(T, w, F_XA2);
@@ -122,7 +123,8 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
 // ------------------------------------------------------------------ k_tor
 // tor_solver with ss = identity(2), src/module_solver.f90:25-142
 __global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, RunK R,
-                                             cd* __restrict__ rec, int* __restrict__ starts) {
+                                             cd* __restrict__ rec, int* __restrict__ starts,
+                                             int* __restrict__ cnt) {
   Tab T = load_tab(gtab, R.nknot);
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= R.npair) return;
@@ -135,7 +137,7 @@ __global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, Ru
   TorRHS f;
   double rs = R.rs, rr = R.rr;
   int is = R.is, ir = R.ir, nk = R.nknot;
-  int sc1 = 0, sc2 = 0, sc3 = 0;
+  int sc1 = 0, sc2 = 0, sc3 = 0, ns = 0;
   cd y1[2], y2[2], y3[2], yo[4];
   bool up = rr >= rs;
   int itop = up ? is : ir;  // lower of source / receiver
@@ -150,15 +152,15 @@ __global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, Ru
   } else {
     tor_ystart(T, s, i1, rt, y1);
   }
-  yint(T, s, nk, i1, itop, rt, rtop, y1, f, &sc1);
+  yint(T, s, nk, i1, itop, rt, rtop, y1, f, &sc1, ns);
   double rsl = T(F_R, L.nsl);
   if (up) {
     y3[0] = y1[0];
     y3[1] = y1[1];
-    if (rr != rs) yint(T, s, nk, is, ir, rs, rr, y3, f, &sc3);
+    if (rr != rs) yint(T, s, nk, is, ir, rs, rr, y3, f, &sc3, ns);
     y2[0] = C(1.0);
     y2[1] = C(0.0);
-    if (rr != rsl) yint(T, s, nk, L.nsl - 1, ir, rsl, rr, y2, f, &sc2);
+    if (rr != rsl) yint(T, s, nk, L.nsl - 1, ir, rsl, rr, y2, f, &sc2, ns);
     cd delta = cinv(y2[1] * y3[0] - y2[0] * y3[1]);
     cd b1 = y1[1], b2 = -y1[0];
     yo[0] = -delta * y2[0] * b1;
@@ -168,10 +170,10 @@ __global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, Ru
   } else {
     y2[0] = C(1.0);
     y2[1] = C(0.0);
-    if (rs != rsl) yint(T, s, nk, L.nsl - 1, is, rsl, rs, y2, f, &sc2);
+    if (rs != rsl) yint(T, s, nk, L.nsl - 1, is, rsl, rs, y2, f, &sc2, ns);
     y3[0] = y2[0];
     y3[1] = y2[1];
-    if (rr != rs) yint(T, s, nk, is, ir, rs, rr, y3, f, &sc3);
+    if (rr != rs) yint(T, s, nk, is, ir, rs, rr, y3, f, &sc3, ns);
     cd delta = cinv(y3[1] * y1[0] - y3[0] * y1[1]);
     cd b1 = y2[1], b2 = -y2[0];
     yo[0] = -delta * y1[0] * b1;
@@ -180,6 +182,7 @@ __global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, Ru
     yo[3] = -delta * y1[1] * b2;
   }
   double sc = ipow_huge(sc3);
+  cnt[5 * p + 0] = ns;
   cd* o = rec + (size_t)p * REC + O_YTOR;
   for (int i = 0; i < 4; i++) o[i] = yo[i] * sc;
 }
@@ -198,51 +201,52 @@ struct SphState {
 
 // integrate minors from the start level up to radius rtop in layer itop
 __device__ inline void minors_up(const Tab& T, const PS& s, const Layers& L, int i1, double rt,
-                                 int itop, double rtop, cd* ms, int* sc) {
+                                 int itop, double rtop, cd* ms, int* sc, int& ns, int& nf) {
   SolidMinorRHS fs;
   FluidMinorRHS ff;
   int nk = L.nknot;
   if (i1 <= L.nic) {
     minor_solid_start(T, s, i1, rt, ms);
-    yint(T, s, nk, i1, L.nic, rt, T(F_R, L.nic), ms, fs, sc);
+    yint(T, s, nk, i1, L.nic, rt, T(F_R, L.nic), ms, fs, sc, ns);
     cd mf[5];
     solid_2_fluid(ms, mf);
-    yint(T, s, nk, L.nic + 1, L.noc, T(F_R, L.nic), T(F_R, L.noc), mf, ff, sc);
+    yint(T, s, nk, L.nic + 1, L.noc, T(F_R, L.nic), T(F_R, L.noc), mf, ff, sc, nf);
     fluid_2_solid(mf, ms);
-    yint(T, s, nk, L.noc + 1, itop, T(F_R, L.noc + 1), rtop, ms, fs, sc);
+    yint(T, s, nk, L.noc + 1, itop, T(F_R, L.noc + 1), rtop, ms, fs, sc, ns);
   } else if (i1 <= L.noc) {
     cd mf[5];
     minor_fluid_start(T, s, i1, rt, mf);
-    yint(T, s, nk, i1, L.noc, rt, T(F_R, L.noc), mf, ff, sc);
+    yint(T, s, nk, i1, L.noc, rt, T(F_R, L.noc), mf, ff, sc, nf);
     fluid_2_solid(mf, ms);
-    yint(T, s, nk, L.noc + 1, itop, T(F_R, L.noc + 1), rtop, ms, fs, sc);
+    yint(T, s, nk, L.noc + 1, itop, T(F_R, L.noc + 1), rtop, ms, fs, sc, ns);
   } else {
     minor_solid_start(T, s, i1, rt, ms);
-    yint(T, s, nk, i1, itop, rt, rtop, ms, fs, sc);
+    yint(T, s, nk, i1, itop, rt, rtop, ms, fs, sc, ns);
   }
 }
 
 // minor vector integrated down from the surface to radius rb in layer ib
 // (the ocean, when present, first as a fluid), src/module_solver.f90:730-741
 __device__ inline void minors_down(const Tab& T, const PS& s, const Layers& L, int itopk, int ib,
-                                   double rb, cd* ms, int* sc) {
+                                   double rb, cd* ms, int* sc, int& ns, int& nf) {
   SolidMinorRHS fs;
   int nk = L.nknot;
   if (L.ocean) {
     FluidMinorRHS ff;
     cd mf[5] = {C(1.0), C(0.0), C(0.0), C(0.0), C(0.0)};
-    yint(T, s, nk, nk - 1, L.nsl + 1, T(F_R, nk), T(F_R, L.nsl + 1), mf, ff, sc);
+    yint(T, s, nk, nk - 1, L.nsl + 1, T(F_R, nk), T(F_R, L.nsl + 1), mf, ff, sc, nf);
     fluid_2_solid(mf, ms);
-    yint(T, s, nk, L.nsl, ib, T(F_R, L.nsl), rb, ms, fs, sc);
+    yint(T, s, nk, L.nsl, ib, T(F_R, L.nsl), rb, ms, fs, sc, ns);
   } else {
     ms[0] = C(1.0);
     for (int i = 1; i < 14; i++) ms[i] = C(0.0);
-    yint(T, s, nk, itopk, ib, T(F_R, nk), rb, ms, fs, sc);
+    yint(T, s, nk, itopk, ib, T(F_R, nk), rb, ms, fs, sc, ns);
   }
 }
 
 __global__ void __launch_bounds__(256) k_sph_minor(const double* __restrict__ gtab, RunK R,
-                                                   SphState S, int* __restrict__ starts) {
+                                                   SphState S, int* __restrict__ starts,
+                                                   int* __restrict__ cnt) {
   Tab T = load_tab(gtab, R.nknot);
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= R.npair) return;
@@ -255,7 +259,7 @@ __global__ void __launch_bounds__(256) k_sph_minor(const double* __restrict__ gt
   SolidMinorRHS fs;
   double rs = R.rs, rr = R.rr;
   int is = R.is, ir = R.ir, nk = R.nknot;
-  int sc1 = 0, sc2 = 0, sc3 = 0;
+  int sc1 = 0, sc2 = 0, sc3 = 0, ns = 0, nf = 0;
   const int cols[4] = {0, 1, 3, 4};
   cd ms1[14], ms2[14];
   cd* bst = S.bst + (size_t)p * 56;
@@ -271,14 +275,14 @@ __global__ void __launch_bounds__(256) k_sph_minor(const double* __restrict__ gt
       i1 = is - 1;
       rt = T(F_R, i1);
     }
-    minors_up(T, s, L, i1, rt, is, rs, ms1, &sc1);
+    minors_up(T, s, L, i1, rt, is, rs, ms1, &sc1, ns, nf);
     for (int c = 0; c < 4; c++) {
       cd b[14];
       b_start_unit(ms1, cols[c], b);
       for (int i = 0; i < 14; i++) bst[c * 14 + i] = b[i];
     }
-    if (rr > rs) yint(T, s, nk, is, ir, rs, rr, ms1, fs, &sc3);  // ms1 becomes ms3
-    minors_down(T, s, L, nk - 1, ir, rr, ms2, &sc2);
+    if (rr > rs) yint(T, s, nk, is, ir, rs, rr, ms1, fs, &sc3, ns);  // ms1 becomes ms3
+    minors_down(T, s, L, nk - 1, ir, rr, ms2, &sc2, ns, nf);
     cd delta = delta_solid(ms1, ms2);
     for (int i = 0; i < 14; i++) S.mfin[(size_t)p * 14 + i] = ms2[i];
     S.fac[p] = -cinv(delta);
@@ -290,11 +294,11 @@ __global__ void __launch_bounds__(256) k_sph_minor(const double* __restrict__ gt
       i1 = ir - 1;
       rt = T(F_R, i1);
     }
-    minors_up(T, s, L, i1, rt, ir, rr, ms1, &sc1);
+    minors_up(T, s, L, i1, rt, ir, rr, ms1, &sc1, ns, nf);
     for (int i = 0; i < 14; i++) S.mfin[(size_t)p * 14 + i] = ms1[i];
-    if (rs > rr) yint(T, s, nk, ir, is, rr, rs, ms1, fs, &sc3);  // ms1 becomes ms3
+    if (rs > rr) yint(T, s, nk, ir, is, rr, rs, ms1, fs, &sc3, ns);  // ms1 becomes ms3
     // the reference starts this sweep at knot nknot (src/module_solver.f90:802)
-    minors_down(T, s, L, nk, is, rs, ms2, &sc2);
+    minors_down(T, s, L, nk, is, rs, ms2, &sc2, ns, nf);
     cd delta = delta_solid(ms1, ms2);
     cd mdi = -cinv(delta);
     for (int c = 0; c < 4; c++) {
@@ -305,13 +309,15 @@ __global__ void __launch_bounds__(256) k_sph_minor(const double* __restrict__ gt
     S.fac[p] = C(1.0);
   }
   S.sc3[p] = sc3;
+  cnt[5 * p + 1] = ns;
+  cnt[5 * p + 2] = nf;
 }
 
 // ------------------------------------------------------------------ k_bvec
 // b-vector from source to receiver and the solution y = sph_ysol(m, b),
 // src/module_solver.f90:745-755, 809-819; one lane per (pair, column)
 __global__ void __launch_bounds__(256) k_bvec(const double* __restrict__ gtab, RunK R, SphState S,
-                                              cd* __restrict__ rec) {
+                                              cd* __restrict__ rec, int* __restrict__ cnt) {
   Tab T = load_tab(gtab, R.nknot);
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int p = t >> 2, c = t & 3;
@@ -324,9 +330,10 @@ __global__ void __launch_bounds__(256) k_bvec(const double* __restrict__ gtab, R
   cd b[14];
   const cd* bs = S.bst + (size_t)p * 56 + c * 14;
   for (int i = 0; i < 14; i++) b[i] = bs[i];
-  int sc4 = 0;
+  int sc4 = 0, nb = 0;
   BvecRHS f;
-  yint(T, s, R.nknot, R.is, R.ir, R.rs, R.rr, b, f, &sc4);
+  yint(T, s, R.nknot, R.is, R.ir, R.rs, R.rr, b, f, &sc4, nb);
+  if (c == 0) cnt[5 * p + 3] = nb;
   cd m[14];
   for (int i = 0; i < 14; i++) m[i] = S.mfin[(size_t)p * 14 + i];
   cd y[6];
@@ -385,7 +392,7 @@ __device__ inline void grid_runs(const Tab& T, int i, double drr, double rs, dou
 }
 
 __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, RunK R,
-                                            cd* __restrict__ rec) {
+                                            cd* __restrict__ rec, int* __restrict__ cnt) {
   Tab T = load_tab(gtab, R.nknot);
   int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (j > R.nw || R.lmin != 0) return;
@@ -396,6 +403,7 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, Run
   int nk = R.nknot;
   double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
   double rs = R.rs, rr = R.rr;
+  int nsr = 0;
   // --- upward sweep over grid points 1 .. source
   cd y1[2], y1r[2] = {C(0.0), C(0.0)};
   bool rec_below = false;  // receiver at or below the source point
@@ -414,9 +422,9 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, Run
           if (k == 2) {
             double rt = fmin(0.1 * dr, 0.1 * rg);
             rad_ystart(T, s, 1, rt, y1);
-            yint(T, s, nk, 1, i, rt, rg, y1, f, (int*)nullptr);
+            yint(T, s, nk, 1, i, rt, rg, y1, f, (int*)nullptr, nsr);
           } else if (k > 2) {
-            yint(T, s, nk, iprev, i, rprev, rg, y1, f, (int*)nullptr);
+            yint(T, s, nk, iprev, i, rprev, rg, y1, f, (int*)nullptr, nsr);
           }
           rprev = rg;
           iprev = i;
@@ -446,7 +454,7 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, Run
       for (int q = G.n - 1; q >= 0 && !done; q--) {
         for (int jj = G.j1[q]; jj >= G.j0[q]; jj--) {
           double rg = G.base[q] + (double)(jj - 1) * G.step[q];
-          if (!first) yint(T, s, nk, iprev, i, rprev, rg, y2, f, (int*)nullptr);
+          if (!first) yint(T, s, nk, iprev, i, rprev, rg, y2, f, (int*)nullptr, nsr);
           first = false;
           rprev = rg;
           iprev = i;
@@ -480,6 +488,7 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, Run
   }
   cd* o = rec + (size_t)p * REC + O_YRAD;
   for (int i = 0; i < 4; i++) o[i] = yo[i];
+  cnt[5 * p + 4] = nsr;
 }
 
 // ------------------------------------------------------------- k_assemble
@@ -666,6 +675,7 @@ struct ysx_ctx {
   double* d_tab = nullptr;
   cd* d_rec = nullptr;
   int* d_starts = nullptr;
+  int* d_cnt = nullptr;
   cd *d_bst = nullptr, *d_mfin = nullptr, *d_fac = nullptr;
   int* d_sc3 = nullptr;
   double *d_xa = nullptr, *d_xpa = nullptr, *d_xca = nullptr, *d_azep = nullptr;
@@ -700,7 +710,7 @@ void dfree(T*& p) {
 }
 
 void free_run(ysx_ctx* c) {
-  dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_bst); dfree(c->d_mfin);
+  dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_cnt); dfree(c->d_bst); dfree(c->d_mfin);
   dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_xa); dfree(c->d_xpa); dfree(c->d_xca);
   dfree(c->d_azep); dfree(c->d_eia); dfree(c->d_spec); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis);
@@ -783,6 +793,7 @@ extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const 
   size_t P = (size_t)k.npair, nt3 = R.xa.size();
   HIPCHK(hipMalloc(&ctx->d_rec, P * 89 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_starts, P * 2 * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_cnt, P * 5 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_bst, P * 56 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_mfin, P * 14 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_fac, P * sizeof(cd)));
@@ -833,20 +844,21 @@ int run_spectra(ysx_ctx* ctx) {
   dim3 b256(256);
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_fac, ctx->d_sc3};
   HIPCHK(hipEventRecord(ctx->ev[0], st));
-  hipLaunchKernelGGL(k_source, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, ctx->d_rec);
+  hipLaunchKernelGGL(k_source, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, ctx->d_rec,
+                     ctx->d_cnt);
   HIPCHK(hipEventRecord(ctx->ev[1], st));
   hipLaunchKernelGGL(k_tor, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, ctx->d_rec,
-                     ctx->d_starts);
+                     ctx->d_starts, ctx->d_cnt);
   HIPCHK(hipEventRecord(ctx->ev[2], st));
   hipLaunchKernelGGL(k_sph_minor, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, S,
-                     ctx->d_starts);
+                     ctx->d_starts, ctx->d_cnt);
   HIPCHK(hipEventRecord(ctx->ev[3], st));
   hipLaunchKernelGGL(k_bvec, dim3((4 * P + 255) / 256), b256, lds, st, ctx->d_tab, k, S,
-                     ctx->d_rec);
+                     ctx->d_rec, ctx->d_cnt);
   HIPCHK(hipEventRecord(ctx->ev[4], st));
   if (k.lmin == 0)
     hipLaunchKernelGGL(k_rad, dim3((k.nw + 63) / 64), dim3(64), lds, st, ctx->d_tab, k,
-                       ctx->d_rec);
+                       ctx->d_rec, ctx->d_cnt);
   HIPCHK(hipEventRecord(ctx->ev[5], st));
   hipLaunchKernelGGL(k_assemble, dim3((P + 255) / 256), b256, 0, st, k, ctx->d_rec);
   HIPCHK(hipEventRecord(ctx->ev[6], st));
@@ -940,5 +952,18 @@ extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j,
 extern "C" int ysx_timings(const ysx_ctx* ctx, double* ms8) {
   if (!ctx || !ms8) return YSX_ERR_ARG;
   std::memcpy(ms8, ctx->ms, sizeof ctx->ms);
+  return YSX_OK;
+}
+
+extern "C" int ysx_step_counts(ysx_ctx* ctx, long long* out5) {
+  if (!ctx || !out5) return YSX_ERR_ARG;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t P = (size_t)ctx->rk.npair;
+  std::vector<int> c(P * 5);
+  HIPCHK(hipMemcpy(c.data(), ctx->d_cnt, c.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (int q = 0; q < 5; q++) out5[q] = 0;
+  for (size_t p = 0; p < P; p++)
+    for (int q = 0; q < 5; q++) out5[q] += c[5 * p + q];
   return YSX_OK;
 }
