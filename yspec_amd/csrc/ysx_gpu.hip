@@ -22,6 +22,7 @@
 
 #include "../../include/ysx.h"
 #include "ysx_device.hpp"
+#include "ysx_march.hpp"
 #include "ysx_host.hpp"
 
 using namespace ysx;
@@ -120,47 +121,142 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
   }
 }
 
-// ------------------------------------------------------------------ k_tor
-// tor_solver with ss = identity(2), src/module_solver.f90:25-142
-__global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, RunK R,
-                                             cd* __restrict__ rec, int* __restrict__ starts,
-                                             int* __restrict__ cnt) {
-  Tab T = load_tab(gtab, R.nknot);
+// ----------------------------------------------------------------- k_start
+// Start levels and starting solutions of every pair (one lane per pair):
+// tor_solver's start (src/module_solver.f90:64-75, 103-114) and
+// sph_solver's start (:678-722, 758-785).  The marches pick these up when
+// they reach each lane's start knot.
+struct StartData {
+  int* tor_i1;   // [npair] adjusted start knot
+  cd* tor_y0;    // [npair][2]
+  int* sph_i1;   // [npair] adjusted start knot
+  cd* sph_m0;    // [npair][14] solid minors, or 5 fluid minors + zeros
+};
+
+__global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab, RunK R,
+                                               StartData SD, int* __restrict__ starts) {
+  Tab T{gtab, R.nknot};
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= R.npair) return;
   int l, j;
   pair_lj(R, p, l, j);
-  if (l < 2) return;
+  if (l < 1) return;
   PS s;
   pair_state(R, l, j, s);
   Layers L = layers(R);
-  TorRHS f;
-  double rs = R.rs, rr = R.rr;
-  int is = R.is, ir = R.ir, nk = R.nknot;
-  int sc1 = 0, sc2 = 0, sc3 = 0, ns = 0;
-  cd y1[2], y2[2], y3[2], yo[4];
-  bool up = rr >= rs;
-  int itop = up ? is : ir;  // lower of source / receiver
-  double rtop = up ? rs : rr;
-  int i1 = tor_start_level(T, L, l, s.rew, itop);
-  starts[2 * p] = i1;
-  if (i1 == itop && i1 > L.noc + 1) i1--;
-  double rt = T(F_R, i1);
-  if (i1 == L.noc + 1) {
-    y1[0] = C(1.0);
-    y1[1] = C(0.0);
-  } else {
-    tor_ystart(T, s, i1, rt, y1);
+  bool up = R.rr >= R.rs;
+  int itop = up ? R.is : R.ir;
+  if (l >= 2) {
+    int i1 = tor_start_level(T, L, l, s.rew, itop);
+    starts[2 * p] = i1;
+    if (i1 == itop && i1 > L.noc + 1) i1--;
+    cd y[2];
+    if (i1 == L.noc + 1) {
+      y[0] = C(1.0);
+      y[1] = C(0.0);
+    } else {
+      tor_ystart(T, s, i1, T(F_R, i1), y);
+    }
+    SD.tor_i1[p] = i1;
+    SD.tor_y0[2 * p] = y[0];
+    SD.tor_y0[2 * p + 1] = y[1];
   }
-  yint(T, s, nk, i1, itop, rt, rtop, y1, f, &sc1, ns);
-  double rsl = T(F_R, L.nsl);
+  int i1 = sph_start_level(T, L, l, s.rew, itop, R.fre_norm);
+  starts[2 * p + 1] = i1;
+  double rt = T(F_R, i1);
+  if (up && i1 == 1) {
+    double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
+    rt = fmin(0.1 * dr, 0.1 * T(F_R, 2));
+  }
+  if (i1 >= itop) {
+    i1 = itop - 1;
+    rt = T(F_R, i1);
+  }
+  cd m[14];
+  for (int i = 0; i < 14; i++) m[i] = C(0.0);
+  if (i1 > L.nic && i1 <= L.noc)
+    minor_fluid_start(T, s, i1, rt, m);
+  else
+    minor_solid_start(T, s, i1, rt, m);
+  SD.sph_i1[p] = i1;
+  for (int i = 0; i < 14; i++) SD.sph_m0[(size_t)p * 14 + i] = m[i];
+}
+
+// ------------------------------------------------------- march helpers
+// yint's interval loop (src/module_int.f90:1888-1963) over knot intervals,
+// with wave-uniform arguments
+template <class F>
+__device__ int march_yint(const Tab& T, const WS& w, const LS& L, int nk, int i1, int i2,
+                          double r1, double r2, cd* y, int& sc, double* sb, const F& f) {
+  int n = 0;
+  if (r2 > r1) {
+    for (int i = i1; i <= i2; i++) {
+      double a = (i == i1) ? r1 : T(F_R, i);
+      double b = (i == i2) ? r2 : T(F_R, i + 1);
+      if (a == b) continue;
+      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f);
+    }
+  } else if (r2 < r1) {
+    for (int i = i1; i >= i2; i--) {
+      double a = (i == i1) ? r1 : T(F_R, i + 1);
+      double b = (i == i2) ? r2 : T(F_R, i);
+      if (a == b) continue;
+      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f);
+    }
+  }
+  return n;
+}
+
+__device__ inline WS wave_omega(const RunK& R, int j) {
+  return omega_state(C(R.wmin + (double)(j - 1) * R.dw, -R.ep), R.tref, R.ats, R.pibigg);
+}
+
+constexpr int NOLANE = 1 << 30;
+
+// ------------------------------------------------------------- k_tor_march
+// tor_solver with ss = identity(2), src/module_solver.f90:25-142; a wave
+// holds 64 consecutive l at one w, largest w first.
+__global__ void __launch_bounds__(64) k_tor_march(const double* __restrict__ gtab, RunK R,
+                                                  StartData SD, cd* __restrict__ rec,
+                                                  int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  Tab T{gtab, R.nknot};
+  int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64;
+  int wv = blockIdx.x;
+  int j = R.nw - wv / nlb;
+  int l = R.lmin + (wv % nlb) * 64 + (int)threadIdx.x;
+  bool valid = (l >= 2 && l <= R.lmax);
+  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  int i1 = valid ? SD.tor_i1[p] : NOLANE;
+  int ib = wave_min(i1);
+  if (ib == NOLANE) return;
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 2);
+  TorM f;
+  int nk = R.nknot, is = R.is, ir = R.ir;
+  double rs = R.rs, rr = R.rr;
+  bool up = rr >= rs;
+  int itop = up ? is : ir;
+  double rtop = up ? rs : rr;
+  cd y1[2] = {C(0.0), C(0.0)};
+  int sc1 = 0, sc2 = 0, sc3 = 0, ns = 0;
+  for (int i = ib; i <= itop; i++) {
+    if (i == i1) {
+      y1[0] = SD.tor_y0[2 * p];
+      y1[1] = SD.tor_y0[2 * p + 1];
+    }
+    double a = T(F_R, i), b = (i == itop) ? rtop : T(F_R, i + 1);
+    if (a == b) continue;
+    int n = march_interval(T, w, L, nk, i, a, b, y1, true, sc1, sb, f);
+    if (i >= i1) ns += n;
+  }
+  double rsl = T(F_R, R.nsl);
+  cd y2[2] = {C(1.0), C(0.0)}, y3[2], yo[4];
   if (up) {
     y3[0] = y1[0];
     y3[1] = y1[1];
-    if (rr != rs) yint(T, s, nk, is, ir, rs, rr, y3, f, &sc3, ns);
-    y2[0] = C(1.0);
-    y2[1] = C(0.0);
-    if (rr != rsl) yint(T, s, nk, L.nsl - 1, ir, rsl, rr, y2, f, &sc2, ns);
+    if (rr != rs) ns += march_yint(T, w, L, nk, is, ir, rs, rr, y3, sc3, sb, f);
+    if (rr != rsl) ns += march_yint(T, w, L, nk, R.nsl - 1, ir, rsl, rr, y2, sc2, sb, f);
     cd delta = cinv(y2[1] * y3[0] - y2[0] * y3[1]);
     cd b1 = y1[1], b2 = -y1[0];
     yo[0] = -delta * y2[0] * b1;
@@ -168,12 +264,10 @@ __global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, Ru
     yo[2] = -delta * y2[0] * b2;
     yo[3] = -delta * y2[1] * b2;
   } else {
-    y2[0] = C(1.0);
-    y2[1] = C(0.0);
-    if (rs != rsl) yint(T, s, nk, L.nsl - 1, is, rsl, rs, y2, f, &sc2, ns);
+    if (rs != rsl) ns += march_yint(T, w, L, nk, R.nsl - 1, is, rsl, rs, y2, sc2, sb, f);
     y3[0] = y2[0];
     y3[1] = y2[1];
-    if (rr != rs) yint(T, s, nk, is, ir, rs, rr, y3, f, &sc3, ns);
+    if (rr != rs) ns += march_yint(T, w, L, nk, is, ir, rs, rr, y3, sc3, sb, f);
     cd delta = cinv(y3[1] * y1[0] - y3[0] * y1[1]);
     cd b1 = y2[1], b2 = -y2[0];
     yo[0] = -delta * y1[0] * b1;
@@ -181,166 +275,177 @@ __global__ void __launch_bounds__(256) k_tor(const double* __restrict__ gtab, Ru
     yo[2] = -delta * y1[0] * b2;
     yo[3] = -delta * y1[1] * b2;
   }
+  if (!valid) return;
   double sc = ipow_huge(sc3);
   cnt[5 * p + 0] = ns;
-  cd* o = rec + (size_t)p * REC + O_YTOR;
+  cd* o = rec + p * REC + O_YTOR;
   for (int i = 0; i < 4; i++) o[i] = yo[i] * sc;
 }
 
-// ---------------------------------------------------------- k_sph_minor
+// ----------------------------------------------------------- k_minor_march
 // sph_solver (grav = 2, solid inner core) up to the b-vector integration,
-// src/module_solver.f90:640-743, 756-806.  Writes, per pair: the four
-// b-vector starts bst[4][14], the minor vector used by sph_ysol (mfin),
-// the factor applied to sph_ysol (fac) and scale3.
+// src/module_solver.f90:640-743, 756-806.  Per pair it leaves the four
+// b-vector starts (bst), the minor vector of sph_ysol (mfin), the factor
+// applied to sph_ysol (fac) and scale3.
 struct SphState {
   cd* bst;   // [npair][4][14]
   cd* mfin;  // [npair][14]
+  cd* m2;    // [npair][14] minor vector from the surface
   cd* fac;   // [npair]
   int* sc3;  // [npair]
 };
 
-// integrate minors from the start level up to radius rtop in layer itop
-__device__ inline void minors_up(const Tab& T, const PS& s, const Layers& L, int i1, double rt,
-                                 int itop, double rtop, cd* ms, int* sc, int& ns, int& nf) {
-  SolidMinorRHS fs;
-  FluidMinorRHS ff;
-  int nk = L.nknot;
-  if (i1 <= L.nic) {
-    minor_solid_start(T, s, i1, rt, ms);
-    yint(T, s, nk, i1, L.nic, rt, T(F_R, L.nic), ms, fs, sc, ns);
-    cd mf[5];
-    solid_2_fluid(ms, mf);
-    yint(T, s, nk, L.nic + 1, L.noc, T(F_R, L.nic), T(F_R, L.noc), mf, ff, sc, nf);
-    fluid_2_solid(mf, ms);
-    yint(T, s, nk, L.noc + 1, itop, T(F_R, L.noc + 1), rtop, ms, fs, sc, ns);
-  } else if (i1 <= L.noc) {
-    cd mf[5];
-    minor_fluid_start(T, s, i1, rt, mf);
-    yint(T, s, nk, i1, L.noc, rt, T(F_R, L.noc), mf, ff, sc, nf);
-    fluid_2_solid(mf, ms);
-    yint(T, s, nk, L.noc + 1, itop, T(F_R, L.noc + 1), rtop, ms, fs, sc, ns);
+// minor vector from the surface down to radius rb in knot interval ib
+// (through the ocean as a fluid when there is one), src/module_solver.f90:730-741
+__device__ inline int minors_from_top(const Tab& T, const WS& w, const LS& L, const RunK& R,
+                                      int itopk, int ib, double rb, cd* m, double* sb,
+                                      int& nsol, int& nflu) {
+  int nk = R.nknot, sc2 = 0;
+  for (int i = 0; i < 14; i++) m[i] = C(0.0);
+  m[0] = C(1.0);
+  if (R.ocean) {
+    nflu += march_yint(T, w, L, nk, nk - 1, R.nsl + 1, T(F_R, nk), T(F_R, R.nsl + 1), m, sc2, sb,
+                       FluidMinorM());
+    cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+    fluid_2_solid(mf, m);
+    nsol += march_yint(T, w, L, nk, R.nsl, ib, T(F_R, R.nsl), rb, m, sc2, sb, SolidMinorM());
   } else {
-    minor_solid_start(T, s, i1, rt, ms);
-    yint(T, s, nk, i1, itop, rt, rtop, ms, fs, sc, ns);
+    nsol += march_yint(T, w, L, nk, itopk, ib, T(F_R, nk), rb, m, sc2, sb, SolidMinorM());
   }
+  return sc2;
 }
 
-// minor vector integrated down from the surface to radius rb in layer ib
-// (the ocean, when present, first as a fluid), src/module_solver.f90:730-741
-__device__ inline void minors_down(const Tab& T, const PS& s, const Layers& L, int itopk, int ib,
-                                   double rb, cd* ms, int* sc, int& ns, int& nf) {
-  SolidMinorRHS fs;
-  int nk = L.nknot;
-  if (L.ocean) {
-    FluidMinorRHS ff;
-    cd mf[5] = {C(1.0), C(0.0), C(0.0), C(0.0), C(0.0)};
-    yint(T, s, nk, nk - 1, L.nsl + 1, T(F_R, nk), T(F_R, L.nsl + 1), mf, ff, sc, nf);
-    fluid_2_solid(mf, ms);
-    yint(T, s, nk, L.nsl, ib, T(F_R, L.nsl), rb, ms, fs, sc, ns);
-  } else {
-    ms[0] = C(1.0);
-    for (int i = 1; i < 14; i++) ms[i] = C(0.0);
-    yint(T, s, nk, itopk, ib, T(F_R, nk), rb, ms, fs, sc, ns);
-  }
-}
-
-__global__ void __launch_bounds__(256) k_sph_minor(const double* __restrict__ gtab, RunK R,
-                                                   SphState S, int* __restrict__ starts,
-                                                   int* __restrict__ cnt) {
-  Tab T = load_tab(gtab, R.nknot);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= R.npair) return;
-  int l, j;
-  pair_lj(R, p, l, j);
-  if (l < 1) return;
-  PS s;
-  pair_state(R, l, j, s);
-  Layers L = layers(R);
-  SolidMinorRHS fs;
+__global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ gtab, RunK R,
+                                                    StartData SD, SphState S,
+                                                    int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  Tab T{gtab, R.nknot};
+  int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64;
+  int wv = blockIdx.x;
+  int j = R.nw - wv / nlb;
+  int l = R.lmin + (wv % nlb) * 64 + (int)threadIdx.x;
+  bool valid = (l >= 1 && l <= R.lmax);
+  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  int i1 = valid ? SD.sph_i1[p] : NOLANE;
+  int ib = wave_min(i1);
+  if (ib == NOLANE) return;
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 1);
+  int nk = R.nknot, is = R.is, ir = R.ir, nic = R.nic, noc = R.noc;
   double rs = R.rs, rr = R.rr;
-  int is = R.is, ir = R.ir, nk = R.nknot;
-  int sc1 = 0, sc2 = 0, sc3 = 0, ns = 0, nf = 0;
+  bool up = rr >= rs;
+  int itop = up ? is : ir;
+  double rtop = up ? rs : rr;
+  int nsol = 0, nflu = 0;
+  cd m[14];
+  // (1) the surface minor vector ms2, down to the lower of source/receiver's partner point
+  {
+    int dummy = 0;
+    if (up)
+      minors_from_top(T, w, L, R, nk - 1, ir, rr, m, sb, nsol, nflu);
+    else
+      minors_from_top(T, w, L, R, nk, is, rs, m, sb, nsol, nflu);
+    (void)dummy;
+    if (valid)
+      for (int i = 0; i < 14; i++) S.m2[p * 14 + i] = m[i];
+  }
+  // (2) ms1 from each lane's start level up to the lower point
+  double rt1 = up ? fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2)) : T(F_R, 1);
+  for (int i = 0; i < 14; i++) m[i] = C(0.0);
+  int layout = -1, sc1 = 0;  // -1 none, 0 solid, 1 fluid
+  for (int i = ib; i <= itop; i++) {
+    int region = (i < nic) ? 0 : ((i > nic && i < noc) ? 1 : 0);
+    bool fluid_i = (i > nic && i < noc);
+    if (layout == 0 && fluid_i) {
+      cd mf[5];
+      solid_2_fluid(m, mf);
+      for (int q = 0; q < 14; q++) m[q] = q < 5 ? mf[q] : C(0.0);
+    } else if (layout == 1 && !fluid_i) {
+      cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+      fluid_2_solid(mf, m);
+    }
+    layout = region;
+    if (i == i1)
+      for (int q = 0; q < 14; q++) m[q] = SD.sph_m0[p * 14 + q];
+    double a = (i == 1) ? rt1 : T(F_R, i);
+    double b = (i == itop) ? rtop : T(F_R, i + 1);
+    if (a == b) continue;
+    int n;
+    if (fluid_i)
+      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, FluidMinorM());
+    else
+      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, SolidMinorM());
+    if (i >= i1) (fluid_i ? nflu : nsol) += n;
+  }
   const int cols[4] = {0, 1, 3, 4};
-  cd ms1[14], ms2[14];
-  cd* bst = S.bst + (size_t)p * 56;
-  if (rr >= rs) {
-    int i1 = sph_start_level(T, L, l, s.rew, is, R.fre_norm);
-    starts[2 * p + 1] = i1;
-    double rt = T(F_R, i1);
-    if (i1 == 1) {
-      double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
-      rt = fmin(0.1 * dr, 0.1 * T(F_R, 2));
-    }
-    if (i1 >= is) {
-      i1 = is - 1;
-      rt = T(F_R, i1);
-    }
-    minors_up(T, s, L, i1, rt, is, rs, ms1, &sc1, ns, nf);
-    for (int c = 0; c < 4; c++) {
-      cd b[14];
-      b_start_unit(ms1, cols[c], b);
-      for (int i = 0; i < 14; i++) bst[c * 14 + i] = b[i];
-    }
-    if (rr > rs) yint(T, s, nk, is, ir, rs, rr, ms1, fs, &sc3, ns);  // ms1 becomes ms3
-    minors_down(T, s, L, nk - 1, ir, rr, ms2, &sc2, ns, nf);
-    cd delta = delta_solid(ms1, ms2);
-    for (int i = 0; i < 14; i++) S.mfin[(size_t)p * 14 + i] = ms2[i];
+  int sc3 = 0;
+  if (up) {
+    if (valid)
+      for (int c = 0; c < 4; c++) {
+        cd bb[14];
+        b_start_unit(m, cols[c], bb);
+        for (int q = 0; q < 14; q++) S.bst[p * 56 + c * 14 + q] = bb[q];
+      }
+    if (rr > rs) nsol += march_yint(T, w, L, nk, is, ir, rs, rr, m, sc3, sb, SolidMinorM());
+  } else {
+    if (valid)
+      for (int q = 0; q < 14; q++) S.mfin[p * 14 + q] = m[q];
+    if (rs > rr) nsol += march_yint(T, w, L, nk, ir, is, rr, rs, m, sc3, sb, SolidMinorM());
+  }
+  if (!valid) return;
+  cd m2[14];
+  for (int q = 0; q < 14; q++) m2[q] = S.m2[p * 14 + q];
+  cd delta = delta_solid(m, m2);
+  if (up) {
+    for (int q = 0; q < 14; q++) S.mfin[p * 14 + q] = m2[q];
     S.fac[p] = -cinv(delta);
   } else {
-    int i1 = sph_start_level(T, L, l, s.rew, ir, R.fre_norm);
-    starts[2 * p + 1] = i1;
-    double rt = T(F_R, i1);
-    if (i1 >= ir) {
-      i1 = ir - 1;
-      rt = T(F_R, i1);
-    }
-    minors_up(T, s, L, i1, rt, ir, rr, ms1, &sc1, ns, nf);
-    for (int i = 0; i < 14; i++) S.mfin[(size_t)p * 14 + i] = ms1[i];
-    if (rs > rr) yint(T, s, nk, ir, is, rr, rs, ms1, fs, &sc3, ns);  // ms1 becomes ms3
-    // the reference starts this sweep at knot nknot (src/module_solver.f90:802)
-    minors_down(T, s, L, nk, is, rs, ms2, &sc2, ns, nf);
-    cd delta = delta_solid(ms1, ms2);
     cd mdi = -cinv(delta);
     for (int c = 0; c < 4; c++) {
-      cd b[14];
-      b_start_unit(ms2, cols[c], b);
-      for (int i = 0; i < 14; i++) bst[c * 14 + i] = b[i] * mdi;
+      cd bb[14];
+      b_start_unit(m2, cols[c], bb);
+      for (int q = 0; q < 14; q++) S.bst[p * 56 + c * 14 + q] = bb[q] * mdi;
     }
     S.fac[p] = C(1.0);
   }
   S.sc3[p] = sc3;
-  cnt[5 * p + 1] = ns;
-  cnt[5 * p + 2] = nf;
+  cnt[5 * p + 1] = nsol;
+  cnt[5 * p + 2] = nflu;
 }
 
-// ------------------------------------------------------------------ k_bvec
-// b-vector from source to receiver and the solution y = sph_ysol(m, b),
-// src/module_solver.f90:745-755, 809-819; one lane per (pair, column)
-__global__ void __launch_bounds__(256) k_bvec(const double* __restrict__ gtab, RunK R, SphState S,
-                                              cd* __restrict__ rec, int* __restrict__ cnt) {
-  Tab T = load_tab(gtab, R.nknot);
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int p = t >> 2, c = t & 3;
-  if (p >= R.npair) return;
-  int l, j;
-  pair_lj(R, p, l, j);
-  if (l < 1) return;
-  PS s;
-  pair_state(R, l, j, s);
+// ------------------------------------------------------------ k_bvec_march
+// b-vector from source to receiver and y = sph_ysol(m, b) for the four
+// source columns, src/module_solver.f90:745-755, 809-819.  A wave holds 16
+// consecutive l x 4 columns at one w.
+__global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gtab, RunK R,
+                                                   SphState S, cd* __restrict__ rec,
+                                                   int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  Tab T{gtab, R.nknot};
+  int nl = R.lmax - R.lmin + 1, nlb = (nl + 15) / 16;
+  int wv = blockIdx.x;
+  int j = R.nw - wv / nlb;
+  int c = threadIdx.x & 3;
+  int l = R.lmin + (wv % nlb) * 16 + (int)(threadIdx.x >> 2);
+  bool valid = (l >= 1 && l <= R.lmax);
+  int any = wave_min(valid ? 0 : 1);
+  if (any) return;  // no valid lane in this wave
+  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 1);
   cd b[14];
-  const cd* bs = S.bst + (size_t)p * 56 + c * 14;
-  for (int i = 0; i < 14; i++) b[i] = bs[i];
-  int sc4 = 0, nb = 0;
-  BvecRHS f;
-  yint(T, s, R.nknot, R.is, R.ir, R.rs, R.rr, b, f, &sc4, nb);
-  if (c == 0) cnt[5 * p + 3] = nb;
+  for (int q = 0; q < 14; q++) b[q] = valid ? S.bst[p * 56 + c * 14 + q] : C(0.0);
+  int sc4 = 0;
+  int n = march_yint(T, w, L, R.nknot, R.is, R.ir, R.rs, R.rr, b, sc4, sb, BvecM());
+  if (!valid) return;
   cd m[14];
-  for (int i = 0; i < 14; i++) m[i] = S.mfin[(size_t)p * 14 + i];
+  for (int q = 0; q < 14; q++) m[q] = S.mfin[p * 14 + q];
   cd y[6];
   ysol(m, b, y);
   cd fac = S.fac[p] * ipow_huge(S.sc3[p] - sc4);
-  cd* o = rec + (size_t)p * REC + O_YSPH + 6 * c;
-  for (int i = 0; i < 6; i++) o[i] = y[i] * fac;
+  cd* o = rec + p * REC + O_YSPH + 6 * c;
+  for (int q = 0; q < 6; q++) o[q] = y[q] * fac;
+  if (c == 0) cnt[5 * p + 3] = n;
 }
 
 // ------------------------------------------------------------------- k_rad
@@ -664,7 +769,7 @@ __global__ void k_twiddles(int nt, cd* tw) {
 // ====================================================================== API
 struct ysx_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr;
   std::string err;
   Norms N;
   HostModel M;
@@ -676,7 +781,9 @@ struct ysx_ctx {
   cd* d_rec = nullptr;
   int* d_starts = nullptr;
   int* d_cnt = nullptr;
-  cd *d_bst = nullptr, *d_mfin = nullptr, *d_fac = nullptr;
+  cd *d_bst = nullptr, *d_mfin = nullptr, *d_m2 = nullptr, *d_fac = nullptr;
+  int *d_tor_i1 = nullptr, *d_sph_i1 = nullptr;
+  cd *d_tor_y0 = nullptr, *d_sph_m0 = nullptr;
   int* d_sc3 = nullptr;
   double *d_xa = nullptr, *d_xpa = nullptr, *d_xca = nullptr, *d_azep = nullptr;
   cd* d_eia = nullptr;
@@ -684,7 +791,7 @@ struct ysx_ctx {
   cd* d_tw = nullptr;
   cd* d_work = nullptr;
   double* d_seis = nullptr;
-  hipEvent_t ev[10] = {};
+  hipEvent_t ev[16] = {};
   double ms[8] = {0};
   RunK rk;
 };
@@ -711,6 +818,7 @@ void dfree(T*& p) {
 
 void free_run(ysx_ctx* c) {
   dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_cnt); dfree(c->d_bst); dfree(c->d_mfin);
+  dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0); dfree(c->d_sph_m0);
   dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_xa); dfree(c->d_xpa); dfree(c->d_xca);
   dfree(c->d_azep); dfree(c->d_eia); dfree(c->d_spec); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis);
@@ -727,6 +835,7 @@ extern "C" int ysx_create(ysx_ctx** out, int device) {
   ctx->N = make_norms();
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -742,6 +851,7 @@ extern "C" void ysx_destroy(ysx_ctx* ctx) {
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
   delete ctx;
 }
 
@@ -796,6 +906,11 @@ extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const 
   HIPCHK(hipMalloc(&ctx->d_cnt, P * 5 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_bst, P * 56 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_mfin, P * 14 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_m2, P * 14 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_tor_i1, P * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_sph_i1, P * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_tor_y0, P * 2 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_sph_m0, P * 14 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_fac, P * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_sc3, P * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_xa, nt3 * sizeof(double)));
@@ -838,44 +953,66 @@ namespace {
 
 int run_spectra(ysx_ctx* ctx) {
   const RunK& k = ctx->rk;
-  hipStream_t st = ctx->stream;
+  hipStream_t st = ctx->stream, st2 = ctx->stream2;
+  hipEvent_t* ev = ctx->ev;
   size_t lds = ctx->lds_bytes;
   int P = k.npair;
-  dim3 b256(256);
-  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_fac, ctx->d_sc3};
-  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  dim3 b256(256), b64(64);
+  int nl = k.lmax - k.lmin + 1;
+  int nw64 = k.nw * ((nl + 63) / 64), nw16 = k.nw * ((nl + 15) / 16);
+  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
+  StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
+  HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(k_source, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, ctx->d_rec,
                      ctx->d_cnt);
-  HIPCHK(hipEventRecord(ctx->ev[1], st));
-  hipLaunchKernelGGL(k_tor, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, ctx->d_rec,
-                     ctx->d_starts, ctx->d_cnt);
-  HIPCHK(hipEventRecord(ctx->ev[2], st));
-  hipLaunchKernelGGL(k_sph_minor, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, S,
-                     ctx->d_starts, ctx->d_cnt);
-  HIPCHK(hipEventRecord(ctx->ev[3], st));
-  hipLaunchKernelGGL(k_bvec, dim3((4 * P + 255) / 256), b256, lds, st, ctx->d_tab, k, S,
-                     ctx->d_rec, ctx->d_cnt);
-  HIPCHK(hipEventRecord(ctx->ev[4], st));
+  HIPCHK(hipEventRecord(ev[1], st));
+  // the radial (l = 0) solves run beside the marches on a second stream
+  HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+  HIPCHK(hipEventRecord(ev[10], st2));
   if (k.lmin == 0)
-    hipLaunchKernelGGL(k_rad, dim3((k.nw + 63) / 64), dim3(64), lds, st, ctx->d_tab, k,
+    hipLaunchKernelGGL(k_rad, dim3((k.nw + 63) / 64), dim3(64), lds, st2, ctx->d_tab, k,
                        ctx->d_rec, ctx->d_cnt);
-  HIPCHK(hipEventRecord(ctx->ev[5], st));
+  HIPCHK(hipEventRecord(ev[11], st2));
+  hipLaunchKernelGGL(k_start, dim3((P + 255) / 256), b256, 0, st, ctx->d_tab, k, SD,
+                     ctx->d_starts);
+  HIPCHK(hipEventRecord(ev[2], st));
+  hipLaunchKernelGGL(k_tor_march, dim3(nw64), b64, 0, st, ctx->d_tab, k, SD, ctx->d_rec,
+                     ctx->d_cnt);
+  HIPCHK(hipEventRecord(ev[3], st));
+  hipLaunchKernelGGL(k_minor_march, dim3(nw64), b64, 0, st, ctx->d_tab, k, SD, S, ctx->d_cnt);
+  HIPCHK(hipEventRecord(ev[4], st));
+  hipLaunchKernelGGL(k_bvec_march, dim3(nw16), b64, 0, st, ctx->d_tab, k, S, ctx->d_rec,
+                     ctx->d_cnt);
+  HIPCHK(hipEventRecord(ev[5], st));
+  HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
+  HIPCHK(hipEventRecord(ev[6], st));
   hipLaunchKernelGGL(k_assemble, dim3((P + 255) / 256), b256, 0, st, k, ctx->d_rec);
-  HIPCHK(hipEventRecord(ctx->ev[6], st));
+  HIPCHK(hipEventRecord(ev[7], st));
   int nwr = k.nw * k.nr;
   hipLaunchKernelGGL(k_sum, dim3((nwr + 255) / 256), b256, 0, st, k, ctx->d_rec, ctx->d_xa,
                      ctx->d_xpa, ctx->d_xca, ctx->d_eia, ctx->d_spec);
-  HIPCHK(hipEventRecord(ctx->ev[7], st));
+  HIPCHK(hipEventRecord(ev[8], st));
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventSynchronize(ctx->ev[7]));
-  for (int i = 0; i < 7; i++) {
+  HIPCHK(hipEventSynchronize(ev[8]));
+  auto el = [&](int a, int b) {
     float t = 0.f;
-    HIPCHK(hipEventElapsedTime(&t, ctx->ev[i], ctx->ev[i + 1]));
-    ctx->ms[i] = t;
-  }
+    (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+    return (double)t;
+  };
+  ctx->ms[0] = el(0, 1) + el(1, 2);  // source vectors + starts
+  ctx->ms[1] = el(2, 3);
+  ctx->ms[2] = el(3, 4);
+  ctx->ms[3] = el(4, 5);
+  ctx->ms[4] = el(10, 11);
+  ctx->ms[5] = el(6, 7);
+  ctx->ms[6] = el(7, 8);
   ctx->have_spec = true;
   return YSX_OK;
 }
+
+}  // namespace
+
+namespace {
 
 int run_seis(ysx_ctx* ctx) {
   const HostRun& R = ctx->R;
@@ -888,14 +1025,14 @@ int run_seis(ysx_ctx* ctx) {
   K.df = R.df; K.dt = R.dt; K.ep = R.ep; K.f11 = R.f11; K.f12 = R.f12; K.f21 = R.f21;
   K.f22 = R.f22;
   K.norm = R.out == 0 ? ctx->N.r_norm : (R.out == 1 ? ctx->N.vel_norm : ctx->N.acl_norm);
-  HIPCHK(hipEventRecord(ctx->ev[8], st));
+  HIPCHK(hipEventRecord(ctx->ev[12], st));
   hipLaunchKernelGGL(k_seis, dim3(R.nr), dim3(1024), 0, st, K, ctx->d_spec, ctx->d_tw,
                      ctx->d_work, ctx->d_azep, ctx->d_seis);
-  HIPCHK(hipEventRecord(ctx->ev[9], st));
+  HIPCHK(hipEventRecord(ctx->ev[13], st));
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventSynchronize(ctx->ev[9]));
+  HIPCHK(hipEventSynchronize(ctx->ev[13]));
   float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, ctx->ev[8], ctx->ev[9]));
+  HIPCHK(hipEventElapsedTime(&t, ctx->ev[12], ctx->ev[13]));
   ctx->ms[7] = t;
   return YSX_OK;
 }
