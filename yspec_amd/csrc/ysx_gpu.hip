@@ -187,21 +187,22 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab, 
 // with wave-uniform arguments
 template <class F>
 __device__ int march_yint(const Tab& T, const WS& w, const LS& L, int nk, int i1, int i2,
-                          double r1, double r2, cd* y, int& sc, double* sb, const F& f) {
+                          double r1, double r2, cd* y, int& sc, double* sb, const F& f,
+                          cd* slot = nullptr) {
   int n = 0;
   if (r2 > r1) {
     for (int i = i1; i <= i2; i++) {
       double a = (i == i1) ? r1 : T(F_R, i);
       double b = (i == i2) ? r2 : T(F_R, i + 1);
       if (a == b) continue;
-      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f);
+      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
     }
   } else if (r2 < r1) {
     for (int i = i1; i >= i2; i--) {
       double a = (i == i1) ? r1 : T(F_R, i + 1);
       double b = (i == i2) ? r2 : T(F_R, i);
       if (a == b) continue;
-      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f);
+      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
     }
   }
   return n;
@@ -297,9 +298,9 @@ struct SphState {
 
 // minor vector from the surface down to radius rb in knot interval ib
 // (through the ocean as a fluid when there is one), src/module_solver.f90:730-741
-__device__ inline int minors_from_top(const Tab& T, const WS& w, const LS& L, const RunK& R,
+__device__ __forceinline__ int minors_from_top(const Tab& T, const WS& w, const LS& L, const RunK& R,
                                       int itopk, int ib, double rb, cd* m, double* sb,
-                                      int& nsol, int& nflu) {
+                                      int& nsol, int& nflu, cd* slot) {
   int nk = R.nknot, sc2 = 0;
   for (int i = 0; i < 14; i++) m[i] = C(0.0);
   m[0] = C(1.0);
@@ -308,9 +309,9 @@ __device__ inline int minors_from_top(const Tab& T, const WS& w, const LS& L, co
                        FluidMinorM());
     cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
     fluid_2_solid(mf, m);
-    nsol += march_yint(T, w, L, nk, R.nsl, ib, T(F_R, R.nsl), rb, m, sc2, sb, SolidMinorM());
+    nsol += march_yint(T, w, L, nk, R.nsl, ib, T(F_R, R.nsl), rb, m, sc2, sb, SolidMinorM(), slot);
   } else {
-    nsol += march_yint(T, w, L, nk, itopk, ib, T(F_R, nk), rb, m, sc2, sb, SolidMinorM());
+    nsol += march_yint(T, w, L, nk, itopk, ib, T(F_R, nk), rb, m, sc2, sb, SolidMinorM(), slot);
   }
   return sc2;
 }
@@ -319,6 +320,8 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
                                                     StartData SD, SphState S,
                                                     int* __restrict__ cnt) {
   __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
   Tab T{gtab, R.nknot};
   int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64;
   int wv = blockIdx.x;
@@ -338,17 +341,6 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
   double rtop = up ? rs : rr;
   int nsol = 0, nflu = 0;
   cd m[14];
-  // (1) the surface minor vector ms2, down to the lower of source/receiver's partner point
-  {
-    int dummy = 0;
-    if (up)
-      minors_from_top(T, w, L, R, nk - 1, ir, rr, m, sb, nsol, nflu);
-    else
-      minors_from_top(T, w, L, R, nk, is, rs, m, sb, nsol, nflu);
-    (void)dummy;
-    if (valid)
-      for (int i = 0; i < 14; i++) S.m2[p * 14 + i] = m[i];
-  }
   // (2) ms1 from each lane's start level up to the lower point
   double rt1 = up ? fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2)) : T(F_R, 1);
   for (int i = 0; i < 14; i++) m[i] = C(0.0);
@@ -374,7 +366,7 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
     if (fluid_i)
       n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, FluidMinorM());
     else
-      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, SolidMinorM());
+      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, SolidMinorM(), slot);
     if (i >= i1) (fluid_i ? nflu : nsol) += n;
   }
   const int cols[4] = {0, 1, 3, 4};
@@ -386,11 +378,11 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
         b_start_unit(m, cols[c], bb);
         for (int q = 0; q < 14; q++) S.bst[p * 56 + c * 14 + q] = bb[q];
       }
-    if (rr > rs) nsol += march_yint(T, w, L, nk, is, ir, rs, rr, m, sc3, sb, SolidMinorM());
+    if (rr > rs) nsol += march_yint(T, w, L, nk, is, ir, rs, rr, m, sc3, sb, SolidMinorM(), slot);
   } else {
     if (valid)
       for (int q = 0; q < 14; q++) S.mfin[p * 14 + q] = m[q];
-    if (rs > rr) nsol += march_yint(T, w, L, nk, ir, is, rr, rs, m, sc3, sb, SolidMinorM());
+    if (rs > rr) nsol += march_yint(T, w, L, nk, ir, is, rr, rs, m, sc3, sb, SolidMinorM(), slot);
   }
   if (!valid) return;
   cd m2[14];
@@ -409,6 +401,36 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
     S.fac[p] = C(1.0);
   }
   S.sc3[p] = sc3;
+  cnt[5 * p + 1] += nsol;
+  cnt[5 * p + 2] += nflu;
+}
+
+// The surface minor vector ms2 of every pair (src/module_solver.f90:730-741,
+// 793-804): from the surface down to the receiver (receiver above the
+// source) or to the source.  Zero steps for a receiver at the surface.
+__global__ void __launch_bounds__(64) k_minor_top(const double* __restrict__ gtab, RunK R,
+                                                  SphState S, int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  Tab T{gtab, R.nknot};
+  int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64;
+  int wv = blockIdx.x;
+  int j = R.nw - wv / nlb;
+  int l = R.lmin + (wv % nlb) * 64 + (int)threadIdx.x;
+  bool valid = (l >= 1 && l <= R.lmax);
+  if (wave_min(valid ? 0 : 1)) return;
+  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 1);
+  int nsol = 0, nflu = 0;
+  cd m[14];
+  if (R.rr >= R.rs)
+    minors_from_top(T, w, L, R, R.nknot - 1, R.ir, R.rr, m, sb, nsol, nflu, slot);
+  else
+    minors_from_top(T, w, L, R, R.nknot, R.is, R.rs, m, sb, nsol, nflu, slot);
+  if (!valid) return;
+  for (int i = 0; i < 14; i++) S.m2[p * 14 + i] = m[i];
   cnt[5 * p + 1] = nsol;
   cnt[5 * p + 2] = nflu;
 }
@@ -421,6 +443,8 @@ __global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gt
                                                    SphState S, cd* __restrict__ rec,
                                                    int* __restrict__ cnt) {
   __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
   Tab T{gtab, R.nknot};
   int nl = R.lmax - R.lmin + 1, nlb = (nl + 15) / 16;
   int wv = blockIdx.x;
@@ -436,7 +460,7 @@ __global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gt
   cd b[14];
   for (int q = 0; q < 14; q++) b[q] = valid ? S.bst[p * 56 + c * 14 + q] : C(0.0);
   int sc4 = 0;
-  int n = march_yint(T, w, L, R.nknot, R.is, R.ir, R.rs, R.rr, b, sc4, sb, BvecM());
+  int n = march_yint(T, w, L, R.nknot, R.is, R.ir, R.rs, R.rr, b, sc4, sb, BvecM(), slot);
   if (!valid) return;
   cd m[14];
   for (int q = 0; q < 14; q++) m[q] = S.mfin[p * 14 + q];
@@ -979,6 +1003,7 @@ int run_spectra(ysx_ctx* ctx) {
   hipLaunchKernelGGL(k_tor_march, dim3(nw64), b64, 0, st, ctx->d_tab, k, SD, ctx->d_rec,
                      ctx->d_cnt);
   HIPCHK(hipEventRecord(ev[3], st));
+  hipLaunchKernelGGL(k_minor_top, dim3(nw64), b64, 0, st, ctx->d_tab, k, S, ctx->d_cnt);
   hipLaunchKernelGGL(k_minor_march, dim3(nw64), b64, 0, st, ctx->d_tab, k, SD, S, ctx->d_cnt);
   HIPCHK(hipEventRecord(ev[4], st));
   hipLaunchKernelGGL(k_bvec_march, dim3(nw16), b64, 0, st, ctx->d_tab, k, S, ctx->d_rec,

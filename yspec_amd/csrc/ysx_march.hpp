@@ -12,6 +12,21 @@
 
 namespace ysx {
 
+// complex multiply-accumulate chains: acc +- a*b in 4 FP64 FMAs
+// (complex a) or 2 (real a)
+__device__ inline cd cmad(cd c, cd a, cd b) {
+  return {fma(-a.y, b.y, fma(a.x, b.x, c.x)), fma(a.y, b.x, fma(a.x, b.y, c.y))};
+}
+__device__ inline cd cmsb(cd c, cd a, cd b) {
+  return {fma(a.y, b.y, fma(-a.x, b.x, c.x)), fma(-a.y, b.x, fma(-a.x, b.y, c.y))};
+}
+__device__ inline cd rmad(cd c, double a, cd b) { return {fma(a, b.x, c.x), fma(a, b.y, c.y)}; }
+__device__ inline cd rmsb(cd c, double a, cd b) { return {fma(-a, b.x, c.x), fma(-a, b.y, c.y)}; }
+__device__ inline cd rmul(double a, cd b) { return {a * b.x, a * b.y}; }
+__device__ inline cd cmul_(cd a, cd b) {
+  return {fma(-a.y, b.y, a.x * b.x), fma(a.y, b.x, a.x * b.y)};
+}
+
 constexpr int CH = 10;       // steps per shared-coefficient chunk (6*CH radii)
 constexpr int BASE_MAX = 18;  // doubles per stage radius
 
@@ -99,78 +114,117 @@ struct SolidB {
   }
 };
 
-// sph_minors_solid_derivs, src/module_int.f90:818-871
-__device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, cd* d) {
-  double t21 = A.t21, t31 = A.t31, k33 = A.k33;
-  cd a123 = A.t11 + (A.t22 + A.t33), a126 = A.t11 + (A.t22 - A.t33),
-     a135 = A.t11 - (A.t22 - A.t33), a234 = (A.t22 + A.t33) - A.t11;
-  d[0] = k33 * m[3] - A.k22 * m[5] + A.k11 * m[9] + m[0] * a123;
-  d[1] = m[0] * A.s13 - A.k22 * m[6] - m[2] * t21 + m[1] * A.t22 - m[3] * t31;
-  d[2] = m[0] * A.s23 - A.k11 * m[10] + m[2] * A.t11 - m[1] * A.t12;
-  d[3] = A.k22 * m[8] - A.k11 * m[11] + m[3] * a126;
-  d[4] = m[9] * A.t12 - k33 * m[7] - m[0] * A.s12 - m[5] * t21 + m[2] * t31 + m[4] * A.t33;
-  d[5] = m[5] * a135 - k33 * m[8] - A.k11 * m[12] - m[0] * A.s22 - 2.0 * (m[4] * A.t12);
-  d[6] = m[2] * A.s12 + m[5] * A.s13 - m[1] * A.s22 - m[4] * A.s23 + m[10] * A.t12 -
-         m[6] * A.t22 + m[8] * t31;
-  d[7] = m[3] * A.s12 - m[2] * A.s13 - m[1] * A.s23 + m[11] * A.t12 - m[8] * t21 - m[7] * A.t33;
-  d[8] = A.k11 * m[13] + m[3] * A.s22 - 2.0 * (m[2] * A.s23) - 2.0 * (m[7] * A.t12) -
-         m[8] * a234;
-  d[9] = A.k22 * m[12] - k33 * m[11] + m[0] * A.s11 + 2.0 * (m[4] * t21) - 2.0 * (m[1] * t31) +
-         m[9] * a234;
-  d[10] = m[1] * A.s12 - m[2] * A.s11 - m[4] * A.s13 - m[9] * A.s23 - m[10] * A.t11 +
-          m[6] * t21 - m[7] * t31;
-  d[11] = 2.0 * (m[1] * A.s13) - A.k22 * m[13] - m[3] * A.s11 + 2.0 * (m[7] * t21) -
-          m[11] * a135;
-  d[12] = k33 * m[13] - m[5] * A.s11 + 2.0 * (m[4] * A.s12) + m[9] * A.s22 + 2.0 * (m[6] * t31) -
-          m[12] * a126;
-  d[13] = m[8] * A.s11 - 2.0 * (m[7] * A.s12) + 2.0 * (m[6] * A.s13) - m[11] * A.s22 +
-          2.0 * (m[10] * A.s23) - m[13] * a123;
+// sph_minors_solid_derivs, src/module_int.f90:818-871, as FMA chains
+template <class Sink>
+__device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, Sink&& sink) {
+  const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
+               s23 = A.s23;
+  const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
+           s22 = A.s22;
+  const double tp = t22 + t33, tm = t22 - t33;
+  const cd a123 = t11 + tp, a126 = t11 + tm, a135 = t11 - tm, a234 = tp - t11;
+  const cd t12x2 = 2.0 * t12, s12x2 = 2.0 * s12;
+  cd acc;
+  acc = rmul(k33, m[3]); acc = cmsb(acc, k22, m[5]); acc = cmad(acc, k11, m[9]);
+  sink(0, cmad(acc, m[0], a123));
+  acc = rmul(s13, m[0]); acc = cmsb(acc, k22, m[6]); acc = rmsb(acc, t21, m[2]);
+  acc = rmad(acc, t22, m[1]); sink(1, rmsb(acc, t31, m[3]));
+  acc = rmul(s23, m[0]); acc = cmsb(acc, k11, m[10]); acc = cmad(acc, t11, m[2]);
+  sink(2, cmsb(acc, t12, m[1]));
+  acc = cmul_(k22, m[8]); acc = cmsb(acc, k11, m[11]); sink(3, cmad(acc, a126, m[3]));
+  acc = cmul_(t12, m[9]); acc = rmsb(acc, k33, m[7]); acc = cmsb(acc, s12, m[0]);
+  acc = rmsb(acc, t21, m[5]); acc = rmad(acc, t31, m[2]); sink(4, rmad(acc, t33, m[4]));
+  acc = cmul_(a135, m[5]); acc = rmsb(acc, k33, m[8]); acc = cmsb(acc, k11, m[12]);
+  acc = cmsb(acc, s22, m[0]); sink(5, cmsb(acc, t12x2, m[4]));
+  acc = cmul_(s12, m[2]); acc = rmad(acc, s13, m[5]); acc = cmsb(acc, s22, m[1]);
+  acc = rmsb(acc, s23, m[4]); acc = cmad(acc, t12, m[10]); acc = rmsb(acc, t22, m[6]);
+  sink(6, rmad(acc, t31, m[8]));
+  acc = cmul_(s12, m[3]); acc = rmsb(acc, s13, m[2]); acc = rmsb(acc, s23, m[1]);
+  acc = cmad(acc, t12, m[11]); acc = rmsb(acc, t21, m[8]); sink(7, rmsb(acc, t33, m[7]));
+  acc = cmul_(k11, m[13]); acc = cmad(acc, s22, m[3]); acc = rmsb(acc, 2.0 * s23, m[2]);
+  acc = cmsb(acc, t12x2, m[7]); sink(8, cmsb(acc, a234, m[8]));
+  acc = cmul_(k22, m[12]); acc = rmsb(acc, k33, m[11]); acc = cmad(acc, s11, m[0]);
+  acc = rmad(acc, 2.0 * t21, m[4]); acc = rmsb(acc, 2.0 * t31, m[1]);
+  sink(9, cmad(acc, a234, m[9]));
+  acc = cmul_(s12, m[1]); acc = cmsb(acc, s11, m[2]); acc = rmsb(acc, s13, m[4]);
+  acc = rmsb(acc, s23, m[9]); acc = cmsb(acc, t11, m[10]); acc = rmad(acc, t21, m[6]);
+  sink(10, rmsb(acc, t31, m[7]));
+  acc = rmul(2.0 * s13, m[1]); acc = cmsb(acc, k22, m[13]); acc = cmsb(acc, s11, m[3]);
+  acc = rmad(acc, 2.0 * t21, m[7]); sink(11, cmsb(acc, a135, m[11]));
+  acc = rmul(k33, m[13]); acc = cmsb(acc, s11, m[5]); acc = cmad(acc, s12x2, m[4]);
+  acc = cmad(acc, s22, m[9]); acc = rmad(acc, 2.0 * t31, m[6]); sink(12, cmsb(acc, a126, m[12]));
+  acc = cmul_(s11, m[8]); acc = cmsb(acc, s12x2, m[7]); acc = rmad(acc, 2.0 * s13, m[6]);
+  acc = cmsb(acc, s22, m[11]); acc = rmad(acc, 2.0 * s23, m[10]);
+  sink(13, cmsb(acc, a123, m[13]));
 }
 
-// sph_bvec_solid_derivs, src/module_int.f90:1016-1057
-__device__ inline void bvec_derivs(const SolidA& A, const cd* b, cd* d) {
-  double t21 = A.t21, t31 = A.t31, k33 = A.k33;
-  cd t1122 = A.t11 + A.t22, t1133 = A.t11 + A.t33;
-  double t2233 = A.t22 + A.t33;
-  d[0] = A.s11 * b[6] + A.s12 * (b[7] - b[2]) + A.s13 * b[8] - A.s22 * b[3] - A.s23 * b[4] -
-         t1122 * b[0] + t31 * b[5];
-  d[1] = A.s11 * b[9] + A.s12 * b[10] - A.s13 * (2.0 * b[2] + b[7]) - A.s23 * b[3] -
-         t21 * b[5] - t1133 * b[1];
-  d[2] = A.s12 * b[11] + A.s13 * b[12] + A.t12 * b[3] - t21 * b[6] - t31 * b[9];
-  d[3] = A.s13 * b[13] - A.k22 * b[0] - A.s11 * b[11] + t21 * (b[2] - b[7]) +
-         (A.t22 - A.t11) * b[3] - t31 * b[10];
-  d[4] = t31 * (2.0 * b[2] + b[7]) - k33 * b[1] - A.s11 * b[12] - A.s12 * b[13] - t21 * b[8] +
-         (A.t33 - A.t11) * b[4];
-  d[5] = A.s12 * b[9] - A.s13 * b[6] + A.s22 * b[10] - A.s23 * (b[2] + 2.0 * b[7]) -
-         A.t12 * b[1] - t2233 * b[5];
-  d[6] = A.k11 * b[0] + A.s22 * b[11] + A.s23 * b[12] - A.t12 * (b[2] - b[7]) +
-         (A.t11 - A.t22) * b[6];
-  d[7] = A.s23 * b[13] - A.s12 * b[11] - A.t12 * b[3] + t21 * b[6];
-  d[8] = t31 * b[6] - k33 * b[5] - A.s12 * b[12] - A.s22 * b[13] - A.t12 * b[4] +
-         (A.t33 - A.t22) * b[8];
-  d[9] = A.k11 * b[1] + A.s23 * b[11] + A.t12 * b[10] + (A.t11 - A.t33) * b[9];
-  d[10] = A.k22 * b[5] - A.s13 * b[11] + t21 * b[9] + (A.t22 - A.t33) * b[10];
-  d[11] = A.k22 * b[6] - A.k11 * b[3] + t1122 * b[11];
-  d[12] = k33 * b[9] - A.k11 * b[4] + A.t12 * b[13] + t1133 * b[12];
-  d[13] = k33 * b[10] - A.k22 * b[8] + t21 * b[12] - t31 * b[11] + t2233 * b[13];
+// sph_bvec_solid_derivs, src/module_int.f90:1016-1057, as FMA chains
+template <class Sink>
+__device__ inline void bvec_derivs(const SolidA& A, const cd* b, Sink&& sink) {
+  const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
+               s23 = A.s23;
+  const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
+           s22 = A.s22;
+  const cd t1122 = t11 + t22, t1133 = t11 + t33;
+  const double t2233 = t22 + t33;
+  const cd b38m = b[7] - b[2], b2p8 = 2.0 * b[2] + b[7], b3p28 = b[2] + 2.0 * b[7];
+  cd acc;
+  acc = cmul_(s11, b[6]); acc = cmad(acc, s12, b38m); acc = rmad(acc, s13, b[8]);
+  acc = cmsb(acc, s22, b[3]); acc = rmsb(acc, s23, b[4]); acc = cmsb(acc, t1122, b[0]);
+  sink(0, rmad(acc, t31, b[5]));
+  acc = cmul_(s11, b[9]); acc = cmad(acc, s12, b[10]); acc = rmsb(acc, s13, b2p8);
+  acc = rmsb(acc, s23, b[3]); acc = rmsb(acc, t21, b[5]); sink(1, cmsb(acc, t1133, b[1]));
+  acc = cmul_(s12, b[11]); acc = rmad(acc, s13, b[12]); acc = cmad(acc, t12, b[3]);
+  acc = rmsb(acc, t21, b[6]); sink(2, rmsb(acc, t31, b[9]));
+  acc = rmul(s13, b[13]); acc = cmsb(acc, k22, b[0]); acc = cmsb(acc, s11, b[11]);
+  acc = rmsb(acc, t21, b38m); acc = cmad(acc, t22 - t11, b[3]); sink(3, rmsb(acc, t31, b[10]));
+  acc = rmul(t31, b2p8); acc = rmsb(acc, k33, b[1]); acc = cmsb(acc, s11, b[12]);
+  acc = cmsb(acc, s12, b[13]); acc = rmsb(acc, t21, b[8]); sink(4, cmad(acc, t33 - t11, b[4]));
+  acc = cmul_(s12, b[9]); acc = rmsb(acc, s13, b[6]); acc = cmad(acc, s22, b[10]);
+  acc = rmsb(acc, s23, b3p28); acc = cmsb(acc, t12, b[1]); sink(5, rmsb(acc, t2233, b[5]));
+  acc = cmul_(k11, b[0]); acc = cmad(acc, s22, b[11]); acc = rmad(acc, s23, b[12]);
+  acc = cmad(acc, t12, b38m); sink(6, cmad(acc, t11 - t22, b[6]));
+  acc = rmul(s23, b[13]); acc = cmsb(acc, s12, b[11]); acc = cmsb(acc, t12, b[3]);
+  sink(7, rmad(acc, t21, b[6]));
+  acc = rmul(t31, b[6]); acc = rmsb(acc, k33, b[5]); acc = cmsb(acc, s12, b[12]);
+  acc = cmsb(acc, s22, b[13]); acc = cmsb(acc, t12, b[4]); sink(8, rmad(acc, t33 - t22, b[8]));
+  acc = cmul_(k11, b[1]); acc = rmad(acc, s23, b[11]); acc = cmad(acc, t12, b[10]);
+  sink(9, cmad(acc, t11 - t33, b[9]));
+  acc = cmul_(k22, b[5]); acc = rmsb(acc, s13, b[11]); acc = rmad(acc, t21, b[9]);
+  sink(10, rmad(acc, t22 - t33, b[10]));
+  acc = cmul_(k22, b[6]); acc = cmsb(acc, k11, b[3]); sink(11, cmad(acc, t1122, b[11]));
+  acc = rmul(k33, b[9]); acc = cmsb(acc, k11, b[4]); acc = cmad(acc, t12, b[13]);
+  sink(12, cmad(acc, t1133, b[12]));
+  acc = rmul(k33, b[10]); acc = cmsb(acc, k22, b[8]); acc = rmad(acc, t21, b[12]);
+  acc = rmsb(acc, t31, b[11]); sink(13, rmad(acc, t2233, b[13]));
 }
 
 struct SolidMinorM : SolidB {
   static constexpr int N = 14;
-  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
-                                    cd* d) const {
+  template <class Sink>
+  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+                                Sink&& sink) const {
     SolidA A;
     amat(b, L, w.pibigg, A);
-    solid_minor_derivs(A, y, d);
+    solid_minor_derivs(A, y, sink);
+  }
+  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
+                                    cd* d) const {
+    stream(b, L, w, y, [&](int i, cd v) { d[i] = v; });
   }
 };
 struct BvecM : SolidB {
   static constexpr int N = 14;
-  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
-                                    cd* d) const {
+  template <class Sink>
+  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+                                Sink&& sink) const {
     SolidA A;
     amat(b, L, w.pibigg, A);
-    bvec_derivs(A, y, d);
+    bvec_derivs(A, y, sink);
+  }
+  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
+                                    cd* d) const {
+    stream(b, L, w, y, [&](int i, cd v) { d[i] = v; });
   }
 };
 
@@ -207,11 +261,12 @@ struct FluidMinorM {
     cd s11 = z2 * C(b[8], b[9]) + C(b[10], b[11]);
     cd s12 = (z2 * rrho) * fa - L.llp1 * rrho * ri;
     cd s22 = (z2 * rrho) * fb;
-    d[0] = k22 * m[2] - k11 * m[3] + m[0] * (t11 + t22);
-    d[1] = m[0] * s12 + m[3] * t12 - t21 * m[2];
-    d[2] = k11 * m[4] + m[0] * s22 - 2.0 * (m[1] * t12) + m[2] * (t11 - t22);
-    d[3] = 2.0 * t21 * m[1] - k22 * m[4] - m[0] * s11 + m[3] * (t22 - t11);
-    d[4] = m[2] * s11 - 2.0 * (m[1] * s12) - m[3] * s22 - m[4] * (t11 + t22);
+    cd tp = t11 + t22, tm = t11 - t22;
+    d[0] = cmad(cmsb(rmul(k22, m[2]), k11, m[3]), tp, m[0]);
+    d[1] = rmsb(cmad(cmul_(s12, m[0]), t12, m[3]), t21, m[2]);
+    d[2] = cmad(cmsb(cmad(cmul_(k11, m[4]), s22, m[0]), 2.0 * t12, m[1]), tm, m[2]);
+    d[3] = cmsb(cmsb(rmsb(rmul(2.0 * t21, m[1]), k22, m[4]), s11, m[0]), tm, m[3]);
+    d[4] = cmsb(cmsb(cmsb(cmul_(s11, m[2]), 2.0 * s12, m[1]), s22, m[3]), tp, m[4]);
   }
 };
 
@@ -235,8 +290,8 @@ struct TorM {
     double t = 2.0 * b[0];
     cd k = C(b[1], b[2]);
     cd sv = L.zeta2m2 * C(b[3], b[4]) + C(b[5], b[6]);
-    d[0] = t * y[0] + k * y[1];
-    d[1] = sv * y[0] - t * y[1];
+    d[0] = cmad(rmul(t, y[0]), k, y[1]);
+    d[1] = rmsb(cmul_(sv, y[0]), t, y[1]);
   }
 };
 
@@ -247,41 +302,109 @@ template <class F>
 __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, double h, cd* y,
                                  const F& f) {
   constexpr int N = F::N, BS = F::BS;
-  constexpr double b21 = 0.2, b31 = 3.0 / 40.0, b32 = 9.0 / 40.0, b41 = 0.3, b42 = -0.9,
-                   b43 = 1.2, b51 = -11.0 / 54.0, b52 = 2.5, b53 = -70.0 / 27.0,
-                   b54 = 35.0 / 27.0, b61 = 1631.0 / 55296.0, b62 = 175.0 / 512.0,
-                   b63 = 575.0 / 13824.0, b64 = 44275.0 / 110592.0, b65 = 253.0 / 4096.0,
-                   c1 = 37.0 / 378.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0,
-                   c6 = 512.0 / 1771.0;
+  // Cash-Karp tableau (src/module_int.f90:2039-2049), pre-multiplied by h
+  const double hb21 = h * 0.2, hb31 = h * (3.0 / 40.0), hb32 = h * (9.0 / 40.0),
+               hb41 = h * 0.3, hb42 = h * -0.9, hb43 = h * 1.2, hb51 = h * (-11.0 / 54.0),
+               hb52 = h * 2.5, hb53 = h * (-70.0 / 27.0), hb54 = h * (35.0 / 27.0),
+               hb61 = h * (1631.0 / 55296.0), hb62 = h * (175.0 / 512.0),
+               hb63 = h * (575.0 / 13824.0), hb64 = h * (44275.0 / 110592.0),
+               hb65 = h * (253.0 / 4096.0), hc1 = h * (37.0 / 378.0), hc3 = h * (250.0 / 621.0),
+               hc4 = h * (125.0 / 594.0), hc6 = h * (512.0 / 1771.0);
   cd k1[N], k2[N], k3[N], t[N];
   f(sb, L, w, y, k1);
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = y[i] + (b21 * h) * k1[i];
+  for (int i = 0; i < N; i++) t[i] = rmad(y[i], hb21, k1[i]);
+  __builtin_amdgcn_sched_barrier(0);
   f(sb + BS, L, w, t, k2);
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = y[i] + h * (b31 * k1[i] + b32 * k2[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], hb31, k1[i]), hb32, k2[i]);
+  __builtin_amdgcn_sched_barrier(0);
   f(sb + 2 * BS, L, w, t, k3);
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = y[i] + h * (b41 * k1[i] + b42 * k2[i] + b43 * k3[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(rmad(rmad(y[i], hb41, k1[i]), hb42, k2[i]), hb43, k3[i]);
   {
     cd k4[N];
+    __builtin_amdgcn_sched_barrier(0);
     f(sb + 3 * BS, L, w, t, k4);
 #pragma unroll
     for (int i = 0; i < N; i++) {
-      cd p5 = b51 * k1[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i];
-      cd p6 = b61 * k1[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i];
-      cd po = c1 * k1[i] + c3 * k3[i] + c4 * k4[i];
-      t[i] = y[i] + h * p5;
+      cd t5 = rmad(rmad(rmad(rmad(y[i], hb51, k1[i]), hb52, k2[i]), hb53, k3[i]), hb54, k4[i]);
+      cd p6 = rmad(rmad(rmad(rmul(hb61, k1[i]), hb62, k2[i]), hb63, k3[i]), hb64, k4[i]);
+      cd po = rmad(rmad(rmul(hc1, k1[i]), hc3, k3[i]), hc4, k4[i]);
+      t[i] = t5;
       k2[i] = p6;
       k1[i] = po;
     }
   }
+  __builtin_amdgcn_sched_barrier(0);
   f(sb + 4 * BS, L, w, t, k3);  // k5
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = y[i] + h * (k2[i] + b65 * k3[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(y[i] + k2[i], hb65, k3[i]);
+  __builtin_amdgcn_sched_barrier(0);
   f(sb + 5 * BS, L, w, t, k3);  // k6
 #pragma unroll
-  for (int i = 0; i < N; i++) y[i] = y[i] + h * (k1[i] + c6 * k3[i]);
+  for (int i = 0; i < N; i++) y[i] = rmad(y[i] + k1[i], hc6, k3[i]);
+}
+
+// Cash-Karp step with two stage vectors kept in per-lane LDS slots.
+// slot points at this lane's first slot entry; entry (s, i) is at
+// slot[(s * N + i) * 64] (lane-contiguous, conflict-free).  VGPRs hold only
+// y, the stage input t and one accumulator u; every RHS output is consumed
+// as soon as it is produced.  Same arithmetic as rk_step_b.
+template <class F>
+__device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, double h, cd* y,
+                                   const F& f, cd* slot) {
+  constexpr int N = F::N, BS = F::BS;
+  const double hb21 = h * 0.2, hb31 = h * (3.0 / 40.0), hb32 = h * (9.0 / 40.0),
+               hb41 = h * 0.3, hb42 = h * -0.9, hb43 = h * 1.2, hb51 = h * (-11.0 / 54.0),
+               hb52 = h * 2.5, hb53 = h * (-70.0 / 27.0), hb54 = h * (35.0 / 27.0),
+               hb61 = h * (1631.0 / 55296.0), hb62 = h * (175.0 / 512.0),
+               hb63 = h * (575.0 / 13824.0), hb64 = h * (44275.0 / 110592.0),
+               hb65 = h * (253.0 / 4096.0), hc1 = h * (37.0 / 378.0), hc3 = h * (250.0 / 621.0),
+               hc4 = h * (125.0 / 594.0), hc6 = h * (512.0 / 1771.0);
+  cd* S1 = slot;
+  cd* S2 = slot + N * 64;
+  cd t[N], u[N];
+  // stage 1: k1 -> S1, t = y + h b21 k1
+  f.stream(sb, L, w, y, [&](int i, cd d) {
+    S1[i * 64] = d;
+    t[i] = rmad(y[i], hb21, d);
+  });
+  __builtin_amdgcn_sched_barrier(0);
+  // stage 2: k2 -> u
+  f.stream(sb + BS, L, w, t, [&](int i, cd d) { u[i] = d; });
+#pragma unroll
+  for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], hb31, S1[i * 64]), hb32, u[i]);
+  __builtin_amdgcn_sched_barrier(0);
+  // stage 3: k3 -> S2; then t = t4, S1 = A5, S2 = A6, u = Aout
+  f.stream(sb + 2 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = d; });
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    cd k1 = S1[i * 64], k2 = u[i], k3 = S2[i * 64];
+    t[i] = rmad(rmad(rmad(y[i], hb41, k1), hb42, k2), hb43, k3);
+    S1[i * 64] = rmad(rmad(rmul(hb51, k1), hb52, k2), hb53, k3);
+    S2[i * 64] = rmad(rmad(rmul(hb61, k1), hb62, k2), hb63, k3);
+    u[i] = rmad(rmul(hc1, k1), hc3, k3);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  // stage 4: fold k4 into A5, A6, Aout
+  f.stream(sb + 3 * BS, L, w, t, [&](int i, cd d) {
+    S1[i * 64] = rmad(S1[i * 64], hb54, d);
+    S2[i * 64] = rmad(S2[i * 64], hb64, d);
+    u[i] = rmad(u[i], hc4, d);
+  });
+#pragma unroll
+  for (int i = 0; i < N; i++) t[i] = y[i] + S1[i * 64];
+  __builtin_amdgcn_sched_barrier(0);
+  // stage 5: fold k5 into A6
+  f.stream(sb + 4 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = rmad(S2[i * 64], hb65, d); });
+#pragma unroll
+  for (int i = 0; i < N; i++) t[i] = y[i] + S2[i * 64];
+  __builtin_amdgcn_sched_barrier(0);
+  // stage 6: fold k6 into Aout, then y += Aout
+  f.stream(sb + 5 * BS, L, w, t, [&](int i, cd d) { u[i] = rmad(u[i], hc6, d); });
+#pragma unroll
+  for (int i = 0; i < N; i++) y[i] = y[i] + u[i];
 }
 
 // One knot interval of a march: the interval's equal steps from ra to rb
@@ -289,11 +412,14 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, dou
 // 1945-1949), optional renormalisation (:1910-1919).  Every lane of the
 // block must call this with identical (i, ra, rb).  Returns the number of
 // steps taken.
+__device__ inline double stage_offset(int k) {
+  return k == 0 ? 0.0 : (k == 1 ? 0.2 : (k == 2 ? 0.3 : (k == 3 ? 0.6 : (k == 4 ? 1.0 : 0.875))));
+}
+
 template <class F>
 __device__ int march_interval(const Tab& T, const WS& w, const LS& L, int nknot, int i,
                               double ra, double rb, cd* y, bool scaled, int& scale,
-                              double* sbase, const F& f) {
-  constexpr double a[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
+                              double* sbase, const F& f, cd* slot = nullptr) {
   double drr0 = step_len_w(T, w.rew, i, nknot);
   int n = 2 + (int)floor(fabs(rb - ra) / drr0);
   double drr = (rb - ra) / (double)(n - 1);
@@ -303,11 +429,15 @@ __device__ int march_interval(const Tab& T, const WS& w, const LS& L, int nknot,
     if (q < 6 * nst) {
       int jj = j0 + q / 6, k = q % 6;
       double x = ra + (double)(jj - 1) * drr;
-      F::base(T, w, i, x + a[k] * drr, sbase + q * F::BS);
+      F::base(T, w, i, x + stage_offset(k) * drr, sbase + q * F::BS);
     }
     __syncthreads();
     for (int jj = 0; jj < nst; jj++) {
-      rk_step_b(sbase + jj * 6 * F::BS, L, w, drr, y, f);
+      if constexpr (F::N == 14) {
+        rk_step_lds(sbase + jj * 6 * F::BS, L, w, drr, y, f, slot);
+      } else {
+        rk_step_b(sbase + jj * 6 * F::BS, L, w, drr, y, f);
+      }
       if (scaled) renorm<F::N>(y, scale);
     }
     __syncthreads();
