@@ -623,7 +623,19 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, Run
 // ------------------------------------------------------------- k_assemble
 // output-type scaling and the uu/vv/ww coefficients, src/yspec.f90:280-288,
 // 309-316, 377-400 (zeta there is single precision, :255)
-__global__ void __launch_bounds__(256) k_assemble(RunK R, cd* __restrict__ rec) {
+struct GemmK {
+  int lpad, jpad, nrpad;
+  cd* cf;          // [15][lpad][jpad]: uu m=-2..2, vv m=-2..2, ww m=-2..2
+  const double* tb;  // [9][lpad][nrpad]: x, dx/dtheta, x/sin(theta) for |m| = 0,1,2
+};
+
+__device__ inline void put_coefs(const RunK& R, const GemmK& G, int l, int j, const cd* o) {
+  size_t plane = (size_t)G.lpad * G.jpad;
+  size_t at = (size_t)l * G.jpad + (j - 1);
+  for (int c = 0; c < 15; c++) G.cf[c * plane + at] = o[O_UU + c];
+}
+
+__global__ void __launch_bounds__(256) k_assemble(RunK R, cd* __restrict__ rec, GemmK G) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= R.npair) return;
   int l, j;
@@ -636,6 +648,7 @@ __global__ void __launch_bounds__(256) k_assemble(RunK R, cd* __restrict__ rec) 
   if (l == 0) {
     cd u3 = (o[O_SR0] * o[O_YRAD + 0] + o[O_SR0 + 1] * o[O_YRAD + 2]) / rr;
     o[O_UU + 2] = fo * u3;
+    put_coefs(R, G, l, j, o);
     return;
   }
   double zeta = (double)sqrtf((float)(l * (l + 1)));
@@ -663,6 +676,97 @@ __global__ void __launch_bounds__(256) k_assemble(RunK R, cd* __restrict__ rec) 
     cd sw = y2[0] * sv[m] + y2[1] * sv[5 + m] + y2[2] * sv[15 + m] + y2[3] * sv[20 + m];
     o[O_UU + m] = su / rr;
     o[O_VV + m] = sw / (zeta * rr);
+  }
+  put_coefs(R, G, l, j, o);
+}
+
+// ---------------------------------------------------------------- k_lsum
+// The receiver sum (src/yspec.f90:290-293, 318-335, 402-419) as FP64 MFMA
+// GEMMs: for each Legendre table X in {x, dx/dtheta, x/sin(theta)} and
+// |m| = a, C[j][k] = sum_l coef(l, w_j) X_a(l, theta_k) over the coefficient
+// sets that table multiplies; the epilogue applies (-1)^a for m < 0, the
+// factors i*m and exp(i m phi_k) and accumulates ur, ut, up.  One wave owns a
+// 16 (w) x 16 (receiver) tile; v_mfma_f64_16x16x4f64 lane layout: A[i][k] at
+// lane i + 16k, B[k][n] at lane n + 16k, D[i][n] at lane n + 16(i%4), reg i/4.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// one (table, |m|) group: GEMM over l, then the epilogue for this group
+template <int X, int A, int NC, int C0, int C1, int C2, int C3>
+__device__ __forceinline__ void lsum_group(const GemmK& G, int j0, int k0, int li, int lq,
+                                           cd ep, cd en, cd* ur, cd* ut, cd* up) {
+  constexpr int cs[4] = {C0, C1, C2, C3};
+  const size_t plane = (size_t)G.lpad * G.jpad;
+  d4 acc[NC][2];
+#pragma unroll
+  for (int q = 0; q < NC; q++) acc[q][0] = acc[q][1] = (d4){0.0, 0.0, 0.0, 0.0};
+  const double* tb = G.tb + (size_t)(X * 3 + A) * G.lpad * G.nrpad + k0 + li;
+  const cd* cf = G.cf + (size_t)lq * G.jpad + j0 + li;
+  for (int l0 = 0; l0 < G.lpad; l0 += 4) {
+    double b = tb[(size_t)(l0 + lq) * G.nrpad];
+    size_t at = (size_t)l0 * G.jpad;
+    cd av[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) av[q] = cf[cs[q] * plane + at];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      acc[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].x, b, acc[q][0], 0, 0, 0);
+      acc[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].y, b, acc[q][1], 0, 0, 0);
+    }
+  }
+  constexpr double sg = (A & 1) ? -1.0 : 1.0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    cd P = C(acc[0][0][v], acc[0][1][v]);
+    if constexpr (A == 0) {
+      if constexpr (X == 0) ur[v] = ur[v] + P;
+      else ut[v] = ut[v] + P;  // vv_0 dx/dtheta
+    } else {
+      cd Mn = C(acc[1][0][v], acc[1][1][v]);
+      cd pos = ep * P, neg = sg * (en * Mn);
+      if constexpr (X == 0) {
+        ur[v] = ur[v] + pos + neg;
+        cd Wp = C(acc[NC - 2][0][v], acc[NC - 2][1][v]), Wn = C(acc[NC - 1][0][v], acc[NC - 1][1][v]);
+        ut[v] = ut[v] + cmuli((double)A * (ep * Wp - sg * (en * Wn)));
+      } else if constexpr (X == 1) {
+        ut[v] = ut[v] + pos + neg;
+        cd Wp = C(acc[NC - 2][0][v], acc[NC - 2][1][v]), Wn = C(acc[NC - 1][0][v], acc[NC - 1][1][v]);
+        up[v] = up[v] - (ep * Wp + sg * (en * Wn));
+      } else {
+        up[v] = up[v] + cmuli((double)A * (pos - neg));
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_lsum(RunK R, GemmK G, const cd* __restrict__ eia,
+                                             cd* __restrict__ spec) {
+  int lane = threadIdx.x;
+  int njt = G.jpad / 16;
+  int j0 = (blockIdx.x % njt) * 16, k0 = (blockIdx.x / njt) * 16;
+  int li = lane & 15, lq = lane >> 4;
+  int k = k0 + li;  // receiver of this lane's output column
+  cd e1 = (k < R.nr) ? eia[k] : C(1.0), e2 = e1 * e1;
+  cd em1 = C(e1.x, -e1.y), em2 = C(e2.x, -e2.y);
+  cd ur[4], ut[4], up[4];
+  for (int v = 0; v < 4; v++) ur[v] = ut[v] = up[v] = C(0.0);
+  // coefficient index c: uu m -> m+2, vv m -> 7+m, ww m -> 12+m
+  lsum_group<0, 0, 1, 2, 0, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // uu0 x
+  lsum_group<0, 1, 4, 3, 1, 13, 11>(G, j0, k0, li, lq, e1, em1, ur, ut, up);   // uu+-1, ww+-1 x
+  lsum_group<0, 2, 4, 4, 0, 14, 10>(G, j0, k0, li, lq, e2, em2, ur, ut, up);   // uu+-2, ww+-2 x
+  lsum_group<1, 0, 1, 7, 0, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // vv0 dx
+  lsum_group<1, 1, 4, 8, 6, 13, 11>(G, j0, k0, li, lq, e1, em1, ur, ut, up);   // vv+-1, ww+-1 dx
+  lsum_group<1, 2, 4, 9, 5, 14, 10>(G, j0, k0, li, lq, e2, em2, ur, ut, up);   // vv+-2, ww+-2 dx
+  lsum_group<2, 1, 2, 8, 6, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // vv+-1 x/sin
+  lsum_group<2, 2, 2, 9, 5, 0, 0>(G, j0, k0, li, lq, e2, em2, ur, ut, up);     // vv+-2 x/sin
+  if (k >= R.nr) return;
+  size_t nwr = (size_t)R.nw * R.nr;
+  for (int v = 0; v < 4; v++) {
+    int j = j0 + lq + 4 * v;
+    if (j >= R.nw) continue;
+    size_t at = (size_t)j + (size_t)R.nw * k;
+    spec[at] = ur[v];
+    spec[nwr + at] = ut[v];
+    spec[2 * nwr + at] = up[v];
   }
 }
 
@@ -812,6 +916,9 @@ struct ysx_ctx {
   double *d_xa = nullptr, *d_xpa = nullptr, *d_xca = nullptr, *d_azep = nullptr;
   cd* d_eia = nullptr;
   cd* d_spec = nullptr;
+  cd* d_cf = nullptr;
+  double* d_tb = nullptr;
+  int lpad = 0, jpad = 0, nrpad = 0;
   cd* d_tw = nullptr;
   cd* d_work = nullptr;
   double* d_seis = nullptr;
@@ -844,7 +951,7 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_cnt); dfree(c->d_bst); dfree(c->d_mfin);
   dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0); dfree(c->d_sph_m0);
   dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_xa); dfree(c->d_xpa); dfree(c->d_xca);
-  dfree(c->d_azep); dfree(c->d_eia); dfree(c->d_spec); dfree(c->d_tw); dfree(c->d_work);
+  dfree(c->d_azep); dfree(c->d_eia); dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis);
   c->have_run = c->have_spec = false;
 }
@@ -943,6 +1050,27 @@ extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const 
   HIPCHK(hipMalloc(&ctx->d_azep, R.nr * sizeof(double)));
   HIPCHK(hipMalloc(&ctx->d_eia, R.nr * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_spec, 3 * (size_t)R.nw * R.nr * sizeof(cd)));
+  // GEMM operands: coefficients [15][lpad][jpad] (zero where no pair), and
+  // the Legendre tables transposed to [X*3 + |m|][lpad][nrpad]
+  ctx->lpad = (R.lmax + 1 + 3) / 4 * 4;
+  ctx->jpad = (R.nw + 15) / 16 * 16;
+  ctx->nrpad = (R.nr + 15) / 16 * 16;
+  {
+    size_t ncf = 15 * (size_t)ctx->lpad * ctx->jpad, ntb = 9 * (size_t)ctx->lpad * ctx->nrpad;
+    HIPCHK(hipMalloc(&ctx->d_cf, ncf * sizeof(cd)));
+    HIPCHK(hipMemset(ctx->d_cf, 0, ncf * sizeof(cd)));
+    std::vector<double> tb(ntb, 0.0);
+    const std::vector<double>* tabs[3] = {&R.xa, &R.xpa, &R.xca};
+    int L1 = R.lmax + 1;
+    for (int X = 0; X < 3; X++)
+      for (int a = 0; a < 3; a++)
+        for (int l = 0; l < L1; l++)
+          for (int kk = 0; kk < R.nr; kk++)
+            tb[((size_t)(X * 3 + a) * ctx->lpad + l) * ctx->nrpad + kk] =
+                (*tabs[X])[(size_t)l + (size_t)L1 * (a + 3 * (size_t)kk)];
+    HIPCHK(hipMalloc(&ctx->d_tb, ntb * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->d_tb, tb.data(), ntb * sizeof(double), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMalloc(&ctx->d_tw, (size_t)R.nt * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_work, 3 * (size_t)R.nt * R.nr * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_seis, 3 * (size_t)R.nt * R.nr * sizeof(double)));
@@ -1011,11 +1139,11 @@ int run_spectra(ysx_ctx* ctx) {
   HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
   HIPCHK(hipEventRecord(ev[6], st));
-  hipLaunchKernelGGL(k_assemble, dim3((P + 255) / 256), b256, 0, st, k, ctx->d_rec);
+  GemmK G{ctx->lpad, ctx->jpad, ctx->nrpad, ctx->d_cf, ctx->d_tb};
+  hipLaunchKernelGGL(k_assemble, dim3((P + 255) / 256), b256, 0, st, k, ctx->d_rec, G);
   HIPCHK(hipEventRecord(ev[7], st));
-  int nwr = k.nw * k.nr;
-  hipLaunchKernelGGL(k_sum, dim3((nwr + 255) / 256), b256, 0, st, k, ctx->d_rec, ctx->d_xa,
-                     ctx->d_xpa, ctx->d_xca, ctx->d_eia, ctx->d_spec);
+  hipLaunchKernelGGL(k_lsum, dim3((ctx->jpad / 16) * (ctx->nrpad / 16)), b64, 0, st, k, G,
+                     ctx->d_eia, ctx->d_spec);
   HIPCHK(hipEventRecord(ev[8], st));
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventSynchronize(ev[8]));
