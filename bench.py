@@ -76,7 +76,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--lmax", type=int, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 8}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the work would shard by frequency band; see DESIGN.md)", file=sys.stderr)
@@ -137,7 +148,7 @@ def main():
                              "78.6 TFLOP/s"},
     }
     if not args.no_cpu:
-        sample = list(range(0, p.lmax + 1, 75))
+        sample = list(range(0, p.lmax + 1, 30))
         cb = cpu_baseline(deck, p, sample)
         if cb:
             rate, pairs, secs, wall = cb

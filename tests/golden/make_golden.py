@@ -6,7 +6,9 @@ FP64 results as compressed .npz files:
 
     case_<name>.npz : deck columns + parameter block (the inputs), model
                       arrays, geometry tables, a spread subset of the per-pair
-                      records, the full spectra and seismograms.
+                      records, the full spectra and seismograms, and the text
+                      of the reference program's per-receiver output files
+                      (oracle/_ref/yspec on the same parameter file).
 
 Cases cover: attenuation off/on, shallow and deep sources, a receiver below
 the source, an ocean layer, a transversely isotropic (ians=1) deck, and the
@@ -27,11 +29,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, HERE]
 
-from yspec_amd.inputs import Deck  # noqa: E402
+from yspec_amd.inputs import Deck, read_deck, read_params  # noqa: E402
 from yspec_amd.synth import synth_deck, workload  # noqa: E402
 from refdump import read_dump, MODEL_NAMES  # noqa: E402
 
 REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+REF_YSPEC = os.path.join(ROOT, "oracle", "_ref", "yspec")
+EXAMPLES = os.path.join(HERE, "ref_examples")
 
 
 def ocean_deck(seed=0):
@@ -60,7 +64,22 @@ def aniso_deck(seed=0):
     return Deck("synth.aniso", 1, 1.0, d.nknot, d.nic, d.noc, cols)
 
 
+def prem_example(deck_file, nr=4, **over):
+    """the reference's example parameter file (ref_examples/input.txt, PREM,
+    29.5 km source, receivers at 3 km depth) cut down to a few seconds"""
+    p = read_params(os.path.join(EXAMPLES, "input.txt"))
+    p.lat, p.lon = p.lat[:nr].copy(), p.lon[:nr].copy()
+    small = dict(lmax=30, f2=5.0, tout=40.0, dt=20.0, f11=1.0, f12=2.0, f21=3.5, f22=4.5)
+    small.update(over)
+    for k, v in small.items():
+        setattr(p, k, v)
+    return read_deck(os.path.join(EXAMPLES, deck_file)), p
+
+
 def cases():
+    yield ("prem_example", *prem_example("prem.200"))
+    yield ("prem_noocean_disp", *prem_example("prem.200.no", ats=0, out=0, cor=0,
+                                              rec_depth=0.0))
     d0 = synth_deck(0)
     yield "cfg1_small", d0, workload("cfg1", lmax=40, nr=3)
     yield "att_deep", d0, workload("cfg1", lmax=30, nr=3, ats=1, src_depth=600.0, f2=6.0,
@@ -81,6 +100,7 @@ def run_case(name, deck, p, outdir, keep_pairs=240):
         with open(os.path.join(td, "model.deck"), "w") as f:
             f.write(deck.to_text())
         p.model_file = "model.deck"
+        p.prefix = "out"
         with open(os.path.join(td, "in.txt"), "w") as f:
             f.write(p.to_text())
         dd = os.path.join(td, "dump")
@@ -88,6 +108,12 @@ def run_case(name, deck, p, outdir, keep_pairs=240):
         subprocess.run([REF_DUMP, "in.txt", dd], cwd=td, check=True,
                        stdout=subprocess.DEVNULL)
         D = read_dump(dd)
+        # the reference program's own per-receiver text output, as written
+        subprocess.run([REF_YSPEC, "in.txt"], cwd=td, check=True, stdout=subprocess.DEVNULL)
+        ascii_out = []
+        for k in range(1, p.nr + 1):
+            with open(os.path.join(td, f"out.{k}")) as f:
+                ascii_out.append(f.read())
     P = D["pairs"]
     n = len(P["l"])
     sel = np.unique(np.linspace(0, n - 1, min(keep_pairs, n)).astype(int))
@@ -103,7 +129,8 @@ def run_case(name, deck, p, outdir, keep_pairs=240):
         pair_l=P["l"][sel], pair_j=P["j"][sel], tor_start=P["tor_start"][sel],
         sph_start=P["sph_start"][sel],
         ur=D["spec"][0], ut=D["spec"][1], up=D["spec"][2],
-        seis=np.stack([D["seis"][c].real for c in range(3)], axis=1), nout=np.array(D["nout"]))
+        seis=np.stack([D["seis"][c].real for c in range(3)], axis=1), nout=np.array(D["nout"]),
+        ascii=np.array(ascii_out))
     for key in ("svt", "ytor", "svs", "ysph", "sr0", "yrad", "uu", "vv", "ww"):
         out["pair_" + key] = P[key][sel]
     path = os.path.join(outdir, f"case_{name}.npz")

@@ -1,0 +1,89 @@
+"""GPU path against the oracle beyond the golden sizes, and size-independent
+properties at the full lmax = 600, fmax = 40 mHz workload.
+
+* the whole lmax = 150, fmax = 10 mHz workload (7,852 pairs, 10 receivers):
+  spectra <= 1e-9, seismograms <= 1e-8 relative L2 against the oracle;
+* the full 787,310-pair workload: per-pair solver outputs of a spread
+  sample of (l, w), highest l and w included, against the oracle's;
+  linearity of the spectra in the moment tensor; run-to-run identity."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from yspec_amd.synth import synth_deck, workload
+
+pytestmark = pytest.mark.gpu
+KEYS = ("svt", "ytor", "svs", "ysph", "sr0", "yrad", "uu", "vv", "ww")
+
+
+@pytest.fixture(scope="module")
+def api():
+    from yspec_amd import api as a
+    a.lib()
+    return a
+
+
+def test_cfg1_full_vs_oracle(api, oracle_mod):
+    deck, p = synth_deck(0), workload("cfg1")
+    g = api.YSpec(deck, p)
+    o = oracle_mod.Oracle(deck, p)
+    assert (g.nw, g.nt) == (o.nw, o.nt)
+    gs = g.spectra()
+    os_ = o.spectra(nthreads=16)
+    for a, b, key in zip(gs, os_, ("ur", "ut", "up")):
+        e = rel_l2(a, b)
+        print("cfg1", key, f"{e:.2e}", flush=True)
+        assert e <= 1e-9
+    e = rel_l2(g.seismograms(), o.seismograms(*os_))
+    print("cfg1 seis", f"{e:.2e}", flush=True)
+    assert e <= 1e-8
+
+
+@pytest.fixture(scope="module")
+def cfg2(api):
+    deck, p = synth_deck(0), workload("cfg2")
+    g = api.YSpec(deck, p)
+    return deck, p, g
+
+
+def test_cfg2_pairs_vs_oracle(cfg2, oracle_mod):
+    deck, p, g = cfg2
+    g.spectra(copy=False)
+    rng = np.random.default_rng(11)
+    ls = np.concatenate([[0, 1, 2, p.lmax, p.lmax, 300], rng.integers(0, p.lmax + 1, 60)])
+    js = np.concatenate([[0, g.nw - 1, g.nw // 2, 0, g.nw - 1, g.nw - 1],
+                         rng.integers(0, g.nw, 60)])
+    rec = g.pair_records(ls, js)
+    o = oracle_mod.Oracle(deck, p)
+    want = {k: [] for k in KEYS}
+    for l, j in zip(ls, js):
+        r = o.pair(int(l), int(j))
+        for k in KEYS:
+            want[k].append(r[k])
+    for k in KEYS:
+        e = rel_l2(rec[k], np.array(want[k]))
+        print("cfg2 pairs", k, f"{e:.2e}", flush=True)
+        assert e <= 1e-9, k
+
+
+def test_cfg2_linear_in_moment_tensor(cfg2, api):
+    deck, p, g = cfg2
+    rng = np.random.default_rng(3)
+    m1, m2 = rng.uniform(-1e20, 1e20, 6), rng.uniform(-1e20, 1e20, 6)
+    out = []
+    for mm in (m1, m2, 2.0 * m1 - 3.0 * m2):
+        p.mm = mm
+        g.set_run(p)
+        out.append(g.spectra())
+    for c in range(3):
+        e = rel_l2(out[2][c], 2.0 * out[0][c] - 3.0 * out[1][c])
+        print("cfg2 linearity", c, f"{e:.2e}", flush=True)
+        assert e < 1e-12
+
+
+def test_cfg2_repeatable(cfg2):
+    deck, p, g = cfg2
+    a = g.forward()
+    b = g.forward()
+    assert np.array_equal(a, b)
+    assert np.all(np.isfinite(a))

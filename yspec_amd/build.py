@@ -60,7 +60,7 @@ def build_fortran():
     exe = os.path.join(FDIR, "yspec")
     srcs = [os.path.join(FDIR, "ysx_bind.f90"), os.path.join(FDIR, "yspec_host.f90")]
     _run([FC, "-O2", "-J", bdir, *srcs, "-o", exe, "-L", HERE, "-lysx",
-          f"-Wl,-rpath,{HERE}"])
+          "-Wl,-rpath,$ORIGIN/.."])
     return exe
 
 

@@ -2,15 +2,18 @@
 // libysx.so (include/ysx.h).
 //
 // Per run the GPU work is:
-//   k_source     source vectors of every (l, w) pair
-//   k_tor        toroidal two-sided shooting, one lane per pair
-//   k_sph_minor  spheroidal minor vectors from the start level to the
-//                source and receiver, secular determinant, b-vector starts
-//   k_bvec       b-vector from source to receiver, one lane per (pair, column)
-//   k_rad        radial (l = 0) solves on the reference's grid
-//   k_assemble   output-type scaling and the uu/vv/ww coefficients
-//   k_sum        sum over l and m against the Legendre tables
-//   k_seis       filtered inverse FFT, exp(eps t), units, rotation
+//   k_source      source vectors of every (l, w) pair
+//   k_start       start levels and starting solutions
+//   k_tor_march   toroidal two-sided shooting, lockstep waves of 64 l at one w
+//   k_minor_top   spheroidal minors from the receiver side down to the source
+//   k_minor_march spheroidal minor vectors from the start level up to the
+//                 source, secular determinant, b-vector starts
+//   k_bvec_march  b-vector from the source to the receiver, 4 columns
+//   k_rad         radial (l = 0) solves on the reference's grid (2nd stream)
+//   k_assemble    output-type scaling and the uu/vv/ww coefficients
+//   k_lsum        sum over l and m against the Legendre tables: FP64 MFMA
+//                 GEMMs with the azimuthal phases fused into the epilogue
+//   k_seis        filtered inverse FFT, exp(eps t), units, rotation
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -768,45 +771,6 @@ __global__ void __launch_bounds__(64) k_lsum(RunK R, GemmK G, const cd* __restri
     spec[nwr + at] = ut[v];
     spec[2 * nwr + at] = up[v];
   }
-}
-
-// ------------------------------------------------------------------ k_sum
-// sum over l and m against the Legendre tables, src/yspec.f90:290-293,
-// 318-335, 402-419.  One lane per (w_j, receiver k).
-__global__ void __launch_bounds__(256) k_sum(RunK R, const cd* __restrict__ rec,
-                                             const double* __restrict__ xa,
-                                             const double* __restrict__ xpa,
-                                             const double* __restrict__ xca,
-                                             const cd* __restrict__ eia, cd* __restrict__ spec) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= R.nw * R.nr) return;
-  int jj = t % R.nw, k = t / R.nw;
-  int L1 = R.lmax + 1;
-  cd e1 = eia[k], e2 = e1 * e1;
-  cd ie1 = cinv(e1), ie2 = cinv(e2);
-  cd ep[5] = {ie2, ie1, C(1.0), e1, e2};
-  cd ur = C(0.0), ut = C(0.0), up = C(0.0);
-  for (int l = R.lmin; l <= R.lmax; l++) {
-    const cd* o = rec + ((size_t)(l - R.lmin) * R.nw + jj) * REC;
-    if (l == 0) {
-      ur = ur + o[O_UU + 2] * xa[(size_t)L1 * 3 * k];
-      continue;
-    }
-    for (int m = -2; m <= 2; m++) {
-      int am = m < 0 ? -m : m;
-      double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
-      size_t idx = (size_t)l + (size_t)L1 * (am + 3 * (size_t)k);
-      double x = sg * xa[idx], xp = sg * xpa[idx], xc = sg * xca[idx];
-      cd e = ep[m + 2];
-      cd uu = o[O_UU + m + 2], vv = o[O_VV + m + 2], ww = o[O_WW + m + 2];
-      ur = ur + (uu * x) * e;
-      ut = ut + (vv * xp + cmuli(ww) * (m * x)) * e;
-      up = up + (cmuli(vv) * (m * xc) - ww * xp) * e;
-    }
-  }
-  spec[t] = ur;
-  spec[(size_t)R.nw * R.nr + t] = ut;
-  spec[2 * (size_t)R.nw * R.nr + t] = up;
 }
 
 // ------------------------------------------------------------------ k_seis
