@@ -44,6 +44,10 @@ from yspec_amd.synth import synth_deck, workload  # noqa: E402
 METRIC = "(l,ω) solves/s + seismograms/s, batched vs plain GPU; gradient/forward time"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the ODE kernels at this workload: rocprofv3
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB device copy),
+# profiles/forward_cfg2_hbm_traffic.md
+TRAFFIC_BYTES = {"bvec": 1362.0e6, "minors": 1539.3e6, "tor": 202.4e6, "lsum": 1333.7e6}
 PHASES = ("source", "tor", "minors", "bvec", "rad", "assemble", "lsum", "ifft")
 
 
@@ -109,6 +113,11 @@ def main():
         print(f"# step {k + 1}/{args.steps} done", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     sec = (t1 - t0) / args.steps
+    # the same forward handing the (nt, 3, nr) seismograms back to host
+    # memory, as the Fortran host receives them (outside the timed steps)
+    t2 = time.perf_counter()
+    g.forward(copy=True)
+    pcie_ms = (time.perf_counter() - t2) * 1e3
     phase /= args.steps
     counts = g.step_counts()
     fl = ode_flops(counts)
@@ -139,13 +148,16 @@ def main():
         "ode_solves_per_s": P / (ode_ms * 1e-3),
         "phase_ms": {k: round(float(v), 4) for k, v in ms.items()},
         "ode_tflops": ode_flop / (ode_ms * 1e-3) / 1e12,
-        "step_counts": {k: int(v) for k, v in zip(("tor", "solid", "fluid", "bvec_col", "rad"),
+        "step_counts": {k: int(v) for k, v in zip(("tor", "solid", "fluid", "bvec", "rad"),
                                                   counts)},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": TRAFFIC_BYTES.get(dom) if args.lmax is None else None,
                      "note": "FP64 vector work; MI355X FP64 vector and matrix peaks are both "
-                             "78.6 TFLOP/s"},
+                             "78.6 TFLOP/s; traffic = HBM bytes per launch from rocprofv3 "
+                             "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy"},
+        "host_copy_ms_per_forward": pcie_ms,
     }
     if not args.no_cpu:
         sample = list(range(0, p.lmax + 1, 30))

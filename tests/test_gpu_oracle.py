@@ -51,8 +51,9 @@ def test_cfg2_pairs_vs_oracle(cfg2, oracle_mod):
     g.spectra(copy=False)
     rng = np.random.default_rng(11)
     ls = np.concatenate([[0, 1, 2, p.lmax, p.lmax, 300], rng.integers(0, p.lmax + 1, 60)])
-    js = np.concatenate([[0, g.nw - 1, g.nw // 2, 0, g.nw - 1, g.nw - 1],
-                         rng.integers(0, g.nw, 60)])
+    # j counts frequencies from 1, as the reference's loop does
+    js = np.concatenate([[1, g.nw, g.nw // 2, 1, g.nw, g.nw],
+                         rng.integers(1, g.nw + 1, 60)])
     rec = g.pair_records(ls, js)
     o = oracle_mod.Oracle(deck, p)
     want = {k: [] for k in KEYS}

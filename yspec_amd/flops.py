@@ -20,7 +20,8 @@ FLOP_PER_STEP = {
 
 
 def ode_flops(counts):
-    """counts = ysx_step_counts: tor, solid, fluid, bvec (per column), rad"""
+    """counts = ysx_step_counts: tor, solid, fluid, bvec, rad steps summed
+    over pairs (a b-vector step advances all four source columns)"""
     tor, solid, fluid, bvec, rad = (int(c) for c in counts)
     return {
         "tor": FLOP_PER_STEP["tor"] * tor,
