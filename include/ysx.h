@@ -80,6 +80,38 @@ int ysx_seismograms(ysx_ctx *ctx, double *seis);
  * spectra. */
 int ysx_forward(ysx_ctx *ctx, double *seis);
 
+/* ---- several events on one model ------------------------------------
+ * The reference computes one event per run of its program
+ * (src/yspec.f90:1-510 for one parameter file); these entry points take
+ * nev events at once: event e has the run parameters ip[7e..7e+6] and
+ * fp[18e..18e+17] (as in ysx_set_run) and the receivers lat/lon[r_e ..
+ * r_e + nr_e - 1], r_e = sum of nr over the events before e.  Each event
+ * is checked as ysx_set_run checks one; an error names the event.
+ * ysx_set_run(ctx, ...) is ysx_set_events(ctx, 1, ...). */
+int ysx_set_events(ysx_ctx *ctx, int nev, const int *ip, const double *fp,
+                   const double *lat, const double *lon);
+
+/* ysx_run_info of event e */
+int ysx_event_info(const ysx_ctx *ctx, int e, int *iout8, double *fout8);
+
+/* The full forward (spectra and seismograms) of every event.  mode 0:
+ * events one after another, each with its own kernel launches (the plain
+ * path).  mode 1: all events in one launch per kernel, their (w, l-block)
+ * work items sorted by estimated cost, replayed from a captured HIP graph
+ * (the batched path).  seis: the events' (nt_e, 3, nr_e) seismograms one
+ * after another, or NULL.  ysx_timings()[7] holds the elapsed ms. */
+int ysx_forward_events(ysx_ctx *ctx, int mode, double *seis);
+
+/* Batched-path tuning: persist = 1 runs persistent blocks
+ * (blocks_per_cu per compute unit) that take `chunk` items per atomic
+ * queue access; persist = 0 launches one block per item. */
+int ysx_set_batch_tuning(ysx_ctx *ctx, int persist, int chunk, int blocks_per_cu);
+
+/* Spectra ((nw_e, nr_e) complex each, or NULL) and seismograms
+ * ((nt_e, 3, nr_e)) of event e from the last forward. */
+int ysx_event_spectra(ysx_ctx *ctx, int e, double *ur, double *ut, double *up);
+int ysx_event_seismograms(ysx_ctx *ctx, int e, double *seis);
+
 /* Per-pair records for n pairs (l[i], j[i]) (j 1-based), 89 complex each:
  * svt(5,2) ytor(2,2) svs(5,6) ysph(6,4) sr0(2) yrad(2,2) uu(5) vv(5) ww(5),
  * the values the reference's loop body holds after its source-vector and

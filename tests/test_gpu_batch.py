@@ -1,0 +1,84 @@
+"""Several events on one model: the plain path (one event after another)
+and the batched path (all events per launch from cost-sorted work lists,
+replayed from a HIP graph; one block per item or persistent blocks on
+atomic queues) give each event exactly the single-event result, which the
+oracle checks at 1e-9 (spectra) / 1e-8 (seismograms)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from yspec_amd.synth import batch_events, synth_deck
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    from yspec_amd import api as a
+    a.lib()
+    return a
+
+
+def small_events():
+    ev = batch_events(nev=5, seed=3, nr=3, fmax_range=(3.0, 6.0), tout=40.0, dt=20.0,
+                      lmax_per_mhz=6.0)
+    ev[1].src_depth = 650.0   # deep: minors start in the core at low l
+    ev[2].src_depth = 1.5     # shallow crustal source
+    ev[3].lmin = 2            # no radial pairs in this event
+    for p in ev:
+        p.f21, p.f22 = 0.6 * p.f2, 0.8 * p.f2
+    return ev
+
+
+@pytest.fixture(scope="module")
+def single(api):
+    deck = synth_deck(0)
+    out = []
+    for p in small_events():
+        g = api.YSpec(deck, p)
+        spec = g.spectra()
+        out.append((spec, g.seismograms()))
+        g.close()
+    return out
+
+
+def _check_events(g, single):
+    for e, (spec, seis) in enumerate(single):
+        got = g.event_spectra(e)
+        for a, b in zip(got, spec):
+            assert np.array_equal(a, b), e
+        assert np.array_equal(g.event_seismograms(e), seis), e
+
+
+def test_plain_and_batched_equal_single(api, single):
+    g = api.YSpec(synth_deck(0))
+    g.set_events(small_events())
+    g.forward_events(0)
+    _check_events(g, single)
+    for rep in range(3):  # graph replays, with eager work in between
+        g.forward_events(1)
+        _check_events(g, single)
+        g.forward_events(0)
+    g.set_batch_tuning(persist=1, chunk=2, blocks_per_cu=2)
+    for rep in range(2):
+        g.forward_events(1)
+        _check_events(g, single)
+
+
+def test_events_vs_oracle(api, oracle_mod, single):
+    deck = synth_deck(0)
+    for e, p in enumerate(small_events()):
+        o = oracle_mod.Oracle(deck, p)
+        ospec = o.spectra(nthreads=16)
+        spec, seis = single[e]
+        for a, b in zip(spec, ospec):
+            assert rel_l2(a, b) <= 1e-9, e
+        assert rel_l2(seis, o.seismograms(*ospec)) <= 1e-8, e
+
+
+def test_bad_event_is_named(api):
+    ev = small_events()
+    ev[2].f2 = 30.0  # above the 25 mHz Nyquist frequency of dt = 20 s
+    g = api.YSpec(synth_deck(0))
+    with pytest.raises(api.YSpecError, match="event 2"):
+        g.set_events(ev)

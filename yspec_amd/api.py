@@ -22,6 +22,8 @@ I = C.POINTER(C.c_int)
 
 EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_set_run",
            "ysx_run_info", "ysx_spectra", "ysx_seismograms", "ysx_forward",
+           "ysx_set_events", "ysx_event_info", "ysx_forward_events", "ysx_set_batch_tuning",
+           "ysx_event_spectra", "ysx_event_seismograms",
            "ysx_pair_records", "ysx_timings", "ysx_step_counts")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
@@ -57,6 +59,12 @@ def lib():
         L.ysx_spectra.argtypes = [vp, D, D, D]
         L.ysx_seismograms.argtypes = [vp, D]
         L.ysx_forward.argtypes = [vp, D]
+        L.ysx_set_events.argtypes = [vp, C.c_int, I, D, D, D]
+        L.ysx_event_info.argtypes = [vp, C.c_int, I, D]
+        L.ysx_forward_events.argtypes = [vp, C.c_int, D]
+        L.ysx_set_batch_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.ysx_event_spectra.argtypes = [vp, C.c_int, D, D, D]
+        L.ysx_event_seismograms.argtypes = [vp, C.c_int, D]
         L.ysx_pair_records.argtypes = [vp, C.c_int, I, I, D]
         L.ysx_timings.argtypes = [vp, D]
         L.ysx_step_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
@@ -165,8 +173,50 @@ class YSpec:
         return t
 
     def step_counts(self):
-        """exact Cash-Karp step totals of the last spectra: toroidal, solid
-        minors, fluid minors, b-vector (per column), radial"""
+        """exact Cash-Karp step totals of the last spectra (all events):
+        toroidal, solid minors, fluid minors, b-vector (steps of the four
+        columns together), radial"""
         out = np.zeros(5, np.int64)
         self._check(self.L.ysx_step_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
         return out
+
+    # ---- several events on one model
+    def set_events(self, events):
+        """events: a list of Params (each with its own receivers)."""
+        ip = np.concatenate([p.int_block() for p in events]).astype(np.int32)
+        fp = np.concatenate([p.float_block() for p in events])
+        lat = np.ascontiguousarray(np.concatenate([p.lat for p in events]), dtype=np.float64)
+        lon = np.ascontiguousarray(np.concatenate([p.lon for p in events]), dtype=np.float64)
+        self._check(self.L.ysx_set_events(self.h, len(events), _ip(ip), _dp(fp), _dp(lat),
+                                          _dp(lon)))
+        self.events = list(events)
+        self.p = events[0]
+        self.info = []
+        for e in range(len(events)):
+            io = np.zeros(8, np.int32)
+            fo = np.zeros(8)
+            self._check(self.L.ysx_event_info(self.h, e, _ip(io), _dp(fo)))
+            self.info.append(dict(zip(("nw", "nt", "i1", "i2", "is", "ir", "nout", "npair"),
+                                      map(int, io))))
+        self.npair = sum(d["npair"] for d in self.info)
+        self.nw, self.nt = self.info[0]["nw"], self.info[0]["nt"]
+
+    def set_batch_tuning(self, persist=0, chunk=1, blocks_per_cu=4):
+        self._check(self.L.ysx_set_batch_tuning(self.h, persist, chunk, blocks_per_cu))
+
+    def forward_events(self, mode=1):
+        """mode 0: one event after another (plain); 1: batched.  Returns ms."""
+        self._check(self.L.ysx_forward_events(self.h, mode, None))
+        return float(self.timings()[7])
+
+    def event_spectra(self, e):
+        d = self.info[e]
+        shp = (d["nw"], self.events[e].nr)
+        out = [np.zeros(shp, np.complex128, order="F") for _ in range(3)]
+        self._check(self.L.ysx_event_spectra(self.h, e, *[a.ctypes.data_as(D) for a in out]))
+        return tuple(out)
+
+    def event_seismograms(self, e):
+        seis = np.zeros((self.info[e]["nt"], 3, self.events[e].nr), order="F")
+        self._check(self.L.ysx_event_seismograms(self.h, e, _dp(seis)))
+        return seis

@@ -1,7 +1,7 @@
 // ysx_gpu.hip -- gfx950 kernels of the spectral path and the C-ABI of
 // libysx.so (include/ysx.h).
 //
-// Per run the GPU work is:
+// Per forward the GPU work is:
 //   k_source      source vectors of every (l, w) pair
 //   k_start       start levels and starting solutions
 //   k_tor_march   toroidal two-sided shooting, lockstep waves of 64 l at one w
@@ -14,12 +14,20 @@
 //   k_lsum        sum over l and m against the Legendre tables: FP64 MFMA
 //                 GEMMs with the azimuthal phases fused into the epilogue
 //   k_seis        filtered inverse FFT, exp(eps t), units, rotation
+//
+// A context holds one or more events (source + receivers + frequency grid)
+// on one model.  Every kernel reads its event's parameters from a device
+// array of RunK and takes its work from a list of items (event, w, l-block):
+// one event at a time with one block per item ("plain"), or all events at
+// once from cost-sorted lists drained by persistent blocks through an atomic
+// queue, the whole forward replayed from a captured HIP graph ("batched").
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -32,13 +40,80 @@ using namespace ysx;
 
 namespace {
 
-// ----------------------------------------------------------------- run data
+// ------------------------------------------------------- per-event run data
 struct RunK {
   int lmin, lmax, nw, nr, is, ir, out, cor, npair;
   int nknot, nic, noc, nsl, fic, ocean;
+  int lpad, jpad, nrpad;        // GEMM padding of this event
+  int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
+  long long pbase;              // first pair of this event in the per-pair arrays
+  long long cf_off, tb_off, spec_off, seis_off;
   double rs, rr, wmin, dw, ep, tref, ats, pibigg, fre_norm, alpha_min, grav_ir;
+  double df, dt, f11, f12, f21, f22, norm;
   double mm[6];
 };
+
+// A work list: items[0..n) of (event, a, b, 0).  counter == nullptr: block
+// b takes items b, b + gridDim.x, ...; otherwise persistent blocks take
+// `chunk` consecutive items per atomicAdd on *counter (zeroed before the
+// launch by k_reset).  March blocks are one wave: lane 0 takes the items
+// and broadcasts them.
+struct Queue {
+  const int4* items;
+  int n;
+  int* counter;
+  int chunk;
+};
+
+__device__ __forceinline__ int4 uniform_item(const int4* items, int it) {
+  int4 v = items[it];
+  return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                   __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
+
+__device__ __forceinline__ int next_base(const Queue& Q) {
+  int base = 0;
+  if (threadIdx.x == 0) base = atomicAdd(Q.counter, Q.chunk);
+  return __builtin_amdgcn_readfirstlane(base);  // lane 0's value, wave-uniform
+}
+
+// Run ITEM_CALL (a statement using `it`, the current int4 item) over the
+// queue: PERSIST = false: block b runs item b (the hardware dispatches
+// blocks in index order, so a cost-sorted list is drained largest first);
+// PERSIST = true: blocks loop over the queue.  Written as a macro, not a
+// lambda: captured references keep the body's state out of registers.
+#define RUN_ITEMS(PERSIST, Q, ITEM_CALL)                                     \
+  do {                                                                       \
+    if constexpr (!(PERSIST)) {                                              \
+      if ((int)blockIdx.x < (Q).n) {                                         \
+        int4 it = uniform_item((Q).items, blockIdx.x);                       \
+        ITEM_CALL;                                                           \
+      }                                                                      \
+    } else {                                                                 \
+      for (int base_ = next_base(Q); base_ < (Q).n; base_ = next_base(Q)) {  \
+        int end_ = min(base_ + (Q).chunk, (Q).n);                            \
+        for (int k_ = base_; k_ < end_; k_++) {                              \
+          int4 it = uniform_item((Q).items, k_);                             \
+          ITEM_CALL;                                                         \
+        }                                                                    \
+      }                                                                      \
+    }                                                                        \
+  } while (0)
+
+__global__ void k_reset(int* counters, int n) {
+  if ((int)threadIdx.x < n) counters[threadIdx.x] = 0;
+}
+
+// event of global pair g among events [e0, e1) (pbase ascending)
+__device__ inline int find_event(const RunK* runs, int e0, int e1, long long g) {
+  int lo = e0, hi = e1 - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (runs[mid].pbase <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 __device__ inline Layers layers(const RunK& R) {
   return Layers{R.nknot, R.nic, R.noc, R.nsl, R.fic, R.ocean};
@@ -65,6 +140,12 @@ __device__ inline Tab load_tab(const double* __restrict__ g, int nknot) {
   return Tab{smem, nknot};
 }
 
+// the pairs of events [e0, e1): global pair index of thread t
+struct PairRange {
+  int e0, e1;
+  long long g0, n;
+};
+
 // per-pair output records (AoS, one record of 89 complex numbers per pair)
 constexpr int REC = 89;
 constexpr int O_SVT = 0, O_YTOR = 10, O_SVS = 14, O_YSPH = 44, O_SR0 = 68, O_YRAD = 70,
@@ -74,18 +155,22 @@ constexpr int O_SVT = 0, O_YTOR = 10, O_SVS = 14, O_YSPH = 44, O_SR0 = 68, O_YRA
 // source_vector_tor / _sph / _rad, src/module_solver.f90:145-215,
 // 566-627, 1747-1847.  source_vector_sph keeps only the real part of its
 // dispersion factors (they are declared real(dp) there, :1771-1772).
-__global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab, RunK R,
+__global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
+                                                const RunK* __restrict__ runs, PairRange PR,
                                                 cd* __restrict__ rec, int* __restrict__ cnt) {
-  Tab T = load_tab(gtab, R.nknot);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= R.npair) return;
+  Tab T = load_tab(gtab, runs[PR.e0].nknot);
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= PR.n) return;
+  long long g = PR.g0 + t;
+  const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
+  int p = (int)(g - R.pbase);
   int l, j;
   pair_lj(R, p, l, j);
   PS s;
   pair_state(R, l, j, s);
-  cd* o = rec + (size_t)p * REC;
+  cd* o = rec + (size_t)g * REC;
   for (int i = 0; i < REC; i++) o[i] = C(0.0);
-  for (int i = 0; i < 5; i++) cnt[5 * p + i] = 0;
+  for (int i = 0; i < 5; i++) cnt[5 * g + i] = 0;
   SplW w = spl_w(T, R.is, R.rs);
   double rlcon = spl(T, w, F_LCON), rccon = spl(T, w, F_CCON), rfcon = spl(T, w, F_FCON),
          rqmu = spl(T, w, F_QMU), rxlam = spl(T, w, F_XLAM), rxa2 = spl(T, w, F_XA2);
@@ -136,13 +221,17 @@ struct StartData {
   cd* sph_m0;    // [npair][14] solid minors, or 5 fluid minors + zeros
 };
 
-__global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab, RunK R,
+__global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
+                                               const RunK* __restrict__ runs, PairRange PR,
                                                StartData SD, int* __restrict__ starts) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= PR.n) return;
+  long long g = PR.g0 + t;
+  const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
   Tab T{gtab, R.nknot};
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= R.npair) return;
   int l, j;
-  pair_lj(R, p, l, j);
+  pair_lj(R, (int)(g - R.pbase), l, j);
+  size_t p = (size_t)g;
   if (l < 1) return;
   PS s;
   pair_state(R, l, j, s);
@@ -220,17 +309,12 @@ constexpr int NOLANE = 1 << 30;
 // ------------------------------------------------------------- k_tor_march
 // tor_solver with ss = identity(2), src/module_solver.f90:25-142; a wave
 // holds 64 consecutive l at one w, largest w first.
-__global__ void __launch_bounds__(64) k_tor_march(const double* __restrict__ gtab, RunK R,
-                                                  StartData SD, cd* __restrict__ rec,
-                                                  int* __restrict__ cnt) {
-  __shared__ double sb[6 * CH * BASE_MAX];
-  Tab T{gtab, R.nknot};
-  int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64;
-  int wv = blockIdx.x;
-  int j = R.nw - wv / nlb;
-  int l = R.lmin + (wv % nlb) * 64 + (int)threadIdx.x;
+__device__ __forceinline__ void tor_item(const Tab& T, const RunK& R, int j, int lb,
+                                         const StartData& SD, cd* __restrict__ rec,
+                                         int* __restrict__ cnt, double* sb) {
+  int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 2 && l <= R.lmax);
-  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
   int i1 = valid ? SD.tor_i1[p] : NOLANE;
   int ib = wave_min(i1);
   if (ib == NOLANE) return;
@@ -286,6 +370,17 @@ __global__ void __launch_bounds__(64) k_tor_march(const double* __restrict__ gta
   for (int i = 0; i < 4; i++) o[i] = yo[i] * sc;
 }
 
+// items (event, j, l-block of 64)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_tor_march(const double* __restrict__ gtab,
+                                                  const RunK* __restrict__ runs, Queue Q,
+                                                  StartData SD, cd* __restrict__ rec,
+                                                  int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  RUN_ITEMS(PERSIST, Q,
+            tor_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, rec, cnt, sb));
+}
+
 // ----------------------------------------------------------- k_minor_march
 // sph_solver (grav = 2, solid inner core) up to the b-vector integration,
 // src/module_solver.f90:640-743, 756-806.  Per pair it leaves the four
@@ -319,19 +414,12 @@ __device__ __forceinline__ int minors_from_top(const Tab& T, const WS& w, const 
   return sc2;
 }
 
-__global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ gtab, RunK R,
-                                                    StartData SD, SphState S,
-                                                    int* __restrict__ cnt) {
-  __shared__ double sb[6 * CH * BASE_MAX];
-  __shared__ cd slots[2 * 14 * 64];
-  cd* slot = slots + threadIdx.x;
-  Tab T{gtab, R.nknot};
-  int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64;
-  int wv = blockIdx.x;
-  int j = R.nw - wv / nlb;
-  int l = R.lmin + (wv % nlb) * 64 + (int)threadIdx.x;
+__device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, int lb,
+                                           const StartData& SD, const SphState& S,
+                                           int* __restrict__ cnt, double* sb, cd* slot) {
+  int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 1 && l <= R.lmax);
-  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
   int i1 = valid ? SD.sph_i1[p] : NOLANE;
   int ib = wave_min(i1);
   if (ib == NOLANE) return;
@@ -408,22 +496,30 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
   cnt[5 * p + 2] += nflu;
 }
 
-// The surface minor vector ms2 of every pair (src/module_solver.f90:730-741,
-// 793-804): from the surface down to the receiver (receiver above the
-// source) or to the source.  Zero steps for a receiver at the surface.
-__global__ void __launch_bounds__(64) k_minor_top(const double* __restrict__ gtab, RunK R,
-                                                  SphState S, int* __restrict__ cnt) {
+// items (event, j, l-block of 64)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ gtab,
+                                                    const RunK* __restrict__ runs, Queue Q,
+                                                    StartData SD, SphState S,
+                                                    int* __restrict__ cnt) {
   __shared__ double sb[6 * CH * BASE_MAX];
   __shared__ cd slots[2 * 14 * 64];
   cd* slot = slots + threadIdx.x;
-  Tab T{gtab, R.nknot};
-  int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64;
-  int wv = blockIdx.x;
-  int j = R.nw - wv / nlb;
-  int l = R.lmin + (wv % nlb) * 64 + (int)threadIdx.x;
+  RUN_ITEMS(PERSIST, Q,
+            minor_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, S, cnt, sb,
+                       slot));
+}
+
+// The surface minor vector ms2 of every pair (src/module_solver.f90:730-741,
+// 793-804): from the surface down to the receiver (receiver above the
+// source) or to the source.  Zero steps for a receiver at the surface.
+__device__ __forceinline__ void minor_top_item(const Tab& T, const RunK& R, int j, int lb,
+                                               const SphState& S, int* __restrict__ cnt,
+                                               double* sb, cd* slot) {
+  int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 1 && l <= R.lmax);
   if (wave_min(valid ? 0 : 1)) return;
-  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
   WS w = wave_omega(R, j);
   LS L = l_state(valid ? l : 1);
   int nsol = 0, nflu = 0;
@@ -438,26 +534,32 @@ __global__ void __launch_bounds__(64) k_minor_top(const double* __restrict__ gta
   cnt[5 * p + 2] = nflu;
 }
 
+// items (event, j, l-block of 64)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_minor_top(const double* __restrict__ gtab,
+                                                  const RunK* __restrict__ runs, Queue Q,
+                                                  SphState S, int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  RUN_ITEMS(PERSIST, Q,
+            minor_top_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, S, cnt, sb,
+                           slot));
+}
+
 // ------------------------------------------------------------ k_bvec_march
 // b-vector from source to receiver and y = sph_ysol(m, b) for the four
 // source columns, src/module_solver.f90:745-755, 809-819.  A wave holds 16
 // consecutive l x 4 columns at one w.
-__global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gtab, RunK R,
-                                                   SphState S, cd* __restrict__ rec,
-                                                   int* __restrict__ cnt) {
-  __shared__ double sb[6 * CH * BASE_MAX];
-  __shared__ cd slots[2 * 14 * 64];
-  cd* slot = slots + threadIdx.x;
-  Tab T{gtab, R.nknot};
-  int nl = R.lmax - R.lmin + 1, nlb = (nl + 15) / 16;
-  int wv = blockIdx.x;
-  int j = R.nw - wv / nlb;
+__device__ __forceinline__ void bvec_item(const Tab& T, const RunK& R, int j, int lb,
+                                          const SphState& S, cd* __restrict__ rec,
+                                          int* __restrict__ cnt, double* sb, cd* slot) {
   int c = threadIdx.x & 3;
-  int l = R.lmin + (wv % nlb) * 16 + (int)(threadIdx.x >> 2);
+  int l = R.lmin + lb * 16 + (int)(threadIdx.x >> 2);
   bool valid = (l >= 1 && l <= R.lmax);
   int any = wave_min(valid ? 0 : 1);
   if (any) return;  // no valid lane in this wave
-  size_t p = (size_t)(l - R.lmin) * R.nw + (j - 1);
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
   WS w = wave_omega(R, j);
   LS L = l_state(valid ? l : 1);
   cd b[14];
@@ -473,6 +575,20 @@ __global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gt
   cd* o = rec + p * REC + O_YSPH + 6 * c;
   for (int q = 0; q < 6; q++) o[q] = y[q] * fac;
   if (c == 0) cnt[5 * p + 3] = n;
+}
+
+// items (event, j, l-block of 16); lanes 4 l + c integrate column c of l
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gtab,
+                                                   const RunK* __restrict__ runs, Queue Q,
+                                                   SphState S, cd* __restrict__ rec,
+                                                   int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  RUN_ITEMS(PERSIST, Q,
+            bvec_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, S, rec, cnt, sb,
+                      slot));
 }
 
 // ------------------------------------------------------------------- k_rad
@@ -523,12 +639,11 @@ __device__ inline void grid_runs(const Tab& T, int i, double drr, double rs, dou
   }
 }
 
-__global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, RunK R,
-                                            cd* __restrict__ rec, int* __restrict__ cnt) {
-  Tab T = load_tab(gtab, R.nknot);
-  int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+__device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
+                                         cd* __restrict__ rec, int* __restrict__ cnt) {
+  int j = jb * 64 + threadIdx.x + 1;
   if (j > R.nw || R.lmin != 0) return;
-  int p = j - 1;  // l = 0 pairs come first
+  size_t p = (size_t)R.pbase + (j - 1);  // l = 0 pairs come first
   PS s;
   pair_state(R, 0, j, s);
   RadRHS f;
@@ -618,32 +733,53 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab, Run
     yo[2] = -delta * y2r[0] * b2;
     yo[3] = -delta * y2r[1] * b2;
   }
-  cd* o = rec + (size_t)p * REC + O_YRAD;
+  cd* o = rec + p * REC + O_YRAD;
   for (int i = 0; i < 4; i++) o[i] = yo[i];
   cnt[5 * p + 4] = nsr;
+}
+
+// items (event, block of 64 frequencies); one lane per l = 0 pair, no
+// lockstep (the grid of make_grid depends on w)
+__global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab,
+                                            const RunK* __restrict__ runs, Queue Q,
+                                            cd* __restrict__ rec, int* __restrict__ cnt) {
+  Tab T = load_tab(gtab, runs[0].nknot);
+  RUN_ITEMS(false, Q, rad_item(T, runs[it.x], it.y, rec, cnt));
 }
 
 // ------------------------------------------------------------- k_assemble
 // output-type scaling and the uu/vv/ww coefficients, src/yspec.f90:280-288,
 // 309-316, 377-400 (zeta there is single precision, :255)
+// GEMM operands of event e, at offsets cf_off / tb_off of RunK:
+//   cf [15][lpad][jpad] complex: uu m=-2..2, vv m=-2..2, ww m=-2..2
+//   tb [9][lpad][nrpad] real: x, dx/dtheta, x/sin(theta) for |m| = 0,1,2
 struct GemmK {
   int lpad, jpad, nrpad;
-  cd* cf;          // [15][lpad][jpad]: uu m=-2..2, vv m=-2..2, ww m=-2..2
-  const double* tb;  // [9][lpad][nrpad]: x, dx/dtheta, x/sin(theta) for |m| = 0,1,2
+  cd* cf;
+  const double* tb;
 };
 
-__device__ inline void put_coefs(const RunK& R, const GemmK& G, int l, int j, const cd* o) {
+__device__ inline GemmK gemm_of(const RunK& R, cd* cf, const double* tb) {
+  return GemmK{R.lpad, R.jpad, R.nrpad, cf + R.cf_off, tb + R.tb_off};
+}
+
+__device__ inline void put_coefs(const GemmK& G, int l, int j, const cd* o) {
   size_t plane = (size_t)G.lpad * G.jpad;
   size_t at = (size_t)l * G.jpad + (j - 1);
   for (int c = 0; c < 15; c++) G.cf[c * plane + at] = o[O_UU + c];
 }
 
-__global__ void __launch_bounds__(256) k_assemble(RunK R, cd* __restrict__ rec, GemmK G) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= R.npair) return;
+__global__ void __launch_bounds__(256) k_assemble(const RunK* __restrict__ runs, PairRange PR,
+                                                  cd* __restrict__ rec, cd* __restrict__ cf,
+                                                  const double* __restrict__ tb) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= PR.n) return;
+  long long g = PR.g0 + t;
+  const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
+  GemmK G = gemm_of(R, cf, tb);
   int l, j;
-  pair_lj(R, p, l, j);
-  cd* o = rec + (size_t)p * REC;
+  pair_lj(R, (int)(g - R.pbase), l, j);
+  cd* o = rec + (size_t)g * REC;
   cd w = C(R.wmin + (double)(j - 1) * R.dw, -R.ep);
   cd fo = (R.out == 1) ? cmuli(w) : (R.out == 2 ? -(w * w) : C(1.0));
   double rr = R.rr;
@@ -651,7 +787,7 @@ __global__ void __launch_bounds__(256) k_assemble(RunK R, cd* __restrict__ rec, 
   if (l == 0) {
     cd u3 = (o[O_SR0] * o[O_YRAD + 0] + o[O_SR0 + 1] * o[O_YRAD + 2]) / rr;
     o[O_UU + 2] = fo * u3;
-    put_coefs(R, G, l, j, o);
+    put_coefs(G, l, j, o);
     return;
   }
   double zeta = (double)sqrtf((float)(l * (l + 1)));
@@ -680,7 +816,7 @@ __global__ void __launch_bounds__(256) k_assemble(RunK R, cd* __restrict__ rec, 
     o[O_UU + m] = su / rr;
     o[O_VV + m] = sw / (zeta * rr);
   }
-  put_coefs(R, G, l, j, o);
+  put_coefs(G, l, j, o);
 }
 
 // ---------------------------------------------------------------- k_lsum
@@ -741,11 +877,11 @@ __device__ __forceinline__ void lsum_group(const GemmK& G, int j0, int k0, int l
   }
 }
 
-__global__ void __launch_bounds__(64) k_lsum(RunK R, GemmK G, const cd* __restrict__ eia,
-                                             cd* __restrict__ spec) {
+__device__ __forceinline__ void lsum_item(const RunK& R, const GemmK& G, int tile,
+                                          const cd* __restrict__ eia, cd* __restrict__ spec) {
   int lane = threadIdx.x;
   int njt = G.jpad / 16;
-  int j0 = (blockIdx.x % njt) * 16, k0 = (blockIdx.x / njt) * 16;
+  int j0 = (tile % njt) * 16, k0 = (tile / njt) * 16;
   int li = lane & 15, lq = lane >> 4;
   int k = k0 + li;  // receiver of this lane's output column
   cd e1 = (k < R.nr) ? eia[k] : C(1.0), e2 = e1 * e1;
@@ -773,17 +909,22 @@ __global__ void __launch_bounds__(64) k_lsum(RunK R, GemmK G, const cd* __restri
   }
 }
 
+// items (event, 16 w x 16 receiver tile); spec of event e at spec_off,
+// [3][nr][nw]
+__global__ void __launch_bounds__(64) k_lsum(const RunK* __restrict__ runs, Queue Q,
+                                             cd* __restrict__ cf, const double* __restrict__ tb,
+                                             const cd* __restrict__ eia, cd* __restrict__ spec) {
+  RUN_ITEMS(false, Q,
+            lsum_item(runs[it.x], gemm_of(runs[it.x], cf, tb), it.y, eia + runs[it.x].roff,
+                      spec + runs[it.x].spec_off));
+}
+
 // ------------------------------------------------------------------ k_seis
 // ifft (src/module_fourier.f90:55-145) and src/yspec.f90:439-473 for one
 // receiver per block: pack bins i1..i2, cosine filters, Hermitian fill,
 // radix-2 DIT inverse FFT with the reference's twiddle angles
 // pi_d*m/mmax (single-rounded pi), /(dt*nt), exp(eps t), units, rotation.
-struct SeisK {
-  int nt, lognt, nw, nr, i1, i2, out;
-  double df, dt, ep, f11, f12, f21, f22, norm;
-};
-
-__device__ inline cd packed_bin(const SeisK& K, const cd* spec, int k, int i) {
+__device__ inline cd packed_bin(const RunK& K, const cd* spec, int k, int i) {
   // 0-based FFT bin i of the packed, filtered, Hermitian-filled row
   bool neg = i > K.nt / 2;
   int b = neg ? K.nt - i : i;
@@ -805,42 +946,55 @@ __device__ inline cd packed_bin(const SeisK& K, const cd* spec, int k, int i) {
 
 __device__ inline unsigned brev(unsigned x, int bits) { return __brev(x) >> (32 - bits); }
 
-__global__ void __launch_bounds__(1024) k_seis(SeisK K, const cd* __restrict__ spec,
-                                               const cd* __restrict__ tw, cd* __restrict__ work,
-                                               const double* __restrict__ azep,
-                                               double* __restrict__ seis) {
-  int k = blockIdx.x;
-  int nt = K.nt;
-  for (int c = 0; c < 3; c++) {
-    const cd* sp = spec + (size_t)c * K.nw * K.nr;
-    cd* row = work + ((size_t)k * 3 + c) * nt;
-    for (int q = threadIdx.x; q < nt; q += blockDim.x)
-      row[q] = packed_bin(K, sp, k, (int)brev((unsigned)q, K.lognt));
-    __syncthreads();
-    for (int mmax = 1; mmax < nt; mmax <<= 1) {
-      for (int b = threadIdx.x; b < nt / 2; b += blockDim.x) {
-        int m = b & (mmax - 1);
-        int i = ((b - m) << 1) + m;
-        int jx = i + mmax;
-        cd temp = tw[mmax + m] * row[jx];
-        cd a = row[i];
-        row[jx] = a - temp;
-        row[i] = a + temp;
-      }
+// items (event, receiver k of the event), static distribution.  Event e's
+// FFT rows are work[seis_off + (3k + c) nt ...] and its seismograms
+// seis[seis_off + i + nt (c + 3k)], (nt, 3, nr) column-major.
+__global__ void __launch_bounds__(1024) k_seis(const RunK* __restrict__ runs, Queue Q,
+                                               const cd* __restrict__ spec_all,
+                                               const cd* __restrict__ tw, cd* __restrict__ work_all,
+                                               const double* __restrict__ azep_all,
+                                               double* __restrict__ seis_all) {
+  for (int it = blockIdx.x; it < Q.n; it += gridDim.x) {
+    int4 item = Q.items[it];
+    const RunK& K = runs[item.x];
+    int k = item.y;
+    int nt = K.nt;
+    const cd* spec = spec_all + K.spec_off;
+    cd* work = work_all + K.seis_off;
+    double* seis = seis_all + K.seis_off;
+    for (int c = 0; c < 3; c++) {
+      const cd* sp = spec + (size_t)c * K.nw * K.nr;
+      cd* row = work + ((size_t)k * 3 + c) * nt;
+      for (int q = threadIdx.x; q < nt; q += blockDim.x)
+        row[q] = packed_bin(K, sp, k, (int)brev((unsigned)q, K.lognt));
       __syncthreads();
+      for (int mmax = 1; mmax < nt; mmax <<= 1) {
+        for (int b = threadIdx.x; b < nt / 2; b += blockDim.x) {
+          int m = b & (mmax - 1);
+          int i = ((b - m) << 1) + m;
+          int jx = i + mmax;
+          cd temp = tw[mmax + m] * row[jx];
+          cd a = row[i];
+          row[jx] = a - temp;
+          row[i] = a + temp;
+        }
+        __syncthreads();
+      }
     }
-  }
-  double ca = cos(azep[k]), sa = sin(azep[k]);
-  const cd* r0 = work + ((size_t)k * 3 + 0) * nt;
-  const cd* r1 = work + ((size_t)k * 3 + 1) * nt;
-  const cd* r2 = work + ((size_t)k * 3 + 2) * nt;
-  double sc = K.norm / (K.dt * nt);
-  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-    double g = exp(K.ep * (i * K.dt)) * sc;
-    double z = r0[i].x * g, ut = r1[i].x * g, up = r2[i].x * g;
-    seis[(size_t)i + (size_t)nt * (0 + 3 * (size_t)k)] = z;
-    seis[(size_t)i + (size_t)nt * (1 + 3 * (size_t)k)] = -ut * ca + up * sa;
-    seis[(size_t)i + (size_t)nt * (2 + 3 * (size_t)k)] = ut * sa + up * ca;
+    double az = azep_all[K.roff + k];
+    double ca = cos(az), sa = sin(az);
+    const cd* r0 = work + ((size_t)k * 3 + 0) * nt;
+    const cd* r1 = work + ((size_t)k * 3 + 1) * nt;
+    const cd* r2 = work + ((size_t)k * 3 + 2) * nt;
+    double sc = K.norm / (K.dt * nt);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+      double g = exp(K.ep * (i * K.dt)) * sc;
+      double z = r0[i].x * g, ut = r1[i].x * g, up = r2[i].x * g;
+      seis[(size_t)i + (size_t)nt * (0 + 3 * (size_t)k)] = z;
+      seis[(size_t)i + (size_t)nt * (1 + 3 * (size_t)k)] = -ut * ca + up * sa;
+      seis[(size_t)i + (size_t)nt * (2 + 3 * (size_t)k)] = ut * sa + up * ca;
+    }
+    __syncthreads();
   }
 }
 
@@ -859,17 +1013,36 @@ __global__ void k_twiddles(int nt, cd* tw) {
 }  // namespace
 
 // ====================================================================== API
+namespace {
+
+enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_N };
+// queue counters of the batched launches (three marches share the W_TOR list)
+enum QueueSlot { Q_TOR = 0, Q_TOP, Q_MINOR, Q_BVEC, Q_RAD, Q_N };
+
+struct WorkList {
+  std::vector<int4> plain;    // event-major, each event in its plain order
+  std::vector<int> start;     // [nev + 1] offsets of each event in plain
+  std::vector<int4> batched;  // all events, cost-sorted
+  int4* d_plain = nullptr;
+  int4* d_batched = nullptr;
+};
+
+}  // namespace
+
 struct ysx_ctx {
-  int device = 0;
+  int device = 0, ncu = 256;
   hipStream_t stream = nullptr, stream2 = nullptr;
   std::string err;
   Norms N;
   HostModel M;
-  HostRun R;
-  bool have_model = false, have_run = false, have_spec = false;
+  std::vector<HostRun> R;  // one per event
+  std::vector<RunK> rk;
+  bool have_model = false, have_run = false, have_spec = false, have_seis = false;
   size_t lds_bytes = 0, lds_max = 0;
+  long long npair = 0;  // all events
   // device buffers
   double* d_tab = nullptr;
+  RunK* d_runs = nullptr;
   cd* d_rec = nullptr;
   int* d_starts = nullptr;
   int* d_cnt = nullptr;
@@ -877,18 +1050,23 @@ struct ysx_ctx {
   int *d_tor_i1 = nullptr, *d_sph_i1 = nullptr;
   cd *d_tor_y0 = nullptr, *d_sph_m0 = nullptr;
   int* d_sc3 = nullptr;
-  double *d_xa = nullptr, *d_xpa = nullptr, *d_xca = nullptr, *d_azep = nullptr;
+  double* d_azep = nullptr;
   cd* d_eia = nullptr;
   cd* d_spec = nullptr;
   cd* d_cf = nullptr;
   double* d_tb = nullptr;
-  int lpad = 0, jpad = 0, nrpad = 0;
   cd* d_tw = nullptr;
   cd* d_work = nullptr;
   double* d_seis = nullptr;
+  int* d_queue = nullptr;  // Q_N counters of the batched queues
+  WorkList W[W_N];
+  // batched mode: persistent blocks or one block per item, items taken per
+  // queue access, persistent blocks per CU, and the captured forward
+  int persist = 0, chunk = 1, blocks_per_cu = 4;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   hipEvent_t ev[16] = {};
   double ms[8] = {0};
-  RunK rk;
 };
 
 namespace {
@@ -911,13 +1089,82 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+void drop_graph(ysx_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
 void free_run(ysx_ctx* c) {
-  dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_cnt); dfree(c->d_bst); dfree(c->d_mfin);
-  dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0); dfree(c->d_sph_m0);
-  dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_xa); dfree(c->d_xpa); dfree(c->d_xca);
-  dfree(c->d_azep); dfree(c->d_eia); dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
-  dfree(c->d_seis);
-  c->have_run = c->have_spec = false;
+  drop_graph(c);
+  dfree(c->d_runs); dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_cnt); dfree(c->d_bst);
+  dfree(c->d_mfin); dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0);
+  dfree(c->d_sph_m0); dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_azep); dfree(c->d_eia);
+  dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
+  dfree(c->d_seis); dfree(c->d_queue);
+  for (auto& w : c->W) {
+    dfree(w.d_plain);
+    dfree(w.d_batched);
+    w = WorkList();
+  }
+  c->R.clear();
+  c->rk.clear();
+  c->npair = 0;
+  c->have_run = c->have_spec = c->have_seis = false;
+}
+
+// Work lists of every kernel kind.  Plain order per event is the order of
+// one launch per event (largest w first for the marches).  The batched
+// lists hold every event's items, sorted by an estimate of their cost:
+// steps grow with Re w; a wave of degrees starting at l0 integrates from
+// about the turning radius (l0 + 1/2) v / w (v ~ 1 in normalised units) up
+// to the top point; the b-vector runs between source and receiver.
+void build_work(ysx_ctx* c) {
+  int nev = (int)c->rk.size();
+  std::vector<std::vector<double>> cost(W_N);
+  for (int k = 0; k < W_N; k++) c->W[k].start.assign(1, 0);
+  for (int e = 0; e < nev; e++) {
+    const RunK& R = c->rk[e];
+    int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64, nlb16 = (nl + 15) / 16;
+    auto rew = [&](int j) { return R.wmin + (double)(j - 1) * R.dw; };
+    double rtop = std::max(R.rs, R.rr);
+    for (int wv = 0; wv < R.nw * nlb; wv++) {
+      int j = R.nw - wv / nlb, lb = wv % nlb;
+      c->W[W_TOR].plain.push_back(make_int4(e, j, lb, 0));
+      double w = rew(j), rt = std::min(1.0, (R.lmin + 64.0 * lb + 0.5) / std::max(w, 1e-30));
+      cost[W_TOR].push_back(w * (rtop - std::min(rt, rtop) + 0.05));
+    }
+    for (int wv = 0; wv < R.nw * nlb16; wv++) {
+      int j = R.nw - wv / nlb16, lb = wv % nlb16;
+      c->W[W_BVEC].plain.push_back(make_int4(e, j, lb, 0));
+      cost[W_BVEC].push_back(rew(j) * (std::fabs(R.rr - R.rs) + 1e-3));
+    }
+    if (R.lmin == 0)
+      for (int jb = 0; jb < (R.nw + 63) / 64; jb++) {
+        c->W[W_RAD].plain.push_back(make_int4(e, jb, 0, 0));
+        cost[W_RAD].push_back(rew(std::min(R.nw, 64 * jb + 64)));
+      }
+    int ntile = (R.jpad / 16) * (R.nrpad / 16);
+    for (int t = 0; t < ntile; t++) {
+      c->W[W_LSUM].plain.push_back(make_int4(e, t, 0, 0));
+      cost[W_LSUM].push_back((double)R.lpad);
+    }
+    for (int k = 0; k < R.nr; k++) {
+      c->W[W_SEIS].plain.push_back(make_int4(e, k, 0, 0));
+      cost[W_SEIS].push_back((double)R.nt);
+    }
+    for (int k = 0; k < W_N; k++) c->W[k].start.push_back((int)c->W[k].plain.size());
+  }
+  for (int k = 0; k < W_N; k++) {
+    WorkList& w = c->W[k];
+    std::vector<int> ord(w.plain.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int b) { return cost[k][a] > cost[k][b]; });
+    w.batched.resize(ord.size());
+    for (size_t i = 0; i < ord.size(); i++) w.batched[i] = w.plain[ord[i]];
+  }
 }
 
 }  // namespace
@@ -935,6 +1182,7 @@ extern "C" int ysx_create(ysx_ctx** out, int device) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
   ctx->lds_max = prop.sharedMemPerBlock;
+  ctx->ncu = prop.multiProcessorCount;
   return YSX_OK;
 }
 
@@ -972,31 +1220,64 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
   return YSX_OK;
 }
 
-extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const double* lat,
-                           const double* lon) {
-  if (!ctx || !ip || !fp || !lat || !lon) return YSX_ERR_ARG;
+extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double* fp,
+                              const double* lat, const double* lon) {
+  if (!ctx || nev < 1 || !ip || !fp || !lat || !lon) return YSX_ERR_ARG;
   if (!ctx->have_model) return set_err(ctx, YSX_ERR_ARG, "no model");
   HIPCHK(hipSetDevice(ctx->device));
   free_run(ctx);
-  std::string e = build_run(ctx->N, ctx->M, ip, fp, lat, lon, ctx->R);
-  if (!e.empty()) return set_err(ctx, YSX_ERR_INPUT, e);
-  const HostRun& R = ctx->R;
   const HostModel& M = ctx->M;
-  RunK& k = ctx->rk;
-  k.lmin = R.lmin; k.lmax = R.lmax; k.nw = R.nw; k.nr = R.nr; k.is = R.is; k.ir = R.ir;
-  k.out = R.out; k.cor = R.cor;
-  k.npair = (R.lmax - R.lmin + 1) * R.nw;
-  k.nknot = M.nknot; k.nic = M.nic; k.noc = M.noc; k.nsl = M.nsl; k.fic = M.fic;
-  k.ocean = M.ocean;
-  k.rs = R.rs; k.rr = R.rr; k.wmin = R.wmin; k.dw = R.dw; k.ep = R.ep; k.tref = M.tref;
-  k.ats = (double)R.ats; k.pibigg = ctx->N.pibigg; k.fre_norm = ctx->N.fre_norm;
   double amin = M.at(F_ALPHA, 1);
   for (int i = 2; i <= M.nknot; i++) amin = std::min(amin, M.at(F_ALPHA, i));
-  k.alpha_min = amin;
-  k.grav_ir = M.at(F_GRAV, R.ir);
-  for (int i = 0; i < 6; i++) k.mm[i] = R.mm[i];
-  size_t P = (size_t)k.npair, nt3 = R.xa.size();
-  HIPCHK(hipMalloc(&ctx->d_rec, P * 89 * sizeof(cd)));
+  ctx->R.resize(nev);
+  ctx->rk.resize(nev);
+  long long pb = 0, cfo = 0, tbo = 0, spo = 0, sio = 0;
+  int ro = 0, ntmax = 0;
+  for (int e = 0; e < nev; e++) {
+    int nr = ip[7 * e + 6];
+    if (nr < 1) return set_err(ctx, YSX_ERR_INPUT, "bad lmin/lmax/receiver count");
+    std::string err = build_run(ctx->N, M, ip + 7 * e, fp + 18 * e, lat + ro, lon + ro, ctx->R[e]);
+    if (!err.empty()) {
+      ctx->R.clear();
+      ctx->rk.clear();
+      return set_err(ctx, YSX_ERR_INPUT, nev > 1 ? "event " + std::to_string(e) + ": " + err : err);
+    }
+    const HostRun& R = ctx->R[e];
+    RunK& k = ctx->rk[e];
+    k = RunK{};
+    k.lmin = R.lmin; k.lmax = R.lmax; k.nw = R.nw; k.nr = R.nr; k.is = R.is; k.ir = R.ir;
+    k.out = R.out; k.cor = R.cor;
+    k.npair = (R.lmax - R.lmin + 1) * R.nw;
+    k.nknot = M.nknot; k.nic = M.nic; k.noc = M.noc; k.nsl = M.nsl; k.fic = M.fic;
+    k.ocean = M.ocean;
+    k.lpad = (R.lmax + 1 + 3) / 4 * 4;
+    k.jpad = (R.nw + 15) / 16 * 16;
+    k.nrpad = (R.nr + 15) / 16 * 16;
+    k.nt = R.nt;
+    k.lognt = 0;
+    while ((1 << k.lognt) < R.nt) k.lognt++;
+    k.i1 = R.i1; k.i2 = R.i2; k.roff = ro;
+    k.pbase = pb; k.cf_off = cfo; k.tb_off = tbo; k.spec_off = spo; k.seis_off = sio;
+    k.rs = R.rs; k.rr = R.rr; k.wmin = R.wmin; k.dw = R.dw; k.ep = R.ep; k.tref = M.tref;
+    k.ats = (double)R.ats; k.pibigg = ctx->N.pibigg; k.fre_norm = ctx->N.fre_norm;
+    k.alpha_min = amin;
+    k.grav_ir = M.at(F_GRAV, R.ir);
+    k.df = R.df; k.dt = R.dt; k.f11 = R.f11; k.f12 = R.f12; k.f21 = R.f21; k.f22 = R.f22;
+    k.norm = R.out == 0 ? ctx->N.r_norm : (R.out == 1 ? ctx->N.vel_norm : ctx->N.acl_norm);
+    for (int i = 0; i < 6; i++) k.mm[i] = R.mm[i];
+    pb += k.npair;
+    cfo += 15LL * k.lpad * k.jpad;
+    tbo += 9LL * k.lpad * k.nrpad;
+    spo += 3LL * R.nw * R.nr;
+    sio += 3LL * R.nt * R.nr;
+    ro += nr;
+    ntmax = std::max(ntmax, R.nt);
+  }
+  ctx->npair = pb;
+  size_t P = (size_t)pb;
+  HIPCHK(hipMalloc(&ctx->d_runs, nev * sizeof(RunK)));
+  HIPCHK(hipMemcpy(ctx->d_runs, ctx->rk.data(), nev * sizeof(RunK), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&ctx->d_rec, P * REC * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_starts, P * 2 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_cnt, P * 5 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_bst, P * 56 * sizeof(cd)));
@@ -1008,42 +1289,54 @@ extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const 
   HIPCHK(hipMalloc(&ctx->d_sph_m0, P * 14 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_fac, P * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_sc3, P * sizeof(int)));
-  HIPCHK(hipMalloc(&ctx->d_xa, nt3 * sizeof(double)));
-  HIPCHK(hipMalloc(&ctx->d_xpa, nt3 * sizeof(double)));
-  HIPCHK(hipMalloc(&ctx->d_xca, nt3 * sizeof(double)));
-  HIPCHK(hipMalloc(&ctx->d_azep, R.nr * sizeof(double)));
-  HIPCHK(hipMalloc(&ctx->d_eia, R.nr * sizeof(cd)));
-  HIPCHK(hipMalloc(&ctx->d_spec, 3 * (size_t)R.nw * R.nr * sizeof(cd)));
-  // GEMM operands: coefficients [15][lpad][jpad] (zero where no pair), and
-  // the Legendre tables transposed to [X*3 + |m|][lpad][nrpad]
-  ctx->lpad = (R.lmax + 1 + 3) / 4 * 4;
-  ctx->jpad = (R.nw + 15) / 16 * 16;
-  ctx->nrpad = (R.nr + 15) / 16 * 16;
+  HIPCHK(hipMalloc(&ctx->d_azep, ro * sizeof(double)));
+  HIPCHK(hipMalloc(&ctx->d_eia, ro * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_spec, spo * sizeof(cd)));
+  // GEMM operands: coefficients (zero where no pair) and the Legendre tables
+  // transposed to [X*3 + |m|][lpad][nrpad]
+  HIPCHK(hipMalloc(&ctx->d_cf, cfo * sizeof(cd)));
+  HIPCHK(hipMemset(ctx->d_cf, 0, cfo * sizeof(cd)));
   {
-    size_t ncf = 15 * (size_t)ctx->lpad * ctx->jpad, ntb = 9 * (size_t)ctx->lpad * ctx->nrpad;
-    HIPCHK(hipMalloc(&ctx->d_cf, ncf * sizeof(cd)));
-    HIPCHK(hipMemset(ctx->d_cf, 0, ncf * sizeof(cd)));
-    std::vector<double> tb(ntb, 0.0);
-    const std::vector<double>* tabs[3] = {&R.xa, &R.xpa, &R.xca};
-    int L1 = R.lmax + 1;
-    for (int X = 0; X < 3; X++)
-      for (int a = 0; a < 3; a++)
-        for (int l = 0; l < L1; l++)
-          for (int kk = 0; kk < R.nr; kk++)
-            tb[((size_t)(X * 3 + a) * ctx->lpad + l) * ctx->nrpad + kk] =
-                (*tabs[X])[(size_t)l + (size_t)L1 * (a + 3 * (size_t)kk)];
-    HIPCHK(hipMalloc(&ctx->d_tb, ntb * sizeof(double)));
-    HIPCHK(hipMemcpy(ctx->d_tb, tb.data(), ntb * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> tb(tbo, 0.0), az(ro);
+    std::vector<std::complex<double>> eia(ro);
+    for (int e = 0; e < nev; e++) {
+      const HostRun& R = ctx->R[e];
+      const RunK& k = ctx->rk[e];
+      const std::vector<double>* tabs[3] = {&R.xa, &R.xpa, &R.xca};
+      int L1 = R.lmax + 1;
+      for (int X = 0; X < 3; X++)
+        for (int a = 0; a < 3; a++)
+          for (int l = 0; l < L1; l++)
+            for (int kk = 0; kk < R.nr; kk++)
+              tb[k.tb_off + ((size_t)(X * 3 + a) * k.lpad + l) * k.nrpad + kk] =
+                  (*tabs[X])[(size_t)l + (size_t)L1 * (a + 3 * (size_t)kk)];
+      for (int kk = 0; kk < R.nr; kk++) {
+        az[k.roff + kk] = R.azep[kk];
+        eia[k.roff + kk] = R.eia[kk];
+      }
+    }
+    HIPCHK(hipMalloc(&ctx->d_tb, tbo * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->d_tb, tb.data(), tbo * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_azep, az.data(), ro * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_eia, eia.data(), ro * sizeof(cd), hipMemcpyHostToDevice));
   }
-  HIPCHK(hipMalloc(&ctx->d_tw, (size_t)R.nt * sizeof(cd)));
-  HIPCHK(hipMalloc(&ctx->d_work, 3 * (size_t)R.nt * R.nr * sizeof(cd)));
-  HIPCHK(hipMalloc(&ctx->d_seis, 3 * (size_t)R.nt * R.nr * sizeof(double)));
-  HIPCHK(hipMemcpy(ctx->d_xa, R.xa.data(), nt3 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_xpa, R.xpa.data(), nt3 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_xca, R.xca.data(), nt3 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_azep, R.azep.data(), R.nr * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_eia, R.eia.data(), R.nr * sizeof(cd), hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(k_twiddles, dim3((R.nt + 255) / 256), dim3(256), 0, ctx->stream, R.nt,
+  HIPCHK(hipMalloc(&ctx->d_tw, (size_t)ntmax * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_work, sio * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_seis, sio * sizeof(double)));
+  HIPCHK(hipMalloc(&ctx->d_queue, Q_N * sizeof(int)));
+  build_work(ctx);
+  for (auto& w : ctx->W) {
+    size_t n = std::max<size_t>(1, w.plain.size());
+    HIPCHK(hipMalloc(&w.d_plain, n * sizeof(int4)));
+    HIPCHK(hipMalloc(&w.d_batched, n * sizeof(int4)));
+    if (!w.plain.empty()) {
+      HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+    }
+  }
+  hipLaunchKernelGGL(k_twiddles, dim3((ntmax + 255) / 256), dim3(256), 0, ctx->stream, ntmax,
                      ctx->d_tw);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1051,11 +1344,16 @@ extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const 
   return YSX_OK;
 }
 
-extern "C" int ysx_run_info(const ysx_ctx* ctx, int* iout, double* fout) {
-  if (!ctx || !ctx->have_run) return YSX_ERR_ARG;
-  const HostRun& R = ctx->R;
+extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const double* lat,
+                           const double* lon) {
+  return ysx_set_events(ctx, 1, ip, fp, lat, lon);
+}
+
+extern "C" int ysx_event_info(const ysx_ctx* ctx, int e, int* iout, double* fout) {
+  if (!ctx || !ctx->have_run || e < 0 || e >= (int)ctx->R.size()) return YSX_ERR_ARG;
+  const HostRun& R = ctx->R[e];
   if (iout) {
-    int v[8] = {R.nw, R.nt, R.i1, R.i2, R.is, R.ir, R.nout, ctx->rk.npair};
+    int v[8] = {R.nw, R.nt, R.i1, R.i2, R.is, R.ir, R.nout, ctx->rk[e].npair};
     std::memcpy(iout, v, sizeof v);
   }
   if (fout) {
@@ -1065,92 +1363,189 @@ extern "C" int ysx_run_info(const ysx_ctx* ctx, int* iout, double* fout) {
   return YSX_OK;
 }
 
+extern "C" int ysx_run_info(const ysx_ctx* ctx, int* iout, double* fout) {
+  return ysx_event_info(ctx, 0, iout, fout);
+}
+
 namespace {
 
-int run_spectra(ysx_ctx* ctx) {
-  const RunK& k = ctx->rk;
+Queue plain_q(const ysx_ctx* c, int k, int e) {
+  const WorkList& w = c->W[k];
+  return Queue{w.d_plain + w.start[e], w.start[e + 1] - w.start[e], nullptr, 1};
+}
+
+// slot < 0: static distribution
+Queue batched_q(const ysx_ctx* c, int k, int slot) {
+  const WorkList& w = c->W[k];
+  return Queue{w.d_batched, (int)w.batched.size(), slot < 0 ? nullptr : c->d_queue + slot,
+               c->chunk};
+}
+
+int grid_of(const Queue& q) { return std::max(1, q.n); }
+
+PairRange pairs_of(const ysx_ctx* c, int e0, int e1) {
+  long long g0 = c->rk[e0].pbase;
+  long long g1 = (e1 < (int)c->rk.size()) ? c->rk[e1].pbase : c->npair;
+  return PairRange{e0, e1, g0, g1 - g0};
+}
+
+// the spectra of events [e0, e1) with one block per item: the plain path.
+// With timing, per-phase HIP events are recorded (one event only).
+int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
   hipStream_t st = ctx->stream, st2 = ctx->stream2;
   hipEvent_t* ev = ctx->ev;
   size_t lds = ctx->lds_bytes;
-  int P = k.npair;
   dim3 b256(256), b64(64);
-  int nl = k.lmax - k.lmin + 1;
-  int nw64 = k.nw * ((nl + 63) / 64), nw16 = k.nw * ((nl + 15) / 16);
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
-  HIPCHK(hipEventRecord(ev[0], st));
-  hipLaunchKernelGGL(k_source, dim3((P + 255) / 256), b256, lds, st, ctx->d_tab, k, ctx->d_rec,
-                     ctx->d_cnt);
-  HIPCHK(hipEventRecord(ev[1], st));
-  // the radial (l = 0) solves run beside the marches on a second stream
-  HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
-  HIPCHK(hipEventRecord(ev[10], st2));
-  if (k.lmin == 0)
-    hipLaunchKernelGGL(k_rad, dim3((k.nw + 63) / 64), dim3(64), lds, st2, ctx->d_tab, k,
+  for (int e = e0; e < e1; e++) {
+    PairRange PR = pairs_of(ctx, e, e + 1);
+    int nb = (int)((PR.n + 255) / 256);
+    Queue qt = plain_q(ctx, W_TOR, e), qb = plain_q(ctx, W_BVEC, e), qr = plain_q(ctx, W_RAD, e),
+          ql = plain_q(ctx, W_LSUM, e);
+    if (timing) HIPCHK(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(k_source, dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR,
                        ctx->d_rec, ctx->d_cnt);
-  HIPCHK(hipEventRecord(ev[11], st2));
-  hipLaunchKernelGGL(k_start, dim3((P + 255) / 256), b256, 0, st, ctx->d_tab, k, SD,
-                     ctx->d_starts);
-  HIPCHK(hipEventRecord(ev[2], st));
-  hipLaunchKernelGGL(k_tor_march, dim3(nw64), b64, 0, st, ctx->d_tab, k, SD, ctx->d_rec,
-                     ctx->d_cnt);
-  HIPCHK(hipEventRecord(ev[3], st));
-  hipLaunchKernelGGL(k_minor_top, dim3(nw64), b64, 0, st, ctx->d_tab, k, S, ctx->d_cnt);
-  hipLaunchKernelGGL(k_minor_march, dim3(nw64), b64, 0, st, ctx->d_tab, k, SD, S, ctx->d_cnt);
-  HIPCHK(hipEventRecord(ev[4], st));
-  hipLaunchKernelGGL(k_bvec_march, dim3(nw16), b64, 0, st, ctx->d_tab, k, S, ctx->d_rec,
-                     ctx->d_cnt);
-  HIPCHK(hipEventRecord(ev[5], st));
-  HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
-  HIPCHK(hipEventRecord(ev[6], st));
-  GemmK G{ctx->lpad, ctx->jpad, ctx->nrpad, ctx->d_cf, ctx->d_tb};
-  hipLaunchKernelGGL(k_assemble, dim3((P + 255) / 256), b256, 0, st, k, ctx->d_rec, G);
-  HIPCHK(hipEventRecord(ev[7], st));
-  hipLaunchKernelGGL(k_lsum, dim3((ctx->jpad / 16) * (ctx->nrpad / 16)), b64, 0, st, k, G,
-                     ctx->d_eia, ctx->d_spec);
-  HIPCHK(hipEventRecord(ev[8], st));
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventSynchronize(ev[8]));
-  auto el = [&](int a, int b) {
-    float t = 0.f;
-    (void)hipEventElapsedTime(&t, ev[a], ev[b]);
-    return (double)t;
-  };
-  ctx->ms[0] = el(0, 1) + el(1, 2);  // source vectors + starts
-  ctx->ms[1] = el(2, 3);
-  ctx->ms[2] = el(3, 4);
-  ctx->ms[3] = el(4, 5);
-  ctx->ms[4] = el(10, 11);
-  ctx->ms[5] = el(6, 7);
-  ctx->ms[6] = el(7, 8);
-  ctx->have_spec = true;
+    HIPCHK(hipEventRecord(ev[1], st));
+    // the radial (l = 0) solves run beside the marches on a second stream
+    HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+    if (timing) HIPCHK(hipEventRecord(ev[10], st2));
+    if (qr.n > 0)
+      hipLaunchKernelGGL(k_rad, dim3(grid_of(qr)), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
+                         ctx->d_rec, ctx->d_cnt);
+    HIPCHK(hipEventRecord(ev[11], st2));
+    hipLaunchKernelGGL(k_start, dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
+                       ctx->d_starts);
+    if (timing) HIPCHK(hipEventRecord(ev[2], st));
+    hipLaunchKernelGGL(k_tor_march<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                       SD, ctx->d_rec, ctx->d_cnt);
+    if (timing) HIPCHK(hipEventRecord(ev[3], st));
+    hipLaunchKernelGGL(k_minor_top<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                       S, ctx->d_cnt);
+    hipLaunchKernelGGL(k_minor_march<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs,
+                       qt, SD, S, ctx->d_cnt);
+    if (timing) HIPCHK(hipEventRecord(ev[4], st));
+    hipLaunchKernelGGL(k_bvec_march<false>, dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs,
+                       qb, S, ctx->d_rec, ctx->d_cnt);
+    if (timing) HIPCHK(hipEventRecord(ev[5], st));
+    HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
+    if (timing) HIPCHK(hipEventRecord(ev[6], st));
+    hipLaunchKernelGGL(k_assemble, dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec,
+                       ctx->d_cf, ctx->d_tb);
+    if (timing) HIPCHK(hipEventRecord(ev[7], st));
+    hipLaunchKernelGGL(k_lsum, dim3(grid_of(ql)), b64, 0, st, ctx->d_runs, ql, ctx->d_cf,
+                       ctx->d_tb, ctx->d_eia, ctx->d_spec);
+    if (timing) HIPCHK(hipEventRecord(ev[8], st));
+    HIPCHK(hipGetLastError());
+  }
   return YSX_OK;
 }
 
-}  // namespace
+int enqueue_seis_plain(ysx_ctx* ctx, int e0, int e1) {
+  for (int e = e0; e < e1; e++) {
+    Queue q = plain_q(ctx, W_SEIS, e);
+    hipLaunchKernelGGL(k_seis, dim3(grid_of(q)), dim3(1024), 0, ctx->stream, ctx->d_runs, q,
+                       ctx->d_spec, ctx->d_tw, ctx->d_work, ctx->d_azep, ctx->d_seis);
+  }
+  HIPCHK(hipGetLastError());
+  return YSX_OK;
+}
 
-namespace {
+// every event at once from the cost-sorted lists: one block per item, or
+// persistent blocks taking items from atomic queues
+int enqueue_batched(ysx_ctx* ctx) {
+  hipStream_t st = ctx->stream, st2 = ctx->stream2;
+  hipEvent_t* ev = ctx->ev;
+  size_t lds = ctx->lds_bytes;
+  dim3 b256(256), b64(64);
+  int nev = (int)ctx->rk.size();
+  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
+  StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
+  PairRange PR = pairs_of(ctx, 0, nev);
+  int nb = (int)((PR.n + 255) / 256);
+  bool pe = ctx->persist != 0;
+  int pers = ctx->blocks_per_cu * ctx->ncu;
+  auto grid = [&](const Queue& q) {
+    return dim3(std::max(1, pe ? std::min(q.n, pers) : q.n));
+  };
+  Queue qt = batched_q(ctx, W_TOR, Q_TOR), qtop = batched_q(ctx, W_TOR, Q_TOP),
+        qmin = batched_q(ctx, W_TOR, Q_MINOR), qb = batched_q(ctx, W_BVEC, Q_BVEC),
+        qr = batched_q(ctx, W_RAD, -1), ql = batched_q(ctx, W_LSUM, -1),
+        qs = batched_q(ctx, W_SEIS, -1);
+  if (pe) hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctx->d_queue, (int)Q_N);
+  hipLaunchKernelGGL(k_source, dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, ctx->d_rec,
+                     ctx->d_cnt);
+  HIPCHK(hipEventRecord(ev[1], st));
+  HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+  if (qr.n > 0)
+    hipLaunchKernelGGL(k_rad, dim3(qr.n), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
+                       ctx->d_rec, ctx->d_cnt);
+  HIPCHK(hipEventRecord(ev[11], st2));
+  hipLaunchKernelGGL(k_start, dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
+                     ctx->d_starts);
+  if (pe) {
+    hipLaunchKernelGGL(k_tor_march<true>, grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD,
+                       ctx->d_rec, ctx->d_cnt);
+    hipLaunchKernelGGL(k_minor_top<true>, grid(qtop), b64, 0, st, ctx->d_tab, ctx->d_runs, qtop,
+                       S, ctx->d_cnt);
+    hipLaunchKernelGGL(k_minor_march<true>, grid(qmin), b64, 0, st, ctx->d_tab, ctx->d_runs,
+                       qmin, SD, S, ctx->d_cnt);
+    hipLaunchKernelGGL(k_bvec_march<true>, grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
+                       ctx->d_rec, ctx->d_cnt);
+  } else {
+    hipLaunchKernelGGL(k_tor_march<false>, grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                       SD, ctx->d_rec, ctx->d_cnt);
+    hipLaunchKernelGGL(k_minor_top<false>, grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt, S,
+                       ctx->d_cnt);
+    hipLaunchKernelGGL(k_minor_march<false>, grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                       SD, S, ctx->d_cnt);
+    hipLaunchKernelGGL(k_bvec_march<false>, grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb,
+                       S, ctx->d_rec, ctx->d_cnt);
+  }
+  HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
+  hipLaunchKernelGGL(k_assemble, dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec, ctx->d_cf,
+                     ctx->d_tb);
+  hipLaunchKernelGGL(k_lsum, dim3(std::max(1, ql.n)), b64, 0, st, ctx->d_runs, ql, ctx->d_cf,
+                     ctx->d_tb, ctx->d_eia, ctx->d_spec);
+  hipLaunchKernelGGL(k_seis, dim3(std::max(1, std::min(qs.n, 4 * ctx->ncu))), dim3(1024), 0, st,
+                     ctx->d_runs, qs, ctx->d_spec, ctx->d_tw, ctx->d_work, ctx->d_azep,
+                     ctx->d_seis);
+  HIPCHK(hipGetLastError());
+  return YSX_OK;
+}
+
+int elapsed(ysx_ctx* ctx, int a, int b, double& out) {
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, ctx->ev[a], ctx->ev[b]));
+  out = t;
+  return YSX_OK;
+}
+
+int run_spectra(ysx_ctx* ctx) {
+  int rc = enqueue_plain(ctx, 0, 1, true);
+  if (rc) return rc;
+  HIPCHK(hipEventSynchronize(ctx->ev[8]));
+  double a, b;
+  if ((rc = elapsed(ctx, 0, 1, a)) || (rc = elapsed(ctx, 1, 2, b))) return rc;
+  ctx->ms[0] = a + b;  // source vectors + starts
+  if ((rc = elapsed(ctx, 2, 3, ctx->ms[1])) || (rc = elapsed(ctx, 3, 4, ctx->ms[2])) ||
+      (rc = elapsed(ctx, 4, 5, ctx->ms[3])) || (rc = elapsed(ctx, 10, 11, ctx->ms[4])) ||
+      (rc = elapsed(ctx, 6, 7, ctx->ms[5])) || (rc = elapsed(ctx, 7, 8, ctx->ms[6])))
+    return rc;
+  ctx->have_spec = true;
+  ctx->have_seis = false;
+  return YSX_OK;
+}
 
 int run_seis(ysx_ctx* ctx) {
-  const HostRun& R = ctx->R;
   hipStream_t st = ctx->stream;
-  SeisK K;
-  K.nt = R.nt;
-  K.lognt = 0;
-  while ((1 << K.lognt) < R.nt) K.lognt++;
-  K.nw = R.nw; K.nr = R.nr; K.i1 = R.i1; K.i2 = R.i2; K.out = R.out;
-  K.df = R.df; K.dt = R.dt; K.ep = R.ep; K.f11 = R.f11; K.f12 = R.f12; K.f21 = R.f21;
-  K.f22 = R.f22;
-  K.norm = R.out == 0 ? ctx->N.r_norm : (R.out == 1 ? ctx->N.vel_norm : ctx->N.acl_norm);
   HIPCHK(hipEventRecord(ctx->ev[12], st));
-  hipLaunchKernelGGL(k_seis, dim3(R.nr), dim3(1024), 0, st, K, ctx->d_spec, ctx->d_tw,
-                     ctx->d_work, ctx->d_azep, ctx->d_seis);
+  int rc = enqueue_seis_plain(ctx, 0, 1);
+  if (rc) return rc;
   HIPCHK(hipEventRecord(ctx->ev[13], st));
-  HIPCHK(hipGetLastError());
   HIPCHK(hipEventSynchronize(ctx->ev[13]));
-  float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, ctx->ev[12], ctx->ev[13]));
-  ctx->ms[7] = t;
+  if ((rc = elapsed(ctx, 12, 13, ctx->ms[7]))) return rc;
+  ctx->have_seis = true;
   return YSX_OK;
 }
 
@@ -1159,10 +1554,11 @@ int run_seis(ysx_ctx* ctx) {
 extern "C" int ysx_spectra(ysx_ctx* ctx, double* ur, double* ut, double* up) {
   if (!ctx) return YSX_ERR_ARG;
   if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
+  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: use ysx_forward_events");
   HIPCHK(hipSetDevice(ctx->device));
   int rc = run_spectra(ctx);
   if (rc) return rc;
-  size_t n = (size_t)ctx->R.nw * ctx->R.nr;
+  size_t n = (size_t)ctx->R[0].nw * ctx->R[0].nr;
   double* outs[3] = {ur, ut, up};
   for (int c = 0; c < 3; c++)
     if (outs[c])
@@ -1173,11 +1569,12 @@ extern "C" int ysx_spectra(ysx_ctx* ctx, double* ur, double* ut, double* up) {
 extern "C" int ysx_seismograms(ysx_ctx* ctx, double* seis) {
   if (!ctx) return YSX_ERR_ARG;
   if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra");
+  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: use ysx_forward_events");
   HIPCHK(hipSetDevice(ctx->device));
   int rc = run_seis(ctx);
   if (rc) return rc;
   if (seis)
-    HIPCHK(hipMemcpy(seis, ctx->d_seis, 3 * (size_t)ctx->R.nt * ctx->R.nr * sizeof(double),
+    HIPCHK(hipMemcpy(seis, ctx->d_seis, 3 * (size_t)ctx->R[0].nt * ctx->R[0].nr * sizeof(double),
                      hipMemcpyDeviceToHost));
   return YSX_OK;
 }
@@ -1188,16 +1585,95 @@ extern "C" int ysx_forward(ysx_ctx* ctx, double* seis) {
   return ysx_seismograms(ctx, seis);
 }
 
+extern "C" int ysx_set_batch_tuning(ysx_ctx* ctx, int persist, int chunk, int blocks_per_cu) {
+  if (!ctx || (persist != 0 && persist != 1) || chunk < 1 || chunk > 64 || blocks_per_cu < 1 ||
+      blocks_per_cu > 32)
+    return YSX_ERR_ARG;
+  if (persist != ctx->persist || chunk != ctx->chunk || blocks_per_cu != ctx->blocks_per_cu)
+    drop_graph(ctx);
+  ctx->persist = persist;
+  ctx->chunk = chunk;
+  ctx->blocks_per_cu = blocks_per_cu;
+  return YSX_OK;
+}
+
+extern "C" int ysx_forward_events(ysx_ctx* ctx, int mode, double* seis) {
+  if (!ctx || (mode != 0 && mode != 1)) return YSX_ERR_ARG;
+  if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
+  HIPCHK(hipSetDevice(ctx->device));
+  int nev = (int)ctx->rk.size();
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipEventRecord(ctx->ev[14], st));
+  if (mode == 0) {
+    int rc = enqueue_plain(ctx, 0, nev, false);
+    if (!rc) rc = enqueue_seis_plain(ctx, 0, nev);
+    if (rc) return rc;
+  } else {
+    if (!ctx->gexec) {
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+      int rc = enqueue_batched(ctx);
+      hipGraph_t g = nullptr;
+      hipError_t ec = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ec != hipSuccess)
+        return set_err(ctx, YSX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      ctx->graph = g;
+      HIPCHK(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(ctx->gexec, st));
+  }
+  HIPCHK(hipEventRecord(ctx->ev[15], st));
+  HIPCHK(hipEventSynchronize(ctx->ev[15]));
+  double t = 0.0;
+  int rc = elapsed(ctx, 14, 15, t);
+  if (rc) return rc;
+  for (double& m : ctx->ms) m = 0.0;
+  ctx->ms[7] = t;
+  ctx->have_spec = ctx->have_seis = true;
+  if (seis) {
+    size_t n = 0;
+    for (const HostRun& R : ctx->R) n += 3 * (size_t)R.nt * R.nr;
+    HIPCHK(hipMemcpy(seis, ctx->d_seis, n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return YSX_OK;
+}
+
+extern "C" int ysx_event_spectra(ysx_ctx* ctx, int e, double* ur, double* ut, double* up) {
+  if (!ctx || e < 0 || e >= (int)ctx->rk.size()) return YSX_ERR_ARG;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t n = (size_t)ctx->R[e].nw * ctx->R[e].nr;
+  double* outs[3] = {ur, ut, up};
+  for (int c = 0; c < 3; c++)
+    if (outs[c])
+      HIPCHK(hipMemcpy(outs[c], ctx->d_spec + ctx->rk[e].spec_off + c * n, n * sizeof(cd),
+                       hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
+extern "C" int ysx_event_seismograms(ysx_ctx* ctx, int e, double* seis) {
+  if (!ctx || !seis || e < 0 || e >= (int)ctx->rk.size()) return YSX_ERR_ARG;
+  if (!ctx->have_seis) return set_err(ctx, YSX_ERR_ARG, "no seismograms computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t n = 3 * (size_t)ctx->R[e].nt * ctx->R[e].nr;
+  HIPCHK(hipMemcpy(seis, ctx->d_seis + ctx->rk[e].seis_off, n * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
 extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j, double* rec) {
   if (!ctx || (n > 0 && (!l || !j || !rec))) return YSX_ERR_ARG;
   if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
   HIPCHK(hipSetDevice(ctx->device));
-  const RunK& k = ctx->rk;
+  const RunK& k = ctx->rk[0];
   for (int i = 0; i < n; i++) {
     if (l[i] < k.lmin || l[i] > k.lmax || j[i] < 1 || j[i] > k.nw)
       return set_err(ctx, YSX_ERR_ARG, "pair outside the run");
     size_t p = (size_t)(l[i] - k.lmin) * k.nw + (j[i] - 1);
-    HIPCHK(hipMemcpy(rec + (size_t)i * 178, ctx->d_rec + p * 89, 89 * sizeof(cd),
+    HIPCHK(hipMemcpy(rec + (size_t)i * 2 * REC, ctx->d_rec + p * REC, REC * sizeof(cd),
                      hipMemcpyDeviceToHost));
   }
   return YSX_OK;
@@ -1213,7 +1689,7 @@ extern "C" int ysx_step_counts(ysx_ctx* ctx, long long* out5) {
   if (!ctx || !out5) return YSX_ERR_ARG;
   if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
   HIPCHK(hipSetDevice(ctx->device));
-  size_t P = (size_t)ctx->rk.npair;
+  size_t P = (size_t)ctx->npair;
   std::vector<int> c(P * 5);
   HIPCHK(hipMemcpy(c.data(), ctx->d_cnt, c.size() * sizeof(int), hipMemcpyDeviceToHost));
   for (int q = 0; q < 5; q++) out5[q] = 0;

@@ -101,3 +101,27 @@ def workload(name: str, seed: int = 0, lmax: int | None = None, nr: int | None =
     for k, v in over.items():
         setattr(p, k, v)
     return p
+
+
+def batch_events(nev: int = 64, seed: int = 0, nr: int = 16, fmax_range=(10.0, 40.0),
+                 depth_range=(0.0, 700.0), tout: float = 180.0, dt: float = 1.0,
+                 lmax_per_mhz: float = 15.0, model_file: str = "synth.200") -> list:
+    """Seeded multi-event workload: per event a source depth uniform in
+    ``depth_range`` km, fmax uniform in ``fmax_range`` mHz with lmax =
+    ``lmax_per_mhz`` * fmax, ``nr`` receivers at the surface, a moment tensor
+    uniform in +-1e20 N m; 180 min at 1 s, attenuation and full
+    self-gravitation; cosine filters 0.2/0.5 mHz and 0.85/0.95 fmax."""
+    rng = np.random.default_rng(5000 + seed)
+    out = []
+    for _ in range(nev):
+        fmax = float(rng.uniform(*fmax_range))
+        depth = float(rng.uniform(*depth_range))
+        mm = rng.uniform(-1e20, 1e20, 6)
+        slat, slon = _receivers(rng, 1)
+        lat, lon = _receivers(rng, nr)
+        out.append(Params(prefix="yspec.out", model_file=model_file, ats=1, grav=2, out=0,
+                          cor=0, lmin=0, lmax=int(lmax_per_mhz * fmax), f1=0.1, f2=fmax,
+                          tout=tout, dt=dt, f11=0.2, f12=0.5, f21=0.85 * fmax,
+                          f22=0.95 * fmax, src_depth=depth, src_lat=float(slat[0]),
+                          src_lon=float(slon[0]), mm=mm, rec_depth=0.0, lat=lat, lon=lon))
+    return out
