@@ -16,7 +16,11 @@ the wall clock around K steps brackets exactly K steps.
 
 Prints one JSON line.  value = (l, w) solves/s of the full forward (pairs per
 step / seconds per step); seismograms/s = receivers per step / seconds per
-step.  roofline: the dominant kernel's algorithmic FP64 FLOPs (exact step
+step.  "batched": the 64-event workload (depth U[0,700] km, fmax U[10,40]
+mHz, lmax = 15 fmax, 16 receivers each, 180 min at 1 s; 10.3 M pairs) as
+full forwards through the plain path (events one after another) and the
+batched path (all events per launch, cost-sorted work, HIP graph), after
+yspec_amd/tune.py has picked the batched launch shape.  roofline: the dominant kernel's algorithmic FP64 FLOPs (exact step
 counts x the per-step constants of yspec_amd/flops.py) over its HIP-event
 time.  cpu_baseline: the reference Fortran (compiled from the reference's
 own sources into oracle/_ref, run through its FP64 driver) timed on this
@@ -39,7 +43,8 @@ sys.path.insert(0, ROOT)
 
 from yspec_amd import api  # noqa: E402
 from yspec_amd.flops import ode_flops, sum_flops, fft_bytes  # noqa: E402
-from yspec_amd.synth import synth_deck, workload  # noqa: E402
+from yspec_amd.synth import batch_events, synth_deck, workload  # noqa: E402
+from yspec_amd.tune import tune  # noqa: E402
 
 METRIC = "(l,ω) solves/s + seismograms/s, batched vs plain GPU; gradient/forward time"
 FP64_PEAK_TFLOPS = 78.6
@@ -73,6 +78,46 @@ def cpu_baseline(deck, p, sample):
     return pairs / sum(secs), pairs, sum(secs), wall
 
 
+def batched_compare(deck, steps, warmup, do_tune=True):
+    """The 64-event workload through the plain path (events one after
+    another, each with its own launches) and the batched path (all events
+    per launch from cost-sorted work lists, replayed from a HIP graph),
+    after the autotuner has picked the batched launch shape."""
+    ev = batch_events()
+    g = api.YSpec(deck)
+    g.set_events(ev)
+    P, nrec = g.npair, sum(p.nr for p in ev)
+    print(f"# workload 64 events: {P} (l,w) pairs, {nrec} receivers", file=sys.stderr, flush=True)
+    tuning = tune(g, log=lambda m: print("# " + m, file=sys.stderr, flush=True)) if do_tune \
+        else None
+    res = {}
+    for mode, name in ((0, "plain"), (1, "batched")):
+        for _ in range(warmup):
+            g.forward_events(mode)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.forward_events(mode)
+        sec = (time.perf_counter() - t0) / steps
+        print(f"# {name}: {sec * 1e3:.1f} ms per forward of all events", file=sys.stderr,
+              flush=True)
+        res[name] = {"ms_per_step": sec * 1e3, "solves_per_s": P / sec,
+                     "seismograms_per_s": nrec / sec}
+    counts = g.step_counts()
+    ode = ode_flops(counts)
+    g.close()
+    return {
+        "workload": "64 events on the same deck: depth U[0,700] km, fmax U[10,40] mHz, "
+                    "lmax = 15 fmax, 16 receivers each, 180 min at dt = 1 s, attenuation + "
+                    "self-gravitation",
+        "events": len(ev), "pairs": P, "receivers": nrec,
+        "plain": res["plain"], "batched": res["batched"],
+        "speedup": res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"],
+        "batched_ode_tflops_lower_bound": sum(ode.values()) / (res["batched"]["ms_per_step"]
+                                                               * 1e-3) / 1e12,
+        "tuning": tuning["best"] if tuning else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +125,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--lmax", type=int, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 64-event plain vs batched comparison")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the 64-event comparison (no tuning), for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -98,6 +147,10 @@ def main():
         sys.exit(2)
 
     deck = synth_deck(0)
+    if args.batched_only:
+        print(json.dumps(batched_compare(deck, args.steps, args.warmup, do_tune=False)),
+              flush=True)
+        return
     p = workload("cfg2") if args.lmax is None else workload("cfg2", lmax=args.lmax)
     g = api.YSpec(deck, p, device=0)
     P, nr = g.npair, p.nr
@@ -159,6 +212,9 @@ def main():
                              "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy"},
         "host_copy_ms_per_forward": pcie_ms,
     }
+    g.close()
+    if not args.no_batched and args.lmax is None:
+        line["batched"] = batched_compare(deck, args.steps, args.warmup)
     if not args.no_cpu:
         sample = list(range(0, p.lmax + 1, 30))
         cb = cpu_baseline(deck, p, sample)

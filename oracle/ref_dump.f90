@@ -51,6 +51,7 @@ program ref_dump
   complex(dpc) :: uu(5), vv(5), ww(5), ctmp1(4), ctmp2(4)
   complex(dpc) :: svt(5,2), svs(5,6), sr0(2)
   complex(dpc) :: s2(2,2), s6(6,4), ytor(2,2), ysph(6,4), yrad(2,2), prad(2,2)
+  complex(dpc) :: s4(4,4), ysphng(4,4)
   complex(dpc) :: ytor_raw(2,2), ysph_raw(6,4), yrad_raw(2,2)
   complex(dpc), allocatable :: eia(:), ur(:,:), ut(:,:), up(:,:)
   real(dp) :: tsec
@@ -158,6 +159,10 @@ program ref_dump
 
   s2 = 0.0_dp; s2(1,1) = 1.0_dp; s2(2,2) = 1.0_dp
   s6 = 0.0_dp; s6(1,1) = 1.0_dp; s6(2,2) = 1.0_dp; s6(4,3) = 1.0_dp; s6(5,4) = 1.0_dp
+  s4 = 0.0_dp
+  do k = 1, 4
+     s4(k,k) = 1.0_dp
+  end do
   call set_ats_switch(ats_in)
 
   if (.not. timing) then
@@ -242,8 +247,33 @@ contains
           end do
        end if
        call source_vector_sph(l, w, is, rs, mm, svs)
-       if (fic .or. grav_switch /= 2) stop 'ref_dump: only grav=2 with a solid inner core is dumped'
-       call sph_solver(l, w, rr, rs, ir, is, s6, ysph)
+       ! solver choice and the 4-vector -> 6-vector layout of the
+       ! reference driver (src/yspec.f90:342-374)
+       if (fic) then
+          if (grav_switch == 0) then
+             call sph_solver_ng_fic(l, w, rr, rs, ir, is, s4, ysphng)
+          else if (grav_switch == 1) then
+             call sph_solver_cow_fic(l, w, rr, rs, ir, is, s4, ysphng)
+          else
+             call sph_solver_fic(l, w, rr, rs, ir, is, s6, ysph)
+          end if
+       else
+          if (grav_switch == 0) then
+             call sph_solver_ng(l, w, rr, rs, ir, is, s4, ysphng)
+          else if (grav_switch == 1) then
+             call sph_solver_cow(l, w, rr, rs, ir, is, s4, ysphng)
+          else
+             call sph_solver(l, w, rr, rs, ir, is, s6, ysph)
+          end if
+       end if
+       if (grav_switch /= 2) then
+          ysph(1,:) = ysphng(1,:)
+          ysph(2,:) = ysphng(2,:)
+          ysph(3,:) = 0.0_dp
+          ysph(4,:) = ysphng(3,:)
+          ysph(5,:) = ysphng(4,:)
+          ysph(6,:) = 0.0_dp
+       end if
        call sph_start_level(l, real(w), merge(is, ir, rr >= rs), 1, isph)
        ysph_raw = ysph
        if (out_switch == 1) then

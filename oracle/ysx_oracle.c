@@ -1452,6 +1452,464 @@ static void sph_solver(pst *s, int l, cplx w, double rr, double rs, int ir,
   }
 }
 
+/* ================================================================== */
+/* variants: gravity 0 (no gravity) / 1 (Cowling), fluid inner core    */
+/* ================================================================== */
+
+/* sph_solid_amat_ng / _cow, src/module_int.f90:564-672 */
+typedef struct {
+  cplx t11, t12, t21, t22, k11, k22, s11, s12, s22;
+} ngc_amat;
+
+static void solid_amat_ngc(const pst *s, double rr, int cow, ngc_amat *A) {
+  const ysxo *o = s->o;
+  int il = s->il;
+  double rrho = SPL(o, RHO, rr, il), racon = SPL(o, ACON, rr, il),
+         rccon = SPL(o, CCON, rr, il), rfcon = SPL(o, FCON, rr, il),
+         rncon = SPL(o, NCON, rr, il), rlcon = SPL(o, LCON, rr, il),
+         rqmu = SPL(o, QMU, rr, il), rxlam = SPL(o, XLAM, rr, il),
+         rxa2 = SPL(o, XA2, rr, il);
+  double gg = cow ? SPL(o, GRAV, rr, il) : 0.0;
+  double zeta = s->zeta, zeta2 = s->zeta2;
+  cplx qaa = 1.0 + rxa2 * s->lno, qff = 1.0 + rxlam * s->lno,
+       qll = 1.0 + rqmu * s->lno;
+  cplx cacon = racon * qaa, cccon = rccon * qaa, cfcon = rfcon * qff,
+       cncon = rncon * qll, clcon = rlcon * qll;
+  cccon = 1.0 / cccon;
+  cplx gamma = cacon - cncon - cfcon * cfcon * cccon;
+  double ri = 1.0 / rr;
+  A->t11 = (1.0 - 2.0 * cfcon * cccon) * ri;
+  A->t12 = zeta * ri * cfcon * cccon;
+  A->t21 = -zeta * ri;
+  A->t22 = 2.0 * ri;
+  A->k11 = cccon;
+  A->k22 = 1.0 / clcon;
+  if (cow) {
+    A->s11 = -s->omega2 * rrho + 4.0 * (gamma - rrho * gg * rr) * ri * ri;
+    A->s12 = zeta * (rrho * gg * rr - 2.0 * gamma) * ri * ri;
+  } else {
+    A->s11 = -s->omega2 * rrho + 4.0 * gamma * ri * ri;
+    A->s12 = -2.0 * zeta * gamma * ri * ri;
+  }
+  A->s22 = -s->omega2 * rrho + (zeta2 * (gamma + cncon) - 2.0 * cncon) * ri * ri;
+}
+
+/* sph_minors_solid_derivs_ng / _cow, src/module_int.f90:874-936 */
+static void minors_solid_ngc(const pst *s, double rr, const cplx *m, cplx *d,
+                             int cow) {
+  ngc_amat A;
+  solid_amat_ngc(s, rr, cow, &A);
+  cplx a11 = A.t11 + A.t22, a22 = A.t11 - A.t22;
+  d[0] = a11 * m[0] + A.k22 * m[2] - A.k11 * m[3];
+  d[1] = A.s12 * m[0] - A.t21 * m[2] + A.t12 * m[3];
+  d[2] = A.s22 * m[0] - 2.0 * A.t12 * m[1] + a22 * m[2] + A.k11 * m[4];
+  d[3] = -A.s11 * m[0] + 2.0 * A.t21 * m[1] - a22 * m[3] - A.k22 * m[4];
+  d[4] = -2.0 * A.s12 * m[1] + A.s11 * m[2] - A.s22 * m[3] - a11 * m[4];
+}
+static void minors_solid_ng(const pst *s, double rr, const cplx *m, cplx *d) {
+  minors_solid_ngc(s, rr, m, d, 0);
+}
+static void minors_solid_cow(const pst *s, double rr, const cplx *m, cplx *d) {
+  minors_solid_ngc(s, rr, m, d, 1);
+}
+
+/* sph_fluid_amat_ng / _cow + sph_minors_fluid_derivs_ng / _cow,
+ * src/module_int.f90:720-786, 971-1012 */
+static void minors_fluid_ngc(const pst *s, double rr, const cplx *m, cplx *d,
+                             int cow) {
+  const ysxo *o = s->o;
+  int il = s->il;
+  double rrho = SPL(o, RHO, rr, il), rccon = SPL(o, CCON, rr, il),
+         rxa2 = SPL(o, XA2, rr, il);
+  double zeta2 = s->zeta2;
+  cplx qaa = 1.0 + rxa2 * s->lno;
+  cplx cccon = rccon * qaa;
+  cccon = 1.0 / cccon;
+  double ri = 1.0 / rr;
+  cplx t, k, sv;
+  if (cow) {
+    double gg = SPL(o, GRAV, rr, il);
+    t = (gg * zeta2 * s->omegai2 * ri - 1.0) * ri;
+    k = cccon - zeta2 * s->omegai2 * ri * ri / rrho;
+    sv = -s->omega2 * rrho + rrho * gg * (gg * zeta2 * ri * s->omegai2 - 4.0) * ri;
+  } else {
+    t = -ri;
+    k = cccon - zeta2 * s->omegai2 * ri * ri / rrho;
+    sv = -s->omega2 * rrho;
+  }
+  d[0] = t * m[0] + k * m[1];
+  d[1] = sv * m[0] - t * m[1];
+}
+static void minors_fluid_ng(const pst *s, double rr, const cplx *m, cplx *d) {
+  minors_fluid_ngc(s, rr, m, d, 0);
+}
+static void minors_fluid_cow(const pst *s, double rr, const cplx *m, cplx *d) {
+  minors_fluid_ngc(s, rr, m, d, 1);
+}
+
+/* sph_bvec_solid_derivs_ng / _cow, src/module_int.f90:1060-1110 */
+static void bvec_solid_ngc(const pst *s, double rr, const cplx *b, cplx *d,
+                           int cow) {
+  ngc_amat A;
+  solid_amat_ngc(s, rr, cow, &A);
+  d[0] = -A.s11 * b[2] - A.s12 * b[3] - A.t11 * b[0] - A.t21 * b[1];
+  d[1] = -A.s12 * b[2] - A.s22 * b[3] - A.t12 * b[0] - A.t22 * b[1];
+  d[2] = -A.k11 * b[0] + A.t11 * b[2] + A.t12 * b[3];
+  d[3] = -A.k22 * b[1] + A.t21 * b[2] + A.t22 * b[3];
+}
+static void bvec_solid_ng(const pst *s, double rr, const cplx *b, cplx *d) {
+  bvec_solid_ngc(s, rr, b, d, 0);
+}
+static void bvec_solid_cow(const pst *s, double rr, const cplx *b, cplx *d) {
+  bvec_solid_ngc(s, rr, b, d, 1);
+}
+
+/* sph_solid_start_ng, src/module_int.f90:1275-1341 (sbessj2 stub
+ * ratio = x / real(2 l + 3) with l = llp1, src/module_function.f90:447-490);
+ * y is (4,2) column-major */
+static void sph_solid_start_ng(const pst *s, int is, double rr, cplx y[2][4]) {
+  const ysxo *o = s->o;
+  double rrho = SPL(o, RHO, rr, is), rmu = SPL(o, MU, rr, is),
+         rqmu = SPL(o, QMU, rr, is), rkappa = SPL(o, KAPPA, rr, is),
+         rqkappa = SPL(o, QKAPPA, rr, is);
+  int ll = s->ll, llp1 = s->llp1;
+  double zeta = s->zeta;
+  cplx cmu = rmu * (1.0 + rqmu * s->lno);
+  cplx ckappa = rkappa * (1.0 + rqkappa * s->lno);
+  cplx calpha = (ckappa + 1.3333333333 * cmu) / rrho;
+  cplx cbeta = cmu / rrho;
+  cplx arg = s->omega * rr / csqrt(calpha);
+  cplx ratio = arg / (double)(float)(2 * llp1 + 3);
+  cplx j1 = 1.0, j2 = ratio;
+  y[0][0] = (double)ll * j1 - arg * j2;
+  y[0][1] = zeta * j1;
+  y[0][2] = -rrho * calpha * arg * arg * j1 +
+            2.0 * cmu * ((double)(ll * (ll - 1)) * j1 + 2.0 * arg * j2);
+  y[0][2] = y[0][2] / rr;
+  y[0][3] = 2.0 * zeta * cmu * ((double)(ll - 1) * j1 - arg * j2);
+  y[0][3] = y[0][3] / rr;
+  arg = s->omega * rr / csqrt(cbeta);
+  ratio = arg / (double)(float)(2 * llp1 + 3);
+  j1 = 1.0;
+  j2 = ratio;
+  y[1][0] = (double)(-ll * llp1) * j1;
+  y[1][1] = -zeta * llp1 * j1 + arg * j2;
+  y[1][2] = -2.0 * cmu * ((double)(ll * (ll * ll - 1)) * j1 -
+                          (double)(ll * llp1) * arg * j2);
+  y[1][2] = y[1][2] / rr;
+  y[1][3] = zeta * cmu * (arg * arg * j1 - 2.0 * (double)(ll * ll - 1) * j1 -
+                          2.0 * arg * j2);
+  y[1][3] = y[1][3] / rr;
+}
+
+/* sph_fluid_start_ng, src/module_int.f90:1407-1447 */
+static void sph_fluid_start_ng(const pst *s, int is, double rr, cplx *y) {
+  const ysxo *o = s->o;
+  double rrho = SPL(o, RHO, rr, is), rkappa = SPL(o, KAPPA, rr, is),
+         rqkappa = SPL(o, QKAPPA, rr, is);
+  cplx ckappa = rkappa * (1.0 + rqkappa * s->lno);
+  cplx calpha = ckappa / rrho;
+  cplx arg = s->omega * rr / csqrt(calpha);
+  cplx ratio = arg / (double)(float)(2 * s->llp1 + 3);
+  y[0] = (double)s->ll - arg * ratio;
+  y[1] = -rrho * calpha * arg * arg;
+  y[1] = y[1] / rr;
+}
+
+/* sph_minor_solid_start_ng, src/module_int.f90:1489-1506 */
+static void sph_minor_solid_start_ng(const pst *s, int is, double rr, cplx *m) {
+  static const int rows[5][2] = {{1, 2}, {1, 3}, {1, 4}, {2, 3}, {3, 4}};
+  cplx y[2][4];
+  sph_solid_start_ng(s, is, rr, y);
+  for (int k = 0; k < 5; k++) {
+    int p = rows[k][0] - 1, q = rows[k][1] - 1;
+    m[k] = det2(y[0][p], y[0][q], y[1][p], y[1][q]);
+  }
+}
+
+/* b_solid_start_ngcow, src/module_int.f90:1596-1615 */
+static void b_solid_start_ngc(const cplx *m, const cplx *sv, cplx *b) {
+  b[0] = -m[4] * sv[1] - m[1] * sv[2] - m[3] * sv[3];
+  b[1] = m[4] * sv[0] - m[2] * sv[2] + m[1] * sv[3];
+  b[2] = m[1] * sv[0] + m[2] * sv[1] - m[0] * sv[3];
+  b[3] = m[3] * sv[0] - m[1] * sv[1] + m[0] * sv[2];
+}
+
+/* sph_minors_solid_2_fluid_ngcow / fluid_2_solid_ngcow, sph_delta_solid_ngcow,
+ * sph_ysol_ngcow: src/module_int.f90 (after :1638) */
+static void solid_2_fluid_ngc(const cplx *m1, cplx *m2) {
+  m2[0] = -m1[2];
+  m2[1] = -m1[4];
+}
+static void fluid_2_solid_ngc(const cplx *m1, cplx *m2) {
+  m2[0] = m1[0];
+  m2[3] = -m1[1];
+  m2[1] = m2[2] = m2[4] = 0.0;
+}
+static cplx delta_solid_ngc(const cplx *m1, const cplx *m2) {
+  return m1[4] * m2[0] + 2.0 * m1[1] * m2[1] + m1[3] * m2[2] + m1[2] * m2[3] +
+         m1[0] * m2[4];
+}
+static void ysol_ngc(const cplx *m, const cplx *b, cplx *y) {
+  y[0] = m[0] * b[1] + m[1] * b[2] + m[2] * b[3];
+  y[1] = -m[0] * b[0] + m[3] * b[2] - m[1] * b[3];
+  y[2] = -m[1] * b[0] - m[3] * b[1] + m[4] * b[3];
+  y[3] = -m[2] * b[0] + m[1] * b[1] - m[4] * b[2];
+}
+
+/* sph_solver_ng / _cow / _ng_fic / _cow_fic with ss = identity(4),
+ * src/module_solver.f90:827-1195, 1386-1740.  yout (4,4) column-major. */
+static void sph_solver_ngc(pst *s, int l, cplx w, double rr, double rs, int ir,
+                           int is, int cow, int fic, cplx *yout, int *start) {
+  const ysxo *o = s->o;
+  const double *r = R(o);
+  int nic = o->nic, noc = o->noc, nsl = o->nsl, nknot = o->nknot;
+  derivs_fn dsol = cow ? minors_solid_cow : minors_solid_ng;
+  derivs_fn dflu = cow ? minors_fluid_cow : minors_fluid_ng;
+  derivs_fn dbv = cow ? bvec_solid_cow : bvec_solid_ng;
+  set_l(s, l);
+  s->ats = o->ats_in;
+  set_omega(s, w);
+  alpha_steps(s);
+  int sc1 = 0, sc2 = 0, sc3 = 0, sc4, i1;
+  cplx mf1[2], mf2[2], ms1[5], ms2[5], ms3[5], b[4], delta;
+  int up = rr >= rs;
+  int iup = up ? is : ir;
+  double rup = up ? rs : rr;
+  i1 = sph_start_level(o, s->ll, creal(s->omega), iup);
+  *start = i1;
+  double rt = r[i1];
+  /* the solid-core solvers take the small first radius only when the
+   * receiver is above the source; the fluid-core ones always */
+  if ((up || fic) && i1 == 1) rt = fmin(0.1 * s->dr, 0.1 * r[2]);
+  if (i1 >= iup) {
+    i1 = iup - 1;
+    rt = r[i1];
+  }
+  if (i1 <= nic && !fic) {
+    sph_minor_solid_start_ng(s, i1, rt, ms1);
+    yint(s, i1, nic, rt, r[nic], ms1, 5, dsol, &sc1);
+    solid_2_fluid_ngc(ms1, mf1);
+    yint(s, nic + 1, noc, r[nic], r[noc], mf1, 2, dflu, &sc1);
+    fluid_2_solid_ngc(mf1, ms1);
+    yint(s, noc + 1, iup, r[noc + 1], rup, ms1, 5, dsol, &sc1);
+  } else if (i1 <= noc) {
+    sph_fluid_start_ng(s, i1, rt, mf1);
+    yint(s, i1, noc, rt, r[noc], mf1, 2, dflu, &sc1);
+    fluid_2_solid_ngc(mf1, ms1);
+    yint(s, noc + 1, iup, r[noc + 1], rup, ms1, 5, dsol, &sc1);
+  } else if (i1 <= nsl) {
+    sph_minor_solid_start_ng(s, i1, rt, ms1);
+    yint(s, i1, iup, rt, rup, ms1, 5, dsol, &sc1);
+  }
+  memcpy(ms3, ms1, sizeof ms3);
+  int ibot = up ? ir : is;
+  double rbot = up ? rr : rs;
+  if (up && rr > rs) yint(s, is, ir, rs, rr, ms3, 5, dsol, &sc3);
+  if (!up && rs > rr) yint(s, ir, is, rr, rs, ms3, 5, dsol, &sc3);
+  if (o->ocean) {
+    mf2[0] = 1.0;
+    mf2[1] = 0.0;
+    yint(s, nknot - 1, nsl + 1, r[nknot], r[nsl + 1], mf2, 2, dflu, &sc2);
+    fluid_2_solid_ngc(mf2, ms2);
+    yint(s, nsl, ibot, r[nsl], rbot, ms2, 5, dsol, &sc2);
+  } else {
+    ms2[0] = 1.0;
+    for (int i = 1; i < 5; i++) ms2[i] = 0.0;
+    yint(s, nknot, ibot, r[nknot], rbot, ms2, 5, dsol, &sc2);
+  }
+  delta = delta_solid_ngc(ms3, ms2);
+  for (int c = 0; c < 4; c++) {
+    cplx ssv[4] = {0, 0, 0, 0};
+    ssv[c] = 1.0;
+    sc4 = 0;
+    cplx *y = yout + 4 * c;
+    if (up) {
+      b_solid_start_ngc(ms1, ssv, b);
+      yint(s, is, ir, rs, rr, b, 4, dbv, &sc4);
+      ysol_ngc(ms2, b, y);
+      for (int i = 0; i < 4; i++) {
+        y[i] = y[i] / delta;
+        y[i] = y[i] * pow(HUGE_, sc3 - sc4);
+      }
+    } else {
+      b_solid_start_ngc(ms2, ssv, b);
+      for (int i = 0; i < 4; i++) b[i] = -b[i] / delta;
+      yint(s, is, ir, rs, rr, b, 4, dbv, &sc4);
+      ysol_ngc(ms1, b, y);
+      for (int i = 0; i < 4; i++) y[i] = y[i] * pow(HUGE_, sc3 - sc4);
+    }
+  }
+}
+
+/* sph_solver_fic (grav = 2, fluid inner core), src/module_solver.f90:
+ * 1203-1382.  The receiver-below-source branch repeats sph_solver's
+ * solid-core start (a solid start in a fluid inner core, :1323-1331);
+ * it is reproduced as written. */
+static void sph_solver_fic(pst *s, int l, cplx w, double rr, double rs, int ir,
+                           int is, cplx *yout, int *start) {
+  const ysxo *o = s->o;
+  const double *r = R(o);
+  int nic = o->nic, noc = o->noc, nsl = o->nsl, nknot = o->nknot;
+  set_l(s, l);
+  s->ats = o->ats_in;
+  set_omega(s, w);
+  alpha_steps(s);
+  int sc1 = 0, sc2 = 0, sc3 = 0, sc4, i1;
+  cplx mf1[5], mf2[5], ms1[14], ms2[14], ms3[14], b[14], delta;
+  int up = rr >= rs;
+  int iup = up ? is : ir;
+  double rup = up ? rs : rr;
+  i1 = sph_start_level(o, s->ll, creal(s->omega), iup);
+  *start = i1;
+  double rt = r[i1];
+  if (i1 == 1) rt = fmin(0.1 * s->dr, 0.1 * r[2]);
+  if (i1 >= iup) {
+    i1 = iup - 1;
+    rt = r[i1];
+  }
+  if (i1 <= nic && !up) {
+    sph_minor_solid_start(s, i1, rt, ms1);
+    yint(s, i1, nic, rt, r[nic], ms1, 14, sph_minors_solid_derivs, &sc1);
+    solid_2_fluid(ms1, mf1);
+    yint(s, nic + 1, noc, r[nic], r[noc], mf1, 5, sph_minors_fluid_derivs,
+         &sc1);
+    fluid_2_solid(mf1, ms1);
+    yint(s, noc + 1, iup, r[noc + 1], rup, ms1, 14, sph_minors_solid_derivs,
+         &sc1);
+  } else if (i1 <= noc) {
+    sph_minor_fluid_start(s, i1, rt, mf1);
+    yint(s, i1, noc, rt, r[noc], mf1, 5, sph_minors_fluid_derivs, &sc1);
+    fluid_2_solid(mf1, ms1);
+    yint(s, noc + 1, iup, r[noc + 1], rup, ms1, 14, sph_minors_solid_derivs,
+         &sc1);
+  } else if (i1 <= nsl) {
+    sph_minor_solid_start(s, i1, rt, ms1);
+    yint(s, i1, iup, rt, rup, ms1, 14, sph_minors_solid_derivs, &sc1);
+  }
+  memcpy(ms3, ms1, sizeof ms3);
+  int ibot = up ? ir : is;
+  double rbot = up ? rr : rs;
+  if (up && rr > rs)
+    yint(s, is, ir, rs, rr, ms3, 14, sph_minors_solid_derivs, &sc3);
+  if (!up && rs > rr)
+    yint(s, ir, is, rr, rs, ms3, 14, sph_minors_solid_derivs, &sc3);
+  if (o->ocean) {
+    mf2[0] = 1.0;
+    for (int i = 1; i < 5; i++) mf2[i] = 0.0;
+    yint(s, nknot - 1, nsl + 1, r[nknot], r[nsl + 1], mf2, 5,
+         sph_minors_fluid_derivs, &sc2);
+    fluid_2_solid(mf2, ms2);
+    yint(s, nsl, ibot, r[nsl], rbot, ms2, 14, sph_minors_solid_derivs, &sc2);
+  } else {
+    ms2[0] = 1.0;
+    for (int i = 1; i < 14; i++) ms2[i] = 0.0;
+    yint(s, up ? nknot - 1 : nknot, ibot, r[nknot], rbot, ms2, 14,
+         sph_minors_solid_derivs, &sc2);
+  }
+  delta = sph_delta_solid(ms3, ms2);
+  for (int c = 0; c < 4; c++) {
+    cplx ssv[6];
+    for (int i = 0; i < 6; i++) ssv[i] = S6[c][i];
+    sc4 = 0;
+    cplx *y = yout + 6 * c;
+    if (up) {
+      b_solid_start(ms1, ssv, b);
+      yint(s, is, ir, rs, rr, b, 14, sph_bvec_solid_derivs, &sc4);
+      sph_ysol(ms2, b, y);
+      for (int i = 0; i < 6; i++) {
+        y[i] = -y[i] / delta;
+        y[i] = y[i] * pow(HUGE_, sc3 - sc4);
+      }
+    } else {
+      b_solid_start(ms2, ssv, b);
+      for (int i = 0; i < 14; i++) b[i] = -b[i] / delta;
+      yint(s, is, ir, rs, rr, b, 14, sph_bvec_solid_derivs, &sc4);
+      sph_ysol(ms1, b, y);
+      for (int i = 0; i < 6; i++) y[i] = y[i] * pow(HUGE_, sc3 - sc4);
+    }
+  }
+}
+
+/* rad_ystart_ng, src/module_int.f90:455-489 */
+static void rad_ystart_ng(const pst *s, int is, double rr, cplx *y) {
+  const ysxo *o = s->o;
+  double rrho = SPL(o, RHO, rr, is), rmu = SPL(o, MU, rr, is),
+         rqmu = SPL(o, QMU, rr, is), rkappa = SPL(o, KAPPA, rr, is),
+         rqkappa = SPL(o, QKAPPA, rr, is);
+  cplx cmu = rmu * (1.0 + rqmu * s->lno);
+  cplx ckappa = rkappa * (1.0 + rqkappa * s->lno);
+  cplx calpha = (ckappa + 4.0 * cmu / 3.0);
+  cplx gamma = s->omega2;
+  gamma = rrho * gamma / calpha;
+  cplx arg = gamma * rr;
+  cplx ratio = arg / 5.0;
+  y[0] = rr * ratio;
+  y[1] = rr * calpha * gamma - 4.0 * cmu * ratio;
+}
+
+/* rad_solver_ng with ss = identity(2), src/module_solver.f90:335-441.  Its
+ * RHS, rad_derivs_ng (src/module_int.f90:377-392), calls rad_amat -- the
+ * self-gravitating coefficients -- not rad_amat_ng; reproduced as written. */
+static void rad_solver_ng(pst *s, cplx w, double rr, double rs, int ir, int is,
+                          cplx *yout) {
+  const ysxo *o = s->o;
+  const double *r = R(o);
+  int nknot = o->nknot, nsl = o->nsl;
+  set_l(s, 0);
+  set_omega(s, w);
+  s->ats = o->ats_in;
+  set_omega(s, w);
+  alpha_steps(s);
+  int sc1 = 0, sc2 = 0, sc3 = 0;
+  cplx y1[2], y2[2], y3[2], delta;
+  int i1 = 1;
+  double rt = fmin(0.1 * s->dr, 0.1 * r[2]);
+  rad_ystart_ng(s, i1, rt, y1);
+  if (rr >= rs) {
+    yint(s, i1, is, rt, rs, y1, 2, rad_derivs, &sc1);
+    y2[0] = 1.0;
+    y2[1] = 0.0;
+    if (rr != r[nknot]) yint(s, nknot - 1, ir, r[nknot], rr, y2, 2, rad_derivs, &sc2);
+    y3[0] = y2[0];
+    y3[1] = y2[1];
+    if (rs != rr) yint(s, ir, is, rr, rs, y3, 2, rad_derivs, &sc3);
+  } else {
+    yint(s, i1, ir, rt, rr, y1, 2, rad_derivs, &sc1);
+    y2[0] = 1.0;
+    y2[1] = 0.0;
+    if (rs != r[nsl]) yint(s, nsl - 1, is, r[nsl], rs, y2, 2, rad_derivs, &sc2);
+    y3[0] = y2[0];
+    y3[1] = y2[1];
+    if (rr != rs) yint(s, is, ir, rs, rr, y3, 2, rad_derivs, &sc3);
+  }
+  delta = y3[1] * y1[0] - y3[0] * y1[1];
+  if (cabs(delta) < 1.0e-100)
+    delta = 0.0;
+  else
+    delta = 1.0 / delta;
+  for (int i = 0; i < 2; i++) {
+    cplx ss1 = (i == 0) ? 1.0 : 0.0, ss2 = (i == 1) ? 1.0 : 0.0;
+    if (rr >= rs) {
+      cplx b = y1[1] * ss1 - y1[0] * ss2;
+      yout[2 * i] = -delta * y2[0] * b;
+      yout[2 * i + 1] = -delta * y2[1] * b;
+    } else {
+      cplx b = y2[1] * ss1 - y2[0] * ss2;
+      yout[2 * i] = -delta * y1[0] * b;
+      yout[2 * i + 1] = -delta * y1[1] * b;
+    }
+  }
+  if (rr >= rs) {
+    if (sc3 != 0)
+      for (int i = 0; i < 4; i++) yout[i] = yout[i] * pow(HUGE_, sc3);
+  } else {
+    /* do i = 1, scale3: nothing when scale3 <= 0 (src/module_solver.f90:433-437) */
+    for (int k = 1; k <= sc3; k++)
+      for (int i = 0; i < 4; i++) yout[i] = yout[i] * HUGE_;
+  }
+}
+
 /* source_vector_tor, src/module_solver.f90:145-215; svt (5,2) */
 static void source_vector_tor(pst *s, int l, cplx w, int is, double rs,
                               const double *mm, cplx *svt) {
@@ -1577,8 +2035,7 @@ int ysxo_set_run(ysxo *o, const int *ip, const double *fp, const double *lat,
   if (!o->nknot) return fail(o, "no model");
   o->lmin = ip[0]; o->lmax = ip[1]; o->ats_in = ip[2]; o->grav = ip[3];
   o->out = ip[4]; o->cor = ip[5]; o->nr = ip[6];
-  if (o->grav != 2 || o->fic)
-    return fail(o, "oracle covers grav=2 with a solid inner core");
+  if (o->grav < 0 || o->grav > 2) return fail(o, "gravity switch must be 0, 1 or 2");
   double f1 = fp[0], f2 = fp[1], tout = fp[2], dt = fp[3];
   double f11 = fp[4], f12 = fp[5], f21 = fp[6], f22 = fp[7];
   double sdep = fp[8], lats = fp[9], lons = fp[10], rdep = fp[17];
@@ -1694,7 +2151,11 @@ static void do_pair(const ysxo *o, int l, int j, pair_out *P) {
   cplx w = o->wmin + (j - 1) * o->dw - I * o->ep;
   if (l == 0) {
     source_vector_rad(&s, w, o->is, rs, o->mm, P->sr0);
-    rad_solver_wg(&s, w, rr, rs, P->yrad);
+    /* src/yspec.f90:271-276 */
+    if (o->grav == 0)
+      rad_solver_ng(&s, w, rr, rs, o->ir, o->is, P->yrad);
+    else
+      rad_solver_wg(&s, w, rr, rs, P->yrad);
     cplx u3 = (P->sr0[0] * P->yrad[0] + P->sr0[1] * P->yrad[2]) / rr;
     if (o->out == 1) u3 = I * w * u3;
     if (o->out == 2) u3 = -w * w * u3;
@@ -1714,7 +2175,27 @@ static void do_pair(const ysxo *o, int l, int j, pair_out *P) {
       P->ww[m] = (P->svt[m] * yt[0] + P->svt[5 + m] * yt[2]) / (zeta * rr);
   }
   source_vector_sph(&s, l, w, o->is, rs, o->mm, P->svs);
-  sph_solver(&s, l, w, rr, rs, o->ir, o->is, P->ysph, &P->starts[1]);
+  /* solver choice and the 4-vector layout, src/yspec.f90:342-374 */
+  if (o->grav == 2) {
+    if (o->fic)
+      sph_solver_fic(&s, l, w, rr, rs, o->ir, o->is, P->ysph, &P->starts[1]);
+    else
+      sph_solver(&s, l, w, rr, rs, o->ir, o->is, P->ysph, &P->starts[1]);
+  } else {
+    cplx yng[16];
+    sph_solver_ngc(&s, l, w, rr, rs, o->ir, o->is, o->grav == 1, o->fic, yng,
+                   &P->starts[1]);
+    for (int c = 0; c < 4; c++) {
+      cplx *y6 = P->ysph + 6 * c;
+      const cplx *y4 = yng + 4 * c;
+      y6[0] = y4[0];
+      y6[1] = y4[1];
+      y6[2] = 0.0;
+      y6[3] = y4[2];
+      y6[4] = y4[3];
+      y6[5] = 0.0;
+    }
+  }
   cplx y[24];
   memcpy(y, P->ysph, sizeof y);
 #define Y(i, c) y[(c)*6 + (i)-1]

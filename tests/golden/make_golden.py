@@ -64,6 +64,16 @@ def aniso_deck(seed=0):
     return Deck("synth.aniso", 1, 1.0, d.nknot, d.nic, d.noc, cols)
 
 
+def fic_deck(seed=0):
+    """synthetic deck with a fluid inner core (vs = 0, Qmu = 0 below the ICB)"""
+    d = synth_deck(seed)
+    cols = d.cols.copy()
+    cols[:d.nic, 3] = 0.0
+    cols[:d.nic, 5] = 0.0
+    cols[:d.nic, 7] = 0.0
+    return Deck("synth.fic", 0, 1.0, d.nknot, d.nic, d.noc, cols)
+
+
 def prem_example(deck_file, nr=4, **over):
     """the reference's example parameter file (ref_examples/input.txt, PREM,
     29.5 km source, receivers at 3 km depth) cut down to a few seconds"""
@@ -77,6 +87,17 @@ def prem_example(deck_file, nr=4, **over):
 
 
 def cases():
+    # gravity switches 0 / 1 and fluid inner cores (src/yspec.f90:271-276, 342-374)
+    small = dict(lmax=24, nr=2, ats=1, f2=5.0, tout=40.0, dt=20.0, f21=3.0, f22=4.5)
+    yield "ng_solid", synth_deck(0), workload("cfg1", grav=0, src_depth=300.0, **small)
+    yield "cow_ocean_vel", ocean_deck(0), workload("cfg1", grav=1, out=1, src_depth=50.0,
+                                                   rec_depth=4.0, **small)
+    yield "fic_grav2", fic_deck(0), workload("cfg1", src_depth=400.0, **small)
+    yield "ng_fic_rec_below", fic_deck(0), workload("cfg1", grav=0, src_depth=20.0,
+                                                    rec_depth=150.0, **small)
+    yield "cow_fic_acc", fic_deck(0), workload("cfg1", grav=1, out=2, cor=1,
+                                               src_depth=100.0, **small)
+    yield ("prem_fic_example", *prem_example("prem.200.fic"))
     yield ("prem_example", *prem_example("prem.200"))
     yield ("prem_noocean_disp", *prem_example("prem.200.no", ats=0, out=0, cor=0,
                                               rec_depth=0.0))
