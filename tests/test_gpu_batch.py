@@ -25,6 +25,8 @@ def small_events():
     ev[1].src_depth = 650.0   # deep: minors start in the core at low l
     ev[2].src_depth = 1.5     # shallow crustal source
     ev[3].lmin = 2            # no radial pairs in this event
+    ev[4].grav = 1            # Cowling: spheroidal solves in the per-lane kernel
+    ev[0].grav = 0            # no gravity: per-lane spheroidal and radial solvers
     for p in ev:
         p.f21, p.f22 = 0.6 * p.f2, 0.8 * p.f2
     return ev

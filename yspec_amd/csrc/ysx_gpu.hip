@@ -34,6 +34,7 @@
 #include "../../include/ysx.h"
 #include "ysx_device.hpp"
 #include "ysx_march.hpp"
+#include "ysx_variants.hpp"
 #include "ysx_host.hpp"
 
 using namespace ysx;
@@ -44,6 +45,7 @@ namespace {
 struct RunK {
   int lmin, lmax, nw, nr, is, ir, out, cor, npair;
   int nknot, nic, noc, nsl, fic, ocean;
+  int grav, var;                // gravity switch; var: spheroidal solves per lane
   int lpad, jpad, nrpad;        // GEMM padding of this event
   int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
   long long pbase;              // first pair of this event in the per-pair arrays
@@ -253,6 +255,7 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
     SD.tor_y0[2 * p] = y[0];
     SD.tor_y0[2 * p + 1] = y[1];
   }
+  if (R.var) return;  // the per-lane variant solvers find their own start
   int i1 = sph_start_level(T, L, l, s.rew, itop, R.fre_norm);
   starts[2 * p + 1] = i1;
   double rt = T(F_R, i1);
@@ -417,6 +420,7 @@ __device__ __forceinline__ int minors_from_top(const Tab& T, const WS& w, const 
 __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, int lb,
                                            const StartData& SD, const SphState& S,
                                            int* __restrict__ cnt, double* sb, cd* slot) {
+  if (R.var) return;  // spheroidal solves of this event run in k_sph_var
   int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 1 && l <= R.lmax);
   size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
@@ -516,6 +520,7 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
 __device__ __forceinline__ void minor_top_item(const Tab& T, const RunK& R, int j, int lb,
                                                const SphState& S, int* __restrict__ cnt,
                                                double* sb, cd* slot) {
+  if (R.var) return;
   int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 1 && l <= R.lmax);
   if (wave_min(valid ? 0 : 1)) return;
@@ -554,6 +559,7 @@ __global__ void __launch_bounds__(64) k_minor_top(const double* __restrict__ gta
 __device__ __forceinline__ void bvec_item(const Tab& T, const RunK& R, int j, int lb,
                                           const SphState& S, cd* __restrict__ rec,
                                           int* __restrict__ cnt, double* sb, cd* slot) {
+  if (R.var) return;
   int c = threadIdx.x & 3;
   int l = R.lmin + lb * 16 + (int)(threadIdx.x >> 2);
   bool valid = (l >= 1 && l <= R.lmax);
@@ -646,6 +652,7 @@ __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
   size_t p = (size_t)R.pbase + (j - 1);  // l = 0 pairs come first
   PS s;
   pair_state(R, 0, j, s);
+  if (R.grav == 0) return;  // rad_solver_ng: k_rad_ng
   RadRHS f;
   int nk = R.nknot;
   double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
@@ -745,6 +752,82 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab,
                                             cd* __restrict__ rec, int* __restrict__ cnt) {
   Tab T = load_tab(gtab, runs[0].nknot);
   RUN_ITEMS(false, Q, rad_item(T, runs[it.x], it.y, rec, cnt));
+}
+
+// rad_solver_ng (gravity switch 0, src/yspec.f90:271-273) for the l = 0
+// pairs of the events with grav = 0; same items as k_rad
+__device__ __forceinline__ void rad_ng_item(const Tab& T, const RunK& R, int jb,
+                                            cd* __restrict__ rec, int* __restrict__ cnt) {
+  int j = jb * 64 + threadIdx.x + 1;
+  if (j > R.nw || R.lmin != 0 || R.grav != 0) return;
+  size_t p = (size_t)R.pbase + (j - 1);
+  PS s;
+  pair_state(R, 0, j, s);
+  VarGeo G{R.nknot, R.nic, R.noc, R.nsl, R.ocean, R.is, R.ir, R.rs, R.rr, R.alpha_min,
+           R.fre_norm};
+  cd yo[4];
+  int n = 0;
+  rad_solve_ng(T, s, G, yo, n);
+  cd* o = rec + p * REC + O_YRAD;
+  for (int i = 0; i < 4; i++) o[i] = yo[i];
+  cnt[5 * p + 4] = n;
+}
+
+__global__ void __launch_bounds__(64) k_rad_ng(const double* __restrict__ gtab,
+                                               const RunK* __restrict__ runs, Queue Q,
+                                               cd* __restrict__ rec, int* __restrict__ cnt) {
+  Tab T = load_tab(gtab, runs[0].nknot);
+  RUN_ITEMS(false, Q, rad_ng_item(T, runs[it.x], it.y, rec, cnt));
+}
+
+// ------------------------------------------------------------- k_sph_var
+// The spheroidal solve of one pair per lane for events outside the lockstep
+// kernels: gravity switch 0 / 1 (sph_solver_ng, _cow, _ng_fic, _cow_fic)
+// or a fluid inner core with self-gravitation (sph_solver_fic), with the
+// reference driver's 4-vector -> 6-vector layout (src/yspec.f90:342-374).
+__device__ __forceinline__ void sph_var_item(const Tab& T, const RunK& R, int pb,
+                                             cd* __restrict__ rec, int* __restrict__ cnt) {
+  int pl = pb * 64 + (int)threadIdx.x;
+  if (pl >= R.npair) return;
+  int l, j;
+  pair_lj(R, pl, l, j);
+  if (l < 1) return;
+  PS s;
+  pair_state(R, l, j, s);
+  VarGeo G{R.nknot, R.nic, R.noc, R.nsl, R.ocean, R.is, R.ir, R.rs, R.rr, R.alpha_min,
+           R.fre_norm};
+  VarCount n{0, 0, 0};
+  size_t p = (size_t)R.pbase + pl;
+  cd* o = rec + p * REC + O_YSPH;
+  if (R.grav == 2) {
+    cd y[24];
+    sph_solve_fic(T, s, G, y, n);
+    for (int q = 0; q < 24; q++) o[q] = y[q];
+  } else {
+    cd y[16];
+    if (R.grav == 1)
+      sph_solve_ngc<true>(T, s, G, R.fic != 0, y, n);
+    else
+      sph_solve_ngc<false>(T, s, G, R.fic != 0, y, n);
+    for (int c = 0; c < 4; c++) {
+      o[6 * c + 0] = y[4 * c + 0];
+      o[6 * c + 1] = y[4 * c + 1];
+      o[6 * c + 2] = C(0.0);
+      o[6 * c + 3] = y[4 * c + 2];
+      o[6 * c + 4] = y[4 * c + 3];
+      o[6 * c + 5] = C(0.0);
+    }
+  }
+  cnt[5 * p + 1] = n.sol;
+  cnt[5 * p + 2] = n.flu;
+  cnt[5 * p + 3] = n.bvec;
+}
+
+// items (event, block of 64 pairs) of the events with var set
+__global__ void __launch_bounds__(64) k_sph_var(const double* __restrict__ gtab,
+                                                const RunK* __restrict__ runs, Queue Q,
+                                                cd* __restrict__ rec, int* __restrict__ cnt) {
+  RUN_ITEMS(false, Q, sph_var_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, rec, cnt));
 }
 
 // ------------------------------------------------------------- k_assemble
@@ -1015,7 +1098,7 @@ __global__ void k_twiddles(int nt, cd* tw) {
 // ====================================================================== API
 namespace {
 
-enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_N };
+enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_VAR, W_N };
 // queue counters of the batched launches (three marches share the W_TOR list)
 enum QueueSlot { Q_TOR = 0, Q_TOP, Q_MINOR, Q_BVEC, Q_RAD, Q_N };
 
@@ -1145,6 +1228,12 @@ void build_work(ysx_ctx* c) {
         c->W[W_RAD].plain.push_back(make_int4(e, jb, 0, 0));
         cost[W_RAD].push_back(rew(std::min(R.nw, 64 * jb + 64)));
       }
+    if (R.var)
+      for (int pb = 0; pb < (R.npair + 63) / 64; pb++) {
+        c->W[W_VAR].plain.push_back(make_int4(e, pb, 0, 0));
+        // the block's highest w sets its step count
+        cost[W_VAR].push_back(rew(std::min(R.nw, (64 * pb + 63) % R.nw + 1)) * (rtop + 0.05));
+      }
     int ntile = (R.jpad / 16) * (R.nrpad / 16);
     for (int t = 0; t < ntile; t++) {
       c->W[W_LSUM].plain.push_back(make_int4(e, t, 0, 0));
@@ -1250,6 +1339,8 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     k.npair = (R.lmax - R.lmin + 1) * R.nw;
     k.nknot = M.nknot; k.nic = M.nic; k.noc = M.noc; k.nsl = M.nsl; k.fic = M.fic;
     k.ocean = M.ocean;
+    k.grav = R.grav;
+    k.var = (R.grav != 2 || M.fic) ? 1 : 0;
     k.lpad = (R.lmax + 1 + 3) / 4 * 4;
     k.jpad = (R.nw + 15) / 16 * 16;
     k.nrpad = (R.nr + 15) / 16 * 16;
@@ -1410,9 +1501,14 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
     // the radial (l = 0) solves run beside the marches on a second stream
     HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
     if (timing) HIPCHK(hipEventRecord(ev[10], st2));
-    if (qr.n > 0)
-      hipLaunchKernelGGL(k_rad, dim3(grid_of(qr)), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
-                         ctx->d_rec, ctx->d_cnt);
+    if (qr.n > 0) {
+      if (ctx->rk[e].grav == 0)
+        hipLaunchKernelGGL(k_rad_ng, dim3(grid_of(qr)), b64, lds, st2, ctx->d_tab, ctx->d_runs,
+                           qr, ctx->d_rec, ctx->d_cnt);
+      else
+        hipLaunchKernelGGL(k_rad, dim3(grid_of(qr)), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
+                           ctx->d_rec, ctx->d_cnt);
+    }
     HIPCHK(hipEventRecord(ev[11], st2));
     hipLaunchKernelGGL(k_start, dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
                        ctx->d_starts);
@@ -1420,13 +1516,20 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
     hipLaunchKernelGGL(k_tor_march<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
                        SD, ctx->d_rec, ctx->d_cnt);
     if (timing) HIPCHK(hipEventRecord(ev[3], st));
-    hipLaunchKernelGGL(k_minor_top<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
-                       S, ctx->d_cnt);
-    hipLaunchKernelGGL(k_minor_march<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs,
-                       qt, SD, S, ctx->d_cnt);
-    if (timing) HIPCHK(hipEventRecord(ev[4], st));
-    hipLaunchKernelGGL(k_bvec_march<false>, dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs,
-                       qb, S, ctx->d_rec, ctx->d_cnt);
+    if (ctx->rk[e].var) {
+      Queue qv = plain_q(ctx, W_VAR, e);
+      hipLaunchKernelGGL(k_sph_var, dim3(grid_of(qv)), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
+                         ctx->d_rec, ctx->d_cnt);
+      if (timing) HIPCHK(hipEventRecord(ev[4], st));
+    } else {
+      hipLaunchKernelGGL(k_minor_top<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
+                         ctx->d_runs, qt, S, ctx->d_cnt);
+      hipLaunchKernelGGL(k_minor_march<false>, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
+                         ctx->d_runs, qt, SD, S, ctx->d_cnt);
+      if (timing) HIPCHK(hipEventRecord(ev[4], st));
+      hipLaunchKernelGGL(k_bvec_march<false>, dim3(grid_of(qb)), b64, 0, st, ctx->d_tab,
+                         ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
+    }
     if (timing) HIPCHK(hipEventRecord(ev[5], st));
     HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
     if (timing) HIPCHK(hipEventRecord(ev[6], st));
@@ -1477,9 +1580,15 @@ int enqueue_batched(ysx_ctx* ctx) {
                      ctx->d_cnt);
   HIPCHK(hipEventRecord(ev[1], st));
   HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
-  if (qr.n > 0)
+  if (qr.n > 0) {
     hipLaunchKernelGGL(k_rad, dim3(qr.n), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
                        ctx->d_rec, ctx->d_cnt);
+    bool any_ng = false;
+    for (const RunK& k : ctx->rk) any_ng = any_ng || (k.grav == 0 && k.lmin == 0);
+    if (any_ng)
+      hipLaunchKernelGGL(k_rad_ng, dim3(qr.n), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
+                         ctx->d_rec, ctx->d_cnt);
+  }
   HIPCHK(hipEventRecord(ev[11], st2));
   hipLaunchKernelGGL(k_start, dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
                      ctx->d_starts);
@@ -1502,6 +1611,10 @@ int enqueue_batched(ysx_ctx* ctx) {
     hipLaunchKernelGGL(k_bvec_march<false>, grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb,
                        S, ctx->d_rec, ctx->d_cnt);
   }
+  Queue qv = batched_q(ctx, W_VAR, -1);
+  if (qv.n > 0)
+    hipLaunchKernelGGL(k_sph_var, dim3(qv.n), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
+                       ctx->d_rec, ctx->d_cnt);
   HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
   hipLaunchKernelGGL(k_assemble, dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec, ctx->d_cf,
                      ctx->d_tb);

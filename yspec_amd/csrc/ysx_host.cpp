@@ -276,8 +276,7 @@ std::string build_run(const Norms& N, const HostModel& M, const int* ip, const d
   R.nr = ip[6];
   if (R.lmin < 0 || R.lmax < R.lmin || R.nr < 1) return "bad lmin/lmax/receiver count";
   if (R.out < 0 || R.out > 2) return "output switch must be 0, 1 or 2";
-  if (R.grav != 2 || M.fic)
-    return "only full self-gravitation (2) with a solid inner core is implemented";
+  if (R.grav < 0 || R.grav > 2) return "gravitation switch must be 0, 1 or 2";
   double f1 = fp[0], f2 = fp[1], tout = fp[2], dt = fp[3];
   double sdep = fp[8], lats = fp[9], lons = fp[10], rdep = fp[17];
   int nr = R.nr, L1 = R.lmax + 1;
