@@ -49,6 +49,19 @@ const char *ysx_last_error(const ysx_ctx *ctx);
 int ysx_set_model(ysx_ctx *ctx, int nknot, int nic, int noc, int ians,
                   double tref_period, const double *deck);
 
+/* The model table the kernels read, built on the device by ysx_set_model:
+ * F_COUNT = 29 fields x nknot knots, field-major -- r, then rho, A, C, F,
+ * L, N, kappa, mu, 1/Qkappa, 1/Qmu, xlam, xa2 each followed by its spline
+ * second derivatives, then g and its second derivatives, alpha, beta, in
+ * the reference's normalised units (the arrays of src/module_model.f90:
+ * 118-160 after read_model).  out holds 29 * nknot doubles. */
+int ysx_model_table(const ysx_ctx *ctx, double *out);
+
+/* The same table computed on the host without a context or a GPU (the
+ * set-up path of the run parameters uses it). */
+int ysx_model_table_host(int nknot, int nic, int noc, int ians,
+                         double tref_period, const double *deck, double *out);
+
 /* Set the run: ip = lmin lmax ats grav out cor nr; fp = f1 f2 tout dt f11
  * f12 f21 f22 src_depth src_lat src_lon Mrr Mrt Mrp Mtt Mtp Mpp rec_depth;
  * lat/lon of the nr receivers.  Receiver geometry, Legendre tables, fcal,

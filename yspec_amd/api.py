@@ -23,7 +23,8 @@ I = C.POINTER(C.c_int)
 EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_set_run",
            "ysx_run_info", "ysx_spectra", "ysx_seismograms", "ysx_forward",
            "ysx_set_events", "ysx_event_info", "ysx_forward_events", "ysx_set_batch_tuning",
-           "ysx_event_spectra", "ysx_event_seismograms",
+           "ysx_event_spectra", "ysx_event_seismograms", "ysx_model_table",
+           "ysx_model_table_host",
            "ysx_pair_records", "ysx_timings", "ysx_step_counts")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
@@ -65,6 +66,8 @@ def lib():
         L.ysx_set_batch_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.ysx_event_spectra.argtypes = [vp, C.c_int, D, D, D]
         L.ysx_event_seismograms.argtypes = [vp, C.c_int, D]
+        L.ysx_model_table.argtypes = [vp, D]
+        L.ysx_model_table_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D]
         L.ysx_pair_records.argtypes = [vp, C.c_int, I, I, D]
         L.ysx_timings.argtypes = [vp, D]
         L.ysx_step_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
@@ -78,6 +81,24 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(I)
+
+
+# names of the 29 fields of the model table (include/ysx.h, ysx_model_table)
+TABLE_FIELDS = ("r", "rho", "rho_cs", "acon", "acon_cs", "ccon", "ccon_cs", "fcon", "fcon_cs",
+                "lcon", "lcon_cs", "ncon", "ncon_cs", "kappa", "kappa_cs", "mu", "mu_cs",
+                "qkappa", "qkappa_cs", "qmu", "qmu_cs", "xlam", "xlam_cs", "xa2", "xa2_cs",
+                "grav", "grav_cs", "alpha", "beta")
+
+
+def model_table_host(deck):
+    """(29, nknot) model table computed on the host (no GPU needed)."""
+    cols = np.ascontiguousarray(deck.cols, dtype=np.float64)
+    out = np.zeros((len(TABLE_FIELDS), deck.nknot))
+    rc = lib().ysx_model_table_host(deck.nknot, deck.nic, deck.noc, deck.ians, deck.tref,
+                                    _dp(cols), _dp(out))
+    if rc:
+        raise YSpecError(rc, "bad deck")
+    return out
 
 
 class YSpec:
@@ -164,6 +185,12 @@ class YSpec:
             cnt = int(np.prod(shape))
             out[name] = rec[:, n:n + cnt].reshape((len(ls),) + shape, order="F")
             n += cnt
+        return out
+
+    def model_table(self):
+        """(29, nknot) model table the kernels read (built on the device)."""
+        out = np.zeros((len(TABLE_FIELDS), self.deck.nknot))
+        self._check(self.L.ysx_model_table(self.h, _dp(out)))
         return out
 
     def timings(self):

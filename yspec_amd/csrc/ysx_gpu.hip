@@ -35,6 +35,7 @@
 #include "ysx_device.hpp"
 #include "ysx_march.hpp"
 #include "ysx_variants.hpp"
+#include "ysx_model.hpp"
 #include "ysx_host.hpp"
 
 using namespace ysx;
@@ -170,9 +171,9 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
   pair_lj(R, p, l, j);
   PS s;
   pair_state(R, l, j, s);
+  // records and step counters are zeroed once when the run is set; every
+  // forward writes the same entries of them again
   cd* o = rec + (size_t)g * REC;
-  for (int i = 0; i < REC; i++) o[i] = C(0.0);
-  for (int i = 0; i < 5; i++) cnt[5 * g + i] = 0;
   SplW w = spl_w(T, R.is, R.rs);
   double rlcon = spl(T, w, F_LCON), rccon = spl(T, w, F_CCON), rfcon = spl(T, w, F_FCON),
          rqmu = spl(T, w, F_QMU), rxlam = spl(T, w, F_XLAM), rxa2 = spl(T, w, F_XA2);
@@ -866,7 +867,6 @@ __global__ void __launch_bounds__(256) k_assemble(const RunK* __restrict__ runs,
   cd w = C(R.wmin + (double)(j - 1) * R.dw, -R.ep);
   cd fo = (R.out == 1) ? cmuli(w) : (R.out == 2 ? -(w * w) : C(1.0));
   double rr = R.rr;
-  for (int m = 0; m < 15; m++) o[O_UU + m] = C(0.0);
   if (l == 0) {
     cd u3 = (o[O_SR0] * o[O_YRAD + 0] + o[O_SR0 + 1] * o[O_YRAD + 2]) / rr;
     o[O_UU + 2] = fo * u3;
@@ -1002,6 +1002,60 @@ __global__ void __launch_bounds__(64) k_lsum(const RunK* __restrict__ runs, Queu
                       spec + runs[it.x].spec_off));
 }
 
+// ----------------------------------------------------------------- k_model
+// The deck -> table transform on the device (one block): moduli per knot,
+// natural splines per layer, gravity quadrature with the reference's
+// sequential accumulation (src/module_model.f90:253-378), through the same
+// functions as the host (ysx_model.hpp).  layer_lo (0-based first knot)
+// and layer_n per layer; work holds 3 (nknot + 1) doubles per (layer,
+// field) solve; gi holds nknot doubles.
+__global__ void __launch_bounds__(256) k_model(const double* __restrict__ deck, int nknot,
+                                               int ifanis, double vel_norm,
+                                               const int* __restrict__ layer_lo,
+                                               const int* __restrict__ layer_n, int nlayer,
+                                               double* __restrict__ tab, double* __restrict__ work,
+                                               double* __restrict__ intg,
+                                               double* __restrict__ qint,
+                                               double* __restrict__ gi) {
+#pragma clang fp contract(off)
+  int tid = threadIdx.x, nt = blockDim.x;
+  size_t ws = 3 * ((size_t)nknot + 1);
+  for (int i = tid; i < nknot; i += nt) knot_fields(deck, nknot, i, ifanis, vel_norm, tab);
+  __syncthreads();
+  const double* r = tab + (size_t)F_R * nknot;
+  const double* rho = tab + (size_t)F_RHO * nknot;
+  for (int i = tid; i < nknot; i += nt) intg[i] = r[i] * r[i] * rho[i];
+  __syncthreads();
+  for (int t = tid; t < 13 * nlayer; t += nt) {
+    int k = t / 13, q = t % 13, lo = layer_lo[k], n = layer_n[k];
+    if (q < 12) {
+      int f = spline_field(q);
+      natural_spline(r + lo, tab + (size_t)f * nknot + lo, tab + (size_t)(f + 1) * nknot + lo, n,
+                     work + t * ws);
+    } else {
+      natural_spline(r + lo, intg + lo, qint + lo, n, work + t * ws);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nknot - 1; i += nt) gi[i] = grav_interval(r, intg, qint, i);
+  __syncthreads();
+  double* grav = tab + (size_t)F_GRAV * nknot;
+  if (tid == 0) {
+    double fint = 0.0;
+    grav[0] = 0.0;
+    for (int i = 0; i < nknot - 1; i++) {
+      fint += gi[i];
+      grav[i + 1] = fint;
+    }
+    for (int i = 0; i < nknot; i++)
+      if (r[i] > 0.0) grav[i] = (4.0 * grav[i]) / (r[i] * r[i]);
+  }
+  __syncthreads();
+  for (int k = tid; k < nlayer; k += nt)
+    natural_spline(r + layer_lo[k], grav + layer_lo[k], tab + (size_t)F_GRAV2 * nknot + layer_lo[k],
+                   layer_n[k], work + k * ws);
+}
+
 // ------------------------------------------------------------------ k_seis
 // ifft (src/module_fourier.f90:55-145) and src/yspec.f90:439-473 for one
 // receiver per block: pack bins i1..i2, cosine filters, Hermitian fill,
@@ -1081,6 +1135,117 @@ __global__ void __launch_bounds__(1024) k_seis(const RunK* __restrict__ runs, Qu
   }
 }
 
+// The same transform in two passes for nt <= FFT_B * FFT_G: stages
+// mmax = 1 .. FFT_B/2 stay inside contiguous blocks of FFT_B elements of the
+// bit-reversed row and run in LDS (k_fft_lds); the remaining stages mix
+// only elements i0 + t FFT_B (t = 0 .. nt/FFT_B - 1) and run in registers,
+// followed by exp(eps t), units and rotation (k_fft_post).  Butterflies,
+// twiddles and their order are those of k_seis, so the results are the same
+// bit for bit.
+constexpr int FFT_B = 4096;
+constexpr int FFT_G = 16;
+
+// items (event, receiver k, c << 16 | block); work rows as in k_seis
+__global__ void __launch_bounds__(512) k_fft_lds(const RunK* __restrict__ runs, Queue Q,
+                                                 const cd* __restrict__ spec_all,
+                                                 const cd* __restrict__ tw,
+                                                 cd* __restrict__ work_all) {
+  __shared__ cd buf[FFT_B];
+  for (int it = blockIdx.x; it < Q.n; it += gridDim.x) {
+    int4 item = Q.items[it];
+    const RunK& K = runs[item.x];
+    int k = item.y, c = item.z >> 16, blk = item.z & 0xffff;
+    int nt = K.nt, bk = min(nt, FFT_B);
+    const cd* sp = spec_all + K.spec_off + (size_t)c * K.nw * K.nr;
+    cd* row = work_all + K.seis_off + ((size_t)k * 3 + c) * nt + (size_t)blk * bk;
+    for (int q = threadIdx.x; q < bk; q += blockDim.x)
+      buf[q] = packed_bin(K, sp, k, (int)brev((unsigned)(blk * bk + q), K.lognt));
+    __syncthreads();
+    for (int mmax = 1; mmax < bk; mmax <<= 1) {
+      for (int b = threadIdx.x; b < bk / 2; b += blockDim.x) {
+        int m = b & (mmax - 1);
+        int i = ((b - m) << 1) + m;
+        int jx = i + mmax;
+        cd temp = tw[mmax + m] * buf[jx];
+        cd a = buf[i];
+        buf[jx] = a - temp;
+        buf[i] = a + temp;
+      }
+      __syncthreads();
+    }
+    for (int q = threadIdx.x; q < bk; q += blockDim.x) row[q] = buf[q];
+    __syncthreads();
+  }
+}
+
+template <int G>
+__device__ __forceinline__ void fft_post_g(const RunK& K, int k, int i0, const cd* __restrict__ tw,
+                                           const cd* __restrict__ work, double ca, double sa,
+                                           double* __restrict__ seis) {
+  int nt = K.nt, bk = nt / G;
+  cd v[3][G];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cd* row = work + ((size_t)k * 3 + c) * nt;
+#pragma unroll
+    for (int t = 0; t < G; t++) v[c][t] = row[i0 + t * bk];
+  }
+#pragma unroll
+  for (int step = 1; step < G; step <<= 1) {
+    int mmax = step * bk;
+#pragma unroll
+    for (int t = 0; t < G; t++) {
+      if (t & step) continue;
+      int tj = t + step;
+      cd w = tw[mmax + ((i0 + t * bk) & (mmax - 1))];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        cd temp = w * v[c][tj];
+        cd a = v[c][t];
+        v[c][tj] = a - temp;
+        v[c][t] = a + temp;
+      }
+    }
+  }
+  double sc = K.norm / (K.dt * nt);
+#pragma unroll
+  for (int t = 0; t < G; t++) {
+    int i = i0 + t * bk;
+    double g = exp(K.ep * (i * K.dt)) * sc;
+    double z = v[0][t].x * g, ut = v[1][t].x * g, up = v[2][t].x * g;
+    seis[(size_t)i + (size_t)nt * (0 + 3 * (size_t)k)] = z;
+    seis[(size_t)i + (size_t)nt * (1 + 3 * (size_t)k)] = -ut * ca + up * sa;
+    seis[(size_t)i + (size_t)nt * (2 + 3 * (size_t)k)] = ut * sa + up * ca;
+  }
+}
+
+// items (event, receiver k, block of 256 column indices i0 < min(nt, FFT_B))
+__global__ void __launch_bounds__(256) k_fft_post(const RunK* __restrict__ runs, Queue Q,
+                                                  const cd* __restrict__ tw,
+                                                  const cd* __restrict__ work_all,
+                                                  const double* __restrict__ azep_all,
+                                                  double* __restrict__ seis_all) {
+  for (int it = blockIdx.x; it < Q.n; it += gridDim.x) {
+    int4 item = Q.items[it];
+    const RunK& K = runs[item.x];
+    int k = item.y;
+    int bk = min(K.nt, FFT_B), g = K.nt / bk;
+    int i0 = item.z * 256 + (int)threadIdx.x;
+    if (i0 >= bk) continue;
+    double az = azep_all[K.roff + k];
+    double ca = cos(az), sa = sin(az);
+    const cd* work = work_all + K.seis_off;
+    double* seis = seis_all + K.seis_off;
+    switch (g) {
+      case 1: fft_post_g<1>(K, k, i0, tw, work, ca, sa, seis); break;
+      case 2: fft_post_g<2>(K, k, i0, tw, work, ca, sa, seis); break;
+      case 4: fft_post_g<4>(K, k, i0, tw, work, ca, sa, seis); break;
+      case 8: fft_post_g<8>(K, k, i0, tw, work, ca, sa, seis); break;
+      default: fft_post_g<16>(K, k, i0, tw, work, ca, sa, seis); break;
+    }
+  }
+}
+
 __global__ void k_twiddles(int nt, cd* tw) {
   // tw[mmax + m] = exp(i pi_d m / mmax), mmax = 1, 2, 4, ..., nt/2
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1098,7 +1263,7 @@ __global__ void k_twiddles(int nt, cd* tw) {
 // ====================================================================== API
 namespace {
 
-enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_VAR, W_N };
+enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_VAR, W_FFTA, W_FFTB, W_N };
 // queue counters of the batched launches (three marches share the W_TOR list)
 enum QueueSlot { Q_TOR = 0, Q_TOP, Q_MINOR, Q_BVEC, Q_RAD, Q_N };
 
@@ -1239,9 +1404,24 @@ void build_work(ysx_ctx* c) {
       c->W[W_LSUM].plain.push_back(make_int4(e, t, 0, 0));
       cost[W_LSUM].push_back((double)R.lpad);
     }
-    for (int k = 0; k < R.nr; k++) {
-      c->W[W_SEIS].plain.push_back(make_int4(e, k, 0, 0));
-      cost[W_SEIS].push_back((double)R.nt);
+    int bk = std::min(R.nt, FFT_B);
+    if (R.nt / bk <= FFT_G) {  // two-pass transform
+      for (int k = 0; k < R.nr; k++)
+        for (int cc = 0; cc < 3; cc++)
+          for (int blk = 0; blk < R.nt / bk; blk++) {
+            c->W[W_FFTA].plain.push_back(make_int4(e, k, (cc << 16) | blk, 0));
+            cost[W_FFTA].push_back(1.0);
+          }
+      for (int k = 0; k < R.nr; k++)
+        for (int ib = 0; ib < (bk + 255) / 256; ib++) {
+          c->W[W_FFTB].plain.push_back(make_int4(e, k, ib, 0));
+          cost[W_FFTB].push_back(1.0);
+        }
+    } else {  // long series: radix-2 passes in global memory
+      for (int k = 0; k < R.nr; k++) {
+        c->W[W_SEIS].plain.push_back(make_int4(e, k, 0, 0));
+        cost[W_SEIS].push_back((double)R.nt);
+      }
     }
     for (int k = 0; k < W_N; k++) c->W[k].start.push_back((int)c->W[k].plain.size());
   }
@@ -1304,8 +1484,62 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
   if (ctx->lds_bytes > ctx->lds_max)
     return set_err(ctx, YSX_ERR_UNSUPPORTED, "deck has too many knots for the LDS-resident table");
   HIPCHK(hipMalloc(&ctx->d_tab, ctx->lds_bytes));
-  HIPCHK(hipMemcpy(ctx->d_tab, ctx->M.tab.data(), ctx->lds_bytes, hipMemcpyHostToDevice));
+  // the table the kernels read is built on the device from the raw deck;
+  // the host copy (same functions) serves the run set-up
+  {
+    const HostModel& M = ctx->M;
+    int nl = M.nlayer;
+    std::vector<int> lo(nl), n(nl);
+    for (int k = 0; k < nl; k++) {
+      lo[k] = M.layer_lo[k] - 1;
+      n[k] = M.layer_hi[k] - M.layer_lo[k] + 1;
+    }
+    size_t ws = 3 * ((size_t)nknot + 1), nwork = ws * 13 * (size_t)nl;
+    double *d_deck = nullptr, *d_work = nullptr;
+    int* d_lay = nullptr;
+    auto cleanup = [&]() {
+      dfree(d_deck);
+      dfree(d_work);
+      dfree(d_lay);
+    };
+    hipError_t e1 = hipMalloc(&d_deck, 9 * (size_t)nknot * sizeof(double));
+    if (e1 == hipSuccess) e1 = hipMalloc(&d_work, (nwork + 3 * (size_t)nknot) * sizeof(double));
+    if (e1 == hipSuccess) e1 = hipMalloc(&d_lay, 2 * (size_t)nl * sizeof(int));
+    if (e1 == hipSuccess)
+      e1 = hipMemcpy(d_deck, deck, 9 * (size_t)nknot * sizeof(double), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) e1 = hipMemcpy(d_lay, lo.data(), nl * sizeof(int), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess)
+      e1 = hipMemcpy(d_lay + nl, n.data(), nl * sizeof(int), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) e1 = hipMemset(ctx->d_tab, 0, ctx->lds_bytes);
+    if (e1 == hipSuccess) {
+      hipLaunchKernelGGL(k_model, dim3(1), dim3(256), 0, ctx->stream, d_deck, nknot, M.ifanis,
+                         ctx->N.vel_norm, d_lay, d_lay + nl, nl, ctx->d_tab, d_work,
+                         d_work + nwork, d_work + nwork + nknot, d_work + nwork + 2 * (size_t)nknot);
+      e1 = hipGetLastError();
+    }
+    if (e1 == hipSuccess) e1 = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e1 != hipSuccess)
+      return set_err(ctx, YSX_ERR_HIP, std::string("model transform: ") + hipGetErrorString(e1));
+  }
   ctx->have_model = true;
+  return YSX_OK;
+}
+
+extern "C" int ysx_model_table(const ysx_ctx* ctx, double* out) {
+  if (!ctx || !out || !ctx->have_model) return YSX_ERR_ARG;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipMemcpy(out, ctx->d_tab, ctx->lds_bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? YSX_OK : YSX_ERR_HIP;
+}
+
+extern "C" int ysx_model_table_host(int nknot, int nic, int noc, int ians, double tref_period,
+                                    const double* deck, double* out) {
+  if (!deck || !out) return YSX_ERR_ARG;
+  HostModel M;
+  if (!build_model(make_norms(), nknot, nic, noc, ians, tref_period, deck, M).empty())
+    return YSX_ERR_INPUT;
+  std::memcpy(out, M.tab.data(), M.tab.size() * sizeof(double));
   return YSX_OK;
 }
 
@@ -1369,8 +1603,10 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipMalloc(&ctx->d_runs, nev * sizeof(RunK)));
   HIPCHK(hipMemcpy(ctx->d_runs, ctx->rk.data(), nev * sizeof(RunK), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx->d_rec, P * REC * sizeof(cd)));
+  HIPCHK(hipMemset(ctx->d_rec, 0, P * REC * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_starts, P * 2 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_cnt, P * 5 * sizeof(int)));
+  HIPCHK(hipMemset(ctx->d_cnt, 0, P * 5 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_bst, P * 56 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_mfin, P * 14 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_m2, P * 14 * sizeof(cd)));
@@ -1544,12 +1780,24 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
   return YSX_OK;
 }
 
+void enqueue_fft(ysx_ctx* ctx, const Queue& qs, const Queue& qa, const Queue& qb, int maxgrid) {
+  hipStream_t st = ctx->stream;
+  auto g = [&](const Queue& q) { return dim3(std::max(1, std::min(q.n, maxgrid))); };
+  if (qs.n > 0)
+    hipLaunchKernelGGL(k_seis, g(qs), dim3(1024), 0, st, ctx->d_runs, qs, ctx->d_spec, ctx->d_tw,
+                       ctx->d_work, ctx->d_azep, ctx->d_seis);
+  if (qa.n > 0)
+    hipLaunchKernelGGL(k_fft_lds, g(qa), dim3(512), 0, st, ctx->d_runs, qa, ctx->d_spec,
+                       ctx->d_tw, ctx->d_work);
+  if (qb.n > 0)
+    hipLaunchKernelGGL(k_fft_post, g(qb), dim3(256), 0, st, ctx->d_runs, qb, ctx->d_tw,
+                       ctx->d_work, ctx->d_azep, ctx->d_seis);
+}
+
 int enqueue_seis_plain(ysx_ctx* ctx, int e0, int e1) {
-  for (int e = e0; e < e1; e++) {
-    Queue q = plain_q(ctx, W_SEIS, e);
-    hipLaunchKernelGGL(k_seis, dim3(grid_of(q)), dim3(1024), 0, ctx->stream, ctx->d_runs, q,
-                       ctx->d_spec, ctx->d_tw, ctx->d_work, ctx->d_azep, ctx->d_seis);
-  }
+  for (int e = e0; e < e1; e++)
+    enqueue_fft(ctx, plain_q(ctx, W_SEIS, e), plain_q(ctx, W_FFTA, e), plain_q(ctx, W_FFTB, e),
+                1 << 30);
   HIPCHK(hipGetLastError());
   return YSX_OK;
 }
@@ -1620,9 +1868,7 @@ int enqueue_batched(ysx_ctx* ctx) {
                      ctx->d_tb);
   hipLaunchKernelGGL(k_lsum, dim3(std::max(1, ql.n)), b64, 0, st, ctx->d_runs, ql, ctx->d_cf,
                      ctx->d_tb, ctx->d_eia, ctx->d_spec);
-  hipLaunchKernelGGL(k_seis, dim3(std::max(1, std::min(qs.n, 4 * ctx->ncu))), dim3(1024), 0, st,
-                     ctx->d_runs, qs, ctx->d_spec, ctx->d_tw, ctx->d_work, ctx->d_azep,
-                     ctx->d_seis);
+  enqueue_fft(ctx, qs, batched_q(ctx, W_FFTA, -1), batched_q(ctx, W_FFTB, -1), 1 << 30);
   HIPCHK(hipGetLastError());
   return YSX_OK;
 }

@@ -1,6 +1,7 @@
 // ysx_host.cpp -- host set-up: deck -> normalised model tables, run
 // parameters, receiver geometry and Legendre tables.
 #include "ysx_host.hpp"
+#include "ysx_model.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -9,49 +10,8 @@ namespace ysx {
 
 namespace {
 
-constexpr double R_NORM = 6.371e+6;    // src/module_model.f90:18
-constexpr double RHO_NORM = 5.515e+3;  // src/module_model.f90:19
-
-// Natural cubic spline second derivatives on one layer (spline_r with
-// yp1 = ypn = 1e40 and tridag_ser_r, src/module_spline.f90:40-73,
-// 426-449).  x, y, y2 point at the layer's first knot; n knots.
-bool natural_spline(const double* x, const double* y, double* y2, int n) {
-  std::vector<double> sub(n + 1, 0.0), dia(n + 1, 0.0), sup(n + 1, 0.0),
-      rhs(n + 1, 0.0), gam(n + 1, 0.0);
-  // 1-based working arrays
-  for (int i = 1; i <= n - 1; i++) sup[i] = x[i] - x[i - 1];
-  for (int i = 1; i <= n - 1; i++) rhs[i] = 6.0 * ((y[i] - y[i - 1]) / sup[i]);
-  for (int i = n - 1; i >= 2; i--) rhs[i] -= rhs[i - 1];
-  for (int i = 2; i <= n - 1; i++) {
-    sub[i] = sup[i - 1];
-    dia[i] = 2.0 * (sup[i] + sub[i]);
-  }
-  dia[1] = dia[n] = 1.0;
-  rhs[1] = rhs[n] = 0.0;
-  sup[1] = 0.0;
-  sub[n] = 0.0;
-  // forward elimination / back substitution
-  double bet = dia[1];
-  if (bet == 0.0) return false;
-  y2[0] = rhs[1] / bet;
-  for (int j = 2; j <= n; j++) {
-    gam[j] = sup[j - 1] / bet;
-    bet = dia[j] - sub[j] * gam[j];
-    if (bet == 0.0) return false;
-    y2[j - 1] = (rhs[j] - sub[j] * y2[j - 2]) / bet;
-  }
-  for (int j = n - 1; j >= 1; j--) y2[j - 1] -= gam[j + 1] * y2[j];
-  return true;
-}
-
-// splint_dis_r, src/module_spline.f90:262-283 (0-based knot index klo)
-double splint(const double* xa, const double* ya, const double* y2a, double x, int klo) {
-  double h = xa[klo + 1] - xa[klo];
-  if (h == 0.0) return ya[klo];
-  double a = (xa[klo + 1] - x) / h, b = (x - xa[klo]) / h;
-  return a * ya[klo] + b * ya[klo + 1] +
-         ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[klo + 1]) * (h * h) / 6.0;
-}
+constexpr double R_NORM = R_NORM_M;
+constexpr double RHO_NORM = RHO_NORM_M;
 
 }  // namespace
 
@@ -81,50 +41,14 @@ std::string build_model(const Norms& N, int nknot, int nic, int noc, int ians,
   M.tref = TWOPI_D / tref_period / N.fre_norm;
   M.tab.assign((size_t)F_COUNT * nknot, 0.0);
   auto col = [&](int f) { return M.tab.data() + (size_t)f * nknot; };
-  double *r = col(F_R), *rho = col(F_RHO), *A = col(F_ACON), *Cc = col(F_CCON),
-         *F = col(F_FCON), *L = col(F_LCON), *Nn = col(F_NCON), *kap = col(F_KAPPA),
-         *mu = col(F_MU), *qk = col(F_QKAPPA), *qm = col(F_QMU), *xlam = col(F_XLAM),
-         *xa2 = col(F_XA2), *grav = col(F_GRAV), *alpha = col(F_ALPHA), *beta = col(F_BETA);
-  for (int i = 0; i < nknot; i++) {
-    const double* d = deck + 9 * (size_t)i;
-    r[i] = d[0];
-    rho[i] = d[1];
-    Cc[i] = d[2];
-    L[i] = d[3];
-    qk[i] = d[4];
-    qm[i] = d[5];
-    A[i] = M.ifanis ? d[6] : d[2];
-    Nn[i] = M.ifanis ? d[7] : d[3];
-    F[i] = M.ifanis ? d[8] : 1.0;
-  }
-  M.ocean = (L[nknot - 1] == 0.0);
+  const double *r = col(F_R), *rho = col(F_RHO), *L0 = deck + 3;
+  M.ocean = (L0[9 * (size_t)(nknot - 1)] == 0.0);
   M.nsl = nknot;
   if (M.ocean)
     for (int i = nknot - 1; i >= 1; i--)
-      if (L[i - 1] > 0.0) { M.nsl = i; break; }
-  M.fic = (L[0] == 0.0);
-  std::vector<double> rat(nknot);
-  for (int i = 0; i < nknot; i++) {
-    r[i] /= R_NORM;
-    rho[i] /= RHO_NORM;
-    double vA = A[i] / N.vel_norm, vC = Cc[i] / N.vel_norm, vL = L[i] / N.vel_norm,
-           vN = Nn[i] / N.vel_norm;
-    A[i] = rho[i] * (vA * vA);
-    Cc[i] = rho[i] * (vC * vC);
-    L[i] = rho[i] * (vL * vL);
-    Nn[i] = rho[i] * (vN * vN);
-    F[i] = (A[i] - 2.0 * L[i]) * F[i];
-    kap[i] = (Cc[i] + 4.0 * A[i] - 4.0 * Nn[i] + 4.0 * F[i]) / 9.0;
-    mu[i] = (Cc[i] + A[i] + 6.0 * L[i] + 5.0 * Nn[i] - 2.0 * F[i]) / 15.0;
-    double lam = kap[i] - 2.0 * mu[i] / 3.0;
-    beta[i] = std::sqrt(mu[i] / rho[i]);
-    alpha[i] = std::sqrt((kap[i] + 1.3333333333 * mu[i]) / rho[i]);
-    if (qk[i] > 0.0) qk[i] = 1.0 / qk[i];
-    if (qm[i] > 0.0) qm[i] = 1.0 / qm[i];
-    rat[i] = 4.0 * mu[i] / (3.0 * (lam + 2.0 * mu[i]));
-    xlam[i] = ((1.0 - rat[i]) * qk[i] - 0.5 * rat[i] * qm[i]) / (1.0 - 1.5 * rat[i]);
-    xa2[i] = (1.0 - rat[i]) * qk[i] + rat[i] * qm[i];
-  }
+      if (L0[9 * (size_t)(i - 1)] > 0.0) { M.nsl = i; break; }
+  M.fic = (L0[0] == 0.0);
+  for (int i = 0; i < nknot; i++) knot_fields(deck, nknot, i, M.ifanis, N.vel_norm, M.tab.data());
   // layers split at duplicated radii (src/module_model.f90:294-308)
   M.layer_lo = {1};
   for (int i = 2; i <= nknot; i++)
@@ -134,13 +58,14 @@ std::string build_model(const Norms& N, int nknot, int nic, int noc, int ians,
     }
   M.layer_hi.push_back(nknot);
   M.nlayer = (int)M.layer_lo.size();
-  static const int fields[] = {F_RHO, F_ACON, F_CCON, F_FCON, F_LCON, F_NCON, F_KAPPA,
-                               F_MU, F_QKAPPA, F_QMU, F_XLAM, F_XA2};
+  std::vector<double> work(3 * ((size_t)nknot + 1));
   for (int k = 0; k < M.nlayer; k++) {
     int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
-    for (int f : fields)
-      if (!natural_spline(r + lo, col(f) + lo, col(f + 1) + lo, n))
+    for (int q = 0; q < 12; q++) {
+      int f = spline_field(q);
+      if (!natural_spline(r + lo, col(f) + lo, col(f + 1) + lo, n, work.data()))
         return "tridag_ser: error at code stage 2";
+    }
   }
   // gravity: 5-point Gauss-Legendre on the spline of r^2 rho
   // (gauslv, src/module_model.f90:356-371, 507-535)
@@ -148,28 +73,20 @@ std::string build_model(const Norms& N, int nknot, int nic, int noc, int ians,
   for (int i = 0; i < nknot; i++) intg[i] = r[i] * r[i] * rho[i];
   for (int k = 0; k < M.nlayer; k++) {
     int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
-    natural_spline(r + lo, intg.data() + lo, qint.data() + lo, n);
+    natural_spline(r + lo, intg.data() + lo, qint.data() + lo, n, work.data());
   }
-  static const double gw[2] = {.478628670499366, .236926885056189};
-  static const double gx[2] = {.538469310105683, .906179845938664};
+  double* grav = col(F_GRAV);
   grav[0] = 0.0;
   double fint = 0.0;
   for (int i = 0; i < nknot - 1; i++) {
-    double y1 = 0.5 * (r[i + 1] + r[i]), y2 = 0.5 * (r[i + 1] - r[i]);
-    double sum = .568888888888889 * splint(r, intg.data(), qint.data(), y1, i);
-    for (int g = 0; g < 2; g++) {
-      double t1 = gx[g] * y2;
-      sum += gw[g] * (splint(r, intg.data(), qint.data(), y1 + t1, i) +
-                      splint(r, intg.data(), qint.data(), y1 - t1, i));
-    }
-    fint += y2 * sum;
+    fint += grav_interval(r, intg.data(), qint.data(), i);
     grav[i + 1] = fint;
   }
   for (int i = 0; i < nknot; i++)
     if (r[i] > 0.0) grav[i] = (4.0 * grav[i]) / (r[i] * r[i]);
   for (int k = 0; k < M.nlayer; k++) {
     int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
-    natural_spline(r + lo, grav + lo, col(F_GRAV2) + lo, n);
+    natural_spline(r + lo, grav + lo, col(F_GRAV2) + lo, n, work.data());
   }
   return "";
 }

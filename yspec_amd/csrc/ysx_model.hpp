@@ -1,0 +1,125 @@
+// ysx_model.hpp -- the deck -> model-table transform, compiled for both the
+// host and the device from this one source: normalisation and moduli per
+// knot, natural cubic splines per layer, gravity by Gauss-Legendre
+// quadrature of the spline of r^2 rho (read_model numerics,
+// src/module_model.f90:253-378; spline_r / tridag_ser_r / splint_dis_r,
+// src/module_spline.f90:40-73, 262-283, 426-449; gauslv,
+// src/module_model.f90:507-535).  Contraction into FMAs is off so the host
+// and the device round every operation alike.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "ysx_common.hpp"
+
+namespace ysx {
+
+constexpr double R_NORM_M = 6.371e+6;    // src/module_model.f90:18
+constexpr double RHO_NORM_M = 5.515e+3;  // src/module_model.f90:19
+
+// Fields splined per layer, each followed by its second-derivative table
+__host__ __device__ inline int spline_field(int k) {
+  const int f[12] = {F_RHO, F_ACON, F_CCON, F_FCON, F_LCON, F_NCON,
+                     F_KAPPA, F_MU, F_QKAPPA, F_QMU, F_XLAM, F_XA2};
+  return f[k];
+}
+
+// Knot i (0-based) of the raw deck (9 columns per knot; ians = 0 decks use
+// the first 6) into the table tab[field * nknot + i]; gravity comes later.
+__host__ __device__ inline void knot_fields(const double* deck, int nknot, int i, int ifanis,
+                                            double vel_norm, double* tab) {
+#pragma clang fp contract(off)
+  const double* d = deck + 9 * (size_t)i;
+  auto at = [&](int f) -> double& { return tab[(size_t)f * nknot + i]; };
+  double r = d[0] / R_NORM_M, rho = d[1] / RHO_NORM_M;
+  double A = ifanis ? d[6] : d[2], Cc = d[2], L = d[3], Nn = ifanis ? d[7] : d[3];
+  double F = ifanis ? d[8] : 1.0, qk = d[4], qm = d[5];
+  double vA = A / vel_norm, vC = Cc / vel_norm, vL = L / vel_norm, vN = Nn / vel_norm;
+  A = rho * (vA * vA);
+  Cc = rho * (vC * vC);
+  L = rho * (vL * vL);
+  Nn = rho * (vN * vN);
+  F = (A - 2.0 * L) * F;
+  double kap = (Cc + 4.0 * A - 4.0 * Nn + 4.0 * F) / 9.0;
+  double mu = (Cc + A + 6.0 * L + 5.0 * Nn - 2.0 * F) / 15.0;
+  double lam = kap - 2.0 * mu / 3.0;
+  if (qk > 0.0) qk = 1.0 / qk;
+  if (qm > 0.0) qm = 1.0 / qm;
+  double rat = 4.0 * mu / (3.0 * (lam + 2.0 * mu));
+  at(F_R) = r;
+  at(F_RHO) = rho;
+  at(F_ACON) = A;
+  at(F_CCON) = Cc;
+  at(F_LCON) = L;
+  at(F_NCON) = Nn;
+  at(F_FCON) = F;
+  at(F_KAPPA) = kap;
+  at(F_MU) = mu;
+  at(F_QKAPPA) = qk;
+  at(F_QMU) = qm;
+  at(F_XLAM) = ((1.0 - rat) * qk - 0.5 * rat * qm) / (1.0 - 1.5 * rat);
+  at(F_XA2) = (1.0 - rat) * qk + rat * qm;
+  at(F_BETA) = sqrt(mu / rho);
+  at(F_ALPHA) = sqrt((kap + 1.3333333333 * mu) / rho);
+}
+
+// Natural cubic spline second derivatives y2[0..n) of y over x (one
+// layer); work holds 3 (n + 1) doubles.  Returns false on a zero pivot.
+__host__ __device__ inline bool natural_spline(const double* x, const double* y, double* y2, int n,
+                                               double* work) {
+#pragma clang fp contract(off)
+  double* sup = work;            // [n + 1]
+  double* rhs = work + (n + 1);  // [n + 1]
+  double* gam = work + 2 * (n + 1);
+  for (int i = 0; i <= n; i++) sup[i] = rhs[i] = gam[i] = 0.0;
+  for (int i = 1; i <= n - 1; i++) sup[i] = x[i] - x[i - 1];
+  for (int i = 1; i <= n - 1; i++) rhs[i] = 6.0 * ((y[i] - y[i - 1]) / sup[i]);
+  for (int i = n - 1; i >= 2; i--) rhs[i] -= rhs[i - 1];
+  rhs[1] = rhs[n] = 0.0;
+  // tridiagonal: sub[i] = sup[i-1], dia[i] = 2 (sup[i] + sub[i]) inside,
+  // 1 at both ends; super-diagonal sup[i] inside, 0 in the first row
+  auto sub = [&](int i) { return (i >= 2 && i <= n - 1) ? sup[i - 1] : 0.0; };
+  auto dia = [&](int i) { return (i >= 2 && i <= n - 1) ? 2.0 * (sup[i] + sup[i - 1]) : 1.0; };
+  auto spr = [&](int i) { return (i >= 2) ? sup[i] : 0.0; };
+  double bet = dia(1);
+  if (bet == 0.0) return false;
+  y2[0] = rhs[1] / bet;
+  for (int j = 2; j <= n; j++) {
+    gam[j] = spr(j - 1) / bet;
+    bet = dia(j) - sub(j) * gam[j];
+    if (bet == 0.0) return false;
+    y2[j - 1] = (rhs[j] - sub(j) * y2[j - 2]) / bet;
+  }
+  for (int j = n - 1; j >= 1; j--) y2[j - 1] -= gam[j + 1] * y2[j];
+  return true;
+}
+
+// splint_dis_r on interval [xa[klo], xa[klo+1]] (0-based klo)
+__host__ __device__ inline double splint(const double* xa, const double* ya, const double* y2a,
+                                         double x, int klo) {
+#pragma clang fp contract(off)
+  double h = xa[klo + 1] - xa[klo];
+  if (h == 0.0) return ya[klo];
+  double a = (xa[klo + 1] - x) / h, b = (x - xa[klo]) / h;
+  return a * ya[klo] + b * ya[klo + 1] +
+         ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[klo + 1]) * (h * h) / 6.0;
+}
+
+// gauslv's 5-point Gauss-Legendre integral of the spline of r^2 rho over
+// knot interval i (0-based), without the accumulation
+__host__ __device__ inline double grav_interval(const double* r, const double* intg,
+                                                const double* qint, int i) {
+#pragma clang fp contract(off)
+  const double gw0 = .478628670499366, gw1 = .236926885056189;
+  const double gx0 = .538469310105683, gx1 = .906179845938664;
+  double y1 = 0.5 * (r[i + 1] + r[i]), y2 = 0.5 * (r[i + 1] - r[i]);
+  double sum = .568888888888889 * splint(r, intg, qint, y1, i);
+  double t1 = gx0 * y2;
+  sum += gw0 * (splint(r, intg, qint, y1 + t1, i) + splint(r, intg, qint, y1 - t1, i));
+  t1 = gx1 * y2;
+  sum += gw1 * (splint(r, intg, qint, y1 + t1, i) + splint(r, intg, qint, y1 - t1, i));
+  return y2 * sum;
+}
+
+}  // namespace ysx
