@@ -1002,6 +1002,30 @@ __global__ void __launch_bounds__(64) k_lsum(const RunK* __restrict__ runs, Queu
                       spec + runs[it.x].spec_off));
 }
 
+// -------------------------------------------------------------- k_legendre
+// X_lm, dX_lm/dtheta, X_lm/sin(theta), |m| <= 2, of event e for degree l =
+// blockIdx.x * 64 + lane and receiver blockIdx.y, into the receiver-sum
+// operand tb[(X * 3 + |m|) * lpad + l][nrpad] (legendre, src/module_function.f90:
+// 6-132, through the host's own arithmetic in ysx_model.hpp).  th holds
+// theta, cos theta, sin theta per receiver.
+__global__ void __launch_bounds__(64) k_legendre(const RunK* __restrict__ runs, int e,
+                                                 const double* __restrict__ th,
+                                                 double* __restrict__ tb_all) {
+  const RunK& R = runs[e];
+  int l = blockIdx.x * 64 + (int)threadIdx.x, k = blockIdx.y;
+  if (l > R.lmax || k >= R.nr) return;
+  const double* t = th + 3 * (size_t)(R.roff + k);
+  int m = l < 2 ? l : 2;
+  double x[3], xp[3], xc[3];
+  legendre_cs(t[0], t[1], t[2], l, m, x, xp, xc);
+  double* tb = tb_all + R.tb_off;
+  for (int a = 0; a <= m; a++) {
+    tb[((size_t)(0 * 3 + a) * R.lpad + l) * R.nrpad + k] = x[a];
+    tb[((size_t)(1 * 3 + a) * R.lpad + l) * R.nrpad + k] = xp[a];
+    tb[((size_t)(2 * 3 + a) * R.lpad + l) * R.nrpad + k] = xc[a];
+  }
+}
+
 // ----------------------------------------------------------------- k_model
 // The deck -> table transform on the device (one block): moduli per knot,
 // natural splines per layer, gravity quadrature with the reference's
@@ -1624,28 +1648,38 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipMalloc(&ctx->d_cf, cfo * sizeof(cd)));
   HIPCHK(hipMemset(ctx->d_cf, 0, cfo * sizeof(cd)));
   {
-    std::vector<double> tb(tbo, 0.0), az(ro);
+    // receiver geometry from the host (O(nr)); the Legendre tables from the
+    // device, written straight into the GEMM operand
+    std::vector<double> az(ro), th(3 * (size_t)ro);
     std::vector<std::complex<double>> eia(ro);
     for (int e = 0; e < nev; e++) {
       const HostRun& R = ctx->R[e];
       const RunK& k = ctx->rk[e];
-      const std::vector<double>* tabs[3] = {&R.xa, &R.xpa, &R.xca};
-      int L1 = R.lmax + 1;
-      for (int X = 0; X < 3; X++)
-        for (int a = 0; a < 3; a++)
-          for (int l = 0; l < L1; l++)
-            for (int kk = 0; kk < R.nr; kk++)
-              tb[k.tb_off + ((size_t)(X * 3 + a) * k.lpad + l) * k.nrpad + kk] =
-                  (*tabs[X])[(size_t)l + (size_t)L1 * (a + 3 * (size_t)kk)];
       for (int kk = 0; kk < R.nr; kk++) {
         az[k.roff + kk] = R.azep[kk];
         eia[k.roff + kk] = R.eia[kk];
+        th[3 * (k.roff + kk)] = R.delta[kk];
+        th[3 * (k.roff + kk) + 1] = std::cos(R.delta[kk]);
+        th[3 * (k.roff + kk) + 2] = std::sin(R.delta[kk]);
       }
     }
+    double* d_th = nullptr;
     HIPCHK(hipMalloc(&ctx->d_tb, tbo * sizeof(double)));
-    HIPCHK(hipMemcpy(ctx->d_tb, tb.data(), tbo * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ctx->d_tb, 0, tbo * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->d_azep, az.data(), ro * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_eia, eia.data(), ro * sizeof(cd), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&d_th, th.size() * sizeof(double)));
+    hipError_t et = hipMemcpy(d_th, th.data(), th.size() * sizeof(double), hipMemcpyHostToDevice);
+    for (int e = 0; e < nev && et == hipSuccess; e++) {
+      const RunK& k = ctx->rk[e];
+      hipLaunchKernelGGL(k_legendre, dim3((k.lmax + 64) / 64, k.nr), dim3(64), 0, ctx->stream,
+                         ctx->d_runs, e, d_th, ctx->d_tb);
+      et = hipGetLastError();
+    }
+    if (et == hipSuccess) et = hipStreamSynchronize(ctx->stream);
+    dfree(d_th);
+    if (et != hipSuccess)
+      return set_err(ctx, YSX_ERR_HIP, std::string("Legendre tables: ") + hipGetErrorString(et));
   }
   HIPCHK(hipMalloc(&ctx->d_tw, (size_t)ntmax * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_work, sio * sizeof(cd)));

@@ -91,58 +91,10 @@ std::string build_model(const Norms& N, int nknot, int nic, int noc, int ians,
   return "";
 }
 
-// legendre, src/module_function.f90:6-132.  X_lm, dX_lm/dtheta and
-// X_lm/sin(theta) for orders 0..m; the reference takes sfl3, f1, f2 and
-// the second normalising square root in single precision.
+// legendre, src/module_function.f90:6-132 (the arithmetic is shared with
+// the device in ysx_model.hpp; cos and sin of theta are taken here)
 void legendre_m2(double theta, int l, int m, double* x, double* xp, double* xc) {
-  double ct = std::cos(theta), st = std::sin(theta);
-  double fct = std::sqrt((double)(float)(2 * l + 1) / FOURPI_D);
-  double sfl3 = (double)std::sqrt((float)(l * (l + 1)));
-  double compar = (double)(float)(2 * l + 1) / FOURPI_D;
-  double small = 1.0e-16 * compar;
-  for (int i = 0; i <= m; i++) x[i] = xp[i] = xc[i] = 0.0;
-  if (l <= 1 || std::fabs(theta) <= 1.0e-5) {
-    x[0] = fct;
-    if (l == 0) return;
-    x[0] = ct * fct;
-    x[1] = -st * fct / sfl3;
-    xp[0] = -st * fct;
-    xp[1] = -0.5 * ct * fct * sfl3;
-    xc[1] = (std::fabs(theta) < 1.0e-5) ? xp[1] : x[1] / st;
-    return;
-  }
-  double p0 = 1.0, p1 = ct;  // Legendre recurrence P_{i-2}, P_{i-1}
-  for (int i = 2; i <= l; i++) {
-    double p2 = ((double)(float)(2 * i - 1) * ct * p1 - (double)(float)(i - 1) * p0) /
-                (double)(float)i;
-    p0 = p1;
-    p1 = p2;
-  }
-  double cot = ct / st, cosec = 1.0 / st;
-  double x0 = p1 * fct;
-  double x1 = (double)(float)l * (p0 - ct * p1) * fct / st;
-  x[0] = x0;
-  xp[0] = -x1;
-  double xp1 = (double)(float)(l * (l + 1)) * x0 - cot * x1;
-  x[1] = -x1 / sfl3;
-  xc[1] = x[1] * cosec;
-  xp[1] = -xp1 / sfl3;
-  double sum = x0 * x0 + 2.0 * x[1] * x[1];
-  if (sum - compar > small) return;
-  double xa = x0, xb = -x1 / (double)std::sqrt((float)(l * (l + 1)));
-  for (int i = 3; i <= m + 1; i++) {
-    double f1 = (double)std::sqrt((float)(l + i - 1) * (float)(l - i + 2));
-    double f2 = (double)std::sqrt((float)(l + i - 2) * (float)(l - i + 3));
-    double xm = i - 1;
-    double x3 = -(2.0 * cot * (xm - 1.0) * xb + f2 * xa) / f1;
-    sum += 2.0 * x3 * x3;
-    if (sum - compar > small && i != l + 1) return;
-    x[i - 1] = x3;
-    xc[i - 1] = x3 * cosec;
-    xa = xb;
-    xp[i - 1] = -(f1 * xb + xm * cot * x3);
-    xb = x3;
-  }
+  legendre_cs(theta, std::cos(theta), std::sin(theta), l, m, x, xp, xc);
 }
 
 // delaz, src/module_function.f90:317-390 (geocentric latitude, the
@@ -196,14 +148,11 @@ std::string build_run(const Norms& N, const HostModel& M, const int* ip, const d
   if (R.grav < 0 || R.grav > 2) return "gravitation switch must be 0, 1 or 2";
   double f1 = fp[0], f2 = fp[1], tout = fp[2], dt = fp[3];
   double sdep = fp[8], lats = fp[9], lons = fp[10], rdep = fp[17];
-  int nr = R.nr, L1 = R.lmax + 1;
+  int nr = R.nr;
   R.delta.resize(nr);
   R.azep.resize(nr);
   R.azst.resize(nr);
   R.eia.resize(nr);
-  R.xa.assign((size_t)L1 * 3 * nr, 0.0);
-  R.xpa.assign((size_t)L1 * 3 * nr, 0.0);
-  R.xca.assign((size_t)L1 * 3 * nr, 0.0);
   if (lats == 90.0) lats = 89.98;
   for (int k = 0; k < nr; k++) {
     double dl, ae, as;
@@ -212,18 +161,9 @@ std::string build_run(const Norms& N, const HostModel& M, const int* ip, const d
     R.azep[k] = TWOPI_D - ae;
     R.azst[k] = PI_D - as;
     R.eia[k] = std::exp(std::complex<double>(0.0, R.azst[k]));
-    for (int l = 0; l <= R.lmax; l++) {
-      int m = l < 2 ? l : 2;
-      double x[3], xp[3], xc[3];
-      legendre_m2(R.delta[k], l, m, x, xp, xc);
-      for (int q = 0; q <= m; q++) {
-        size_t idx = (size_t)l + (size_t)L1 * (q + 3 * (size_t)k);
-        R.xa[idx] = x[q];
-        R.xpa[idx] = xp[q];
-        R.xca[idx] = xc[q];
-      }
-    }
   }
+  // the (lmax+1) x 3 x nr Legendre tables are built on the device
+  // (legendre_cs, k_legendre) straight into the receiver-sum operand
   // normalisation (src/yspec.f90:179-189)
   double rtop = M.at(F_R, M.nknot);
   f1 = f1 * N.t_norm / 1000.0;

@@ -33,7 +33,6 @@ struct HostRun {
   double rs = 0, rr = 0, dt = 0, tout = 0, ep = 0, df = 0, wmin = 0, dw = 0;
   double f11 = 0, f12 = 0, f21 = 0, f22 = 0, mm[6] = {0};
   std::vector<double> delta, azep, azst;
-  std::vector<double> xa, xpa, xca;  // (lmax+1, 3, nr) column-major
   std::vector<std::complex<double>> eia;
 };
 

@@ -1,11 +1,12 @@
-// ysx_model.hpp -- the deck -> model-table transform, compiled for both the
-// host and the device from this one source: normalisation and moduli per
-// knot, natural cubic splines per layer, gravity by Gauss-Legendre
-// quadrature of the spline of r^2 rho (read_model numerics,
+// ysx_model.hpp -- numerics compiled for both the host and the device from
+// this one source: the deck -> model-table transform (normalisation and
+// moduli per knot, natural cubic splines per layer, gravity by
+// Gauss-Legendre quadrature of the spline of r^2 rho: read_model numerics,
 // src/module_model.f90:253-378; spline_r / tridag_ser_r / splint_dis_r,
 // src/module_spline.f90:40-73, 262-283, 426-449; gauslv,
-// src/module_model.f90:507-535).  Contraction into FMAs is off so the host
-// and the device round every operation alike.
+// src/module_model.f90:507-535) and the receiver Legendre functions
+// (src/module_function.f90:6-132).  Contraction into FMAs is off so the
+// host and the device round every operation alike.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -120,6 +121,62 @@ __host__ __device__ inline double grav_interval(const double* r, const double* i
   t1 = gx1 * y2;
   sum += gw1 * (splint(r, intg, qint, y1 + t1, i) + splint(r, intg, qint, y1 - t1, i));
   return y2 * sum;
+}
+
+// legendre, src/module_function.f90:6-132.  X_lm, dX_lm/dtheta and
+// X_lm/sin(theta) for orders 0..m at colatitude theta (ct = cos theta,
+// st = sin theta); the reference takes sfl3, f1, f2 and the second
+// normalising square root in single precision.
+__host__ __device__ inline void legendre_cs(double theta, double ct, double st, int l, int m,
+                                            double* x, double* xp, double* xc) {
+#pragma clang fp contract(off)
+  double fct = sqrt((double)(float)(2 * l + 1) / FOURPI_D);
+  double sfl3 = (double)sqrtf((float)(l * (l + 1)));
+  double compar = (double)(float)(2 * l + 1) / FOURPI_D;
+  double small = 1.0e-16 * compar;
+  for (int i = 0; i <= m; i++) x[i] = xp[i] = xc[i] = 0.0;
+  if (l <= 1 || fabs(theta) <= 1.0e-5) {
+    x[0] = fct;
+    if (l == 0) return;
+    x[0] = ct * fct;
+    x[1] = -st * fct / sfl3;
+    xp[0] = -st * fct;
+    xp[1] = -0.5 * ct * fct * sfl3;
+    xc[1] = (fabs(theta) < 1.0e-5) ? xp[1] : x[1] / st;
+    return;
+  }
+  double p0 = 1.0, p1 = ct;  // Legendre recurrence P_{i-2}, P_{i-1}
+  for (int i = 2; i <= l; i++) {
+    double p2 = ((double)(float)(2 * i - 1) * ct * p1 - (double)(float)(i - 1) * p0) /
+                (double)(float)i;
+    p0 = p1;
+    p1 = p2;
+  }
+  double cot = ct / st, cosec = 1.0 / st;
+  double x0 = p1 * fct;
+  double x1 = (double)(float)l * (p0 - ct * p1) * fct / st;
+  x[0] = x0;
+  xp[0] = -x1;
+  double xp1 = (double)(float)(l * (l + 1)) * x0 - cot * x1;
+  x[1] = -x1 / sfl3;
+  xc[1] = x[1] * cosec;
+  xp[1] = -xp1 / sfl3;
+  double sum = x0 * x0 + 2.0 * x[1] * x[1];
+  if (sum - compar > small) return;
+  double xa = x0, xb = -x1 / (double)sqrtf((float)(l * (l + 1)));
+  for (int i = 3; i <= m + 1; i++) {
+    double f1 = (double)sqrtf((float)(l + i - 1) * (float)(l - i + 2));
+    double f2 = (double)sqrtf((float)(l + i - 2) * (float)(l - i + 3));
+    double xm = i - 1;
+    double x3 = -(2.0 * cot * (xm - 1.0) * xb + f2 * xa) / f1;
+    sum += 2.0 * x3 * x3;
+    if (sum - compar > small && i != l + 1) return;
+    x[i - 1] = x3;
+    xc[i - 1] = x3 * cosec;
+    xa = xb;
+    xp[i - 1] = -(f1 * xb + xm * cot * x3);
+    xb = x3;
+  }
 }
 
 }  // namespace ysx
