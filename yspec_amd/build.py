@@ -12,6 +12,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -21,7 +22,13 @@ FDIR = os.path.join(HERE, "fortran")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("YSX_FC", "/opt/rocm/bin/amdflang")
 
-SOURCES = ["ysx_gpu.hip", "ysx_host.cpp"]
+# (source, object, defines): the march kernels dominate compile time, so
+# ysx_k_march.hip is compiled once per kernel and work-distribution mode and
+# all objects compile in parallel.
+SOURCES = [("ysx_k_march.hip", f"ysx_k_march_{k}{p}.o", [f"-DYSX_MARCH={k}", f"-DYSX_PERSIST={p}"])
+           for k in range(4) for p in range(2)]
+SOURCES += [("ysx_k_misc.hip", "ysx_k_misc.o", []), ("ysx_api.hip", "ysx_api.o", []),
+            ("ysx_host.cpp", "ysx_host.o", [])]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
@@ -32,19 +39,35 @@ def _run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+def _compile(entry, extra):
+    src, obj, defs = entry
+    obj = os.path.join(HERE, "_build", obj)
+    inc = ["-I", os.path.join(ROOT, "include"), *defs]
+    if src.endswith(".cpp"):
+        cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", *inc, "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC, *HIP_FLAGS, *(extra or []), *inc, "-c", os.path.join(CSRC, src), "-o", obj]
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return obj, r
+
+
 def build_lib(extra=None):
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as ex:
+        results = list(ex.map(lambda s: _compile(s, extra), SOURCES))
     objs = []
-    bdir = os.path.join(HERE, "_build")
-    os.makedirs(bdir, exist_ok=True)
-    for src in SOURCES:
-        obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *HIP_FLAGS, *(extra or []), "-I", os.path.join(ROOT, "include"),
-               "-c", os.path.join(CSRC, src), "-o", obj]
-        if src.endswith(".cpp"):
-            cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"),
-                   "-c", os.path.join(CSRC, src), "-o", obj]
-        _run(cmd)
+    failed = []
+    for obj, r in results:
+        if r.stdout:
+            print(r.stdout, end="", flush=True)
+        if r.stderr:
+            print(r.stderr, end="", file=sys.stderr, flush=True)
+        if r.returncode != 0:
+            failed.append(os.path.basename(obj))
         objs.append(obj)
+    if failed:
+        raise RuntimeError("compilation failed: " + ", ".join(failed))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -1,0 +1,849 @@
+// ysx_api.hip -- the C-ABI of libysx.so (include/ysx.h): contexts, model
+// and run set-up, work lists, the plain and batched forwards and their
+// captured HIP graph, result copies.  Kernels are launched through the
+// launch functions of ysx_kern.hpp.
+//
+// Per forward the GPU work is:
+//   k_source      source vectors of every (l, w) pair
+//   k_start       start levels and starting solutions
+//   k_tor_march   toroidal two-sided shooting, lockstep waves of 64 l at one w
+//   k_minor_top   spheroidal minors from the receiver side down to the source
+//   k_minor_march spheroidal minor vectors from the start level up to the
+//                 source, secular determinant, b-vector starts
+//   k_bvec_march  b-vector from the source to the receiver, 4 columns
+//   k_rad         radial (l = 0) solves on the reference's grid (2nd stream)
+//   k_sph_var, k_rad_ng   per-lane solvers of the gravity/inner-core variants
+//   k_assemble    output-type scaling and the uu/vv/ww coefficients
+//   k_lsum        sum over l and m against the Legendre tables: FP64 MFMA
+//                 GEMMs with the azimuthal phases fused into the epilogue
+//   k_fft_lds, k_fft_post (k_seis for long series)
+//                 filtered inverse FFT, exp(eps t), units, rotation
+//
+// A context holds one or more events (source + receivers + frequency grid)
+// on one model.  Every kernel reads its event's parameters from a device
+// array of RunK and takes its work from a list of items (event, w, l-block):
+// one event at a time with one block per item ("plain"), or all events at
+// once from cost-sorted lists drained by persistent blocks through an atomic
+// queue, the whole forward replayed from a captured HIP graph ("batched").
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/ysx.h"
+#include "ysx_host.hpp"
+#include "ysx_kern.hpp"
+
+using namespace ysx;
+
+// ====================================================================== API
+namespace {
+
+enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_VAR, W_FFTA, W_FFTB, W_N };
+// queue counters of the batched launches (three marches share the W_TOR list)
+enum QueueSlot { Q_TOR = 0, Q_TOP, Q_MINOR, Q_BVEC, Q_RAD, Q_N };
+
+struct WorkList {
+  std::vector<int4> plain;    // event-major, each event in its plain order
+  std::vector<int> start;     // [nev + 1] offsets of each event in plain
+  std::vector<int4> batched;  // all events, cost-sorted
+  int4* d_plain = nullptr;
+  int4* d_batched = nullptr;
+};
+
+}  // namespace
+
+struct ysx_ctx {
+  int device = 0, ncu = 256;
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  std::string err;
+  Norms N;
+  HostModel M;
+  std::vector<HostRun> R;  // one per event
+  std::vector<RunK> rk;
+  bool have_model = false, have_run = false, have_spec = false, have_seis = false;
+  size_t lds_bytes = 0, lds_max = 0;
+  long long npair = 0;  // all events
+  // device buffers
+  double* d_tab = nullptr;
+  RunK* d_runs = nullptr;
+  cd* d_rec = nullptr;
+  int* d_starts = nullptr;
+  int* d_cnt = nullptr;
+  cd *d_bst = nullptr, *d_mfin = nullptr, *d_m2 = nullptr, *d_fac = nullptr;
+  int *d_tor_i1 = nullptr, *d_sph_i1 = nullptr;
+  cd *d_tor_y0 = nullptr, *d_sph_m0 = nullptr;
+  int* d_sc3 = nullptr;
+  double* d_azep = nullptr;
+  cd* d_eia = nullptr;
+  cd* d_spec = nullptr;
+  cd* d_cf = nullptr;
+  double* d_tb = nullptr;
+  cd* d_tw = nullptr;
+  cd* d_work = nullptr;
+  double* d_seis = nullptr;
+  int* d_queue = nullptr;  // Q_N counters of the batched queues
+  WorkList W[W_N];
+  // batched mode: persistent blocks or one block per item, items taken per
+  // queue access, persistent blocks per CU, and the captured forward
+  int persist = 0, chunk = 1, blocks_per_cu = 4;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev[16] = {};
+  double ms[8] = {0};
+};
+
+namespace {
+
+int set_err(ysx_ctx* c, int code, const std::string& m) {
+  c->err = m;
+  return code;
+}
+
+#define HIPCHK(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return set_err(ctx, YSX_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void drop_graph(ysx_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+void free_run(ysx_ctx* c) {
+  drop_graph(c);
+  dfree(c->d_runs); dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_cnt); dfree(c->d_bst);
+  dfree(c->d_mfin); dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0);
+  dfree(c->d_sph_m0); dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_azep); dfree(c->d_eia);
+  dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
+  dfree(c->d_seis); dfree(c->d_queue);
+  for (auto& w : c->W) {
+    dfree(w.d_plain);
+    dfree(w.d_batched);
+    w = WorkList();
+  }
+  c->R.clear();
+  c->rk.clear();
+  c->npair = 0;
+  c->have_run = c->have_spec = c->have_seis = false;
+}
+
+// Work lists of every kernel kind.  Plain order per event is the order of
+// one launch per event (largest w first for the marches).  The batched
+// lists hold every event's items, sorted by an estimate of their cost:
+// steps grow with Re w; a wave of degrees starting at l0 integrates from
+// about the turning radius (l0 + 1/2) v / w (v ~ 1 in normalised units) up
+// to the top point; the b-vector runs between source and receiver.
+void build_work(ysx_ctx* c) {
+  int nev = (int)c->rk.size();
+  std::vector<std::vector<double>> cost(W_N);
+  for (int k = 0; k < W_N; k++) c->W[k].start.assign(1, 0);
+  for (int e = 0; e < nev; e++) {
+    const RunK& R = c->rk[e];
+    int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64, nlb16 = (nl + 15) / 16;
+    auto rew = [&](int j) { return R.wmin + (double)(j - 1) * R.dw; };
+    double rtop = std::max(R.rs, R.rr);
+    for (int wv = 0; wv < R.nw * nlb; wv++) {
+      int j = R.nw - wv / nlb, lb = wv % nlb;
+      c->W[W_TOR].plain.push_back(make_int4(e, j, lb, 0));
+      double w = rew(j), rt = std::min(1.0, (R.lmin + 64.0 * lb + 0.5) / std::max(w, 1e-30));
+      cost[W_TOR].push_back(w * (rtop - std::min(rt, rtop) + 0.05));
+    }
+    for (int wv = 0; wv < R.nw * nlb16; wv++) {
+      int j = R.nw - wv / nlb16, lb = wv % nlb16;
+      c->W[W_BVEC].plain.push_back(make_int4(e, j, lb, 0));
+      cost[W_BVEC].push_back(rew(j) * (std::fabs(R.rr - R.rs) + 1e-3));
+    }
+    if (R.lmin == 0)
+      for (int jb = 0; jb < (R.nw + 63) / 64; jb++) {
+        c->W[W_RAD].plain.push_back(make_int4(e, jb, 0, 0));
+        cost[W_RAD].push_back(rew(std::min(R.nw, 64 * jb + 64)));
+      }
+    if (R.var)
+      for (int pb = 0; pb < (R.npair + 63) / 64; pb++) {
+        c->W[W_VAR].plain.push_back(make_int4(e, pb, 0, 0));
+        // the block's highest w sets its step count
+        cost[W_VAR].push_back(rew(std::min(R.nw, (64 * pb + 63) % R.nw + 1)) * (rtop + 0.05));
+      }
+    int ntile = (R.jpad / 16) * (R.nrpad / 16);
+    for (int t = 0; t < ntile; t++) {
+      c->W[W_LSUM].plain.push_back(make_int4(e, t, 0, 0));
+      cost[W_LSUM].push_back((double)R.lpad);
+    }
+    int bk = std::min(R.nt, FFT_B);
+    if (R.nt / bk <= FFT_G) {  // two-pass transform
+      for (int k = 0; k < R.nr; k++)
+        for (int cc = 0; cc < 3; cc++)
+          for (int blk = 0; blk < R.nt / bk; blk++) {
+            c->W[W_FFTA].plain.push_back(make_int4(e, k, (cc << 16) | blk, 0));
+            cost[W_FFTA].push_back(1.0);
+          }
+      for (int k = 0; k < R.nr; k++)
+        for (int ib = 0; ib < (bk + 255) / 256; ib++) {
+          c->W[W_FFTB].plain.push_back(make_int4(e, k, ib, 0));
+          cost[W_FFTB].push_back(1.0);
+        }
+    } else {  // long series: radix-2 passes in global memory
+      for (int k = 0; k < R.nr; k++) {
+        c->W[W_SEIS].plain.push_back(make_int4(e, k, 0, 0));
+        cost[W_SEIS].push_back((double)R.nt);
+      }
+    }
+    for (int k = 0; k < W_N; k++) c->W[k].start.push_back((int)c->W[k].plain.size());
+  }
+  for (int k = 0; k < W_N; k++) {
+    WorkList& w = c->W[k];
+    std::vector<int> ord(w.plain.size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int b) { return cost[k][a] > cost[k][b]; });
+    w.batched.resize(ord.size());
+    for (size_t i = 0; i < ord.size(); i++) w.batched[i] = w.plain[ord[i]];
+  }
+}
+
+}  // namespace
+
+extern "C" int ysx_create(ysx_ctx** out, int device) {
+  if (!out) return YSX_ERR_ARG;
+  ysx_ctx* ctx = new ysx_ctx();
+  *out = ctx;
+  ctx->device = device;
+  ctx->N = make_norms();
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  ctx->lds_max = prop.sharedMemPerBlock;
+  ctx->ncu = prop.multiProcessorCount;
+  return YSX_OK;
+}
+
+extern "C" void ysx_destroy(ysx_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  free_run(ctx);
+  dfree(ctx->d_tab);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  delete ctx;
+}
+
+extern "C" const char* ysx_last_error(const ysx_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians,
+                             double tref_period, const double* deck) {
+  if (!ctx || !deck) return YSX_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  free_run(ctx);
+  dfree(ctx->d_tab);
+  ctx->have_model = false;
+  std::string e = build_model(ctx->N, nknot, nic, noc, ians, tref_period, deck, ctx->M);
+  if (!e.empty()) return set_err(ctx, YSX_ERR_INPUT, e);
+  ctx->lds_bytes = sizeof(double) * F_COUNT * (size_t)nknot;
+  if (ctx->lds_bytes > ctx->lds_max)
+    return set_err(ctx, YSX_ERR_UNSUPPORTED, "deck has too many knots for the LDS-resident table");
+  HIPCHK(hipMalloc(&ctx->d_tab, ctx->lds_bytes));
+  // the table the kernels read is built on the device from the raw deck;
+  // the host copy (same functions) serves the run set-up
+  {
+    const HostModel& M = ctx->M;
+    int nl = M.nlayer;
+    std::vector<int> lo(nl), n(nl);
+    for (int k = 0; k < nl; k++) {
+      lo[k] = M.layer_lo[k] - 1;
+      n[k] = M.layer_hi[k] - M.layer_lo[k] + 1;
+    }
+    size_t ws = 3 * ((size_t)nknot + 1), nwork = ws * 13 * (size_t)nl;
+    double *d_deck = nullptr, *d_work = nullptr;
+    int* d_lay = nullptr;
+    auto cleanup = [&]() {
+      dfree(d_deck);
+      dfree(d_work);
+      dfree(d_lay);
+    };
+    hipError_t e1 = hipMalloc(&d_deck, 9 * (size_t)nknot * sizeof(double));
+    if (e1 == hipSuccess) e1 = hipMalloc(&d_work, (nwork + 3 * (size_t)nknot) * sizeof(double));
+    if (e1 == hipSuccess) e1 = hipMalloc(&d_lay, 2 * (size_t)nl * sizeof(int));
+    if (e1 == hipSuccess)
+      e1 = hipMemcpy(d_deck, deck, 9 * (size_t)nknot * sizeof(double), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) e1 = hipMemcpy(d_lay, lo.data(), nl * sizeof(int), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess)
+      e1 = hipMemcpy(d_lay + nl, n.data(), nl * sizeof(int), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) e1 = hipMemset(ctx->d_tab, 0, ctx->lds_bytes);
+    if (e1 == hipSuccess) {
+      launch_model(dim3(1), dim3(256), 0, ctx->stream, d_deck, nknot, M.ifanis,
+                         ctx->N.vel_norm, d_lay, d_lay + nl, nl, ctx->d_tab, d_work,
+                         d_work + nwork, d_work + nwork + nknot, d_work + nwork + 2 * (size_t)nknot);
+      e1 = hipGetLastError();
+    }
+    if (e1 == hipSuccess) e1 = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e1 != hipSuccess)
+      return set_err(ctx, YSX_ERR_HIP, std::string("model transform: ") + hipGetErrorString(e1));
+  }
+  ctx->have_model = true;
+  return YSX_OK;
+}
+
+extern "C" int ysx_model_table(const ysx_ctx* ctx, double* out) {
+  if (!ctx || !out || !ctx->have_model) return YSX_ERR_ARG;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipMemcpy(out, ctx->d_tab, ctx->lds_bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? YSX_OK : YSX_ERR_HIP;
+}
+
+extern "C" int ysx_model_table_host(int nknot, int nic, int noc, int ians, double tref_period,
+                                    const double* deck, double* out) {
+  if (!deck || !out) return YSX_ERR_ARG;
+  HostModel M;
+  if (!build_model(make_norms(), nknot, nic, noc, ians, tref_period, deck, M).empty())
+    return YSX_ERR_INPUT;
+  std::memcpy(out, M.tab.data(), M.tab.size() * sizeof(double));
+  return YSX_OK;
+}
+
+extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double* fp,
+                              const double* lat, const double* lon) {
+  if (!ctx || nev < 1 || !ip || !fp || !lat || !lon) return YSX_ERR_ARG;
+  if (!ctx->have_model) return set_err(ctx, YSX_ERR_ARG, "no model");
+  HIPCHK(hipSetDevice(ctx->device));
+  free_run(ctx);
+  const HostModel& M = ctx->M;
+  double amin = M.at(F_ALPHA, 1);
+  for (int i = 2; i <= M.nknot; i++) amin = std::min(amin, M.at(F_ALPHA, i));
+  ctx->R.resize(nev);
+  ctx->rk.resize(nev);
+  long long pb = 0, cfo = 0, tbo = 0, spo = 0, sio = 0;
+  int ro = 0, ntmax = 0;
+  for (int e = 0; e < nev; e++) {
+    int nr = ip[7 * e + 6];
+    if (nr < 1) return set_err(ctx, YSX_ERR_INPUT, "bad lmin/lmax/receiver count");
+    std::string err = build_run(ctx->N, M, ip + 7 * e, fp + 18 * e, lat + ro, lon + ro, ctx->R[e]);
+    if (!err.empty()) {
+      ctx->R.clear();
+      ctx->rk.clear();
+      return set_err(ctx, YSX_ERR_INPUT, nev > 1 ? "event " + std::to_string(e) + ": " + err : err);
+    }
+    const HostRun& R = ctx->R[e];
+    RunK& k = ctx->rk[e];
+    k = RunK{};
+    k.lmin = R.lmin; k.lmax = R.lmax; k.nw = R.nw; k.nr = R.nr; k.is = R.is; k.ir = R.ir;
+    k.out = R.out; k.cor = R.cor;
+    k.npair = (R.lmax - R.lmin + 1) * R.nw;
+    k.nknot = M.nknot; k.nic = M.nic; k.noc = M.noc; k.nsl = M.nsl; k.fic = M.fic;
+    k.ocean = M.ocean;
+    k.grav = R.grav;
+    k.var = (R.grav != 2 || M.fic) ? 1 : 0;
+    k.lpad = (R.lmax + 1 + 3) / 4 * 4;
+    k.jpad = (R.nw + 15) / 16 * 16;
+    k.nrpad = (R.nr + 15) / 16 * 16;
+    k.nt = R.nt;
+    k.lognt = 0;
+    while ((1 << k.lognt) < R.nt) k.lognt++;
+    k.i1 = R.i1; k.i2 = R.i2; k.roff = ro;
+    k.pbase = pb; k.cf_off = cfo; k.tb_off = tbo; k.spec_off = spo; k.seis_off = sio;
+    k.rs = R.rs; k.rr = R.rr; k.wmin = R.wmin; k.dw = R.dw; k.ep = R.ep; k.tref = M.tref;
+    k.ats = (double)R.ats; k.pibigg = ctx->N.pibigg; k.fre_norm = ctx->N.fre_norm;
+    k.alpha_min = amin;
+    k.grav_ir = M.at(F_GRAV, R.ir);
+    k.df = R.df; k.dt = R.dt; k.f11 = R.f11; k.f12 = R.f12; k.f21 = R.f21; k.f22 = R.f22;
+    k.norm = R.out == 0 ? ctx->N.r_norm : (R.out == 1 ? ctx->N.vel_norm : ctx->N.acl_norm);
+    for (int i = 0; i < 6; i++) k.mm[i] = R.mm[i];
+    pb += k.npair;
+    cfo += 15LL * k.lpad * k.jpad;
+    tbo += 9LL * k.lpad * k.nrpad;
+    spo += 3LL * R.nw * R.nr;
+    sio += 3LL * R.nt * R.nr;
+    ro += nr;
+    ntmax = std::max(ntmax, R.nt);
+  }
+  ctx->npair = pb;
+  size_t P = (size_t)pb;
+  HIPCHK(hipMalloc(&ctx->d_runs, nev * sizeof(RunK)));
+  HIPCHK(hipMemcpy(ctx->d_runs, ctx->rk.data(), nev * sizeof(RunK), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&ctx->d_rec, P * REC * sizeof(cd)));
+  HIPCHK(hipMemset(ctx->d_rec, 0, P * REC * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_starts, P * 2 * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_cnt, P * 5 * sizeof(int)));
+  HIPCHK(hipMemset(ctx->d_cnt, 0, P * 5 * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_bst, P * 56 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_mfin, P * 14 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_m2, P * 14 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_tor_i1, P * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_sph_i1, P * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_tor_y0, P * 2 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_sph_m0, P * 14 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_fac, P * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_sc3, P * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_azep, ro * sizeof(double)));
+  HIPCHK(hipMalloc(&ctx->d_eia, ro * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_spec, spo * sizeof(cd)));
+  // GEMM operands: coefficients (zero where no pair) and the Legendre tables
+  // transposed to [X*3 + |m|][lpad][nrpad]
+  HIPCHK(hipMalloc(&ctx->d_cf, cfo * sizeof(cd)));
+  HIPCHK(hipMemset(ctx->d_cf, 0, cfo * sizeof(cd)));
+  {
+    // receiver geometry from the host (O(nr)); the Legendre tables from the
+    // device, written straight into the GEMM operand
+    std::vector<double> az(ro), th(3 * (size_t)ro);
+    std::vector<std::complex<double>> eia(ro);
+    for (int e = 0; e < nev; e++) {
+      const HostRun& R = ctx->R[e];
+      const RunK& k = ctx->rk[e];
+      for (int kk = 0; kk < R.nr; kk++) {
+        az[k.roff + kk] = R.azep[kk];
+        eia[k.roff + kk] = R.eia[kk];
+        th[3 * (k.roff + kk)] = R.delta[kk];
+        th[3 * (k.roff + kk) + 1] = std::cos(R.delta[kk]);
+        th[3 * (k.roff + kk) + 2] = std::sin(R.delta[kk]);
+      }
+    }
+    double* d_th = nullptr;
+    HIPCHK(hipMalloc(&ctx->d_tb, tbo * sizeof(double)));
+    HIPCHK(hipMemset(ctx->d_tb, 0, tbo * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->d_azep, az.data(), ro * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_eia, eia.data(), ro * sizeof(cd), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&d_th, th.size() * sizeof(double)));
+    hipError_t et = hipMemcpy(d_th, th.data(), th.size() * sizeof(double), hipMemcpyHostToDevice);
+    for (int e = 0; e < nev && et == hipSuccess; e++) {
+      const RunK& k = ctx->rk[e];
+      launch_legendre(dim3((k.lmax + 64) / 64, k.nr), dim3(64), 0, ctx->stream,
+                         ctx->d_runs, e, d_th, ctx->d_tb);
+      et = hipGetLastError();
+    }
+    if (et == hipSuccess) et = hipStreamSynchronize(ctx->stream);
+    dfree(d_th);
+    if (et != hipSuccess)
+      return set_err(ctx, YSX_ERR_HIP, std::string("Legendre tables: ") + hipGetErrorString(et));
+  }
+  HIPCHK(hipMalloc(&ctx->d_tw, (size_t)ntmax * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_work, sio * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_seis, sio * sizeof(double)));
+  HIPCHK(hipMalloc(&ctx->d_queue, Q_N * sizeof(int)));
+  build_work(ctx);
+  for (auto& w : ctx->W) {
+    size_t n = std::max<size_t>(1, w.plain.size());
+    HIPCHK(hipMalloc(&w.d_plain, n * sizeof(int4)));
+    HIPCHK(hipMalloc(&w.d_batched, n * sizeof(int4)));
+    if (!w.plain.empty()) {
+      HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+    }
+  }
+  launch_twiddles(dim3((ntmax + 255) / 256), dim3(256), 0, ctx->stream, ntmax,
+                     ctx->d_tw);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->have_run = true;
+  return YSX_OK;
+}
+
+extern "C" int ysx_set_run(ysx_ctx* ctx, const int* ip, const double* fp, const double* lat,
+                           const double* lon) {
+  return ysx_set_events(ctx, 1, ip, fp, lat, lon);
+}
+
+extern "C" int ysx_event_info(const ysx_ctx* ctx, int e, int* iout, double* fout) {
+  if (!ctx || !ctx->have_run || e < 0 || e >= (int)ctx->R.size()) return YSX_ERR_ARG;
+  const HostRun& R = ctx->R[e];
+  if (iout) {
+    int v[8] = {R.nw, R.nt, R.i1, R.i2, R.is, R.ir, R.nout, ctx->rk[e].npair};
+    std::memcpy(iout, v, sizeof v);
+  }
+  if (fout) {
+    double v[8] = {R.rs, R.rr, R.dt, R.tout, R.ep, R.df, R.wmin, R.dw};
+    std::memcpy(fout, v, sizeof v);
+  }
+  return YSX_OK;
+}
+
+extern "C" int ysx_run_info(const ysx_ctx* ctx, int* iout, double* fout) {
+  return ysx_event_info(ctx, 0, iout, fout);
+}
+
+namespace {
+
+Queue plain_q(const ysx_ctx* c, int k, int e) {
+  const WorkList& w = c->W[k];
+  return Queue{w.d_plain + w.start[e], w.start[e + 1] - w.start[e], nullptr, 1};
+}
+
+// slot < 0: static distribution
+Queue batched_q(const ysx_ctx* c, int k, int slot) {
+  const WorkList& w = c->W[k];
+  return Queue{w.d_batched, (int)w.batched.size(), slot < 0 ? nullptr : c->d_queue + slot,
+               c->chunk};
+}
+
+int grid_of(const Queue& q) { return std::max(1, q.n); }
+
+PairRange pairs_of(const ysx_ctx* c, int e0, int e1) {
+  long long g0 = c->rk[e0].pbase;
+  long long g1 = (e1 < (int)c->rk.size()) ? c->rk[e1].pbase : c->npair;
+  return PairRange{e0, e1, g0, g1 - g0};
+}
+
+// the spectra of events [e0, e1) with one block per item: the plain path.
+// With timing, per-phase HIP events are recorded (one event only).
+int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
+  hipStream_t st = ctx->stream, st2 = ctx->stream2;
+  hipEvent_t* ev = ctx->ev;
+  size_t lds = ctx->lds_bytes;
+  dim3 b256(256), b64(64);
+  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
+  StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
+  for (int e = e0; e < e1; e++) {
+    PairRange PR = pairs_of(ctx, e, e + 1);
+    int nb = (int)((PR.n + 255) / 256);
+    Queue qt = plain_q(ctx, W_TOR, e), qb = plain_q(ctx, W_BVEC, e), qr = plain_q(ctx, W_RAD, e),
+          ql = plain_q(ctx, W_LSUM, e);
+    if (timing) HIPCHK(hipEventRecord(ev[0], st));
+    launch_source(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR,
+                       ctx->d_rec, ctx->d_cnt);
+    HIPCHK(hipEventRecord(ev[1], st));
+    // the radial (l = 0) solves run beside the marches on a second stream
+    HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+    if (timing) HIPCHK(hipEventRecord(ev[10], st2));
+    if (qr.n > 0) {
+      if (ctx->rk[e].grav == 0)
+        launch_rad_ng(dim3(grid_of(qr)), b64, lds, st2, ctx->d_tab, ctx->d_runs,
+                           qr, ctx->d_rec, ctx->d_cnt);
+      else
+        launch_rad(dim3(grid_of(qr)), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
+                           ctx->d_rec, ctx->d_cnt);
+    }
+    HIPCHK(hipEventRecord(ev[11], st2));
+    launch_start(dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
+                       ctx->d_starts);
+    if (timing) HIPCHK(hipEventRecord(ev[2], st));
+    launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                       SD, ctx->d_rec, ctx->d_cnt);
+    if (timing) HIPCHK(hipEventRecord(ev[3], st));
+    if (ctx->rk[e].var) {
+      Queue qv = plain_q(ctx, W_VAR, e);
+      launch_sph_var(dim3(grid_of(qv)), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
+                         ctx->d_rec, ctx->d_cnt);
+      if (timing) HIPCHK(hipEventRecord(ev[4], st));
+    } else {
+      launch_minor_top_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
+                         ctx->d_runs, qt, S, ctx->d_cnt);
+      launch_minor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
+                         ctx->d_runs, qt, SD, S, ctx->d_cnt);
+      if (timing) HIPCHK(hipEventRecord(ev[4], st));
+      launch_bvec_march_item(dim3(grid_of(qb)), b64, 0, st, ctx->d_tab,
+                         ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
+    }
+    if (timing) HIPCHK(hipEventRecord(ev[5], st));
+    HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
+    if (timing) HIPCHK(hipEventRecord(ev[6], st));
+    launch_assemble(dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec,
+                       ctx->d_cf, ctx->d_tb);
+    if (timing) HIPCHK(hipEventRecord(ev[7], st));
+    launch_lsum(dim3(grid_of(ql)), b64, 0, st, ctx->d_runs, ql, ctx->d_cf,
+                       ctx->d_tb, ctx->d_eia, ctx->d_spec);
+    if (timing) HIPCHK(hipEventRecord(ev[8], st));
+    HIPCHK(hipGetLastError());
+  }
+  return YSX_OK;
+}
+
+void enqueue_fft(ysx_ctx* ctx, const Queue& qs, const Queue& qa, const Queue& qb, int maxgrid) {
+  hipStream_t st = ctx->stream;
+  auto g = [&](const Queue& q) { return dim3(std::max(1, std::min(q.n, maxgrid))); };
+  if (qs.n > 0)
+    launch_seis(g(qs), dim3(1024), 0, st, ctx->d_runs, qs, ctx->d_spec, ctx->d_tw,
+                       ctx->d_work, ctx->d_azep, ctx->d_seis);
+  if (qa.n > 0)
+    launch_fft_lds(g(qa), dim3(512), 0, st, ctx->d_runs, qa, ctx->d_spec,
+                       ctx->d_tw, ctx->d_work);
+  if (qb.n > 0)
+    launch_fft_post(g(qb), dim3(256), 0, st, ctx->d_runs, qb, ctx->d_tw,
+                       ctx->d_work, ctx->d_azep, ctx->d_seis);
+}
+
+int enqueue_seis_plain(ysx_ctx* ctx, int e0, int e1) {
+  for (int e = e0; e < e1; e++)
+    enqueue_fft(ctx, plain_q(ctx, W_SEIS, e), plain_q(ctx, W_FFTA, e), plain_q(ctx, W_FFTB, e),
+                1 << 30);
+  HIPCHK(hipGetLastError());
+  return YSX_OK;
+}
+
+// every event at once from the cost-sorted lists: one block per item, or
+// persistent blocks taking items from atomic queues
+int enqueue_batched(ysx_ctx* ctx) {
+  hipStream_t st = ctx->stream, st2 = ctx->stream2;
+  hipEvent_t* ev = ctx->ev;
+  size_t lds = ctx->lds_bytes;
+  dim3 b256(256), b64(64);
+  int nev = (int)ctx->rk.size();
+  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
+  StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
+  PairRange PR = pairs_of(ctx, 0, nev);
+  int nb = (int)((PR.n + 255) / 256);
+  bool pe = ctx->persist != 0;
+  int pers = ctx->blocks_per_cu * ctx->ncu;
+  auto grid = [&](const Queue& q) {
+    return dim3(std::max(1, pe ? std::min(q.n, pers) : q.n));
+  };
+  Queue qt = batched_q(ctx, W_TOR, Q_TOR), qtop = batched_q(ctx, W_TOR, Q_TOP),
+        qmin = batched_q(ctx, W_TOR, Q_MINOR), qb = batched_q(ctx, W_BVEC, Q_BVEC),
+        qr = batched_q(ctx, W_RAD, -1), ql = batched_q(ctx, W_LSUM, -1),
+        qs = batched_q(ctx, W_SEIS, -1);
+  if (pe) launch_reset(dim3(1), dim3(64), 0, st, ctx->d_queue, (int)Q_N);
+  launch_source(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, ctx->d_rec,
+                     ctx->d_cnt);
+  HIPCHK(hipEventRecord(ev[1], st));
+  HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+  if (qr.n > 0) {
+    launch_rad(dim3(qr.n), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
+                       ctx->d_rec, ctx->d_cnt);
+    bool any_ng = false;
+    for (const RunK& k : ctx->rk) any_ng = any_ng || (k.grav == 0 && k.lmin == 0);
+    if (any_ng)
+      launch_rad_ng(dim3(qr.n), b64, lds, st2, ctx->d_tab, ctx->d_runs, qr,
+                         ctx->d_rec, ctx->d_cnt);
+  }
+  HIPCHK(hipEventRecord(ev[11], st2));
+  launch_start(dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
+                     ctx->d_starts);
+  if (pe) {
+    launch_tor_march_pers(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD,
+                       ctx->d_rec, ctx->d_cnt);
+    launch_minor_top_pers(grid(qtop), b64, 0, st, ctx->d_tab, ctx->d_runs, qtop,
+                       S, ctx->d_cnt);
+    launch_minor_march_pers(grid(qmin), b64, 0, st, ctx->d_tab, ctx->d_runs,
+                       qmin, SD, S, ctx->d_cnt);
+    launch_bvec_march_pers(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
+                       ctx->d_rec, ctx->d_cnt);
+  } else {
+    launch_tor_march_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                       SD, ctx->d_rec, ctx->d_cnt);
+    launch_minor_top_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt, S,
+                       ctx->d_cnt);
+    launch_minor_march_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                       SD, S, ctx->d_cnt);
+    launch_bvec_march_item(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb,
+                       S, ctx->d_rec, ctx->d_cnt);
+  }
+  Queue qv = batched_q(ctx, W_VAR, -1);
+  if (qv.n > 0)
+    launch_sph_var(dim3(qv.n), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
+                       ctx->d_rec, ctx->d_cnt);
+  HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
+  launch_assemble(dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec, ctx->d_cf,
+                     ctx->d_tb);
+  launch_lsum(dim3(std::max(1, ql.n)), b64, 0, st, ctx->d_runs, ql, ctx->d_cf,
+                     ctx->d_tb, ctx->d_eia, ctx->d_spec);
+  enqueue_fft(ctx, qs, batched_q(ctx, W_FFTA, -1), batched_q(ctx, W_FFTB, -1), 1 << 30);
+  HIPCHK(hipGetLastError());
+  return YSX_OK;
+}
+
+int elapsed(ysx_ctx* ctx, int a, int b, double& out) {
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, ctx->ev[a], ctx->ev[b]));
+  out = t;
+  return YSX_OK;
+}
+
+int run_spectra(ysx_ctx* ctx) {
+  int rc = enqueue_plain(ctx, 0, 1, true);
+  if (rc) return rc;
+  HIPCHK(hipEventSynchronize(ctx->ev[8]));
+  double a, b;
+  if ((rc = elapsed(ctx, 0, 1, a)) || (rc = elapsed(ctx, 1, 2, b))) return rc;
+  ctx->ms[0] = a + b;  // source vectors + starts
+  if ((rc = elapsed(ctx, 2, 3, ctx->ms[1])) || (rc = elapsed(ctx, 3, 4, ctx->ms[2])) ||
+      (rc = elapsed(ctx, 4, 5, ctx->ms[3])) || (rc = elapsed(ctx, 10, 11, ctx->ms[4])) ||
+      (rc = elapsed(ctx, 6, 7, ctx->ms[5])) || (rc = elapsed(ctx, 7, 8, ctx->ms[6])))
+    return rc;
+  ctx->have_spec = true;
+  ctx->have_seis = false;
+  return YSX_OK;
+}
+
+int run_seis(ysx_ctx* ctx) {
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipEventRecord(ctx->ev[12], st));
+  int rc = enqueue_seis_plain(ctx, 0, 1);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(ctx->ev[13], st));
+  HIPCHK(hipEventSynchronize(ctx->ev[13]));
+  if ((rc = elapsed(ctx, 12, 13, ctx->ms[7]))) return rc;
+  ctx->have_seis = true;
+  return YSX_OK;
+}
+
+}  // namespace
+
+extern "C" int ysx_spectra(ysx_ctx* ctx, double* ur, double* ut, double* up) {
+  if (!ctx) return YSX_ERR_ARG;
+  if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
+  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: use ysx_forward_events");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = run_spectra(ctx);
+  if (rc) return rc;
+  size_t n = (size_t)ctx->R[0].nw * ctx->R[0].nr;
+  double* outs[3] = {ur, ut, up};
+  for (int c = 0; c < 3; c++)
+    if (outs[c])
+      HIPCHK(hipMemcpy(outs[c], ctx->d_spec + c * n, n * sizeof(cd), hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
+extern "C" int ysx_seismograms(ysx_ctx* ctx, double* seis) {
+  if (!ctx) return YSX_ERR_ARG;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra");
+  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: use ysx_forward_events");
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = run_seis(ctx);
+  if (rc) return rc;
+  if (seis)
+    HIPCHK(hipMemcpy(seis, ctx->d_seis, 3 * (size_t)ctx->R[0].nt * ctx->R[0].nr * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
+extern "C" int ysx_forward(ysx_ctx* ctx, double* seis) {
+  int rc = ysx_spectra(ctx, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  return ysx_seismograms(ctx, seis);
+}
+
+extern "C" int ysx_set_batch_tuning(ysx_ctx* ctx, int persist, int chunk, int blocks_per_cu) {
+  if (!ctx || (persist != 0 && persist != 1) || chunk < 1 || chunk > 64 || blocks_per_cu < 1 ||
+      blocks_per_cu > 32)
+    return YSX_ERR_ARG;
+  if (persist != ctx->persist || chunk != ctx->chunk || blocks_per_cu != ctx->blocks_per_cu)
+    drop_graph(ctx);
+  ctx->persist = persist;
+  ctx->chunk = chunk;
+  ctx->blocks_per_cu = blocks_per_cu;
+  return YSX_OK;
+}
+
+extern "C" int ysx_forward_events(ysx_ctx* ctx, int mode, double* seis) {
+  if (!ctx || (mode != 0 && mode != 1)) return YSX_ERR_ARG;
+  if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
+  HIPCHK(hipSetDevice(ctx->device));
+  int nev = (int)ctx->rk.size();
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipEventRecord(ctx->ev[14], st));
+  if (mode == 0) {
+    int rc = enqueue_plain(ctx, 0, nev, false);
+    if (!rc) rc = enqueue_seis_plain(ctx, 0, nev);
+    if (rc) return rc;
+  } else {
+    if (!ctx->gexec) {
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeGlobal));
+      int rc = enqueue_batched(ctx);
+      hipGraph_t g = nullptr;
+      hipError_t ec = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ec != hipSuccess)
+        return set_err(ctx, YSX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      ctx->graph = g;
+      HIPCHK(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(ctx->gexec, st));
+  }
+  HIPCHK(hipEventRecord(ctx->ev[15], st));
+  HIPCHK(hipEventSynchronize(ctx->ev[15]));
+  double t = 0.0;
+  int rc = elapsed(ctx, 14, 15, t);
+  if (rc) return rc;
+  for (double& m : ctx->ms) m = 0.0;
+  ctx->ms[7] = t;
+  ctx->have_spec = ctx->have_seis = true;
+  if (seis) {
+    size_t n = 0;
+    for (const HostRun& R : ctx->R) n += 3 * (size_t)R.nt * R.nr;
+    HIPCHK(hipMemcpy(seis, ctx->d_seis, n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return YSX_OK;
+}
+
+extern "C" int ysx_event_spectra(ysx_ctx* ctx, int e, double* ur, double* ut, double* up) {
+  if (!ctx || e < 0 || e >= (int)ctx->rk.size()) return YSX_ERR_ARG;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t n = (size_t)ctx->R[e].nw * ctx->R[e].nr;
+  double* outs[3] = {ur, ut, up};
+  for (int c = 0; c < 3; c++)
+    if (outs[c])
+      HIPCHK(hipMemcpy(outs[c], ctx->d_spec + ctx->rk[e].spec_off + c * n, n * sizeof(cd),
+                       hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
+extern "C" int ysx_event_seismograms(ysx_ctx* ctx, int e, double* seis) {
+  if (!ctx || !seis || e < 0 || e >= (int)ctx->rk.size()) return YSX_ERR_ARG;
+  if (!ctx->have_seis) return set_err(ctx, YSX_ERR_ARG, "no seismograms computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t n = 3 * (size_t)ctx->R[e].nt * ctx->R[e].nr;
+  HIPCHK(hipMemcpy(seis, ctx->d_seis + ctx->rk[e].seis_off, n * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
+extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j, double* rec) {
+  if (!ctx || (n > 0 && (!l || !j || !rec))) return YSX_ERR_ARG;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  const RunK& k = ctx->rk[0];
+  for (int i = 0; i < n; i++) {
+    if (l[i] < k.lmin || l[i] > k.lmax || j[i] < 1 || j[i] > k.nw)
+      return set_err(ctx, YSX_ERR_ARG, "pair outside the run");
+    size_t p = (size_t)(l[i] - k.lmin) * k.nw + (j[i] - 1);
+    HIPCHK(hipMemcpy(rec + (size_t)i * 2 * REC, ctx->d_rec + p * REC, REC * sizeof(cd),
+                     hipMemcpyDeviceToHost));
+  }
+  return YSX_OK;
+}
+
+extern "C" int ysx_timings(const ysx_ctx* ctx, double* ms8) {
+  if (!ctx || !ms8) return YSX_ERR_ARG;
+  std::memcpy(ms8, ctx->ms, sizeof ctx->ms);
+  return YSX_OK;
+}
+
+extern "C" int ysx_step_counts(ysx_ctx* ctx, long long* out5) {
+  if (!ctx || !out5) return YSX_ERR_ARG;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  size_t P = (size_t)ctx->npair;
+  std::vector<int> c(P * 5);
+  HIPCHK(hipMemcpy(c.data(), ctx->d_cnt, c.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (int q = 0; q < 5; q++) out5[q] = 0;
+  for (size_t p = 0; p < P; p++)
+    for (int q = 0; q < 5; q++) out5[q] += c[5 * p + q];
+  return YSX_OK;
+}

@@ -5,7 +5,7 @@
 // else in registers.  The arithmetic follows the reference routines cited
 // at each function; the GPU-specific parts are the data layout, the
 // register-lean Cash-Karp step and the kernels that drive these functions
-// (ysx_gpu.hip).
+// (ysx_march_kernels.hpp, ysx_k_misc.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 

@@ -1,0 +1,45 @@
+// ysx_k_march.hip -- one march kernel and its launch function per object
+// file: the build compiles this file eight times, once per kernel
+// (YSX_MARCH = 0 toroidal, 1 minors from the surface, 2 minors from the
+// start level, 3 b-vector) and work-distribution mode (YSX_PERSIST = 0 one
+// block per work item, 1 persistent blocks on atomic queues), so that the
+// large kernels compile in parallel.
+#include "ysx_march_kernels.hpp"
+
+#if !defined(YSX_MARCH) || !defined(YSX_PERSIST)
+#error "compile with -DYSX_MARCH=0..3 -DYSX_PERSIST=0|1"
+#endif
+
+#if YSX_PERSIST
+#define YSX_LAUNCH(name) launch_##name##_pers
+#else
+#define YSX_LAUNCH(name) launch_##name##_item
+#endif
+
+namespace ysx {
+
+#if YSX_MARCH == 0
+void YSX_LAUNCH(tor_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                           const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_tor_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, SD, rec, cnt);
+}
+#elif YSX_MARCH == 1
+void YSX_LAUNCH(minor_top)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                           const RunK* runs, Queue Q, SphState S, int* cnt) {
+  hipLaunchKernelGGL(k_minor_top<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, cnt);
+}
+#elif YSX_MARCH == 2
+void YSX_LAUNCH(minor_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                             const RunK* runs, Queue Q, StartData SD, SphState S, int* cnt) {
+  hipLaunchKernelGGL(k_minor_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, SD, S, cnt);
+}
+#elif YSX_MARCH == 3
+void YSX_LAUNCH(bvec_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                            const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_bvec_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, rec, cnt);
+}
+#else
+#error "YSX_MARCH must be 0..3"
+#endif
+
+}  // namespace ysx

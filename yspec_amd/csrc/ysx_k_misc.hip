@@ -1,0 +1,855 @@
+// ysx_k_misc.hip -- the kernels of libysx.so other than the lockstep
+// marches (source vectors, start levels, radial and variant solvers,
+// coefficient assembly, receiver sum, Legendre tables, model transform,
+// inverse FFT and post-processing), and their launch functions.
+#include <cmath>
+
+#include "ysx_kern.hpp"
+
+namespace ysx {
+namespace {
+
+__global__ void k_reset(int* counters, int n) {
+  if ((int)threadIdx.x < n) counters[threadIdx.x] = 0;
+}
+
+// -------------------------------------------------------------- k_source
+// source_vector_tor / _sph / _rad, src/module_solver.f90:145-215,
+// 566-627, 1747-1847.  source_vector_sph keeps only the real part of its
+// dispersion factors (they are declared real(dp) there, :1771-1772).
+__global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
+                                                const RunK* __restrict__ runs, PairRange PR,
+                                                cd* __restrict__ rec, int* __restrict__ cnt) {
+  Tab T = load_tab(gtab, runs[PR.e0].nknot);
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= PR.n) return;
+  long long g = PR.g0 + t;
+  const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
+  int p = (int)(g - R.pbase);
+  int l, j;
+  pair_lj(R, p, l, j);
+  PS s;
+  pair_state(R, l, j, s);
+  // records and step counters are zeroed once when the run is set; every
+  // forward writes the same entries of them again
+  cd* o = rec + (size_t)g * REC;
+  SplW w = spl_w(T, R.is, R.rs);
+  double rlcon = spl(T, w, F_LCON), rccon = spl(T, w, F_CCON), rfcon = spl(T, w, F_FCON),
+         rqmu = spl(T, w, F_QMU), rxlam = spl(T, w, F_XLAM), rxa2 = spl(T, w, F_XA2);
+  const double* mm = R.mm;
+  double rs = R.rs;
+  cd den = cinv(cmuli(s.omega) * (rs * rs));  // 1/(i w rs^2)
+  cd nd = s.nu * den;
+  if (l == 0) {
+    cd qaa = 1.0 + rxa2 * s.lno, qff = 1.0 + rxlam * s.lno;
+    cd cc = cinv(rccon * qaa);
+    cd cf = rfcon * qff;
+    o[O_SR0 + 0] = nd * ((rs * mm[0]) * cc);
+    o[O_SR0 + 1] = nd * ((2.0 * mm[0]) * (cf * cc) - (mm[3] + mm[5]));
+    return;
+  }
+  if (l > 1) {
+    cd cl = rlcon * (1.0 + rqmu * s.lno);
+    cd icl = cinv(cl);
+    o[O_SVT + 1] = nd * ((0.5 * rs) * C(-mm[2], mm[1]) * icl);
+    o[O_SVT + 3] = nd * ((0.5 * rs) * C(mm[2], mm[1]) * icl);
+    o[O_SVT + 5 + 0] = nd * ((0.25 * s.szeta2m2) * C(2.0 * mm[4], mm[5] - mm[3]));
+    o[O_SVT + 5 + 4] = nd * ((0.25 * s.szeta2m2) * C(2.0 * mm[4], mm[3] - mm[5]));
+  }
+  double qaa = 1.0 + rxa2 * s.lno.x, qff = 1.0 + rxlam * s.lno.x, qll = 1.0 + rqmu * s.lno.x;
+  double cc = 1.0 / (rccon * qaa), cf = rfcon * qff, cl = rlcon * qll;
+  double zeta = s.zeta;
+  cd* sv = o + O_SVS;  // svs(5,6) column-major
+  sv[1 + 5 * 1] = nd * (C(0.5 * rs * mm[1] / cl, 0.5 * rs * mm[2] / cl));
+  sv[2 + 5 * 0] = nd * C(rs * mm[0] * cc);
+  sv[2 + 5 * 3] = nd * C(-mm[3] - mm[5] + 2.0 * mm[0] * cf * cc);
+  sv[2 + 5 * 4] = nd * C(0.5 * zeta * (mm[3] + mm[5]) - zeta * mm[0] * cf * cc);
+  sv[3 + 5 * 1] = nd * (C(-0.5 * rs * mm[1] / cl, 0.5 * rs * mm[2] / cl));
+  if (l > 1) {
+    sv[0 + 5 * 4] = nd * ((0.25 * s.szeta2m2) * C(mm[5] - mm[3], -2.0 * mm[4]));
+    sv[4 + 5 * 4] = nd * ((0.25 * s.szeta2m2) * C(mm[5] - mm[3], 2.0 * mm[4]));
+  }
+}
+
+// ----------------------------------------------------------------- k_start
+// Start levels and starting solutions of every pair (one lane per pair):
+// tor_solver's start (src/module_solver.f90:64-75, 103-114) and
+// sph_solver's start (:678-722, 758-785).  The marches pick these up when
+// they reach each lane's start knot.
+__global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
+                                               const RunK* __restrict__ runs, PairRange PR,
+                                               StartData SD, int* __restrict__ starts) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= PR.n) return;
+  long long g = PR.g0 + t;
+  const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
+  Tab T{gtab, R.nknot};
+  int l, j;
+  pair_lj(R, (int)(g - R.pbase), l, j);
+  size_t p = (size_t)g;
+  if (l < 1) return;
+  PS s;
+  pair_state(R, l, j, s);
+  Layers L = layers(R);
+  bool up = R.rr >= R.rs;
+  int itop = up ? R.is : R.ir;
+  if (l >= 2) {
+    int i1 = tor_start_level(T, L, l, s.rew, itop);
+    starts[2 * p] = i1;
+    if (i1 == itop && i1 > L.noc + 1) i1--;
+    cd y[2];
+    if (i1 == L.noc + 1) {
+      y[0] = C(1.0);
+      y[1] = C(0.0);
+    } else {
+      tor_ystart(T, s, i1, T(F_R, i1), y);
+    }
+    SD.tor_i1[p] = i1;
+    SD.tor_y0[2 * p] = y[0];
+    SD.tor_y0[2 * p + 1] = y[1];
+  }
+  if (R.var) return;  // the per-lane variant solvers find their own start
+  int i1 = sph_start_level(T, L, l, s.rew, itop, R.fre_norm);
+  starts[2 * p + 1] = i1;
+  double rt = T(F_R, i1);
+  if (up && i1 == 1) {
+    double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
+    rt = fmin(0.1 * dr, 0.1 * T(F_R, 2));
+  }
+  if (i1 >= itop) {
+    i1 = itop - 1;
+    rt = T(F_R, i1);
+  }
+  cd m[14];
+  for (int i = 0; i < 14; i++) m[i] = C(0.0);
+  if (i1 > L.nic && i1 <= L.noc)
+    minor_fluid_start(T, s, i1, rt, m);
+  else
+    minor_solid_start(T, s, i1, rt, m);
+  SD.sph_i1[p] = i1;
+  for (int i = 0; i < 14; i++) SD.sph_m0[(size_t)p * 14 + i] = m[i];
+}
+
+// ------------------------------------------------------------------- k_rad
+// rad_solver_wg with ss = identity(2), src/module_solver.f90:446-561, on
+// the grid of make_grid, src/module_model.f90:638-787.  Only yout is
+// formed; the caller discards pout (src/yspec.f90:277).
+struct Run3 {
+  double base[3], step[3];
+  int j0[3], j1[3], n;
+  int src, rec;  // run whose last point is the source / receiver (-1: none)
+};
+
+__device__ inline void grid_runs(const Tab& T, int i, double drr, double rs, double rr, Run3& G) {
+  double r1 = T(F_R, i), r2 = T(F_R, i + 1);
+  G.n = 0;
+  G.src = G.rec = -1;
+  if (r1 == r2) return;
+  bool ls = (rs > r1 && rs <= r2), lr = (rr > r1 && rr <= r2);
+  auto add = [&](double a, double b, int j0) {
+    int n = (int)floor((b - a) / drr) + 2;
+    G.base[G.n] = a;
+    G.step[G.n] = (b - a) / (double)(n - 1);
+    G.j0[G.n] = j0;
+    G.j1[G.n] = n;
+    G.n++;
+  };
+  if (ls && lr) {
+    if (rs < rr) {
+      add(r1, rs, 1); G.src = 0;
+      add(rs, rr, 2); G.rec = 1;
+      add(rr, r2, 2);
+    } else if (rr < rs) {
+      add(r1, rr, 1); G.rec = 0;
+      add(rr, rs, 2); G.src = 1;
+      add(rs, r2, 2);
+    } else {
+      add(r1, rs, 1); G.src = 0; G.rec = 0;
+      add(rs, r2, 2);
+    }
+  } else if (ls) {
+    add(r1, rs, 1); G.src = 0;
+    add(rs, r2, 2);
+  } else if (lr) {
+    add(r1, rr, 1); G.rec = 0;
+    add(rr, r2, 2);
+  } else {
+    add(r1, r2, 1);
+  }
+}
+
+__device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
+                                         cd* __restrict__ rec, int* __restrict__ cnt) {
+  int j = jb * 64 + threadIdx.x + 1;
+  if (j > R.nw || R.lmin != 0) return;
+  size_t p = (size_t)R.pbase + (j - 1);  // l = 0 pairs come first
+  PS s;
+  pair_state(R, 0, j, s);
+  if (R.grav == 0) return;  // rad_solver_ng: k_rad_ng
+  RadRHS f;
+  int nk = R.nknot;
+  double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
+  double rs = R.rs, rr = R.rr;
+  int nsr = 0;
+  // --- upward sweep over grid points 1 .. source
+  cd y1[2], y1r[2] = {C(0.0), C(0.0)};
+  bool rec_below = false;  // receiver at or below the source point
+  {
+    int k = 0;
+    double rprev = 0.0;
+    int iprev = 1;
+    bool done = false;
+    for (int i = 1; i <= nk - 1 && !done; i++) {
+      Run3 G;
+      grid_runs(T, i, dr, rs, rr, G);
+      for (int q = 0; q < G.n && !done; q++) {
+        for (int jj = G.j0[q]; jj <= G.j1[q]; jj++) {
+          double rg = G.base[q] + (double)(jj - 1) * G.step[q];
+          k++;
+          if (k == 2) {
+            double rt = fmin(0.1 * dr, 0.1 * rg);
+            rad_ystart(T, s, 1, rt, y1);
+            yint(T, s, nk, 1, i, rt, rg, y1, f, (int*)nullptr, nsr);
+          } else if (k > 2) {
+            yint(T, s, nk, iprev, i, rprev, rg, y1, f, (int*)nullptr, nsr);
+          }
+          rprev = rg;
+          iprev = i;
+          bool last = (jj == G.j1[q]);
+          if (last && q == G.rec) {
+            rec_below = true;
+            y1r[0] = y1[0];
+            y1r[1] = y1[1];
+          }
+          if (last && q == G.src) {
+            done = true;
+            break;
+          }
+        }
+      }
+    }
+  }
+  // --- downward sweep from the top grid point to the source point
+  cd y2[2] = {C(1.0), C(0.0)}, y2r[2] = {C(1.0), C(0.0)};
+  {
+    bool first = true, done = false;
+    double rprev = 0.0;
+    int iprev = nk - 1;
+    for (int i = nk - 1; i >= 1 && !done; i--) {
+      Run3 G;
+      grid_runs(T, i, dr, rs, rr, G);
+      for (int q = G.n - 1; q >= 0 && !done; q--) {
+        for (int jj = G.j1[q]; jj >= G.j0[q]; jj--) {
+          double rg = G.base[q] + (double)(jj - 1) * G.step[q];
+          if (!first) yint(T, s, nk, iprev, i, rprev, rg, y2, f, (int*)nullptr, nsr);
+          first = false;
+          rprev = rg;
+          iprev = i;
+          bool last = (jj == G.j1[q]);
+          if (last && q == G.src) {
+            done = true;
+            break;
+          }
+          if (last && q == G.rec && !rec_below) {
+            y2r[0] = y2[0];
+            y2r[1] = y2[1];
+          }
+        }
+      }
+    }
+  }
+  cd delta = cinv(y2[1] * y1[0] - y2[0] * y1[1]);
+  cd yo[4];
+  if (rec_below) {
+    cd b1 = y2[1], b2 = -y2[0];
+    yo[0] = -delta * y1r[0] * b1;
+    yo[1] = -delta * y1r[1] * b1;
+    yo[2] = -delta * y1r[0] * b2;
+    yo[3] = -delta * y1r[1] * b2;
+  } else {
+    cd b1 = y1[1], b2 = -y1[0];
+    yo[0] = -delta * y2r[0] * b1;
+    yo[1] = -delta * y2r[1] * b1;
+    yo[2] = -delta * y2r[0] * b2;
+    yo[3] = -delta * y2r[1] * b2;
+  }
+  cd* o = rec + p * REC + O_YRAD;
+  for (int i = 0; i < 4; i++) o[i] = yo[i];
+  cnt[5 * p + 4] = nsr;
+}
+
+// items (event, block of 64 frequencies); one lane per l = 0 pair, no
+// lockstep (the grid of make_grid depends on w)
+__global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab,
+                                            const RunK* __restrict__ runs, Queue Q,
+                                            cd* __restrict__ rec, int* __restrict__ cnt) {
+  Tab T = load_tab(gtab, runs[0].nknot);
+  RUN_ITEMS(false, Q, rad_item(T, runs[it.x], it.y, rec, cnt));
+}
+
+// rad_solver_ng (gravity switch 0, src/yspec.f90:271-273) for the l = 0
+// pairs of the events with grav = 0; same items as k_rad
+__device__ __forceinline__ void rad_ng_item(const Tab& T, const RunK& R, int jb,
+                                            cd* __restrict__ rec, int* __restrict__ cnt) {
+  int j = jb * 64 + threadIdx.x + 1;
+  if (j > R.nw || R.lmin != 0 || R.grav != 0) return;
+  size_t p = (size_t)R.pbase + (j - 1);
+  PS s;
+  pair_state(R, 0, j, s);
+  VarGeo G{R.nknot, R.nic, R.noc, R.nsl, R.ocean, R.is, R.ir, R.rs, R.rr, R.alpha_min,
+           R.fre_norm};
+  cd yo[4];
+  int n = 0;
+  rad_solve_ng(T, s, G, yo, n);
+  cd* o = rec + p * REC + O_YRAD;
+  for (int i = 0; i < 4; i++) o[i] = yo[i];
+  cnt[5 * p + 4] = n;
+}
+
+__global__ void __launch_bounds__(64) k_rad_ng(const double* __restrict__ gtab,
+                                               const RunK* __restrict__ runs, Queue Q,
+                                               cd* __restrict__ rec, int* __restrict__ cnt) {
+  Tab T = load_tab(gtab, runs[0].nknot);
+  RUN_ITEMS(false, Q, rad_ng_item(T, runs[it.x], it.y, rec, cnt));
+}
+
+// ------------------------------------------------------------- k_sph_var
+// The spheroidal solve of one pair per lane for events outside the lockstep
+// kernels: gravity switch 0 / 1 (sph_solver_ng, _cow, _ng_fic, _cow_fic)
+// or a fluid inner core with self-gravitation (sph_solver_fic), with the
+// reference driver's 4-vector -> 6-vector layout (src/yspec.f90:342-374).
+__device__ __forceinline__ void sph_var_item(const Tab& T, const RunK& R, int pb,
+                                             cd* __restrict__ rec, int* __restrict__ cnt) {
+  int pl = pb * 64 + (int)threadIdx.x;
+  if (pl >= R.npair) return;
+  int l, j;
+  pair_lj(R, pl, l, j);
+  if (l < 1) return;
+  PS s;
+  pair_state(R, l, j, s);
+  VarGeo G{R.nknot, R.nic, R.noc, R.nsl, R.ocean, R.is, R.ir, R.rs, R.rr, R.alpha_min,
+           R.fre_norm};
+  VarCount n{0, 0, 0};
+  size_t p = (size_t)R.pbase + pl;
+  cd* o = rec + p * REC + O_YSPH;
+  if (R.grav == 2) {
+    cd y[24];
+    sph_solve_fic(T, s, G, y, n);
+    for (int q = 0; q < 24; q++) o[q] = y[q];
+  } else {
+    cd y[16];
+    if (R.grav == 1)
+      sph_solve_ngc<true>(T, s, G, R.fic != 0, y, n);
+    else
+      sph_solve_ngc<false>(T, s, G, R.fic != 0, y, n);
+    for (int c = 0; c < 4; c++) {
+      o[6 * c + 0] = y[4 * c + 0];
+      o[6 * c + 1] = y[4 * c + 1];
+      o[6 * c + 2] = C(0.0);
+      o[6 * c + 3] = y[4 * c + 2];
+      o[6 * c + 4] = y[4 * c + 3];
+      o[6 * c + 5] = C(0.0);
+    }
+  }
+  cnt[5 * p + 1] = n.sol;
+  cnt[5 * p + 2] = n.flu;
+  cnt[5 * p + 3] = n.bvec;
+}
+
+// items (event, block of 64 pairs) of the events with var set
+__global__ void __launch_bounds__(64) k_sph_var(const double* __restrict__ gtab,
+                                                const RunK* __restrict__ runs, Queue Q,
+                                                cd* __restrict__ rec, int* __restrict__ cnt) {
+  RUN_ITEMS(false, Q, sph_var_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, rec, cnt));
+}
+
+// ------------------------------------------------------------- k_assemble
+// output-type scaling and the uu/vv/ww coefficients, src/yspec.f90:280-288,
+// 309-316, 377-400 (zeta there is single precision, :255)
+__global__ void __launch_bounds__(256) k_assemble(const RunK* __restrict__ runs, PairRange PR,
+                                                  cd* __restrict__ rec, cd* __restrict__ cf,
+                                                  const double* __restrict__ tb) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= PR.n) return;
+  long long g = PR.g0 + t;
+  const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
+  GemmK G = gemm_of(R, cf, tb);
+  int l, j;
+  pair_lj(R, (int)(g - R.pbase), l, j);
+  cd* o = rec + (size_t)g * REC;
+  cd w = C(R.wmin + (double)(j - 1) * R.dw, -R.ep);
+  cd fo = (R.out == 1) ? cmuli(w) : (R.out == 2 ? -(w * w) : C(1.0));
+  double rr = R.rr;
+  if (l == 0) {
+    cd u3 = (o[O_SR0] * o[O_YRAD + 0] + o[O_SR0 + 1] * o[O_YRAD + 2]) / rr;
+    o[O_UU + 2] = fo * u3;
+    put_coefs(G, l, j, o);
+    return;
+  }
+  double zeta = (double)sqrtf((float)(l * (l + 1)));
+  if (l > 1) {
+    cd t1 = fo * o[O_YTOR + 0], t2 = fo * o[O_YTOR + 2];
+    for (int m = 0; m < 5; m++)
+      o[O_WW + m] = (o[O_SVT + m] * t1 + o[O_SVT + 5 + m] * t2) / (zeta * rr);
+  }
+  cd y1[4], y2[4];
+  for (int c = 0; c < 4; c++) {
+    cd a = o[O_YSPH + 6 * c + 0], b = o[O_YSPH + 6 * c + 1], g3 = o[O_YSPH + 6 * c + 2];
+    if (R.out == 2 && R.cor != 0) {
+      cd c1 = -((double)(l + 1) * g3 + (2.0 * R.grav_ir) * a) / rr;
+      cd c2 = zeta * (g3 + R.grav_ir * a) / rr;
+      y1[c] = fo * a + c1;
+      y2[c] = fo * b + c2;
+    } else {
+      y1[c] = fo * a;
+      y2[c] = fo * b;
+    }
+  }
+  const cd* sv = o + O_SVS;
+  for (int m = 0; m < 5; m++) {
+    cd su = y1[0] * sv[m] + y1[1] * sv[5 + m] + y1[2] * sv[15 + m] + y1[3] * sv[20 + m];
+    cd sw = y2[0] * sv[m] + y2[1] * sv[5 + m] + y2[2] * sv[15 + m] + y2[3] * sv[20 + m];
+    o[O_UU + m] = su / rr;
+    o[O_VV + m] = sw / (zeta * rr);
+  }
+  put_coefs(G, l, j, o);
+}
+
+// ---------------------------------------------------------------- k_lsum
+// The receiver sum (src/yspec.f90:290-293, 318-335, 402-419) as FP64 MFMA
+// GEMMs: for each Legendre table X in {x, dx/dtheta, x/sin(theta)} and
+// |m| = a, C[j][k] = sum_l coef(l, w_j) X_a(l, theta_k) over the coefficient
+// sets that table multiplies; the epilogue applies (-1)^a for m < 0, the
+// factors i*m and exp(i m phi_k) and accumulates ur, ut, up.  One wave owns a
+// 16 (w) x 16 (receiver) tile; v_mfma_f64_16x16x4f64 lane layout: A[i][k] at
+// lane i + 16k, B[k][n] at lane n + 16k, D[i][n] at lane n + 16(i%4), reg i/4.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// one (table, |m|) group: GEMM over l, then the epilogue for this group
+template <int X, int A, int NC, int C0, int C1, int C2, int C3>
+__device__ __forceinline__ void lsum_group(const GemmK& G, int j0, int k0, int li, int lq,
+                                           cd ep, cd en, cd* ur, cd* ut, cd* up) {
+  constexpr int cs[4] = {C0, C1, C2, C3};
+  const size_t plane = (size_t)G.lpad * G.jpad;
+  d4 acc[NC][2];
+#pragma unroll
+  for (int q = 0; q < NC; q++) acc[q][0] = acc[q][1] = (d4){0.0, 0.0, 0.0, 0.0};
+  const double* tb = G.tb + (size_t)(X * 3 + A) * G.lpad * G.nrpad + k0 + li;
+  const cd* cf = G.cf + (size_t)lq * G.jpad + j0 + li;
+  for (int l0 = 0; l0 < G.lpad; l0 += 4) {
+    double b = tb[(size_t)(l0 + lq) * G.nrpad];
+    size_t at = (size_t)l0 * G.jpad;
+    cd av[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) av[q] = cf[cs[q] * plane + at];
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      acc[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].x, b, acc[q][0], 0, 0, 0);
+      acc[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].y, b, acc[q][1], 0, 0, 0);
+    }
+  }
+  constexpr double sg = (A & 1) ? -1.0 : 1.0;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    cd P = C(acc[0][0][v], acc[0][1][v]);
+    if constexpr (A == 0) {
+      if constexpr (X == 0) ur[v] = ur[v] + P;
+      else ut[v] = ut[v] + P;  // vv_0 dx/dtheta
+    } else {
+      cd Mn = C(acc[1][0][v], acc[1][1][v]);
+      cd pos = ep * P, neg = sg * (en * Mn);
+      if constexpr (X == 0) {
+        ur[v] = ur[v] + pos + neg;
+        cd Wp = C(acc[NC - 2][0][v], acc[NC - 2][1][v]), Wn = C(acc[NC - 1][0][v], acc[NC - 1][1][v]);
+        ut[v] = ut[v] + cmuli((double)A * (ep * Wp - sg * (en * Wn)));
+      } else if constexpr (X == 1) {
+        ut[v] = ut[v] + pos + neg;
+        cd Wp = C(acc[NC - 2][0][v], acc[NC - 2][1][v]), Wn = C(acc[NC - 1][0][v], acc[NC - 1][1][v]);
+        up[v] = up[v] - (ep * Wp + sg * (en * Wn));
+      } else {
+        up[v] = up[v] + cmuli((double)A * (pos - neg));
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void lsum_item(const RunK& R, const GemmK& G, int tile,
+                                          const cd* __restrict__ eia, cd* __restrict__ spec) {
+  int lane = threadIdx.x;
+  int njt = G.jpad / 16;
+  int j0 = (tile % njt) * 16, k0 = (tile / njt) * 16;
+  int li = lane & 15, lq = lane >> 4;
+  int k = k0 + li;  // receiver of this lane's output column
+  cd e1 = (k < R.nr) ? eia[k] : C(1.0), e2 = e1 * e1;
+  cd em1 = C(e1.x, -e1.y), em2 = C(e2.x, -e2.y);
+  cd ur[4], ut[4], up[4];
+  for (int v = 0; v < 4; v++) ur[v] = ut[v] = up[v] = C(0.0);
+  // coefficient index c: uu m -> m+2, vv m -> 7+m, ww m -> 12+m
+  lsum_group<0, 0, 1, 2, 0, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // uu0 x
+  lsum_group<0, 1, 4, 3, 1, 13, 11>(G, j0, k0, li, lq, e1, em1, ur, ut, up);   // uu+-1, ww+-1 x
+  lsum_group<0, 2, 4, 4, 0, 14, 10>(G, j0, k0, li, lq, e2, em2, ur, ut, up);   // uu+-2, ww+-2 x
+  lsum_group<1, 0, 1, 7, 0, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // vv0 dx
+  lsum_group<1, 1, 4, 8, 6, 13, 11>(G, j0, k0, li, lq, e1, em1, ur, ut, up);   // vv+-1, ww+-1 dx
+  lsum_group<1, 2, 4, 9, 5, 14, 10>(G, j0, k0, li, lq, e2, em2, ur, ut, up);   // vv+-2, ww+-2 dx
+  lsum_group<2, 1, 2, 8, 6, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // vv+-1 x/sin
+  lsum_group<2, 2, 2, 9, 5, 0, 0>(G, j0, k0, li, lq, e2, em2, ur, ut, up);     // vv+-2 x/sin
+  if (k >= R.nr) return;
+  size_t nwr = (size_t)R.nw * R.nr;
+  for (int v = 0; v < 4; v++) {
+    int j = j0 + lq + 4 * v;
+    if (j >= R.nw) continue;
+    size_t at = (size_t)j + (size_t)R.nw * k;
+    spec[at] = ur[v];
+    spec[nwr + at] = ut[v];
+    spec[2 * nwr + at] = up[v];
+  }
+}
+
+// items (event, 16 w x 16 receiver tile); spec of event e at spec_off,
+// [3][nr][nw]
+__global__ void __launch_bounds__(64) k_lsum(const RunK* __restrict__ runs, Queue Q,
+                                             cd* __restrict__ cf, const double* __restrict__ tb,
+                                             const cd* __restrict__ eia, cd* __restrict__ spec) {
+  RUN_ITEMS(false, Q,
+            lsum_item(runs[it.x], gemm_of(runs[it.x], cf, tb), it.y, eia + runs[it.x].roff,
+                      spec + runs[it.x].spec_off));
+}
+
+// -------------------------------------------------------------- k_legendre
+// X_lm, dX_lm/dtheta, X_lm/sin(theta), |m| <= 2, of event e for degree l =
+// blockIdx.x * 64 + lane and receiver blockIdx.y, into the receiver-sum
+// operand tb[(X * 3 + |m|) * lpad + l][nrpad] (legendre, src/module_function.f90:
+// 6-132, through the host's own arithmetic in ysx_model.hpp).  th holds
+// theta, cos theta, sin theta per receiver.
+__global__ void __launch_bounds__(64) k_legendre(const RunK* __restrict__ runs, int e,
+                                                 const double* __restrict__ th,
+                                                 double* __restrict__ tb_all) {
+  const RunK& R = runs[e];
+  int l = blockIdx.x * 64 + (int)threadIdx.x, k = blockIdx.y;
+  if (l > R.lmax || k >= R.nr) return;
+  const double* t = th + 3 * (size_t)(R.roff + k);
+  int m = l < 2 ? l : 2;
+  double x[3], xp[3], xc[3];
+  legendre_cs(t[0], t[1], t[2], l, m, x, xp, xc);
+  double* tb = tb_all + R.tb_off;
+  for (int a = 0; a <= m; a++) {
+    tb[((size_t)(0 * 3 + a) * R.lpad + l) * R.nrpad + k] = x[a];
+    tb[((size_t)(1 * 3 + a) * R.lpad + l) * R.nrpad + k] = xp[a];
+    tb[((size_t)(2 * 3 + a) * R.lpad + l) * R.nrpad + k] = xc[a];
+  }
+}
+
+// ----------------------------------------------------------------- k_model
+// The deck -> table transform on the device (one block): moduli per knot,
+// natural splines per layer, gravity quadrature with the reference's
+// sequential accumulation (src/module_model.f90:253-378), through the same
+// functions as the host (ysx_model.hpp).  layer_lo (0-based first knot)
+// and layer_n per layer; work holds 3 (nknot + 1) doubles per (layer,
+// field) solve; gi holds nknot doubles.
+__global__ void __launch_bounds__(256) k_model(const double* __restrict__ deck, int nknot,
+                                               int ifanis, double vel_norm,
+                                               const int* __restrict__ layer_lo,
+                                               const int* __restrict__ layer_n, int nlayer,
+                                               double* __restrict__ tab, double* __restrict__ work,
+                                               double* __restrict__ intg,
+                                               double* __restrict__ qint,
+                                               double* __restrict__ gi) {
+#pragma clang fp contract(off)
+  int tid = threadIdx.x, nt = blockDim.x;
+  size_t ws = 3 * ((size_t)nknot + 1);
+  for (int i = tid; i < nknot; i += nt) knot_fields(deck, nknot, i, ifanis, vel_norm, tab);
+  __syncthreads();
+  const double* r = tab + (size_t)F_R * nknot;
+  const double* rho = tab + (size_t)F_RHO * nknot;
+  for (int i = tid; i < nknot; i += nt) intg[i] = r[i] * r[i] * rho[i];
+  __syncthreads();
+  for (int t = tid; t < 13 * nlayer; t += nt) {
+    int k = t / 13, q = t % 13, lo = layer_lo[k], n = layer_n[k];
+    if (q < 12) {
+      int f = spline_field(q);
+      natural_spline(r + lo, tab + (size_t)f * nknot + lo, tab + (size_t)(f + 1) * nknot + lo, n,
+                     work + t * ws);
+    } else {
+      natural_spline(r + lo, intg + lo, qint + lo, n, work + t * ws);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nknot - 1; i += nt) gi[i] = grav_interval(r, intg, qint, i);
+  __syncthreads();
+  double* grav = tab + (size_t)F_GRAV * nknot;
+  if (tid == 0) {
+    double fint = 0.0;
+    grav[0] = 0.0;
+    for (int i = 0; i < nknot - 1; i++) {
+      fint += gi[i];
+      grav[i + 1] = fint;
+    }
+    for (int i = 0; i < nknot; i++)
+      if (r[i] > 0.0) grav[i] = (4.0 * grav[i]) / (r[i] * r[i]);
+  }
+  __syncthreads();
+  for (int k = tid; k < nlayer; k += nt)
+    natural_spline(r + layer_lo[k], grav + layer_lo[k], tab + (size_t)F_GRAV2 * nknot + layer_lo[k],
+                   layer_n[k], work + k * ws);
+}
+
+// ------------------------------------------------------------------ k_seis
+// ifft (src/module_fourier.f90:55-145) and src/yspec.f90:439-473 for one
+// receiver per block: pack bins i1..i2, cosine filters, Hermitian fill,
+// radix-2 DIT inverse FFT with the reference's twiddle angles
+// pi_d*m/mmax (single-rounded pi), /(dt*nt), exp(eps t), units, rotation.
+__device__ inline cd packed_bin(const RunK& K, const cd* spec, int k, int i) {
+  // 0-based FFT bin i of the packed, filtered, Hermitian-filled row
+  bool neg = i > K.nt / 2;
+  int b = neg ? K.nt - i : i;
+  cd v = C(0.0);
+  if (b >= K.i1 - 1 && b <= K.i2 - 1) v = spec[(size_t)(b - (K.i1 - 1)) + (size_t)K.nw * k];
+  double w = b * K.df, f = 1.0;
+  bool nof = (K.f11 == 0.0 && K.f12 == 0.0 && K.f21 == 0.0 && K.f22 == 0.0);
+  if (!nof) {
+    if (w < K.f11) f = 0.0;
+    else if (w < K.f12) f = 0.5 * (1.0 - cos(PI_D * (w - K.f11) / (K.f12 - K.f11)));
+    else if (w < K.f21) f = 1.0;
+    else if (w < K.f22) f = 0.5 * (1.0 - cos(PI_D * (K.f22 - w) / (K.f22 - K.f21)));
+    else f = 0.0;
+  }
+  v = v * f;
+  if (neg) v.y = -v.y;
+  return v;
+}
+
+__device__ inline unsigned brev(unsigned x, int bits) { return __brev(x) >> (32 - bits); }
+
+// items (event, receiver k of the event), static distribution.  Event e's
+// FFT rows are work[seis_off + (3k + c) nt ...] and its seismograms
+// seis[seis_off + i + nt (c + 3k)], (nt, 3, nr) column-major.
+__global__ void __launch_bounds__(1024) k_seis(const RunK* __restrict__ runs, Queue Q,
+                                               const cd* __restrict__ spec_all,
+                                               const cd* __restrict__ tw, cd* __restrict__ work_all,
+                                               const double* __restrict__ azep_all,
+                                               double* __restrict__ seis_all) {
+  for (int it = blockIdx.x; it < Q.n; it += gridDim.x) {
+    int4 item = Q.items[it];
+    const RunK& K = runs[item.x];
+    int k = item.y;
+    int nt = K.nt;
+    const cd* spec = spec_all + K.spec_off;
+    cd* work = work_all + K.seis_off;
+    double* seis = seis_all + K.seis_off;
+    for (int c = 0; c < 3; c++) {
+      const cd* sp = spec + (size_t)c * K.nw * K.nr;
+      cd* row = work + ((size_t)k * 3 + c) * nt;
+      for (int q = threadIdx.x; q < nt; q += blockDim.x)
+        row[q] = packed_bin(K, sp, k, (int)brev((unsigned)q, K.lognt));
+      __syncthreads();
+      for (int mmax = 1; mmax < nt; mmax <<= 1) {
+        for (int b = threadIdx.x; b < nt / 2; b += blockDim.x) {
+          int m = b & (mmax - 1);
+          int i = ((b - m) << 1) + m;
+          int jx = i + mmax;
+          cd temp = tw[mmax + m] * row[jx];
+          cd a = row[i];
+          row[jx] = a - temp;
+          row[i] = a + temp;
+        }
+        __syncthreads();
+      }
+    }
+    double az = azep_all[K.roff + k];
+    double ca = cos(az), sa = sin(az);
+    const cd* r0 = work + ((size_t)k * 3 + 0) * nt;
+    const cd* r1 = work + ((size_t)k * 3 + 1) * nt;
+    const cd* r2 = work + ((size_t)k * 3 + 2) * nt;
+    double sc = K.norm / (K.dt * nt);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+      double g = exp(K.ep * (i * K.dt)) * sc;
+      double z = r0[i].x * g, ut = r1[i].x * g, up = r2[i].x * g;
+      seis[(size_t)i + (size_t)nt * (0 + 3 * (size_t)k)] = z;
+      seis[(size_t)i + (size_t)nt * (1 + 3 * (size_t)k)] = -ut * ca + up * sa;
+      seis[(size_t)i + (size_t)nt * (2 + 3 * (size_t)k)] = ut * sa + up * ca;
+    }
+    __syncthreads();
+  }
+}
+
+// The same transform in two passes for nt <= FFT_B * FFT_G: stages
+// mmax = 1 .. FFT_B/2 stay inside contiguous blocks of FFT_B elements of the
+// bit-reversed row and run in LDS (k_fft_lds); the remaining stages mix
+// only elements i0 + t FFT_B (t = 0 .. nt/FFT_B - 1) and run in registers,
+// followed by exp(eps t), units and rotation (k_fft_post).  Butterflies,
+// twiddles and their order are those of k_seis, so the results are the same
+// bit for bit.
+
+// items (event, receiver k, c << 16 | block); work rows as in k_seis
+__global__ void __launch_bounds__(512) k_fft_lds(const RunK* __restrict__ runs, Queue Q,
+                                                 const cd* __restrict__ spec_all,
+                                                 const cd* __restrict__ tw,
+                                                 cd* __restrict__ work_all) {
+  __shared__ cd buf[FFT_B];
+  for (int it = blockIdx.x; it < Q.n; it += gridDim.x) {
+    int4 item = Q.items[it];
+    const RunK& K = runs[item.x];
+    int k = item.y, c = item.z >> 16, blk = item.z & 0xffff;
+    int nt = K.nt, bk = min(nt, FFT_B);
+    const cd* sp = spec_all + K.spec_off + (size_t)c * K.nw * K.nr;
+    cd* row = work_all + K.seis_off + ((size_t)k * 3 + c) * nt + (size_t)blk * bk;
+    for (int q = threadIdx.x; q < bk; q += blockDim.x)
+      buf[q] = packed_bin(K, sp, k, (int)brev((unsigned)(blk * bk + q), K.lognt));
+    __syncthreads();
+    for (int mmax = 1; mmax < bk; mmax <<= 1) {
+      for (int b = threadIdx.x; b < bk / 2; b += blockDim.x) {
+        int m = b & (mmax - 1);
+        int i = ((b - m) << 1) + m;
+        int jx = i + mmax;
+        cd temp = tw[mmax + m] * buf[jx];
+        cd a = buf[i];
+        buf[jx] = a - temp;
+        buf[i] = a + temp;
+      }
+      __syncthreads();
+    }
+    for (int q = threadIdx.x; q < bk; q += blockDim.x) row[q] = buf[q];
+    __syncthreads();
+  }
+}
+
+template <int G>
+__device__ __forceinline__ void fft_post_g(const RunK& K, int k, int i0, const cd* __restrict__ tw,
+                                           const cd* __restrict__ work, double ca, double sa,
+                                           double* __restrict__ seis) {
+  int nt = K.nt, bk = nt / G;
+  cd v[3][G];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cd* row = work + ((size_t)k * 3 + c) * nt;
+#pragma unroll
+    for (int t = 0; t < G; t++) v[c][t] = row[i0 + t * bk];
+  }
+#pragma unroll
+  for (int step = 1; step < G; step <<= 1) {
+    int mmax = step * bk;
+#pragma unroll
+    for (int t = 0; t < G; t++) {
+      if (t & step) continue;
+      int tj = t + step;
+      cd w = tw[mmax + ((i0 + t * bk) & (mmax - 1))];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        cd temp = w * v[c][tj];
+        cd a = v[c][t];
+        v[c][tj] = a - temp;
+        v[c][t] = a + temp;
+      }
+    }
+  }
+  double sc = K.norm / (K.dt * nt);
+#pragma unroll
+  for (int t = 0; t < G; t++) {
+    int i = i0 + t * bk;
+    double g = exp(K.ep * (i * K.dt)) * sc;
+    double z = v[0][t].x * g, ut = v[1][t].x * g, up = v[2][t].x * g;
+    seis[(size_t)i + (size_t)nt * (0 + 3 * (size_t)k)] = z;
+    seis[(size_t)i + (size_t)nt * (1 + 3 * (size_t)k)] = -ut * ca + up * sa;
+    seis[(size_t)i + (size_t)nt * (2 + 3 * (size_t)k)] = ut * sa + up * ca;
+  }
+}
+
+// items (event, receiver k, block of 256 column indices i0 < min(nt, FFT_B))
+__global__ void __launch_bounds__(256) k_fft_post(const RunK* __restrict__ runs, Queue Q,
+                                                  const cd* __restrict__ tw,
+                                                  const cd* __restrict__ work_all,
+                                                  const double* __restrict__ azep_all,
+                                                  double* __restrict__ seis_all) {
+  for (int it = blockIdx.x; it < Q.n; it += gridDim.x) {
+    int4 item = Q.items[it];
+    const RunK& K = runs[item.x];
+    int k = item.y;
+    int bk = min(K.nt, FFT_B), g = K.nt / bk;
+    int i0 = item.z * 256 + (int)threadIdx.x;
+    if (i0 >= bk) continue;
+    double az = azep_all[K.roff + k];
+    double ca = cos(az), sa = sin(az);
+    const cd* work = work_all + K.seis_off;
+    double* seis = seis_all + K.seis_off;
+    switch (g) {
+      case 1: fft_post_g<1>(K, k, i0, tw, work, ca, sa, seis); break;
+      case 2: fft_post_g<2>(K, k, i0, tw, work, ca, sa, seis); break;
+      case 4: fft_post_g<4>(K, k, i0, tw, work, ca, sa, seis); break;
+      case 8: fft_post_g<8>(K, k, i0, tw, work, ca, sa, seis); break;
+      default: fft_post_g<16>(K, k, i0, tw, work, ca, sa, seis); break;
+    }
+  }
+}
+
+__global__ void k_twiddles(int nt, cd* tw) {
+  // tw[mmax + m] = exp(i pi_d m / mmax), mmax = 1, 2, 4, ..., nt/2
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 1 || t >= nt) return;
+  int mmax = 1 << (31 - __clz(t));
+  int m = t - mmax;
+  double th = PI_D * (double)m / (double)mmax;
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  tw[t] = C(cs, sn);
+}
+
+}  // namespace
+
+void launch_reset(dim3 g, dim3 b, size_t lds, hipStream_t s, int* counters, int n) {
+  hipLaunchKernelGGL(k_reset, g, b, lds, s, counters, n);
+}
+void launch_source(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                   const RunK* runs, PairRange PR, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_source, g, b, lds, s, gtab, runs, PR, rec, cnt);
+}
+void launch_start(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                  const RunK* runs, PairRange PR, StartData SD, int* starts) {
+  hipLaunchKernelGGL(k_start, g, b, lds, s, gtab, runs, PR, SD, starts);
+}
+void launch_rad(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, const RunK* runs,
+                Queue Q, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_rad, g, b, lds, s, gtab, runs, Q, rec, cnt);
+}
+void launch_rad_ng(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                   const RunK* runs, Queue Q, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_rad_ng, g, b, lds, s, gtab, runs, Q, rec, cnt);
+}
+void launch_sph_var(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                    const RunK* runs, Queue Q, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_sph_var, g, b, lds, s, gtab, runs, Q, rec, cnt);
+}
+void launch_assemble(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, PairRange PR,
+                     cd* rec, cd* cf, const double* tb) {
+  hipLaunchKernelGGL(k_assemble, g, b, lds, s, runs, PR, rec, cf, tb);
+}
+void launch_lsum(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q, cd* cf,
+                 const double* tb, const cd* eia, cd* spec) {
+  hipLaunchKernelGGL(k_lsum, g, b, lds, s, runs, Q, cf, tb, eia, spec);
+}
+void launch_legendre(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                     const double* th, double* tb) {
+  hipLaunchKernelGGL(k_legendre, g, b, lds, s, runs, e, th, tb);
+}
+void launch_model(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck, int nknot,
+                  int ifanis, double vel_norm, const int* layer_lo, const int* layer_n,
+                  int nlayer, double* tab, double* work, double* intg, double* qint, double* gi) {
+  hipLaunchKernelGGL(k_model, g, b, lds, s, deck, nknot, ifanis, vel_norm, layer_lo, layer_n,
+                     nlayer, tab, work, intg, qint, gi);
+}
+void launch_seis(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
+                 const cd* spec, const cd* tw, cd* work, const double* azep, double* seis) {
+  hipLaunchKernelGGL(k_seis, g, b, lds, s, runs, Q, spec, tw, work, azep, seis);
+}
+void launch_fft_lds(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
+                    const cd* spec, const cd* tw, cd* work) {
+  hipLaunchKernelGGL(k_fft_lds, g, b, lds, s, runs, Q, spec, tw, work);
+}
+void launch_fft_post(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
+                     const cd* tw, const cd* work, const double* azep, double* seis) {
+  hipLaunchKernelGGL(k_fft_post, g, b, lds, s, runs, Q, tw, work, azep, seis);
+}
+void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw) {
+  hipLaunchKernelGGL(k_twiddles, g, b, lds, s, nt, tw);
+}
+
+}  // namespace ysx

@@ -1,0 +1,252 @@
+// ysx_kern.hpp -- types and device helpers shared by the kernel files of
+// libysx.so and the host API: per-event run data (RunK), work lists and
+// the item loop, pair indexing, per-pair record layout, the state passed
+// between the spheroidal kernels, the receiver-sum operands, and the host
+// launch functions of every kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ysx_device.hpp"
+#include "ysx_march.hpp"
+#include "ysx_model.hpp"
+#include "ysx_variants.hpp"
+
+namespace ysx {
+
+// ------------------------------------------------------- per-event run data
+struct RunK {
+  int lmin, lmax, nw, nr, is, ir, out, cor, npair;
+  int nknot, nic, noc, nsl, fic, ocean;
+  int grav, var;                // gravity switch; var: spheroidal solves per lane
+  int lpad, jpad, nrpad;        // GEMM padding of this event
+  int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
+  long long pbase;              // first pair of this event in the per-pair arrays
+  long long cf_off, tb_off, spec_off, seis_off;
+  double rs, rr, wmin, dw, ep, tref, ats, pibigg, fre_norm, alpha_min, grav_ir;
+  double df, dt, f11, f12, f21, f22, norm;
+  double mm[6];
+};
+
+// A work list: items[0..n) of (event, a, b, 0).  counter == nullptr: block
+// b takes items b, b + gridDim.x, ...; otherwise persistent blocks take
+// `chunk` consecutive items per atomicAdd on *counter (zeroed before the
+// launch by k_reset).  March blocks are one wave: lane 0 takes the items
+// and broadcasts them.
+struct Queue {
+  const int4* items;
+  int n;
+  int* counter;
+  int chunk;
+};
+
+__device__ __forceinline__ int4 uniform_item(const int4* items, int it) {
+  int4 v = items[it];
+  return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                   __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
+
+__device__ __forceinline__ int next_base(const Queue& Q) {
+  int base = 0;
+  if (threadIdx.x == 0) base = atomicAdd(Q.counter, Q.chunk);
+  return __builtin_amdgcn_readfirstlane(base);  // lane 0's value, wave-uniform
+}
+
+// Run ITEM_CALL (a statement using `it`, the current int4 item) over the
+// queue: PERSIST = false: block b runs item b (the hardware dispatches
+// blocks in index order, so a cost-sorted list is drained largest first);
+// PERSIST = true: blocks loop over the queue.  Written as a macro, not a
+// lambda: captured references keep the body's state out of registers.
+#define RUN_ITEMS(PERSIST, Q, ITEM_CALL)                                     \
+  do {                                                                       \
+    if constexpr (!(PERSIST)) {                                              \
+      if ((int)blockIdx.x < (Q).n) {                                         \
+        int4 it = uniform_item((Q).items, blockIdx.x);                       \
+        ITEM_CALL;                                                           \
+      }                                                                      \
+    } else {                                                                 \
+      for (int base_ = next_base(Q); base_ < (Q).n; base_ = next_base(Q)) {  \
+        int end_ = min(base_ + (Q).chunk, (Q).n);                            \
+        for (int k_ = base_; k_ < end_; k_++) {                              \
+          int4 it = uniform_item((Q).items, k_);                             \
+          ITEM_CALL;                                                         \
+        }                                                                    \
+      }                                                                      \
+    }                                                                        \
+  } while (0)
+
+// event of global pair g among events [e0, e1) (pbase ascending)
+__device__ inline int find_event(const RunK* runs, int e0, int e1, long long g) {
+  int lo = e0, hi = e1 - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (runs[mid].pbase <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline Layers layers(const RunK& R) {
+  return Layers{R.nknot, R.nic, R.noc, R.nsl, R.fic, R.ocean};
+}
+
+__device__ inline void pair_lj(const RunK& R, int p, int& l, int& j) {
+  l = R.lmin + p / R.nw;
+  j = p % R.nw + 1;
+}
+
+__device__ inline void pair_state(const RunK& R, int l, int j, PS& s) {
+  set_l(s, l);
+  s.pibigg = R.pibigg;
+  cd w = C(R.wmin + (double)(j - 1) * R.dw, -R.ep);
+  set_omega(s, w, R.tref, R.ats);
+}
+
+// copy the model table into LDS; every thread of the block must call it
+__device__ inline Tab load_tab(const double* __restrict__ g, int nknot) {
+  extern __shared__ double smem[];
+  int n = F_COUNT * nknot;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) smem[i] = g[i];
+  __syncthreads();
+  return Tab{smem, nknot};
+}
+
+// the pairs of events [e0, e1): global pair index of thread t
+struct PairRange {
+  int e0, e1;
+  long long g0, n;
+};
+
+// per-pair output records (AoS, one record of 89 complex numbers per pair)
+constexpr int REC = 89;
+constexpr int O_SVT = 0, O_YTOR = 10, O_SVS = 14, O_YSPH = 44, O_SR0 = 68, O_YRAD = 70,
+              O_UU = 74, O_VV = 79, O_WW = 84;
+
+// ------------------------------------------------ state between the kernels
+// Start levels and starting solutions of every pair, written by k_start:
+// tor_solver's start (src/module_solver.f90:64-75, 103-114) and
+// sph_solver's start (:678-722, 758-785).  The marches pick these up when
+// they reach each lane's start knot.
+struct StartData {
+  int* tor_i1;   // [npair] adjusted start knot
+  cd* tor_y0;    // [npair][2]
+  int* sph_i1;   // [npair] adjusted start knot
+  cd* sph_m0;    // [npair][14] solid minors, or 5 fluid minors + zeros
+};
+
+// What k_minor_top and k_minor_march leave per pair for k_bvec_march
+// (sph_solver, grav = 2, solid inner core, up to the b-vector integration,
+// src/module_solver.f90:640-743, 756-806): the four b-vector starts (bst),
+// the minor vector of sph_ysol (mfin), the factor applied to sph_ysol (fac)
+// and scale3.
+struct SphState {
+  cd* bst;   // [npair][4][14]
+  cd* mfin;  // [npair][14]
+  cd* m2;    // [npair][14] minor vector from the surface
+  cd* fac;   // [npair]
+  int* sc3;  // [npair]
+};
+
+// ------------------------------------------------------- march helpers
+// yint's interval loop (src/module_int.f90:1888-1963) over knot intervals,
+// with wave-uniform arguments
+template <class F>
+__device__ int march_yint(const Tab& T, const WS& w, const LS& L, int nk, int i1, int i2,
+                          double r1, double r2, cd* y, int& sc, double* sb, const F& f,
+                          cd* slot = nullptr) {
+  int n = 0;
+  if (r2 > r1) {
+    for (int i = i1; i <= i2; i++) {
+      double a = (i == i1) ? r1 : T(F_R, i);
+      double b = (i == i2) ? r2 : T(F_R, i + 1);
+      if (a == b) continue;
+      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
+    }
+  } else if (r2 < r1) {
+    for (int i = i1; i >= i2; i--) {
+      double a = (i == i1) ? r1 : T(F_R, i + 1);
+      double b = (i == i2) ? r2 : T(F_R, i);
+      if (a == b) continue;
+      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
+    }
+  }
+  return n;
+}
+
+__device__ inline WS wave_omega(const RunK& R, int j) {
+  return omega_state(C(R.wmin + (double)(j - 1) * R.dw, -R.ep), R.tref, R.ats, R.pibigg);
+}
+
+constexpr int NOLANE = 1 << 30;
+
+// ------------------------------------------------- receiver-sum operands
+// GEMM operands of event e, at offsets cf_off / tb_off of RunK:
+//   cf [15][lpad][jpad] complex: uu m=-2..2, vv m=-2..2, ww m=-2..2
+//   tb [9][lpad][nrpad] real: x, dx/dtheta, x/sin(theta) for |m| = 0,1,2
+struct GemmK {
+  int lpad, jpad, nrpad;
+  cd* cf;
+  const double* tb;
+};
+
+__device__ inline GemmK gemm_of(const RunK& R, cd* cf, const double* tb) {
+  return GemmK{R.lpad, R.jpad, R.nrpad, cf + R.cf_off, tb + R.tb_off};
+}
+
+__device__ inline void put_coefs(const GemmK& G, int l, int j, const cd* o) {
+  size_t plane = (size_t)G.lpad * G.jpad;
+  size_t at = (size_t)l * G.jpad + (j - 1);
+  for (int c = 0; c < 15; c++) G.cf[c * plane + at] = o[O_UU + c];
+}
+
+// inverse FFT: series up to FFT_B points are transformed in LDS, FFT_G
+// points per thread in the post-processing pass
+constexpr int FFT_B = 4096;
+constexpr int FFT_G = 16;
+
+// ---------------------------------------------------------------- launchers
+// Host-side launch functions, one per kernel, defined in the kernel files
+// (ysx_k_march.hip: the marches, _item with one block per work item, _pers
+// with persistent blocks; ysx_k_misc.hip: everything else).  Arguments
+// after the stream are the kernel's.
+void launch_reset(dim3 g, dim3 b, size_t lds, hipStream_t s, int* counters, int n);
+void launch_source(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                   const RunK* runs, PairRange PR, cd* rec, int* cnt);
+void launch_start(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                  const RunK* runs, PairRange PR, StartData SD, int* starts);
+void launch_rad(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, const RunK* runs,
+                Queue Q, cd* rec, int* cnt);
+void launch_rad_ng(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                   const RunK* runs, Queue Q, cd* rec, int* cnt);
+void launch_sph_var(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                    const RunK* runs, Queue Q, cd* rec, int* cnt);
+void launch_assemble(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, PairRange PR,
+                     cd* rec, cd* cf, const double* tb);
+void launch_lsum(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q, cd* cf,
+                 const double* tb, const cd* eia, cd* spec);
+void launch_legendre(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                     const double* th, double* tb);
+void launch_model(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck, int nknot,
+                  int ifanis, double vel_norm, const int* layer_lo, const int* layer_n,
+                  int nlayer, double* tab, double* work, double* intg, double* qint, double* gi);
+void launch_seis(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
+                 const cd* spec, const cd* tw, cd* work, const double* azep, double* seis);
+void launch_fft_lds(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
+                    const cd* spec, const cd* tw, cd* work);
+void launch_fft_post(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
+                     const cd* tw, const cd* work, const double* azep, double* seis);
+void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw);
+#define YSX_MARCH_LAUNCHERS(SUFFIX)                                                              \
+  void launch_tor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
+                                 const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt);   \
+  void launch_minor_top_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
+                                 const RunK* runs, Queue Q, SphState S, int* cnt);              \
+  void launch_minor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s,                   \
+                                   const double* gtab, const RunK* runs, Queue Q, StartData SD, \
+                                   SphState S, int* cnt);                                       \
+  void launch_bvec_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
+                                  const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt);
+YSX_MARCH_LAUNCHERS(item)
+YSX_MARCH_LAUNCHERS(pers)
+#undef YSX_MARCH_LAUNCHERS
+
+}  // namespace ysx

@@ -1,0 +1,291 @@
+// ysx_march_kernels.hpp -- the lockstep march kernels (toroidal, spheroidal
+// minors from the surface and from the start level, b-vector), templated on
+// the work-distribution mode; instantiated one per object file by
+// ysx_k_march.hip.
+#pragma once
+#include "ysx_kern.hpp"
+
+namespace ysx {
+namespace {
+
+// ------------------------------------------------------------- k_tor_march
+// tor_solver with ss = identity(2), src/module_solver.f90:25-142; a wave
+// holds 64 consecutive l at one w, largest w first.
+__device__ __forceinline__ void tor_item(const Tab& T, const RunK& R, int j, int lb,
+                                         const StartData& SD, cd* __restrict__ rec,
+                                         int* __restrict__ cnt, double* sb) {
+  int l = R.lmin + lb * 64 + (int)threadIdx.x;
+  bool valid = (l >= 2 && l <= R.lmax);
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  int i1 = valid ? SD.tor_i1[p] : NOLANE;
+  int ib = wave_min(i1);
+  if (ib == NOLANE) return;
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 2);
+  TorM f;
+  int nk = R.nknot, is = R.is, ir = R.ir;
+  double rs = R.rs, rr = R.rr;
+  bool up = rr >= rs;
+  int itop = up ? is : ir;
+  double rtop = up ? rs : rr;
+  cd y1[2] = {C(0.0), C(0.0)};
+  int sc1 = 0, sc2 = 0, sc3 = 0, ns = 0;
+  for (int i = ib; i <= itop; i++) {
+    if (i == i1) {
+      y1[0] = SD.tor_y0[2 * p];
+      y1[1] = SD.tor_y0[2 * p + 1];
+    }
+    double a = T(F_R, i), b = (i == itop) ? rtop : T(F_R, i + 1);
+    if (a == b) continue;
+    int n = march_interval(T, w, L, nk, i, a, b, y1, true, sc1, sb, f);
+    if (i >= i1) ns += n;
+  }
+  double rsl = T(F_R, R.nsl);
+  cd y2[2] = {C(1.0), C(0.0)}, y3[2], yo[4];
+  if (up) {
+    y3[0] = y1[0];
+    y3[1] = y1[1];
+    if (rr != rs) ns += march_yint(T, w, L, nk, is, ir, rs, rr, y3, sc3, sb, f);
+    if (rr != rsl) ns += march_yint(T, w, L, nk, R.nsl - 1, ir, rsl, rr, y2, sc2, sb, f);
+    cd delta = cinv(y2[1] * y3[0] - y2[0] * y3[1]);
+    cd b1 = y1[1], b2 = -y1[0];
+    yo[0] = -delta * y2[0] * b1;
+    yo[1] = -delta * y2[1] * b1;
+    yo[2] = -delta * y2[0] * b2;
+    yo[3] = -delta * y2[1] * b2;
+  } else {
+    if (rs != rsl) ns += march_yint(T, w, L, nk, R.nsl - 1, is, rsl, rs, y2, sc2, sb, f);
+    y3[0] = y2[0];
+    y3[1] = y2[1];
+    if (rr != rs) ns += march_yint(T, w, L, nk, is, ir, rs, rr, y3, sc3, sb, f);
+    cd delta = cinv(y3[1] * y1[0] - y3[0] * y1[1]);
+    cd b1 = y2[1], b2 = -y2[0];
+    yo[0] = -delta * y1[0] * b1;
+    yo[1] = -delta * y1[1] * b1;
+    yo[2] = -delta * y1[0] * b2;
+    yo[3] = -delta * y1[1] * b2;
+  }
+  if (!valid) return;
+  double sc = ipow_huge(sc3);
+  cnt[5 * p + 0] = ns;
+  cd* o = rec + p * REC + O_YTOR;
+  for (int i = 0; i < 4; i++) o[i] = yo[i] * sc;
+}
+
+// items (event, j, l-block of 64)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_tor_march(const double* __restrict__ gtab,
+                                                  const RunK* __restrict__ runs, Queue Q,
+                                                  StartData SD, cd* __restrict__ rec,
+                                                  int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  RUN_ITEMS(PERSIST, Q,
+            tor_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, rec, cnt, sb));
+}
+
+// ------------------------------------------------ k_minor_top, k_minor_march
+// sph_solver (grav = 2, solid inner core) up to the b-vector integration,
+// src/module_solver.f90:640-743, 756-806, leaving SphState per pair.
+
+// minor vector from the surface down to radius rb in knot interval ib
+// (through the ocean as a fluid when there is one), src/module_solver.f90:730-741
+__device__ __forceinline__ int minors_from_top(const Tab& T, const WS& w, const LS& L, const RunK& R,
+                                      int itopk, int ib, double rb, cd* m, double* sb,
+                                      int& nsol, int& nflu, cd* slot) {
+  int nk = R.nknot, sc2 = 0;
+  for (int i = 0; i < 14; i++) m[i] = C(0.0);
+  m[0] = C(1.0);
+  if (R.ocean) {
+    nflu += march_yint(T, w, L, nk, nk - 1, R.nsl + 1, T(F_R, nk), T(F_R, R.nsl + 1), m, sc2, sb,
+                       FluidMinorM());
+    cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+    fluid_2_solid(mf, m);
+    nsol += march_yint(T, w, L, nk, R.nsl, ib, T(F_R, R.nsl), rb, m, sc2, sb, SolidMinorM(), slot);
+  } else {
+    nsol += march_yint(T, w, L, nk, itopk, ib, T(F_R, nk), rb, m, sc2, sb, SolidMinorM(), slot);
+  }
+  return sc2;
+}
+
+__device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, int lb,
+                                           const StartData& SD, const SphState& S,
+                                           int* __restrict__ cnt, double* sb, cd* slot) {
+  if (R.var) return;  // spheroidal solves of this event run in k_sph_var
+  int l = R.lmin + lb * 64 + (int)threadIdx.x;
+  bool valid = (l >= 1 && l <= R.lmax);
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  int i1 = valid ? SD.sph_i1[p] : NOLANE;
+  int ib = wave_min(i1);
+  if (ib == NOLANE) return;
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 1);
+  int nk = R.nknot, is = R.is, ir = R.ir, nic = R.nic, noc = R.noc;
+  double rs = R.rs, rr = R.rr;
+  bool up = rr >= rs;
+  int itop = up ? is : ir;
+  double rtop = up ? rs : rr;
+  int nsol = 0, nflu = 0;
+  cd m[14];
+  // (2) ms1 from each lane's start level up to the lower point
+  double rt1 = up ? fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2)) : T(F_R, 1);
+  for (int i = 0; i < 14; i++) m[i] = C(0.0);
+  int layout = -1, sc1 = 0;  // -1 none, 0 solid, 1 fluid
+  for (int i = ib; i <= itop; i++) {
+    int region = (i < nic) ? 0 : ((i > nic && i < noc) ? 1 : 0);
+    bool fluid_i = (i > nic && i < noc);
+    if (layout == 0 && fluid_i) {
+      cd mf[5];
+      solid_2_fluid(m, mf);
+      for (int q = 0; q < 14; q++) m[q] = q < 5 ? mf[q] : C(0.0);
+    } else if (layout == 1 && !fluid_i) {
+      cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+      fluid_2_solid(mf, m);
+    }
+    layout = region;
+    if (i == i1)
+      for (int q = 0; q < 14; q++) m[q] = SD.sph_m0[p * 14 + q];
+    double a = (i == 1) ? rt1 : T(F_R, i);
+    double b = (i == itop) ? rtop : T(F_R, i + 1);
+    if (a == b) continue;
+    int n;
+    if (fluid_i)
+      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, FluidMinorM());
+    else
+      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, SolidMinorM(), slot);
+    if (i >= i1) (fluid_i ? nflu : nsol) += n;
+  }
+  const int cols[4] = {0, 1, 3, 4};
+  int sc3 = 0;
+  if (up) {
+    if (valid)
+      for (int c = 0; c < 4; c++) {
+        cd bb[14];
+        b_start_unit(m, cols[c], bb);
+        for (int q = 0; q < 14; q++) S.bst[p * 56 + c * 14 + q] = bb[q];
+      }
+    if (rr > rs) nsol += march_yint(T, w, L, nk, is, ir, rs, rr, m, sc3, sb, SolidMinorM(), slot);
+  } else {
+    if (valid)
+      for (int q = 0; q < 14; q++) S.mfin[p * 14 + q] = m[q];
+    if (rs > rr) nsol += march_yint(T, w, L, nk, ir, is, rr, rs, m, sc3, sb, SolidMinorM(), slot);
+  }
+  if (!valid) return;
+  cd m2[14];
+  for (int q = 0; q < 14; q++) m2[q] = S.m2[p * 14 + q];
+  cd delta = delta_solid(m, m2);
+  if (up) {
+    for (int q = 0; q < 14; q++) S.mfin[p * 14 + q] = m2[q];
+    S.fac[p] = -cinv(delta);
+  } else {
+    cd mdi = -cinv(delta);
+    for (int c = 0; c < 4; c++) {
+      cd bb[14];
+      b_start_unit(m2, cols[c], bb);
+      for (int q = 0; q < 14; q++) S.bst[p * 56 + c * 14 + q] = bb[q] * mdi;
+    }
+    S.fac[p] = C(1.0);
+  }
+  S.sc3[p] = sc3;
+  cnt[5 * p + 1] += nsol;
+  cnt[5 * p + 2] += nflu;
+}
+
+// items (event, j, l-block of 64)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ gtab,
+                                                    const RunK* __restrict__ runs, Queue Q,
+                                                    StartData SD, SphState S,
+                                                    int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  RUN_ITEMS(PERSIST, Q,
+            minor_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, S, cnt, sb,
+                       slot));
+}
+
+// The surface minor vector ms2 of every pair (src/module_solver.f90:730-741,
+// 793-804): from the surface down to the receiver (receiver above the
+// source) or to the source.  Zero steps for a receiver at the surface.
+__device__ __forceinline__ void minor_top_item(const Tab& T, const RunK& R, int j, int lb,
+                                               const SphState& S, int* __restrict__ cnt,
+                                               double* sb, cd* slot) {
+  if (R.var) return;
+  int l = R.lmin + lb * 64 + (int)threadIdx.x;
+  bool valid = (l >= 1 && l <= R.lmax);
+  if (wave_min(valid ? 0 : 1)) return;
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 1);
+  int nsol = 0, nflu = 0;
+  cd m[14];
+  if (R.rr >= R.rs)
+    minors_from_top(T, w, L, R, R.nknot - 1, R.ir, R.rr, m, sb, nsol, nflu, slot);
+  else
+    minors_from_top(T, w, L, R, R.nknot, R.is, R.rs, m, sb, nsol, nflu, slot);
+  if (!valid) return;
+  for (int i = 0; i < 14; i++) S.m2[p * 14 + i] = m[i];
+  cnt[5 * p + 1] = nsol;
+  cnt[5 * p + 2] = nflu;
+}
+
+// items (event, j, l-block of 64)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_minor_top(const double* __restrict__ gtab,
+                                                  const RunK* __restrict__ runs, Queue Q,
+                                                  SphState S, int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  RUN_ITEMS(PERSIST, Q,
+            minor_top_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, S, cnt, sb,
+                           slot));
+}
+
+// ------------------------------------------------------------ k_bvec_march
+// b-vector from source to receiver and y = sph_ysol(m, b) for the four
+// source columns, src/module_solver.f90:745-755, 809-819.  A wave holds 16
+// consecutive l x 4 columns at one w.
+__device__ __forceinline__ void bvec_item(const Tab& T, const RunK& R, int j, int lb,
+                                          const SphState& S, cd* __restrict__ rec,
+                                          int* __restrict__ cnt, double* sb, cd* slot) {
+  if (R.var) return;
+  int c = threadIdx.x & 3;
+  int l = R.lmin + lb * 16 + (int)(threadIdx.x >> 2);
+  bool valid = (l >= 1 && l <= R.lmax);
+  int any = wave_min(valid ? 0 : 1);
+  if (any) return;  // no valid lane in this wave
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 1);
+  cd b[14];
+  for (int q = 0; q < 14; q++) b[q] = valid ? S.bst[p * 56 + c * 14 + q] : C(0.0);
+  int sc4 = 0;
+  int n = march_yint(T, w, L, R.nknot, R.is, R.ir, R.rs, R.rr, b, sc4, sb, BvecM(), slot);
+  if (!valid) return;
+  cd m[14];
+  for (int q = 0; q < 14; q++) m[q] = S.mfin[p * 14 + q];
+  cd y[6];
+  ysol(m, b, y);
+  cd fac = S.fac[p] * ipow_huge(S.sc3[p] - sc4);
+  cd* o = rec + p * REC + O_YSPH + 6 * c;
+  for (int q = 0; q < 6; q++) o[q] = y[q] * fac;
+  if (c == 0) cnt[5 * p + 3] = n;
+}
+
+// items (event, j, l-block of 16); lanes 4 l + c integrate column c of l
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gtab,
+                                                   const RunK* __restrict__ runs, Queue Q,
+                                                   SphState S, cd* __restrict__ rec,
+                                                   int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  RUN_ITEMS(PERSIST, Q,
+            bvec_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, S, rec, cnt, sb,
+                      slot));
+}
+
+}  // namespace
+}  // namespace ysx
