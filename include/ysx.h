@@ -133,6 +133,13 @@ int ysx_event_seismograms(ysx_ctx *ctx, int e, double *seis);
 int ysx_pair_records(ysx_ctx *ctx, int n, const int *l, const int *j,
                      double *rec);
 
+/* Cash-Karp step counts of n pairs (l[i], j[i]) (j 1-based) of the last
+ * ysx_spectra, 5 per pair in the order of ysx_step_counts.  The reference
+ * keeps no such count; its fixed step rule (yint, src/module_int.f90:
+ * 1903-1905) fixes them, so they expose each pair's start level and
+ * integration path. */
+int ysx_pair_steps(ysx_ctx *ctx, int n, const int *l, const int *j, int *out5);
+
 /* Timings of the last ysx_spectra / ysx_seismograms, milliseconds, from
  * HIP events on the context's stream: [0] source vectors + start levels,
  * [1] toroidal, [2] spheroidal minors, [3] b-vector, [4] radial,

@@ -25,7 +25,7 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_set_events", "ysx_event_info", "ysx_forward_events", "ysx_set_batch_tuning",
            "ysx_event_spectra", "ysx_event_seismograms", "ysx_model_table",
            "ysx_model_table_host",
-           "ysx_pair_records", "ysx_timings", "ysx_step_counts")
+           "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -69,6 +69,7 @@ def lib():
         L.ysx_model_table.argtypes = [vp, D]
         L.ysx_model_table_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D]
         L.ysx_pair_records.argtypes = [vp, C.c_int, I, I, D]
+        L.ysx_pair_steps.argtypes = [vp, C.c_int, I, I, I]
         L.ysx_timings.argtypes = [vp, D]
         L.ysx_step_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
         _LIB = L
@@ -173,6 +174,15 @@ class YSpec:
         seis = np.zeros((self.nt, 3, self.p.nr), order="F")
         self._check(self.L.ysx_forward(self.h, _dp(seis)))
         return seis
+
+    def pair_steps(self, ls, js):
+        """(n, 5) step counts: toroidal, solid minors, fluid minors,
+        b-vector, radial."""
+        ls = np.ascontiguousarray(ls, dtype=np.int32)
+        js = np.ascontiguousarray(js, dtype=np.int32)
+        out = np.zeros((len(ls), 5), np.int32)
+        self._check(self.L.ysx_pair_steps(self.h, len(ls), _ip(ls), _ip(js), _ip(out)))
+        return out
 
     def pair_records(self, ls, js):
         ls = np.ascontiguousarray(ls, dtype=np.int32)

@@ -829,6 +829,23 @@ extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j,
   return YSX_OK;
 }
 
+extern "C" int ysx_pair_steps(ysx_ctx* ctx, int n, const int* l, const int* j, int* out5) {
+  if (!ctx || (n > 0 && (!l || !j || !out5))) return YSX_ERR_ARG;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  HIPCHK(hipSetDevice(ctx->device));
+  const RunK& k = ctx->rk[0];
+  size_t P = (size_t)k.npair;
+  std::vector<int> c(P * 5);
+  HIPCHK(hipMemcpy(c.data(), ctx->d_cnt, c.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++) {
+    if (l[i] < k.lmin || l[i] > k.lmax || j[i] < 1 || j[i] > k.nw)
+      return set_err(ctx, YSX_ERR_ARG, "pair outside the run");
+    size_t p = (size_t)(l[i] - k.lmin) * k.nw + (j[i] - 1);
+    for (int q = 0; q < 5; q++) out5[5 * (size_t)i + q] = c[5 * p + q];
+  }
+  return YSX_OK;
+}
+
 extern "C" int ysx_timings(const ysx_ctx* ctx, double* ms8) {
   if (!ctx || !ms8) return YSX_ERR_ARG;
   std::memcpy(ms8, ctx->ms, sizeof ctx->ms);

@@ -7,6 +7,10 @@
 // of the coefficient matrices once per stage radius (one lane per radius)
 // into LDS; each lane then adds its own l-dependent terms and runs its
 // Cash-Karp step.  Control flow is uniform across the wave.
+//
+// The systems are linear (dy/dr = A y), so the base coefficients are stored
+// multiplied by the step length h: the right-hand sides then give h dy/dr
+// directly and the Cash-Karp tableau enters as plain constants.
 #pragma once
 #include "ysx_device.hpp"
 
@@ -28,7 +32,7 @@ __device__ inline cd cmul_(cd a, cd b) {
 }
 
 constexpr int CH = 10;       // steps per shared-coefficient chunk (6*CH radii)
-constexpr int BASE_MAX = 18;  // doubles per stage radius
+constexpr int BASE_MAX = 20;  // doubles per stage radius
 
 // w-dependent constants shared by the wave (set_omega, src/module_int.f90:2091-2102)
 struct WS {
@@ -66,10 +70,10 @@ __device__ inline double step_len_w(const Tab& T, double rew, int i, int nknot) 
 
 // ------------------------------------------------------------ solid region
 // sph_solid_amat (src/module_int.f90:500-560) split into an l-independent
-// base per radius and the l-dependent entries added per lane.
+// base per radius (times h) and the l-dependent entries added per lane.
 struct SolidB {
-  static constexpr int BS = 18;
-  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double* o) {
+  static constexpr int BS = 20;
+  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
     SplW sw = spl_w(T, il, rr);
     double rrho = spl(T, sw, F_RHO), racon = spl(T, sw, F_ACON), rccon = spl(T, sw, F_CCON),
            rfcon = spl(T, sw, F_FCON), rncon = spl(T, sw, F_NCON), rlcon = spl(T, sw, F_LCON),
@@ -90,13 +94,16 @@ struct SolidB {
     cd q12 = ri2 * (rgr - 2.0 * gamma);
     cd a22 = mw2r - (2.0 * ri2) * cncon;
     cd gc = ri2 * (gamma + cncon);
-    o[0] = ri; o[1] = rrho;
-    o[2] = t11.x; o[3] = t11.y; o[4] = cccon.x; o[5] = cccon.y; o[6] = k22.x; o[7] = k22.y;
-    o[8] = s11.x; o[9] = s11.y; o[10] = fcri.x; o[11] = fcri.y; o[12] = q12.x; o[13] = q12.y;
-    o[14] = a22.x; o[15] = a22.y; o[16] = gc.x; o[17] = gc.y;
+    o[0] = h * ri; o[1] = rrho;
+    o[2] = h * t11.x; o[3] = h * t11.y; o[4] = h * cccon.x; o[5] = h * cccon.y;
+    o[6] = h * k22.x; o[7] = h * k22.y; o[8] = h * s11.x; o[9] = h * s11.y;
+    o[10] = h * fcri.x; o[11] = h * fcri.y; o[12] = h * q12.x; o[13] = h * q12.y;
+    o[14] = h * a22.x; o[15] = h * a22.y; o[16] = h * gc.x; o[17] = h * gc.y;
+    o[18] = h * (-4.0 * w.pibigg * rrho); o[19] = h * (4.0 * w.pibigg);
   }
-  __device__ static inline void amat(const double* b, const LS& L, double pib, SolidA& A) {
-    double ri = b[0], rrho = b[1];
+  // h A from the base of one stage radius
+  __device__ static inline void amat(const double* b, const LS& L, SolidA& A) {
+    double hri = b[0], rrho = b[1];
     A.t11 = C(b[2], b[3]);
     A.k11 = C(b[4], b[5]);
     A.k22 = C(b[6], b[7]);
@@ -104,13 +111,13 @@ struct SolidB {
     A.t12 = L.zeta * C(b[10], b[11]);
     A.s12 = L.zeta * C(b[12], b[13]);
     A.s22 = C(b[14], b[15]) + L.zeta2 * C(b[16], b[17]);
-    A.t21 = -L.zeta * ri;
-    A.t22 = 2.0 * ri;
-    A.t31 = -4.0 * pib * rrho;
-    A.t33 = -L.lld * ri;
-    A.k33 = 4.0 * pib;
-    A.s13 = -rrho * L.llp1 * ri;
-    A.s23 = rrho * L.zeta * ri;
+    A.t21 = -L.zeta * hri;
+    A.t22 = 2.0 * hri;
+    A.t31 = b[18];
+    A.t33 = -L.lld * hri;
+    A.k33 = b[19];
+    A.s13 = -rrho * L.llp1 * hri;
+    A.s23 = rrho * L.zeta * hri;
   }
 };
 
@@ -205,7 +212,7 @@ struct SolidMinorM : SolidB {
   __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
                                 Sink&& sink) const {
     SolidA A;
-    amat(b, L, w.pibigg, A);
+    amat(b, L, A);
     solid_minor_derivs(A, y, sink);
   }
   __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
@@ -219,7 +226,7 @@ struct BvecM : SolidB {
   __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
                                 Sink&& sink) const {
     SolidA A;
-    amat(b, L, w.pibigg, A);
+    amat(b, L, A);
     bvec_derivs(A, y, sink);
   }
   __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
@@ -230,10 +237,11 @@ struct BvecM : SolidB {
 
 // ------------------------------------------------------------ fluid region
 // sph_fluid_amat + sph_minors_fluid_derivs, src/module_int.f90:676-715,
-// 941-967, split into an l-independent base and l-dependent entries.
+// 941-967, split into an l-independent base (times h) and l-dependent
+// entries.
 struct FluidMinorM {
-  static constexpr int N = 5, BS = 12;
-  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double* o) {
+  static constexpr int N = 5, BS = 14;
+  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
     SplW sw = spl_w(T, il, rr);
     double rrho = spl(T, sw, F_RHO), rccon = spl(T, sw, F_CCON), gg = spl(T, sw, F_GRAV),
            rxa2 = spl(T, sw, F_XA2);
@@ -243,21 +251,24 @@ struct FluidMinorM {
     cd fa = gg * fb;           // gg ri^2 / w^2
     cd s11a = (rrho * gg) * fa;
     cd s11b = (-4.0 * rrho * gg * ri) - rrho * w.omega2;
-    o[0] = ri; o[1] = rrho;
-    o[2] = fa.x; o[3] = fa.y; o[4] = fb.x; o[5] = fb.y; o[6] = cccon.x; o[7] = cccon.y;
-    o[8] = s11a.x; o[9] = s11a.y; o[10] = s11b.x; o[11] = s11b.y;
+    o[0] = h * ri; o[1] = rrho;
+    o[2] = h * fa.x; o[3] = h * fa.y; o[4] = h * fb.x; o[5] = h * fb.y;
+    o[6] = h * cccon.x; o[7] = h * cccon.y; o[8] = h * s11a.x; o[9] = h * s11a.y;
+    o[10] = h * s11b.x; o[11] = h * s11b.y;
+    o[12] = h * (-4.0 * w.pibigg * rrho); o[13] = h * (4.0 * w.pibigg);
   }
+  // h dm/dr
   __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* m,
                                     cd* d) const {
-    double ri = b[0], rrho = b[1];
+    double ri = b[0], rrho = b[1];  // ri: h / r
     cd fa = C(b[2], b[3]), fb = C(b[4], b[5]), cccon = C(b[6], b[7]);
     double z2 = L.zeta2;
     cd t11 = z2 * fa - ri;
     cd t12 = z2 * fb;
-    double t21 = -4.0 * w.pibigg * rrho;
+    double t21 = b[12];
     double t22 = -L.lld * ri;
     cd k11 = cccon - (z2 / rrho) * fb;
-    double k22 = 4.0 * w.pibigg;
+    double k22 = b[13];
     cd s11 = z2 * C(b[8], b[9]) + C(b[10], b[11]);
     cd s12 = (z2 * rrho) * fa - L.llp1 * rrho * ri;
     cd s22 = (z2 * rrho) * fb;
@@ -271,10 +282,10 @@ struct FluidMinorM {
 };
 
 // ---------------------------------------------------------------- toroidal
-// tor_amat + tor_derivs, src/module_int.f90:134-184
+// tor_amat + tor_derivs, src/module_int.f90:134-184; base times h
 struct TorM {
   static constexpr int N = 2, BS = 7;
-  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double* o) {
+  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
     SplW sw = spl_w(T, il, rr);
     double rrho = spl(T, sw, F_RHO), rlcon = spl(T, sw, F_LCON), rncon = spl(T, sw, F_NCON),
            rqmu = spl(T, sw, F_QMU);
@@ -283,7 +294,8 @@ struct TorM {
     cd k = cinv(rlcon * qll);
     cd sv0 = (ri * ri * rncon) * qll;
     cd mw = -rrho * w.omega2;
-    o[0] = ri; o[1] = k.x; o[2] = k.y; o[3] = sv0.x; o[4] = sv0.y; o[5] = mw.x; o[6] = mw.y;
+    o[0] = h * ri; o[1] = h * k.x; o[2] = h * k.y; o[3] = h * sv0.x; o[4] = h * sv0.y;
+    o[5] = h * mw.x; o[6] = h * mw.y;
   }
   __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
                                     cd* d) const {
@@ -297,40 +309,42 @@ struct TorM {
 
 // ----------------------------------------------------- Cash-Karp, shared base
 // rkck (src/module_int.f90:2016-2064) with the six stage coefficient sets
-// read from the wave's LDS chunk (stride F::BS doubles per stage).
+// read from the wave's LDS chunk (stride F::BS doubles per stage); f gives
+// h dy/dr, so the tableau (src/module_int.f90:2039-2049) enters unscaled.
+namespace ck {
+constexpr double b21 = 0.2, b31 = 3.0 / 40.0, b32 = 9.0 / 40.0, b41 = 0.3, b42 = -0.9,
+                 b43 = 1.2, b51 = -11.0 / 54.0, b52 = 2.5, b53 = -70.0 / 27.0,
+                 b54 = 35.0 / 27.0, b61 = 1631.0 / 55296.0, b62 = 175.0 / 512.0,
+                 b63 = 575.0 / 13824.0, b64 = 44275.0 / 110592.0, b65 = 253.0 / 4096.0,
+                 c1 = 37.0 / 378.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0,
+                 c6 = 512.0 / 1771.0;
+}  // namespace ck
+
 template <class F>
-__device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, double h, cd* y,
-                                 const F& f) {
+__device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd* y, const F& f) {
+  using namespace ck;
   constexpr int N = F::N, BS = F::BS;
-  // Cash-Karp tableau (src/module_int.f90:2039-2049), pre-multiplied by h
-  const double hb21 = h * 0.2, hb31 = h * (3.0 / 40.0), hb32 = h * (9.0 / 40.0),
-               hb41 = h * 0.3, hb42 = h * -0.9, hb43 = h * 1.2, hb51 = h * (-11.0 / 54.0),
-               hb52 = h * 2.5, hb53 = h * (-70.0 / 27.0), hb54 = h * (35.0 / 27.0),
-               hb61 = h * (1631.0 / 55296.0), hb62 = h * (175.0 / 512.0),
-               hb63 = h * (575.0 / 13824.0), hb64 = h * (44275.0 / 110592.0),
-               hb65 = h * (253.0 / 4096.0), hc1 = h * (37.0 / 378.0), hc3 = h * (250.0 / 621.0),
-               hc4 = h * (125.0 / 594.0), hc6 = h * (512.0 / 1771.0);
   cd k1[N], k2[N], k3[N], t[N];
   f(sb, L, w, y, k1);
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = rmad(y[i], hb21, k1[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(y[i], b21, k1[i]);
   __builtin_amdgcn_sched_barrier(0);
   f(sb + BS, L, w, t, k2);
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], hb31, k1[i]), hb32, k2[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], b31, k1[i]), b32, k2[i]);
   __builtin_amdgcn_sched_barrier(0);
   f(sb + 2 * BS, L, w, t, k3);
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = rmad(rmad(rmad(y[i], hb41, k1[i]), hb42, k2[i]), hb43, k3[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(rmad(rmad(y[i], b41, k1[i]), b42, k2[i]), b43, k3[i]);
   {
     cd k4[N];
     __builtin_amdgcn_sched_barrier(0);
     f(sb + 3 * BS, L, w, t, k4);
 #pragma unroll
     for (int i = 0; i < N; i++) {
-      cd t5 = rmad(rmad(rmad(rmad(y[i], hb51, k1[i]), hb52, k2[i]), hb53, k3[i]), hb54, k4[i]);
-      cd p6 = rmad(rmad(rmad(rmul(hb61, k1[i]), hb62, k2[i]), hb63, k3[i]), hb64, k4[i]);
-      cd po = rmad(rmad(rmul(hc1, k1[i]), hc3, k3[i]), hc4, k4[i]);
+      cd t5 = rmad(rmad(rmad(rmad(y[i], b51, k1[i]), b52, k2[i]), b53, k3[i]), b54, k4[i]);
+      cd p6 = rmad(rmad(rmad(rmad(y[i], b61, k1[i]), b62, k2[i]), b63, k3[i]), b64, k4[i]);
+      cd po = rmad(rmad(rmad(y[i], c1, k1[i]), c3, k3[i]), c4, k4[i]);
       t[i] = t5;
       k2[i] = p6;
       k1[i] = po;
@@ -339,72 +353,66 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, dou
   __builtin_amdgcn_sched_barrier(0);
   f(sb + 4 * BS, L, w, t, k3);  // k5
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = rmad(y[i] + k2[i], hb65, k3[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(k2[i], b65, k3[i]);
   __builtin_amdgcn_sched_barrier(0);
   f(sb + 5 * BS, L, w, t, k3);  // k6
 #pragma unroll
-  for (int i = 0; i < N; i++) y[i] = rmad(y[i] + k1[i], hc6, k3[i]);
+  for (int i = 0; i < N; i++) y[i] = rmad(k1[i], c6, k3[i]);
 }
 
 // Cash-Karp step with two stage vectors kept in per-lane LDS slots.
 // slot points at this lane's first slot entry; entry (s, i) is at
-// slot[(s * N + i) * 64] (lane-contiguous, conflict-free).  VGPRs hold only
-// y, the stage input t and one accumulator u; every RHS output is consumed
-// as soon as it is produced.  Same arithmetic as rk_step_b.
+// slot[(s * N + i) * 64] (lane-contiguous, conflict-free).  VGPRs hold y
+// (until stage 3), the stage input t and one accumulator u; every RHS
+// output is consumed as soon as it is produced.  After stage 3 the
+// accumulators of stage 5, stage 6 and the output hold y plus their sums,
+// so y is not kept.  Same arithmetic as rk_step_b.
 template <class F>
-__device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, double h, cd* y,
-                                   const F& f, cd* slot) {
+__device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, cd* y, const F& f,
+                                   cd* slot) {
+  using namespace ck;
   constexpr int N = F::N, BS = F::BS;
-  const double hb21 = h * 0.2, hb31 = h * (3.0 / 40.0), hb32 = h * (9.0 / 40.0),
-               hb41 = h * 0.3, hb42 = h * -0.9, hb43 = h * 1.2, hb51 = h * (-11.0 / 54.0),
-               hb52 = h * 2.5, hb53 = h * (-70.0 / 27.0), hb54 = h * (35.0 / 27.0),
-               hb61 = h * (1631.0 / 55296.0), hb62 = h * (175.0 / 512.0),
-               hb63 = h * (575.0 / 13824.0), hb64 = h * (44275.0 / 110592.0),
-               hb65 = h * (253.0 / 4096.0), hc1 = h * (37.0 / 378.0), hc3 = h * (250.0 / 621.0),
-               hc4 = h * (125.0 / 594.0), hc6 = h * (512.0 / 1771.0);
   cd* S1 = slot;
   cd* S2 = slot + N * 64;
   cd t[N], u[N];
-  // stage 1: k1 -> S1, t = y + h b21 k1
+  // stage 1: k1 -> S1, t = y + b21 k1
   f.stream(sb, L, w, y, [&](int i, cd d) {
     S1[i * 64] = d;
-    t[i] = rmad(y[i], hb21, d);
+    t[i] = rmad(y[i], b21, d);
   });
   __builtin_amdgcn_sched_barrier(0);
   // stage 2: k2 -> u
   f.stream(sb + BS, L, w, t, [&](int i, cd d) { u[i] = d; });
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], hb31, S1[i * 64]), hb32, u[i]);
+  for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], b31, S1[i * 64]), b32, u[i]);
   __builtin_amdgcn_sched_barrier(0);
-  // stage 3: k3 -> S2; then t = t4, S1 = A5, S2 = A6, u = Aout
+  // stage 3: k3 -> S2; then t = t4, S1 = y + A5, S2 = y + A6, u = y + Aout
   f.stream(sb + 2 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = d; });
 #pragma unroll
   for (int i = 0; i < N; i++) {
     cd k1 = S1[i * 64], k2 = u[i], k3 = S2[i * 64];
-    t[i] = rmad(rmad(rmad(y[i], hb41, k1), hb42, k2), hb43, k3);
-    S1[i * 64] = rmad(rmad(rmul(hb51, k1), hb52, k2), hb53, k3);
-    S2[i * 64] = rmad(rmad(rmul(hb61, k1), hb62, k2), hb63, k3);
-    u[i] = rmad(rmul(hc1, k1), hc3, k3);
+    t[i] = rmad(rmad(rmad(y[i], b41, k1), b42, k2), b43, k3);
+    S1[i * 64] = rmad(rmad(rmad(y[i], b51, k1), b52, k2), b53, k3);
+    S2[i * 64] = rmad(rmad(rmad(y[i], b61, k1), b62, k2), b63, k3);
+    u[i] = rmad(rmad(y[i], c1, k1), c3, k3);
   }
   __builtin_amdgcn_sched_barrier(0);
-  // stage 4: fold k4 into A5, A6, Aout
+  // stage 4: fold k4 into the three accumulators; t = t5
   f.stream(sb + 3 * BS, L, w, t, [&](int i, cd d) {
-    S1[i * 64] = rmad(S1[i * 64], hb54, d);
-    S2[i * 64] = rmad(S2[i * 64], hb64, d);
-    u[i] = rmad(u[i], hc4, d);
+    S1[i * 64] = rmad(S1[i * 64], b54, d);
+    S2[i * 64] = rmad(S2[i * 64], b64, d);
+    u[i] = rmad(u[i], c4, d);
   });
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = y[i] + S1[i * 64];
+  for (int i = 0; i < N; i++) t[i] = S1[i * 64];
   __builtin_amdgcn_sched_barrier(0);
-  // stage 5: fold k5 into A6
-  f.stream(sb + 4 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = rmad(S2[i * 64], hb65, d); });
+  // stage 5: fold k5 into the stage-6 input
+  f.stream(sb + 4 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = rmad(S2[i * 64], b65, d); });
 #pragma unroll
-  for (int i = 0; i < N; i++) t[i] = y[i] + S2[i * 64];
+  for (int i = 0; i < N; i++) t[i] = S2[i * 64];
   __builtin_amdgcn_sched_barrier(0);
-  // stage 6: fold k6 into Aout, then y += Aout
-  f.stream(sb + 5 * BS, L, w, t, [&](int i, cd d) { u[i] = rmad(u[i], hc6, d); });
-#pragma unroll
-  for (int i = 0; i < N; i++) y[i] = y[i] + u[i];
+  // stage 6: y = y + Aout + c6 k6
+  f.stream(sb + 5 * BS, L, w, t, [&](int i, cd d) { y[i] = rmad(u[i], c6, d); });
 }
 
 // One knot interval of a march: the interval's equal steps from ra to rb
@@ -429,14 +437,14 @@ __device__ int march_interval(const Tab& T, const WS& w, const LS& L, int nknot,
     if (q < 6 * nst) {
       int jj = j0 + q / 6, k = q % 6;
       double x = ra + (double)(jj - 1) * drr;
-      F::base(T, w, i, x + stage_offset(k) * drr, sbase + q * F::BS);
+      F::base(T, w, i, x + stage_offset(k) * drr, drr, sbase + q * F::BS);
     }
     __syncthreads();
     for (int jj = 0; jj < nst; jj++) {
       if constexpr (F::N == 14) {
-        rk_step_lds(sbase + jj * 6 * F::BS, L, w, drr, y, f, slot);
+        rk_step_lds(sbase + jj * 6 * F::BS, L, w, y, f, slot);
       } else {
-        rk_step_b(sbase + jj * 6 * F::BS, L, w, drr, y, f);
+        rk_step_b(sbase + jj * 6 * F::BS, L, w, y, f);
       }
       if (scaled) renorm<F::N>(y, scale);
     }
