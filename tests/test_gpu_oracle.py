@@ -88,3 +88,37 @@ def test_cfg2_repeatable(cfg2):
     b = g.forward()
     assert np.array_equal(a, b)
     assert np.all(np.isfinite(a))
+
+
+def test_cfg2_pair_steps(cfg2, api):
+    """Per-pair step counts: they add up to the run's totals; l = 0 pairs
+    take only radial steps; every l >= 1 pair at one w takes the b-vector
+    steps that the reference's step rule gives from the source up to the
+    receiver (yint, src/module_int.f90:1903-1907: 1 + floor(dr / drr)
+    steps per knot interval, drr = 1 / (4 Re w / max(alpha_i, alpha_i+1)),
+    steps, src/module_int.f90:2121-2138), restated here in numpy."""
+    deck, p, g = cfg2
+    g.spectra(copy=False)
+    nl = p.lmax - p.lmin + 1
+    ls = np.repeat(np.arange(p.lmin, p.lmax + 1), g.nw)
+    js = np.tile(np.arange(1, g.nw + 1), nl)
+    c = g.pair_steps(ls, js).reshape(nl, g.nw, 5)
+    assert np.array_equal(c.reshape(-1, 5).sum(0, dtype=np.int64), g.step_counts())
+    assert (c[0, :, :4] == 0).all() and (c[0, :, 4] > 0).all()
+    assert (c[1:, :, 4] == 0).all() and (c[1:, :, 3] > 0).all()
+    tab = g.model_table()
+    r, alpha = tab[0], tab[api.TABLE_FIELDS.index("alpha")]
+    nk = len(r)
+    assert g.rr >= g.rs
+    want = np.zeros(g.nw, np.int64)
+    for jj in range(g.nw):
+        rew = g.wmin + jj * g.dw
+        for i in range(g.is_, g.ir + 1):  # 1-based knot intervals
+            a = g.rs if i == g.is_ else r[i - 1]
+            b = g.rr if i == g.ir else r[i]
+            if a == b:
+                continue
+            tmp = max(alpha[i - 1], alpha[i]) if i < nk else alpha[i - 1]
+            drr = 1.0 / (4.0 * rew / tmp)
+            want[jj] += 1 + int(np.floor(abs(b - a) / drr))
+    assert np.array_equal(c[1:, :, 3], np.broadcast_to(want, (nl - 1, g.nw)))

@@ -153,21 +153,15 @@ template <class F>
 __device__ int march_yint(const Tab& T, const WS& w, const LS& L, int nk, int i1, int i2,
                           double r1, double r2, cd* y, int& sc, double* sb, const F& f,
                           cd* slot = nullptr) {
+  // one call site of march_interval for both directions
   int n = 0;
-  if (r2 > r1) {
-    for (int i = i1; i <= i2; i++) {
-      double a = (i == i1) ? r1 : T(F_R, i);
-      double b = (i == i2) ? r2 : T(F_R, i + 1);
-      if (a == b) continue;
-      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
-    }
-  } else if (r2 < r1) {
-    for (int i = i1; i >= i2; i--) {
-      double a = (i == i1) ? r1 : T(F_R, i + 1);
-      double b = (i == i2) ? r2 : T(F_R, i);
-      if (a == b) continue;
-      n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
-    }
+  if (r2 == r1) return 0;
+  const int dir = (r2 > r1) ? 1 : -1, lo = (dir > 0) ? 0 : 1;
+  for (int i = i1; (i - i2) * dir <= 0; i += dir) {
+    double a = (i == i1) ? r1 : T(F_R, i + lo);
+    double b = (i == i2) ? r2 : T(F_R, i + 1 - lo);
+    if (a == b) continue;
+    n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
   }
   return n;
 }

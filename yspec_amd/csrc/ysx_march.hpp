@@ -360,13 +360,15 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd*
   for (int i = 0; i < N; i++) y[i] = rmad(k1[i], c6, k3[i]);
 }
 
-// Cash-Karp step with two stage vectors kept in per-lane LDS slots.
+// Cash-Karp step with stage vectors kept in per-lane LDS slots.
 // slot points at this lane's first slot entry; entry (s, i) is at
-// slot[(s * N + i) * 64] (lane-contiguous, conflict-free).  VGPRs hold y
-// (until stage 3), the stage input t and one accumulator u; every RHS
-// output is consumed as soon as it is produced.  After stage 3 the
-// accumulators of stage 5, stage 6 and the output hold y plus their sums,
-// so y is not kept.  Same arithmetic as rk_step_b.
+// slot[(s * N + i) * 64] (lane-contiguous, conflict-free).  Three register
+// vectors (y, the stage input t, one accumulator u) and two LDS vectors
+// (k1, then y + the stage-6 sum) hold the step; every RHS output is consumed
+// as soon as it is produced.  After stage 3, y is dead and its registers
+// accumulate the stage-5 input; the stage-6 input is formed in t and the
+// result in y, so each component takes 9 LDS accesses per step.  Same
+// arithmetic as rk_step_b.
 template <class F>
 __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, cd* y, const F& f,
                                    cd* slot) {
@@ -386,30 +388,26 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
 #pragma unroll
   for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], b31, S1[i * 64]), b32, u[i]);
   __builtin_amdgcn_sched_barrier(0);
-  // stage 3: k3 -> S2; then t = t4, S1 = y + A5, S2 = y + A6, u = y + Aout
+  // stage 3: k3 -> S2; then t = t4, y = y + A5, S2 = y + A6, u = y + Aout
   f.stream(sb + 2 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = d; });
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    cd k1 = S1[i * 64], k2 = u[i], k3 = S2[i * 64];
-    t[i] = rmad(rmad(rmad(y[i], b41, k1), b42, k2), b43, k3);
-    S1[i * 64] = rmad(rmad(rmad(y[i], b51, k1), b52, k2), b53, k3);
-    S2[i * 64] = rmad(rmad(rmad(y[i], b61, k1), b62, k2), b63, k3);
-    u[i] = rmad(rmad(y[i], c1, k1), c3, k3);
+    cd k1 = S1[i * 64], k2 = u[i], k3 = S2[i * 64], yi = y[i];
+    t[i] = rmad(rmad(rmad(yi, b41, k1), b42, k2), b43, k3);
+    y[i] = rmad(rmad(rmad(yi, b51, k1), b52, k2), b53, k3);
+    S2[i * 64] = rmad(rmad(rmad(yi, b61, k1), b62, k2), b63, k3);
+    u[i] = rmad(rmad(yi, c1, k1), c3, k3);
   }
   __builtin_amdgcn_sched_barrier(0);
-  // stage 4: fold k4 into the three accumulators; t = t5
+  // stage 4: fold k4 into the three accumulators; y is now the stage-5 input
   f.stream(sb + 3 * BS, L, w, t, [&](int i, cd d) {
-    S1[i * 64] = rmad(S1[i * 64], b54, d);
+    y[i] = rmad(y[i], b54, d);
     S2[i * 64] = rmad(S2[i * 64], b64, d);
     u[i] = rmad(u[i], c4, d);
   });
-#pragma unroll
-  for (int i = 0; i < N; i++) t[i] = S1[i * 64];
   __builtin_amdgcn_sched_barrier(0);
-  // stage 5: fold k5 into the stage-6 input
-  f.stream(sb + 4 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = rmad(S2[i * 64], b65, d); });
-#pragma unroll
-  for (int i = 0; i < N; i++) t[i] = S2[i * 64];
+  // stage 5: t = stage-6 input
+  f.stream(sb + 4 * BS, L, w, y, [&](int i, cd d) { t[i] = rmad(S2[i * 64], b65, d); });
   __builtin_amdgcn_sched_barrier(0);
   // stage 6: y = y + Aout + c6 k6
   f.stream(sb + 5 * BS, L, w, t, [&](int i, cd d) { y[i] = rmad(u[i], c6, d); });

@@ -163,12 +163,13 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
         b_start_unit(m, cols[c], bb);
         for (int q = 0; q < 14; q++) S.bst[p * 56 + c * 14 + q] = bb[q];
       }
-    if (rr > rs) nsol += march_yint(T, w, L, nk, is, ir, rs, rr, m, sc3, sb, SolidMinorM(), slot);
   } else {
     if (valid)
       for (int q = 0; q < 14; q++) S.mfin[p * 14 + q] = m[q];
-    if (rs > rr) nsol += march_yint(T, w, L, nk, ir, is, rr, rs, m, sc3, sb, SolidMinorM(), slot);
   }
+  // on from the lower point to the upper one (one call site for both cases)
+  nsol += march_yint(T, w, L, nk, up ? is : ir, up ? ir : is, rtop, up ? rr : rs, m, sc3, sb,
+                     SolidMinorM(), slot);
   if (!valid) return;
   cd m2[14];
   for (int q = 0; q < 14; q++) m2[q] = S.m2[p * 14 + q];
