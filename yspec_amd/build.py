@@ -22,13 +22,15 @@ FDIR = os.path.join(HERE, "fortran")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("YSX_FC", "/opt/rocm/bin/amdflang")
 
-# (source, object, defines): the march kernels dominate compile time, so
+# (source, object, extra flags): the march kernels dominate compile time, so
 # ysx_k_march.hip is compiled once per kernel and work-distribution mode and
 # all objects compile in parallel.
 SOURCES = [("ysx_k_march.hip", f"ysx_k_march_{k}{p}.o", [f"-DYSX_MARCH={k}", f"-DYSX_PERSIST={p}"])
            for k in range(4) for p in range(2)]
-SOURCES += [("ysx_k_misc.hip", "ysx_k_misc.o", []), ("ysx_api.hip", "ysx_api.o", []),
-            ("ysx_host.cpp", "ysx_host.o", [])]
+# The receiver-sum MFMAs accumulate in VGPRs: with the default AGPR form the
+# loop-carried accumulators are copied between AGPRs and VGPRs every step.
+SOURCES += [("ysx_k_misc.hip", "ysx_k_misc.o", ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]),
+            ("ysx_api.hip", "ysx_api.o", []), ("ysx_host.cpp", "ysx_host.o", [])]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]

@@ -563,7 +563,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
     launch_assemble(dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec,
                        ctx->d_cf, ctx->d_tb);
     if (timing) HIPCHK(hipEventRecord(ev[7], st));
-    launch_lsum(dim3(grid_of(ql)), b64, 0, st, ctx->d_runs, ql, ctx->d_cf,
+    launch_lsum(dim3(grid_of(ql)), dim3(256), 0, st, ctx->d_runs, ql, ctx->d_cf,
                        ctx->d_tb, ctx->d_eia, ctx->d_spec);
     if (timing) HIPCHK(hipEventRecord(ev[8], st));
     HIPCHK(hipGetLastError());
@@ -657,7 +657,7 @@ int enqueue_batched(ysx_ctx* ctx) {
   HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
   launch_assemble(dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec, ctx->d_cf,
                      ctx->d_tb);
-  launch_lsum(dim3(std::max(1, ql.n)), b64, 0, st, ctx->d_runs, ql, ctx->d_cf,
+  launch_lsum(dim3(std::max(1, ql.n)), dim3(256), 0, st, ctx->d_runs, ql, ctx->d_cf,
                      ctx->d_tb, ctx->d_eia, ctx->d_spec);
   enqueue_fft(ctx, qs, batched_q(ctx, W_FFTA, -1), batched_q(ctx, W_FFTB, -1), 1 << 30);
   HIPCHK(hipGetLastError());

@@ -420,15 +420,28 @@ __global__ void __launch_bounds__(256) k_assemble(const RunK* __restrict__ runs,
 // GEMMs: for each Legendre table X in {x, dx/dtheta, x/sin(theta)} and
 // |m| = a, C[j][k] = sum_l coef(l, w_j) X_a(l, theta_k) over the coefficient
 // sets that table multiplies; the epilogue applies (-1)^a for m < 0, the
-// factors i*m and exp(i m phi_k) and accumulates ur, ut, up.  One wave owns a
-// 16 (w) x 16 (receiver) tile; v_mfma_f64_16x16x4f64 lane layout: A[i][k] at
-// lane i + 16k, B[k][n] at lane n + 16k, D[i][n] at lane n + 16(i%4), reg i/4.
+// factors i*m and exp(i m phi_k) and accumulates ur, ut, up.  A block of
+// LSUM_WAVES waves owns a 16 (w) x 16 (receiver) tile; wave q sums the
+// degrees l0 = 4 (q + LSUM_WAVES i) .. l0 + 3 and applies the (linear)
+// epilogue to its partial sums, which one LDS reduction then adds.
+// v_mfma_f64_16x16x4f64 lane layout: A[i][k] at lane i + 16k, B[k][n] at
+// lane n + 16k, D[i][n] at lane n + 16(i%4), reg i/4.
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // one (table, |m|) group: GEMM over l, then the epilogue for this group
+constexpr int LSUM_WAVES = 4;
+
+// An empty asm that claims to modify the running sums: the epilogue of one
+// group then stays ahead of the next group's loop instead of being sunk to
+// the end, which would keep every group's accumulators alive at once.
+__device__ __forceinline__ void pin_sums(cd* o) {
+#pragma unroll
+  for (int v = 0; v < 12; v++) asm volatile("" : "+v"(o[v].x), "+v"(o[v].y));
+}
+
 template <int X, int A, int NC, int C0, int C1, int C2, int C3>
 __device__ __forceinline__ void lsum_group(const GemmK& G, int j0, int k0, int li, int lq,
-                                           cd ep, cd en, cd* ur, cd* ut, cd* up) {
+                                           int wave, cd ep, cd en, cd* ur, cd* ut, cd* up) {
   constexpr int cs[4] = {C0, C1, C2, C3};
   const size_t plane = (size_t)G.lpad * G.jpad;
   d4 acc[NC][2];
@@ -436,17 +449,31 @@ __device__ __forceinline__ void lsum_group(const GemmK& G, int j0, int k0, int l
   for (int q = 0; q < NC; q++) acc[q][0] = acc[q][1] = (d4){0.0, 0.0, 0.0, 0.0};
   const double* tb = G.tb + (size_t)(X * 3 + A) * G.lpad * G.nrpad + k0 + li;
   const cd* cf = G.cf + (size_t)lq * G.jpad + j0 + li;
-  for (int l0 = 0; l0 < G.lpad; l0 += 4) {
-    double b = tb[(size_t)(l0 + lq) * G.nrpad];
-    size_t at = (size_t)l0 * G.jpad;
-    cd av[NC];
+  // this wave's degrees l0 = 4 wave + 16 i; the loads of the next l0 are
+  // issued before the MFMAs of the current one (the last one is loaded
+  // twice rather than branched around)
+  const int step = 4 * LSUM_WAVES;
+  int l0 = 4 * wave;
+  if (l0 >= G.lpad) return;
+  const int last = l0 + (G.lpad - 1 - l0) / step * step;
+  double b = tb[(size_t)(l0 + lq) * G.nrpad];
+  cd av[NC];
 #pragma unroll
-    for (int q = 0; q < NC; q++) av[q] = cf[cs[q] * plane + at];
+  for (int q = 0; q < NC; q++) av[q] = cf[cs[q] * plane + (size_t)l0 * G.jpad];
+  for (; l0 <= last; l0 += step) {
+    const int ln = min(l0 + step, last);
+    double bn = tb[(size_t)(ln + lq) * G.nrpad];
+    cd an[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) an[q] = cf[cs[q] * plane + (size_t)ln * G.jpad];
 #pragma unroll
     for (int q = 0; q < NC; q++) {
       acc[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].x, b, acc[q][0], 0, 0, 0);
       acc[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q].y, b, acc[q][1], 0, 0, 0);
     }
+    b = bn;
+#pragma unroll
+    for (int q = 0; q < NC; q++) av[q] = an[q];
   }
   constexpr double sg = (A & 1) ? -1.0 : 1.0;
 #pragma unroll
@@ -474,45 +501,69 @@ __device__ __forceinline__ void lsum_group(const GemmK& G, int j0, int k0, int l
 }
 
 __device__ __forceinline__ void lsum_item(const RunK& R, const GemmK& G, int tile,
-                                          const cd* __restrict__ eia, cd* __restrict__ spec) {
-  int lane = threadIdx.x;
+                                          const cd* __restrict__ eia, cd* __restrict__ spec,
+                                          cd (*red)[12][64]) {
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int njt = G.jpad / 16;
   int j0 = (tile % njt) * 16, k0 = (tile / njt) * 16;
   int li = lane & 15, lq = lane >> 4;
   int k = k0 + li;  // receiver of this lane's output column
   cd e1 = (k < R.nr) ? eia[k] : C(1.0), e2 = e1 * e1;
   cd em1 = C(e1.x, -e1.y), em2 = C(e2.x, -e2.y);
-  cd ur[4], ut[4], up[4];
-  for (int v = 0; v < 4; v++) ur[v] = ut[v] = up[v] = C(0.0);
+  cd o[12];  // ur[4], ut[4], up[4]
+  cd *ur = o, *ut = o + 4, *up = o + 8;
+  for (int v = 0; v < 12; v++) o[v] = C(0.0);
   // coefficient index c: uu m -> m+2, vv m -> 7+m, ww m -> 12+m
-  lsum_group<0, 0, 1, 2, 0, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // uu0 x
-  lsum_group<0, 1, 4, 3, 1, 13, 11>(G, j0, k0, li, lq, e1, em1, ur, ut, up);   // uu+-1, ww+-1 x
-  lsum_group<0, 2, 4, 4, 0, 14, 10>(G, j0, k0, li, lq, e2, em2, ur, ut, up);   // uu+-2, ww+-2 x
-  lsum_group<1, 0, 1, 7, 0, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // vv0 dx
-  lsum_group<1, 1, 4, 8, 6, 13, 11>(G, j0, k0, li, lq, e1, em1, ur, ut, up);   // vv+-1, ww+-1 dx
-  lsum_group<1, 2, 4, 9, 5, 14, 10>(G, j0, k0, li, lq, e2, em2, ur, ut, up);   // vv+-2, ww+-2 dx
-  lsum_group<2, 1, 2, 8, 6, 0, 0>(G, j0, k0, li, lq, e1, em1, ur, ut, up);     // vv+-1 x/sin
-  lsum_group<2, 2, 2, 9, 5, 0, 0>(G, j0, k0, li, lq, e2, em2, ur, ut, up);     // vv+-2 x/sin
-  if (k >= R.nr) return;
-  size_t nwr = (size_t)R.nw * R.nr;
-  for (int v = 0; v < 4; v++) {
-    int j = j0 + lq + 4 * v;
-    if (j >= R.nw) continue;
-    size_t at = (size_t)j + (size_t)R.nw * k;
-    spec[at] = ur[v];
-    spec[nwr + at] = ut[v];
-    spec[2 * nwr + at] = up[v];
+  lsum_group<0, 0, 1, 2, 0, 0, 0>(G, j0, k0, li, lq, wave, e1, em1, ur, ut, up);    // uu0 x
+  pin_sums(o);
+  lsum_group<0, 1, 4, 3, 1, 13, 11>(G, j0, k0, li, lq, wave, e1, em1, ur, ut, up);  // uu+-1, ww+-1 x
+  pin_sums(o);
+  lsum_group<0, 2, 4, 4, 0, 14, 10>(G, j0, k0, li, lq, wave, e2, em2, ur, ut, up);  // uu+-2, ww+-2 x
+  pin_sums(o);
+  lsum_group<1, 0, 1, 7, 0, 0, 0>(G, j0, k0, li, lq, wave, e1, em1, ur, ut, up);    // vv0 dx
+  pin_sums(o);
+  lsum_group<1, 1, 4, 8, 6, 13, 11>(G, j0, k0, li, lq, wave, e1, em1, ur, ut, up);  // vv+-1, ww+-1 dx
+  pin_sums(o);
+  lsum_group<1, 2, 4, 9, 5, 14, 10>(G, j0, k0, li, lq, wave, e2, em2, ur, ut, up);  // vv+-2, ww+-2 dx
+  pin_sums(o);
+  lsum_group<2, 1, 2, 8, 6, 0, 0>(G, j0, k0, li, lq, wave, e1, em1, ur, ut, up);    // vv+-1 x/sin
+  pin_sums(o);
+  lsum_group<2, 2, 2, 9, 5, 0, 0>(G, j0, k0, li, lq, wave, e2, em2, ur, ut, up);    // vv+-2 x/sin
+  pin_sums(o);
+  // tree reduction over the waves: 2,3 -> 0,1, then 1 -> 0
+  static_assert(LSUM_WAVES == 4, "reduction written for four waves");
+  if (wave >= 2)
+    for (int v = 0; v < 12; v++) red[wave - 2][v][lane] = o[v];
+  __syncthreads();
+  if (wave < 2)
+    for (int v = 0; v < 12; v++) o[v] = o[v] + red[wave][v][lane];
+  __syncthreads();
+  if (wave == 1)
+    for (int v = 0; v < 12; v++) red[0][v][lane] = o[v];
+  __syncthreads();
+  if (wave == 0 && k < R.nr) {
+    size_t nwr = (size_t)R.nw * R.nr;
+    for (int v = 0; v < 4; v++) {
+      int j = j0 + lq + 4 * v;
+      if (j >= R.nw) continue;
+      size_t at = (size_t)j + (size_t)R.nw * k;
+      spec[at] = ur[v] + red[0][v][lane];
+      spec[nwr + at] = ut[v] + red[0][4 + v][lane];
+      spec[2 * nwr + at] = up[v] + red[0][8 + v][lane];
+    }
   }
+  __syncthreads();  // red is reused by the block's next item
 }
 
 // items (event, 16 w x 16 receiver tile); spec of event e at spec_off,
-// [3][nr][nw]
-__global__ void __launch_bounds__(64) k_lsum(const RunK* __restrict__ runs, Queue Q,
-                                             cd* __restrict__ cf, const double* __restrict__ tb,
-                                             const cd* __restrict__ eia, cd* __restrict__ spec) {
+// [3][nr][nw]; blocks of 64 LSUM_WAVES threads
+__global__ void __launch_bounds__(256) k_lsum(const RunK* __restrict__ runs, Queue Q,
+                                              cd* __restrict__ cf, const double* __restrict__ tb,
+                                              const cd* __restrict__ eia, cd* __restrict__ spec) {
+  __shared__ cd red[2][12][64];
   RUN_ITEMS(false, Q,
             lsum_item(runs[it.x], gemm_of(runs[it.x], cf, tb), it.y, eia + runs[it.x].roff,
-                      spec + runs[it.x].spec_off));
+                      spec + runs[it.x].spec_off, red));
 }
 
 // -------------------------------------------------------------- k_legendre

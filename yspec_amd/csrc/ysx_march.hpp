@@ -320,6 +320,10 @@ constexpr double b21 = 0.2, b31 = 3.0 / 40.0, b32 = 9.0 / 40.0, b41 = 0.3, b42 =
                  c6 = 512.0 / 1771.0;
 }  // namespace ck
 
+// The stages are kept apart by scheduling barriers (the RHS of one stage
+// would otherwise be interleaved with the next and spill); mask 0x100 lets
+// only LDS reads cross them, so the next stage's base coefficients and slot
+// values can be fetched early.
 template <class F>
 __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd* y, const F& f) {
   using namespace ck;
@@ -328,17 +332,17 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd*
   f(sb, L, w, y, k1);
 #pragma unroll
   for (int i = 0; i < N; i++) t[i] = rmad(y[i], b21, k1[i]);
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(0x100);
   f(sb + BS, L, w, t, k2);
 #pragma unroll
   for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], b31, k1[i]), b32, k2[i]);
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(0x100);
   f(sb + 2 * BS, L, w, t, k3);
 #pragma unroll
   for (int i = 0; i < N; i++) t[i] = rmad(rmad(rmad(y[i], b41, k1[i]), b42, k2[i]), b43, k3[i]);
   {
     cd k4[N];
-    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_sched_barrier(0x100);
     f(sb + 3 * BS, L, w, t, k4);
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -350,11 +354,11 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd*
       k1[i] = po;
     }
   }
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(0x100);
   f(sb + 4 * BS, L, w, t, k3);  // k5
 #pragma unroll
   for (int i = 0; i < N; i++) t[i] = rmad(k2[i], b65, k3[i]);
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(0x100);
   f(sb + 5 * BS, L, w, t, k3);  // k6
 #pragma unroll
   for (int i = 0; i < N; i++) y[i] = rmad(k1[i], c6, k3[i]);
@@ -377,38 +381,62 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
   cd* S1 = slot;
   cd* S2 = slot + N * 64;
   cd t[N], u[N];
+  // Slot reads are issued one component ahead of their use (p1, p2), so
+  // that LDS latency overlaps the arithmetic of the previous component.
+  cd p1, p2;  // stages 2 and 3
   // stage 1: k1 -> S1, t = y + b21 k1
   f.stream(sb, L, w, y, [&](int i, cd d) {
     S1[i * 64] = d;
     t[i] = rmad(y[i], b21, d);
   });
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(0x100);
   // stage 2: k2 -> u
   f.stream(sb + BS, L, w, t, [&](int i, cd d) { u[i] = d; });
-#pragma unroll
-  for (int i = 0; i < N; i++) t[i] = rmad(rmad(y[i], b31, S1[i * 64]), b32, u[i]);
-  __builtin_amdgcn_sched_barrier(0);
-  // stage 3: k3 -> S2; then t = t4, y = y + A5, S2 = y + A6, u = y + Aout
-  f.stream(sb + 2 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = d; });
+  p1 = S1[0];
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    cd k1 = S1[i * 64], k2 = u[i], k3 = S2[i * 64], yi = y[i];
+    cd k1 = p1;
+    if (i + 1 < N) p1 = S1[(i + 1) * 64];
+    t[i] = rmad(rmad(y[i], b31, k1), b32, u[i]);
+  }
+  __builtin_amdgcn_sched_barrier(0x100);
+  // stage 3: k3 -> S2; then t = t4, y = y + A5, S2 = y + A6, u = y + Aout
+  f.stream(sb + 2 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = d; });
+  p1 = S1[0];
+  p2 = S2[0];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    cd k1 = p1, k2 = u[i], k3 = p2, yi = y[i];
+    if (i + 1 < N) {
+      p1 = S1[(i + 1) * 64];
+      p2 = S2[(i + 1) * 64];
+    }
     t[i] = rmad(rmad(rmad(yi, b41, k1), b42, k2), b43, k3);
     y[i] = rmad(rmad(rmad(yi, b51, k1), b52, k2), b53, k3);
     S2[i * 64] = rmad(rmad(rmad(yi, b61, k1), b62, k2), b63, k3);
     u[i] = rmad(rmad(yi, c1, k1), c3, k3);
   }
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(0x100);
   // stage 4: fold k4 into the three accumulators; y is now the stage-5 input
+  // (reads two components ahead: q[i & 1])
+  cd q[2] = {S2[0], S2[64]};
   f.stream(sb + 3 * BS, L, w, t, [&](int i, cd d) {
+    cd a6 = q[i & 1];
+    if (i + 2 < N) q[i & 1] = S2[(i + 2) * 64];
     y[i] = rmad(y[i], b54, d);
-    S2[i * 64] = rmad(S2[i * 64], b64, d);
+    S2[i * 64] = rmad(a6, b64, d);
     u[i] = rmad(u[i], c4, d);
   });
-  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_sched_barrier(0x100);
   // stage 5: t = stage-6 input
-  f.stream(sb + 4 * BS, L, w, y, [&](int i, cd d) { t[i] = rmad(S2[i * 64], b65, d); });
-  __builtin_amdgcn_sched_barrier(0);
+  q[0] = S2[0];
+  q[1] = S2[64];
+  f.stream(sb + 4 * BS, L, w, y, [&](int i, cd d) {
+    cd a6 = q[i & 1];
+    if (i + 2 < N) q[i & 1] = S2[(i + 2) * 64];
+    t[i] = rmad(a6, b65, d);
+  });
+  __builtin_amdgcn_sched_barrier(0x100);
   // stage 6: y = y + Aout + c6 k6
   f.stream(sb + 5 * BS, L, w, t, [&](int i, cd d) { y[i] = rmad(u[i], c6, d); });
 }
