@@ -180,10 +180,23 @@ void build_work(ysx_ctx* c) {
         // the block's highest w sets its step count
         cost[W_VAR].push_back(rew(std::min(R.nw, (64 * pb + 63) % R.nw + 1)) * (rtop + 0.05));
       }
-    int ntile = (R.jpad / 16) * (R.nrpad / 16);
-    for (int t = 0; t < ntile; t++) {
-      c->W[W_LSUM].plain.push_back(make_int4(e, t, 0, 0));
-      cost[W_LSUM].push_back((double)R.lpad);
+    // receiver-sum tiles t = jt + njt kt (16 w x 16 receivers).  Blocks go
+    // to the 8 XCDs in turn (block b on XCD b % 8), so the list gives each
+    // XCD whole frequency tiles, all their receiver tiles one after another:
+    // the coefficient rows of a frequency tile are then read into one L2.
+    {
+      const int njt = R.jpad / 16, nkt = R.nrpad / 16, nx = 8;
+      std::vector<std::vector<int>> seq(nx);
+      for (int jt = 0; jt < njt; jt++)
+        for (int kt = 0; kt < nkt; kt++) seq[jt % nx].push_back(jt + njt * kt);
+      size_t longest = 0;
+      for (auto& q : seq) longest = std::max(longest, q.size());
+      for (size_t i = 0; i < longest; i++)
+        for (int x = 0; x < nx; x++)
+          if (i < seq[x].size()) {
+            c->W[W_LSUM].plain.push_back(make_int4(e, seq[x][i], 0, 0));
+            cost[W_LSUM].push_back((double)R.lpad);
+          }
     }
     int bk = std::min(R.nt, FFT_B);
     if (R.nt / bk <= FFT_G) {  // two-pass transform
