@@ -24,7 +24,9 @@ yspec_amd/tune.py has picked the batched launch shape.  roofline: the dominant k
 counts x the per-step constants of yspec_amd/flops.py) over its HIP-event
 time.  cpu_baseline: the reference Fortran (compiled from the reference's
 own sources into oracle/_ref, run through its FP64 driver) timed on this
-host on a stride sample of degrees, all frequencies and receivers.
+host on a stride sample of degrees, all frequencies and receivers, one core;
+cpu_baseline_all_cores: the same program as one process per core (at most
+16) on a finer degree sample split between them, pairs over wall time.
 """
 from __future__ import annotations
 
@@ -56,26 +58,54 @@ TRAFFIC_BYTES = {"bvec": 1368.8e6, "minors": 1541.7e6, "tor": 204.8e6, "lsum": 2
 PHASES = ("source", "tor", "minors", "bvec", "rad", "assemble", "lsum", "ifft")
 
 
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+
+
+def _ref_inputs(td, deck):
+    with open(os.path.join(td, "synth.200"), "w") as f:
+        f.write(deck.to_text())
+    q = workload("cfg2", model_file="synth.200")
+    with open(os.path.join(td, "in.txt"), "w") as f:
+        f.write(q.to_text())
+
+
+def _parse_times(out):
+    rows = [line.split() for line in out.splitlines() if line.startswith("l ")]
+    return int(sum(int(r[3]) for r in rows)), sum(float(r[-1]) for r in rows)
+
+
 def cpu_baseline(deck, p, sample):
     """Reference yspec (oracle/_ref/ref_dump --time) on degrees `sample`,
     every frequency and receiver, one core.  Returns (pairs/s, pairs, s)."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
-    if not os.path.exists(exe):
+    if not os.path.exists(REF_EXE):
         return None
     with tempfile.TemporaryDirectory() as td:
-        with open(os.path.join(td, "synth.200"), "w") as f:
-            f.write(deck.to_text())
-        q = workload("cfg2", model_file="synth.200")
-        with open(os.path.join(td, "in.txt"), "w") as f:
-            f.write(q.to_text())
+        _ref_inputs(td, deck)
         t0 = time.perf_counter()
-        out = subprocess.run([exe, "in.txt", "--time", *map(str, sample)], cwd=td, check=True,
+        out = subprocess.run([REF_EXE, "in.txt", "--time", *map(str, sample)], cwd=td, check=True,
                              capture_output=True, text=True).stdout
         wall = time.perf_counter() - t0
-    secs = [float(line.split()[-1]) for line in out.splitlines() if line.startswith("l ")]
-    nws = [int(line.split()[3]) for line in out.splitlines() if line.startswith("l ")]
-    pairs = int(sum(nws))
-    return pairs / sum(secs), pairs, sum(secs), wall
+    pairs, secs = _parse_times(out)
+    return pairs / secs, pairs, secs, wall
+
+
+def cpu_baseline_all_cores(deck, sample, nproc):
+    """The same program as `nproc` processes at once, degrees of `sample`
+    dealt out in turn; pairs over the wall time of the whole run."""
+    if not os.path.exists(REF_EXE):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        _ref_inputs(td, deck)
+        t0 = time.perf_counter()
+        procs = [subprocess.Popen([REF_EXE, "in.txt", "--time", *map(str, sample[k::nproc])],
+                                  cwd=td, stdout=subprocess.PIPE, text=True)
+                 for k in range(nproc) if sample[k::nproc]]
+        outs = [pr.communicate()[0] for pr in procs]
+        wall = time.perf_counter() - t0
+        if any(pr.returncode for pr in procs):
+            return None
+    pairs = sum(_parse_times(o)[0] for o in outs)
+    return pairs / wall, pairs, wall, len(procs)
 
 
 def batched_compare(deck, steps, warmup, do_tune=True):
@@ -224,6 +254,17 @@ def main():
                 "value": rate, "unit": "(l,ω) solves/s", "cores": 1, "kind": "reference",
                 "sample": f"reference yspec (amdflang -O2) on l in {sample}, all {g.nw} w, "
                           f"{nr} receivers: {pairs} pairs in {secs:.1f} s",
+            }
+        nproc = max(1, min(16, os.cpu_count() or 1))
+        sample_all = list(range(0, p.lmax + 1, 5))
+        ca = cpu_baseline_all_cores(deck, sample_all, nproc)
+        if ca:
+            rate, pairs, wall, used = ca
+            line["cpu_baseline_all_cores"] = {
+                "value": rate, "unit": "(l,ω) solves/s", "cores": used, "kind": "reference",
+                "sample": f"reference yspec (amdflang -O2), {used} processes, l = 0, 5, ..., "
+                          f"{p.lmax} dealt out in turn, all {g.nw} w, {nr} receivers: "
+                          f"{pairs} pairs in {wall:.1f} s wall",
             }
     print(json.dumps(line), flush=True)
 

@@ -11,10 +11,13 @@ FP64 results as compressed .npz files:
                       (oracle/_ref/yspec on the same parameter file).
 
 Cases cover: attenuation off/on, shallow and deep sources, a receiver below
-the source, an ocean layer, a transversely isotropic (ians=1) deck, and the
-three output types with the potential/tilt correction.
+the source, an ocean layer, a transversely isotropic (ians=1) deck, the
+three output types with the potential/tilt correction, gravity switches and
+fluid inner cores, and edge cases of the degree range and geometry (radial
+only, one degree, a ragged degree range, a receiver at the epicentre and
+one near the antipode).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case names]   # all cases by default
 """
 from __future__ import annotations
 
@@ -114,6 +117,16 @@ def cases():
     yield "aniso_acc_cor", aniso_deck(0), workload("cfg1", lmax=24, nr=2, ats=1, out=2,
                                                    cor=1, src_depth=300.0, f2=5.0,
                                                    tout=40.0, dt=20.0, f21=3.0, f22=4.5)
+    # edge cases: l = 0 only (radial solver alone), a single degree l = 1,
+    # and a ragged degree range starting at 2 with receivers at the source
+    # epicentre (the theta -> 0 branch of legendre, src/module_function.f90:
+    # 29-40) and 1 degree from the antipode
+    yield "radial_only", d0, workload("cfg1", lmin=0, lmax=0, nr=1, ats=1, src_depth=200.0)
+    yield "one_degree", d0, workload("cfg1", lmin=1, lmax=1, nr=2, src_depth=35.0)
+    pe = workload("cfg1", lmin=2, lmax=77, nr=3, ats=1, src_depth=120.0)
+    pe.lat = np.array([pe.src_lat, -pe.src_lat, pe.lat[2]])
+    pe.lon = np.array([pe.src_lon, pe.src_lon + 179.0 - 360.0 * (pe.src_lon > 0), pe.lon[2]])
+    yield "epicentre_ragged", d0, pe
 
 
 def run_case(name, deck, p, outdir, keep_pairs=240):
@@ -162,7 +175,10 @@ def run_case(name, deck, p, outdir, keep_pairs=240):
 def main():
     if not os.path.exists(REF_DUMP):
         sys.exit("build the reference first: make -C oracle ref")
+    only = set(sys.argv[1:])
     for name, deck, p in cases():
+        if only and name not in only:
+            continue
         path, size = run_case(name, deck, p, HERE)
         print(f"{os.path.basename(path)}: {size / 1024:.1f} KiB")
 
