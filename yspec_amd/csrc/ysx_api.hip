@@ -368,7 +368,11 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     k.nknot = M.nknot; k.nic = M.nic; k.noc = M.noc; k.nsl = M.nsl; k.fic = M.fic;
     k.ocean = M.ocean;
     k.grav = R.grav;
-    k.var = (R.grav != 2 || M.fic) ? 1 : 0;
+    // self-gravitating runs go through the lockstep kernels, a fluid inner
+    // core too when the receiver is not below the source (sph_solver_fic
+    // then differs from sph_solver only in its start, handled by k_start
+    // and the marches); everything else runs the per-lane variant solvers
+    k.var = (R.grav != 2 || (M.fic && R.rr < R.rs)) ? 1 : 0;
     k.lpad = (R.lmax + 1 + 3) / 4 * 4;
     k.jpad = (R.nw + 15) / 16 * 16;
     k.nrpad = (R.nr + 15) / 16 * 16;

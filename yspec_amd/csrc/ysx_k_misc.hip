@@ -74,8 +74,10 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
 // ----------------------------------------------------------------- k_start
 // Start levels and starting solutions of every pair (one lane per pair):
 // tor_solver's start (src/module_solver.f90:64-75, 103-114) and
-// sph_solver's start (:678-722, 758-785).  The marches pick these up when
-// they reach each lane's start knot.
+// sph_solver's start (:678-722, 758-785); with a fluid inner core
+// (receiver not below the source) sph_solver_fic's (:1240-1265), which
+// starts the fluid minors in the inner core.  The marches pick these up
+// when they reach each lane's start knot.
 __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
                                                const RunK* __restrict__ runs, PairRange PR,
                                                StartData SD, int* __restrict__ starts) {
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
   }
   cd m[14];
   for (int i = 0; i < 14; i++) m[i] = C(0.0);
-  if (i1 > L.nic && i1 <= L.noc)
+  if (i1 <= L.noc && (i1 > L.nic || R.fic))
     minor_fluid_start(T, s, i1, rt, m);
   else
     minor_solid_start(T, s, i1, rt, m);

@@ -85,7 +85,9 @@ __global__ void __launch_bounds__(64) k_tor_march(const double* __restrict__ gta
 
 // ------------------------------------------------ k_minor_top, k_minor_march
 // sph_solver (grav = 2, solid inner core) up to the b-vector integration,
-// src/module_solver.f90:640-743, 756-806, leaving SphState per pair.
+// src/module_solver.f90:640-743, 756-806, leaving SphState per pair; with a
+// fluid inner core and the receiver not below the source, sph_solver_fic
+// (:1240-1301), whose minors stay fluid from the inner core to the CMB.
 
 // minor vector from the surface down to radius rb in knot interval ib
 // (through the ocean as a fluid when there is one), src/module_solver.f90:730-741
@@ -131,8 +133,10 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
   for (int i = 0; i < 14; i++) m[i] = C(0.0);
   int layout = -1, sc1 = 0;  // -1 none, 0 solid, 1 fluid
   for (int i = ib; i <= itop; i++) {
-    int region = (i < nic) ? 0 : ((i > nic && i < noc) ? 1 : 0);
-    bool fluid_i = (i > nic && i < noc);
+    // fluid intervals: the outer core, and the inner core when it is fluid
+    // (only such events with the receiver above the source run here)
+    bool fluid_i = R.fic ? (i < noc) : (i > nic && i < noc);
+    int region = fluid_i ? 1 : 0;
     if (layout == 0 && fluid_i) {
       cd mf[5];
       solid_2_fluid(m, mf);
