@@ -371,8 +371,11 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     // self-gravitating runs go through the lockstep kernels, a fluid inner
     // core too when the receiver is not below the source (sph_solver_fic
     // then differs from sph_solver only in its start, handled by k_start
-    // and the marches); everything else runs the per-lane variant solvers
-    k.var = (R.grav != 2 || (M.fic && R.rr < R.rs)) ? 1 : 0;
+    // and the marches); gravity switches 0 / 1 through k_ngc_march; a fluid
+    // inner core with the receiver below the source through the per-lane
+    // solver
+    k.ngc = (R.grav != 2) ? 1 : 0;
+    k.var = (R.grav == 2 && M.fic && R.rr < R.rs) ? 1 : 0;
     k.lpad = (R.lmax + 1 + 3) / 4 * 4;
     k.jpad = (R.nw + 15) / 16 * 16;
     k.nrpad = (R.nr + 15) / 16 * 16;
@@ -565,6 +568,10 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
       launch_sph_var(dim3(grid_of(qv)), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
                          ctx->d_rec, ctx->d_cnt);
       if (timing) HIPCHK(hipEventRecord(ev[4], st));
+    } else if (ctx->rk[e].ngc) {
+      launch_ngc_march(ctx->rk[e].grav == 1, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
+                       ctx->d_runs, qt, ctx->d_rec, ctx->d_cnt);
+      if (timing) HIPCHK(hipEventRecord(ev[4], st));
     } else {
       launch_minor_top_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                          ctx->d_runs, qt, S, ctx->d_cnt);
@@ -671,6 +678,16 @@ int enqueue_batched(ysx_ctx* ctx) {
   if (qv.n > 0)
     launch_sph_var(dim3(qv.n), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
                        ctx->d_rec, ctx->d_cnt);
+  // gravity switches 0 / 1: one launch each over the cost-sorted wave list
+  // (the kernel skips the other events' items)
+  Queue qn = batched_q(ctx, W_TOR, -1);
+  for (int sw = 0; sw <= 1; sw++) {
+    bool any = false;
+    for (const RunK& k : ctx->rk) any = any || (k.ngc && k.grav == sw);
+    if (any)
+      launch_ngc_march(sw == 1, dim3(std::max(1, qn.n)), b64, 0, st, ctx->d_tab, ctx->d_runs,
+                       qn, ctx->d_rec, ctx->d_cnt);
+  }
   HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
   launch_assemble(dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec, ctx->d_cf,
                      ctx->d_tb);

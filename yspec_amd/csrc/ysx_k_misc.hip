@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
     SD.tor_y0[2 * p] = y[0];
     SD.tor_y0[2 * p + 1] = y[1];
   }
-  if (R.var) return;  // the per-lane variant solvers find their own start
+  if (R.var || R.ngc) return;  // the variant solvers find their own start
   int i1 = sph_start_level(T, L, l, s.rew, itop, R.fre_norm);
   starts[2 * p + 1] = i1;
   double rt = T(F_R, i1);

@@ -17,7 +17,8 @@ namespace ysx {
 struct RunK {
   int lmin, lmax, nw, nr, is, ir, out, cor, npair;
   int nknot, nic, noc, nsl, fic, ocean;
-  int grav, var;                // gravity switch; var: spheroidal solves per lane
+  int grav, var, ngc;           // gravity switch; spheroidal solves per lane (var) or
+                                // by the gravity-0/1 lockstep march (ngc)
   int lpad, jpad, nrpad;        // GEMM padding of this event
   int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
   long long pbase;              // first pair of this event in the per-pair arrays
@@ -242,5 +243,8 @@ void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw);
 YSX_MARCH_LAUNCHERS(item)
 YSX_MARCH_LAUNCHERS(pers)
 #undef YSX_MARCH_LAUNCHERS
+// gravity switch 1 (cow) or 0, ysx_k_ngc.hip
+void launch_ngc_march(bool cow, dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                      const RunK* runs, Queue Q, cd* rec, int* cnt);
 
 }  // namespace ysx

@@ -71,35 +71,44 @@ __device__ inline double step_len_w(const Tab& T, double rew, int i, int nknot) 
 // ------------------------------------------------------------ solid region
 // sph_solid_amat (src/module_int.f90:500-560) split into an l-independent
 // base per radius (times h) and the l-dependent entries added per lane.
+// GRAV = false takes gravity as zero (gravity switch 0, sph_solid_amat_ng,
+// src/module_int.f90:564-610); the entries used there and with switch 1
+// (sph_solid_amat_cow, :613-672) have the same form as with switch 2.
+template <bool GRAV>
+__device__ inline void solid_base(const Tab& T, const WS& w, int il, double rr, double h,
+                                double* o) {
+  SplW sw = spl_w(T, il, rr);
+  double rrho = spl(T, sw, F_RHO), racon = spl(T, sw, F_ACON), rccon = spl(T, sw, F_CCON),
+         rfcon = spl(T, sw, F_FCON), rncon = spl(T, sw, F_NCON), rlcon = spl(T, sw, F_LCON),
+         gg = GRAV ? spl(T, sw, F_GRAV) : 0.0, rqmu = spl(T, sw, F_QMU),
+         rxlam = spl(T, sw, F_XLAM), rxa2 = spl(T, sw, F_XA2);
+  cd qaa = 1.0 + rxa2 * w.lno, qff = 1.0 + rxlam * w.lno, qll = 1.0 + rqmu * w.lno;
+  cd cacon = racon * qaa, cfcon = rfcon * qff, cncon = rncon * qll;
+  cd cccon = cinv(rccon * qaa);
+  cd fc = cfcon * cccon;
+  cd gamma = cacon - cncon - cfcon * fc;
+  double ri = 1.0 / rr, ri2 = ri * ri;
+  double rgr = rrho * gg * rr;
+  cd mw2r = -rrho * w.omega2;
+  cd t11 = (1.0 - 2.0 * fc) * ri;
+  cd k22 = cinv(rlcon * qll);
+  cd s11 = mw2r + (4.0 * ri2) * (gamma - rgr);
+  cd fcri = ri * fc;
+  cd q12 = ri2 * (rgr - 2.0 * gamma);
+  cd a22 = mw2r - (2.0 * ri2) * cncon;
+  cd gc = ri2 * (gamma + cncon);
+  o[0] = h * ri; o[1] = rrho;
+  o[2] = h * t11.x; o[3] = h * t11.y; o[4] = h * cccon.x; o[5] = h * cccon.y;
+  o[6] = h * k22.x; o[7] = h * k22.y; o[8] = h * s11.x; o[9] = h * s11.y;
+  o[10] = h * fcri.x; o[11] = h * fcri.y; o[12] = h * q12.x; o[13] = h * q12.y;
+  o[14] = h * a22.x; o[15] = h * a22.y; o[16] = h * gc.x; o[17] = h * gc.y;
+  o[18] = h * (-4.0 * w.pibigg * rrho); o[19] = h * (4.0 * w.pibigg);
+}
+
 struct SolidB {
   static constexpr int BS = 20;
   __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
-    SplW sw = spl_w(T, il, rr);
-    double rrho = spl(T, sw, F_RHO), racon = spl(T, sw, F_ACON), rccon = spl(T, sw, F_CCON),
-           rfcon = spl(T, sw, F_FCON), rncon = spl(T, sw, F_NCON), rlcon = spl(T, sw, F_LCON),
-           gg = spl(T, sw, F_GRAV), rqmu = spl(T, sw, F_QMU), rxlam = spl(T, sw, F_XLAM),
-           rxa2 = spl(T, sw, F_XA2);
-    cd qaa = 1.0 + rxa2 * w.lno, qff = 1.0 + rxlam * w.lno, qll = 1.0 + rqmu * w.lno;
-    cd cacon = racon * qaa, cfcon = rfcon * qff, cncon = rncon * qll;
-    cd cccon = cinv(rccon * qaa);
-    cd fc = cfcon * cccon;
-    cd gamma = cacon - cncon - cfcon * fc;
-    double ri = 1.0 / rr, ri2 = ri * ri;
-    double rgr = rrho * gg * rr;
-    cd mw2r = -rrho * w.omega2;
-    cd t11 = (1.0 - 2.0 * fc) * ri;
-    cd k22 = cinv(rlcon * qll);
-    cd s11 = mw2r + (4.0 * ri2) * (gamma - rgr);
-    cd fcri = ri * fc;
-    cd q12 = ri2 * (rgr - 2.0 * gamma);
-    cd a22 = mw2r - (2.0 * ri2) * cncon;
-    cd gc = ri2 * (gamma + cncon);
-    o[0] = h * ri; o[1] = rrho;
-    o[2] = h * t11.x; o[3] = h * t11.y; o[4] = h * cccon.x; o[5] = h * cccon.y;
-    o[6] = h * k22.x; o[7] = h * k22.y; o[8] = h * s11.x; o[9] = h * s11.y;
-    o[10] = h * fcri.x; o[11] = h * fcri.y; o[12] = h * q12.x; o[13] = h * q12.y;
-    o[14] = h * a22.x; o[15] = h * a22.y; o[16] = h * gc.x; o[17] = h * gc.y;
-    o[18] = h * (-4.0 * w.pibigg * rrho); o[19] = h * (4.0 * w.pibigg);
+    solid_base<true>(T, w, il, rr, h, o);
   }
   // h A from the base of one stage radius
   __device__ static inline void amat(const double* b, const LS& L, SolidA& A) {

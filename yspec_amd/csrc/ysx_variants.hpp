@@ -158,12 +158,14 @@ __device__ inline void minor_solid_start_ng(const Tab& T, const PS& s, int is, d
 // b_solid_start_ngcow, src/module_int.f90:1596-1615, for the unit source
 // vector e_c
 __device__ inline void b_start_ngc(const cd* m, int c, cd* b) {
-  cd sv[4] = {C(0.0), C(0.0), C(0.0), C(0.0)};
-  sv[c] = C(1.0);
-  b[0] = -(m[4] * sv[1]) - m[1] * sv[2] - m[3] * sv[3];
-  b[1] = m[4] * sv[0] - m[2] * sv[2] + m[1] * sv[3];
-  b[2] = m[1] * sv[0] + m[2] * sv[1] - m[0] * sv[3];
-  b[3] = m[3] * sv[0] - m[1] * sv[1] + m[0] * sv[2];
+  // e_c's entries as values, not an array indexed by c (which would live
+  // in scratch memory)
+  const cd sv0 = C(c == 0 ? 1.0 : 0.0), sv1 = C(c == 1 ? 1.0 : 0.0),
+           sv2 = C(c == 2 ? 1.0 : 0.0), sv3 = C(c == 3 ? 1.0 : 0.0);
+  b[0] = -(m[4] * sv1) - m[1] * sv2 - m[3] * sv3;
+  b[1] = m[4] * sv0 - m[2] * sv2 + m[1] * sv3;
+  b[2] = m[1] * sv0 + m[2] * sv1 - m[0] * sv3;
+  b[3] = m[3] * sv0 - m[1] * sv1 + m[0] * sv2;
 }
 
 // sph_minors_*_2_*_ngcow, sph_delta_solid_ngcow, sph_ysol_ngcow
