@@ -316,10 +316,10 @@ __global__ void __launch_bounds__(64) k_rad_ng(const double* __restrict__ gtab,
 }
 
 // ------------------------------------------------------------- k_sph_var
-// The spheroidal solve of one pair per lane for events outside the lockstep
-// kernels: gravity switch 0 / 1 (sph_solver_ng, _cow, _ng_fic, _cow_fic)
-// or a fluid inner core with self-gravitation (sph_solver_fic), with the
-// reference driver's 4-vector -> 6-vector layout (src/yspec.f90:342-374).
+// The spheroidal solve of one pair per lane for the events outside the
+// lockstep kernels: a fluid inner core with self-gravitation and the
+// receiver below the source (sph_solver_fic, src/module_solver.f90:
+// 1203-1382).
 __device__ __forceinline__ void sph_var_item(const Tab& T, const RunK& R, int pb,
                                              cd* __restrict__ rec, int* __restrict__ cnt) {
   int pl = pb * 64 + (int)threadIdx.x;
@@ -334,25 +334,9 @@ __device__ __forceinline__ void sph_var_item(const Tab& T, const RunK& R, int pb
   VarCount n{0, 0, 0};
   size_t p = (size_t)R.pbase + pl;
   cd* o = rec + p * REC + O_YSPH;
-  if (R.grav == 2) {
-    cd y[24];
-    sph_solve_fic(T, s, G, y, n);
-    for (int q = 0; q < 24; q++) o[q] = y[q];
-  } else {
-    cd y[16];
-    if (R.grav == 1)
-      sph_solve_ngc<true>(T, s, G, R.fic != 0, y, n);
-    else
-      sph_solve_ngc<false>(T, s, G, R.fic != 0, y, n);
-    for (int c = 0; c < 4; c++) {
-      o[6 * c + 0] = y[4 * c + 0];
-      o[6 * c + 1] = y[4 * c + 1];
-      o[6 * c + 2] = C(0.0);
-      o[6 * c + 3] = y[4 * c + 2];
-      o[6 * c + 4] = y[4 * c + 3];
-      o[6 * c + 5] = C(0.0);
-    }
-  }
+  cd y[24];
+  sph_solve_fic(T, s, G, y, n);
+  for (int q = 0; q < 24; q++) o[q] = y[q];
   cnt[5 * p + 1] = n.sol;
   cnt[5 * p + 2] = n.flu;
   cnt[5 * p + 3] = n.bvec;

@@ -105,10 +105,11 @@ struct NgcFluidM {
 
 namespace {
 
-// One wave: degrees lb*64 .. lb*64+63 at frequency j; sph_solve_ngc of
-// ysx_variants.hpp with every integration a lockstep march.  Writes the
-// 4-vector solutions into the records in the reference driver's 6-vector
-// layout (src/yspec.f90:342-374) and the step counts.
+// One wave: degrees lb*64 .. lb*64+63 at frequency j; sph_solver_ng / _cow
+// / _ng_fic / _cow_fic with ss = identity(4) (src/module_solver.f90:
+// 827-1195, 1386-1740) with every integration a lockstep march.  Writes
+// the 4-vector solutions into the records in the reference driver's
+// 6-vector layout (src/yspec.f90:342-374) and the step counts.
 template <bool COW>
 __device__ __forceinline__ void ngc_item(const Tab& T, const RunK& R, int j, int lb,
                                          cd* __restrict__ rec, int* __restrict__ cnt,

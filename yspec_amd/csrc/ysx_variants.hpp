@@ -1,108 +1,20 @@
-// ysx_variants.hpp -- per-lane solvers of the reference's variant systems:
-// gravity switch 0 (no gravity) and 1 (Cowling) with a solid or a fluid
-// inner core, the self-gravitating solver with a fluid inner core, and the
-// no-gravity radial solver.  One lane integrates one (l, w) pair through
-// the per-lane Cash-Karp integrator of ysx_device.hpp; these paths serve
-// decks and switches outside the lockstep kernels (grav = 2, solid core).
+// ysx_variants.hpp -- the reference's variant systems: the matrices, start
+// solutions, region transforms and solution algebra of gravity switches 0
+// (no gravity) and 1 (Cowling), used by the lockstep march of
+// ysx_march_ngc.hpp; and two per-lane solvers (one lane integrates one
+// (l, w) pair through the Cash-Karp integrator of ysx_device.hpp): the
+// self-gravitating solver with a fluid inner core for a receiver below the
+// source, and the no-gravity radial solver.
 #pragma once
 #include "ysx_device.hpp"
 
 namespace ysx {
 
-// sph_solid_amat_ng / _cow, src/module_int.f90:564-672
+// the matrix of sph_solid_amat_ng / _cow, src/module_int.f90:564-672 (built
+// per lane from the shared base by NgcMinorM::amat)
 struct NgcA {
   cd t11, t12, k11, k22, s11, s12, s22;
   double t21, t22;
-};
-template <bool COW>
-__device__ inline void solid_amat_ngc(const Tab& T, const PS& s, int il, double rr, NgcA& A) {
-  SplW w = spl_w(T, il, rr);
-  double rrho = spl(T, w, F_RHO), racon = spl(T, w, F_ACON), rccon = spl(T, w, F_CCON),
-         rfcon = spl(T, w, F_FCON), rncon = spl(T, w, F_NCON), rlcon = spl(T, w, F_LCON),
-         rqmu = spl(T, w, F_QMU), rxlam = spl(T, w, F_XLAM), rxa2 = spl(T, w, F_XA2);
-  double gg = COW ? spl(T, w, F_GRAV) : 0.0;
-  cd qaa = 1.0 + rxa2 * s.lno, qff = 1.0 + rxlam * s.lno, qll = 1.0 + rqmu * s.lno;
-  cd cacon = racon * qaa, cfcon = rfcon * qff, cncon = rncon * qll;
-  cd cccon = cinv(rccon * qaa);
-  cd fc = cfcon * cccon;
-  cd gamma = cacon - cncon - cfcon * fc;
-  double ri = 1.0 / rr, ri2 = ri * ri;
-  A.t11 = (1.0 - 2.0 * fc) * ri;
-  A.t12 = (s.zeta * ri) * fc;
-  A.t21 = -s.zeta * ri;
-  A.t22 = 2.0 * ri;
-  A.k11 = cccon;
-  A.k22 = cinv(rlcon * qll);
-  cd mw2r = -rrho * s.omega2;
-  if (COW) {
-    double rgr = rrho * gg * rr;
-    A.s11 = mw2r + (4.0 * ri2) * (gamma - rgr);
-    A.s12 = (s.zeta * ri2) * (rgr - 2.0 * gamma);
-  } else {
-    A.s11 = mw2r + (4.0 * ri2) * gamma;
-    A.s12 = (-2.0 * s.zeta * ri2) * gamma;
-  }
-  A.s22 = mw2r + ri2 * (s.zeta2 * (gamma + cncon) - 2.0 * cncon);
-}
-
-// sph_minors_solid_derivs_ng / _cow, src/module_int.f90:874-936 (5 minors)
-template <bool COW>
-struct SolidMinorNgc {
-  static constexpr int N = 5;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* m,
-                                    cd* d) const {
-    NgcA A;
-    solid_amat_ngc<COW>(T, s, il, rr, A);
-    cd a11 = A.t11 + A.t22, a22 = A.t11 - A.t22;
-    d[0] = a11 * m[0] + A.k22 * m[2] - A.k11 * m[3];
-    d[1] = A.s12 * m[0] - A.t21 * m[2] + A.t12 * m[3];
-    d[2] = A.s22 * m[0] - 2.0 * (A.t12 * m[1]) + a22 * m[2] + A.k11 * m[4];
-    d[3] = 2.0 * A.t21 * m[1] - A.s11 * m[0] - a22 * m[3] - A.k22 * m[4];
-    d[4] = A.s11 * m[2] - 2.0 * (A.s12 * m[1]) - A.s22 * m[3] - a11 * m[4];
-  }
-};
-
-// sph_fluid_amat_ng / _cow + sph_minors_fluid_derivs_ng / _cow,
-// src/module_int.f90:720-786, 971-1012 (a 2-vector)
-template <bool COW>
-struct FluidNgc {
-  static constexpr int N = 2;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* m,
-                                    cd* d) const {
-    SplW w = spl_w(T, il, rr);
-    double rrho = spl(T, w, F_RHO), rccon = spl(T, w, F_CCON), rxa2 = spl(T, w, F_XA2);
-    cd cccon = cinv(rccon * (1.0 + rxa2 * s.lno));
-    double ri = 1.0 / rr;
-    cd zoi = s.zeta2 * s.omegai2;
-    cd k = cccon - ((ri * ri) / rrho) * zoi;
-    cd t, sv;
-    if (COW) {
-      double gg = spl(T, w, F_GRAV);
-      cd g = (gg * ri) * zoi;
-      t = (g - 1.0) * ri;
-      sv = (rrho * gg * ri) * (g - 4.0) - rrho * s.omega2;
-    } else {
-      t = C(-ri);
-      sv = -rrho * s.omega2;
-    }
-    d[0] = t * m[0] + k * m[1];
-    d[1] = sv * m[0] - t * m[1];
-  }
-};
-
-// sph_bvec_solid_derivs_ng / _cow, src/module_int.f90:1060-1110
-template <bool COW>
-struct BvecNgc {
-  static constexpr int N = 4;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* b,
-                                    cd* d) const {
-    NgcA A;
-    solid_amat_ngc<COW>(T, s, il, rr, A);
-    d[0] = -(A.s11 * b[2]) - A.s12 * b[3] - A.t11 * b[0] - A.t21 * b[1];
-    d[1] = -(A.s12 * b[2]) - A.s22 * b[3] - A.t12 * b[0] - A.t22 * b[1];
-    d[2] = A.t11 * b[2] - A.k11 * b[0] + A.t12 * b[3];
-    d[3] = A.t21 * b[2] - A.k22 * b[1] + A.t22 * b[3];
-  }
 };
 
 // sph_solid_start_ng, src/module_int.f90:1275-1341, with the sbessj2 stub
@@ -198,87 +110,6 @@ struct VarGeo {
 struct VarCount {
   int sol, flu, bvec;
 };
-
-// sph_solver_ng / _cow (fic = false) and sph_solver_ng_fic / _cow_fic
-// (fic = true) with ss = identity(4), src/module_solver.f90:827-1195,
-// 1386-1740: yout (4,4) column-major
-template <bool COW>
-__device__ void sph_solve_ngc(const Tab& T, const PS& s, const VarGeo& G, bool fic, cd* yout,
-                              VarCount& n) {
-  SolidMinorNgc<COW> fs;
-  FluidNgc<COW> ff;
-  BvecNgc<COW> fb;
-  Layers L{G.nknot, G.nic, G.noc, G.nsl, fic ? 1 : 0, G.ocean};
-  bool up = G.rr >= G.rs;
-  int iup = up ? G.is : G.ir, ibot = up ? G.ir : G.is;
-  double rup = up ? G.rs : G.rr, rbot = up ? G.rr : G.rs;
-  int nk = G.nknot;
-  int sc1 = 0, sc2 = 0, sc3 = 0;
-  cd mf1[2], mf2[2], ms1[5], ms2[5], ms3[5];
-  int i1 = sph_start_level(T, L, s.ll, s.rew, iup, G.fre_norm);
-  double rt = T(F_R, i1);
-  if ((up || fic) && i1 == 1) {
-    double dr = 1.0 / (4.0 * s.rew / G.alpha_min);
-    rt = fmin(0.1 * dr, 0.1 * T(F_R, 2));
-  }
-  if (i1 >= iup) {
-    i1 = iup - 1;
-    rt = T(F_R, i1);
-  }
-  for (int q = 0; q < 5; q++) ms1[q] = C(0.0);
-  if (i1 <= G.nic && !fic) {
-    minor_solid_start_ng(T, s, i1, rt, ms1);
-    yint(T, s, nk, i1, G.nic, rt, T(F_R, G.nic), ms1, fs, &sc1, n.sol);
-    solid_2_fluid_ngc(ms1, mf1);
-    yint(T, s, nk, G.nic + 1, G.noc, T(F_R, G.nic), T(F_R, G.noc), mf1, ff, &sc1, n.flu);
-    fluid_2_solid_ngc(mf1, ms1);
-    yint(T, s, nk, G.noc + 1, iup, T(F_R, G.noc + 1), rup, ms1, fs, &sc1, n.sol);
-  } else if (i1 <= G.noc) {
-    fluid_start_ng(T, s, i1, rt, mf1);
-    yint(T, s, nk, i1, G.noc, rt, T(F_R, G.noc), mf1, ff, &sc1, n.flu);
-    fluid_2_solid_ngc(mf1, ms1);
-    yint(T, s, nk, G.noc + 1, iup, T(F_R, G.noc + 1), rup, ms1, fs, &sc1, n.sol);
-  } else if (i1 <= G.nsl) {
-    minor_solid_start_ng(T, s, i1, rt, ms1);
-    yint(T, s, nk, i1, iup, rt, rup, ms1, fs, &sc1, n.sol);
-  }
-  for (int q = 0; q < 5; q++) ms3[q] = ms1[q];
-  if (up && G.rr > G.rs) yint(T, s, nk, G.is, G.ir, G.rs, G.rr, ms3, fs, &sc3, n.sol);
-  if (!up && G.rs > G.rr) yint(T, s, nk, G.ir, G.is, G.rr, G.rs, ms3, fs, &sc3, n.sol);
-  if (G.ocean) {
-    mf2[0] = C(1.0);
-    mf2[1] = C(0.0);
-    yint(T, s, nk, nk - 1, G.nsl + 1, T(F_R, nk), T(F_R, G.nsl + 1), mf2, ff, &sc2, n.flu);
-    fluid_2_solid_ngc(mf2, ms2);
-    yint(T, s, nk, G.nsl, ibot, T(F_R, G.nsl), rbot, ms2, fs, &sc2, n.sol);
-  } else {
-    ms2[0] = C(1.0);
-    for (int q = 1; q < 5; q++) ms2[q] = C(0.0);
-    yint(T, s, nk, nk, ibot, T(F_R, nk), rbot, ms2, fs, &sc2, n.sol);
-  }
-  cd delta = delta_ngc(ms3, ms2);
-  cd idelta = cinv(delta);
-  for (int c = 0; c < 4; c++) {
-    int sc4 = 0, nb = 0;
-    cd b[4];
-    cd* y = yout + 4 * c;
-    if (up) {
-      b_start_ngc(ms1, c, b);
-      yint(T, s, nk, G.is, G.ir, G.rs, G.rr, b, fb, &sc4, nb);
-      ysol_ngc(ms2, b, y);
-      double f = ipow_huge(sc3 - sc4);
-      for (int q = 0; q < 4; q++) y[q] = (y[q] / delta) * f;
-    } else {
-      b_start_ngc(ms2, c, b);
-      for (int q = 0; q < 4; q++) b[q] = -(b[q] * idelta);
-      yint(T, s, nk, G.is, G.ir, G.rs, G.rr, b, fb, &sc4, nb);
-      ysol_ngc(ms1, b, y);
-      double f = ipow_huge(sc3 - sc4);
-      for (int q = 0; q < 4; q++) y[q] = y[q] * f;
-    }
-    if (c == 0) n.bvec = nb;
-  }
-}
 
 // sph_solver_fic (grav = 2, fluid inner core) with ss = s6tmp,
 // src/module_solver.f90:1203-1382: yout (6,4) column-major.  Below a
