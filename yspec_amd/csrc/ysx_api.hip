@@ -169,10 +169,10 @@ void build_work(ysx_ctx* c) {
       c->W[W_BVEC].plain.push_back(make_int4(e, j, lb, 0));
       cost[W_BVEC].push_back(rew(j) * (std::fabs(R.rr - R.rs) + 1e-3));
     }
-    if (R.lmin == 0)
-      for (int jb = 0; jb < (R.nw + 63) / 64; jb++) {
+    if (R.lmin == 0)  // blocks of 8 frequencies, a group of 8 lanes each
+      for (int jb = 0; jb < (R.nw + 7) / 8; jb++) {
         c->W[W_RAD].plain.push_back(make_int4(e, jb, 0, 0));
-        cost[W_RAD].push_back(rew(std::min(R.nw, 64 * jb + 64)));
+        cost[W_RAD].push_back(rew(std::min(R.nw, 8 * jb + 8)));
       }
     if (R.var)
       for (int pb = 0; pb < (R.npair + 63) / 64; pb++) {

@@ -9,6 +9,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ysx_common.hpp"
 
 namespace ysx {
@@ -284,10 +286,15 @@ struct TorRHS {
 };
 
 // rad_amat + rad_derivs, src/module_int.f90:261-372
+// coef: the matrix at one radius (independent of y); apply: its product
+// with y.  The radial solves share the six stage evaluations of a step
+// between the lanes of a group (rk_step_grp).
 struct RadRHS {
   static constexpr int N = 2;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* y,
-                                    cd* d) const {
+  struct Coef {
+    cd t, k, sv;
+  };
+  __device__ inline Coef coef(const Tab& T, const PS& s, int il, double rr) const {
     SplW w = spl_w(T, il, rr);
     double rrho = spl(T, w, F_RHO), racon = spl(T, w, F_ACON), rccon = spl(T, w, F_CCON),
            rfcon = spl(T, w, F_FCON), rncon = spl(T, w, F_NCON), gg = spl(T, w, F_GRAV),
@@ -300,9 +307,25 @@ struct RadRHS {
     double ri = 1.0 / rr;
     cd t = (1.0 - 2.0 * fc) * ri;
     cd sv = (4.0 * ri * ri) * (gamma - rrho * gg * rr) - rrho * s.omega2;
-    d[0] = t * y[0] + cccon * y[1];
-    d[1] = sv * y[0] - t * y[1];
+    return Coef{t, cccon, sv};
   }
+  __device__ static inline void apply(const Coef& c, const cd* y, cd* d) {
+    d[0] = c.t * y[0] + c.k * y[1];
+    d[1] = c.sv * y[0] - c.t * y[1];
+  }
+  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* y,
+                                    cd* d) const {
+    apply(coef(T, s, il, rr), y, d);
+  }
+};
+
+// Lanes of one integration: the default is one lane.  RadGroup: lane q of
+// a group of 8 that integrate the same solution; sh is the group's LDS slot
+// of six stage coefficient sets.
+struct NoGroup {};
+struct RadGroup {
+  int q;
+  RadRHS::Coef* sh;
 };
 
 // ----------------------------------------------------------- Cash-Karp step
@@ -350,6 +373,60 @@ __device__ inline void rk_step(const Tab& T, const PS& s, int il, double x, doub
   for (int i = 0; i < N; i++) y[i] = y[i] + h * (k1[i] + c6 * k3[i]);
 }
 
+// rk_step for a lane group: lane q < 6 evaluates the coefficients of stage
+// q + 1 into the group's LDS slot, then every lane runs the stage updates
+// (the same arithmetic as rk_step, so every lane holds the same y).  LDS
+// accesses of one wave take effect in program order, so the slot needs no
+// barrier; groups of a wave may be at different points of their solves.
+template <class F>
+__device__ inline void rk_step_grp(const Tab& T, const PS& s, int il, double x, double h, cd* y,
+                                   const F& f, const RadGroup& g) {
+  constexpr int N = F::N;
+  constexpr double a2 = 0.2, a3 = 0.3, a4 = 0.6, a5 = 1.0, a6 = 0.875, b21 = 0.2,
+                   b31 = 3.0 / 40.0, b32 = 9.0 / 40.0, b41 = 0.3, b42 = -0.9, b43 = 1.2,
+                   b51 = -11.0 / 54.0, b52 = 2.5, b53 = -70.0 / 27.0, b54 = 35.0 / 27.0,
+                   b61 = 1631.0 / 55296.0, b62 = 175.0 / 512.0, b63 = 575.0 / 13824.0,
+                   b64 = 44275.0 / 110592.0, b65 = 253.0 / 4096.0, c1 = 37.0 / 378.0,
+                   c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c6 = 512.0 / 1771.0;
+  if (g.q < 6) {
+    const int q = g.q;
+    double aq = (q == 1) ? a2 : (q == 2) ? a3 : (q == 3) ? a4 : (q == 4) ? a5 : a6;
+    g.sh[q] = f.coef(T, s, il, (q == 0) ? x : x + aq * h);
+  }
+  typename F::Coef c[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) c[k] = g.sh[k];
+  cd k1[N], k2[N], k3[N], t[N];
+  F::apply(c[0], y, k1);
+#pragma unroll
+  for (int i = 0; i < N; i++) t[i] = y[i] + (b21 * h) * k1[i];
+  F::apply(c[1], t, k2);
+#pragma unroll
+  for (int i = 0; i < N; i++) t[i] = y[i] + h * (b31 * k1[i] + b32 * k2[i]);
+  F::apply(c[2], t, k3);
+#pragma unroll
+  for (int i = 0; i < N; i++) t[i] = y[i] + h * (b41 * k1[i] + b42 * k2[i] + b43 * k3[i]);
+  {
+    cd k4[N];
+    F::apply(c[3], t, k4);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      cd p5 = b51 * k1[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i];
+      cd p6 = b61 * k1[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i];
+      cd po = c1 * k1[i] + c3 * k3[i] + c4 * k4[i];
+      t[i] = y[i] + h * p5;
+      k2[i] = p6;
+      k1[i] = po;
+    }
+  }
+  F::apply(c[4], t, k3);  // k5
+#pragma unroll
+  for (int i = 0; i < N; i++) t[i] = y[i] + h * (k2[i] + b65 * k3[i]);
+  F::apply(c[5], t, k3);  // k6
+#pragma unroll
+  for (int i = 0; i < N; i++) y[i] = y[i] + h * (k1[i] + c6 * k3[i]);
+}
+
 // renormalisation of yint, src/module_int.f90:1910-1919
 template <int N>
 __device__ inline void renorm(cd* y, int& scale) {
@@ -378,9 +455,9 @@ __device__ inline double step_len(const Tab& T, const PS& s, int i, int nknot) {
 }
 
 // yint, src/module_int.f90:1847-1968: fixed equal steps per knot interval
-template <class F>
+template <class F, class G = NoGroup>
 __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, double r1, double r2,
-                     cd* y, const F& f, int* scale, int& nsteps) {
+                     cd* y, const F& f, int* scale, int& nsteps, const G& grp = G()) {
   if (r2 > r1) {
     for (int i = i1; i <= i2; i++) {
       double rr1 = (i == i1) ? r1 : T(F_R, i);
@@ -392,7 +469,10 @@ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, doubl
       nsteps += n - 1;
       for (int j = 1; j <= n - 1; j++) {
         double rr = rr1 + (double)(j - 1) * drr;
-        rk_step(T, s, i, rr, drr, y, f);
+        if constexpr (std::is_same<G, RadGroup>::value)
+          rk_step_grp(T, s, i, rr, drr, y, f, grp);
+        else
+          rk_step(T, s, i, rr, drr, y, f);
         if (scale) renorm<F::N>(y, *scale);
       }
     }
@@ -407,7 +487,10 @@ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, doubl
       nsteps += n - 1;
       for (int j = 1; j <= n - 1; j++) {
         double rr = rr1 + (double)(j - 1) * drr;
-        rk_step(T, s, i, rr, drr, y, f);
+        if constexpr (std::is_same<G, RadGroup>::value)
+          rk_step_grp(T, s, i, rr, drr, y, f, grp);
+        else
+          rk_step(T, s, i, rr, drr, y, f);
         if (scale) renorm<F::N>(y, *scale);
       }
     }

@@ -180,10 +180,17 @@ __device__ inline void grid_runs(const Tab& T, int i, double drr, double rs, dou
   }
 }
 
+// A group of 8 lanes per frequency (RAD_GROUP) integrates one solution:
+// the control flow depends on w only, so the group follows it together and
+// shares the stage evaluations of every step (rk_step_grp); lane 0 writes.
+constexpr int RAD_GROUP = 8;
+
 __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
-                                         cd* __restrict__ rec, int* __restrict__ cnt) {
-  int j = jb * 64 + threadIdx.x + 1;
+                                         cd* __restrict__ rec, int* __restrict__ cnt,
+                                         RadRHS::Coef (*rsh)[6]) {
+  int j = jb * (64 / RAD_GROUP) + (int)threadIdx.x / RAD_GROUP + 1;
   if (j > R.nw || R.lmin != 0) return;
+  const RadGroup grp{(int)threadIdx.x % RAD_GROUP, rsh[threadIdx.x / RAD_GROUP]};
   size_t p = (size_t)R.pbase + (j - 1);  // l = 0 pairs come first
   PS s;
   pair_state(R, 0, j, s);
@@ -211,9 +218,9 @@ __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
           if (k == 2) {
             double rt = fmin(0.1 * dr, 0.1 * rg);
             rad_ystart(T, s, 1, rt, y1);
-            yint(T, s, nk, 1, i, rt, rg, y1, f, (int*)nullptr, nsr);
+            yint(T, s, nk, 1, i, rt, rg, y1, f, (int*)nullptr, nsr, grp);
           } else if (k > 2) {
-            yint(T, s, nk, iprev, i, rprev, rg, y1, f, (int*)nullptr, nsr);
+            yint(T, s, nk, iprev, i, rprev, rg, y1, f, (int*)nullptr, nsr, grp);
           }
           rprev = rg;
           iprev = i;
@@ -243,7 +250,7 @@ __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
       for (int q = G.n - 1; q >= 0 && !done; q--) {
         for (int jj = G.j1[q]; jj >= G.j0[q]; jj--) {
           double rg = G.base[q] + (double)(jj - 1) * G.step[q];
-          if (!first) yint(T, s, nk, iprev, i, rprev, rg, y2, f, (int*)nullptr, nsr);
+          if (!first) yint(T, s, nk, iprev, i, rprev, rg, y2, f, (int*)nullptr, nsr, grp);
           first = false;
           rprev = rg;
           iprev = i;
@@ -275,26 +282,30 @@ __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
     yo[2] = -delta * y2r[0] * b2;
     yo[3] = -delta * y2r[1] * b2;
   }
+  if (grp.q != 0) return;
   cd* o = rec + p * REC + O_YRAD;
   for (int i = 0; i < 4; i++) o[i] = yo[i];
   cnt[5 * p + 4] = nsr;
 }
 
-// items (event, block of 64 frequencies); one lane per l = 0 pair, no
-// lockstep (the grid of make_grid depends on w)
+// items (event, block of 8 frequencies); a lane group per l = 0 pair, no
+// lockstep between groups (the grid of make_grid depends on w)
 __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab,
                                             const RunK* __restrict__ runs, Queue Q,
                                             cd* __restrict__ rec, int* __restrict__ cnt) {
+  __shared__ RadRHS::Coef rsh[64 / RAD_GROUP][6];
   Tab T = load_tab(gtab, runs[0].nknot);
-  RUN_ITEMS(false, Q, rad_item(T, runs[it.x], it.y, rec, cnt));
+  RUN_ITEMS(false, Q, rad_item(T, runs[it.x], it.y, rec, cnt, rsh));
 }
 
 // rad_solver_ng (gravity switch 0, src/yspec.f90:271-273) for the l = 0
 // pairs of the events with grav = 0; same items as k_rad
 __device__ __forceinline__ void rad_ng_item(const Tab& T, const RunK& R, int jb,
-                                            cd* __restrict__ rec, int* __restrict__ cnt) {
-  int j = jb * 64 + threadIdx.x + 1;
+                                            cd* __restrict__ rec, int* __restrict__ cnt,
+                                            RadRHS::Coef (*rsh)[6]) {
+  int j = jb * (64 / RAD_GROUP) + (int)threadIdx.x / RAD_GROUP + 1;
   if (j > R.nw || R.lmin != 0 || R.grav != 0) return;
+  const RadGroup grp{(int)threadIdx.x % RAD_GROUP, rsh[threadIdx.x / RAD_GROUP]};
   size_t p = (size_t)R.pbase + (j - 1);
   PS s;
   pair_state(R, 0, j, s);
@@ -302,7 +313,8 @@ __device__ __forceinline__ void rad_ng_item(const Tab& T, const RunK& R, int jb,
            R.fre_norm};
   cd yo[4];
   int n = 0;
-  rad_solve_ng(T, s, G, yo, n);
+  rad_solve_ng(T, s, G, yo, n, grp);
+  if (grp.q != 0) return;
   cd* o = rec + p * REC + O_YRAD;
   for (int i = 0; i < 4; i++) o[i] = yo[i];
   cnt[5 * p + 4] = n;
@@ -311,8 +323,9 @@ __device__ __forceinline__ void rad_ng_item(const Tab& T, const RunK& R, int jb,
 __global__ void __launch_bounds__(64) k_rad_ng(const double* __restrict__ gtab,
                                                const RunK* __restrict__ runs, Queue Q,
                                                cd* __restrict__ rec, int* __restrict__ cnt) {
+  __shared__ RadRHS::Coef rsh[64 / RAD_GROUP][6];
   Tab T = load_tab(gtab, runs[0].nknot);
-  RUN_ITEMS(false, Q, rad_ng_item(T, runs[it.x], it.y, rec, cnt));
+  RUN_ITEMS(false, Q, rad_ng_item(T, runs[it.x], it.y, rec, cnt, rsh));
 }
 
 // ------------------------------------------------------------- k_sph_var

@@ -211,8 +211,9 @@ __device__ inline void rad_ystart_ng(const Tab& T, const PS& s, int is, double r
 // rad_solver_ng with ss = identity(2), src/module_solver.f90:335-441.  Its
 // RHS rad_derivs_ng (src/module_int.f90:377-392) evaluates rad_amat, the
 // self-gravitating coefficients; reproduced as written.  yout (2,2).
+template <class Grp>
 __device__ inline void rad_solve_ng(const Tab& T, const PS& s, const VarGeo& G, cd* yout,
-                                    int& nsteps) {
+                                    int& nsteps, const Grp& grp) {
   RadRHS f;
   int nk = G.nknot;
   double dr = 1.0 / (4.0 * s.rew / G.alpha_min);
@@ -222,18 +223,19 @@ __device__ inline void rad_solve_ng(const Tab& T, const PS& s, const VarGeo& G, 
   rad_ystart_ng(T, s, 1, rt, y1);
   bool up = G.rr >= G.rs;
   if (up) {
-    yint(T, s, nk, 1, G.is, rt, G.rs, y1, f, &sc1, nsteps);
-    if (G.rr != T(F_R, nk)) yint(T, s, nk, nk - 1, G.ir, T(F_R, nk), G.rr, y2, f, &sc2, nsteps);
+    yint(T, s, nk, 1, G.is, rt, G.rs, y1, f, &sc1, nsteps, grp);
+    if (G.rr != T(F_R, nk))
+      yint(T, s, nk, nk - 1, G.ir, T(F_R, nk), G.rr, y2, f, &sc2, nsteps, grp);
     y3[0] = y2[0];
     y3[1] = y2[1];
-    if (G.rs != G.rr) yint(T, s, nk, G.ir, G.is, G.rr, G.rs, y3, f, &sc3, nsteps);
+    if (G.rs != G.rr) yint(T, s, nk, G.ir, G.is, G.rr, G.rs, y3, f, &sc3, nsteps, grp);
   } else {
-    yint(T, s, nk, 1, G.ir, rt, G.rr, y1, f, &sc1, nsteps);
+    yint(T, s, nk, 1, G.ir, rt, G.rr, y1, f, &sc1, nsteps, grp);
     double rsl = T(F_R, G.nsl);
-    if (G.rs != rsl) yint(T, s, nk, G.nsl - 1, G.is, rsl, G.rs, y2, f, &sc2, nsteps);
+    if (G.rs != rsl) yint(T, s, nk, G.nsl - 1, G.is, rsl, G.rs, y2, f, &sc2, nsteps, grp);
     y3[0] = y2[0];
     y3[1] = y2[1];
-    if (G.rr != G.rs) yint(T, s, nk, G.is, G.ir, G.rs, G.rr, y3, f, &sc3, nsteps);
+    if (G.rr != G.rs) yint(T, s, nk, G.is, G.ir, G.rs, G.rr, y3, f, &sc3, nsteps, grp);
   }
   cd delta = y3[1] * y1[0] - y3[0] * y1[1];
   delta = (cabs_(delta) < 1.0e-100) ? C(0.0) : cinv(delta);
