@@ -129,12 +129,13 @@ int ysx_event_seismograms(ysx_ctx *ctx, int e, double *seis);
  * svt(5,2) ytor(2,2) svs(5,6) ysph(6,4) sr0(2) yrad(2,2) uu(5) vv(5) ww(5),
  * the values the reference's loop body holds after its source-vector and
  * solver calls (src/yspec.f90:270-400); the pairs must have been computed
- * by the last ysx_spectra. */
+ * by the last ysx_spectra (one-event contexts only). */
 int ysx_pair_records(ysx_ctx *ctx, int n, const int *l, const int *j,
                      double *rec);
 
 /* Cash-Karp step counts of n pairs (l[i], j[i]) (j 1-based) of the last
- * ysx_spectra, 5 per pair in the order of ysx_step_counts.  The reference
+ * ysx_spectra (one-event contexts only), 5 per pair in the order of
+ * ysx_step_counts.  The reference
  * keeps no such count; its fixed step rule (yint, src/module_int.f90:
  * 1903-1905) fixes them, so they expose each pair's start level and
  * integration path. */

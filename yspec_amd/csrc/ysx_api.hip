@@ -851,6 +851,7 @@ extern "C" int ysx_event_seismograms(ysx_ctx* ctx, int e, double* seis) {
 extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j, double* rec) {
   if (!ctx || (n > 0 && (!l || !j || !rec))) return YSX_ERR_ARG;
   if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: one-event diagnostic");
   HIPCHK(hipSetDevice(ctx->device));
   const RunK& k = ctx->rk[0];
   for (int i = 0; i < n; i++) {
@@ -866,6 +867,7 @@ extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j,
 extern "C" int ysx_pair_steps(ysx_ctx* ctx, int n, const int* l, const int* j, int* out5) {
   if (!ctx || (n > 0 && (!l || !j || !out5))) return YSX_ERR_ARG;
   if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: one-event diagnostic");
   HIPCHK(hipSetDevice(ctx->device));
   const RunK& k = ctx->rk[0];
   size_t P = (size_t)k.npair;
