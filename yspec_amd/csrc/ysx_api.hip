@@ -169,11 +169,13 @@ void build_work(ysx_ctx* c) {
       c->W[W_BVEC].plain.push_back(make_int4(e, j, lb, 0));
       cost[W_BVEC].push_back(rew(j) * (std::fabs(R.rr - R.rs) + 1e-3));
     }
-    if (R.lmin == 0)  // blocks of 8 frequencies, a group of 8 lanes each
-      for (int jb = 0; jb < (R.nw + 7) / 8; jb++) {
+    if (R.lmin == 0) {  // blocks of 64 / radg frequencies, radg lanes each
+      const int per = 64 / R.radg;
+      for (int jb = 0; jb < (R.nw + per - 1) / per; jb++) {
         c->W[W_RAD].plain.push_back(make_int4(e, jb, 0, 0));
-        cost[W_RAD].push_back(rew(std::min(R.nw, 8 * jb + 8)));
+        cost[W_RAD].push_back(rew(std::min(R.nw, per * jb + per)));
       }
+    }
     if (R.var)
       for (int pb = 0; pb < (R.npair + 63) / 64; pb++) {
         c->W[W_VAR].plain.push_back(make_int4(e, pb, 0, 0));
@@ -377,6 +379,10 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     k.ngc = (R.grav != 2) ? 1 : 0;
     k.var = (R.grav == 2 && M.fic && R.rr < R.rs) ? 1 : 0;
     k.lpad = (R.lmax + 1 + 3) / 4 * 4;
+    // radial solves: 8 lanes per frequency, or one when there are many
+    // frequencies (then the radial kernel runs hidden beside the marches and
+    // one lane each keeps it to few compute units)
+    k.radg = (R.nw <= 640) ? 8 : 1;
     k.jpad = (R.nw + 15) / 16 * 16;
     k.nrpad = (R.nr + 15) / 16 * 16;
     k.nt = R.nt;

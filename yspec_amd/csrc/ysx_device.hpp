@@ -320,11 +320,11 @@ struct RadRHS {
 };
 
 // Lanes of one integration: the default is one lane.  RadGroup: lane q of
-// a group of 8 that integrate the same solution; sh is the group's LDS slot
-// of six stage coefficient sets.
+// a group of G (1 or 8) lanes that integrate the same solution; sh is the
+// group's LDS slot of six stage coefficient sets.
 struct NoGroup {};
 struct RadGroup {
-  int q;
+  int q, G;
   RadRHS::Coef* sh;
 };
 
@@ -373,11 +373,12 @@ __device__ inline void rk_step(const Tab& T, const PS& s, int il, double x, doub
   for (int i = 0; i < N; i++) y[i] = y[i] + h * (k1[i] + c6 * k3[i]);
 }
 
-// rk_step for a lane group: lane q < 6 evaluates the coefficients of stage
-// q + 1 into the group's LDS slot, then every lane runs the stage updates
-// (the same arithmetic as rk_step, so every lane holds the same y).  LDS
-// accesses of one wave take effect in program order, so the slot needs no
-// barrier; groups of a wave may be at different points of their solves.
+// rk_step for a lane group of G = 8: lane q < 6 evaluates the coefficients
+// of stage q + 1 into the group's LDS slot, then every lane runs the stage
+// updates (the same arithmetic as rk_step, so every lane holds the same y).
+// LDS accesses of one wave take effect in program order, so the slot needs
+// no barrier; groups of a wave may be at different points of their solves.
+// G = 1 is the plain rk_step.
 template <class F>
 __device__ inline void rk_step_grp(const Tab& T, const PS& s, int il, double x, double h, cd* y,
                                    const F& f, const RadGroup& g) {
@@ -388,6 +389,10 @@ __device__ inline void rk_step_grp(const Tab& T, const PS& s, int il, double x, 
                    b61 = 1631.0 / 55296.0, b62 = 175.0 / 512.0, b63 = 575.0 / 13824.0,
                    b64 = 44275.0 / 110592.0, b65 = 253.0 / 4096.0, c1 = 37.0 / 378.0,
                    c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c6 = 512.0 / 1771.0;
+  if (g.G == 1) {
+    rk_step(T, s, il, x, h, y, f);
+    return;
+  }
   if (g.q < 6) {
     const int q = g.q;
     double aq = (q == 1) ? a2 : (q == 2) ? a3 : (q == 3) ? a4 : (q == 4) ? a5 : a6;

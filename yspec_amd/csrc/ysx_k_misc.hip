@@ -180,17 +180,20 @@ __device__ inline void grid_runs(const Tab& T, int i, double drr, double rs, dou
   }
 }
 
-// A group of 8 lanes per frequency (RAD_GROUP) integrates one solution:
-// the control flow depends on w only, so the group follows it together and
+// A group of R.radg lanes per frequency (8, or 1 for runs with many
+// frequencies, where the radial kernel is hidden beside the spheroidal
+// marches and should take few compute units) integrates one solution: the
+// control flow depends on w only, so the group follows it together and
 // shares the stage evaluations of every step (rk_step_grp); lane 0 writes.
-constexpr int RAD_GROUP = 8;
+constexpr int RAD_GROUP_MAX = 8;
 
 __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
                                          cd* __restrict__ rec, int* __restrict__ cnt,
                                          RadRHS::Coef (*rsh)[6]) {
-  int j = jb * (64 / RAD_GROUP) + (int)threadIdx.x / RAD_GROUP + 1;
+  const int G = R.radg;
+  int j = jb * (64 / G) + (int)threadIdx.x / G + 1;
   if (j > R.nw || R.lmin != 0) return;
-  const RadGroup grp{(int)threadIdx.x % RAD_GROUP, rsh[threadIdx.x / RAD_GROUP]};
+  const RadGroup grp{(int)threadIdx.x % G, G, rsh[(threadIdx.x / G) % RAD_GROUP_MAX]};
   size_t p = (size_t)R.pbase + (j - 1);  // l = 0 pairs come first
   PS s;
   pair_state(R, 0, j, s);
@@ -288,12 +291,12 @@ __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
   cnt[5 * p + 4] = nsr;
 }
 
-// items (event, block of 8 frequencies); a lane group per l = 0 pair, no
-// lockstep between groups (the grid of make_grid depends on w)
+// items (event, block of 64 / radg frequencies); a lane group per l = 0
+// pair, no lockstep between groups (the grid of make_grid depends on w)
 __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab,
                                             const RunK* __restrict__ runs, Queue Q,
                                             cd* __restrict__ rec, int* __restrict__ cnt) {
-  __shared__ RadRHS::Coef rsh[64 / RAD_GROUP][6];
+  __shared__ RadRHS::Coef rsh[64 / RAD_GROUP_MAX][6];
   Tab T = load_tab(gtab, runs[0].nknot);
   RUN_ITEMS(false, Q, rad_item(T, runs[it.x], it.y, rec, cnt, rsh));
 }
@@ -303,9 +306,10 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab,
 __device__ __forceinline__ void rad_ng_item(const Tab& T, const RunK& R, int jb,
                                             cd* __restrict__ rec, int* __restrict__ cnt,
                                             RadRHS::Coef (*rsh)[6]) {
-  int j = jb * (64 / RAD_GROUP) + (int)threadIdx.x / RAD_GROUP + 1;
+  const int ng = R.radg;
+  int j = jb * (64 / ng) + (int)threadIdx.x / ng + 1;
   if (j > R.nw || R.lmin != 0 || R.grav != 0) return;
-  const RadGroup grp{(int)threadIdx.x % RAD_GROUP, rsh[threadIdx.x / RAD_GROUP]};
+  const RadGroup grp{(int)threadIdx.x % ng, ng, rsh[(threadIdx.x / ng) % RAD_GROUP_MAX]};
   size_t p = (size_t)R.pbase + (j - 1);
   PS s;
   pair_state(R, 0, j, s);
@@ -323,7 +327,7 @@ __device__ __forceinline__ void rad_ng_item(const Tab& T, const RunK& R, int jb,
 __global__ void __launch_bounds__(64) k_rad_ng(const double* __restrict__ gtab,
                                                const RunK* __restrict__ runs, Queue Q,
                                                cd* __restrict__ rec, int* __restrict__ cnt) {
-  __shared__ RadRHS::Coef rsh[64 / RAD_GROUP][6];
+  __shared__ RadRHS::Coef rsh[64 / RAD_GROUP_MAX][6];
   Tab T = load_tab(gtab, runs[0].nknot);
   RUN_ITEMS(false, Q, rad_ng_item(T, runs[it.x], it.y, rec, cnt, rsh));
 }

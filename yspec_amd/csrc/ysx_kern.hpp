@@ -20,6 +20,7 @@ struct RunK {
   int grav, var, ngc;           // gravity switch; spheroidal solves per lane (var) or
                                 // by the gravity-0/1 lockstep march (ngc)
   int lpad, jpad, nrpad;        // GEMM padding of this event
+  int radg;                     // lanes per radial (l = 0) solve: 1 or 8
   int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
   long long pbase;              // first pair of this event in the per-pair arrays
   long long cf_off, tb_off, spec_off, seis_off;
