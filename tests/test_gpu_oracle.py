@@ -122,3 +122,39 @@ def test_cfg2_pair_steps(cfg2, api):
             drr = 1.0 / (4.0 * rew / tmp)
             want[jj] += 1 + int(np.floor(abs(b - a) / drr))
     assert np.array_equal(c[1:, :, 3], np.broadcast_to(want, (nl - 1, g.nw)))
+
+
+def _refine(deck, factor):
+    """The deck with every region resampled `factor` times finer (linear in
+    each column between the original knots; discontinuities kept)."""
+    from yspec_amd.inputs import Deck
+    c = deck.cols
+    cuts = [0] + [i + 1 for i in range(len(c) - 1) if c[i + 1, 0] == c[i, 0]] + [len(c)]
+    rows, nic, noc = [], None, None
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        seg = c[a:b]
+        t = np.linspace(0.0, len(seg) - 1.0, factor * (len(seg) - 1) + 1)
+        rows.append(np.stack([np.interp(t, np.arange(len(seg)), seg[:, k])
+                              for k in range(c.shape[1])], axis=1))
+        n = sum(len(r) for r in rows)
+        if b == deck.nic:
+            nic = n
+        if b == deck.noc:
+            noc = n
+    cols = np.concatenate(rows)
+    return Deck("refined", deck.ians, deck.tref, len(cols), nic, noc, cols)
+
+
+def test_large_deck_vs_oracle(api, oracle_mod):
+    """A deck of about 790 knots: its model table (29 doubles per knot) is
+    larger than the LDS of a block, so the kernels that stage the table in
+    LDS read it from memory instead; results as for any deck."""
+    deck = _refine(synth_deck(0), 4)
+    assert deck.nknot * 29 * 8 > 160 * 1024
+    p = workload("cfg1", lmax=20, nr=2, ats=1, src_depth=300.0)
+    g = api.YSpec(deck, p)
+    o = oracle_mod.Oracle(deck, p)
+    gs, os_ = g.spectra(), o.spectra(nthreads=16)
+    for a, b in zip(gs, os_):
+        assert rel_l2(a, b) <= 1e-9
+    assert rel_l2(g.seismograms(), o.seismograms(*os_)) <= 1e-8

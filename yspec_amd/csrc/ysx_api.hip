@@ -67,7 +67,8 @@ struct ysx_ctx {
   std::vector<HostRun> R;  // one per event
   std::vector<RunK> rk;
   bool have_model = false, have_run = false, have_spec = false, have_seis = false;
-  size_t lds_bytes = 0, lds_max = 0;
+  size_t lds_bytes = 0, lds_max = 0;  // model table bytes, LDS per block
+  bool tab_lds = true;                 // the table fits in LDS beside the kernels' own
   long long npair = 0;  // all events
   // device buffers
   double* d_tab = nullptr;
@@ -277,8 +278,9 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
   std::string e = build_model(ctx->N, nknot, nic, noc, ians, tref_period, deck, ctx->M);
   if (!e.empty()) return set_err(ctx, YSX_ERR_INPUT, e);
   ctx->lds_bytes = sizeof(double) * F_COUNT * (size_t)nknot;
-  if (ctx->lds_bytes > ctx->lds_max)
-    return set_err(ctx, YSX_ERR_UNSUPPORTED, "deck has too many knots for the LDS-resident table");
+  // k_source and the radial kernels stage the table in LDS when it fits
+  // (with room for their own 2.3 KB); larger decks are read through the cache
+  ctx->tab_lds = ctx->lds_bytes + 4096 <= ctx->lds_max;
   HIPCHK(hipMalloc(&ctx->d_tab, ctx->lds_bytes));
   // the table the kernels read is built on the device from the raw deck;
   // the host copy (same functions) serves the run set-up
@@ -383,6 +385,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     // frequencies (then the radial kernel runs hidden beside the marches and
     // one lane each keeps it to few compute units)
     k.radg = (R.nw <= 640) ? 8 : 1;
+    k.tablds = ctx->tab_lds ? 1 : 0;
     k.jpad = (R.nw + 15) / 16 * 16;
     k.nrpad = (R.nr + 15) / 16 * 16;
     k.nt = R.nt;
@@ -538,7 +541,7 @@ PairRange pairs_of(const ysx_ctx* c, int e0, int e1) {
 int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
   hipStream_t st = ctx->stream, st2 = ctx->stream2;
   hipEvent_t* ev = ctx->ev;
-  size_t lds = ctx->lds_bytes;
+  size_t lds = ctx->tab_lds ? ctx->lds_bytes : 0;
   dim3 b256(256), b64(64);
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
@@ -628,7 +631,7 @@ int enqueue_seis_plain(ysx_ctx* ctx, int e0, int e1) {
 int enqueue_batched(ysx_ctx* ctx) {
   hipStream_t st = ctx->stream, st2 = ctx->stream2;
   hipEvent_t* ev = ctx->ev;
-  size_t lds = ctx->lds_bytes;
+  size_t lds = ctx->tab_lds ? ctx->lds_bytes : 0;
   dim3 b256(256), b64(64);
   int nev = (int)ctx->rk.size();
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};

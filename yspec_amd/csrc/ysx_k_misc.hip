@@ -20,7 +20,7 @@ __global__ void k_reset(int* counters, int n) {
 __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
                                                 const RunK* __restrict__ runs, PairRange PR,
                                                 cd* __restrict__ rec, int* __restrict__ cnt) {
-  Tab T = load_tab(gtab, runs[PR.e0].nknot);
+  Tab T = load_tab(gtab, runs[PR.e0].nknot, runs[PR.e0].tablds);
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= PR.n) return;
   long long g = PR.g0 + t;
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(64) k_rad(const double* __restrict__ gtab,
                                             const RunK* __restrict__ runs, Queue Q,
                                             cd* __restrict__ rec, int* __restrict__ cnt) {
   __shared__ RadRHS::Coef rsh[64 / RAD_GROUP_MAX][6];
-  Tab T = load_tab(gtab, runs[0].nknot);
+  Tab T = load_tab(gtab, runs[0].nknot, runs[0].tablds);
   RUN_ITEMS(false, Q, rad_item(T, runs[it.x], it.y, rec, cnt, rsh));
 }
 
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(64) k_rad_ng(const double* __restrict__ gtab,
                                                const RunK* __restrict__ runs, Queue Q,
                                                cd* __restrict__ rec, int* __restrict__ cnt) {
   __shared__ RadRHS::Coef rsh[64 / RAD_GROUP_MAX][6];
-  Tab T = load_tab(gtab, runs[0].nknot);
+  Tab T = load_tab(gtab, runs[0].nknot, runs[0].tablds);
   RUN_ITEMS(false, Q, rad_ng_item(T, runs[it.x], it.y, rec, cnt, rsh));
 }
 

@@ -21,6 +21,7 @@ struct RunK {
                                 // by the gravity-0/1 lockstep march (ngc)
   int lpad, jpad, nrpad;        // GEMM padding of this event
   int radg;                     // lanes per radial (l = 0) solve: 1 or 8
+  int tablds;                   // model table staged in LDS (else read from HBM)
   int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
   long long pbase;              // first pair of this event in the per-pair arrays
   long long cf_off, tb_off, spec_off, seis_off;
@@ -103,8 +104,10 @@ __device__ inline void pair_state(const RunK& R, int l, int j, PS& s) {
   set_omega(s, w, R.tref, R.ats);
 }
 
-// copy the model table into LDS; every thread of the block must call it
-__device__ inline Tab load_tab(const double* __restrict__ g, int nknot) {
+// copy the model table into LDS (in_lds; every thread of the block must
+// call it), or read it through the cache when it does not fit
+__device__ inline Tab load_tab(const double* __restrict__ g, int nknot, int in_lds) {
+  if (!in_lds) return Tab{g, nknot};
   extern __shared__ double smem[];
   int n = F_COUNT * nknot;
   for (int i = threadIdx.x; i < n; i += blockDim.x) smem[i] = g[i];
