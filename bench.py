@@ -144,6 +144,9 @@ def batched_compare(deck, steps, warmup, do_tune=True):
         "speedup": res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"],
         "batched_ode_tflops_lower_bound": sum(ode.values()) / (res["batched"]["ms_per_step"]
                                                                * 1e-3) / 1e12,
+        # the same over the FP64 peak (the whole forward's time, so a lower bound)
+        "batched_ode_frac_lower_bound": sum(ode.values()) / (res["batched"]["ms_per_step"]
+                                                             * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "tuning": tuning["best"] if tuning else None,
     }
 
@@ -231,6 +234,8 @@ def main():
         "ode_solves_per_s": P / (ode_ms * 1e-3),
         "phase_ms": {k: round(float(v), 4) for k, v in ms.items()},
         "ode_tflops": ode_flop / (ode_ms * 1e-3) / 1e12,
+        # solves/s as a fraction of the FP64 roofline: ODE FLOP rate / peak
+        "ode_roofline_frac": ode_flop / (ode_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "step_counts": {k: int(v) for k, v in zip(("tor", "solid", "fluid", "bvec", "rad"),
                                                   counts)},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved,
