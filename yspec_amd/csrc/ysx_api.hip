@@ -144,9 +144,9 @@ void free_run(ysx_ctx* c) {
   c->have_run = c->have_spec = c->have_seis = false;
 }
 
-// Work lists of every kernel kind.  Plain order per event is the order of
-// one launch per event (largest w first for the marches).  The batched
-// lists hold every event's items, sorted by an estimate of their cost:
+// Work lists of every kernel kind.  The plain list holds each event's items
+// (one launch per event), the batched lists every event's items, both
+// sorted by an estimate of their cost, most expensive first:
 // steps grow with Re w; a wave of degrees starting at l0 integrates from
 // about the turning radius (l0 + 1/2) v / w (v ~ 1 in normalised units) up
 // to the top point; the b-vector runs between source and receiver.
@@ -221,6 +221,26 @@ void build_work(ysx_ctx* c) {
       }
     }
     for (int k = 0; k < W_N; k++) c->W[k].start.push_back((int)c->W[k].plain.size());
+  }
+  // plain order: each event's items by decreasing estimated cost as well
+  // (stable, so lists of equal costs keep their order)
+  for (int k = 0; k < W_N; k++) {
+    WorkList& w = c->W[k];
+    for (int e = 0; e < nev; e++) {
+      int a = w.start[e], b = w.start[e + 1];
+      std::vector<int> ord(b - a);
+      std::iota(ord.begin(), ord.end(), a);
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](int x, int y) { return cost[k][x] > cost[k][y]; });
+      std::vector<int4> items(b - a);
+      std::vector<double> cs(b - a);
+      for (int i = 0; i < b - a; i++) {
+        items[i] = w.plain[ord[i]];
+        cs[i] = cost[k][ord[i]];
+      }
+      std::copy(items.begin(), items.end(), w.plain.begin() + a);
+      std::copy(cs.begin(), cs.end(), cost[k].begin() + a);
+    }
   }
   for (int k = 0; k < W_N; k++) {
     WorkList& w = c->W[k];
