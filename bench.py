@@ -246,6 +246,9 @@ def main():
                              "78.6 TFLOP/s; traffic = HBM bytes per launch from rocprofv3 "
                              "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy"},
         "host_copy_ms_per_forward": pcie_ms,
+        # the metric's third part: no discrete adjoint is built, so there is
+        # no gradient time to report
+        "gradient_over_forward": None,
     }
     g.close()
     if not args.no_batched and args.lmax is None:
