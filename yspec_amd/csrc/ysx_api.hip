@@ -67,6 +67,7 @@ struct ysx_ctx {
   std::vector<HostRun> R;  // one per event
   std::vector<RunK> rk;
   bool have_model = false, have_run = false, have_spec = false, have_seis = false;
+  bool refined = false;  // wave lists reordered by measured cost (refine_work)
   size_t lds_bytes = 0, lds_max = 0;  // model table bytes, LDS per block
   bool tab_lds = true;                 // the table fits in LDS beside the kernels' own
   long long npair = 0;  // all events
@@ -251,6 +252,52 @@ void build_work(ysx_ctx* c) {
     w.batched.resize(ord.size());
     for (size_t i = 0; i < ord.size(); i++) w.batched[i] = w.plain[ord[i]];
   }
+}
+
+// After the first forward of a run the exact cost of every minor-march wave
+// is known from the per-pair step counts: the wave marches as long as its
+// longest lane (5102 FLOP per solid step, 1697 per fluid step).  The wave
+// lists (shared by the toroidal and minor marches) are re-sorted by it,
+// most expensive first, once per run; results do not depend on the order.
+int refine_work(ysx_ctx* c) {
+  if (c->refined) return YSX_OK;
+  c->refined = true;
+  size_t P = (size_t)c->npair;
+  std::vector<int> cnt(P * 5);
+  {
+    ysx_ctx* ctx = c;
+    HIPCHK(hipMemcpy(cnt.data(), c->d_cnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  auto wave_cost = [&](const int4& it) {
+    const RunK& R = c->rk[it.x];
+    double m = 0.0;
+    for (int q = 0; q < 64; q++) {
+      int l = R.lmin + it.z * 64 + q;
+      if (l > R.lmax) break;
+      size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (it.y - 1);
+      m = std::max(m, 5102.0 * cnt[5 * p + 1] + 1697.0 * cnt[5 * p + 2]);
+    }
+    return m;
+  };
+  WorkList& w = c->W[W_TOR];
+  auto resort = [&](int4* first, int4* last) {
+    std::vector<std::pair<double, int4>> v;
+    for (int4* i = first; i != last; ++i) v.push_back({wave_cost(*i), *i});
+    std::stable_sort(v.begin(), v.end(),
+                     [](const auto& a, const auto& b) { return a.first > b.first; });
+    for (size_t k = 0; k < v.size(); k++) first[k] = v[k].second;
+  };
+  for (size_t e = 0; e + 1 < w.start.size(); e++)
+    resort(w.plain.data() + w.start[e], w.plain.data() + w.start[e + 1]);
+  resort(w.batched.data(), w.batched.data() + w.batched.size());
+  ysx_ctx* ctx = c;
+  if (!w.plain.empty()) {
+    HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
+                     hipMemcpyHostToDevice));
+  }
+  return YSX_OK;
 }
 
 }  // namespace
@@ -508,6 +555,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->have_run = true;
+  ctx->refined = false;
   return YSX_OK;
 }
 
@@ -747,7 +795,7 @@ int run_spectra(ysx_ctx* ctx) {
     return rc;
   ctx->have_spec = true;
   ctx->have_seis = false;
-  return YSX_OK;
+  return refine_work(ctx);
 }
 
 int run_seis(ysx_ctx* ctx) {
@@ -846,6 +894,7 @@ extern "C" int ysx_forward_events(ysx_ctx* ctx, int mode, double* seis) {
   for (double& m : ctx->ms) m = 0.0;
   ctx->ms[7] = t;
   ctx->have_spec = ctx->have_seis = true;
+  if (int rc2 = refine_work(ctx)) return rc2;
   if (seis) {
     size_t n = 0;
     for (const HostRun& R : ctx->R) n += 3 * (size_t)R.nt * R.nr;
