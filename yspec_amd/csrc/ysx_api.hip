@@ -254,11 +254,13 @@ void build_work(ysx_ctx* c) {
   }
 }
 
-// After the first forward of a run the exact cost of every minor-march wave
-// is known from the per-pair step counts: the wave marches as long as its
-// longest lane (5102 FLOP per solid step, 1697 per fluid step).  The wave
-// lists (shared by the toroidal and minor marches) are re-sorted by it,
-// most expensive first, once per run; results do not depend on the order.
+// After the first forward of a run the exact cost of every march wave is
+// known from the per-pair step counts: a minor-march wave marches as long
+// as its longest lane (5102 FLOP per solid step, 1697 per fluid step), a
+// b-vector wave as long as its pairs' b-vector steps.  The wave lists
+// (W_TOR, shared by the toroidal and minor marches, and W_BVEC) are
+// re-sorted by them, most expensive first, once per run; results do not
+// depend on the order.
 int refine_work(ysx_ctx* c) {
   if (c->refined) return YSX_OK;
   c->refined = true;
@@ -268,34 +270,41 @@ int refine_work(ysx_ctx* c) {
     ysx_ctx* ctx = c;
     HIPCHK(hipMemcpy(cnt.data(), c->d_cnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost));
   }
-  auto wave_cost = [&](const int4& it) {
+  // lanes of a wave: 64 degrees (W_TOR) or 16 degrees (W_BVEC)
+  auto wave_cost = [&](const int4& it, int kind) {
     const RunK& R = c->rk[it.x];
+    const int nl = (kind == W_TOR) ? 64 : 16;
     double m = 0.0;
-    for (int q = 0; q < 64; q++) {
-      int l = R.lmin + it.z * 64 + q;
+    for (int q = 0; q < nl; q++) {
+      int l = R.lmin + it.z * nl + q;
       if (l > R.lmax) break;
+      if (l < 1) continue;
       size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (it.y - 1);
-      m = std::max(m, 5102.0 * cnt[5 * p + 1] + 1697.0 * cnt[5 * p + 2]);
+      double v = (kind == W_TOR) ? 5102.0 * cnt[5 * p + 1] + 1697.0 * cnt[5 * p + 2]
+                                 : 16100.0 * cnt[5 * p + 3];
+      m = std::max(m, v);
     }
     return m;
   };
-  WorkList& w = c->W[W_TOR];
-  auto resort = [&](int4* first, int4* last) {
-    std::vector<std::pair<double, int4>> v;
-    for (int4* i = first; i != last; ++i) v.push_back({wave_cost(*i), *i});
-    std::stable_sort(v.begin(), v.end(),
-                     [](const auto& a, const auto& b) { return a.first > b.first; });
-    for (size_t k = 0; k < v.size(); k++) first[k] = v[k].second;
-  };
-  for (size_t e = 0; e + 1 < w.start.size(); e++)
-    resort(w.plain.data() + w.start[e], w.plain.data() + w.start[e + 1]);
-  resort(w.batched.data(), w.batched.data() + w.batched.size());
   ysx_ctx* ctx = c;
-  if (!w.plain.empty()) {
-    HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
-                     hipMemcpyHostToDevice));
+  for (int kind : {(int)W_TOR, (int)W_BVEC}) {
+    WorkList& w = c->W[kind];
+    auto resort = [&](int4* first, int4* last) {
+      std::vector<std::pair<double, int4>> v;
+      for (int4* i = first; i != last; ++i) v.push_back({wave_cost(*i, kind), *i});
+      std::stable_sort(v.begin(), v.end(),
+                       [](const auto& a, const auto& b) { return a.first > b.first; });
+      for (size_t k = 0; k < v.size(); k++) first[k] = v[k].second;
+    };
+    for (size_t e = 0; e + 1 < w.start.size(); e++)
+      resort(w.plain.data() + w.start[e], w.plain.data() + w.start[e + 1]);
+    resort(w.batched.data(), w.batched.data() + w.batched.size());
+    if (!w.plain.empty()) {
+      HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+    }
   }
   return YSX_OK;
 }
