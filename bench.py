@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the ODE kernels at this workload: rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB device copy),
 # profiles/forward_cfg2_hbm_traffic.md
-TRAFFIC_BYTES = {"bvec": 1368.7e6, "minors": 1541.0e6, "tor": 204.8e6, "lsum": 478.0e6}
+TRAFFIC_BYTES = {"bvec": 1368.7e6, "minors": 1560.5e6, "tor": 200.8e6, "lsum": 477.0e6}
 PHASES = ("source", "tor", "minors", "bvec", "rad", "assemble", "lsum", "ifft")
 
 
