@@ -130,9 +130,24 @@ struct SolidB {
   }
 };
 
+// First term of output i of a derivative chain: the plain product, or the
+// product added to an initial value g(i) (a Runge-Kutta combination that
+// takes this stage's derivative once, folded into the chain).
+struct NoInit {
+  __device__ inline cd r(int, double a, cd m) const { return rmul(a, m); }
+  __device__ inline cd c(int, cd a, cd m) const { return cmul_(a, m); }
+};
+template <class G>
+struct AccInit {
+  const G& g;
+  __device__ inline cd r(int i, double a, cd m) const { return rmad(g(i), a, m); }
+  __device__ inline cd c(int i, cd a, cd m) const { return cmad(g(i), a, m); }
+};
+
 // sph_minors_solid_derivs, src/module_int.f90:818-871, as FMA chains
-template <class Sink>
-__device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, Sink&& sink) {
+template <class Init, class Sink>
+__device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, const Init& I,
+                                          Sink&& sink) {
   const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
                s23 = A.s23;
   const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
@@ -141,42 +156,42 @@ __device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, Sink&& s
   const cd a123 = t11 + tp, a126 = t11 + tm, a135 = t11 - tm, a234 = tp - t11;
   const cd t12x2 = 2.0 * t12, s12x2 = 2.0 * s12;
   cd acc;
-  acc = rmul(k33, m[3]); acc = cmsb(acc, k22, m[5]); acc = cmad(acc, k11, m[9]);
+  acc = I.r(0, k33, m[3]); acc = cmsb(acc, k22, m[5]); acc = cmad(acc, k11, m[9]);
   sink(0, cmad(acc, m[0], a123));
-  acc = rmul(s13, m[0]); acc = cmsb(acc, k22, m[6]); acc = rmsb(acc, t21, m[2]);
+  acc = I.r(1, s13, m[0]); acc = cmsb(acc, k22, m[6]); acc = rmsb(acc, t21, m[2]);
   acc = rmad(acc, t22, m[1]); sink(1, rmsb(acc, t31, m[3]));
-  acc = rmul(s23, m[0]); acc = cmsb(acc, k11, m[10]); acc = cmad(acc, t11, m[2]);
+  acc = I.r(2, s23, m[0]); acc = cmsb(acc, k11, m[10]); acc = cmad(acc, t11, m[2]);
   sink(2, cmsb(acc, t12, m[1]));
-  acc = cmul_(k22, m[8]); acc = cmsb(acc, k11, m[11]); sink(3, cmad(acc, a126, m[3]));
-  acc = cmul_(t12, m[9]); acc = rmsb(acc, k33, m[7]); acc = cmsb(acc, s12, m[0]);
+  acc = I.c(3, k22, m[8]); acc = cmsb(acc, k11, m[11]); sink(3, cmad(acc, a126, m[3]));
+  acc = I.c(4, t12, m[9]); acc = rmsb(acc, k33, m[7]); acc = cmsb(acc, s12, m[0]);
   acc = rmsb(acc, t21, m[5]); acc = rmad(acc, t31, m[2]); sink(4, rmad(acc, t33, m[4]));
-  acc = cmul_(a135, m[5]); acc = rmsb(acc, k33, m[8]); acc = cmsb(acc, k11, m[12]);
+  acc = I.c(5, a135, m[5]); acc = rmsb(acc, k33, m[8]); acc = cmsb(acc, k11, m[12]);
   acc = cmsb(acc, s22, m[0]); sink(5, cmsb(acc, t12x2, m[4]));
-  acc = cmul_(s12, m[2]); acc = rmad(acc, s13, m[5]); acc = cmsb(acc, s22, m[1]);
+  acc = I.c(6, s12, m[2]); acc = rmad(acc, s13, m[5]); acc = cmsb(acc, s22, m[1]);
   acc = rmsb(acc, s23, m[4]); acc = cmad(acc, t12, m[10]); acc = rmsb(acc, t22, m[6]);
   sink(6, rmad(acc, t31, m[8]));
-  acc = cmul_(s12, m[3]); acc = rmsb(acc, s13, m[2]); acc = rmsb(acc, s23, m[1]);
+  acc = I.c(7, s12, m[3]); acc = rmsb(acc, s13, m[2]); acc = rmsb(acc, s23, m[1]);
   acc = cmad(acc, t12, m[11]); acc = rmsb(acc, t21, m[8]); sink(7, rmsb(acc, t33, m[7]));
-  acc = cmul_(k11, m[13]); acc = cmad(acc, s22, m[3]); acc = rmsb(acc, 2.0 * s23, m[2]);
+  acc = I.c(8, k11, m[13]); acc = cmad(acc, s22, m[3]); acc = rmsb(acc, 2.0 * s23, m[2]);
   acc = cmsb(acc, t12x2, m[7]); sink(8, cmsb(acc, a234, m[8]));
-  acc = cmul_(k22, m[12]); acc = rmsb(acc, k33, m[11]); acc = cmad(acc, s11, m[0]);
+  acc = I.c(9, k22, m[12]); acc = rmsb(acc, k33, m[11]); acc = cmad(acc, s11, m[0]);
   acc = rmad(acc, 2.0 * t21, m[4]); acc = rmsb(acc, 2.0 * t31, m[1]);
   sink(9, cmad(acc, a234, m[9]));
-  acc = cmul_(s12, m[1]); acc = cmsb(acc, s11, m[2]); acc = rmsb(acc, s13, m[4]);
+  acc = I.c(10, s12, m[1]); acc = cmsb(acc, s11, m[2]); acc = rmsb(acc, s13, m[4]);
   acc = rmsb(acc, s23, m[9]); acc = cmsb(acc, t11, m[10]); acc = rmad(acc, t21, m[6]);
   sink(10, rmsb(acc, t31, m[7]));
-  acc = rmul(2.0 * s13, m[1]); acc = cmsb(acc, k22, m[13]); acc = cmsb(acc, s11, m[3]);
+  acc = I.r(11, 2.0 * s13, m[1]); acc = cmsb(acc, k22, m[13]); acc = cmsb(acc, s11, m[3]);
   acc = rmad(acc, 2.0 * t21, m[7]); sink(11, cmsb(acc, a135, m[11]));
-  acc = rmul(k33, m[13]); acc = cmsb(acc, s11, m[5]); acc = cmad(acc, s12x2, m[4]);
+  acc = I.r(12, k33, m[13]); acc = cmsb(acc, s11, m[5]); acc = cmad(acc, s12x2, m[4]);
   acc = cmad(acc, s22, m[9]); acc = rmad(acc, 2.0 * t31, m[6]); sink(12, cmsb(acc, a126, m[12]));
-  acc = cmul_(s11, m[8]); acc = cmsb(acc, s12x2, m[7]); acc = rmad(acc, 2.0 * s13, m[6]);
+  acc = I.c(13, s11, m[8]); acc = cmsb(acc, s12x2, m[7]); acc = rmad(acc, 2.0 * s13, m[6]);
   acc = cmsb(acc, s22, m[11]); acc = rmad(acc, 2.0 * s23, m[10]);
   sink(13, cmsb(acc, a123, m[13]));
 }
 
 // sph_bvec_solid_derivs, src/module_int.f90:1016-1057, as FMA chains
-template <class Sink>
-__device__ inline void bvec_derivs(const SolidA& A, const cd* b, Sink&& sink) {
+template <class Init, class Sink>
+__device__ inline void bvec_derivs(const SolidA& A, const cd* b, const Init& I, Sink&& sink) {
   const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
                s23 = A.s23;
   const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
@@ -185,33 +200,33 @@ __device__ inline void bvec_derivs(const SolidA& A, const cd* b, Sink&& sink) {
   const double t2233 = t22 + t33;
   const cd b38m = b[7] - b[2], b2p8 = 2.0 * b[2] + b[7], b3p28 = b[2] + 2.0 * b[7];
   cd acc;
-  acc = cmul_(s11, b[6]); acc = cmad(acc, s12, b38m); acc = rmad(acc, s13, b[8]);
+  acc = I.c(0, s11, b[6]); acc = cmad(acc, s12, b38m); acc = rmad(acc, s13, b[8]);
   acc = cmsb(acc, s22, b[3]); acc = rmsb(acc, s23, b[4]); acc = cmsb(acc, t1122, b[0]);
   sink(0, rmad(acc, t31, b[5]));
-  acc = cmul_(s11, b[9]); acc = cmad(acc, s12, b[10]); acc = rmsb(acc, s13, b2p8);
+  acc = I.c(1, s11, b[9]); acc = cmad(acc, s12, b[10]); acc = rmsb(acc, s13, b2p8);
   acc = rmsb(acc, s23, b[3]); acc = rmsb(acc, t21, b[5]); sink(1, cmsb(acc, t1133, b[1]));
-  acc = cmul_(s12, b[11]); acc = rmad(acc, s13, b[12]); acc = cmad(acc, t12, b[3]);
+  acc = I.c(2, s12, b[11]); acc = rmad(acc, s13, b[12]); acc = cmad(acc, t12, b[3]);
   acc = rmsb(acc, t21, b[6]); sink(2, rmsb(acc, t31, b[9]));
-  acc = rmul(s13, b[13]); acc = cmsb(acc, k22, b[0]); acc = cmsb(acc, s11, b[11]);
+  acc = I.r(3, s13, b[13]); acc = cmsb(acc, k22, b[0]); acc = cmsb(acc, s11, b[11]);
   acc = rmsb(acc, t21, b38m); acc = cmad(acc, t22 - t11, b[3]); sink(3, rmsb(acc, t31, b[10]));
-  acc = rmul(t31, b2p8); acc = rmsb(acc, k33, b[1]); acc = cmsb(acc, s11, b[12]);
+  acc = I.r(4, t31, b2p8); acc = rmsb(acc, k33, b[1]); acc = cmsb(acc, s11, b[12]);
   acc = cmsb(acc, s12, b[13]); acc = rmsb(acc, t21, b[8]); sink(4, cmad(acc, t33 - t11, b[4]));
-  acc = cmul_(s12, b[9]); acc = rmsb(acc, s13, b[6]); acc = cmad(acc, s22, b[10]);
+  acc = I.c(5, s12, b[9]); acc = rmsb(acc, s13, b[6]); acc = cmad(acc, s22, b[10]);
   acc = rmsb(acc, s23, b3p28); acc = cmsb(acc, t12, b[1]); sink(5, rmsb(acc, t2233, b[5]));
-  acc = cmul_(k11, b[0]); acc = cmad(acc, s22, b[11]); acc = rmad(acc, s23, b[12]);
+  acc = I.c(6, k11, b[0]); acc = cmad(acc, s22, b[11]); acc = rmad(acc, s23, b[12]);
   acc = cmad(acc, t12, b38m); sink(6, cmad(acc, t11 - t22, b[6]));
-  acc = rmul(s23, b[13]); acc = cmsb(acc, s12, b[11]); acc = cmsb(acc, t12, b[3]);
+  acc = I.r(7, s23, b[13]); acc = cmsb(acc, s12, b[11]); acc = cmsb(acc, t12, b[3]);
   sink(7, rmad(acc, t21, b[6]));
-  acc = rmul(t31, b[6]); acc = rmsb(acc, k33, b[5]); acc = cmsb(acc, s12, b[12]);
+  acc = I.r(8, t31, b[6]); acc = rmsb(acc, k33, b[5]); acc = cmsb(acc, s12, b[12]);
   acc = cmsb(acc, s22, b[13]); acc = cmsb(acc, t12, b[4]); sink(8, rmad(acc, t33 - t22, b[8]));
-  acc = cmul_(k11, b[1]); acc = rmad(acc, s23, b[11]); acc = cmad(acc, t12, b[10]);
+  acc = I.c(9, k11, b[1]); acc = rmad(acc, s23, b[11]); acc = cmad(acc, t12, b[10]);
   sink(9, cmad(acc, t11 - t33, b[9]));
-  acc = cmul_(k22, b[5]); acc = rmsb(acc, s13, b[11]); acc = rmad(acc, t21, b[9]);
+  acc = I.c(10, k22, b[5]); acc = rmsb(acc, s13, b[11]); acc = rmad(acc, t21, b[9]);
   sink(10, rmad(acc, t22 - t33, b[10]));
-  acc = cmul_(k22, b[6]); acc = cmsb(acc, k11, b[3]); sink(11, cmad(acc, t1122, b[11]));
-  acc = rmul(k33, b[9]); acc = cmsb(acc, k11, b[4]); acc = cmad(acc, t12, b[13]);
+  acc = I.c(11, k22, b[6]); acc = cmsb(acc, k11, b[3]); sink(11, cmad(acc, t1122, b[11]));
+  acc = I.r(12, k33, b[9]); acc = cmsb(acc, k11, b[4]); acc = cmad(acc, t12, b[13]);
   sink(12, cmad(acc, t1133, b[12]));
-  acc = rmul(k33, b[10]); acc = cmsb(acc, k22, b[8]); acc = rmad(acc, t21, b[12]);
+  acc = I.r(13, k33, b[10]); acc = cmsb(acc, k22, b[8]); acc = rmad(acc, t21, b[12]);
   acc = rmsb(acc, t31, b[11]); sink(13, rmad(acc, t2233, b[13]));
 }
 
@@ -222,7 +237,15 @@ struct SolidMinorM : SolidB {
                                 Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
-    solid_minor_derivs(A, y, sink);
+    solid_minor_derivs(A, y, NoInit(), sink);
+  }
+  // sink(i, g(i) + h A y) for a base scaled by h
+  template <class G, class Sink>
+  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+                                    const G& g, Sink&& sink) const {
+    SolidA A;
+    amat(b, L, A);
+    solid_minor_derivs(A, y, AccInit<G>{g}, sink);
   }
   __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
                                     cd* d) const {
@@ -236,7 +259,14 @@ struct BvecM : SolidB {
                                 Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
-    bvec_derivs(A, y, sink);
+    bvec_derivs(A, y, NoInit(), sink);
+  }
+  template <class G, class Sink>
+  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+                                    const G& g, Sink&& sink) const {
+    SolidA A;
+    amat(b, L, A);
+    bvec_derivs(A, y, AccInit<G>{g}, sink);
   }
   __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
                                     cd* d) const {
@@ -437,17 +467,14 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
     u[i] = rmad(u[i], c4, d);
   });
   __builtin_amdgcn_sched_barrier(0x100);
-  // stage 5: t = stage-6 input
-  q[0] = S2[0];
-  q[1] = S2[64];
-  f.stream(sb + 4 * BS, L, w, y, [&](int i, cd d) {
-    cd a6 = q[i & 1];
-    if (i + 2 < N) q[i & 1] = S2[(i + 2) * 64];
-    t[i] = rmad(a6, b65, d);
-  });
+  // stage 5: t = stage-6 input (the stage-5 base is scaled by b65 h, so the
+  // chains started from S2 give y + A6 + b65 k5 directly)
+  f.stream_acc(sb + 4 * BS, L, w, y, [&](int i) { return S2[i * 64]; },
+               [&](int i, cd d) { t[i] = d; });
   __builtin_amdgcn_sched_barrier(0x100);
-  // stage 6: y = y + Aout + c6 k6
-  f.stream(sb + 5 * BS, L, w, t, [&](int i, cd d) { y[i] = rmad(u[i], c6, d); });
+  // stage 6: y = y + Aout + c6 k6 (base scaled by c6 h, chains started from u)
+  f.stream_acc(sb + 5 * BS, L, w, t, [&](int i) { return u[i]; },
+               [&](int i, cd d) { y[i] = d; });
 }
 
 // One knot interval of a march: the interval's equal steps from ra to rb
@@ -457,6 +484,15 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
 // steps taken.
 __device__ inline double stage_offset(int k) {
   return k == 0 ? 0.0 : (k == 1 ? 0.2 : (k == 2 ? 0.3 : (k == 3 ? 0.6 : (k == 4 ? 1.0 : 0.875))));
+}
+
+// Factor on h in the base of stage k: rk_step_lds folds b65 (stage 5) and
+// c6 (stage 6) into the derivative, since those stages' derivatives enter
+// one combination each.
+template <int N>
+__device__ inline double stage_scale(int k) {
+  if constexpr (N == 14) return k == 4 ? ck::b65 : (k == 5 ? ck::c6 : 1.0);
+  return 1.0;
 }
 
 template <class F>
@@ -472,7 +508,7 @@ __device__ int march_interval(const Tab& T, const WS& w, const LS& L, int nknot,
     if (q < 6 * nst) {
       int jj = j0 + q / 6, k = q % 6;
       double x = ra + (double)(jj - 1) * drr;
-      F::base(T, w, i, x + stage_offset(k) * drr, drr, sbase + q * F::BS);
+      F::base(T, w, i, x + stage_offset(k) * drr, drr * stage_scale<F::N>(k), sbase + q * F::BS);
     }
     __syncthreads();
     for (int jj = 0; jj < nst; jj++) {
