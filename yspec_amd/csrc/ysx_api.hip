@@ -670,7 +670,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
     if (timing) HIPCHK(hipEventRecord(ev[5], st));
     HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
     if (timing) HIPCHK(hipEventRecord(ev[6], st));
-    launch_assemble(dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec,
+    launch_assemble(dim3(asm_blocks(PR)), b64, 0, st, ctx->d_runs, PR, ctx->d_rec,
                        ctx->d_cf, ctx->d_tb);
     if (timing) HIPCHK(hipEventRecord(ev[7], st));
     launch_lsum(dim3(grid_of(ql)), dim3(256), 0, st, ctx->d_runs, ql, ctx->d_cf,
@@ -775,7 +775,7 @@ int enqueue_batched(ysx_ctx* ctx) {
                        qn, ctx->d_rec, ctx->d_cnt);
   }
   HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
-  launch_assemble(dim3(nb), b256, 0, st, ctx->d_runs, PR, ctx->d_rec, ctx->d_cf,
+  launch_assemble(dim3(asm_blocks(PR)), b64, 0, st, ctx->d_runs, PR, ctx->d_rec, ctx->d_cf,
                      ctx->d_tb);
   launch_lsum(dim3(std::max(1, ql.n)), dim3(256), 0, st, ctx->d_runs, ql, ctx->d_cf,
                      ctx->d_tb, ctx->d_eia, ctx->d_spec);

@@ -368,54 +368,100 @@ __global__ void __launch_bounds__(64) k_sph_var(const double* __restrict__ gtab,
 
 // ------------------------------------------------------------- k_assemble
 // output-type scaling and the uu/vv/ww coefficients, src/yspec.f90:280-288,
-// 309-316, 377-400 (zeta there is single precision, :255)
-__global__ void __launch_bounds__(256) k_assemble(const RunK* __restrict__ runs, PairRange PR,
-                                                  cd* __restrict__ rec, cd* __restrict__ cf,
-                                                  const double* __restrict__ tb) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= PR.n) return;
-  long long g = PR.g0 + t;
-  const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
-  GemmK G = gemm_of(R, cf, tb);
-  int l, j;
-  pair_lj(R, (int)(g - R.pbase), l, j);
-  cd* o = rec + (size_t)g * REC;
-  cd w = C(R.wmin + (double)(j - 1) * R.dw, -R.ep);
-  cd fo = (R.out == 1) ? cmuli(w) : (R.out == 2 ? -(w * w) : C(1.0));
-  double rr = R.rr;
-  if (l == 0) {
-    cd u3 = (o[O_SR0] * o[O_YRAD + 0] + o[O_SR0 + 1] * o[O_YRAD + 2]) / rr;
-    o[O_UU + 2] = fo * u3;
-    put_coefs(G, l, j, o);
-    return;
-  }
-  double zeta = (double)sqrtf((float)(l * (l + 1)));
-  if (l > 1) {
-    cd t1 = fo * o[O_YTOR + 0], t2 = fo * o[O_YTOR + 2];
-    for (int m = 0; m < 5; m++)
-      o[O_WW + m] = (o[O_SVT + m] * t1 + o[O_SVT + 5 + m] * t2) / (zeta * rr);
-  }
-  cd y1[4], y2[4];
-  for (int c = 0; c < 4; c++) {
-    cd a = o[O_YSPH + 6 * c + 0], b = o[O_YSPH + 6 * c + 1], g3 = o[O_YSPH + 6 * c + 2];
-    if (R.out == 2 && R.cor != 0) {
-      cd c1 = -((double)(l + 1) * g3 + (2.0 * R.grav_ir) * a) / rr;
-      cd c2 = zeta * (g3 + R.grav_ir * a) / rr;
-      y1[c] = fo * a + c1;
-      y2[c] = fo * b + c2;
-    } else {
-      y1[c] = fo * a;
-      y2[c] = fo * b;
+// 309-316, 377-400 (zeta there is single precision, :255).  A block takes
+// ASM_P consecutive pairs: the input fields [0, O_UU) of their records are
+// copied to LDS with coalesced loads, one lane per pair computes, and the
+// 15 coefficients go back to the records through LDS the same way (reading
+// the 1.4 KB records field by field from one lane per pair fetched each
+// cache line several times).
+constexpr int ASM_LD = O_UU + 1;  // LDS stride per pair, odd: conflict-free 16-byte reads
+static_assert(O_UU == O_YRAD + 4 && O_VV == O_UU + 5 && O_WW == O_VV + 5 && REC == O_WW + 5,
+              "record layout: inputs first, then uu, vv, ww");
+
+__global__ void __launch_bounds__(64) k_assemble(const RunK* __restrict__ runs, PairRange PR,
+                                                 cd* __restrict__ rec, cd* __restrict__ cf,
+                                                 const double* __restrict__ tb) {
+  __shared__ cd s[ASM_P * ASM_LD];
+  long long b0 = (long long)blockIdx.x * ASM_P;
+  if (b0 >= PR.n) return;
+  int np = (int)min((long long)ASM_P, PR.n - b0);
+  const cd* src = rec + (size_t)(PR.g0 + b0) * REC;
+  {
+    // All of a lane's loads are issued before the first LDS write.  Indices
+    // past the block's last field are clamped to it (the lanes concerned
+    // load and store that field's value again), so no load is predicated.
+    constexpr int NL = (ASM_P * O_UU + 63) / 64;
+    const int kmax = np * O_UU - 1;
+    cd v[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      int k = min((int)threadIdx.x + 64 * i, kmax), q = k / O_UU, f = k - q * O_UU;
+      v[i] = src[(size_t)q * REC + f];
+    }
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      int k = min((int)threadIdx.x + 64 * i, kmax), q = k / O_UU, f = k - q * O_UU;
+      s[q * ASM_LD + f] = v[i];
     }
   }
-  const cd* sv = o + O_SVS;
-  for (int m = 0; m < 5; m++) {
-    cd su = y1[0] * sv[m] + y1[1] * sv[5 + m] + y1[2] * sv[15 + m] + y1[3] * sv[20 + m];
-    cd sw = y2[0] * sv[m] + y2[1] * sv[5 + m] + y2[2] * sv[15 + m] + y2[3] * sv[20 + m];
-    o[O_UU + m] = su / rr;
-    o[O_VV + m] = sw / (zeta * rr);
+  __syncthreads();
+  int t = threadIdx.x;
+  if (t < np) {
+    long long g = PR.g0 + b0 + t;
+    const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
+    GemmK G = gemm_of(R, cf, tb);
+    int l, j;
+    pair_lj(R, (int)(g - R.pbase), l, j);
+    cd* o = s + t * ASM_LD;
+    cd w = C(R.wmin + (double)(j - 1) * R.dw, -R.ep);
+    cd fo = (R.out == 1) ? cmuli(w) : (R.out == 2 ? -(w * w) : C(1.0));
+    double rr = R.rr;
+    cd c[15];
+    for (int m = 0; m < 15; m++) c[m] = C(0.0);
+    if (l == 0) {
+      cd u3 = (o[O_SR0] * o[O_YRAD + 0] + o[O_SR0 + 1] * o[O_YRAD + 2]) / rr;
+      c[2] = fo * u3;
+    } else {
+      double zeta = (double)sqrtf((float)(l * (l + 1)));
+      if (l > 1) {
+        cd t1 = fo * o[O_YTOR + 0], t2 = fo * o[O_YTOR + 2];
+        for (int m = 0; m < 5; m++)
+          c[10 + m] = (o[O_SVT + m] * t1 + o[O_SVT + 5 + m] * t2) / (zeta * rr);
+      }
+      cd y1[4], y2[4];
+      for (int k = 0; k < 4; k++) {
+        cd a = o[O_YSPH + 6 * k + 0], bb = o[O_YSPH + 6 * k + 1], g3 = o[O_YSPH + 6 * k + 2];
+        if (R.out == 2 && R.cor != 0) {
+          cd c1 = -((double)(l + 1) * g3 + (2.0 * R.grav_ir) * a) / rr;
+          cd c2 = zeta * (g3 + R.grav_ir * a) / rr;
+          y1[k] = fo * a + c1;
+          y2[k] = fo * bb + c2;
+        } else {
+          y1[k] = fo * a;
+          y2[k] = fo * bb;
+        }
+      }
+      const cd* sv = o + O_SVS;
+      for (int m = 0; m < 5; m++) {
+        cd su = y1[0] * sv[m] + y1[1] * sv[5 + m] + y1[2] * sv[15 + m] + y1[3] * sv[20 + m];
+        cd sw = y2[0] * sv[m] + y2[1] * sv[5 + m] + y2[2] * sv[15 + m] + y2[3] * sv[20 + m];
+        c[m] = su / rr;
+        c[5 + m] = sw / (zeta * rr);
+      }
+    }
+    size_t plane = (size_t)G.lpad * G.jpad, at = (size_t)l * G.jpad + (j - 1);
+    for (int m = 0; m < 15; m++) {
+      G.cf[m * plane + at] = c[m];
+      o[m] = c[m];  // this lane's own slot; its inputs have all been read
+    }
   }
-  put_coefs(G, l, j, o);
+  __syncthreads();
+  cd* dst = rec + (size_t)(PR.g0 + b0) * REC + O_UU;
+#pragma unroll
+  for (int i = 0; i < (ASM_P * 15 + 63) / 64; i++) {
+    int k = threadIdx.x + 64 * i, q = k / 15, f = k - q * 15;
+    if (k < np * 15) dst[(size_t)q * REC + f] = s[q * ASM_LD + f];
+  }
 }
 
 // ---------------------------------------------------------------- k_lsum

@@ -126,6 +126,10 @@ constexpr int REC = 89;
 constexpr int O_SVT = 0, O_YTOR = 10, O_SVS = 14, O_YSPH = 44, O_SR0 = 68, O_YRAD = 70,
               O_UU = 74, O_VV = 79, O_WW = 84;
 
+// k_assemble: pairs per 64-lane block
+constexpr int ASM_P = 32;
+inline int asm_blocks(const PairRange& PR) { return (int)std::max(1LL, (PR.n + ASM_P - 1) / ASM_P); }
+
 // ------------------------------------------------ state between the kernels
 // Start levels and starting solutions of every pair, written by k_start:
 // tor_solver's start (src/module_solver.f90:64-75, 103-114) and
