@@ -120,6 +120,16 @@ int ysx_forward_events(ysx_ctx *ctx, int mode, double *seis);
  * queue access; persist = 0 launches one block per item. */
 int ysx_set_batch_tuning(ysx_ctx *ctx, int persist, int chunk, int blocks_per_cu);
 
+/* Diagnostic switch, no counterpart in the reference: full = 1 makes runs
+ * set afterwards integrate the spheroidal b-vector system as the
+ * reference's four source-side columns (sph_solver,
+ * src/module_solver.f90:745-755, 809-819), so that ysx_pair_records
+ * returns all six rows of ysph.  full = 0 (the default) marches the two or
+ * three receiver-side rows the seismograms need back to the source with the
+ * transposed steps of the same step sequence: the same results up to
+ * rounding, about half the b-vector work. */
+int ysx_set_records(ysx_ctx *ctx, int full);
+
 /* Spectra ((nw_e, nr_e) complex each, or NULL) and seismograms
  * ((nt_e, 3, nr_e)) of event e from the last forward. */
 int ysx_event_spectra(ysx_ctx *ctx, int e, double *ur, double *ut, double *up);
@@ -129,7 +139,10 @@ int ysx_event_seismograms(ysx_ctx *ctx, int e, double *seis);
  * svt(5,2) ytor(2,2) svs(5,6) ysph(6,4) sr0(2) yrad(2,2) uu(5) vv(5) ww(5),
  * the values the reference's loop body holds after its source-vector and
  * solver calls (src/yspec.f90:270-400); the pairs must have been computed
- * by the last ysx_spectra (one-event contexts only). */
+ * by the last ysx_spectra (one-event contexts only).  ysph holds all six
+ * rows only for runs set after ysx_set_records(ctx, 1); otherwise rows 0..1
+ * (0..2 with the potential/tilt correction), the rows the seismograms use,
+ * and zeros. */
 int ysx_pair_records(ysx_ctx *ctx, int n, const int *l, const int *j,
                      double *rec);
 

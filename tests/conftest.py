@@ -38,6 +38,13 @@ def load_case(name):
     return deck, p, z
 
 
+def ysph_rows(p):
+    """Rows of the pair records' ysph (n, 6, 4) that the default b-vector path
+    (receiver-side rows) fills: 0..1, and 0..2 with the acceleration
+    potential/tilt correction; all six need ysx_set_records(ctx, 1)."""
+    return 3 if (p.out == 2 and p.cor != 0) else 2
+
+
 def rel_l2(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

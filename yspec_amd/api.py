@@ -23,6 +23,7 @@ I = C.POINTER(C.c_int)
 EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_set_run",
            "ysx_run_info", "ysx_spectra", "ysx_seismograms", "ysx_forward",
            "ysx_set_events", "ysx_event_info", "ysx_forward_events", "ysx_set_batch_tuning",
+           "ysx_set_records",
            "ysx_event_spectra", "ysx_event_seismograms", "ysx_model_table",
            "ysx_model_table_host",
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts")
@@ -64,6 +65,7 @@ def lib():
         L.ysx_event_info.argtypes = [vp, C.c_int, I, D]
         L.ysx_forward_events.argtypes = [vp, C.c_int, D]
         L.ysx_set_batch_tuning.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.ysx_set_records.argtypes = [vp, C.c_int]
         L.ysx_event_spectra.argtypes = [vp, C.c_int, D, D, D]
         L.ysx_event_seismograms.argtypes = [vp, C.c_int, D]
         L.ysx_model_table.argtypes = [vp, D]
@@ -147,6 +149,15 @@ class YSpec:
         self.nw, self.nt, self.i1, self.i2, self.is_, self.ir, self.nout, self.npair = map(int, io)
         (self.rs, self.rr, self.dt, self.tout, self.ep, self.df, self.wmin,
          self.dw) = map(float, fo)
+
+    def set_records(self, full):
+        """full=True: ysph of the pair records holds all six rows (the
+        b-vector system integrated as the reference's source-side columns);
+        False (default): the rows the seismograms use.  Re-applies the
+        current run, if any."""
+        self._check(self.L.ysx_set_records(self.h, 1 if full else 0))
+        if getattr(self, "p", None) is not None and not getattr(self, "events", None):
+            self.set_run(self.p)
 
     def spectra(self, copy=True):
         """ur, ut, up: (nw, nr) complex (Fortran order), or None if copy=False."""

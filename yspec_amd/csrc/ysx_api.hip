@@ -10,7 +10,9 @@
 //   k_minor_top   spheroidal minors from the receiver side down to the source
 //   k_minor_march spheroidal minor vectors from the start level up to the
 //                 source, secular determinant, b-vector starts
-//   k_bvec_march  b-vector from the source to the receiver, 4 columns
+//   k_brow_march  b-vector system: receiver-side rows marched back to the
+//                 source with transposed steps (k_bvec_march: the reference's
+//                 4 source-side columns, for full ysph records)
 //   k_rad         radial (l = 0) solves on the reference's grid (2nd stream)
 //   k_sph_var, k_rad_ng   per-lane solvers of the gravity/inner-core variants
 //   k_assemble    output-type scaling and the uu/vv/ww coefficients
@@ -68,6 +70,7 @@ struct ysx_ctx {
   std::vector<RunK> rk;
   bool have_model = false, have_run = false, have_spec = false, have_seis = false;
   bool refined = false;  // wave lists reordered by measured cost (refine_work)
+  bool bcols = false;    // b-vector as source-side columns (full ysph records)
   size_t lds_bytes = 0, lds_max = 0;  // model table bytes, LDS per block
   bool tab_lds = true;                 // the table fits in LDS beside the kernels' own
   long long npair = 0;  // all events
@@ -157,7 +160,7 @@ void build_work(ysx_ctx* c) {
   for (int k = 0; k < W_N; k++) c->W[k].start.assign(1, 0);
   for (int e = 0; e < nev; e++) {
     const RunK& R = c->rk[e];
-    int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64, nlb16 = (nl + 15) / 16;
+    int nl = R.lmax - R.lmin + 1, nlb = (nl + 63) / 64, nlbb = (nl + R.bl - 1) / R.bl;
     auto rew = [&](int j) { return R.wmin + (double)(j - 1) * R.dw; };
     double rtop = std::max(R.rs, R.rr);
     for (int wv = 0; wv < R.nw * nlb; wv++) {
@@ -166,8 +169,8 @@ void build_work(ysx_ctx* c) {
       double w = rew(j), rt = std::min(1.0, (R.lmin + 64.0 * lb + 0.5) / std::max(w, 1e-30));
       cost[W_TOR].push_back(w * (rtop - std::min(rt, rtop) + 0.05));
     }
-    for (int wv = 0; wv < R.nw * nlb16; wv++) {
-      int j = R.nw - wv / nlb16, lb = wv % nlb16;
+    for (int wv = 0; wv < R.nw * nlbb; wv++) {
+      int j = R.nw - wv / nlbb, lb = wv % nlbb;
       c->W[W_BVEC].plain.push_back(make_int4(e, j, lb, 0));
       cost[W_BVEC].push_back(rew(j) * (std::fabs(R.rr - R.rs) + 1e-3));
     }
@@ -270,10 +273,10 @@ int refine_work(ysx_ctx* c) {
     ysx_ctx* ctx = c;
     HIPCHK(hipMemcpy(cnt.data(), c->d_cnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost));
   }
-  // lanes of a wave: 64 degrees (W_TOR) or 16 degrees (W_BVEC)
+  // degrees of a wave: 64 (W_TOR) or bl (W_BVEC)
   auto wave_cost = [&](const int4& it, int kind) {
     const RunK& R = c->rk[it.x];
-    const int nl = (kind == W_TOR) ? 64 : 16;
+    const int nl = (kind == W_TOR) ? 64 : R.bl;
     double m = 0.0;
     for (int q = 0; q < nl; q++) {
       int l = R.lmin + it.z * nl + q;
@@ -462,6 +465,10 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     // one lane each keeps it to few compute units)
     k.radg = (R.nw <= 640) ? 8 : 1;
     k.tablds = ctx->tab_lds ? 1 : 0;
+    // b-vector: 2 receiver-side rows per degree (4 lanes, rows 0..2, with the
+    // potential/tilt correction), or the 4 source-side columns
+    k.brow = ctx->bcols ? 0 : 1;
+    k.bl = (k.brow && !(R.out == 2 && R.cor != 0)) ? 32 : 16;
     k.jpad = (R.nw + 15) / 16 * 16;
     k.nrpad = (R.nr + 15) / 16 * 16;
     k.nt = R.nt;
@@ -664,8 +671,8 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
       launch_minor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                          ctx->d_runs, qt, SD, S, ctx->d_cnt);
       if (timing) HIPCHK(hipEventRecord(ev[4], st));
-      launch_bvec_march_item(dim3(grid_of(qb)), b64, 0, st, ctx->d_tab,
-                         ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
+      (ctx->rk[0].brow ? launch_brow_march_item : launch_bvec_march_item)(
+          dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
     }
     if (timing) HIPCHK(hipEventRecord(ev[5], st));
     HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
@@ -748,8 +755,8 @@ int enqueue_batched(ysx_ctx* ctx) {
                        S, ctx->d_cnt);
     launch_minor_march_pers(grid(qmin), b64, 0, st, ctx->d_tab, ctx->d_runs,
                        qmin, SD, S, ctx->d_cnt);
-    launch_bvec_march_pers(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
-                       ctx->d_rec, ctx->d_cnt);
+    (ctx->rk[0].brow ? launch_brow_march_pers : launch_bvec_march_pers)(
+        grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
   } else {
     launch_tor_march_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
                        SD, ctx->d_rec, ctx->d_cnt);
@@ -757,8 +764,8 @@ int enqueue_batched(ysx_ctx* ctx) {
                        ctx->d_cnt);
     launch_minor_march_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
                        SD, S, ctx->d_cnt);
-    launch_bvec_march_item(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb,
-                       S, ctx->d_rec, ctx->d_cnt);
+    (ctx->rk[0].brow ? launch_brow_march_item : launch_bvec_march_item)(
+        grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
   }
   Queue qv = batched_q(ctx, W_VAR, -1);
   if (qv.n > 0)
@@ -853,6 +860,14 @@ extern "C" int ysx_forward(ysx_ctx* ctx, double* seis) {
   int rc = ysx_spectra(ctx, nullptr, nullptr, nullptr);
   if (rc) return rc;
   return ysx_seismograms(ctx, seis);
+}
+
+// takes effect at the next ysx_set_run / ysx_set_events (the b-vector
+// work lists are built for one of the two kernels)
+extern "C" int ysx_set_records(ysx_ctx* ctx, int full) {
+  if (!ctx || (full != 0 && full != 1)) return YSX_ERR_ARG;
+  ctx->bcols = (full == 1);
+  return YSX_OK;
 }
 
 extern "C" int ysx_set_batch_tuning(ysx_ctx* ctx, int persist, int chunk, int blocks_per_cu) {

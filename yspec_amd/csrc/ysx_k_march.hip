@@ -1,13 +1,13 @@
 // ysx_k_march.hip -- one march kernel and its launch function per object
-// file: the build compiles this file eight times, once per kernel
+// file: the build compiles this file ten times, once per kernel
 // (YSX_MARCH = 0 toroidal, 1 minors from the surface, 2 minors from the
-// start level, 3 b-vector) and work-distribution mode (YSX_PERSIST = 0 one
+// start level, 3 b-vector columns, 4 b-vector rows) and work-distribution mode (YSX_PERSIST = 0 one
 // block per work item, 1 persistent blocks on atomic queues), so that the
 // large kernels compile in parallel.
 #include "ysx_march_kernels.hpp"
 
 #if !defined(YSX_MARCH) || !defined(YSX_PERSIST)
-#error "compile with -DYSX_MARCH=0..3 -DYSX_PERSIST=0|1"
+#error "compile with -DYSX_MARCH=0..4 -DYSX_PERSIST=0|1"
 #endif
 
 #if YSX_PERSIST
@@ -38,8 +38,13 @@ void YSX_LAUNCH(bvec_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const dou
                             const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt) {
   hipLaunchKernelGGL(k_bvec_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, rec, cnt);
 }
+#elif YSX_MARCH == 4
+void YSX_LAUNCH(brow_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                            const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_brow_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, rec, cnt);
+}
 #else
-#error "YSX_MARCH must be 0..3"
+#error "YSX_MARCH must be 0..4"
 #endif
 
 }  // namespace ysx

@@ -21,6 +21,8 @@ struct RunK {
                                 // by the gravity-0/1 lockstep march (ngc)
   int lpad, jpad, nrpad;        // GEMM padding of this event
   int radg;                     // lanes per radial (l = 0) solve: 1 or 8
+  int brow, bl;                 // b-vector as receiver-side rows (1) or source-side
+                                // columns (0); degrees per b-vector wave
   int tablds;                   // model table staged in LDS (else read from HBM)
   int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
   long long pbase;              // first pair of this event in the per-pair arrays
@@ -175,6 +177,25 @@ __device__ int march_yint(const Tab& T, const WS& w, const LS& L, int nk, int i1
   return n;
 }
 
+// march_yint's intervals and steps in reverse, each step transposed: for a
+// row x, x <- x P_N ... P_1 where march_yint over the same arguments gives
+// y <- P_N ... P_1 y
+template <class F>
+__device__ int march_yint_T(const Tab& T, const WS& w, const LS& L, int nk, int i1, int i2,
+                            double r1, double r2, cd* x, int& sc, double* sb, const F& f,
+                            cd* slot) {
+  int n = 0;
+  if (r2 == r1) return 0;
+  const int dir = (r2 > r1) ? 1 : -1, lo = (dir > 0) ? 0 : 1;
+  for (int i = i2; (i - i1) * dir >= 0; i -= dir) {
+    double a = (i == i1) ? r1 : T(F_R, i + lo);
+    double b = (i == i2) ? r2 : T(F_R, i + 1 - lo);
+    if (a == b) continue;
+    n += march_interval_T(T, w, L, nk, i, a, b, x, sc, sb, f, slot);
+  }
+  return n;
+}
+
 __device__ inline WS wave_omega(const RunK& R, int j) {
   return omega_state(C(R.wmin + (double)(j - 1) * R.dw, -R.ep), R.tref, R.ats, R.pibigg);
 }
@@ -247,6 +268,8 @@ void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw);
                                    const double* gtab, const RunK* runs, Queue Q, StartData SD, \
                                    SphState S, int* cnt);                                       \
   void launch_bvec_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
+                                  const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt);  \
+  void launch_brow_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
                                   const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt);
 YSX_MARCH_LAUNCHERS(item)
 YSX_MARCH_LAUNCHERS(pers)

@@ -230,6 +230,50 @@ __device__ inline void bvec_derivs(const SolidA& A, const cd* b, const Init& I, 
   acc = rmsb(acc, t31, b[11]); sink(13, rmad(acc, t2233, b[13]));
 }
 
+// Transpose of sph_bvec_solid_derivs (src/module_int.f90:1016-1057):
+// e = A^T x for the b-vector system's matrix A, as FMA chains.  It is the
+// derivative of the receiver-side row march (rk_step_T_lds).
+template <class Init, class Sink>
+__device__ inline void bvecT_derivs(const SolidA& A, const cd* x, const Init& I, Sink&& sink) {
+  const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
+               s23 = A.s23;
+  const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
+           s22 = A.s22;
+  const cd t1122 = t11 + t22, t1133 = t11 + t33;
+  const double t2233 = t22 + t33;
+  const cd x2m7 = x[2] - x[7];
+  cd acc;
+  acc = I.c(0, k11, x[6]); acc = cmsb(acc, k22, x[3]); sink(0, cmsb(acc, t1122, x[0]));
+  acc = I.c(1, k11, x[9]); acc = rmsb(acc, k33, x[4]); acc = cmsb(acc, t12, x[5]);
+  sink(1, cmsb(acc, t1133, x[1]));
+  acc = I.r(2, t21, x[3]); acc = cmsb(acc, s12, x[0]); acc = rmsb(acc, 2.0 * s13, x[1]);
+  acc = rmad(acc, 2.0 * t31, x[4]); acc = rmsb(acc, s23, x[5]); sink(2, cmsb(acc, t12, x[6]));
+  acc = I.c(3, t12, x2m7); acc = cmsb(acc, s22, x[0]); acc = rmsb(acc, s23, x[1]);
+  acc = cmad(acc, t22 - t11, x[3]); sink(3, cmsb(acc, k11, x[11]));
+  acc = I.c(4, t33 - t11, x[4]); acc = rmsb(acc, s23, x[0]); acc = cmsb(acc, t12, x[8]);
+  sink(4, cmsb(acc, k11, x[12]));
+  acc = I.c(5, k22, x[10]); acc = rmad(acc, t31, x[0]); acc = rmsb(acc, t21, x[1]);
+  acc = rmsb(acc, t2233, x[5]); sink(5, rmsb(acc, k33, x[8]));
+  acc = I.c(6, s11, x[0]); acc = rmsb(acc, t21, x2m7); acc = rmsb(acc, s13, x[5]);
+  acc = cmad(acc, t11 - t22, x[6]); acc = rmad(acc, t31, x[8]); sink(6, cmad(acc, k22, x[11]));
+  acc = I.c(7, s12, x[0]); acc = rmsb(acc, s13, x[1]); acc = rmsb(acc, t21, x[3]);
+  acc = rmad(acc, t31, x[4]); acc = rmsb(acc, 2.0 * s23, x[5]); sink(7, cmad(acc, t12, x[6]));
+  acc = I.r(8, s13, x[0]); acc = rmsb(acc, t21, x[4]); acc = rmad(acc, t33 - t22, x[8]);
+  sink(8, cmsb(acc, k22, x[13]));
+  acc = I.c(9, s11, x[1]); acc = rmsb(acc, t31, x[2]); acc = cmad(acc, s12, x[5]);
+  acc = cmad(acc, t11 - t33, x[9]); acc = rmad(acc, t21, x[10]); sink(9, rmad(acc, k33, x[12]));
+  acc = I.c(10, s12, x[1]); acc = rmsb(acc, t31, x[3]); acc = cmad(acc, s22, x[5]);
+  acc = cmad(acc, t12, x[9]); acc = rmad(acc, t22 - t33, x[10]);
+  sink(10, rmad(acc, k33, x[13]));
+  acc = I.c(11, s12, x2m7); acc = cmsb(acc, s11, x[3]); acc = cmad(acc, s22, x[6]);
+  acc = rmad(acc, s23, x[9]); acc = rmsb(acc, s13, x[10]); acc = cmad(acc, t1122, x[11]);
+  sink(11, rmsb(acc, t31, x[13]));
+  acc = I.r(12, s13, x[2]); acc = cmsb(acc, s11, x[4]); acc = rmad(acc, s23, x[6]);
+  acc = cmsb(acc, s12, x[8]); acc = cmad(acc, t1133, x[12]); sink(12, rmad(acc, t21, x[13]));
+  acc = I.r(13, s13, x[3]); acc = cmsb(acc, s12, x[4]); acc = rmad(acc, s23, x[7]);
+  acc = cmsb(acc, s22, x[8]); acc = cmad(acc, t12, x[12]); sink(13, rmad(acc, t2233, x[13]));
+}
+
 struct SolidMinorM : SolidB {
   static constexpr int N = 14;
   template <class Sink>
@@ -271,6 +315,25 @@ struct BvecM : SolidB {
   __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
                                     cd* d) const {
     stream(b, L, w, y, [&](int i, cd v) { d[i] = v; });
+  }
+};
+
+// the transposed b-vector system (receiver-side rows)
+struct BvecTM : SolidB {
+  static constexpr int N = 14;
+  template <class Sink>
+  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+                                Sink&& sink) const {
+    SolidA A;
+    amat(b, L, A);
+    bvecT_derivs(A, y, NoInit(), sink);
+  }
+  template <class G, class Sink>
+  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+                                    const G& g, Sink&& sink) const {
+    SolidA A;
+    amat(b, L, A);
+    bvecT_derivs(A, y, AccInit<G>{g}, sink);
   }
 };
 
@@ -477,6 +540,66 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
                [&](int i, cd d) { y[i] = d; });
 }
 
+// The transpose of rk_step_lds: x <- P^T x for the step matrix P of one
+// Cash-Karp step of a linear system (the stages' coefficients as in
+// rk_step_lds, stage bases 5 and 6 scaled by b65 and c6).  With stage
+// inputs z_i and derivatives k_i = h A_i z_i the step is
+// y' = y + sum c_i k_i, z_i = y + sum_{m<i} b_im k_m; its transpose, applied
+// to a row x, takes the stages in reverse:
+//   w_i = c_i x + sum_{m>i} b_mi zb_m,  zb_i = (h A_i)^T w_i,
+//   x' = x + sum_i zb_i.
+// Registers y, z and the slots S1, S2 hold the partial sums
+// (A1..A3 -> w1..w3, A2 -> w2, L -> x'), so the step uses the same
+// storage as the forward one.
+template <class F>
+__device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w, cd* x,
+                                     const F& f, cd* slot) {
+  using namespace ck;
+  constexpr int N = F::N, BS = F::BS;
+  cd* S1 = slot;
+  cd* S2 = slot + N * 64;
+  cd y[N], z[N];
+  // stage 6 (base scaled by c6 h): y = zb6; stage 5 (by b65 h): z = zb5
+  f.stream(sb + 5 * BS, L, w, x, [&](int i, cd d) { y[i] = d; });
+  __builtin_amdgcn_sched_barrier(0x100);
+  f.stream(sb + 4 * BS, L, w, y, [&](int i, cd d) { z[i] = d; });
+  __builtin_amdgcn_sched_barrier(0x100);
+  // x = w4, y = A3, z = A1, S1 = A2, S2 = L
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    cd xi = x[i], z6 = y[i], z5 = z[i];
+    S2[i * 64] = xi + z5 + z6;
+    S1[i * 64] = rmad(rmul(b52, z5), b62, z6);
+    x[i] = rmad(rmad(rmul(c4, xi), b54, z5), b64, z6);
+    y[i] = rmad(rmad(rmul(c3, xi), b53, z5), b63, z6);
+    z[i] = rmad(rmad(rmul(c1, xi), b51, z5), b61, z6);
+  }
+  __builtin_amdgcn_sched_barrier(0x100);
+  // stage 4: zb4 from w4; y = w3
+  f.stream(sb + 3 * BS, L, w, x, [&](int i, cd d) {
+    y[i] = rmad(y[i], b43, d);
+    z[i] = rmad(z[i], b41, d);
+    S1[i * 64] = rmad(S1[i * 64], b42, d);
+    S2[i * 64] = S2[i * 64] + d;
+  });
+  __builtin_amdgcn_sched_barrier(0x100);
+  // stage 3: zb3 from w3; x = w2
+  f.stream(sb + 2 * BS, L, w, y, [&](int i, cd d) {
+    x[i] = rmad(S1[i * 64], b32, d);
+    z[i] = rmad(z[i], b31, d);
+    S2[i * 64] = S2[i * 64] + d;
+  });
+  __builtin_amdgcn_sched_barrier(0x100);
+  // stage 2: zb2 from w2; z = w1
+  f.stream(sb + BS, L, w, x, [&](int i, cd d) {
+    z[i] = rmad(z[i], b21, d);
+    S2[i * 64] = S2[i * 64] + d;
+  });
+  __builtin_amdgcn_sched_barrier(0x100);
+  // stage 1: x' = L + zb1 (the chains start from S2)
+  f.stream_acc(sb, L, w, z, [&](int i) { return S2[i * 64]; }, [&](int i, cd d) { x[i] = d; });
+}
+
 // One knot interval of a march: the interval's equal steps from ra to rb
 // (ra != rb), the step rule of yint (src/module_int.f90:1903-1907,
 // 1945-1949), optional renormalisation (:1910-1919).  Every lane of the
@@ -518,6 +641,35 @@ __device__ int march_interval(const Tab& T, const WS& w, const LS& L, int nknot,
         rk_step_b(sbase + jj * 6 * F::BS, L, w, y, f);
       }
       if (scaled) renorm<F::N>(y, scale);
+    }
+    __syncthreads();
+  }
+  return n - 1;
+}
+
+// march_interval's steps in reverse order, each applied as its transpose
+// (rk_step_T_lds): the same knot interval, step count, step length and
+// stage radii.  Returns the number of steps.
+template <class F>
+__device__ int march_interval_T(const Tab& T, const WS& w, const LS& L, int nknot, int i,
+                                double ra, double rb, cd* y, int& scale, double* sbase,
+                                const F& f, cd* slot) {
+  static_assert(F::N == 14, "the transposed march is for the 14-component systems");
+  double drr0 = step_len_w(T, w.rew, i, nknot);
+  int n = 2 + (int)floor(fabs(rb - ra) / drr0);
+  double drr = (rb - ra) / (double)(n - 1);
+  for (int jhi = n - 1; jhi >= 1; jhi -= CH) {
+    int nst = min(CH, jhi);  // steps jhi, jhi - 1, ..., jhi - nst + 1
+    int q = threadIdx.x;
+    if (q < 6 * nst) {
+      int jj = jhi - q / 6, k = q % 6;
+      double x = ra + (double)(jj - 1) * drr;
+      F::base(T, w, i, x + stage_offset(k) * drr, drr * stage_scale<F::N>(k), sbase + q * F::BS);
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nst; jj++) {
+      rk_step_T_lds(sbase + jj * 6 * F::BS, L, w, y, f, slot);
+      renorm<F::N>(y, scale);
     }
     __syncthreads();
   }

@@ -292,5 +292,85 @@ __global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gt
                       slot));
 }
 
+// ------------------------------------------------------------ k_brow_march
+// The same y = sph_ysol(mfin, b(rr)) fac for the four source columns
+// (src/module_solver.f90:745-755, 809-819) from the other side: ysol is
+// linear in b, so row k of y is x_k . b(rr) with x_k row k of ysol(mfin, .),
+// and x_k . (P b(rs)) = (x_k P) . b(rs) for the b-vector march's product of
+// step matrices P.  The rows x_k P are marched from the receiver back to
+// the source with the transposed steps (march_yint_T), so two rows (three
+// with the potential/tilt correction) replace four columns; the results
+// differ from the column march by rounding only.  Rows 3..5 of ysph are not
+// produced (zero).  A wave holds bl degrees x 64 / bl rows at one w.
+__device__ __forceinline__ cd pick3(int k, cd a, cd b, cd c) {
+  return (k == 0) ? a : ((k == 1) ? b : ((k == 2) ? c : C(0.0)));
+}
+// coefficients of b in row k = 0, 1, 2 of sph_ysol (src/module_int.f90:1785-1815)
+__device__ __forceinline__ void ysol_row(const cd* m, int k, cd* x) {
+  const cd z = C(0.0);
+  x[0] = pick3(k, z, z, m[0]);
+  x[1] = pick3(k, z, -m[0], z);
+  x[2] = pick3(k, m[2], -2.0 * m[1], -m[4]);
+  x[3] = pick3(k, z, -m[2], -m[5]);
+  x[4] = pick3(k, z, -m[3], m[2]);
+  x[5] = pick3(k, m[0], z, z);
+  x[6] = pick3(k, m[1], z, -m[9]);
+  x[7] = pick3(k, 2.0 * m[2], -m[1], m[4]);
+  x[8] = pick3(k, m[3], z, -m[1]);
+  x[9] = pick3(k, m[4], m[9], z);
+  x[10] = pick3(k, m[5], -m[4], z);
+  x[11] = pick3(k, m[6], m[10], m[12]);
+  x[12] = pick3(k, m[7], m[11], -m[10]);
+  x[13] = pick3(k, m[8], -m[7], m[6]);
+}
+
+__device__ __forceinline__ void brow_item(const Tab& T, const RunK& R, int j, int lb,
+                                          const SphState& S, cd* __restrict__ rec,
+                                          int* __restrict__ cnt, double* sb, cd* slot) {
+  if (R.var || R.ngc) return;
+  const int rows = 64 / R.bl;  // lanes per degree: row k = lane % rows
+  int k = threadIdx.x % rows;
+  int l = R.lmin + lb * R.bl + (int)(threadIdx.x / rows);
+  bool valid = (l >= 1 && l <= R.lmax);
+  if (wave_min(valid ? 0 : 1)) return;  // no valid lane in this wave
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  WS w = wave_omega(R, j);
+  LS L = l_state(valid ? l : 1);
+  cd x[14];
+  {
+    cd m[14];
+    for (int q = 0; q < 14; q++) m[q] = valid ? S.mfin[p * 14 + q] : C(0.0);
+    ysol_row(m, k, x);
+  }
+  int sl = 0;
+  int n = march_yint_T(T, w, L, R.nknot, R.is, R.ir, R.rs, R.rr, x, sl, sb, BvecTM(), slot);
+  if (!valid) return;
+  cd fac = S.fac[p] * ipow_huge(S.sc3[p] - sl);
+  cd* o = rec + p * REC + O_YSPH;
+  for (int c = 0; c < 4; c++) {
+    cd v = C(0.0);
+#pragma unroll
+    for (int q = 0; q < 14; q++) v = cmad(v, x[q], S.bst[p * 56 + c * 14 + q]);
+    if (k < 3) o[6 * c + k] = v * fac;
+    if (k == 0)
+      for (int r = (rows < 3 ? rows : 3); r < 6; r++) o[6 * c + r] = C(0.0);
+  }
+  if (k == 0) cnt[5 * p + 3] = n;
+}
+
+// items (event, j, l-block of bl)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_brow_march(const double* __restrict__ gtab,
+                                                   const RunK* __restrict__ runs, Queue Q,
+                                                   SphState S, cd* __restrict__ rec,
+                                                   int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  RUN_ITEMS(PERSIST, Q,
+            brow_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, S, rec, cnt, sb,
+                      slot));
+}
+
 }  // namespace
 }  // namespace ysx
