@@ -549,7 +549,7 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
 //   w_i = c_i x + sum_{m>i} b_mi zb_m,  zb_i = (h A_i)^T w_i,
 //   x' = x + sum_i zb_i.
 // Registers y, z and the slots S1, S2 hold the partial sums
-// (A1..A3 -> w1..w3, A2 -> w2, L -> x'), so the step uses the same
+// (A3 -> w3, A2 -> w2, A1 -> w1, L -> x'), so the step uses the same
 // storage as the forward one.
 template <class F>
 __device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w, cd* x,
@@ -590,14 +590,14 @@ __device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w,
     S2[i * 64] = S2[i * 64] + d;
   });
   __builtin_amdgcn_sched_barrier(0x100);
-  // stage 2: zb2 from w2; z = w1
+  // stage 2: zb2 from w2; z = w1; L moves to y (free once stage 3 has run)
   f.stream(sb + BS, L, w, x, [&](int i, cd d) {
     z[i] = rmad(z[i], b21, d);
-    S2[i * 64] = S2[i * 64] + d;
+    y[i] = S2[i * 64] + d;
   });
   __builtin_amdgcn_sched_barrier(0x100);
-  // stage 1: x' = L + zb1 (the chains start from S2)
-  f.stream_acc(sb, L, w, z, [&](int i) { return S2[i * 64]; }, [&](int i, cd d) { x[i] = d; });
+  // stage 1: x' = L + zb1 (the chains start from y)
+  f.stream_acc(sb, L, w, z, [&](int i) { return y[i]; }, [&](int i, cd d) { x[i] = d; });
 }
 
 // One knot interval of a march: the interval's equal steps from ra to rb
