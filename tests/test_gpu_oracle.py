@@ -187,3 +187,28 @@ def test_large_deck_vs_oracle(api, oracle_mod):
     for a, b in zip(gs, os_):
         assert rel_l2(a, b) <= 1e-9
     assert rel_l2(g.seismograms(), o.seismograms(*os_)) <= 1e-8
+
+
+# receiver exactly at the source depth (a zero-length b-vector and minor
+# continuation), a surface source and receiver, and a receiver below the
+# source in the mantle; both b-vector paths
+@pytest.mark.parametrize("src,rec", [(100.0, 100.0), (0.0, 0.0), (35.0, 410.0)],
+                         ids=["rec_at_source", "surface_both", "rec_deeper"])
+@pytest.mark.parametrize("full", [False, True], ids=["rows", "columns"])
+def test_geometry_vs_oracle(api, oracle_mod, src, rec, full):
+    deck = synth_deck(0)
+    p = workload("cfg1", lmax=40, nr=3, src_depth=src, rec_depth=rec)
+    g = api.YSpec(deck)
+    g.set_records(full)
+    g.set_run(p)
+    o = oracle_mod.Oracle(deck, p)
+    gs = g.spectra()
+    os_ = o.spectra(nthreads=16)
+    for a, b, key in zip(gs, os_, ("ur", "ut", "up")):
+        e = rel_l2(a, b)
+        print(src, rec, "full" if full else "rows", key, f"{e:.2e}", flush=True)
+        assert e <= 1e-9, key
+    e = rel_l2(g.seismograms(), o.seismograms(*os_))
+    print(src, rec, "seis", f"{e:.2e}", flush=True)
+    assert e <= 1e-8
+    g.close()
