@@ -94,6 +94,13 @@ struct ysx_ctx {
   double* d_seis = nullptr;
   int* d_queue = nullptr;  // Q_N counters of the batched queues
   WorkList W[W_N];
+  // events whose b-vector rows share one march in the batched path (same
+  // receiver depth and frequency grid), by source radius, shallowest first
+  std::vector<int> sh_order;
+  int sh_lmin = 0, sh_lmax = 0, sh_nw = 0, n_union = 0;
+  int* d_shorder = nullptr;
+  cd* d_shx = nullptr;
+  int* d_shs = nullptr;
   // batched mode: persistent blocks or one block per item, items taken per
   // queue access, persistent blocks per CU, and the captured forward
   int persist = 0, chunk = 1, blocks_per_cu = 4;
@@ -136,7 +143,9 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_mfin); dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0);
   dfree(c->d_sph_m0); dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_azep); dfree(c->d_eia);
   dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
-  dfree(c->d_seis); dfree(c->d_queue);
+  dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
+  c->sh_order.clear();
+  c->n_union = 0;
   for (auto& w : c->W) {
     dfree(w.d_plain);
     dfree(w.d_batched);
@@ -246,14 +255,85 @@ void build_work(ysx_ctx* c) {
       std::copy(cs.begin(), cs.end(), cost[k].begin() + a);
     }
   }
+  // Batched b-vector rows: events with the same receiver depth, frequency
+  // grid and row count (all receivers above their sources, full gravity,
+  // no per-lane variant) share one row march per (w, degree block) of
+  // their union grid; their own items leave the batched list.
+  c->sh_order.clear();
+  c->n_union = 0;
+  std::vector<char> member(nev, 0);
+  {
+    int e0 = -1;
+    for (int e = 0; e < nev && e0 < 0; e++) {
+      const RunK& k = c->rk[e];
+      if (k.brow && !k.var && !k.ngc && k.rr >= k.rs && k.lmax >= 1) e0 = e;
+    }
+    if (e0 >= 0) {
+      const RunK& a = c->rk[e0];
+      for (int e = 0; e < nev; e++) {
+        const RunK& k = c->rk[e];
+        member[e] = k.brow && !k.var && !k.ngc && k.rr >= k.rs && k.lmax >= 1 && k.rr == a.rr &&
+                    k.ir == a.ir && k.wmin == a.wmin && k.dw == a.dw && k.ep == a.ep &&
+                    k.tref == a.tref && k.ats == a.ats && k.bl == a.bl;
+        if (member[e]) c->sh_order.push_back(e);
+      }
+    }
+    if (c->sh_order.size() < 2) {
+      c->sh_order.clear();
+      std::fill(member.begin(), member.end(), 0);
+    }
+    std::stable_sort(c->sh_order.begin(), c->sh_order.end(),
+                     [&](int x, int y) { return c->rk[x].rs > c->rk[y].rs; });
+  }
+  std::vector<int4> uitems;
+  std::vector<double> ucost;
+  if (!c->sh_order.empty()) {
+    const RunK& a = c->rk[c->sh_order[0]];
+    int lmin = 1 << 30, lmax = 0, nw = 0;
+    for (int e : c->sh_order) {
+      lmin = std::min(lmin, c->rk[e].lmin);
+      lmax = std::max(lmax, c->rk[e].lmax);
+      nw = std::max(nw, c->rk[e].nw);
+    }
+    c->sh_lmin = lmin;
+    c->sh_lmax = lmax;
+    c->sh_nw = nw;
+    const int bl = a.bl, nlb = (lmax - lmin + 1 + bl - 1) / bl;
+    for (int j = nw; j >= 1; j--)
+      for (int lb = 0; lb < nlb; lb++) {
+        int l0 = lmin + lb * bl, need = 0;
+        double rsmin = a.rr;
+        for (int e : c->sh_order) {
+          const RunK& k = c->rk[e];
+          if (j <= k.nw && l0 <= k.lmax && l0 + bl - 1 >= std::max(k.lmin, 1)) {
+            need++;
+            rsmin = std::min(rsmin, k.rs);
+          }
+        }
+        if (!need) continue;
+        uitems.push_back(make_int4(-1, j, lb, (int)uitems.size()));
+        ucost.push_back((a.wmin + (double)(j - 1) * a.dw) * (a.rr - rsmin + 0.02 * need + 1e-3));
+      }
+    c->n_union = (int)uitems.size();
+  }
   for (int k = 0; k < W_N; k++) {
     WorkList& w = c->W[k];
-    std::vector<int> ord(w.plain.size());
+    std::vector<int4> items;
+    std::vector<double> cs;
+    for (size_t i = 0; i < w.plain.size(); i++) {
+      if (k == W_BVEC && member[w.plain[i].x]) continue;
+      items.push_back(w.plain[i]);
+      cs.push_back(cost[k][i]);
+    }
+    if (k == W_BVEC) {
+      items.insert(items.end(), uitems.begin(), uitems.end());
+      cs.insert(cs.end(), ucost.begin(), ucost.end());
+    }
+    std::vector<int> ord(items.size());
     std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(),
-                     [&](int a, int b) { return cost[k][a] > cost[k][b]; });
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cs[a] > cs[b]; });
     w.batched.resize(ord.size());
-    for (size_t i = 0; i < ord.size(); i++) w.batched[i] = w.plain[ord[i]];
+    for (size_t i = 0; i < ord.size(); i++) w.batched[i] = items[ord[i]];
   }
 }
 
@@ -275,6 +355,20 @@ int refine_work(ysx_ctx* c) {
   }
   // degrees of a wave: 64 (W_TOR) or bl (W_BVEC)
   auto wave_cost = [&](const int4& it, int kind) {
+    if (it.x < 0) {  // shared rows: the deepest of the events' marches, plus their branches
+      double m = 0.0;
+      int n = 0;
+      const int bl = c->rk[c->sh_order[0]].bl, l0 = c->sh_lmin + it.z * bl;
+      for (int e : c->sh_order) {
+        const RunK& R = c->rk[e];
+        if (it.y > R.nw || l0 > R.lmax || l0 + bl - 1 < std::max(R.lmin, 1)) continue;
+        int l = std::min(l0 + bl - 1, R.lmax);
+        size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (it.y - 1);
+        m = std::max(m, 16100.0 * cnt[5 * p + 3]);
+        n++;
+      }
+      return m * (1.0 + 0.05 * n);
+    }
     const RunK& R = c->rk[it.x];
     const int nl = (kind == W_TOR) ? 64 : R.bl;
     double m = 0.0;
@@ -556,15 +650,22 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipMalloc(&ctx->d_queue, Q_N * sizeof(int)));
   build_work(ctx);
   for (auto& w : ctx->W) {
-    size_t n = std::max<size_t>(1, w.plain.size());
+    size_t n = std::max<size_t>(1, std::max(w.plain.size(), w.batched.size()));
     HIPCHK(hipMalloc(&w.d_plain, n * sizeof(int4)));
     HIPCHK(hipMalloc(&w.d_batched, n * sizeof(int4)));
-    if (!w.plain.empty()) {
+    if (!w.plain.empty())
       HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
                        hipMemcpyHostToDevice));
+    if (!w.batched.empty())
       HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
                        hipMemcpyHostToDevice));
-    }
+  }
+  if (ctx->n_union > 0) {
+    HIPCHK(hipMalloc(&ctx->d_shorder, ctx->sh_order.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(ctx->d_shorder, ctx->sh_order.data(), ctx->sh_order.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&ctx->d_shx, (size_t)ctx->n_union * 14 * 64 * sizeof(cd)));
+    HIPCHK(hipMalloc(&ctx->d_shs, (size_t)ctx->n_union * 64 * sizeof(int)));
   }
   launch_twiddles(dim3((ntmax + 255) / 256), dim3(256), 0, ctx->stream, ntmax,
                      ctx->d_tw);
@@ -671,8 +772,12 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
       launch_minor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                          ctx->d_runs, qt, SD, S, ctx->d_cnt);
       if (timing) HIPCHK(hipEventRecord(ev[4], st));
-      (ctx->rk[0].brow ? launch_brow_march_item : launch_bvec_march_item)(
-          dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
+      if (ctx->rk[0].brow)
+        launch_brow_march_item(dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
+                               ShareK{}, ctx->d_rec, ctx->d_cnt);
+      else
+        launch_bvec_march_item(dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
+                               ctx->d_rec, ctx->d_cnt);
     }
     if (timing) HIPCHK(hipEventRecord(ev[5], st));
     HIPCHK(hipStreamWaitEvent(st, ev[11], 0));
@@ -720,6 +825,8 @@ int enqueue_batched(ysx_ctx* ctx) {
   int nev = (int)ctx->rk.size();
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
+  ShareK SH{ctx->d_shorder, ctx->n_union > 0 ? (int)ctx->sh_order.size() : 0, ctx->sh_lmin,
+            ctx->sh_lmax, ctx->sh_nw, ctx->d_shx, ctx->d_shs};
   PairRange PR = pairs_of(ctx, 0, nev);
   int nb = (int)((PR.n + 255) / 256);
   bool pe = ctx->persist != 0;
@@ -755,8 +862,12 @@ int enqueue_batched(ysx_ctx* ctx) {
                        S, ctx->d_cnt);
     launch_minor_march_pers(grid(qmin), b64, 0, st, ctx->d_tab, ctx->d_runs,
                        qmin, SD, S, ctx->d_cnt);
-    (ctx->rk[0].brow ? launch_brow_march_pers : launch_bvec_march_pers)(
-        grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
+    if (ctx->rk[0].brow)
+      launch_brow_march_pers(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, SH,
+                             ctx->d_rec, ctx->d_cnt);
+    else
+      launch_bvec_march_pers(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
+                             ctx->d_rec, ctx->d_cnt);
   } else {
     launch_tor_march_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
                        SD, ctx->d_rec, ctx->d_cnt);
@@ -764,8 +875,12 @@ int enqueue_batched(ysx_ctx* ctx) {
                        ctx->d_cnt);
     launch_minor_march_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
                        SD, S, ctx->d_cnt);
-    (ctx->rk[0].brow ? launch_brow_march_item : launch_bvec_march_item)(
-        grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_cnt);
+    if (ctx->rk[0].brow)
+      launch_brow_march_item(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, SH,
+                             ctx->d_rec, ctx->d_cnt);
+    else
+      launch_bvec_march_item(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
+                             ctx->d_rec, ctx->d_cnt);
   }
   Queue qv = batched_q(ctx, W_VAR, -1);
   if (qv.n > 0)

@@ -40,8 +40,8 @@ void YSX_LAUNCH(bvec_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const dou
 }
 #elif YSX_MARCH == 4
 void YSX_LAUNCH(brow_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                            const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt) {
-  hipLaunchKernelGGL(k_brow_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, rec, cnt);
+                            const RunK* runs, Queue Q, SphState S, ShareK SH, cd* rec, int* cnt) {
+  hipLaunchKernelGGL(k_brow_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, SH, rec, cnt);
 }
 #else
 #error "YSX_MARCH must be 0..4"

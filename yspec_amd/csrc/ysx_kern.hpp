@@ -118,6 +118,20 @@ __device__ inline Tab load_tab(const double* __restrict__ g, int nknot, int in_l
 }
 
 // the pairs of events [e0, e1): global pair index of thread t
+// Several events sharing the receiver depth and the frequency grid: the
+// b-vector row march of a (w, degree block) is the same for all of them from
+// the receiver down to each event's source interval, so one march serves
+// them all (k_brow_march items with event -1).  order: the events by source
+// radius, shallowest first; sx/ss: per union item and lane, the march state
+// saved while an event's last (partial) interval is marched.
+struct ShareK {
+  const int* order;
+  int n;                 // events in order (0: no shared items)
+  int lmin, lmax, nw;    // union grid
+  cd* sx;                // [items][14][64]
+  int* ss;               // [items][64]
+};
+
 struct PairRange {
   int e0, e1;
   long long g0, n;
@@ -270,7 +284,8 @@ void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw);
   void launch_bvec_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
                                   const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt);  \
   void launch_brow_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
-                                  const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt);
+                                  const RunK* runs, Queue Q, SphState S, ShareK SH, cd* rec,    \
+                                  int* cnt);
 YSX_MARCH_LAUNCHERS(item)
 YSX_MARCH_LAUNCHERS(pers)
 #undef YSX_MARCH_LAUNCHERS

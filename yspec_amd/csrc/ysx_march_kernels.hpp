@@ -358,18 +358,126 @@ __device__ __forceinline__ void brow_item(const Tab& T, const RunK& R, int j, in
   if (k == 0) cnt[5 * p + 3] = n;
 }
 
-// items (event, j, l-block of bl)
+// Rows of the shared events' pair (l, j): y_c[k] = x . bst_c * fac *
+// huge^(sc3 - sl) written to event e's record, as brow_item does.
+__device__ __forceinline__ void brow_finish(const RunK& R, int l, int j, int k, int rows,
+                                            const cd* x, int sl, int nsteps, const SphState& S,
+                                            cd* __restrict__ rec, int* __restrict__ cnt) {
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  cd fac = S.fac[p] * ipow_huge(S.sc3[p] - sl);
+  cd* o = rec + p * REC + O_YSPH;
+  for (int c = 0; c < 4; c++) {
+    cd v = C(0.0);
+#pragma unroll
+    for (int q = 0; q < 14; q++) v = cmad(v, x[q], S.bst[p * 56 + c * 14 + q]);
+    if (k < 3) o[6 * c + k] = v * fac;
+    if (k == 0)
+      for (int r = (rows < 3 ? rows : 3); r < 6; r++) o[6 * c + r] = C(0.0);
+  }
+  if (k == 0) cnt[5 * p + 3] = nsteps;
+}
+
+__device__ __forceinline__ bool lane_has(const RunK& R, int l, int j) {
+  return l >= 1 && l >= R.lmin && l <= R.lmax && j <= R.nw;
+}
+
+// One (w_j, degree block) of the union grid of the shared events: the row
+// march from the common receiver down, branching at each event's source
+// interval.  Each event's rows see exactly the steps of its own march
+// (march_yint_T over is_e..ir): the trunk takes the intervals above the
+// source interval whole, the branch the source interval from rs_e, so the
+// results are those of brow_item for every event.
+__device__ __forceinline__ void brow_shared_item(const double* gtab, const RunK* runs,
+                                                 const ShareK& SH, int j, int lb, int u,
+                                                 const SphState& S, cd* __restrict__ rec,
+                                                 int* __restrict__ cnt, double* sb, cd* slot) {
+  const RunK& R0 = runs[SH.order[0]];
+  const Tab T{gtab, R0.nknot};
+  const int rows = 64 / R0.bl;
+  const int k = threadIdx.x % rows;
+  const int l0 = SH.lmin + lb * R0.bl;
+  const int l = l0 + (int)(threadIdx.x / rows);
+  WS w = wave_omega(R0, j);
+  LS L = l_state((l >= 1 && l <= SH.lmax) ? l : 1);
+  // start rows from the first event holding this lane's pair (the minors
+  // from the receiver, mfin, are the same for all shared events)
+  cd x[14];
+  {
+    int pe = -1;
+    for (int q = 0; q < SH.n; q++)
+      if (pe < 0 && lane_has(runs[SH.order[q]], l, j)) pe = SH.order[q];
+    cd m[14];
+    if (pe >= 0) {
+      const RunK& R = runs[pe];
+      size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+      for (int q = 0; q < 14; q++) m[q] = S.mfin[p * 14 + q];
+    } else {
+      for (int q = 0; q < 14; q++) m[q] = C(0.0);
+    }
+    ysol_row(m, k, x);
+  }
+  cd* sx = SH.sx + (size_t)u * 14 * 64 + threadIdx.x;
+  int* ss = SH.ss + (size_t)u * 64 + threadIdx.x;
+  const int ir = R0.ir;
+  const double rr = R0.rr;
+  int sl = 0, ntr = 0, icur = ir, q = 0;
+  // one march call site: each pass marches either the next whole trunk
+  // interval, or an event's source interval from rs_e (its branch: the
+  // trunk state is saved, the event's rows finished, the state restored)
+  while (true) {
+    // next event with a pair in this wave (wave-uniform)
+    while (q < SH.n) {
+      const RunK& R = runs[SH.order[q]];
+      if (j <= R.nw && l0 <= R.lmax && l0 + R0.bl - 1 >= max(R.lmin, 1)) break;
+      q++;
+    }
+    if (q == SH.n) break;
+    const RunK& R = runs[SH.order[q]];
+    // branch once the trunk has reached the source interval; a source at
+    // the receiver takes no step at all (march_yint returns at once)
+    const bool branch = (icur <= R.is) || (R.rs == rr);
+    int iv;
+    double a, b;
+    if (!branch) {
+      iv = icur;
+      a = T(F_R, icur);
+      b = (icur == ir) ? rr : T(F_R, icur + 1);
+      icur--;
+    } else {
+      for (int c = 0; c < 14; c++) sx[c * 64] = x[c];
+      *ss = sl;
+      iv = R.is;
+      a = (R.rs == rr) ? rr : R.rs;
+      b = (R.rs == rr || iv == ir) ? rr : T(F_R, iv + 1);
+    }
+    int n = (a != b) ? march_interval_T(T, w, L, R0.nknot, iv, a, b, x, sl, sb, BvecTM(), slot)
+                     : 0;
+    if (!branch) {
+      ntr += n;
+    } else {
+      if (lane_has(R, l, j)) brow_finish(R, l, j, k, rows, x, sl, (R.rs == rr) ? 0 : ntr + n, S, rec, cnt);
+      for (int c = 0; c < 14; c++) x[c] = sx[c * 64];
+      sl = *ss;
+      q++;
+    }
+  }
+}
+
+// items (event, j, l-block of bl); event -1: a shared union item (.w its index)
 template <bool PERSIST>
 __global__ void __launch_bounds__(64) k_brow_march(const double* __restrict__ gtab,
                                                    const RunK* __restrict__ runs, Queue Q,
-                                                   SphState S, cd* __restrict__ rec,
+                                                   SphState S, ShareK SH, cd* __restrict__ rec,
                                                    int* __restrict__ cnt) {
   __shared__ double sb[6 * CH * BASE_MAX];
   __shared__ cd slots[2 * 14 * 64];
   cd* slot = slots + threadIdx.x;
-  RUN_ITEMS(PERSIST, Q,
-            brow_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, S, rec, cnt, sb,
-                      slot));
+  RUN_ITEMS(PERSIST, Q, {
+    if (it.x >= 0)
+      brow_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, S, rec, cnt, sb, slot);
+    else
+      brow_shared_item(gtab, runs, SH, it.y, it.z, it.w, S, rec, cnt, sb, slot);
+  });
 }
 
 }  // namespace
