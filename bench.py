@@ -149,11 +149,12 @@ def batched_compare(deck, steps, warmup, do_tune=True):
         "events": len(ev), "pairs": P, "receivers": nrec,
         "plain": res["plain"], "batched": res["batched"],
         "speedup": res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"],
-        "batched_ode_tflops_lower_bound": sum(ode.values()) / (res["batched"]["ms_per_step"]
-                                                               * 1e-3) / 1e12,
-        # the same over the FP64 peak (the whole forward's time, so a lower bound)
-        "batched_ode_frac_lower_bound": sum(ode.values()) / (res["batched"]["ms_per_step"]
-                                                             * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+        # the events' own ODE work (what the plain path computes) over the
+        # batched forward's time: an effective rate, since the batched path
+        # marches the b-vector rows of events with a common receiver depth
+        # and frequency grid once for all of them
+        "batched_ode_tflops_effective": sum(ode.values()) / (res["batched"]["ms_per_step"]
+                                                             * 1e-3) / 1e12,
         "tuning": tuning["best"] if tuning else None,
     }
 
