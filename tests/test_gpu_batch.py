@@ -1,8 +1,9 @@
 """Several events on one model: the plain path (one event after another)
-and the batched path (all events per launch from cost-sorted work lists,
-replayed from a HIP graph; one block per item or persistent blocks on
-atomic queues) give each event exactly the single-event result, which the
-oracle checks at 1e-9 (spectra) / 1e-8 (seismograms)."""
+gives each event exactly the single-event result, and the batched path
+(all events per launch from cost-sorted work lists, replayed from a HIP
+graph; one block per item or persistent blocks on atomic queues; marches
+shared between events) gives it to rounding; the oracle checks both at 1e-9
+(spectra) / 1e-8 (seismograms)."""
 import numpy as np
 import pytest
 
@@ -44,27 +45,44 @@ def single(api):
     return out
 
 
-def _check_events(g, single):
+def _check_events(g, single, tol=0.0):
+    """tol = 0: bit-identical; else relative L2"""
     for e, (spec, seis) in enumerate(single):
         got = g.event_spectra(e)
         for a, b in zip(got, spec):
-            assert np.array_equal(a, b), e
-        assert np.array_equal(g.event_seismograms(e), seis), e
+            assert (np.array_equal(a, b) if tol == 0.0 else rel_l2(a, b) <= tol), e
+        s = g.event_seismograms(e)
+        assert (np.array_equal(s, seis) if tol == 0.0 else rel_l2(s, seis) <= tol), e
+
+
+def _snapshot(g, n):
+    return [(g.event_spectra(e), g.event_seismograms(e)) for e in range(n)]
 
 
 def test_plain_and_batched_equal_single(api, single):
+    """The plain path gives every event its single-event result bit for bit.
+    The batched path shares the b-vector rows (bit-identical) and, for the
+    events that share marches, takes the secular determinant from the
+    transposed minor row (rounding-level differences); its graph replays and
+    launch shapes agree bit for bit with each other."""
     g = api.YSpec(synth_deck(0))
     g.set_events(small_events())
     g.forward_events(0)
     _check_events(g, single)
+    g.forward_events(1)
+    _check_events(g, single, tol=1e-12)
+    first = _snapshot(g, len(single))
     for rep in range(3):  # graph replays, with eager work in between
-        g.forward_events(1)
-        _check_events(g, single)
         g.forward_events(0)
+        _check_events(g, single)
+        g.forward_events(1)
+        for (sa, ka), (sb, kb) in zip(_snapshot(g, len(single)), first):
+            assert all(np.array_equal(x, y) for x, y in zip(sa, sb)) and np.array_equal(ka, kb)
     g.set_batch_tuning(persist=1, chunk=2, blocks_per_cu=2)
     for rep in range(2):
         g.forward_events(1)
-        _check_events(g, single)
+        for (sa, ka), (sb, kb) in zip(_snapshot(g, len(single)), first):
+            assert all(np.array_equal(x, y) for x, y in zip(sa, sb)) and np.array_equal(ka, kb)
 
 
 def test_events_vs_oracle(api, oracle_mod, single):

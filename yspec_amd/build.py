@@ -26,7 +26,7 @@ FC = os.environ.get("YSX_FC", "/opt/rocm/bin/amdflang")
 # ysx_k_march.hip is compiled once per kernel and work-distribution mode and
 # all objects compile in parallel.
 SOURCES = [("ysx_k_march.hip", f"ysx_k_march_{k}{p}.o", [f"-DYSX_MARCH={k}", f"-DYSX_PERSIST={p}"])
-           for k in range(5) for p in range(2)]
+           for k in range(6) for p in range(2)]
 # The receiver-sum MFMAs accumulate in VGPRs: with the default AGPR form the
 # loop-carried accumulators are copied between AGPRs and VGPRs every step.
 SOURCES += [("ysx_k_misc.hip", "ysx_k_misc.o", ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]),

@@ -46,9 +46,9 @@ using namespace ysx;
 // ====================================================================== API
 namespace {
 
-enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_VAR, W_FFTA, W_FFTB, W_N };
+enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_VAR, W_FFTA, W_FFTB, W_DROW, W_N };
 // queue counters of the batched launches (three marches share the W_TOR list)
-enum QueueSlot { Q_TOR = 0, Q_TOP, Q_MINOR, Q_BVEC, Q_RAD, Q_N };
+enum QueueSlot { Q_TOR = 0, Q_TOP, Q_MINOR, Q_BVEC, Q_RAD, Q_DROW, Q_N };
 
 struct WorkList {
   std::vector<int4> plain;    // event-major, each event in its plain order
@@ -80,7 +80,7 @@ struct ysx_ctx {
   cd* d_rec = nullptr;
   int* d_starts = nullptr;
   int* d_cnt = nullptr;
-  cd *d_bst = nullptr, *d_mfin = nullptr, *d_m2 = nullptr, *d_fac = nullptr;
+  cd *d_bst = nullptr, *d_mfin = nullptr, *d_m2 = nullptr, *d_fac = nullptr, *d_ms1 = nullptr;
   int *d_tor_i1 = nullptr, *d_sph_i1 = nullptr;
   cd *d_tor_y0 = nullptr, *d_sph_m0 = nullptr;
   int* d_sc3 = nullptr;
@@ -97,7 +97,9 @@ struct ysx_ctx {
   // events whose b-vector rows share one march in the batched path (same
   // receiver depth and frequency grid), by source radius, shallowest first
   std::vector<int> sh_order;
-  int sh_lmin = 0, sh_lmax = 0, sh_nw = 0, n_union = 0;
+  int sh_lmin = 0, sh_lmax = 0, sh_nw = 0;
+  int n_union = 0;   // scratch slots: the larger of the two union lists
+  int n_ubrow = 0;   // union items of the b-vector rows
   int* d_shorder = nullptr;
   cd* d_shx = nullptr;
   int* d_shs = nullptr;
@@ -140,7 +142,7 @@ void drop_graph(ysx_ctx* c) {
 void free_run(ysx_ctx* c) {
   drop_graph(c);
   dfree(c->d_runs); dfree(c->d_rec); dfree(c->d_starts); dfree(c->d_cnt); dfree(c->d_bst);
-  dfree(c->d_mfin); dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0);
+  dfree(c->d_mfin); dfree(c->d_ms1); dfree(c->d_m2); dfree(c->d_tor_i1); dfree(c->d_sph_i1); dfree(c->d_tor_y0);
   dfree(c->d_sph_m0); dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_azep); dfree(c->d_eia);
   dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
@@ -284,9 +286,10 @@ void build_work(ysx_ctx* c) {
     }
     std::stable_sort(c->sh_order.begin(), c->sh_order.end(),
                      [&](int x, int y) { return c->rk[x].rs > c->rk[y].rs; });
+    for (int e = 0; e < nev; e++) c->rk[e].drow = member[e] ? 1 : 0;
   }
-  std::vector<int4> uitems;
-  std::vector<double> ucost;
+  std::vector<int4> uitems, ditems;
+  std::vector<double> ucost, dcost;
   if (!c->sh_order.empty()) {
     const RunK& a = c->rk[c->sh_order[0]];
     int lmin = 1 << 30, lmax = 0, nw = 0;
@@ -314,7 +317,25 @@ void build_work(ysx_ctx* c) {
         uitems.push_back(make_int4(-1, j, lb, (int)uitems.size()));
         ucost.push_back((a.wmin + (double)(j - 1) * a.dw) * (a.rr - rsmin + 0.02 * need + 1e-3));
       }
-    c->n_union = (int)uitems.size();
+    // the determinant rows: blocks of 64 degrees
+    const int nlbd = (lmax - lmin + 1 + 63) / 64;
+    for (int j = nw; j >= 1; j--)
+      for (int lb = 0; lb < nlbd; lb++) {
+        int l0 = lmin + lb * 64, need = 0;
+        double rsmin = a.rr;
+        for (int e : c->sh_order) {
+          const RunK& k = c->rk[e];
+          if (j <= k.nw && l0 <= k.lmax && l0 + 63 >= std::max(k.lmin, 1)) {
+            need++;
+            rsmin = std::min(rsmin, k.rs);
+          }
+        }
+        if (!need) continue;
+        ditems.push_back(make_int4(-1, j, lb, (int)ditems.size()));
+        dcost.push_back((a.wmin + (double)(j - 1) * a.dw) * (a.rr - rsmin + 0.02 * need + 1e-3));
+      }
+    c->n_union = (int)std::max(uitems.size(), ditems.size());
+    c->n_ubrow = (int)uitems.size();
   }
   for (int k = 0; k < W_N; k++) {
     WorkList& w = c->W[k];
@@ -328,6 +349,10 @@ void build_work(ysx_ctx* c) {
     if (k == W_BVEC) {
       items.insert(items.end(), uitems.begin(), uitems.end());
       cs.insert(cs.end(), ucost.begin(), ucost.end());
+    }
+    if (k == W_DROW) {
+      items.insert(items.end(), ditems.begin(), ditems.end());
+      cs.insert(cs.end(), dcost.begin(), dcost.end());
     }
     std::vector<int> ord(items.size());
     std::iota(ord.begin(), ord.end(), 0);
@@ -563,6 +588,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     // potential/tilt correction), or the 4 source-side columns
     k.brow = ctx->bcols ? 0 : 1;
     k.bl = (k.brow && !(R.out == 2 && R.cor != 0)) ? 32 : 16;
+    k.drow = 0;  // set by build_work for the events that share marches
     k.jpad = (R.nw + 15) / 16 * 16;
     k.nrpad = (R.nr + 15) / 16 * 16;
     k.nt = R.nt;
@@ -596,6 +622,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipMemset(ctx->d_cnt, 0, P * 5 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_bst, P * 56 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_mfin, P * 14 * sizeof(cd)));
+  HIPCHK(hipMalloc(&ctx->d_ms1, P * 14 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_m2, P * 14 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_tor_i1, P * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_sph_i1, P * sizeof(int)));
@@ -649,6 +676,8 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipMalloc(&ctx->d_seis, sio * sizeof(double)));
   HIPCHK(hipMalloc(&ctx->d_queue, Q_N * sizeof(int)));
   build_work(ctx);
+  // build_work marks the events that share marches (RunK::drow)
+  HIPCHK(hipMemcpy(ctx->d_runs, ctx->rk.data(), nev * sizeof(RunK), hipMemcpyHostToDevice));
   for (auto& w : ctx->W) {
     size_t n = std::max<size_t>(1, std::max(w.plain.size(), w.batched.size()));
     HIPCHK(hipMalloc(&w.d_plain, n * sizeof(int4)));
@@ -728,7 +757,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
   hipEvent_t* ev = ctx->ev;
   size_t lds = ctx->tab_lds ? ctx->lds_bytes : 0;
   dim3 b256(256), b64(64);
-  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
+  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
   for (int e = e0; e < e1; e++) {
     PairRange PR = pairs_of(ctx, e, e + 1);
@@ -770,7 +799,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
       launch_minor_top_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                          ctx->d_runs, qt, S, ctx->d_cnt);
       launch_minor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
-                         ctx->d_runs, qt, SD, S, ctx->d_cnt);
+                         ctx->d_runs, qt, SD, S, ctx->d_cnt, 0);
       if (timing) HIPCHK(hipEventRecord(ev[4], st));
       if (ctx->rk[0].brow)
         launch_brow_march_item(dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
@@ -823,7 +852,7 @@ int enqueue_batched(ysx_ctx* ctx) {
   size_t lds = ctx->tab_lds ? ctx->lds_bytes : 0;
   dim3 b256(256), b64(64);
   int nev = (int)ctx->rk.size();
-  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3};
+  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
   ShareK SH{ctx->d_shorder, ctx->n_union > 0 ? (int)ctx->sh_order.size() : 0, ctx->sh_lmin,
             ctx->sh_lmax, ctx->sh_nw, ctx->d_shx, ctx->d_shs};
@@ -836,6 +865,7 @@ int enqueue_batched(ysx_ctx* ctx) {
   };
   Queue qt = batched_q(ctx, W_TOR, Q_TOR), qtop = batched_q(ctx, W_TOR, Q_TOP),
         qmin = batched_q(ctx, W_TOR, Q_MINOR), qb = batched_q(ctx, W_BVEC, Q_BVEC),
+        qd = batched_q(ctx, W_DROW, Q_DROW),
         qr = batched_q(ctx, W_RAD, -1), ql = batched_q(ctx, W_LSUM, -1),
         qs = batched_q(ctx, W_SEIS, -1);
   if (pe) launch_reset(dim3(1), dim3(64), 0, st, ctx->d_queue, (int)Q_N);
@@ -861,7 +891,10 @@ int enqueue_batched(ysx_ctx* ctx) {
     launch_minor_top_pers(grid(qtop), b64, 0, st, ctx->d_tab, ctx->d_runs, qtop,
                        S, ctx->d_cnt);
     launch_minor_march_pers(grid(qmin), b64, 0, st, ctx->d_tab, ctx->d_runs,
-                       qmin, SD, S, ctx->d_cnt);
+                       qmin, SD, S, ctx->d_cnt, 1);
+    if (qd.n > 0)
+      launch_drow_march_pers(grid(qd), b64, 0, st, ctx->d_tab, ctx->d_runs, qd, S, SH,
+                             ctx->d_cnt);
     if (ctx->rk[0].brow)
       launch_brow_march_pers(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, SH,
                              ctx->d_rec, ctx->d_cnt);
@@ -874,7 +907,10 @@ int enqueue_batched(ysx_ctx* ctx) {
     launch_minor_top_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt, S,
                        ctx->d_cnt);
     launch_minor_march_item(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
-                       SD, S, ctx->d_cnt);
+                       SD, S, ctx->d_cnt, 1);
+    if (qd.n > 0)
+      launch_drow_march_item(grid(qd), b64, 0, st, ctx->d_tab, ctx->d_runs, qd, S, SH,
+                             ctx->d_cnt);
     if (ctx->rk[0].brow)
       launch_brow_march_item(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, SH,
                              ctx->d_rec, ctx->d_cnt);

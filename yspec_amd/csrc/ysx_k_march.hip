@@ -1,13 +1,14 @@
 // ysx_k_march.hip -- one march kernel and its launch function per object
-// file: the build compiles this file ten times, once per kernel
+// file: the build compiles this file twelve times, once per kernel
 // (YSX_MARCH = 0 toroidal, 1 minors from the surface, 2 minors from the
-// start level, 3 b-vector columns, 4 b-vector rows) and work-distribution mode (YSX_PERSIST = 0 one
+// start level, 3 b-vector columns, 4 b-vector rows, 5 determinant rows) and
+// work-distribution mode (YSX_PERSIST = 0 one
 // block per work item, 1 persistent blocks on atomic queues), so that the
 // large kernels compile in parallel.
 #include "ysx_march_kernels.hpp"
 
 #if !defined(YSX_MARCH) || !defined(YSX_PERSIST)
-#error "compile with -DYSX_MARCH=0..4 -DYSX_PERSIST=0|1"
+#error "compile with -DYSX_MARCH=0..5 -DYSX_PERSIST=0|1"
 #endif
 
 #if YSX_PERSIST
@@ -30,8 +31,13 @@ void YSX_LAUNCH(minor_top)(dim3 g, dim3 b, size_t lds, hipStream_t s, const doub
 }
 #elif YSX_MARCH == 2
 void YSX_LAUNCH(minor_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                             const RunK* runs, Queue Q, StartData SD, SphState S, int* cnt) {
-  hipLaunchKernelGGL(k_minor_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, SD, S, cnt);
+                             const RunK* runs, Queue Q, StartData SD, SphState S, int* cnt,
+                             int drow_on) {
+  if (drow_on)
+    hipLaunchKernelGGL((k_minor_march<YSX_PERSIST, true>), g, b, lds, s, gtab, runs, Q, SD, S, cnt);
+  else
+    hipLaunchKernelGGL((k_minor_march<YSX_PERSIST, false>), g, b, lds, s, gtab, runs, Q, SD, S,
+                       cnt);
 }
 #elif YSX_MARCH == 3
 void YSX_LAUNCH(bvec_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
@@ -43,8 +49,13 @@ void YSX_LAUNCH(brow_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const dou
                             const RunK* runs, Queue Q, SphState S, ShareK SH, cd* rec, int* cnt) {
   hipLaunchKernelGGL(k_brow_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, SH, rec, cnt);
 }
+#elif YSX_MARCH == 5
+void YSX_LAUNCH(drow_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                            const RunK* runs, Queue Q, SphState S, ShareK SH, int* cnt) {
+  hipLaunchKernelGGL(k_drow_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, S, SH, cnt);
+}
 #else
-#error "YSX_MARCH must be 0..4"
+#error "YSX_MARCH must be 0..5"
 #endif
 
 }  // namespace ysx

@@ -23,6 +23,9 @@ struct RunK {
   int radg;                     // lanes per radial (l = 0) solve: 1 or 8
   int brow, bl;                 // b-vector as receiver-side rows (1) or source-side
                                 // columns (0); degrees per b-vector wave
+  int drow;                     // shared with other events in the batched path: there the
+                                // secular determinant comes from the receiver-side row of
+                                // the transposed minor system (k_drow_march)
   int tablds;                   // model table staged in LDS (else read from HBM)
   int nt, lognt, i1, i2, roff;  // FFT length, frequency bins, first receiver
   long long pbase;              // first pair of this event in the per-pair arrays
@@ -169,6 +172,7 @@ struct SphState {
   cd* m2;    // [npair][14] minor vector from the surface
   cd* fac;   // [npair]
   int* sc3;  // [npair]
+  cd* ms1;   // [npair][14] minors at the source (events with the determinant row, drow)
 };
 
 // ------------------------------------------------------- march helpers
@@ -280,12 +284,14 @@ void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw);
                                  const RunK* runs, Queue Q, SphState S, int* cnt);              \
   void launch_minor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s,                   \
                                    const double* gtab, const RunK* runs, Queue Q, StartData SD, \
-                                   SphState S, int* cnt);                                       \
+                                   SphState S, int* cnt, int drow_on);                          \
   void launch_bvec_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
                                   const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt);  \
   void launch_brow_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
                                   const RunK* runs, Queue Q, SphState S, ShareK SH, cd* rec,    \
-                                  int* cnt);
+                                  int* cnt);                                                     \
+  void launch_drow_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
+                                  const RunK* runs, Queue Q, SphState S, ShareK SH, int* cnt);
 YSX_MARCH_LAUNCHERS(item)
 YSX_MARCH_LAUNCHERS(pers)
 #undef YSX_MARCH_LAUNCHERS

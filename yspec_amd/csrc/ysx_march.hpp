@@ -318,6 +318,70 @@ struct BvecM : SolidB {
   }
 };
 
+// Transpose of sph_minors_solid_derivs (src/module_int.f90:818-871):
+// e = A^T x for the solid minor system's matrix A.  Marched from the
+// receiver back to the source it carries the coefficients of the secular
+// determinant (delta_solid in its first argument) down to the source.
+template <class Init, class Sink>
+__device__ inline void minorT_derivs(const SolidA& A, const cd* x, const Init& I, Sink&& sink) {
+  const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
+               s23 = A.s23;
+  const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
+           s22 = A.s22;
+  const double tp = t22 + t33, tm = t22 - t33;
+  const cd a123 = t11 + tp, a126 = t11 + tm, a135 = t11 - tm, a234 = tp - t11;
+  const cd t12x2 = 2.0 * t12, s12x2 = 2.0 * s12;
+  cd acc;
+  acc = I.c(0, s11, x[9]); acc = cmad(acc, a123, x[0]); acc = rmad(acc, s13, x[1]);
+  acc = rmad(acc, s23, x[2]); acc = cmsb(acc, s12, x[4]); sink(0, cmsb(acc, s22, x[5]));
+  acc = I.c(1, s12, x[10]); acc = rmad(acc, t22, x[1]); acc = cmsb(acc, t12, x[2]);
+  acc = cmsb(acc, s22, x[6]); acc = rmsb(acc, s23, x[7]); acc = rmsb(acc, 2.0 * t31, x[9]);
+  sink(1, rmad(acc, 2.0 * s13, x[11]));
+  acc = I.c(2, t11, x[2]); acc = rmsb(acc, t21, x[1]); acc = rmad(acc, t31, x[4]);
+  acc = cmad(acc, s12, x[6]); acc = rmsb(acc, s13, x[7]); acc = rmsb(acc, 2.0 * s23, x[8]);
+  sink(2, cmsb(acc, s11, x[10]));
+  acc = I.c(3, a126, x[3]); acc = rmad(acc, k33, x[0]); acc = rmsb(acc, t31, x[1]);
+  acc = cmad(acc, s12, x[7]); acc = cmad(acc, s22, x[8]); sink(3, cmsb(acc, s11, x[11]));
+  acc = I.c(4, s12x2, x[12]); acc = rmad(acc, t33, x[4]); acc = cmsb(acc, t12x2, x[5]);
+  acc = rmsb(acc, s23, x[6]); acc = rmad(acc, 2.0 * t21, x[9]); sink(4, rmsb(acc, s13, x[10]));
+  acc = I.c(5, a135, x[5]); acc = cmsb(acc, k22, x[0]); acc = rmsb(acc, t21, x[4]);
+  acc = rmad(acc, s13, x[6]); sink(5, cmsb(acc, s11, x[12]));
+  acc = I.r(6, t21, x[10]); acc = cmsb(acc, k22, x[1]); acc = rmsb(acc, t22, x[6]);
+  acc = rmad(acc, 2.0 * t31, x[12]); sink(6, rmad(acc, 2.0 * s13, x[13]));
+  acc = I.r(7, 2.0 * t21, x[11]); acc = rmsb(acc, k33, x[4]); acc = rmsb(acc, t33, x[7]);
+  acc = cmsb(acc, t12x2, x[8]); acc = rmsb(acc, t31, x[10]); sink(7, cmsb(acc, s12x2, x[13]));
+  acc = I.c(8, k22, x[3]); acc = rmsb(acc, k33, x[5]); acc = rmad(acc, t31, x[6]);
+  acc = rmsb(acc, t21, x[7]); acc = cmsb(acc, a234, x[8]); sink(8, cmad(acc, s11, x[13]));
+  acc = I.c(9, k11, x[0]); acc = cmad(acc, t12, x[4]); acc = cmad(acc, a234, x[9]);
+  acc = rmsb(acc, s23, x[10]); sink(9, cmad(acc, s22, x[12]));
+  acc = I.c(10, t12, x[6]); acc = cmsb(acc, k11, x[2]); acc = cmsb(acc, t11, x[10]);
+  sink(10, rmad(acc, 2.0 * s23, x[13]));
+  acc = I.c(11, t12, x[7]); acc = cmsb(acc, k11, x[3]); acc = rmsb(acc, k33, x[9]);
+  acc = cmsb(acc, a135, x[11]); sink(11, cmsb(acc, s22, x[13]));
+  acc = I.c(12, k22, x[9]); acc = cmsb(acc, k11, x[5]); sink(12, cmsb(acc, a126, x[12]));
+  acc = I.c(13, k11, x[8]); acc = cmsb(acc, k22, x[11]); acc = rmad(acc, k33, x[12]);
+  sink(13, cmsb(acc, a123, x[13]));
+}
+
+// the transposed solid minor system (the secular determinant's row)
+struct MinorTM : SolidB {
+  static constexpr int N = 14;
+  template <class Sink>
+  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+                                Sink&& sink) const {
+    SolidA A;
+    amat(b, L, A);
+    minorT_derivs(A, y, NoInit(), sink);
+  }
+  template <class G, class Sink>
+  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+                                    const G& g, Sink&& sink) const {
+    SolidA A;
+    amat(b, L, A);
+    minorT_derivs(A, y, AccInit<G>{g}, sink);
+  }
+};
+
 // the transposed b-vector system (receiver-side rows)
 struct BvecTM : SolidB {
   static constexpr int N = 14;

@@ -109,6 +109,7 @@ __device__ __forceinline__ int minors_from_top(const Tab& T, const WS& w, const 
   return sc2;
 }
 
+template <bool DROW>
 __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, int lb,
                                            const StartData& SD, const SphState& S,
                                            int* __restrict__ cnt, double* sb, cd* slot) {
@@ -167,6 +168,18 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
         b_start_unit(m, cols[c], bb);
         for (int q = 0; q < 14; q++) S.bst[p * 56 + c * 14 + q] = bb[q];
       }
+    if (DROW && R.drow) {
+      // the determinant comes from k_drow_march's receiver-side row; keep
+      // the minors at the source for it
+      if (!valid) return;
+      for (int q = 0; q < 14; q++) {
+        S.ms1[p * 14 + q] = m[q];
+        S.mfin[p * 14 + q] = S.m2[p * 14 + q];
+      }
+      cnt[5 * p + 1] += nsol;
+      cnt[5 * p + 2] += nflu;
+      return;
+    }
   } else {
     if (valid)
       for (int q = 0; q < 14; q++) S.mfin[p * 14 + q] = m[q];
@@ -195,8 +208,9 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
   cnt[5 * p + 2] += nflu;
 }
 
-// items (event, j, l-block of 64)
-template <bool PERSIST>
+// items (event, j, l-block of 64).  DROW: events marked drow stop at the
+// source and leave the determinant to k_drow_march (batched path)
+template <bool PERSIST, bool DROW>
 __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ gtab,
                                                     const RunK* __restrict__ runs, Queue Q,
                                                     StartData SD, SphState S,
@@ -205,8 +219,8 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
   __shared__ cd slots[2 * 14 * 64];
   cd* slot = slots + threadIdx.x;
   RUN_ITEMS(PERSIST, Q,
-            minor_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, S, cnt, sb,
-                       slot));
+            minor_item<DROW>(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, S, cnt, sb,
+                             slot));
 }
 
 // The surface minor vector ms2 of every pair (src/module_solver.f90:730-741,
@@ -381,54 +395,32 @@ __device__ __forceinline__ bool lane_has(const RunK& R, int l, int j) {
   return l >= 1 && l >= R.lmin && l <= R.lmax && j <= R.nw;
 }
 
-// One (w_j, degree block) of the union grid of the shared events: the row
-// march from the common receiver down, branching at each event's source
-// interval.  Each event's rows see exactly the steps of its own march
-// (march_yint_T over is_e..ir): the trunk takes the intervals above the
-// source interval whole, the branch the source interval from rs_e, so the
-// results are those of brow_item for every event.
-__device__ __forceinline__ void brow_shared_item(const double* gtab, const RunK* runs,
-                                                 const ShareK& SH, int j, int lb, int u,
-                                                 const SphState& S, cd* __restrict__ rec,
-                                                 int* __restrict__ cnt, double* sb, cd* slot) {
+// The receiver-side row march of one (w_j, degree block l0 .. l0+bl-1) of
+// the shared events' union grid, from the common receiver down, branching
+// at each event's source interval.  Each event's rows see exactly the steps
+// of its own march (march_yint_T over is_e..ir): the trunk takes the
+// intervals above the source interval whole, the branch the source interval
+// from rs_e (trunk state saved in global scratch meanwhile), so the results
+// are those of the per-event items.  finish(R, x, scale, steps) is called
+// (wave-uniformly) at the end of each event's branch.
+template <class F, class Fin>
+__device__ __forceinline__ void shared_march(const RunK* runs, const ShareK& SH, const Tab& T,
+                                             const WS& w, const LS& L, int j, int l0, int bl,
+                                             int u, cd* x, double* sb, cd* slot, const F& f,
+                                             Fin&& finish) {
   const RunK& R0 = runs[SH.order[0]];
-  const Tab T{gtab, R0.nknot};
-  const int rows = 64 / R0.bl;
-  const int k = threadIdx.x % rows;
-  const int l0 = SH.lmin + lb * R0.bl;
-  const int l = l0 + (int)(threadIdx.x / rows);
-  WS w = wave_omega(R0, j);
-  LS L = l_state((l >= 1 && l <= SH.lmax) ? l : 1);
-  // start rows from the first event holding this lane's pair (the minors
-  // from the receiver, mfin, are the same for all shared events)
-  cd x[14];
-  {
-    int pe = -1;
-    for (int q = 0; q < SH.n; q++)
-      if (pe < 0 && lane_has(runs[SH.order[q]], l, j)) pe = SH.order[q];
-    cd m[14];
-    if (pe >= 0) {
-      const RunK& R = runs[pe];
-      size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
-      for (int q = 0; q < 14; q++) m[q] = S.mfin[p * 14 + q];
-    } else {
-      for (int q = 0; q < 14; q++) m[q] = C(0.0);
-    }
-    ysol_row(m, k, x);
-  }
   cd* sx = SH.sx + (size_t)u * 14 * 64 + threadIdx.x;
   int* ss = SH.ss + (size_t)u * 64 + threadIdx.x;
   const int ir = R0.ir;
   const double rr = R0.rr;
   int sl = 0, ntr = 0, icur = ir, q = 0;
   // one march call site: each pass marches either the next whole trunk
-  // interval, or an event's source interval from rs_e (its branch: the
-  // trunk state is saved, the event's rows finished, the state restored)
+  // interval or an event's source interval
   while (true) {
     // next event with a pair in this wave (wave-uniform)
     while (q < SH.n) {
       const RunK& R = runs[SH.order[q]];
-      if (j <= R.nw && l0 <= R.lmax && l0 + R0.bl - 1 >= max(R.lmin, 1)) break;
+      if (j <= R.nw && l0 <= R.lmax && l0 + bl - 1 >= max(R.lmin, 1)) break;
       q++;
     }
     if (q == SH.n) break;
@@ -450,17 +442,56 @@ __device__ __forceinline__ void brow_shared_item(const double* gtab, const RunK*
       a = (R.rs == rr) ? rr : R.rs;
       b = (R.rs == rr || iv == ir) ? rr : T(F_R, iv + 1);
     }
-    int n = (a != b) ? march_interval_T(T, w, L, R0.nknot, iv, a, b, x, sl, sb, BvecTM(), slot)
-                     : 0;
+    int n = (a != b) ? march_interval_T(T, w, L, R0.nknot, iv, a, b, x, sl, sb, f, slot) : 0;
     if (!branch) {
       ntr += n;
     } else {
-      if (lane_has(R, l, j)) brow_finish(R, l, j, k, rows, x, sl, (R.rs == rr) ? 0 : ntr + n, S, rec, cnt);
+      finish(R, x, sl, (R.rs == rr) ? 0 : ntr + n);
       for (int c = 0; c < 14; c++) x[c] = sx[c * 64];
       sl = *ss;
       q++;
     }
   }
+}
+
+// pair index of lane l in event R (lane_has(R, l, j) must hold)
+__device__ __forceinline__ size_t pair_of(const RunK& R, int l, int j) {
+  return (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+}
+
+// the first shared event holding pair (l, j), or -1
+__device__ __forceinline__ int first_holder(const RunK* runs, const ShareK& SH, int l, int j) {
+  int pe = -1;
+  for (int q = 0; q < SH.n; q++)
+    if (pe < 0 && lane_has(runs[SH.order[q]], l, j)) pe = SH.order[q];
+  return pe;
+}
+
+__device__ __forceinline__ void brow_shared_item(const double* gtab, const RunK* runs,
+                                                 const ShareK& SH, int j, int lb, int u,
+                                                 const SphState& S, cd* __restrict__ rec,
+                                                 int* __restrict__ cnt, double* sb, cd* slot) {
+  const RunK& R0 = runs[SH.order[0]];
+  const Tab T{gtab, R0.nknot};
+  const int rows = 64 / R0.bl;
+  const int k = threadIdx.x % rows;
+  const int l0 = SH.lmin + lb * R0.bl;
+  const int l = l0 + (int)(threadIdx.x / rows);
+  WS w = wave_omega(R0, j);
+  LS L = l_state((l >= 1 && l <= SH.lmax) ? l : 1);
+  // rows from the first event holding this lane's pair (the minors from the
+  // receiver, mfin, are the same for all shared events)
+  cd x[14];
+  {
+    int pe = first_holder(runs, SH, l, j);
+    cd m[14];
+    for (int q = 0; q < 14; q++) m[q] = (pe >= 0) ? S.mfin[pair_of(runs[pe], l, j) * 14 + q] : C(0.0);
+    ysol_row(m, k, x);
+  }
+  shared_march(runs, SH, T, w, L, j, l0, R0.bl, u, x, sb, slot, BvecTM(),
+               [&](const RunK& R, const cd* xe, int sl, int nsteps) {
+                 if (lane_has(R, l, j)) brow_finish(R, l, j, k, rows, xe, sl, nsteps, S, rec, cnt);
+               });
 }
 
 // items (event, j, l-block of bl); event -1: a shared union item (.w its index)
@@ -478,6 +509,79 @@ __global__ void __launch_bounds__(64) k_brow_march(const double* __restrict__ gt
     else
       brow_shared_item(gtab, runs, SH, it.y, it.z, it.w, S, rec, cnt, sb, slot);
   });
+}
+
+// ------------------------------------------------------------ k_drow_march
+// The secular determinant delta = delta_solid(ms3(rr), ms2(rr)) of
+// sph_solver (src/module_solver.f90:730-743) with ms3 = P ms1(rs) the minor
+// march from the source to the receiver: delta is linear in ms3, so
+// delta = (d P) . ms1(rs) with d its coefficients (given ms2(rr)), and the
+// row d P is marched from the receiver back to the source with the
+// transposed steps of the same step sequence.  Writes fac = -1/delta and
+// the row's scale count as sc3 (so y = ysol(...) fac huge^(sc3 - sc4) keeps
+// its form); rounding-level differences from the forward continuation.
+__device__ __forceinline__ void delta_row(const cd* m2, cd* x) {
+  // coefficients of m1 in sph_delta_solid(m1, m2), src/module_int.f90:1705-1725
+  x[0] = m2[13];
+  x[1] = -2.0 * m2[6];
+  x[2] = -2.0 * m2[10];
+  x[3] = -m2[12];
+  x[4] = -2.0 * m2[7];
+  x[5] = -m2[11];
+  x[6] = 2.0 * m2[1];
+  x[7] = 2.0 * m2[4];
+  x[8] = m2[9];
+  x[9] = -m2[8];
+  x[10] = 2.0 * m2[2];
+  x[11] = m2[5];
+  x[12] = m2[3];
+  x[13] = -m2[0];
+}
+
+__device__ __forceinline__ void drow_finish(const RunK& R, int l, int j, const cd* x, int sd,
+                                            int nsteps, const SphState& S,
+                                            int* __restrict__ cnt) {
+  size_t p = pair_of(R, l, j);
+  cd delta = C(0.0);
+#pragma unroll
+  for (int q = 0; q < 14; q++) delta = cmad(delta, x[q], S.ms1[p * 14 + q]);
+  S.fac[p] = -cinv(delta);
+  S.sc3[p] = sd;
+  cnt[5 * p + 1] += nsteps;
+}
+
+__device__ __forceinline__ void drow_shared_item(const double* gtab, const RunK* runs,
+                                                 const ShareK& SH, int j, int lb, int u,
+                                                 const SphState& S, int* __restrict__ cnt,
+                                                 double* sb, cd* slot) {
+  const RunK& R0 = runs[SH.order[0]];
+  const Tab T{gtab, R0.nknot};
+  const int l0 = SH.lmin + lb * 64;
+  const int l = l0 + (int)threadIdx.x;
+  WS w = wave_omega(R0, j);
+  LS L = l_state((l >= 1 && l <= SH.lmax) ? l : 1);
+  cd x[14];
+  {
+    int pe = first_holder(runs, SH, l, j);
+    cd m2[14];
+    for (int q = 0; q < 14; q++) m2[q] = (pe >= 0) ? S.mfin[pair_of(runs[pe], l, j) * 14 + q] : C(0.0);
+    delta_row(m2, x);
+  }
+  shared_march(runs, SH, T, w, L, j, l0, 64, u, x, sb, slot, MinorTM(),
+               [&](const RunK& R, const cd* xe, int sd, int nsteps) {
+                 if (lane_has(R, l, j)) drow_finish(R, l, j, xe, sd, nsteps, S, cnt);
+               });
+}
+
+// items (-1, j, l-block of 64, union index) of the shared events (batched path)
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_drow_march(const double* __restrict__ gtab,
+                                                   const RunK* __restrict__ runs, Queue Q,
+                                                   SphState S, ShareK SH, int* __restrict__ cnt) {
+  __shared__ double sb[6 * CH * BASE_MAX];
+  __shared__ cd slots[2 * 14 * 64];
+  cd* slot = slots + threadIdx.x;
+  RUN_ITEMS(PERSIST, Q, drow_shared_item(gtab, runs, SH, it.y, it.z, it.w, S, cnt, sb, slot));
 }
 
 }  // namespace
