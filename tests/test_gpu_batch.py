@@ -118,3 +118,33 @@ def test_batched_columns_match_rows(api, single):
             assert rel_l2(a, b) <= 1e-11, e
         assert rel_l2(g.event_seismograms(e), seis) <= 1e-11, e
     g.close()
+
+
+def test_shared_marches_edge_cases(api, oracle_mod):
+    """Events sharing the receiver-side row marches: two sources in one knot
+    interval, two identical sources, a source at the receiver (zero-length
+    march) and different frequency ranges; batched results against the
+    single-event ones (1e-12) and the oracle (1e-9 / 1e-8)."""
+    deck = synth_deck(0)
+    ev = batch_events(nev=5, seed=7, nr=2, fmax_range=(4.0, 6.0), tout=40.0, dt=20.0,
+                      lmax_per_mhz=5.0)
+    depths = (300.0, 301.0, 120.0, 120.0, 0.0)
+    for p, d in zip(ev, depths):
+        p.src_depth = d
+        p.f21, p.f22 = 0.6 * p.f2, 0.8 * p.f2
+    g = api.YSpec(deck)
+    g.set_events(ev)
+    g.forward_events(1)
+    for e, p in enumerate(ev):
+        h = api.YSpec(deck, p)
+        spec, seis = h.spectra(), h.seismograms()
+        h.close()
+        for a, b in zip(g.event_spectra(e), spec):
+            assert rel_l2(a, b) <= 1e-12, (e, p.src_depth)
+        assert rel_l2(g.event_seismograms(e), seis) <= 1e-12, (e, p.src_depth)
+        o = oracle_mod.Oracle(deck, p)
+        os_ = o.spectra(nthreads=16)
+        for a, b in zip(g.event_spectra(e), os_):
+            assert rel_l2(a, b) <= 1e-9, (e, p.src_depth)
+        assert rel_l2(g.event_seismograms(e), o.seismograms(*os_)) <= 1e-8, (e, p.src_depth)
+    g.close()
