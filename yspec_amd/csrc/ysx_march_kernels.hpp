@@ -338,6 +338,26 @@ __device__ __forceinline__ void ysol_row(const cd* m, int k, cd* x) {
   x[13] = pick3(k, m[8], -m[7], m[6]);
 }
 
+// Rows k of pair (l, j) of event R from the marched row x (scale count sl):
+// y_c[k] = x . bst_c * fac * huge^(sc3 - sl), c = 0..3, into the record;
+// rows the march does not produce are zeroed.
+__device__ __forceinline__ void brow_finish(const RunK& R, int l, int j, int k, int rows,
+                                            const cd* x, int sl, int nsteps, const SphState& S,
+                                            cd* __restrict__ rec, int* __restrict__ cnt) {
+  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  cd fac = S.fac[p] * ipow_huge(S.sc3[p] - sl);
+  cd* o = rec + p * REC + O_YSPH;
+  for (int c = 0; c < 4; c++) {
+    cd v = C(0.0);
+#pragma unroll
+    for (int q = 0; q < 14; q++) v = cmad(v, x[q], S.bst[p * 56 + c * 14 + q]);
+    if (k < 3) o[6 * c + k] = v * fac;
+    if (k == 0)
+      for (int r = (rows < 3 ? rows : 3); r < 6; r++) o[6 * c + r] = C(0.0);
+  }
+  if (k == 0) cnt[5 * p + 3] = nsteps;
+}
+
 __device__ __forceinline__ void brow_item(const Tab& T, const RunK& R, int j, int lb,
                                           const SphState& S, cd* __restrict__ rec,
                                           int* __restrict__ cnt, double* sb, cd* slot) {
@@ -359,36 +379,7 @@ __device__ __forceinline__ void brow_item(const Tab& T, const RunK& R, int j, in
   int sl = 0;
   int n = march_yint_T(T, w, L, R.nknot, R.is, R.ir, R.rs, R.rr, x, sl, sb, BvecTM(), slot);
   if (!valid) return;
-  cd fac = S.fac[p] * ipow_huge(S.sc3[p] - sl);
-  cd* o = rec + p * REC + O_YSPH;
-  for (int c = 0; c < 4; c++) {
-    cd v = C(0.0);
-#pragma unroll
-    for (int q = 0; q < 14; q++) v = cmad(v, x[q], S.bst[p * 56 + c * 14 + q]);
-    if (k < 3) o[6 * c + k] = v * fac;
-    if (k == 0)
-      for (int r = (rows < 3 ? rows : 3); r < 6; r++) o[6 * c + r] = C(0.0);
-  }
-  if (k == 0) cnt[5 * p + 3] = n;
-}
-
-// Rows of the shared events' pair (l, j): y_c[k] = x . bst_c * fac *
-// huge^(sc3 - sl) written to event e's record, as brow_item does.
-__device__ __forceinline__ void brow_finish(const RunK& R, int l, int j, int k, int rows,
-                                            const cd* x, int sl, int nsteps, const SphState& S,
-                                            cd* __restrict__ rec, int* __restrict__ cnt) {
-  size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
-  cd fac = S.fac[p] * ipow_huge(S.sc3[p] - sl);
-  cd* o = rec + p * REC + O_YSPH;
-  for (int c = 0; c < 4; c++) {
-    cd v = C(0.0);
-#pragma unroll
-    for (int q = 0; q < 14; q++) v = cmad(v, x[q], S.bst[p * 56 + c * 14 + q]);
-    if (k < 3) o[6 * c + k] = v * fac;
-    if (k == 0)
-      for (int r = (rows < 3 ? rows : 3); r < 6; r++) o[6 * c + r] = C(0.0);
-  }
-  if (k == 0) cnt[5 * p + 3] = nsteps;
+  brow_finish(R, l, j, k, rows, x, sl, n, S, rec, cnt);
 }
 
 __device__ __forceinline__ bool lane_has(const RunK& R, int l, int j) {
