@@ -62,6 +62,15 @@ int ysx_model_table(const ysx_ctx *ctx, double *out);
 int ysx_model_table_host(int nknot, int nic, int noc, int ians,
                          double tref_period, const double *deck, double *out);
 
+/* Adjoint of the deck -> model-table transform (no counterpart in the
+ * reference; the first stage of a misfit gradient): deck_bar (nknot x 9,
+ * the deck's layout) = J^T table_bar for table = ysx_model_table_host(deck)
+ * and J its Jacobian with respect to the deck columns rho, vpv, vsv, qkappa,
+ * qmu, vph, vsh, eta.  Knot radii are not differentiated; at a zero shear
+ * modulus (fluid) beta = sqrt(mu / rho) contributes nothing.  Host only. */
+int ysx_model_table_vjp(int nknot, int nic, int noc, int ians, double tref_period,
+                        const double *deck, const double *table_bar, double *deck_bar);
+
 /* Set the run: ip = lmin lmax ats grav out cor nr; fp = f1 f2 tout dt f11
  * f12 f21 f22 src_depth src_lat src_lon Mrr Mrt Mrp Mtt Mtp Mpp rec_depth;
  * lat/lon of the nr receivers.  Receiver geometry, Legendre tables, fcal,

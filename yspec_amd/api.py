@@ -25,7 +25,7 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_set_events", "ysx_event_info", "ysx_forward_events", "ysx_set_batch_tuning",
            "ysx_set_records",
            "ysx_event_spectra", "ysx_event_seismograms", "ysx_model_table",
-           "ysx_model_table_host",
+           "ysx_model_table_host", "ysx_model_table_vjp",
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
@@ -70,6 +70,7 @@ def lib():
         L.ysx_event_seismograms.argtypes = [vp, C.c_int, D]
         L.ysx_model_table.argtypes = [vp, D]
         L.ysx_model_table_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D]
+        L.ysx_model_table_vjp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, D, D, D]
         L.ysx_pair_records.argtypes = [vp, C.c_int, I, I, D]
         L.ysx_pair_steps.argtypes = [vp, C.c_int, I, I, I]
         L.ysx_timings.argtypes = [vp, D]
@@ -99,6 +100,21 @@ def model_table_host(deck):
     out = np.zeros((len(TABLE_FIELDS), deck.nknot))
     rc = lib().ysx_model_table_host(deck.nknot, deck.nic, deck.noc, deck.ians, deck.tref,
                                     _dp(cols), _dp(out))
+    if rc:
+        raise YSpecError(rc, "bad deck")
+    return out
+
+
+def model_table_vjp(deck, table_bar):
+    """(nknot, 9) deck_bar = J^T table_bar for the deck -> table transform
+    (table_bar: (29, nknot), the layout of model_table_host)."""
+    cols = np.ascontiguousarray(deck.cols, dtype=np.float64)
+    tb = np.ascontiguousarray(table_bar, dtype=np.float64)
+    if tb.shape != (len(TABLE_FIELDS), deck.nknot):
+        raise ValueError("table_bar must be (29, nknot)")
+    out = np.zeros((deck.nknot, 9))
+    rc = lib().ysx_model_table_vjp(deck.nknot, deck.nic, deck.noc, deck.ians, deck.tref,
+                                   _dp(cols), _dp(tb), _dp(out))
     if rc:
         raise YSpecError(rc, "bad deck")
     return out

@@ -539,6 +539,17 @@ extern "C" int ysx_model_table_host(int nknot, int nic, int noc, int ians, doubl
   return YSX_OK;
 }
 
+extern "C" int ysx_model_table_vjp(int nknot, int nic, int noc, int ians, double tref_period,
+                                   const double* deck, const double* table_bar,
+                                   double* deck_bar) {
+  if (!deck || !table_bar || !deck_bar) return YSX_ERR_ARG;
+  if (!model_table_vjp(make_norms(), nknot, nic, noc, ians, tref_period, deck, table_bar,
+                       deck_bar)
+           .empty())
+    return YSX_ERR_INPUT;
+  return YSX_OK;
+}
+
 extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double* fp,
                               const double* lat, const double* lon) {
   if (!ctx || nev < 1 || !ip || !fp || !lat || !lon) return YSX_ERR_ARG;

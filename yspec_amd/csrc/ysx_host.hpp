@@ -40,6 +40,11 @@ Norms make_norms();
 // returns "" on success or the reference's stop message
 std::string build_model(const Norms& N, int nknot, int nic, int noc, int ians,
                         double tref_period, const double* deck, HostModel& M);
+// deck_bar = J^T table_bar for table = build_model(deck) (J: d table /
+// d deck); knot radii are not differentiated
+std::string model_table_vjp(const Norms& N, int nknot, int nic, int noc, int ians,
+                            double tref_period, const double* deck, const double* tbar,
+                            double* dbar);
 std::string build_run(const Norms& N, const HostModel& M, const int* ip,
                       const double* fp, const double* lat, const double* lon,
                       HostRun& R);
