@@ -54,7 +54,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the ODE kernels at this workload: rocprofv3
 # FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a 1 GiB device copy),
 # profiles/forward_cfg2_hbm_traffic.md
-TRAFFIC_BYTES = {"bvec": 1429.8e6, "minors": 1560.6e6, "tor": 200.8e6, "lsum": 480.6e6}
+TRAFFIC_BYTES = {"bvec": 1429.7e6, "minors": 1560.2e6, "tor": 200.8e6, "lsum": 483.2e6}
 PHASES = ("source", "tor", "minors", "bvec", "rad", "assemble", "lsum", "ifft")
 
 
