@@ -171,8 +171,9 @@ int ysx_timings(const ysx_ctx *ctx, double *ms8);
 
 /* Exact Cash-Karp step counts of the last ysx_spectra, summed over pairs:
  * [0] toroidal, [1] solid-region minors, [2] fluid-region minors,
- * [3] b-vector (one count per step; each step advances all 4 source
- * columns), [4] radial.
+ * [3] b-vector (one count per step of the reference's march; each step
+ * advances all 4 source columns there, and the receiver-side rows here take
+ * the same steps), [4] radial.
  * The step rule is the reference's fixed-step rule (yint,
  * src/module_int.f90:1903-1905), so these are the reference's counts too;
  * with per-step FLOP constants they give the algorithmic work. */

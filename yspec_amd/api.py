@@ -238,8 +238,9 @@ class YSpec:
 
     def step_counts(self):
         """exact Cash-Karp step totals of the last spectra (all events):
-        toroidal, solid minors, fluid minors, b-vector (steps of the four
-        columns together), radial"""
+        toroidal, solid minors, fluid minors, b-vector (one per step of the
+        march, which the four columns or the receiver-side rows take
+        together), radial"""
         out = np.zeros(5, np.int64)
         self._check(self.L.ysx_step_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
         return out
