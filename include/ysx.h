@@ -163,6 +163,44 @@ int ysx_pair_records(ysx_ctx *ctx, int n, const int *l, const int *j,
  * integration path. */
 int ysx_pair_steps(ysx_ctx *ctx, int n, const int *l, const int *j, int *out5);
 
+/* ---- adjoints of the linear back end (no counterpart in the reference;
+ * stages of a misfit gradient, one-event runs) ---------------------------
+ * For a real function F of the seismograms, a complex quantity z gets the
+ * adjoint dF/dRe z + i dF/dIm z.  Each entry point applies the exact
+ * transpose of the discrete map the forward applies (its twiddles, filter
+ * weights and factors), so <x_bar, J x> = <J^T x_bar, x> holds to rounding.
+ *
+ * Adjoint of ysx_seismograms: seis_bar ((nt, 3, nr) real, the layout of the
+ * seismograms) -> ur_bar, ut_bar, up_bar ((nw, nr) complex each, or NULL;
+ * the result also stays on the device for ysx_spectra_vjp).  Needs the run
+ * only, not a forward. */
+int ysx_seismograms_vjp(ysx_ctx *ctx, const double *seis_bar, double *ur_bar,
+                        double *ut_bar, double *up_bar);
+
+/* Adjoint of the receiver sum and the coefficient assembly of the last
+ * ysx_spectra: from the spectra adjoints (the three (nw, nr) arrays, or all
+ * NULL for the device-resident result of ysx_seismograms_vjp) to the
+ * adjoints of every pair's record, computed for all pairs on the device;
+ * rec_bar gets those of the n pairs (l[i], j[i]) (j 1-based) in the layout
+ * of ysx_pair_records: the adjoints of svt, ytor, svs, ysph (rows 0..2),
+ * sr0, yrad, and of uu, vv, ww in their slots. */
+int ysx_spectra_vjp(ysx_ctx *ctx, const double *ur_bar, const double *ut_bar,
+                    const double *up_bar, int n, const int *l, const int *j, double *rec_bar);
+
+/* Gradient with respect to the moment tensor (Mrr, Mrt, Mrp, Mtt, Mtp, Mpp,
+ * per N m as in the parameter file) from the record adjoints of the last
+ * ysx_spectra_vjp: the source vectors (source_vector_tor / _sph / _rad,
+ * src/module_solver.f90:145-215, 566-627, 1747-1847) are linear in the
+ * moment tensor, so mm_bar[i] = sum over pairs of <sv_bar, dsv/dM_i>.  After
+ * ysx_seismograms_vjp(seis_bar) and ysx_spectra_vjp it is the exact gradient
+ * of sum(seis_bar * seismograms). */
+int ysx_moment_tensor_vjp(ysx_ctx *ctx, double *mm_bar);
+
+/* Kernel milliseconds (HIP events on the context's stream, host copies
+ * excluded) of the last call of [0] ysx_seismograms_vjp, [1] ysx_spectra_vjp,
+ * [2] ysx_moment_tensor_vjp. */
+int ysx_vjp_timings(const ysx_ctx *ctx, double *ms3);
+
 /* Timings of the last ysx_spectra / ysx_seismograms, milliseconds, from
  * HIP events on the context's stream: [0] source vectors + start levels,
  * [1] toroidal, [2] spheroidal minors, [3] b-vector, [4] radial,

@@ -1,0 +1,203 @@
+"""Adjoints of the linear back end (ysx_seismograms_vjp, ysx_spectra_vjp)
+through the C-ABI.  The reference has no adjoint, so these are pinned by
+what the mathematics fixes:
+
+- the dot-product identity <x_bar, J x> = <J^T x_bar, x> with J the
+  forward's own discrete map (seismograms from the spectra, spectra from
+  the coefficients uu, vv, ww), x the forward's values and x_bar random;
+  complex pairs enter as sum Re(conj(a) b);
+- Euler's identity for the coefficient assembly, which is linear in the
+  source vectors for fixed solutions and linear in the solutions for fixed
+  source vectors: per pair, <c_bar, c> = <sv_bar, sv> = <y_bar, y>.
+
+A plausible mistake (a missing conjugate, a wrong sign or factor on one
+coefficient plane, a dropped Hermitian-fill term or filter weight) breaks
+one of them: every case has a general moment tensor, so no coefficient plane
+that reaches the spectra is zero."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from conftest import load_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["cfg1_small", "aniso_acc_cor", "cow_ocean_vel", "epicentre_ragged", "radial_only",
+         "one_degree", "prem_example", "no_filter", "receivers_37"]
+SRC = ("svt", "svs", "sr0")
+SOL = ("ytor", "ysph", "yrad")
+TOL = 1e-11
+
+
+@pytest.fixture(scope="module")
+def api():
+    from yspec_amd import api as a
+    a.lib()
+    return a
+
+
+def _case(name):
+    if name == "no_filter":
+        deck, p, _ = load_case("prem_noocean_disp")
+        return deck, dataclasses.replace(p, f11=0.0, f12=0.0, f21=0.0, f22=0.0)
+    if name == "receivers_37":
+        deck, p, _ = load_case("cfg1_small")
+        rng = np.random.default_rng(37)
+        lat = rng.uniform(-80.0, 80.0, 37)
+        lon = rng.uniform(-180.0, 180.0, 37)
+        return deck, dataclasses.replace(p, lat=lat, lon=lon)
+    deck, p, _ = load_case(name)
+    return deck, p
+
+
+def _forward(api, name):
+    deck, p = _case(name)
+    g = api.YSpec(deck)
+    g.set_run(p)
+    spec = g.spectra()
+    seis = g.seismograms()
+    nl = p.lmax - p.lmin + 1
+    ls = np.repeat(np.arange(p.lmin, p.lmax + 1), g.nw)
+    js = np.tile(np.arange(1, g.nw + 1), nl)
+    return g, p, spec, seis, ls, js
+
+
+def _rdot(a, b):
+    return float(np.sum((np.conj(a) * b).real))
+
+
+def _coefs(rec):
+    return np.concatenate([rec["uu"], rec["vv"], rec["ww"]], axis=1)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_seismograms_vjp_dot_product(name, api):
+    g, p, spec, seis, _, _ = _forward(api, name)
+    rng = np.random.default_rng(1)
+    seis_bar = rng.standard_normal(seis.shape)
+    spec_bar = g.seismograms_vjp(seis_bar)
+    lhs = float(np.sum(seis_bar * seis))
+    rhs = sum(_rdot(b, s) for b, s in zip(spec_bar, spec))
+    scale = np.linalg.norm(seis_bar) * np.linalg.norm(seis)
+    print(name, f"<sb, J s> = {lhs:.15e}  <J^T sb, s> = {rhs:.15e}  "
+          f"rel {abs(lhs - rhs) / scale:.2e}", flush=True)
+    assert abs(lhs - rhs) <= TOL * scale
+    # bins past nt/2 never reach the series
+    for b in spec_bar:
+        assert np.all(b[np.arange(g.nw) + g.i1 - 1 > g.nt // 2] == 0)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_spectra_vjp_dot_product(name, api):
+    g, p, spec, _, ls, js = _forward(api, name)
+    rng = np.random.default_rng(2)
+    spec_bar = [rng.standard_normal(s.shape) + 1j * rng.standard_normal(s.shape) for s in spec]
+    rb = g.spectra_vjp(ls, js, spec_bar)
+    c = _coefs(g.pair_records(ls, js))
+    cb = _coefs(rb)
+    lhs = sum(_rdot(b, s) for b, s in zip(spec_bar, spec))
+    rhs = _rdot(cb, c)
+    scale = max(sum(np.linalg.norm(b) * np.linalg.norm(s) for b, s in zip(spec_bar, spec)),
+                float(np.sum(np.abs(cb) * np.abs(c))))
+    print(name, f"<sb, L c> = {lhs:.15e}  <L^T sb, c> = {rhs:.15e}  "
+          f"rel {abs(lhs - rhs) / scale:.2e}", flush=True)
+    assert abs(lhs - rhs) <= TOL * scale
+    # ww0 never reaches the spectra
+    assert np.all(rb["ww"][:, 2] == 0)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_assembly_vjp_euler(name, api):
+    g, p, spec, seis, ls, js = _forward(api, name)
+    rng = np.random.default_rng(3)
+    g.seismograms_vjp(rng.standard_normal(seis.shape))
+    rb = g.spectra_vjp(ls, js)  # from the device-resident spectra adjoint
+    rec = g.pair_records(ls, js)
+    n = len(ls)
+
+    def per_pair(keys, a, b):
+        return sum(np.sum((np.conj(a[k].reshape(n, -1)) * b[k].reshape(n, -1)).real, axis=1)
+                   for k in keys)
+
+    cc = np.sum((np.conj(_coefs(rb)) * _coefs(rec)).real, axis=1)
+    scale = np.sum(np.abs(_coefs(rb)) * np.abs(_coefs(rec)), axis=1) + 1e-300
+    es = np.abs(per_pair(SRC, rb, rec) - cc) / scale
+    ey = np.abs(per_pair(SOL, rb, rec) - cc) / scale
+    print(name, f"pairs {n}  source side {es.max():.2e}  solution side {ey.max():.2e}",
+          flush=True)
+    assert es.max() <= 1e-12 and ey.max() <= 1e-12
+    # rows 3..5 of ysph and source-vector columns 3 and 6 do not enter
+    assert np.all(rb["ysph"][:, 3:, :] == 0)
+    assert np.all(rb["svs"][:, :, [2, 5]] == 0)
+
+
+def test_chain_dot_product(api):
+    """seismograms_vjp followed by spectra_vjp on the device: <sb, seis> =
+    sum over pairs of <c_bar, c>"""
+    g, p, spec, seis, ls, js = _forward(api, "cfg1_small")
+    rng = np.random.default_rng(4)
+    sb = rng.standard_normal(seis.shape)
+    g.seismograms_vjp(sb)
+    cb = _coefs(g.spectra_vjp(ls, js))
+    c = _coefs(g.pair_records(ls, js))
+    lhs = float(np.sum(sb * seis))
+    rhs = _rdot(cb, c)
+    scale = max(np.linalg.norm(sb) * np.linalg.norm(seis), float(np.sum(np.abs(cb) * np.abs(c))))
+    assert abs(lhs - rhs) <= TOL * scale
+
+
+def test_vjp_errors(api):
+    deck, p = _case("cfg1_small")
+    g = api.YSpec(deck)
+    g.set_run(p)
+    with pytest.raises(api.YSpecError):
+        g.spectra_vjp([p.lmin], [1])  # no forward yet
+    g.spectra()
+    with pytest.raises(api.YSpecError):
+        g.spectra_vjp([p.lmin], [1])  # no spectra adjoint on the device yet
+    with pytest.raises(api.YSpecError):
+        g.spectra_vjp([p.lmax + 1], [1], [np.zeros((g.nw, p.nr), complex)] * 3)
+    with pytest.raises(ValueError):
+        g.seismograms_vjp(np.zeros((g.nt + 1, 3, p.nr)))
+
+
+@pytest.mark.parametrize("name", ["cfg1_small", "aniso_acc_cor", "prem_example", "radial_only",
+                                  "epicentre_ragged"])
+def test_moment_tensor_gradient_vs_forwards(name, api):
+    """The seismograms are linear in the moment tensor, so the gradient of
+    sum(seis_bar * seis) is <seis_bar, seis(e_i)> / M0 with seis(e_i) the
+    forward for the i-th unit tensor of size M0: six independent forwards
+    pin the whole chain (seismogram, receiver-sum, assembly and source
+    adjoints)."""
+    deck, p = _case(name)
+    g = api.YSpec(deck)
+    g.set_run(p)
+    seis = g.forward()
+    rng = np.random.default_rng(5)
+    sb = rng.standard_normal(seis.shape)
+    grad = g.misfit_gradient_mt(sb)
+    m0 = 1e19
+    want = np.zeros(6)
+    for i in range(6):
+        mm = np.zeros(6)
+        mm[i] = m0
+        gi = api.YSpec(deck)
+        gi.set_run(dataclasses.replace(p, mm=mm))
+        want[i] = float(np.sum(sb * gi.forward())) / m0
+    # linearity of the forward itself: seis = sum_i M_i seis(e_i)
+    err = np.abs(grad - want) / np.linalg.norm(want)
+    print(name, "gradient", grad, "forwards", want, f"max rel {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-10
+    # <seis_bar, seis> = sum_i M_i dF/dM_i
+    assert abs(float(np.sum(sb * seis)) - float(np.dot(p.mm, grad))) <= 1e-10 * (
+        np.linalg.norm(sb) * np.linalg.norm(seis))
+
+
+def test_moment_tensor_vjp_needs_record_adjoints(api):
+    deck, p = _case("cfg1_small")
+    g = api.YSpec(deck)
+    g.set_run(p)
+    g.forward()
+    with pytest.raises(api.YSpecError):
+        g.moment_tensor_vjp()

@@ -26,11 +26,23 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_set_records",
            "ysx_event_spectra", "ysx_event_seismograms", "ysx_model_table",
            "ysx_model_table_host", "ysx_model_table_vjp",
-           "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts")
+           "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts",
+           "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
+           "ysx_vjp_timings")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
                ("uu", (5,)), ("vv", (5,)), ("ww", (5,)))
+
+
+def _split_records(rec):
+    """(n, 89) complex records -> dict of the fields of PAIR_FIELDS"""
+    out, n = {}, 0
+    for name, shape in PAIR_FIELDS:
+        cnt = int(np.prod(shape))
+        out[name] = rec[:, n:n + cnt].reshape((len(rec),) + shape, order="F")
+        n += cnt
+    return out
 
 
 class YSpecError(RuntimeError):
@@ -75,6 +87,10 @@ def lib():
         L.ysx_pair_steps.argtypes = [vp, C.c_int, I, I, I]
         L.ysx_timings.argtypes = [vp, D]
         L.ysx_step_counts.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.ysx_seismograms_vjp.argtypes = [vp, D, D, D, D]
+        L.ysx_spectra_vjp.argtypes = [vp, D, D, D, C.c_int, I, I, D]
+        L.ysx_moment_tensor_vjp.argtypes = [vp, D]
+        L.ysx_vjp_timings.argtypes = [vp, D]
         _LIB = L
     return _LIB
 
@@ -217,12 +233,58 @@ class YSpec:
         rec = np.zeros((len(ls), 89), np.complex128)
         self._check(self.L.ysx_pair_records(self.h, len(ls), _ip(ls), _ip(js),
                                             rec.ctypes.data_as(D)))
-        out, n = {}, 0
-        for name, shape in PAIR_FIELDS:
-            cnt = int(np.prod(shape))
-            out[name] = rec[:, n:n + cnt].reshape((len(ls),) + shape, order="F")
-            n += cnt
+        return _split_records(rec)
+
+    def seismograms_vjp(self, seis_bar):
+        """Adjoint of seismograms(): (nt, 3, nr) real seis_bar -> ur_bar,
+        ut_bar, up_bar, (nw, nr) complex each (dF/dRe + i dF/dIm)."""
+        sb = np.asfortranarray(seis_bar, dtype=np.float64)
+        if sb.shape != (self.nt, 3, self.p.nr):
+            raise ValueError("seis_bar must be (nt, 3, nr)")
+        shp = (self.nw, self.p.nr)
+        out = [np.zeros(shp, np.complex128, order="F") for _ in range(3)]
+        self._check(self.L.ysx_seismograms_vjp(self.h, _dp(sb),
+                                               *[a.ctypes.data_as(D) for a in out]))
+        return tuple(out)
+
+    def spectra_vjp(self, ls, js, spec_bar=None):
+        """Adjoints of the records of pairs (ls, js) (the fields of
+        pair_records) from the spectra adjoints spec_bar = (ur_bar, ut_bar,
+        up_bar), or from the device-resident result of seismograms_vjp."""
+        ls = np.ascontiguousarray(ls, dtype=np.int32)
+        js = np.ascontiguousarray(js, dtype=np.int32)
+        rec = np.zeros((len(ls), 89), np.complex128)
+        if spec_bar is None:
+            ins = [None, None, None]
+        else:
+            shp = (self.nw, self.p.nr)
+            arrs = [np.asfortranarray(a, dtype=np.complex128) for a in spec_bar]
+            if any(a.shape != shp for a in arrs):
+                raise ValueError("spec_bar: three (nw, nr) arrays")
+            ins = [a.ctypes.data_as(D) for a in arrs]
+        self._check(self.L.ysx_spectra_vjp(self.h, *ins, len(ls), _ip(ls), _ip(js),
+                                           rec.ctypes.data_as(D)))
+        return _split_records(rec)
+
+    def moment_tensor_vjp(self):
+        """(6,) gradient with respect to Mrr, Mrt, Mrp, Mtt, Mtp, Mpp (per N m)
+        from the record adjoints of the last spectra_vjp."""
+        out = np.zeros(6)
+        self._check(self.L.ysx_moment_tensor_vjp(self.h, _dp(out)))
         return out
+
+    def vjp_timings(self):
+        """kernel ms of the last seismograms_vjp, spectra_vjp, moment_tensor_vjp"""
+        out = np.zeros(3)
+        self._check(self.L.ysx_vjp_timings(self.h, _dp(out)))
+        return out
+
+    def misfit_gradient_mt(self, seis_bar):
+        """d/dM of sum(seis_bar * seismograms) after a forward: the three
+        adjoint stages on the device, no per-pair copies to the host."""
+        self.seismograms_vjp(seis_bar)
+        self.spectra_vjp(np.zeros(0, np.int32), np.zeros(0, np.int32))
+        return self.moment_tensor_vjp()
 
     def model_table(self):
         """(29, nknot) model table the kernels read (built on the device)."""

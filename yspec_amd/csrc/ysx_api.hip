@@ -93,6 +93,14 @@ struct ysx_ctx {
   cd* d_work = nullptr;
   double* d_seis = nullptr;
   int* d_queue = nullptr;  // Q_N counters of the batched queues
+  // adjoints (one-event runs): seismograms, spectra, coefficient planes and
+  // per-pair records, allocated by the first ysx_seismograms_vjp or
+  // ysx_spectra_vjp of a run
+  double* d_seisb = nullptr;
+  cd *d_specb = nullptr, *d_cfb = nullptr, *d_recb = nullptr;
+  double* d_mmpart = nullptr;  // per-block sums of the moment-tensor gradient
+  double ms_vjp[3] = {0};      // kernel ms of the last call of each adjoint entry point
+  bool have_specb = false, have_recb = false;
   WorkList W[W_N];
   // events whose b-vector rows share one march in the batched path (same
   // receiver depth and frequency grid), by source radius, shallowest first
@@ -146,6 +154,8 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_sph_m0); dfree(c->d_fac); dfree(c->d_sc3); dfree(c->d_azep); dfree(c->d_eia);
   dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
+  dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
+  c->have_specb = c->have_recb = false;
   c->sh_order.clear();
   c->n_union = 0;
   for (auto& w : c->W) {
@@ -1125,6 +1135,125 @@ extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j,
     HIPCHK(hipMemcpy(rec + (size_t)i * 2 * REC, ctx->d_rec + p * REC, REC * sizeof(cd),
                      hipMemcpyDeviceToHost));
   }
+  return YSX_OK;
+}
+
+namespace {
+
+int alloc_adjoint(ysx_ctx* ctx) {
+  const HostRun& R = ctx->R[0];
+  const RunK& k = ctx->rk[0];
+  if (!ctx->d_seisb) HIPCHK(hipMalloc(&ctx->d_seisb, 3 * (size_t)R.nt * R.nr * sizeof(double)));
+  if (!ctx->d_specb) HIPCHK(hipMalloc(&ctx->d_specb, 3 * (size_t)R.nw * R.nr * sizeof(cd)));
+  if (!ctx->d_cfb) HIPCHK(hipMalloc(&ctx->d_cfb, 15 * (size_t)k.lpad * k.jpad * sizeof(cd)));
+  if (!ctx->d_recb) HIPCHK(hipMalloc(&ctx->d_recb, (size_t)k.npair * REC * sizeof(cd)));
+  if (!ctx->d_mmpart)
+    HIPCHK(hipMalloc(&ctx->d_mmpart, 6 * (size_t)((k.npair + 255) / 256) * sizeof(double)));
+  return YSX_OK;
+}
+
+int one_event_run(ysx_ctx* ctx) {
+  if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
+  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: one-event runs only");
+  HIPCHK(hipSetDevice(ctx->device));
+  return alloc_adjoint(ctx);
+}
+
+}  // namespace
+
+extern "C" int ysx_seismograms_vjp(ysx_ctx* ctx, const double* seis_bar, double* ur_bar,
+                                   double* ut_bar, double* up_bar) {
+  if (!ctx || !seis_bar) return YSX_ERR_ARG;
+  if (int rc = one_event_run(ctx)) return rc;
+  const HostRun& R = ctx->R[0];
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipMemcpy(ctx->d_seisb, seis_bar, 3 * (size_t)R.nt * R.nr * sizeof(double),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  launch_seis_T(dim3(std::max(1, R.nr)), dim3(1024), 0, st, ctx->d_runs, 0, ctx->d_seisb,
+                ctx->d_tw, ctx->d_work, ctx->d_azep, ctx->d_specb);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[1], st));
+  HIPCHK(hipEventSynchronize(ctx->ev[1]));
+  if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[0])) return rc;
+  ctx->have_specb = true;
+  size_t n = (size_t)R.nw * R.nr;
+  double* outs[3] = {ur_bar, ut_bar, up_bar};
+  for (int c = 0; c < 3; c++)
+    if (outs[c])
+      HIPCHK(hipMemcpy(outs[c], ctx->d_specb + c * n, n * sizeof(cd), hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
+extern "C" int ysx_spectra_vjp(ysx_ctx* ctx, const double* ur_bar, const double* ut_bar,
+                               const double* up_bar, int n, const int* l, const int* j,
+                               double* rec_bar) {
+  if (!ctx || n < 0 || (n > 0 && (!l || !j || !rec_bar))) return YSX_ERR_ARG;
+  const bool given = ur_bar || ut_bar || up_bar;
+  if (given && !(ur_bar && ut_bar && up_bar))
+    return set_err(ctx, YSX_ERR_ARG, "ur_bar, ut_bar, up_bar: give all three or none");
+  if (int rc = one_event_run(ctx)) return rc;
+  if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
+  const HostRun& R = ctx->R[0];
+  const RunK& k = ctx->rk[0];
+  for (int i = 0; i < n; i++)
+    if (l[i] < k.lmin || l[i] > k.lmax || j[i] < 1 || j[i] > k.nw)
+      return set_err(ctx, YSX_ERR_ARG, "pair outside the run");
+  size_t ns = (size_t)R.nw * R.nr;
+  if (given) {
+    const double* ins[3] = {ur_bar, ut_bar, up_bar};
+    for (int c = 0; c < 3; c++)
+      HIPCHK(hipMemcpy(ctx->d_specb + c * ns, ins[c], ns * sizeof(cd), hipMemcpyHostToDevice));
+    ctx->have_specb = true;
+  } else if (!ctx->have_specb) {
+    return set_err(ctx, YSX_ERR_ARG, "no spectra adjoint: call ysx_seismograms_vjp first");
+  }
+  hipStream_t st = ctx->stream;
+  int ntile = (k.lpad + 15) / 16 * (k.jpad / 16);
+  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  launch_lsum_T(dim3(ntile), dim3(256), 0, st, ctx->d_runs, 0, ctx->d_tb, ctx->d_eia,
+                ctx->d_specb, ctx->d_cfb);
+  launch_assemble_T(dim3((unsigned)((k.npair + 63) / 64)), dim3(64), 0, st, ctx->d_runs, 0,
+                    ctx->d_rec, ctx->d_cfb, ctx->d_recb);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[1], st));
+  HIPCHK(hipEventSynchronize(ctx->ev[1]));
+  if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[1])) return rc;
+  ctx->have_recb = true;
+  for (int i = 0; i < n; i++) {
+    size_t p = (size_t)(l[i] - k.lmin) * k.nw + (j[i] - 1);
+    HIPCHK(hipMemcpy(rec_bar + (size_t)i * 2 * REC, ctx->d_recb + p * REC, REC * sizeof(cd),
+                     hipMemcpyDeviceToHost));
+  }
+  return YSX_OK;
+}
+
+extern "C" int ysx_moment_tensor_vjp(ysx_ctx* ctx, double* mm_bar) {
+  if (!ctx || !mm_bar) return YSX_ERR_ARG;
+  if (int rc = one_event_run(ctx)) return rc;
+  if (!ctx->have_recb) return set_err(ctx, YSX_ERR_ARG, "no record adjoints: call ysx_spectra_vjp first");
+  const RunK& k = ctx->rk[0];
+  int nb = (int)((k.npair + 255) / 256);
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  launch_source_T(dim3(nb), dim3(256), 0, ctx->stream, ctx->d_tab, ctx->d_runs, 0, ctx->d_recb,
+                  ctx->d_mmpart);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPCHK(hipEventSynchronize(ctx->ev[1]));
+  if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[2])) return rc;
+  std::vector<double> part(6 * (size_t)nb);
+  HIPCHK(hipMemcpy(part.data(), ctx->d_mmpart, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 6; i++) {
+    double a = 0.0;
+    for (int b = 0; b < nb; b++) a += part[6 * (size_t)b + i];
+    mm_bar[i] = a / ctx->N.moment_norm;  // normalised -> parameter-file units
+  }
+  return YSX_OK;
+}
+
+extern "C" int ysx_vjp_timings(const ysx_ctx* ctx, double* ms3) {
+  if (!ctx || !ms3) return YSX_ERR_ARG;
+  std::memcpy(ms3, ctx->ms_vjp, sizeof ctx->ms_vjp);
   return YSX_OK;
 }
 

@@ -17,6 +17,48 @@ __global__ void k_reset(int* counters, int n) {
 // source_vector_tor / _sph / _rad, src/module_solver.f90:145-215,
 // 566-627, 1747-1847.  source_vector_sph keeps only the real part of its
 // dispersion factors (they are declared real(dp) there, :1771-1772).
+// The source vectors of one pair for moment tensor mm (normalised units),
+// linear in mm: put(f, v) receives the value v of record field f (svt,
+// svs, sr0 entries; the others stay zero).
+template <class Put>
+__device__ __forceinline__ void source_vectors(const Tab& T, const RunK& R, const PS& s, int l,
+                                               const double* mm, Put put) {
+  SplW w = spl_w(T, R.is, R.rs);
+  double rlcon = spl(T, w, F_LCON), rccon = spl(T, w, F_CCON), rfcon = spl(T, w, F_FCON),
+         rqmu = spl(T, w, F_QMU), rxlam = spl(T, w, F_XLAM), rxa2 = spl(T, w, F_XA2);
+  double rs = R.rs;
+  cd den = cinv(cmuli(s.omega) * (rs * rs));  // 1/(i w rs^2)
+  cd nd = s.nu * den;
+  if (l == 0) {
+    cd qaa = 1.0 + rxa2 * s.lno, qff = 1.0 + rxlam * s.lno;
+    cd cc = cinv(rccon * qaa);
+    cd cf = rfcon * qff;
+    put(O_SR0 + 0, nd * ((rs * mm[0]) * cc));
+    put(O_SR0 + 1, nd * ((2.0 * mm[0]) * (cf * cc) - (mm[3] + mm[5])));
+    return;
+  }
+  if (l > 1) {
+    cd cl = rlcon * (1.0 + rqmu * s.lno);
+    cd icl = cinv(cl);
+    put(O_SVT + 1, nd * ((0.5 * rs) * C(-mm[2], mm[1]) * icl));
+    put(O_SVT + 3, nd * ((0.5 * rs) * C(mm[2], mm[1]) * icl));
+    put(O_SVT + 5 + 0, nd * ((0.25 * s.szeta2m2) * C(2.0 * mm[4], mm[5] - mm[3])));
+    put(O_SVT + 5 + 4, nd * ((0.25 * s.szeta2m2) * C(2.0 * mm[4], mm[3] - mm[5])));
+  }
+  double qaa = 1.0 + rxa2 * s.lno.x, qff = 1.0 + rxlam * s.lno.x, qll = 1.0 + rqmu * s.lno.x;
+  double cc = 1.0 / (rccon * qaa), cf = rfcon * qff, cl = rlcon * qll;
+  double zeta = s.zeta;
+  put(O_SVS + 1 + 5 * 1, nd * (C(0.5 * rs * mm[1] / cl, 0.5 * rs * mm[2] / cl)));
+  put(O_SVS + 2 + 5 * 0, nd * C(rs * mm[0] * cc));
+  put(O_SVS + 2 + 5 * 3, nd * C(-mm[3] - mm[5] + 2.0 * mm[0] * cf * cc));
+  put(O_SVS + 2 + 5 * 4, nd * C(0.5 * zeta * (mm[3] + mm[5]) - zeta * mm[0] * cf * cc));
+  put(O_SVS + 3 + 5 * 1, nd * (C(-0.5 * rs * mm[1] / cl, 0.5 * rs * mm[2] / cl)));
+  if (l > 1) {
+    put(O_SVS + 0 + 5 * 4, nd * ((0.25 * s.szeta2m2) * C(mm[5] - mm[3], -2.0 * mm[4])));
+    put(O_SVS + 4 + 5 * 4, nd * ((0.25 * s.szeta2m2) * C(mm[5] - mm[3], 2.0 * mm[4])));
+  }
+}
+
 __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
                                                 const RunK* __restrict__ runs, PairRange PR,
                                                 cd* __restrict__ rec, int* __restrict__ cnt) {
@@ -33,42 +75,7 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
   // records and step counters are zeroed once when the run is set; every
   // forward writes the same entries of them again
   cd* o = rec + (size_t)g * REC;
-  SplW w = spl_w(T, R.is, R.rs);
-  double rlcon = spl(T, w, F_LCON), rccon = spl(T, w, F_CCON), rfcon = spl(T, w, F_FCON),
-         rqmu = spl(T, w, F_QMU), rxlam = spl(T, w, F_XLAM), rxa2 = spl(T, w, F_XA2);
-  const double* mm = R.mm;
-  double rs = R.rs;
-  cd den = cinv(cmuli(s.omega) * (rs * rs));  // 1/(i w rs^2)
-  cd nd = s.nu * den;
-  if (l == 0) {
-    cd qaa = 1.0 + rxa2 * s.lno, qff = 1.0 + rxlam * s.lno;
-    cd cc = cinv(rccon * qaa);
-    cd cf = rfcon * qff;
-    o[O_SR0 + 0] = nd * ((rs * mm[0]) * cc);
-    o[O_SR0 + 1] = nd * ((2.0 * mm[0]) * (cf * cc) - (mm[3] + mm[5]));
-    return;
-  }
-  if (l > 1) {
-    cd cl = rlcon * (1.0 + rqmu * s.lno);
-    cd icl = cinv(cl);
-    o[O_SVT + 1] = nd * ((0.5 * rs) * C(-mm[2], mm[1]) * icl);
-    o[O_SVT + 3] = nd * ((0.5 * rs) * C(mm[2], mm[1]) * icl);
-    o[O_SVT + 5 + 0] = nd * ((0.25 * s.szeta2m2) * C(2.0 * mm[4], mm[5] - mm[3]));
-    o[O_SVT + 5 + 4] = nd * ((0.25 * s.szeta2m2) * C(2.0 * mm[4], mm[3] - mm[5]));
-  }
-  double qaa = 1.0 + rxa2 * s.lno.x, qff = 1.0 + rxlam * s.lno.x, qll = 1.0 + rqmu * s.lno.x;
-  double cc = 1.0 / (rccon * qaa), cf = rfcon * qff, cl = rlcon * qll;
-  double zeta = s.zeta;
-  cd* sv = o + O_SVS;  // svs(5,6) column-major
-  sv[1 + 5 * 1] = nd * (C(0.5 * rs * mm[1] / cl, 0.5 * rs * mm[2] / cl));
-  sv[2 + 5 * 0] = nd * C(rs * mm[0] * cc);
-  sv[2 + 5 * 3] = nd * C(-mm[3] - mm[5] + 2.0 * mm[0] * cf * cc);
-  sv[2 + 5 * 4] = nd * C(0.5 * zeta * (mm[3] + mm[5]) - zeta * mm[0] * cf * cc);
-  sv[3 + 5 * 1] = nd * (C(-0.5 * rs * mm[1] / cl, 0.5 * rs * mm[2] / cl));
-  if (l > 1) {
-    sv[0 + 5 * 4] = nd * ((0.25 * s.szeta2m2) * C(mm[5] - mm[3], -2.0 * mm[4]));
-    sv[4 + 5 * 4] = nd * ((0.25 * s.szeta2m2) * C(mm[5] - mm[3], 2.0 * mm[4]));
-  }
+  source_vectors(T, R, s, l, R.mm, [&](int f, cd v) { o[f] = v; });
 }
 
 // ----------------------------------------------------------------- k_start
@@ -698,12 +705,8 @@ __global__ void __launch_bounds__(256) k_model(const double* __restrict__ deck, 
 // receiver per block: pack bins i1..i2, cosine filters, Hermitian fill,
 // radix-2 DIT inverse FFT with the reference's twiddle angles
 // pi_d*m/mmax (single-rounded pi), /(dt*nt), exp(eps t), units, rotation.
-__device__ inline cd packed_bin(const RunK& K, const cd* spec, int k, int i) {
-  // 0-based FFT bin i of the packed, filtered, Hermitian-filled row
-  bool neg = i > K.nt / 2;
-  int b = neg ? K.nt - i : i;
-  cd v = C(0.0);
-  if (b >= K.i1 - 1 && b <= K.i2 - 1) v = spec[(size_t)(b - (K.i1 - 1)) + (size_t)K.nw * k];
+// cosine filter weight of frequency bin b (0-based)
+__device__ inline double bin_filter(const RunK& K, int b) {
   double w = b * K.df, f = 1.0;
   bool nof = (K.f11 == 0.0 && K.f12 == 0.0 && K.f21 == 0.0 && K.f22 == 0.0);
   if (!nof) {
@@ -713,7 +716,16 @@ __device__ inline cd packed_bin(const RunK& K, const cd* spec, int k, int i) {
     else if (w < K.f22) f = 0.5 * (1.0 - cos(PI_D * (K.f22 - w) / (K.f22 - K.f21)));
     else f = 0.0;
   }
-  v = v * f;
+  return f;
+}
+
+__device__ inline cd packed_bin(const RunK& K, const cd* spec, int k, int i) {
+  // 0-based FFT bin i of the packed, filtered, Hermitian-filled row
+  bool neg = i > K.nt / 2;
+  int b = neg ? K.nt - i : i;
+  cd v = C(0.0);
+  if (b >= K.i1 - 1 && b <= K.i2 - 1) v = spec[(size_t)(b - (K.i1 - 1)) + (size_t)K.nw * k];
+  v = v * bin_filter(K, b);
   if (neg) v.y = -v.y;
   return v;
 }
@@ -893,6 +905,351 @@ __global__ void k_twiddles(int nt, cd* tw) {
   tw[t] = C(cs, sn);
 }
 
+// ================================================================ adjoints
+// Vector-Jacobian products of the linear back end of the forward, for a
+// real function of the seismograms: a complex quantity z gets the adjoint
+// dF/dRe z + i dF/dIm z, so a complex-linear map y = a z has z_bar =
+// conj(a) y_bar and y = Re z has z_bar = y_bar.  Each kernel is the exact
+// transpose of the discrete map its forward kernel applies (same twiddles,
+// filter weights and factors), not of its continuous counterpart.
+
+__device__ inline cd conj_(cd a) { return C(a.x, -a.y); }
+
+// ---------------------------------------------------------------- k_seis_T
+// Adjoint of k_seis / k_fft_lds + k_fft_post for one event, one receiver k
+// per block: rotation, units and exp(eps t) transposed; the radix-2
+// butterfly network run backwards with each butterfly (a, b) -> (a + w b,
+// a - w b) replaced by its conjugate transpose (x, y) -> (x + y,
+// conj(w) (x - y)); then the bit-reversal, the Hermitian fill (bin b < nt/2
+// also collects the conjugate of bin nt - b) and the filter weights give
+// spec_bar of bins i1..i2.  Work rows as in k_seis.
+__global__ void __launch_bounds__(1024) k_seis_T(const RunK* __restrict__ runs, int e,
+                                                 const double* __restrict__ seisb_all,
+                                                 const cd* __restrict__ tw,
+                                                 cd* __restrict__ work_all,
+                                                 const double* __restrict__ azep_all,
+                                                 cd* __restrict__ specb_all) {
+  const RunK& K = runs[e];
+  const int nt = K.nt;
+  const double* seisb = seisb_all + K.seis_off;
+  cd* work = work_all + K.seis_off;
+  cd* specb = specb_all + K.spec_off;
+  for (int k = blockIdx.x; k < K.nr; k += gridDim.x) {
+    double az = azep_all[K.roff + k];
+    double ca = cos(az), sa = sin(az);
+    cd* r0 = work + ((size_t)k * 3 + 0) * nt;
+    cd* r1 = work + ((size_t)k * 3 + 1) * nt;
+    cd* r2 = work + ((size_t)k * 3 + 2) * nt;
+    double sc = K.norm / (K.dt * nt);
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+      double g = exp(K.ep * (i * K.dt)) * sc;
+      double z = seisb[(size_t)i + (size_t)nt * (0 + 3 * (size_t)k)];
+      double n = seisb[(size_t)i + (size_t)nt * (1 + 3 * (size_t)k)];
+      double ea = seisb[(size_t)i + (size_t)nt * (2 + 3 * (size_t)k)];
+      r0[i] = C(z * g);
+      r1[i] = C((-ca * n + sa * ea) * g);
+      r2[i] = C((sa * n + ca * ea) * g);
+    }
+    __syncthreads();
+    for (int c = 0; c < 3; c++) {
+      cd* row = work + ((size_t)k * 3 + c) * nt;
+      for (int mmax = nt >> 1; mmax >= 1; mmax >>= 1) {
+        for (int b = threadIdx.x; b < nt / 2; b += blockDim.x) {
+          int m = b & (mmax - 1);
+          int i = ((b - m) << 1) + m;
+          int jx = i + mmax;
+          cd x = row[i], y = row[jx];
+          row[i] = x + y;
+          row[jx] = conj_(tw[mmax + m]) * (x - y);
+        }
+        __syncthreads();
+      }
+      cd* sb = specb + (size_t)c * K.nw * K.nr + (size_t)K.nw * k;
+      for (int jj = threadIdx.x; jj < K.nw; jj += blockDim.x) {
+        int b = jj + K.i1 - 1;
+        cd v = C(0.0);
+        if (b <= nt / 2) {
+          v = row[brev((unsigned)b, K.lognt)];
+          if (b > 0 && b < nt / 2) v = v + conj_(row[brev((unsigned)(nt - b), K.lognt)]);
+          v = v * bin_filter(K, b);
+        }
+        sb[jj] = v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// --------------------------------------------------------------- k_lsum_T
+// Adjoint of k_lsum for one event: cf_bar[q](l, j) = sum over receivers k
+// of X(l, k) conj(F(k)) spec_bar(k, j) for every Legendre table X and
+// epilogue factor F through which coefficient plane q reaches the spectra,
+// as FP64 MFMA GEMMs over the receivers.  A block owns a 16 (l) x 16 (w)
+// tile; each of its four waves owns a fixed set of planes and runs the full
+// receiver loop.  Operands of v_mfma_f64_16x16x4f64: A[i][k] = X(l0 + i,
+// k0 + k) at lane i + 16k, B[k][n] = W(k0 + k, j0 + n) at lane n + 16k,
+// result D[i][n] at lane n + 16 (i % 4), register i / 4.
+
+// s * i^im * (cj ? conj(e^p) : e^p), e = exp(i phi_k)
+template <int P, bool CJ, int S, bool IM>
+__device__ __forceinline__ cd efac(cd e1, cd e2) {
+  cd v = (P == 0) ? C(1.0) : (P == 1 ? e1 : e2);
+  if (CJ) v.y = -v.y;
+  v = (double)S * v;
+  return IM ? cmuli(v) : v;
+}
+
+// acc += X_T x (factor * out_bar[O])
+template <int T, int O, int P, bool CJ, int S, bool IM>
+__device__ __forceinline__ void lsumT_term(d4* acc, const double* a, const cd* ob, cd e1, cd e2) {
+  cd w = efac<P, CJ, S, IM>(e1, e2) * ob[O];
+  acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[T], w.x, acc[0], 0, 0, 0);
+  acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[T], w.y, acc[1], 0, 0, 0);
+}
+
+// the conjugated factors of the forward epilogue (lsum_group), per plane:
+// uu m -> plane m + 2, vv m -> 7 + m, ww m -> 12 + m; table t = 3 X + |m|
+// (X: 0 x, 1 dx/dtheta, 2 x/sin theta); out 0 ur, 1 ut, 2 up
+template <int WAVE>
+__device__ __forceinline__ void lsumT_terms(d4 (*acc)[2], const double* a, const cd* ob, cd e1,
+                                            cd e2) {
+  if constexpr (WAVE == 0) {
+    lsumT_term<0, 0, 0, false, 1, false>(acc[0], a, ob, e1, e2);   // uu0:  x0, ur, 1
+    lsumT_term<1, 0, 1, true, 1, false>(acc[1], a, ob, e1, e2);    // uu+1: x1, ur, conj e
+    lsumT_term<1, 0, 1, false, -1, false>(acc[2], a, ob, e1, e2);  // uu-1: x1, ur, -e
+    lsumT_term<2, 0, 2, true, 1, false>(acc[3], a, ob, e1, e2);    // uu+2: x2, ur, conj e^2
+    lsumT_term<2, 0, 2, false, 1, false>(acc[4], a, ob, e1, e2);   // uu-2: x2, ur, e^2
+    lsumT_term<3, 1, 0, false, 1, false>(acc[5], a, ob, e1, e2);   // vv0:  dx0, ut, 1
+  } else if constexpr (WAVE == 1) {
+    lsumT_term<4, 1, 1, true, 1, false>(acc[0], a, ob, e1, e2);    // vv+1: dx1, ut, conj e
+    lsumT_term<7, 2, 1, true, -1, true>(acc[0], a, ob, e1, e2);    //       xs1, up, -i conj e
+    lsumT_term<4, 1, 1, false, -1, false>(acc[1], a, ob, e1, e2);  // vv-1: dx1, ut, -e
+    lsumT_term<7, 2, 1, false, -1, true>(acc[1], a, ob, e1, e2);   //       xs1, up, -i e
+    lsumT_term<5, 1, 2, true, 1, false>(acc[2], a, ob, e1, e2);    // vv+2: dx2, ut, conj e^2
+    lsumT_term<8, 2, 2, true, -2, true>(acc[2], a, ob, e1, e2);    //       xs2, up, -2i conj e^2
+  } else if constexpr (WAVE == 2) {
+    lsumT_term<5, 1, 2, false, 1, false>(acc[0], a, ob, e1, e2);   // vv-2: dx2, ut, e^2
+    lsumT_term<8, 2, 2, false, 2, true>(acc[0], a, ob, e1, e2);    //       xs2, up, 2i e^2
+    lsumT_term<1, 1, 1, true, -1, true>(acc[1], a, ob, e1, e2);    // ww+1: x1, ut, -i conj e
+    lsumT_term<4, 2, 1, true, -1, false>(acc[1], a, ob, e1, e2);   //       dx1, up, -conj e
+    lsumT_term<1, 1, 1, false, -1, true>(acc[2], a, ob, e1, e2);   // ww-1: x1, ut, -i e
+    lsumT_term<4, 2, 1, false, 1, false>(acc[2], a, ob, e1, e2);   //       dx1, up, e
+  } else {
+    lsumT_term<2, 1, 2, true, -2, true>(acc[0], a, ob, e1, e2);    // ww+2: x2, ut, -2i conj e^2
+    lsumT_term<5, 2, 2, true, -1, false>(acc[0], a, ob, e1, e2);   //       dx2, up, -conj e^2
+    lsumT_term<2, 1, 2, false, 2, true>(acc[1], a, ob, e1, e2);    // ww-2: x2, ut, 2i e^2
+    lsumT_term<5, 2, 2, false, -1, false>(acc[1], a, ob, e1, e2);  //       dx2, up, -e^2
+  }
+}
+
+template <int WAVE>
+__device__ __forceinline__ void lsumT_wave(const RunK& R, const double* __restrict__ tb,
+                                           const cd* __restrict__ eia,
+                                           const cd* __restrict__ specb, cd* __restrict__ cfb,
+                                           int l0, int j0) {
+  // planes of this wave, in the order of lsumT_terms
+  constexpr int NP = WAVE == 0 ? 6 : 3;
+  constexpr int planes[4][6] = {{2, 3, 1, 4, 0, 7}, {8, 6, 9}, {5, 13, 11}, {14, 10, 12}};
+  constexpr int ntab = WAVE == 0 ? 4 : 9;
+  const int lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
+  const size_t tplane = (size_t)R.lpad * R.nrpad, nwr = (size_t)R.nw * R.nr;
+  const int l = l0 + li, j = j0 + li;
+  const bool lok = l < R.lpad, jok = j < R.nw;
+  d4 acc[NP][2];
+#pragma unroll
+  for (int s = 0; s < NP; s++) acc[s][0] = acc[s][1] = (d4){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < R.nrpad; k0 += 4) {
+    const int k = k0 + lq;
+    const bool kok = k < R.nr;
+    double a[ntab];
+#pragma unroll
+    for (int t = 0; t < ntab; t++) a[t] = 0.0;
+    // only the tables this wave's terms read
+#pragma unroll
+    for (int t = 0; t < ntab; t++) {
+      constexpr unsigned used[4] = {0xfu, 0x1b0u, 0x132u, 0x24u};
+      if ((used[WAVE] >> t) & 1u) a[t] = lok ? tb[t * tplane + (size_t)l * R.nrpad + k] : 0.0;
+    }
+    cd ob[3];
+#pragma unroll
+    for (int o = 0; o < 3; o++)
+      ob[o] = (kok && jok) ? specb[o * nwr + (size_t)k * R.nw + j] : C(0.0);
+    cd e1 = kok ? eia[k] : C(1.0), e2 = e1 * e1;
+    lsumT_terms<WAVE>(acc, a, ob, e1, e2);
+  }
+  const size_t cplane = (size_t)R.lpad * R.jpad;
+#pragma unroll
+  for (int s = 0; s < NP; s++) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      int lo = l0 + lq + 4 * v;
+      if (lo < R.lpad) cfb[planes[WAVE][s] * cplane + (size_t)lo * R.jpad + j] = C(acc[s][0][v], acc[s][1][v]);
+    }
+  }
+  if constexpr (WAVE == 3) {
+    // ww0 does not reach the spectra
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      int lo = l0 + lq + 4 * v;
+      if (lo < R.lpad) cfb[12 * cplane + (size_t)lo * R.jpad + j] = C(0.0);
+    }
+  }
+}
+
+// one block of four waves per 16 x 16 tile; grid ceil(lpad / 16) * jpad / 16
+__global__ void __launch_bounds__(256) k_lsum_T(const RunK* __restrict__ runs, int e,
+                                                const double* __restrict__ tb_all,
+                                                const cd* __restrict__ eia_all,
+                                                const cd* __restrict__ specb_all,
+                                                cd* __restrict__ cfb_all) {
+  const RunK& R = runs[e];
+  const int njt = R.jpad / 16;
+  const int j0 = (blockIdx.x % njt) * 16, l0 = (blockIdx.x / njt) * 16;
+  const double* tb = tb_all + R.tb_off;
+  const cd* eia = eia_all + R.roff;
+  const cd* specb = specb_all + R.spec_off;
+  cd* cfb = cfb_all + R.cf_off;
+  switch (threadIdx.x >> 6) {
+    case 0: lsumT_wave<0>(R, tb, eia, specb, cfb, l0, j0); break;
+    case 1: lsumT_wave<1>(R, tb, eia, specb, cfb, l0, j0); break;
+    case 2: lsumT_wave<2>(R, tb, eia, specb, cfb, l0, j0); break;
+    default: lsumT_wave<3>(R, tb, eia, specb, cfb, l0, j0); break;
+  }
+}
+
+// ----------------------------------------------------------- k_assemble_T
+// Adjoint of k_assemble for the pairs of one event, one lane per pair: from
+// the coefficient adjoints cf_bar (uu, vv, ww) and the pair's record (the
+// forward's inputs) to the adjoints of the record's inputs svt, ytor, svs,
+// ysph (rows 0..2), sr0, yrad.  recb gets the record layout: input
+// adjoints in [0, O_UU), the coefficient adjoints in the uu, vv, ww slots.
+__global__ void __launch_bounds__(64) k_assemble_T(const RunK* __restrict__ runs, int e,
+                                                   const cd* __restrict__ rec,
+                                                   const cd* __restrict__ cfb_all,
+                                                   cd* __restrict__ recb) {
+  const RunK& R = runs[e];
+  int p = blockIdx.x * 64 + (int)threadIdx.x;
+  if (p >= R.npair) return;
+  int l, j;
+  pair_lj(R, p, l, j);
+  const cd* o = rec + (size_t)(R.pbase + p) * REC;
+  cd* ob = recb + (size_t)(R.pbase + p) * REC;
+  const cd* cfb = cfb_all + R.cf_off;
+  const size_t plane = (size_t)R.lpad * R.jpad, at = (size_t)l * R.jpad + (j - 1);
+  cd c[15];
+  for (int m = 0; m < 15; m++) c[m] = cfb[m * plane + at];
+  cd w = C(R.wmin + (double)(j - 1) * R.dw, -R.ep);
+  cd fo = (R.out == 1) ? cmuli(w) : (R.out == 2 ? -(w * w) : C(1.0));
+  cd fc = conj_(fo);
+  double rr = R.rr;
+  for (int f = 0; f < O_UU; f++) ob[f] = C(0.0);
+  if (l == 0) {
+    // c2 = fo (sr0_1 yrad_11 + sr0_2 yrad_12) / rr
+    cd g = fc * c[2] / rr;
+    ob[O_SR0] = conj_(o[O_YRAD + 0]) * g;
+    ob[O_SR0 + 1] = conj_(o[O_YRAD + 2]) * g;
+    ob[O_YRAD + 0] = conj_(o[O_SR0]) * g;
+    ob[O_YRAD + 2] = conj_(o[O_SR0 + 1]) * g;
+  } else {
+    double zeta = (double)sqrtf((float)(l * (l + 1)));
+    if (l > 1) {
+      // ww_m = (svt_m1 fo ytor_11 + svt_m2 fo ytor_12) / (zeta rr)
+      cd t1 = fo * o[O_YTOR + 0], t2 = fo * o[O_YTOR + 2], t1b = C(0.0), t2b = C(0.0);
+      for (int m = 0; m < 5; m++) {
+        cd g = c[10 + m] / (zeta * rr);
+        ob[O_SVT + m] = conj_(t1) * g;
+        ob[O_SVT + 5 + m] = conj_(t2) * g;
+        t1b = t1b + conj_(o[O_SVT + m]) * g;
+        t2b = t2b + conj_(o[O_SVT + 5 + m]) * g;
+      }
+      ob[O_YTOR + 0] = fc * t1b;
+      ob[O_YTOR + 2] = fc * t2b;
+    }
+    const bool cor = (R.out == 2 && R.cor != 0);
+    cd y1[4], y2[4];
+    for (int k = 0; k < 4; k++) {
+      cd a = o[O_YSPH + 6 * k + 0], bb = o[O_YSPH + 6 * k + 1], g3 = o[O_YSPH + 6 * k + 2];
+      if (cor) {
+        y1[k] = fo * a + (-((double)(l + 1) * g3 + (2.0 * R.grav_ir) * a) / rr);
+        y2[k] = fo * bb + zeta * (g3 + R.grav_ir * a) / rr;
+      } else {
+        y1[k] = fo * a;
+        y2[k] = fo * bb;
+      }
+    }
+    // uu_m = sum_k y1_k sv_(m, col k) / rr, vv_m = sum_k y2_k sv_(m, col k) / (zeta rr)
+    // over the source-vector columns 1, 2, 4, 5
+    constexpr int off[4] = {0, 5, 15, 20};
+    cd y1b[4], y2b[4];
+    for (int k = 0; k < 4; k++) y1b[k] = y2b[k] = C(0.0);
+    for (int m = 0; m < 5; m++) {
+      cd g1 = c[m] / rr, g2 = c[5 + m] / (zeta * rr);
+      for (int k = 0; k < 4; k++) {
+        cd sv = o[O_SVS + off[k] + m];
+        ob[O_SVS + off[k] + m] = conj_(y1[k]) * g1 + conj_(y2[k]) * g2;
+        y1b[k] = y1b[k] + conj_(sv) * g1;
+        y2b[k] = y2b[k] + conj_(sv) * g2;
+      }
+    }
+    for (int k = 0; k < 4; k++) {
+      cd* yb = ob + O_YSPH + 6 * k;
+      if (cor) {
+        yb[0] = conj_(fo - 2.0 * R.grav_ir / rr) * y1b[k] + (zeta * R.grav_ir / rr) * y2b[k];
+        yb[1] = fc * y2b[k];
+        yb[2] = (-(double)(l + 1) / rr) * y1b[k] + (zeta / rr) * y2b[k];
+      } else {
+        yb[0] = fc * y1b[k];
+        yb[1] = fc * y2b[k];
+      }
+    }
+  }
+  for (int m = 0; m < 15; m++) ob[O_UU + m] = c[m];
+}
+
+// ------------------------------------------------------------- k_source_T
+// Gradient with respect to the normalised moment tensor from the record
+// adjoints recb of one event: the source vectors are linear in mm, so
+// dF/dmm_i = sum over pairs and source-vector fields f of
+// Re(conj(recb_f) sv_f(e_i)), e_i the i-th unit tensor.  One lane per pair;
+// part[6 b + i] gets block b's sum (the host adds the blocks in order, so
+// the result does not depend on scheduling).
+__global__ void __launch_bounds__(256) k_source_T(const double* __restrict__ gtab,
+                                                  const RunK* __restrict__ runs, int e,
+                                                  const cd* __restrict__ recb,
+                                                  double* __restrict__ part) {
+  __shared__ double red[4][6];
+  const RunK& R = runs[e];
+  Tab T{gtab, R.nknot};
+  const int p = blockIdx.x * 256 + (int)threadIdx.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (p < R.npair) {
+    int l, j;
+    pair_lj(R, p, l, j);
+    PS s;
+    pair_state(R, l, j, s);
+    const cd* b = recb + (size_t)(R.pbase + p) * REC;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+      double mi[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      mi[i] = 1.0;
+      double a = 0.0;
+      source_vectors(T, R, s, l, mi, [&](int f, cd v) { a += b[f].x * v.x + b[f].y * v.y; });
+      acc[i] = a;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+    for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_down(acc[i], off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int i = 0; i < 6; i++) red[wv][i] = acc[i];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int i = threadIdx.x;
+    part[(size_t)blockIdx.x * 6 + i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
+  }
+}
+
 }  // namespace
 
 void launch_reset(dim3 g, dim3 b, size_t lds, hipStream_t s, int* counters, int n) {
@@ -950,6 +1307,22 @@ void launch_fft_post(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs
 }
 void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw) {
   hipLaunchKernelGGL(k_twiddles, g, b, lds, s, nt, tw);
+}
+void launch_seis_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const double* seisb, const cd* tw, cd* work, const double* azep, cd* specb) {
+  hipLaunchKernelGGL(k_seis_T, g, b, lds, s, runs, e, seisb, tw, work, azep, specb);
+}
+void launch_lsum_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const double* tb, const cd* eia, const cd* specb, cd* cfb) {
+  hipLaunchKernelGGL(k_lsum_T, g, b, lds, s, runs, e, tb, eia, specb, cfb);
+}
+void launch_assemble_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                       const cd* rec, const cd* cfb, cd* recb) {
+  hipLaunchKernelGGL(k_assemble_T, g, b, lds, s, runs, e, rec, cfb, recb);
+}
+void launch_source_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, int e, const cd* recb, double* part) {
+  hipLaunchKernelGGL(k_source_T, g, b, lds, s, gtab, runs, e, recb, part);
 }
 
 }  // namespace ysx

@@ -277,6 +277,15 @@ void launch_fft_lds(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs,
 void launch_fft_post(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
                      const cd* tw, const cd* work, const double* azep, double* seis);
 void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw);
+// adjoints of the seismogram, receiver-sum and assembly stages (event e)
+void launch_seis_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const double* seisb, const cd* tw, cd* work, const double* azep, cd* specb);
+void launch_lsum_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const double* tb, const cd* eia, const cd* specb, cd* cfb);
+void launch_assemble_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                       const cd* rec, const cd* cfb, cd* recb);
+void launch_source_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, int e, const cd* recb, double* part);
 #define YSX_MARCH_LAUNCHERS(SUFFIX)                                                              \
   void launch_tor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
                                  const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt);   \
