@@ -23,8 +23,11 @@ from conftest import load_case
 
 pytestmark = pytest.mark.gpu
 
+# no_filter: all filter corners zero; receivers_37: more than two 16-receiver
+# tiles; long_series: nt = 131072, past the two-pass FFT limit (the
+# single-pass global-memory transforms on both sides)
 CASES = ["cfg1_small", "aniso_acc_cor", "cow_ocean_vel", "epicentre_ragged", "radial_only",
-         "one_degree", "prem_example", "no_filter", "receivers_37"]
+         "one_degree", "prem_example", "no_filter", "receivers_37", "long_series"]
 SRC = ("svt", "svs", "sr0")
 SOL = ("ytor", "ysph", "yrad")
 TOL = 1e-11
@@ -47,6 +50,10 @@ def _case(name):
         lat = rng.uniform(-80.0, 80.0, 37)
         lon = rng.uniform(-180.0, 180.0, 37)
         return deck, dataclasses.replace(p, lat=lat, lon=lon)
+    if name == "long_series":
+        deck, p, _ = load_case("cfg1_small")
+        return deck, dataclasses.replace(p, lmax=10, f1=0.1, f2=1.0, tout=1500.0, dt=1.0,
+                                         f11=0.2, f12=0.3, f21=0.7, f22=0.9)
     deck, p, _ = load_case(name)
     return deck, p
 
@@ -201,3 +208,37 @@ def test_moment_tensor_vjp_needs_record_adjoints(api):
     g.forward()
     with pytest.raises(api.YSpecError):
         g.moment_tensor_vjp()
+
+
+def test_autograd_misfit_vs_central_differences(api):
+    """L2 misfit through the torch.autograd wrapper against central finite
+    differences (h = 1e-6 relative) of the GPU forward, per tensor component"""
+    import torch
+
+    from yspec_amd.autograd import seismograms
+
+    deck, p = _case("prem_example")
+    g = api.YSpec(deck)
+    m0 = np.asarray(p.mm, dtype=np.float64)
+    rng = np.random.default_rng(6)
+    obs_mm = m0 * (1.0 + 0.2 * rng.standard_normal(6))
+    g.set_run(dataclasses.replace(p, mm=obs_mm))
+    obs = torch.from_numpy(np.ascontiguousarray(g.forward()))
+
+    def misfit(m):
+        return 0.5 * torch.sum((seismograms(m, g, p) - obs) ** 2)
+
+    mm = torch.tensor(m0, requires_grad=True)
+    misfit(mm).backward()
+    grad = mm.grad.numpy()
+    fd = np.zeros(6)
+    for i in range(6):
+        h = 1e-6 * np.abs(m0).max()
+        mp, mn = m0.copy(), m0.copy()
+        mp[i] += h
+        mn[i] -= h
+        with torch.no_grad():
+            fd[i] = (misfit(torch.tensor(mp)) - misfit(torch.tensor(mn))).item() / (2 * h)
+    err = np.abs(grad - fd) / np.linalg.norm(fd)
+    print("autograd", grad, "central differences", fd, f"max rel {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-6

@@ -1170,8 +1170,18 @@ extern "C" int ysx_seismograms_vjp(ysx_ctx* ctx, const double* seis_bar, double*
   HIPCHK(hipMemcpy(ctx->d_seisb, seis_bar, 3 * (size_t)R.nt * R.nr * sizeof(double),
                    hipMemcpyHostToDevice));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
-  launch_seis_T(dim3(std::max(1, R.nr)), dim3(1024), 0, st, ctx->d_runs, 0, ctx->d_seisb,
-                ctx->d_tw, ctx->d_work, ctx->d_azep, ctx->d_specb);
+  const int bk = std::min(R.nt, FFT_B), nblk = R.nt / bk;
+  if (nblk <= FFT_G) {  // three passes, the transpose of k_fft_lds + k_fft_post
+    launch_post_T(dim3(R.nr * ((bk + 255) / 256)), dim3(256), 0, st, ctx->d_runs, 0,
+                  ctx->d_seisb, ctx->d_tw, ctx->d_azep, ctx->d_work);
+    launch_fft_lds_T(dim3(R.nr * 3 * nblk), dim3(512), 0, st, ctx->d_runs, 0, ctx->d_tw,
+                     ctx->d_work);
+    launch_pack_T(dim3(R.nr * 3 * ((R.nw + 255) / 256)), dim3(256), 0, st, ctx->d_runs, 0,
+                  ctx->d_work, ctx->d_specb);
+  } else {
+    launch_seis_T(dim3(std::max(1, R.nr)), dim3(1024), 0, st, ctx->d_runs, 0, ctx->d_seisb,
+                  ctx->d_tw, ctx->d_work, ctx->d_azep, ctx->d_specb);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev[1], st));
   HIPCHK(hipEventSynchronize(ctx->ev[1]));
@@ -1213,7 +1223,7 @@ extern "C" int ysx_spectra_vjp(ysx_ctx* ctx, const double* ur_bar, const double*
   HIPCHK(hipEventRecord(ctx->ev[0], st));
   launch_lsum_T(dim3(ntile), dim3(256), 0, st, ctx->d_runs, 0, ctx->d_tb, ctx->d_eia,
                 ctx->d_specb, ctx->d_cfb);
-  launch_assemble_T(dim3((unsigned)((k.npair + 63) / 64)), dim3(64), 0, st, ctx->d_runs, 0,
+  launch_assemble_T(dim3((unsigned)((k.npair + ASMT_P - 1) / ASMT_P)), dim3(64), 0, st, ctx->d_runs, 0,
                     ctx->d_rec, ctx->d_cfb, ctx->d_recb);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev[1], st));

@@ -980,6 +980,125 @@ __global__ void __launch_bounds__(1024) k_seis_T(const RunK* __restrict__ runs, 
   }
 }
 
+// The same adjoint in three passes for nt <= FFT_B * FFT_G, the transpose
+// of k_fft_lds + k_fft_post: k_post_T transposes the post-processing and the
+// register stages (mmax = nt/2 .. FFT_B), k_fft_lds_T the LDS stages
+// (mmax = FFT_B/2 .. 1) of each contiguous block, and k_pack_T gathers the
+// bit-reversed, Hermitian-folded, filtered bins.  Same butterflies and
+// twiddles as k_seis_T, so the results are the same bit for bit.
+
+template <int G>
+__device__ __forceinline__ void post_T_g(const RunK& K, int k, int i0, const cd* __restrict__ tw,
+                                         const double* __restrict__ seisb, double ca, double sa,
+                                         cd* __restrict__ work) {
+  const int nt = K.nt, bk = nt / G;
+  const double sc = K.norm / (K.dt * nt);
+  cd v[3][G];
+#pragma unroll
+  for (int t = 0; t < G; t++) {
+    int i = i0 + t * bk;
+    double g = exp(K.ep * (i * K.dt)) * sc;
+    double z = seisb[(size_t)i + (size_t)nt * (0 + 3 * (size_t)k)];
+    double n = seisb[(size_t)i + (size_t)nt * (1 + 3 * (size_t)k)];
+    double ea = seisb[(size_t)i + (size_t)nt * (2 + 3 * (size_t)k)];
+    v[0][t] = C(z * g);
+    v[1][t] = C((-ca * n + sa * ea) * g);
+    v[2][t] = C((sa * n + ca * ea) * g);
+  }
+#pragma unroll
+  for (int step = G >> 1; step >= 1; step >>= 1) {
+    int mmax = step * bk;
+#pragma unroll
+    for (int t = 0; t < G; t++) {
+      if (t & step) continue;
+      int tj = t + step;
+      cd w = conj_(tw[mmax + ((i0 + t * bk) & (mmax - 1))]);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        cd x = v[c][t], y = v[c][tj];
+        v[c][t] = x + y;
+        v[c][tj] = w * (x - y);
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    cd* row = work + ((size_t)k * 3 + c) * nt;
+#pragma unroll
+    for (int t = 0; t < G; t++) row[i0 + t * bk] = v[c][t];
+  }
+}
+
+// blocks (receiver k, block of 256 column indices i0 < min(nt, FFT_B))
+__global__ void __launch_bounds__(256) k_post_T(const RunK* __restrict__ runs, int e,
+                                                const double* __restrict__ seisb_all,
+                                                const cd* __restrict__ tw,
+                                                const double* __restrict__ azep_all,
+                                                cd* __restrict__ work_all) {
+  const RunK& K = runs[e];
+  const int bk = min(K.nt, FFT_B), g = K.nt / bk, nib = (bk + 255) / 256;
+  const int k = blockIdx.x / nib, i0 = (blockIdx.x % nib) * 256 + (int)threadIdx.x;
+  if (k >= K.nr || i0 >= bk) return;
+  double az = azep_all[K.roff + k];
+  double ca = cos(az), sa = sin(az);
+  const double* seisb = seisb_all + K.seis_off;
+  cd* work = work_all + K.seis_off;
+  switch (g) {
+    case 1: post_T_g<1>(K, k, i0, tw, seisb, ca, sa, work); break;
+    case 2: post_T_g<2>(K, k, i0, tw, seisb, ca, sa, work); break;
+    case 4: post_T_g<4>(K, k, i0, tw, seisb, ca, sa, work); break;
+    case 8: post_T_g<8>(K, k, i0, tw, seisb, ca, sa, work); break;
+    default: post_T_g<16>(K, k, i0, tw, seisb, ca, sa, work); break;
+  }
+}
+
+// blocks (receiver k, component c, block of min(nt, FFT_B) elements), in place
+__global__ void __launch_bounds__(512) k_fft_lds_T(const RunK* __restrict__ runs, int e,
+                                                   const cd* __restrict__ tw,
+                                                   cd* __restrict__ work_all) {
+  __shared__ cd buf[FFT_B];
+  const RunK& K = runs[e];
+  const int nt = K.nt, bk = min(nt, FFT_B), nblk = nt / bk;
+  const int k = blockIdx.x / (3 * nblk), rem = blockIdx.x % (3 * nblk);
+  const int c = rem / nblk, blk = rem % nblk;
+  if (k >= K.nr) return;
+  cd* row = work_all + K.seis_off + ((size_t)k * 3 + c) * nt + (size_t)blk * bk;
+  for (int q = threadIdx.x; q < bk; q += blockDim.x) buf[q] = row[q];
+  __syncthreads();
+  for (int mmax = bk >> 1; mmax >= 1; mmax >>= 1) {
+    for (int b = threadIdx.x; b < bk / 2; b += blockDim.x) {
+      int m = b & (mmax - 1);
+      int i = ((b - m) << 1) + m;
+      int jx = i + mmax;
+      cd x = buf[i], y = buf[jx];
+      buf[i] = x + y;
+      buf[jx] = conj_(tw[mmax + m]) * (x - y);
+    }
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < bk; q += blockDim.x) row[q] = buf[q];
+}
+
+// blocks (receiver k, component c, block of 256 bins)
+__global__ void __launch_bounds__(256) k_pack_T(const RunK* __restrict__ runs, int e,
+                                                const cd* __restrict__ work_all,
+                                                cd* __restrict__ specb_all) {
+  const RunK& K = runs[e];
+  const int nt = K.nt, nwb = (K.nw + 255) / 256;
+  const int k = blockIdx.x / (3 * nwb), rem = blockIdx.x % (3 * nwb);
+  const int c = rem / nwb, jj = (rem % nwb) * 256 + (int)threadIdx.x;
+  if (k >= K.nr || jj >= K.nw) return;
+  const cd* row = work_all + K.seis_off + ((size_t)k * 3 + c) * nt;
+  int b = jj + K.i1 - 1;
+  cd v = C(0.0);
+  if (b <= nt / 2) {
+    v = row[brev((unsigned)b, K.lognt)];
+    if (b > 0 && b < nt / 2) v = v + conj_(row[brev((unsigned)(nt - b), K.lognt)]);
+    v = v * bin_filter(K, b);
+  }
+  specb_all[K.spec_off + (size_t)c * K.nw * K.nr + (size_t)K.nw * k + jj] = v;
+}
+
 // --------------------------------------------------------------- k_lsum_T
 // Adjoint of k_lsum for one event: cf_bar[q](l, j) = sum over receivers k
 // of X(l, k) conj(F(k)) spec_bar(k, j) for every Legendre table X and
@@ -1118,23 +1237,13 @@ __global__ void __launch_bounds__(256) k_lsum_T(const RunK* __restrict__ runs, i
 }
 
 // ----------------------------------------------------------- k_assemble_T
-// Adjoint of k_assemble for the pairs of one event, one lane per pair: from
-// the coefficient adjoints cf_bar (uu, vv, ww) and the pair's record (the
-// forward's inputs) to the adjoints of the record's inputs svt, ytor, svs,
-// ysph (rows 0..2), sr0, yrad.  recb gets the record layout: input
-// adjoints in [0, O_UU), the coefficient adjoints in the uu, vv, ww slots.
-__global__ void __launch_bounds__(64) k_assemble_T(const RunK* __restrict__ runs, int e,
-                                                   const cd* __restrict__ rec,
-                                                   const cd* __restrict__ cfb_all,
-                                                   cd* __restrict__ recb) {
-  const RunK& R = runs[e];
-  int p = blockIdx.x * 64 + (int)threadIdx.x;
-  if (p >= R.npair) return;
-  int l, j;
-  pair_lj(R, p, l, j);
-  const cd* o = rec + (size_t)(R.pbase + p) * REC;
-  cd* ob = recb + (size_t)(R.pbase + p) * REC;
-  const cd* cfb = cfb_all + R.cf_off;
+// Adjoint of k_assemble, for one pair: from the coefficient adjoints cf_bar
+// (uu, vv, ww) and the pair's inputs o (the forward's record fields
+// [0, O_UU)) to the adjoints of the inputs svt, ytor, svs, ysph (rows
+// 0..2), sr0, yrad.  ob gets the record layout: input adjoints in
+// [0, O_UU), the coefficient adjoints in the uu, vv, ww slots.
+__device__ __forceinline__ void assemble_T_pair(const RunK& R, int l, int j, const cd* o,
+                                                const cd* __restrict__ cfb, cd* ob) {
   const size_t plane = (size_t)R.lpad * R.jpad, at = (size_t)l * R.jpad + (j - 1);
   cd c[15];
   for (int m = 0; m < 15; m++) c[m] = cfb[m * plane + at];
@@ -1204,6 +1313,50 @@ __global__ void __launch_bounds__(64) k_assemble_T(const RunK* __restrict__ runs
     }
   }
   for (int m = 0; m < 15; m++) ob[O_UU + m] = c[m];
+}
+
+// ASMT_P consecutive pairs of one event per 64-lane block: their inputs are
+// copied to LDS with coalesced loads, one lane per pair computes into a
+// second LDS array, and the ASMT_P contiguous 89-field record adjoints are
+// stored linearly.
+__global__ void __launch_bounds__(64) k_assemble_T(const RunK* __restrict__ runs, int e,
+                                                   const cd* __restrict__ rec,
+                                                   const cd* __restrict__ cfb_all,
+                                                   cd* __restrict__ recb) {
+  __shared__ cd si[ASMT_P * ASM_LD];
+  __shared__ cd so[ASMT_P * REC];  // stride 89: the 16 lanes' 16-byte writes hit distinct banks
+  const RunK& R = runs[e];
+  const long long p0 = (long long)blockIdx.x * ASMT_P;
+  if (p0 >= R.npair) return;
+  const int np = (int)min((long long)ASMT_P, (long long)R.npair - p0);
+  const cd* src = rec + (size_t)(R.pbase + p0) * REC;
+  {
+    // all loads issued before the first LDS write; indices past the last
+    // field are clamped to it, so no load is predicated
+    constexpr int NL = (ASMT_P * O_UU + 63) / 64;
+    const int kmax = np * O_UU - 1;
+    cd v[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      int k = min((int)threadIdx.x + 64 * i, kmax), q = k / O_UU, f = k - q * O_UU;
+      v[i] = src[(size_t)q * REC + f];
+    }
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      int k = min((int)threadIdx.x + 64 * i, kmax), q = k / O_UU, f = k - q * O_UU;
+      si[q * ASM_LD + f] = v[i];
+    }
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < np) {
+    int l, j;
+    pair_lj(R, (int)(p0 + t), l, j);
+    assemble_T_pair(R, l, j, si + t * ASM_LD, cfb_all + R.cf_off, so + t * REC);
+  }
+  __syncthreads();
+  cd* dst = recb + (size_t)(R.pbase + p0) * REC;
+  for (int k = t; k < np * REC; k += 64) dst[k] = so[k];
 }
 
 // ------------------------------------------------------------- k_source_T
@@ -1311,6 +1464,18 @@ void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw) 
 void launch_seis_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
                    const double* seisb, const cd* tw, cd* work, const double* azep, cd* specb) {
   hipLaunchKernelGGL(k_seis_T, g, b, lds, s, runs, e, seisb, tw, work, azep, specb);
+}
+void launch_post_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const double* seisb, const cd* tw, const double* azep, cd* work) {
+  hipLaunchKernelGGL(k_post_T, g, b, lds, s, runs, e, seisb, tw, azep, work);
+}
+void launch_fft_lds_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                      const cd* tw, cd* work) {
+  hipLaunchKernelGGL(k_fft_lds_T, g, b, lds, s, runs, e, tw, work);
+}
+void launch_pack_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const cd* work, cd* specb) {
+  hipLaunchKernelGGL(k_pack_T, g, b, lds, s, runs, e, work, specb);
 }
 void launch_lsum_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
                    const double* tb, const cd* eia, const cd* specb, cd* cfb) {

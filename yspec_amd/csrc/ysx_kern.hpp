@@ -148,6 +148,8 @@ constexpr int O_SVT = 0, O_YTOR = 10, O_SVS = 14, O_YSPH = 44, O_SR0 = 68, O_YRA
 // k_assemble: pairs per 64-lane block
 constexpr int ASM_P = 32;
 inline int asm_blocks(const PairRange& PR) { return (int)std::max(1LL, (PR.n + ASM_P - 1) / ASM_P); }
+// k_assemble_T: pairs per 64-lane block
+constexpr int ASMT_P = 16;
 
 // ------------------------------------------------ state between the kernels
 // Start levels and starting solutions of every pair, written by k_start:
@@ -280,6 +282,12 @@ void launch_twiddles(dim3 g, dim3 b, size_t lds, hipStream_t s, int nt, cd* tw);
 // adjoints of the seismogram, receiver-sum and assembly stages (event e)
 void launch_seis_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
                    const double* seisb, const cd* tw, cd* work, const double* azep, cd* specb);
+void launch_post_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const double* seisb, const cd* tw, const double* azep, cd* work);
+void launch_fft_lds_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                      const cd* tw, cd* work);
+void launch_pack_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
+                   const cd* work, cd* specb);
 void launch_lsum_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
                    const double* tb, const cd* eia, const cd* specb, cd* cfb);
 void launch_assemble_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, int e,
