@@ -196,6 +196,15 @@ int ysx_spectra_vjp(ysx_ctx *ctx, const double *ur_bar, const double *ut_bar,
  * of sum(seis_bar * seismograms). */
 int ysx_moment_tensor_vjp(ysx_ctx *ctx, double *mm_bar);
 
+/* The source-vector term of the gradient with respect to the model: the
+ * source vectors read L, C, F, 1/Qmu, xlam and xa2 at the source radius
+ * (spline values on the source's knot interval), so the record adjoints of
+ * the last ysx_spectra_vjp give adjoints of those table entries.
+ * table_bar: F_COUNT x nknot in the layout of ysx_model_table (zero except
+ * the source interval's entries), the input of ysx_model_table_vjp.  The
+ * terms of the per-pair solves are not included. */
+int ysx_source_model_vjp(ysx_ctx *ctx, double *table_bar);
+
 /* Kernel milliseconds (HIP events on the context's stream, host copies
  * excluded) of the last call of [0] ysx_seismograms_vjp, [1] ysx_spectra_vjp,
  * [2] ysx_moment_tensor_vjp. */

@@ -28,7 +28,7 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_model_table_host", "ysx_model_table_vjp",
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts",
            "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
-           "ysx_vjp_timings")
+           "ysx_vjp_timings", "ysx_source_model_vjp")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -91,6 +91,7 @@ def lib():
         L.ysx_spectra_vjp.argtypes = [vp, D, D, D, C.c_int, I, I, D]
         L.ysx_moment_tensor_vjp.argtypes = [vp, D]
         L.ysx_vjp_timings.argtypes = [vp, D]
+        L.ysx_source_model_vjp.argtypes = [vp, D]
         _LIB = L
     return _LIB
 
@@ -271,6 +272,13 @@ class YSpec:
         from the record adjoints of the last spectra_vjp."""
         out = np.zeros(6)
         self._check(self.L.ysx_moment_tensor_vjp(self.h, _dp(out)))
+        return out
+
+    def source_model_vjp(self):
+        """(29, nknot) table adjoint of the source-vector term, from the record
+        adjoints of the last spectra_vjp (input of model_table_vjp)."""
+        out = np.zeros((len(TABLE_FIELDS), self.deck.nknot))
+        self._check(self.L.ysx_source_model_vjp(self.h, _dp(out)))
         return out
 
     def vjp_timings(self):

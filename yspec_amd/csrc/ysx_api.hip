@@ -1261,6 +1261,49 @@ extern "C" int ysx_moment_tensor_vjp(ysx_ctx* ctx, double* mm_bar) {
   return YSX_OK;
 }
 
+extern "C" int ysx_source_model_vjp(ysx_ctx* ctx, double* table_bar) {
+  if (!ctx || !table_bar) return YSX_ERR_ARG;
+  if (int rc = one_event_run(ctx)) return rc;
+  if (!ctx->have_recb) return set_err(ctx, YSX_ERR_ARG, "no record adjoints: call ysx_spectra_vjp first");
+  const RunK& k = ctx->rk[0];
+  const int nb = (int)((k.npair + 255) / 256), nk = k.nknot;
+  launch_source_model_T(dim3(nb), dim3(256), 0, ctx->stream, ctx->d_tab, ctx->d_runs, 0,
+                        ctx->d_recb, ctx->d_mmpart);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<double> part(6 * (size_t)nb);
+  HIPCHK(hipMemcpy(part.data(), ctx->d_mmpart, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double G[6];
+  for (int i = 0; i < 6; i++) {
+    G[i] = 0.0;
+    for (int b = 0; b < nb; b++) G[i] += part[6 * (size_t)b + i];
+  }
+  // the spline weights of the source radius on knot interval is (1-based),
+  // as spl_w computes them; every pair reads the same six values
+  double r[2];
+  HIPCHK(hipMemcpy(r, ctx->d_tab + (size_t)F_R * nk + (k.is - 1), 2 * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  const double h = r[1] - r[0];
+  const bool deg = (h == 0.0);
+  const double ih = deg ? 0.0 : 1.0 / h, a = (r[1] - k.rs) * ih, bw = (k.rs - r[0]) * ih;
+  const double hh = h * h / 6.0, a3 = (a * a * a - a) * hh, b3 = (bw * bw * bw - bw) * hh;
+  std::fill(table_bar, table_bar + (size_t)F_COUNT * nk, 0.0);
+  const int fields[6] = {F_LCON, F_CCON, F_FCON, F_QMU, F_XLAM, F_XA2};
+  for (int i = 0; i < 6; i++) {
+    double* y = table_bar + (size_t)fields[i] * nk + (k.is - 1);
+    double* y2 = table_bar + (size_t)(fields[i] + 1) * nk + (k.is - 1);
+    if (deg) {
+      y[0] += G[i];
+      continue;
+    }
+    y[0] += a * G[i];
+    y[1] += bw * G[i];
+    y2[0] += a3 * G[i];
+    y2[1] += b3 * G[i];
+  }
+  return YSX_OK;
+}
+
 extern "C" int ysx_vjp_timings(const ysx_ctx* ctx, double* ms3) {
   if (!ctx || !ms3) return YSX_ERR_ARG;
   std::memcpy(ms3, ctx->ms_vjp, sizeof ctx->ms_vjp);

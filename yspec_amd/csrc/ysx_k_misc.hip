@@ -1403,7 +1403,120 @@ __global__ void __launch_bounds__(256) k_source_T(const double* __restrict__ gta
   }
 }
 
+// ------------------------------------------------------- k_source_model_T
+// Partial derivatives of the source vectors of one pair with respect to the
+// six model values they read at the source radius, v = (L, C, F, 1/Qmu,
+// xlam, xa2) (source_vectors above): put(f, d) receives for record field f
+// the six complex derivatives d[0..5].  Fields that do not depend on v are
+// not reported.
+template <class Put>
+__device__ __forceinline__ void source_vectors_dv(const Tab& T, const RunK& R, const PS& s, int l,
+                                                  const double* mm, Put put) {
+  SplW w = spl_w(T, R.is, R.rs);
+  double L = spl(T, w, F_LCON), Cc = spl(T, w, F_CCON), Fc = spl(T, w, F_FCON),
+         Qm = spl(T, w, F_QMU), X = spl(T, w, F_XLAM), A2 = spl(T, w, F_XA2);
+  double rs = R.rs;
+  cd nd = s.nu * cinv(cmuli(s.omega) * (rs * rs));
+  cd lam = s.lno;
+  const cd z = C(0.0);
+  if (l == 0) {
+    cd qaa = 1.0 + A2 * lam, qff = 1.0 + X * lam;
+    cd cc = cinv(Cc * qaa), cf = Fc * qff;
+    cd dcc_C = -(cc * cc) * qaa, dcc_A2 = -(cc * cc) * (Cc * lam);
+    cd u0 = nd * (rs * mm[0]);
+    cd d0[6] = {z, u0 * dcc_C, z, z, z, u0 * dcc_A2};
+    put(O_SR0 + 0, d0);
+    cd u1 = nd * (2.0 * mm[0]);
+    cd d1[6] = {z, u1 * cf * dcc_C, u1 * qff * cc, z, u1 * (Fc * lam) * cc, u1 * cf * dcc_A2};
+    put(O_SR0 + 1, d1);
+    return;
+  }
+  if (l > 1) {
+    cd ql = 1.0 + Qm * lam;
+    cd icl = cinv(L * ql);
+    cd dicl_L = -(icl * icl) * ql, dicl_Q = -(icl * icl) * (L * lam);
+    cd u1 = nd * ((0.5 * rs) * C(-mm[2], mm[1])), u3 = nd * ((0.5 * rs) * C(mm[2], mm[1]));
+    cd d1[6] = {u1 * dicl_L, z, z, u1 * dicl_Q, z, z};
+    cd d3[6] = {u3 * dicl_L, z, z, u3 * dicl_Q, z, z};
+    put(O_SVT + 1, d1);
+    put(O_SVT + 3, d3);
+  }
+  double lx = lam.x;
+  double qaa = 1.0 + A2 * lx, qff = 1.0 + X * lx, qll = 1.0 + Qm * lx;
+  double cc = 1.0 / (Cc * qaa), cf = Fc * qff, cl = L * qll;
+  double dcc_C = -(cc * cc) * qaa, dcc_A2 = -(cc * cc) * (Cc * lx);
+  double dicl_L = -qll / (cl * cl), dicl_Q = -(L * lx) / (cl * cl);
+  double zeta = s.zeta;
+  {
+    cd u = nd * C(0.5 * rs * mm[1], 0.5 * rs * mm[2]);
+    cd d[6] = {u * dicl_L, z, z, u * dicl_Q, z, z};
+    put(O_SVS + 1 + 5 * 1, d);
+  }
+  {
+    cd u = nd * (rs * mm[0]);
+    cd d[6] = {z, u * dcc_C, z, z, z, u * dcc_A2};
+    put(O_SVS + 2 + 5 * 0, d);
+  }
+  {
+    cd u = nd * (2.0 * mm[0]);
+    cd d[6] = {z, u * (cf * dcc_C), u * (qff * cc), z, u * (Fc * lx * cc), u * (cf * dcc_A2)};
+    put(O_SVS + 2 + 5 * 3, d);
+  }
+  {
+    cd u = nd * (-zeta * mm[0]);
+    cd d[6] = {z, u * (cf * dcc_C), u * (qff * cc), z, u * (Fc * lx * cc), u * (cf * dcc_A2)};
+    put(O_SVS + 2 + 5 * 4, d);
+  }
+  {
+    cd u = nd * C(-0.5 * rs * mm[1], 0.5 * rs * mm[2]);
+    cd d[6] = {u * dicl_L, z, z, u * dicl_Q, z, z};
+    put(O_SVS + 3 + 5 * 1, d);
+  }
+}
+
+// dF/dv for the six source values v from the record adjoints recb of one
+// event: sum over pairs and fields of Re(conj(recb_f) dsv_f/dv).  Layout and
+// reduction as k_source_T.
+__global__ void __launch_bounds__(256) k_source_model_T(const double* __restrict__ gtab,
+                                                        const RunK* __restrict__ runs, int e,
+                                                        const cd* __restrict__ recb,
+                                                        double* __restrict__ part) {
+  __shared__ double red[4][6];
+  const RunK& R = runs[e];
+  Tab T{gtab, R.nknot};
+  const int p = blockIdx.x * 256 + (int)threadIdx.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (p < R.npair) {
+    int l, j;
+    pair_lj(R, p, l, j);
+    PS s;
+    pair_state(R, l, j, s);
+    const cd* b = recb + (size_t)(R.pbase + p) * REC;
+    source_vectors_dv(T, R, s, l, R.mm, [&](int f, const cd* d) {
+      cd bf = b[f];
+#pragma unroll
+      for (int k = 0; k < 6; k++) acc[k] += bf.x * d[k].x + bf.y * d[k].y;
+    });
+  }
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+    for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_down(acc[i], off, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int i = 0; i < 6; i++) red[wv][i] = acc[i];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int i = threadIdx.x;
+    part[(size_t)blockIdx.x * 6 + i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
+  }
+}
+
 }  // namespace
+
+void launch_source_model_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                           const RunK* runs, int e, const cd* recb, double* part) {
+  hipLaunchKernelGGL(k_source_model_T, g, b, lds, s, gtab, runs, e, recb, part);
+}
 
 void launch_reset(dim3 g, dim3 b, size_t lds, hipStream_t s, int* counters, int n) {
   hipLaunchKernelGGL(k_reset, g, b, lds, s, counters, n);

@@ -294,6 +294,8 @@ void launch_assemble_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* ru
                        const cd* rec, const cd* cfb, cd* recb);
 void launch_source_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                      const RunK* runs, int e, const cd* recb, double* part);
+void launch_source_model_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                           const RunK* runs, int e, const cd* recb, double* part);
 #define YSX_MARCH_LAUNCHERS(SUFFIX)                                                              \
   void launch_tor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
                                  const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt);   \
