@@ -164,7 +164,12 @@ int ysx_pair_records(ysx_ctx *ctx, int n, const int *l, const int *j,
 int ysx_pair_steps(ysx_ctx *ctx, int n, const int *l, const int *j, int *out5);
 
 /* ---- adjoints of the linear back end (no counterpart in the reference;
- * stages of a misfit gradient, one-event runs) ---------------------------
+ * stages of a misfit gradient) -------------------------------------------
+ * With several events (ysx_set_events, then ysx_forward_events) seis_bar
+ * holds the events' (nt_e, 3, nr_e) arrays one after another, the spectra
+ * and record adjoints stay on the device (ur_bar, ut_bar, up_bar, the
+ * spectra inputs and rec_bar must be NULL, n = 0), mm_bar gets 6 values per
+ * event and table_bar the sum over the events.
  * For a real function F of the seismograms, a complex quantity z gets the
  * adjoint dF/dRe z + i dF/dIm z.  Each entry point applies the exact
  * transpose of the discrete map the forward applies (its twiddles, filter
@@ -188,8 +193,8 @@ int ysx_spectra_vjp(ysx_ctx *ctx, const double *ur_bar, const double *ut_bar,
                     const double *up_bar, int n, const int *l, const int *j, double *rec_bar);
 
 /* Gradient with respect to the moment tensor (Mrr, Mrt, Mrp, Mtt, Mtp, Mpp,
- * per N m as in the parameter file) from the record adjoints of the last
- * ysx_spectra_vjp: the source vectors (source_vector_tor / _sph / _rad,
+ * per N m as in the parameter file; 6 per event) from the record adjoints
+ * of the last ysx_spectra_vjp: the source vectors (source_vector_tor / _sph / _rad,
  * src/module_solver.f90:145-215, 566-627, 1747-1847) are linear in the
  * moment tensor, so mm_bar[i] = sum over pairs of <sv_bar, dsv/dM_i>.  After
  * ysx_seismograms_vjp(seis_bar) and ysx_spectra_vjp it is the exact gradient

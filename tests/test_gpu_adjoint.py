@@ -291,3 +291,52 @@ def test_source_model_gradient_vs_central_differences(name, api):
     print(name, "knots", knots, "max rel", f"{err.max():.2e}", "entries", len(got), flush=True)
     assert np.abs(want).max() > 0
     assert err.max() <= 1e-6
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["plain", "batched"])
+def test_events_gradients_match_single_events(mode, api):
+    """Several events in one context: per-event moment-tensor gradients and
+    the summed source-vector model term, after the plain (mode 0) or batched
+    (mode 1) forward, against one-event contexts: bitwise-level after the
+    plain forward, to rounding after the batched one (its shared
+    receiver-side marches reorder the same arithmetic)."""
+    from yspec_amd.synth import batch_events, synth_deck
+
+    ev = batch_events(nev=4, seed=21, nr=3, fmax_range=(3.0, 5.0), tout=40.0, dt=20.0,
+                      lmax_per_mhz=6.0)
+    ev[1].src_depth = 650.0
+    ev[2].src_depth = 1.5
+    ev[3].grav = 0          # per-lane spheroidal and radial solvers
+    deck = synth_deck(0)
+    g = api.YSpec(deck)
+    g.set_events(ev)
+    g.forward_events(mode)
+    rng = np.random.default_rng(9)
+    sbs = [rng.standard_normal(g.event_seismograms(e).shape) for e in range(len(ev))]
+    G = g.events_gradient_mt(sbs)
+    tb = g.source_model_vjp()
+    want_tb = np.zeros_like(tb)
+    for e, p in enumerate(ev):
+        gi = api.YSpec(deck)
+        gi.set_run(p)
+        gi.forward()
+        want = gi.misfit_gradient_mt(sbs[e])
+        want_tb += gi.source_model_vjp()
+        err = np.abs(G[e] - want).max() / np.abs(want).max()
+        print("mode", mode, "event", e, f"moment-tensor gradient rel {err:.2e}", flush=True)
+        assert err <= (1e-13 if mode == 0 else 1e-10)
+    err = np.abs(tb - want_tb).max() / np.abs(want_tb).max()
+    print("mode", mode, f"source model term rel {err:.2e}", flush=True)
+    assert err <= (1e-13 if mode == 0 else 1e-10)
+
+
+def test_events_reject_host_copies(api):
+    from yspec_amd.synth import batch_events, synth_deck
+
+    ev = batch_events(nev=2, seed=22, nr=2, fmax_range=(3.0, 4.0), tout=40.0, dt=20.0,
+                      lmax_per_mhz=6.0)
+    g = api.YSpec(synth_deck(0))
+    g.set_events(ev)
+    g.forward_events(0)
+    with pytest.raises(api.YSpecError):
+        g.spectra_vjp([ev[0].lmin], [1])

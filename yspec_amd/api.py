@@ -176,6 +176,7 @@ class YSpec:
         lon = np.ascontiguousarray(p.lon, dtype=np.float64)
         self._check(self.L.ysx_set_run(self.h, _ip(ip), _dp(fp), _dp(lat), _dp(lon)))
         self.p = p
+        self.events = [p]
         io = np.zeros(8, np.int32)
         fo = np.zeros(8)
         self._check(self.L.ysx_run_info(self.h, _ip(io), _dp(fo)))
@@ -268,11 +269,13 @@ class YSpec:
         return _split_records(rec)
 
     def moment_tensor_vjp(self):
-        """(6,) gradient with respect to Mrr, Mrt, Mrp, Mtt, Mtp, Mpp (per N m)
-        from the record adjoints of the last spectra_vjp."""
-        out = np.zeros(6)
+        """Gradient with respect to Mrr, Mrt, Mrp, Mtt, Mtp, Mpp (per N m) from
+        the record adjoints of the last spectra_vjp: (6,) for one event,
+        (nev, 6) for several."""
+        nev = len(self.events)
+        out = np.zeros(6 * nev)
         self._check(self.L.ysx_moment_tensor_vjp(self.h, _dp(out)))
-        return out
+        return out if nev == 1 else out.reshape(nev, 6)
 
     def source_model_vjp(self):
         """(29, nknot) table adjoint of the source-vector term, from the record
@@ -286,6 +289,16 @@ class YSpec:
         out = np.zeros(3)
         self._check(self.L.ysx_vjp_timings(self.h, _dp(out)))
         return out
+
+    def events_gradient_mt(self, seis_bars):
+        """(nev, 6): d/dM_e of sum_e sum(seis_bar_e * seismograms_e) after
+        forward_events; seis_bars: one (nt_e, 3, nr_e) array per event."""
+        flat = np.concatenate([np.asfortranarray(b, dtype=np.float64).ravel(order="F")
+                               for b in seis_bars])
+        self._check(self.L.ysx_seismograms_vjp(self.h, _dp(flat), None, None, None))
+        z = np.zeros(0, np.int32)
+        self._check(self.L.ysx_spectra_vjp(self.h, None, None, None, 0, _ip(z), _ip(z), None))
+        return self.moment_tensor_vjp().reshape(len(self.events), 6)
 
     def misfit_gradient_mt(self, seis_bar):
         """d/dM of sum(seis_bar * seismograms) after a forward: the three

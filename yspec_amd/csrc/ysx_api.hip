@@ -1140,23 +1140,60 @@ extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j,
 
 namespace {
 
+// per-event blocks of the source-adjoint kernels (256 pairs each)
+int src_blocks(const RunK& k) { return (int)((k.npair + 255) / 256); }
+
 int alloc_adjoint(ysx_ctx* ctx) {
-  const HostRun& R = ctx->R[0];
-  const RunK& k = ctx->rk[0];
-  if (!ctx->d_seisb) HIPCHK(hipMalloc(&ctx->d_seisb, 3 * (size_t)R.nt * R.nr * sizeof(double)));
-  if (!ctx->d_specb) HIPCHK(hipMalloc(&ctx->d_specb, 3 * (size_t)R.nw * R.nr * sizeof(cd)));
-  if (!ctx->d_cfb) HIPCHK(hipMalloc(&ctx->d_cfb, 15 * (size_t)k.lpad * k.jpad * sizeof(cd)));
-  if (!ctx->d_recb) HIPCHK(hipMalloc(&ctx->d_recb, (size_t)k.npair * REC * sizeof(cd)));
-  if (!ctx->d_mmpart)
-    HIPCHK(hipMalloc(&ctx->d_mmpart, 6 * (size_t)((k.npair + 255) / 256) * sizeof(double)));
+  size_t ns = 0, nspec = 0, ncf = 0, nb = 0;
+  for (size_t e = 0; e < ctx->rk.size(); e++) {
+    const HostRun& R = ctx->R[e];
+    const RunK& k = ctx->rk[e];
+    ns += 3 * (size_t)R.nt * R.nr;
+    nspec += 3 * (size_t)R.nw * R.nr;
+    ncf += 15 * (size_t)k.lpad * k.jpad;
+    nb += (size_t)src_blocks(k);
+  }
+  if (!ctx->d_seisb) HIPCHK(hipMalloc(&ctx->d_seisb, ns * sizeof(double)));
+  if (!ctx->d_specb) HIPCHK(hipMalloc(&ctx->d_specb, nspec * sizeof(cd)));
+  if (!ctx->d_cfb) HIPCHK(hipMalloc(&ctx->d_cfb, ncf * sizeof(cd)));
+  if (!ctx->d_recb) HIPCHK(hipMalloc(&ctx->d_recb, (size_t)ctx->npair * REC * sizeof(cd)));
+  if (!ctx->d_mmpart) HIPCHK(hipMalloc(&ctx->d_mmpart, 6 * nb * sizeof(double)));
   return YSX_OK;
 }
 
-int one_event_run(ysx_ctx* ctx) {
+int adjoint_run(ysx_ctx* ctx) {
   if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
-  if (ctx->rk.size() != 1) return set_err(ctx, YSX_ERR_ARG, "several events: one-event runs only");
   HIPCHK(hipSetDevice(ctx->device));
   return alloc_adjoint(ctx);
+}
+
+// The per-block sums of k_source_T / k_source_model_T of every event, added
+// per event in block order: out[6 e + i].
+int source_sums(ysx_ctx* ctx, bool model, std::vector<double>& out) {
+  const int nev = (int)ctx->rk.size();
+  hipStream_t st = ctx->stream;
+  std::vector<size_t> b0(nev + 1, 0);
+  for (int e = 0; e < nev; e++) b0[e + 1] = b0[e] + src_blocks(ctx->rk[e]);
+  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  for (int e = 0; e < nev; e++) {
+    double* part = ctx->d_mmpart + 6 * b0[e];
+    dim3 g(src_blocks(ctx->rk[e]));
+    if (model)
+      launch_source_model_T(g, dim3(256), 0, st, ctx->d_tab, ctx->d_runs, e, ctx->d_recb, part);
+    else
+      launch_source_T(g, dim3(256), 0, st, ctx->d_tab, ctx->d_runs, e, ctx->d_recb, part);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[1], st));
+  HIPCHK(hipEventSynchronize(ctx->ev[1]));
+  if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[2])) return rc;
+  std::vector<double> part(6 * b0[nev]);
+  HIPCHK(hipMemcpy(part.data(), ctx->d_mmpart, part.size() * sizeof(double), hipMemcpyDeviceToHost));
+  out.assign(6 * (size_t)nev, 0.0);
+  for (int e = 0; e < nev; e++)
+    for (int i = 0; i < 6; i++)
+      for (size_t b = b0[e]; b < b0[e + 1]; b++) out[6 * (size_t)e + i] += part[6 * b + i];
+  return YSX_OK;
 }
 
 }  // namespace
@@ -1164,30 +1201,36 @@ int one_event_run(ysx_ctx* ctx) {
 extern "C" int ysx_seismograms_vjp(ysx_ctx* ctx, const double* seis_bar, double* ur_bar,
                                    double* ut_bar, double* up_bar) {
   if (!ctx || !seis_bar) return YSX_ERR_ARG;
-  if (int rc = one_event_run(ctx)) return rc;
-  const HostRun& R = ctx->R[0];
+  if (int rc = adjoint_run(ctx)) return rc;
+  const int nev = (int)ctx->rk.size();
+  if (nev > 1 && (ur_bar || ut_bar || up_bar))
+    return set_err(ctx, YSX_ERR_ARG, "several events: the spectra adjoints stay on the device");
+  size_t ns = 0;
+  for (const HostRun& R : ctx->R) ns += 3 * (size_t)R.nt * R.nr;
   hipStream_t st = ctx->stream;
-  HIPCHK(hipMemcpy(ctx->d_seisb, seis_bar, 3 * (size_t)R.nt * R.nr * sizeof(double),
-                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx->d_seisb, seis_bar, ns * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
-  const int bk = std::min(R.nt, FFT_B), nblk = R.nt / bk;
-  if (nblk <= FFT_G) {  // three passes, the transpose of k_fft_lds + k_fft_post
-    launch_post_T(dim3(R.nr * ((bk + 255) / 256)), dim3(256), 0, st, ctx->d_runs, 0,
-                  ctx->d_seisb, ctx->d_tw, ctx->d_azep, ctx->d_work);
-    launch_fft_lds_T(dim3(R.nr * 3 * nblk), dim3(512), 0, st, ctx->d_runs, 0, ctx->d_tw,
-                     ctx->d_work);
-    launch_pack_T(dim3(R.nr * 3 * ((R.nw + 255) / 256)), dim3(256), 0, st, ctx->d_runs, 0,
-                  ctx->d_work, ctx->d_specb);
-  } else {
-    launch_seis_T(dim3(std::max(1, R.nr)), dim3(1024), 0, st, ctx->d_runs, 0, ctx->d_seisb,
-                  ctx->d_tw, ctx->d_work, ctx->d_azep, ctx->d_specb);
+  for (int e = 0; e < nev; e++) {
+    const HostRun& R = ctx->R[e];
+    const int bk = std::min(R.nt, FFT_B), nblk = R.nt / bk;
+    if (nblk <= FFT_G) {  // three passes, the transpose of k_fft_lds + k_fft_post
+      launch_post_T(dim3(R.nr * ((bk + 255) / 256)), dim3(256), 0, st, ctx->d_runs, e,
+                    ctx->d_seisb, ctx->d_tw, ctx->d_azep, ctx->d_work);
+      launch_fft_lds_T(dim3(R.nr * 3 * nblk), dim3(512), 0, st, ctx->d_runs, e, ctx->d_tw,
+                       ctx->d_work);
+      launch_pack_T(dim3(R.nr * 3 * ((R.nw + 255) / 256)), dim3(256), 0, st, ctx->d_runs, e,
+                    ctx->d_work, ctx->d_specb);
+    } else {
+      launch_seis_T(dim3(std::max(1, R.nr)), dim3(1024), 0, st, ctx->d_runs, e, ctx->d_seisb,
+                    ctx->d_tw, ctx->d_work, ctx->d_azep, ctx->d_specb);
+    }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev[1], st));
   HIPCHK(hipEventSynchronize(ctx->ev[1]));
   if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[0])) return rc;
   ctx->have_specb = true;
-  size_t n = (size_t)R.nw * R.nr;
+  size_t n = (size_t)ctx->R[0].nw * ctx->R[0].nr;
   double* outs[3] = {ur_bar, ut_bar, up_bar};
   for (int c = 0; c < 3; c++)
     if (outs[c])
@@ -1202,15 +1245,18 @@ extern "C" int ysx_spectra_vjp(ysx_ctx* ctx, const double* ur_bar, const double*
   const bool given = ur_bar || ut_bar || up_bar;
   if (given && !(ur_bar && ut_bar && up_bar))
     return set_err(ctx, YSX_ERR_ARG, "ur_bar, ut_bar, up_bar: give all three or none");
-  if (int rc = one_event_run(ctx)) return rc;
+  if (int rc = adjoint_run(ctx)) return rc;
   if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra computed");
-  const HostRun& R = ctx->R[0];
-  const RunK& k = ctx->rk[0];
+  const int nev = (int)ctx->rk.size();
+  if (nev > 1 && (given || n > 0))
+    return set_err(ctx, YSX_ERR_ARG,
+                   "several events: spectra adjoints from the device, no per-pair copies");
+  const RunK& k0 = ctx->rk[0];
   for (int i = 0; i < n; i++)
-    if (l[i] < k.lmin || l[i] > k.lmax || j[i] < 1 || j[i] > k.nw)
+    if (l[i] < k0.lmin || l[i] > k0.lmax || j[i] < 1 || j[i] > k0.nw)
       return set_err(ctx, YSX_ERR_ARG, "pair outside the run");
-  size_t ns = (size_t)R.nw * R.nr;
   if (given) {
+    size_t ns = (size_t)ctx->R[0].nw * ctx->R[0].nr;
     const double* ins[3] = {ur_bar, ut_bar, up_bar};
     for (int c = 0; c < 3; c++)
       HIPCHK(hipMemcpy(ctx->d_specb + c * ns, ins[c], ns * sizeof(cd), hipMemcpyHostToDevice));
@@ -1219,19 +1265,22 @@ extern "C" int ysx_spectra_vjp(ysx_ctx* ctx, const double* ur_bar, const double*
     return set_err(ctx, YSX_ERR_ARG, "no spectra adjoint: call ysx_seismograms_vjp first");
   }
   hipStream_t st = ctx->stream;
-  int ntile = (k.lpad + 15) / 16 * (k.jpad / 16);
   HIPCHK(hipEventRecord(ctx->ev[0], st));
-  launch_lsum_T(dim3(ntile), dim3(256), 0, st, ctx->d_runs, 0, ctx->d_tb, ctx->d_eia,
-                ctx->d_specb, ctx->d_cfb);
-  launch_assemble_T(dim3((unsigned)((k.npair + ASMT_P - 1) / ASMT_P)), dim3(64), 0, st, ctx->d_runs, 0,
-                    ctx->d_rec, ctx->d_cfb, ctx->d_recb);
+  for (int e = 0; e < nev; e++) {
+    const RunK& k = ctx->rk[e];
+    int ntile = (k.lpad + 15) / 16 * (k.jpad / 16);
+    launch_lsum_T(dim3(ntile), dim3(256), 0, st, ctx->d_runs, e, ctx->d_tb, ctx->d_eia,
+                  ctx->d_specb, ctx->d_cfb);
+    launch_assemble_T(dim3((unsigned)((k.npair + ASMT_P - 1) / ASMT_P)), dim3(64), 0, st,
+                      ctx->d_runs, e, ctx->d_rec, ctx->d_cfb, ctx->d_recb);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev[1], st));
   HIPCHK(hipEventSynchronize(ctx->ev[1]));
   if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[1])) return rc;
   ctx->have_recb = true;
   for (int i = 0; i < n; i++) {
-    size_t p = (size_t)(l[i] - k.lmin) * k.nw + (j[i] - 1);
+    size_t p = (size_t)(l[i] - k0.lmin) * k0.nw + (j[i] - 1);
     HIPCHK(hipMemcpy(rec_bar + (size_t)i * 2 * REC, ctx->d_recb + p * REC, REC * sizeof(cd),
                      hipMemcpyDeviceToHost));
   }
@@ -1240,66 +1289,48 @@ extern "C" int ysx_spectra_vjp(ysx_ctx* ctx, const double* ur_bar, const double*
 
 extern "C" int ysx_moment_tensor_vjp(ysx_ctx* ctx, double* mm_bar) {
   if (!ctx || !mm_bar) return YSX_ERR_ARG;
-  if (int rc = one_event_run(ctx)) return rc;
+  if (int rc = adjoint_run(ctx)) return rc;
   if (!ctx->have_recb) return set_err(ctx, YSX_ERR_ARG, "no record adjoints: call ysx_spectra_vjp first");
-  const RunK& k = ctx->rk[0];
-  int nb = (int)((k.npair + 255) / 256);
-  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-  launch_source_T(dim3(nb), dim3(256), 0, ctx->stream, ctx->d_tab, ctx->d_runs, 0, ctx->d_recb,
-                  ctx->d_mmpart);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
-  HIPCHK(hipEventSynchronize(ctx->ev[1]));
-  if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[2])) return rc;
-  std::vector<double> part(6 * (size_t)nb);
-  HIPCHK(hipMemcpy(part.data(), ctx->d_mmpart, part.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int i = 0; i < 6; i++) {
-    double a = 0.0;
-    for (int b = 0; b < nb; b++) a += part[6 * (size_t)b + i];
-    mm_bar[i] = a / ctx->N.moment_norm;  // normalised -> parameter-file units
-  }
+  std::vector<double> G;
+  if (int rc = source_sums(ctx, false, G)) return rc;
+  for (size_t i = 0; i < G.size(); i++)
+    mm_bar[i] = G[i] / ctx->N.moment_norm;  // normalised -> parameter-file units
   return YSX_OK;
 }
 
 extern "C" int ysx_source_model_vjp(ysx_ctx* ctx, double* table_bar) {
   if (!ctx || !table_bar) return YSX_ERR_ARG;
-  if (int rc = one_event_run(ctx)) return rc;
+  if (int rc = adjoint_run(ctx)) return rc;
   if (!ctx->have_recb) return set_err(ctx, YSX_ERR_ARG, "no record adjoints: call ysx_spectra_vjp first");
-  const RunK& k = ctx->rk[0];
-  const int nb = (int)((k.npair + 255) / 256), nk = k.nknot;
-  launch_source_model_T(dim3(nb), dim3(256), 0, ctx->stream, ctx->d_tab, ctx->d_runs, 0,
-                        ctx->d_recb, ctx->d_mmpart);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  std::vector<double> part(6 * (size_t)nb);
-  HIPCHK(hipMemcpy(part.data(), ctx->d_mmpart, part.size() * sizeof(double), hipMemcpyDeviceToHost));
-  double G[6];
-  for (int i = 0; i < 6; i++) {
-    G[i] = 0.0;
-    for (int b = 0; b < nb; b++) G[i] += part[6 * (size_t)b + i];
-  }
-  // the spline weights of the source radius on knot interval is (1-based),
-  // as spl_w computes them; every pair reads the same six values
-  double r[2];
-  HIPCHK(hipMemcpy(r, ctx->d_tab + (size_t)F_R * nk + (k.is - 1), 2 * sizeof(double),
-                   hipMemcpyDeviceToHost));
-  const double h = r[1] - r[0];
-  const bool deg = (h == 0.0);
-  const double ih = deg ? 0.0 : 1.0 / h, a = (r[1] - k.rs) * ih, bw = (k.rs - r[0]) * ih;
-  const double hh = h * h / 6.0, a3 = (a * a * a - a) * hh, b3 = (bw * bw * bw - bw) * hh;
+  std::vector<double> G;
+  if (int rc = source_sums(ctx, true, G)) return rc;
+  const int nk = ctx->rk[0].nknot;
   std::fill(table_bar, table_bar + (size_t)F_COUNT * nk, 0.0);
   const int fields[6] = {F_LCON, F_CCON, F_FCON, F_QMU, F_XLAM, F_XA2};
-  for (int i = 0; i < 6; i++) {
-    double* y = table_bar + (size_t)fields[i] * nk + (k.is - 1);
-    double* y2 = table_bar + (size_t)(fields[i] + 1) * nk + (k.is - 1);
-    if (deg) {
-      y[0] += G[i];
-      continue;
+  for (size_t e = 0; e < ctx->rk.size(); e++) {
+    const RunK& k = ctx->rk[e];
+    // the spline weights of the event's source radius on its knot interval
+    // is (1-based), as spl_w computes them; every pair reads the same values
+    double r[2];
+    HIPCHK(hipMemcpy(r, ctx->d_tab + (size_t)F_R * nk + (k.is - 1), 2 * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    const double h = r[1] - r[0];
+    const bool deg = (h == 0.0);
+    const double ih = deg ? 0.0 : 1.0 / h, a = (r[1] - k.rs) * ih, bw = (k.rs - r[0]) * ih;
+    const double hh = h * h / 6.0, a3 = (a * a * a - a) * hh, b3 = (bw * bw * bw - bw) * hh;
+    for (int i = 0; i < 6; i++) {
+      const double g = G[6 * e + i];
+      double* y = table_bar + (size_t)fields[i] * nk + (k.is - 1);
+      double* y2 = table_bar + (size_t)(fields[i] + 1) * nk + (k.is - 1);
+      if (deg) {
+        y[0] += g;
+        continue;
+      }
+      y[0] += a * g;
+      y[1] += bw * g;
+      y2[0] += a3 * g;
+      y2[1] += b3 * g;
     }
-    y[0] += a * G[i];
-    y[1] += bw * G[i];
-    y2[0] += a3 * G[i];
-    y2[1] += b3 * G[i];
   }
   return YSX_OK;
 }
