@@ -141,6 +141,15 @@ def batched_compare(deck, steps, warmup, do_tune=True):
     rows = {brow_rows(e) for e in ev}
     assert len(rows) == 1, "events with and without the potential/tilt correction"
     ode = ode_flops(counts, brows=rows.pop())
+    # per-event moment-tensor gradients after the batched forward (the
+    # records are that forward's): kernel time of the three adjoint stages
+    rng = np.random.default_rng(11)
+    sbs = [rng.standard_normal((g.info[e]["nt"], 3, p.nr)) for e, p in enumerate(ev)]
+    g.forward_events(1)
+    g.events_gradient_mt(sbs)
+    grad_ms = float(g.vjp_timings().sum())
+    print(f"# moment-tensor gradients of all events: {grad_ms:.2f} ms of kernels",
+          file=sys.stderr, flush=True)
     g.close()
     return {
         "workload": "64 events on the same deck: depth U[0,700] km, fmax U[10,40] mHz, "
@@ -156,6 +165,12 @@ def batched_compare(deck, steps, warmup, do_tune=True):
         "batched_ode_tflops_effective": sum(ode.values()) / (res["batched"]["ms_per_step"]
                                                              * 1e-3) / 1e12,
         "tuning": tuning["best"] if tuning else None,
+        "moment_tensor_gradients": {
+            "kernel_ms": grad_ms,
+            "over_batched_forward_ms": grad_ms / res["batched"]["ms_per_step"],
+            "note": "per-event d/dM of sum(seis_bar_e * seismograms_e) from the batched "
+                    "forward's records; seis_bar upload excluded",
+        },
     }
 
 
