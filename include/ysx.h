@@ -212,7 +212,7 @@ int ysx_source_model_vjp(ysx_ctx *ctx, double *table_bar);
 
 /* Kernel milliseconds (HIP events on the context's stream, host copies
  * excluded) of the last call of [0] ysx_seismograms_vjp, [1] ysx_spectra_vjp,
- * [2] ysx_moment_tensor_vjp. */
+ * [2] ysx_moment_tensor_vjp or ysx_source_model_vjp, whichever ran last. */
 int ysx_vjp_timings(const ysx_ctx *ctx, double *ms3);
 
 /* Timings of the last ysx_spectra / ysx_seismograms, milliseconds, from

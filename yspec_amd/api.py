@@ -285,7 +285,8 @@ class YSpec:
         return out
 
     def vjp_timings(self):
-        """kernel ms of the last seismograms_vjp, spectra_vjp, moment_tensor_vjp"""
+        """kernel ms of the last seismograms_vjp, spectra_vjp, and
+        moment_tensor_vjp or source_model_vjp (whichever ran last)"""
         out = np.zeros(3)
         self._check(self.L.ysx_vjp_timings(self.h, _dp(out)))
         return out
