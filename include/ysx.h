@@ -62,6 +62,21 @@ int ysx_model_table(const ysx_ctx *ctx, double *out);
 int ysx_model_table_host(int nknot, int nic, int noc, int ians,
                          double tref_period, const double *deck, double *out);
 
+/* Allocation-free form of ysx_set_model followed by ysx_model_table.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  The
+ * deck is host memory as for ysx_set_model; d_table is caller-owned device
+ * memory on the context's device for F_COUNT * nknot doubles, which receives
+ * the table ysx_model_table would return (the same kernel, the same values).
+ * The transform's work arrays live in a fixed 2 MiB device array per GPU,
+ * held until the process ends; a deck that needs more (large knot counts,
+ * more than 64 layers) is rejected with YSX_ERR_ARG and a message.  Makes no
+ * device allocation; does not touch the context's model.  Synchronous on the
+ * context's stream.  Calls that overlap on one GPU (several threads or
+ * contexts) race on that array. */
+int ysx_model_table_noalloc(ysx_ctx *ctx, int nknot, int nic, int noc, int ians,
+                            double tref_period, const double *deck, double *d_table);
+
 /* Adjoint of the deck -> model-table transform (no counterpart in the
  * reference; the first stage of a misfit gradient): deck_bar (nknot x 9,
  * the deck's layout) = J^T table_bar for table = ysx_model_table_host(deck)

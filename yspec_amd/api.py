@@ -28,7 +28,7 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_model_table_host", "ysx_model_table_vjp",
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts",
            "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
-           "ysx_vjp_timings", "ysx_source_model_vjp")
+           "ysx_vjp_timings", "ysx_source_model_vjp", "ysx_model_table_noalloc")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -92,6 +92,8 @@ def lib():
         L.ysx_moment_tensor_vjp.argtypes = [vp, D]
         L.ysx_vjp_timings.argtypes = [vp, D]
         L.ysx_source_model_vjp.argtypes = [vp, D]
+        L.ysx_model_table_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                              C.c_double, D, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -313,6 +315,15 @@ class YSpec:
         out = np.zeros((len(TABLE_FIELDS), self.deck.nknot))
         self._check(self.L.ysx_model_table(self.h, _dp(out)))
         return out
+
+    def model_table_noalloc(self, deck, d_table):
+        """The model table of `deck` written into the caller's device buffer
+        d_table (address of F_COUNT * nknot doubles on this context's device)
+        by ysx_model_table_noalloc; the context's own model is untouched."""
+        cols = np.ascontiguousarray(deck.cols, dtype=np.float64)
+        self._check(self.L.ysx_model_table_noalloc(self.h, deck.nknot, deck.nic, deck.noc,
+                                                   deck.ians, deck.tref, _dp(cols),
+                                                   C.c_void_p(int(d_table))))
 
     def timings(self):
         """ms: source, toroidal, minors, b-vector, radial, assemble, l-sum, IFFT"""

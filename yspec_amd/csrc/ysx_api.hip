@@ -33,6 +33,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -118,6 +119,7 @@ struct ysx_ctx {
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev[16] = {};
   double ms[8] = {0};
+  double* scratch = nullptr;  // this device's copy of `scratch` (ysx_model_table_noalloc)
 };
 
 namespace {
@@ -443,6 +445,37 @@ int refine_work(ysx_ctx* c) {
 
 }  // namespace
 
+// ----------------------------------------------- allocation-free model table
+// Device scratch of ysx_model_table_noalloc: the staged deck, the layer
+// lists and the spline/gravity work arrays of k_model.  Every GPU holds its
+// own copy at its own address until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns for that device
+// (scratch_of), never through the array's name.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)2 << 20;  // 2 MiB
+constexpr int NOALLOC_MAX_LAYERS = 64;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// the address of `scratch` on `device`, looked up once per device with that
+// device current (the caller has made it current)
+hipError_t scratch_of(int device, double** ptr) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
 extern "C" int ysx_create(ysx_ctx** out, int device) {
   if (!out) return YSX_ERR_ARG;
   ysx_ctx* ctx = new ysx_ctx();
@@ -457,6 +490,7 @@ extern "C" int ysx_create(ysx_ctx** out, int device) {
   HIPCHK(hipGetDeviceProperties(&prop, device));
   ctx->lds_max = prop.sharedMemPerBlock;
   ctx->ncu = prop.multiProcessorCount;
+  HIPCHK(scratch_of(device, &ctx->scratch));
   return YSX_OK;
 }
 
@@ -519,7 +553,7 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
     if (e1 == hipSuccess) e1 = hipMemset(ctx->d_tab, 0, ctx->lds_bytes);
     if (e1 == hipSuccess) {
       launch_model(dim3(1), dim3(256), 0, ctx->stream, d_deck, nknot, M.ifanis,
-                         ctx->N.vel_norm, d_lay, d_lay + nl, nl, ctx->d_tab, d_work,
+                         ctx->N.vel_norm, d_lay, d_lay + nl, nl, ctx->d_tab, d_work, ws,
                          d_work + nwork, d_work + nwork + nknot, d_work + nwork + 2 * (size_t)nknot);
       e1 = hipGetLastError();
     }
@@ -537,6 +571,61 @@ extern "C" int ysx_model_table(const ysx_ctx* ctx, double* out) {
   hipError_t e = hipSetDevice(ctx->device);
   if (e == hipSuccess) e = hipMemcpy(out, ctx->d_tab, ctx->lds_bytes, hipMemcpyDeviceToHost);
   return e == hipSuccess ? YSX_OK : YSX_ERR_HIP;
+}
+
+extern "C" int ysx_model_table_noalloc(ysx_ctx* ctx, int nknot, int nic, int noc, int ians,
+                                       double tref_period, const double* deck,
+                                       double* d_table) {
+  if (!ctx || !deck || !d_table) return YSX_ERR_ARG;
+  if (!ctx->scratch) return set_err(ctx, YSX_ERR_ARG, "no device scratch for this context");
+  if (nknot < 3 || nic < 1 || noc <= nic || noc >= nknot)
+    return set_err(ctx, YSX_ERR_INPUT, "bad deck header");
+  // layers split at duplicated radii (src/module_model.f90:294-308)
+  int lay[2 * NOALLOC_MAX_LAYERS];
+  int nl = 0, lo = 0, nmax = 0;
+  for (int i = 1; i <= nknot; i++) {
+    // compared after normalisation, as build_model compares them
+    if (i == nknot || deck[9 * (size_t)i] / R_NORM_M == deck[9 * (size_t)(i - 1)] / R_NORM_M) {
+      if (nl == NOALLOC_MAX_LAYERS)
+        return set_err(ctx, YSX_ERR_ARG,
+                       "ysx_model_table_noalloc: more than 64 layers in the deck");
+      lay[nl] = lo;
+      lay[NOALLOC_MAX_LAYERS + nl] = i - lo;
+      nmax = std::max(nmax, i - lo);
+      nl++;
+      lo = i;
+    }
+  }
+  // scratch layout (doubles): deck [9 nknot] | work [13 nl ws] | intg, qint,
+  // gi [nknot each] | layer_lo, layer_n [nl ints each]
+  const size_t ws = 3 * ((size_t)nmax + 1);
+  const size_t o_work = 9 * (size_t)nknot, o_intg = o_work + 13 * (size_t)nl * ws,
+               o_lay = o_intg + 3 * (size_t)nknot;
+  const size_t need = (o_lay * sizeof(double) + 2 * (size_t)nl * sizeof(int));
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return set_err(ctx, YSX_ERR_ARG,
+                   "ysx_model_table_noalloc: this deck needs " + std::to_string(need) +
+                       " bytes of device scratch, more than its fixed " +
+                       std::to_string(NOALLOC_SCRATCH_BYTES) + " (use ysx_set_model)");
+  HIPCHK(hipSetDevice(ctx->device));
+  double* sc = ctx->scratch;
+  int* slay = reinterpret_cast<int*>(sc + o_lay);
+  HIPCHK(hipMemcpyAsync(sc, deck, 9 * (size_t)nknot * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(slay, lay, nl * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(slay + nl, lay + NOALLOC_MAX_LAYERS, nl * sizeof(int),
+                        hipMemcpyHostToDevice, ctx->stream));
+  // the fields k_model does not write are zero, as in ysx_set_model
+  const size_t ntab = (size_t)F_COUNT * nknot;
+  launch_fill0(dim3((unsigned)std::min<size_t>(1024, (ntab + 255) / 256)), dim3(256), 0,
+               ctx->stream, d_table, ntab);
+  HIPCHK(hipGetLastError());
+  launch_model(dim3(1), dim3(256), 0, ctx->stream, sc, nknot, ians == 1, ctx->N.vel_norm, slay,
+               slay + nl, nl, d_table, sc + o_work, ws, sc + o_intg, sc + o_intg + nknot,
+               sc + o_intg + 2 * (size_t)nknot);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return YSX_OK;
 }
 
 extern "C" int ysx_model_table_host(int nknot, int nic, int noc, int ians, double tref_period,

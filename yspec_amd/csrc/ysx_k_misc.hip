@@ -13,6 +13,14 @@ __global__ void k_reset(int* counters, int n) {
   if ((int)threadIdx.x < n) counters[threadIdx.x] = 0;
 }
 
+// zero n doubles (a kernel rather than a memset node, so that the call can
+// sit in a captured graph)
+__global__ void k_fill0(double* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
 // -------------------------------------------------------------- k_source
 // source_vector_tor / _sph / _rad, src/module_solver.f90:145-215,
 // 566-627, 1747-1847.  source_vector_sph keeps only the real part of its
@@ -651,19 +659,18 @@ __global__ void __launch_bounds__(64) k_legendre(const RunK* __restrict__ runs, 
 // natural splines per layer, gravity quadrature with the reference's
 // sequential accumulation (src/module_model.f90:253-378), through the same
 // functions as the host (ysx_model.hpp).  layer_lo (0-based first knot)
-// and layer_n per layer; work holds 3 (nknot + 1) doubles per (layer,
-// field) solve; gi holds nknot doubles.
+// and layer_n per layer; work holds ws >= 3 (largest layer_n + 1) doubles
+// per (layer, field) solve, 13 nlayer solves; gi holds nknot doubles.
 __global__ void __launch_bounds__(256) k_model(const double* __restrict__ deck, int nknot,
                                                int ifanis, double vel_norm,
                                                const int* __restrict__ layer_lo,
                                                const int* __restrict__ layer_n, int nlayer,
                                                double* __restrict__ tab, double* __restrict__ work,
-                                               double* __restrict__ intg,
+                                               size_t ws, double* __restrict__ intg,
                                                double* __restrict__ qint,
                                                double* __restrict__ gi) {
 #pragma clang fp contract(off)
   int tid = threadIdx.x, nt = blockDim.x;
-  size_t ws = 3 * ((size_t)nknot + 1);
   for (int i = tid; i < nknot; i += nt) knot_fields(deck, nknot, i, ifanis, vel_norm, tab);
   __syncthreads();
   const double* r = tab + (size_t)F_R * nknot;
@@ -1555,9 +1562,13 @@ void launch_legendre(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs
 }
 void launch_model(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck, int nknot,
                   int ifanis, double vel_norm, const int* layer_lo, const int* layer_n,
-                  int nlayer, double* tab, double* work, double* intg, double* qint, double* gi) {
+                  int nlayer, double* tab, double* work, size_t ws, double* intg, double* qint,
+                  double* gi) {
   hipLaunchKernelGGL(k_model, g, b, lds, s, deck, nknot, ifanis, vel_norm, layer_lo, layer_n,
-                     nlayer, tab, work, intg, qint, gi);
+                     nlayer, tab, work, ws, intg, qint, gi);
+}
+void launch_fill0(dim3 g, dim3 b, size_t lds, hipStream_t s, double* p, size_t n) {
+  hipLaunchKernelGGL(k_fill0, g, b, lds, s, p, n);
 }
 void launch_seis(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
                  const cd* spec, const cd* tw, cd* work, const double* azep, double* seis) {

@@ -271,7 +271,9 @@ void launch_legendre(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs
                      const double* th, double* tb);
 void launch_model(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck, int nknot,
                   int ifanis, double vel_norm, const int* layer_lo, const int* layer_n,
-                  int nlayer, double* tab, double* work, double* intg, double* qint, double* gi);
+                  int nlayer, double* tab, double* work, size_t ws, double* intg, double* qint,
+                  double* gi);
+void launch_fill0(dim3 g, dim3 b, size_t lds, hipStream_t s, double* p, size_t n);
 void launch_seis(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
                  const cd* spec, const cd* tw, cd* work, const double* azep, double* seis);
 void launch_fft_lds(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
