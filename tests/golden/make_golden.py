@@ -18,6 +18,10 @@ only, one degree, a ragged degree range, a receiver at the epicentre and
 one near the antipode).
 
     python tests/golden/make_golden.py [case names]   # all cases by default
+    python tests/golden/make_golden.py cfg2_records   # cfg2_records.npz:
+        per-pair records of the lmax = 600, fmax = 40 mHz workload at
+        l = 0, 1, 2, 150, 300, 450, 600 (a spread subset of each degree's
+        frequencies)
 """
 from __future__ import annotations
 
@@ -34,7 +38,7 @@ sys.path[:0] = [ROOT, HERE]
 
 from yspec_amd.inputs import Deck, read_deck, read_params  # noqa: E402
 from yspec_amd.synth import synth_deck, workload  # noqa: E402
-from refdump import read_dump, MODEL_NAMES  # noqa: E402
+from refdump import read_dump, MODEL_NAMES, PAIR_FIELDS as PAIR_FIELDS_REF  # noqa: E402
 
 REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
 REF_YSPEC = os.path.join(ROOT, "oracle", "_ref", "yspec")
@@ -100,6 +104,11 @@ def cases():
                                                     rec_depth=150.0, **small)
     yield "cow_fic_acc", fic_deck(0), workload("cfg1", grav=1, out=2, cor=1,
                                                src_depth=100.0, **small)
+    # (self-gravitation with a fluid inner core and the receiver below the
+    # source has no case: the reference starts solid minors in the fluid
+    # inner core on that branch, src/module_solver.f90:1323-1331, and its
+    # program did not finish such a run, fic_deck(0) with a 20 km source and
+    # the receiver at 150 km, within 40 minutes)
     yield ("prem_fic_example", *prem_example("prem.200.fic"))
     yield ("prem_example", *prem_example("prem.200"))
     yield ("prem_noocean_disp", *prem_example("prem.200.no", ats=0, out=0, cor=0,
@@ -172,10 +181,59 @@ def run_case(name, deck, p, outdir, keep_pairs=240):
     return path, os.path.getsize(path)
 
 
+# degrees of the lmax = 600, fmax = 40 mHz workload whose per-pair records
+# are kept (every frequency is run; a spread subset is stored)
+CFG2_DEGREES = (0, 1, 2, 150, 300, 450, 600)
+
+
+def cfg2_records(outdir, keep_per_l=44):
+    """Per-pair records of the lmax = 600, fmax = 40 mHz workload (seeded
+    deck, attenuation, self-gravitation, 600 km source) from the reference,
+    one ref_dump run per degree l (lmin = lmax = l, all frequencies, two
+    receivers), a spread subset of the frequencies of each l kept."""
+    deck = synth_deck(0)
+    base = workload("cfg2")
+    keep = {k: [] for k in ("l", "j", "tor_start", "sph_start")}
+    recs = []
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "model.deck"), "w") as f:
+            f.write(deck.to_text())
+        for l in CFG2_DEGREES:
+            p = workload("cfg2", lmin=l, lmax=l, nr=2, model_file="model.deck")
+            p.prefix = "out"
+            with open(os.path.join(td, "in.txt"), "w") as f:
+                f.write(p.to_text())
+            dd = os.path.join(td, f"dump{l}")
+            os.makedirs(dd)
+            subprocess.run([REF_DUMP, "in.txt", dd], cwd=td, check=True,
+                           stdout=subprocess.DEVNULL)
+            P = read_dump(dd)["pairs"]
+            n = len(P["l"])
+            sel = np.unique(np.linspace(0, n - 1, min(keep_per_l, n)).astype(int))
+            for k in keep:
+                keep[k].append(P[k][sel])
+            recs.append(np.concatenate([P[name][sel].reshape(len(sel), -1, order="F")
+                                        for name, _ in PAIR_FIELDS_REF], axis=1))
+    out = dict(deck_cols=deck.cols,
+               deck_head=np.array([deck.nknot, deck.nic, deck.noc, deck.ians]),
+               deck_tref=np.array(deck.tref), ip=base.int_block(), fp=base.float_block(),
+               lat=base.lat, lon=base.lon, rec=np.concatenate(recs),
+               **{"pair_" + k: np.concatenate(v) for k, v in keep.items()})
+    path = os.path.join(outdir, "cfg2_records.npz")
+    np.savez_compressed(path, **out)
+    return path, os.path.getsize(path)
+
+
 def main():
     if not os.path.exists(REF_DUMP):
         sys.exit("build the reference first: make -C oracle ref")
     only = set(sys.argv[1:])
+    if "cfg2_records" in only:
+        path, size = cfg2_records(HERE)
+        print(f"{os.path.basename(path)}: {size / 1024:.1f} KiB")
+        only.discard("cfg2_records")
+        if not only:
+            return
     for name, deck, p in cases():
         if only and name not in only:
             continue

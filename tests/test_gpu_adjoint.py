@@ -208,6 +208,16 @@ def test_moment_tensor_vjp_needs_record_adjoints(api):
     g.forward()
     with pytest.raises(api.YSpecError):
         g.moment_tensor_vjp()
+    # record adjoints belong to the forward whose records they were built
+    # from: a new forward invalidates them
+    g.seismograms_vjp(np.ones((g.nt, 3, p.nr)))
+    g.spectra_vjp([2], [1])
+    g.moment_tensor_vjp()
+    g.forward()
+    with pytest.raises(api.YSpecError, match="no record adjoints"):
+        g.moment_tensor_vjp()
+    with pytest.raises(api.YSpecError, match="no record adjoints"):
+        g.source_model_vjp()
 
 
 def test_autograd_misfit_vs_central_differences(api):

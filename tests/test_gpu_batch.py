@@ -148,3 +148,63 @@ def test_shared_marches_edge_cases(api, oracle_mod):
             assert rel_l2(a, b) <= 1e-9, (e, p.src_depth)
         assert rel_l2(g.event_seismograms(e), o.seismograms(*os_)) <= 1e-8, (e, p.src_depth)
     g.close()
+
+
+def test_64_event_workload(api, oracle_mod):
+    """The 64-event workload the batched benchmark times (depth U[0,700] km,
+    fmax U[10,40] mHz, lmax = 15 fmax, 16 receivers each, 180 min at 1 s):
+    the batched path under each launch shape the tuner may pick (one block
+    per item; persistent blocks) gives every event its plain-path spectra
+    and seismograms to 1e-12, and the three smallest events match the
+    oracle at 1e-9 (spectra) / 1e-8 (seismograms)."""
+    ev = batch_events()
+    deck = synth_deck(0)
+    g = api.YSpec(deck)
+    g.set_events(ev)
+    g.forward_events(0)
+    plain = [(g.event_spectra(e), g.event_seismograms(e)) for e in range(len(ev))]
+    for shape in ((0, 1, 4), (1, 1, 4), (1, 4, 8)):
+        g.set_batch_tuning(*shape)
+        g.forward_events(1)
+        worst = 0.0
+        for e, (spec, seis) in enumerate(plain):
+            for a, b in zip(g.event_spectra(e), spec):
+                worst = max(worst, rel_l2(a, b))
+            worst = max(worst, rel_l2(g.event_seismograms(e), seis))
+        print("64 events, batched shape", shape, f"worst {worst:.2e}", flush=True)
+        assert worst <= 1e-12, shape
+    npairs = [g.info[e]["npair"] for e in range(len(ev))]
+    for e in np.argsort(npairs)[:3]:
+        o = oracle_mod.Oracle(deck, ev[e])
+        os_ = o.spectra(nthreads=16)
+        es = max(rel_l2(a, b) for a, b in zip(plain[e][0], os_))
+        ek = rel_l2(plain[e][1], o.seismograms(*os_))
+        print(f"event {e}: {npairs[e]} pairs, oracle spectra {es:.2e} seis {ek:.2e}",
+              flush=True)
+        assert es <= 1e-9 and ek <= 1e-8, e
+    g.close()
+
+
+def test_fluid_core_receiver_below_source_is_rejected(api):
+    """Self-gravitation, a fluid inner core and the receiver below the
+    source: the reference's sph_solver_fic starts solid minors in the fluid
+    inner core on that branch (src/module_solver.f90:1323-1331) and gives no
+    result to match, so the run is refused with YSX_ERR_UNSUPPORTED (4)
+    rather than computed; the same deck with the receiver above the source,
+    or l = 0 only, is accepted."""
+    from make_golden import fic_deck
+    from yspec_amd.synth import workload
+
+    deck = fic_deck(0)
+    g = api.YSpec(deck)
+    below = workload("cfg1", lmax=8, nr=1, src_depth=20.0, rec_depth=150.0)
+    with pytest.raises(api.YSpecError) as ei:
+        g.set_run(below)
+    assert ei.value.code == 4 and "fluid inner core" in ei.value.msg
+    ev = small_events()[:2]
+    ev[1].grav, ev[1].src_depth, ev[1].rec_depth = 2, 20.0, 150.0
+    with pytest.raises(api.YSpecError, match="event 1"):
+        g.set_events(ev)
+    g.set_run(workload("cfg1", lmax=8, nr=1, src_depth=150.0, rec_depth=20.0))
+    g.set_run(workload("cfg1", lmin=0, lmax=0, nr=1, src_depth=20.0, rec_depth=150.0))
+    g.close()

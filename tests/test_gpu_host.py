@@ -56,3 +56,29 @@ def test_host_program_matches_reference_output(name, tmp_path):
             bumped += 1
     total = p.nr * int(z["nout"])
     assert bumped <= max(2, total // 100), f"{bumped} of {total} lines differ in the last digit"
+
+
+@pytest.mark.gpu
+def test_host_program_stops_like_the_reference(tmp_path):
+    """Input the reference stops on (a source in the ocean,
+    src/yspec.f90:210-215): the drop-in program prints the same text and,
+    like the reference program, ends with exit status 0 and no output files."""
+    from make_golden import ocean_deck
+    from yspec_amd.synth import workload
+
+    ref_exe = os.path.join(ROOT, "oracle", "_ref", "yspec")
+    if not os.path.exists(EXE):
+        pytest.fail("Fortran host not built: python -m yspec_amd.build")
+    p = workload("cfg1", lmax=4, nr=1, src_depth=1.0, model_file="model.deck")
+    p.prefix = "out"
+    (tmp_path / "model.deck").write_text(ocean_deck(0).to_text(digits=17))
+    (tmp_path / "in.txt").write_text(p.to_text())
+    r = subprocess.run([EXE, "in.txt"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "source in the ocean" in r.stdout + r.stderr
+    assert not (tmp_path / "out.1").exists()
+    if os.path.exists(ref_exe):  # the reference program, built from its sources
+        q = subprocess.run([ref_exe, "in.txt"], cwd=tmp_path, capture_output=True, text=True,
+                           timeout=100)
+        assert q.returncode == 0 and "source in the ocean" in q.stdout + q.stderr

@@ -70,6 +70,32 @@ def test_cfg2_pairs_vs_oracle(cfg2, oracle_mod):
         assert e <= 1e-9, k
 
 
+def test_cfg2_pairs_vs_reference(cfg2):
+    """Per-pair records of the lmax = 600, fmax = 40 mHz workload against the
+    reference's own at l = 0, 1, 2, 150, 300, 450, 600 (a spread of each
+    degree's 1310 frequencies; tests/golden/cfg2_records.npz from
+    oracle/_ref/ref_dump)."""
+    from refdump import PAIR_FIELDS
+    from test_oracle_golden import _cfg2_fixture
+
+    deck, p, g = cfg2
+    _, q, z = _cfg2_fixture()
+    assert np.array_equal(q.int_block(), p.int_block())
+    g.spectra(copy=False)
+    rec = g.pair_records(z["pair_l"], z["pair_j"])
+    n, at = ysph_rows(p), 0
+    for k, shape in PAIR_FIELDS:
+        cnt = int(np.prod(shape))
+        want = z["rec"][:, at:at + cnt].reshape((len(z["rec"]),) + shape, order="F")
+        at += cnt
+        a = rec[k]
+        if k == "ysph":  # rows the default b-vector path fills
+            a, want = a[:, :n, :], want[:, :n, :]
+        e = rel_l2(a, want)
+        print("cfg2 pairs vs reference", k, f"{e:.2e}", flush=True)
+        assert e <= 1e-9, k
+
+
 def test_cfg2_rows_match_columns(cfg2, api):
     """The receiver-side row march (transposed steps) against the
     reference's source-side column march on the same pairs: the same
@@ -211,4 +237,31 @@ def test_geometry_vs_oracle(api, oracle_mod, src, rec, full):
     e = rel_l2(g.seismograms(), o.seismograms(*os_))
     print(src, rec, "seis", f"{e:.2e}", flush=True)
     assert e <= 1e-8
+    g.close()
+
+
+def test_long_series_single_pass_ifft(api, oracle_mod):
+    """nt = 131072, past the 65536 limit of the two-pass transform: the
+    single-pass inverse FFT (k_seis) against the oracle's ifft and
+    post-processing applied to the GPU's own spectra, and the whole forward
+    against the oracle."""
+    import dataclasses
+
+    from conftest import load_case
+
+    deck, p, _ = load_case("cfg1_small")
+    p = dataclasses.replace(p, lmax=10, f1=0.1, f2=1.0, tout=1500.0, dt=1.0, f11=0.2,
+                            f12=0.3, f21=0.7, f22=0.9)
+    g = api.YSpec(deck, p)
+    assert g.nt == 131072
+    spec = g.spectra()
+    seis = g.seismograms()
+    o = oracle_mod.Oracle(deck, p)
+    e_ifft = rel_l2(seis, o.seismograms(*spec))
+    os_ = o.spectra(nthreads=16)
+    e_spec = max(rel_l2(a, b) for a, b in zip(spec, os_))
+    e_seis = rel_l2(seis, o.seismograms(*os_))
+    print(f"long series: ifft {e_ifft:.2e}, spectra {e_spec:.2e}, seismograms {e_seis:.2e}",
+          flush=True)
+    assert e_ifft <= 1e-12 and e_spec <= 1e-9 and e_seis <= 1e-8
     g.close()

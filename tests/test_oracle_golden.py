@@ -43,3 +43,38 @@ def test_spectra_and_seismograms(name, oracle_mod):
         assert rel_l2(a, z[key]) < 1e-12
     s = o.seismograms(ur, ut, up)
     assert rel_l2(s, z["seis"]) < 1e-12
+
+
+def _cfg2_fixture():
+    import os
+    from conftest import GOLDEN
+    from yspec_amd.synth import synth_deck, workload
+
+    z = dict(np.load(os.path.join(GOLDEN, "cfg2_records.npz"), allow_pickle=False))
+    deck, p = synth_deck(0), workload("cfg2")
+    # the fixture was written from this very workload
+    np.testing.assert_array_equal(z["deck_cols"], deck.cols)
+    np.testing.assert_array_equal(z["ip"], p.int_block())
+    np.testing.assert_array_equal(z["fp"], p.float_block())
+    return deck, p, z
+
+
+def test_cfg2_records_l600_40mhz(oracle_mod):
+    """The oracle where the headline throughput is measured: per-pair
+    records of the lmax = 600, fmax = 40 mHz workload (attenuation,
+    self-gravitation, 600 km source) at l = 0, 1, 2, 150, 300, 450, 600
+    against the reference's own (tests/golden/cfg2_records.npz), start knots
+    exactly."""
+    from refdump import PAIR_FIELDS
+
+    deck, p, z = _cfg2_fixture()
+    o = oracle_mod.Oracle(deck, p)
+    worst = 0.0
+    for i, (l, j) in enumerate(zip(z["pair_l"], z["pair_j"])):
+        q = o.pair(int(l), int(j))
+        assert q["starts"][0] == z["pair_tor_start"][i]
+        assert q["starts"][1] == z["pair_sph_start"][i]
+        got = np.concatenate([np.asarray(q[k]).reshape(-1, order="F") for k, _ in PAIR_FIELDS])
+        worst = max(worst, rel_l2(got, z["rec"][i]))
+    assert sorted(set(z["pair_l"].tolist())) == [0, 1, 2, 150, 300, 450, 600]
+    assert worst < 1e-12, worst

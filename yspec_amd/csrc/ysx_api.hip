@@ -14,7 +14,7 @@
 //                 source with transposed steps (k_bvec_march: the reference's
 //                 4 source-side columns, for full ysph records)
 //   k_rad         radial (l = 0) solves on the reference's grid (2nd stream)
-//   k_sph_var, k_rad_ng   per-lane solvers of the gravity/inner-core variants
+//   k_ngc_march, k_rad_ng solvers of the gravity switches 0 / 1
 //   k_assemble    output-type scaling and the uu/vv/ww coefficients
 //   k_lsum        sum over l and m against the Legendre tables: FP64 MFMA
 //                 GEMMs with the azimuthal phases fused into the epilogue
@@ -47,7 +47,7 @@ using namespace ysx;
 // ====================================================================== API
 namespace {
 
-enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_VAR, W_FFTA, W_FFTB, W_DROW, W_N };
+enum WorkKind { W_TOR = 0, W_BVEC, W_RAD, W_LSUM, W_SEIS, W_FFTA, W_FFTB, W_DROW, W_N };
 // queue counters of the batched launches (three marches share the W_TOR list)
 enum QueueSlot { Q_TOR = 0, Q_TOP, Q_MINOR, Q_BVEC, Q_RAD, Q_DROW, Q_N };
 
@@ -204,12 +204,6 @@ void build_work(ysx_ctx* c) {
         cost[W_RAD].push_back(rew(std::min(R.nw, per * jb + per)));
       }
     }
-    if (R.var)
-      for (int pb = 0; pb < (R.npair + 63) / 64; pb++) {
-        c->W[W_VAR].plain.push_back(make_int4(e, pb, 0, 0));
-        // the block's highest w sets its step count
-        cost[W_VAR].push_back(rew(std::min(R.nw, (64 * pb + 63) % R.nw + 1)) * (rtop + 0.05));
-      }
     // receiver-sum tiles t = jt + njt kt (16 w x 16 receivers).  Blocks go
     // to the 8 XCDs in turn (block b on XCD b % 8), so the list gives each
     // XCD whole frequency tiles, all their receiver tiles one after another:
@@ -280,13 +274,13 @@ void build_work(ysx_ctx* c) {
     int e0 = -1;
     for (int e = 0; e < nev && e0 < 0; e++) {
       const RunK& k = c->rk[e];
-      if (k.brow && !k.var && !k.ngc && k.rr >= k.rs && k.lmax >= 1) e0 = e;
+      if (k.brow && !k.ngc && k.rr >= k.rs && k.lmax >= 1) e0 = e;
     }
     if (e0 >= 0) {
       const RunK& a = c->rk[e0];
       for (int e = 0; e < nev; e++) {
         const RunK& k = c->rk[e];
-        member[e] = k.brow && !k.var && !k.ngc && k.rr >= k.rs && k.lmax >= 1 && k.rr == a.rr &&
+        member[e] = k.brow && !k.ngc && k.rr >= k.rs && k.lmax >= 1 && k.rr == a.rr &&
                     k.ir == a.ir && k.wmin == a.wmin && k.dw == a.dw && k.ep == a.ep &&
                     k.tref == a.tref && k.ats == a.ats && k.bl == a.bl;
         if (member[e]) c->sh_order.push_back(e);
@@ -683,11 +677,19 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     // self-gravitating runs go through the lockstep kernels, a fluid inner
     // core too when the receiver is not below the source (sph_solver_fic
     // then differs from sph_solver only in its start, handled by k_start
-    // and the marches); gravity switches 0 / 1 through k_ngc_march; a fluid
-    // inner core with the receiver below the source through the per-lane
-    // solver
+    // and the marches); gravity switches 0 / 1 through k_ngc_march.  With
+    // a fluid inner core and the receiver below the source the reference
+    // starts solid minors in the fluid inner core (src/module_solver.f90:
+    // 1323-1331) and its program does not complete on such a run, so there
+    // is no reference result to match: rejected.
+    if (R.grav == 2 && M.fic && R.rr < R.rs && R.lmax >= 1) {
+      ctx->R.clear();
+      ctx->rk.clear();
+      std::string m = "self-gravitation with a fluid inner core and the receiver below the "
+                      "source (sph_solver_fic's receiver-below branch) is not supported";
+      return set_err(ctx, YSX_ERR_UNSUPPORTED, nev > 1 ? "event " + std::to_string(e) + ": " + m : m);
+    }
     k.ngc = (R.grav != 2) ? 1 : 0;
-    k.var = (R.grav == 2 && M.fic && R.rr < R.rs) ? 1 : 0;
     k.lpad = (R.lmax + 1 + 3) / 4 * 4;
     // radial solves: 8 lanes per frequency, or one when there are many
     // frequencies (then the radial kernel runs hidden beside the marches and
@@ -896,12 +898,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
     launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
                        SD, ctx->d_rec, ctx->d_cnt);
     if (timing) HIPCHK(hipEventRecord(ev[3], st));
-    if (ctx->rk[e].var) {
-      Queue qv = plain_q(ctx, W_VAR, e);
-      launch_sph_var(dim3(grid_of(qv)), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
-                         ctx->d_rec, ctx->d_cnt);
-      if (timing) HIPCHK(hipEventRecord(ev[4], st));
-    } else if (ctx->rk[e].ngc) {
+    if (ctx->rk[e].ngc) {
       launch_ngc_march(ctx->rk[e].grav == 1, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                        ctx->d_runs, qt, ctx->d_rec, ctx->d_cnt);
       if (timing) HIPCHK(hipEventRecord(ev[4], st));
@@ -1028,10 +1025,6 @@ int enqueue_batched(ysx_ctx* ctx) {
       launch_bvec_march_item(grid(qb), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
                              ctx->d_rec, ctx->d_cnt);
   }
-  Queue qv = batched_q(ctx, W_VAR, -1);
-  if (qv.n > 0)
-    launch_sph_var(dim3(qv.n), b64, 0, st, ctx->d_tab, ctx->d_runs, qv,
-                       ctx->d_rec, ctx->d_cnt);
   // gravity switches 0 / 1: one launch each over the cost-sorted wave list
   // (the kernel skips the other events' items)
   Queue qn = batched_q(ctx, W_TOR, -1);
@@ -1072,6 +1065,7 @@ int run_spectra(ysx_ctx* ctx) {
     return rc;
   ctx->have_spec = true;
   ctx->have_seis = false;
+  ctx->have_recb = false;  // record adjoints belong to the records of a forward
   return refine_work(ctx);
 }
 
@@ -1179,6 +1173,7 @@ extern "C" int ysx_forward_events(ysx_ctx* ctx, int mode, double* seis) {
   for (double& m : ctx->ms) m = 0.0;
   ctx->ms[7] = t;
   ctx->have_spec = ctx->have_seis = true;
+  ctx->have_recb = false;  // record adjoints belong to the records of a forward
   if (int rc2 = refine_work(ctx)) return rc2;
   if (seis) {
     size_t n = 0;

@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
     SD.tor_y0[2 * p] = y[0];
     SD.tor_y0[2 * p + 1] = y[1];
   }
-  if (R.var || R.ngc) return;  // the variant solvers find their own start
+  if (R.ngc) return;  // the gravity-0/1 march finds its own start
   int i1 = sph_start_level(T, L, l, s.rew, itop, R.fre_norm);
   starts[2 * p + 1] = i1;
   double rt = T(F_R, i1);
@@ -345,40 +345,6 @@ __global__ void __launch_bounds__(64) k_rad_ng(const double* __restrict__ gtab,
   __shared__ RadRHS::Coef rsh[64 / RAD_GROUP_MAX][6];
   Tab T = load_tab(gtab, runs[0].nknot, runs[0].tablds);
   RUN_ITEMS(false, Q, rad_ng_item(T, runs[it.x], it.y, rec, cnt, rsh));
-}
-
-// ------------------------------------------------------------- k_sph_var
-// The spheroidal solve of one pair per lane for the events outside the
-// lockstep kernels: a fluid inner core with self-gravitation and the
-// receiver below the source (sph_solver_fic, src/module_solver.f90:
-// 1203-1382).
-__device__ __forceinline__ void sph_var_item(const Tab& T, const RunK& R, int pb,
-                                             cd* __restrict__ rec, int* __restrict__ cnt) {
-  int pl = pb * 64 + (int)threadIdx.x;
-  if (pl >= R.npair) return;
-  int l, j;
-  pair_lj(R, pl, l, j);
-  if (l < 1) return;
-  PS s;
-  pair_state(R, l, j, s);
-  VarGeo G{R.nknot, R.nic, R.noc, R.nsl, R.ocean, R.is, R.ir, R.rs, R.rr, R.alpha_min,
-           R.fre_norm};
-  VarCount n{0, 0, 0};
-  size_t p = (size_t)R.pbase + pl;
-  cd* o = rec + p * REC + O_YSPH;
-  cd y[24];
-  sph_solve_fic(T, s, G, y, n);
-  for (int q = 0; q < 24; q++) o[q] = y[q];
-  cnt[5 * p + 1] = n.sol;
-  cnt[5 * p + 2] = n.flu;
-  cnt[5 * p + 3] = n.bvec;
-}
-
-// items (event, block of 64 pairs) of the events with var set
-__global__ void __launch_bounds__(64) k_sph_var(const double* __restrict__ gtab,
-                                                const RunK* __restrict__ runs, Queue Q,
-                                                cd* __restrict__ rec, int* __restrict__ cnt) {
-  RUN_ITEMS(false, Q, sph_var_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, rec, cnt));
 }
 
 // ------------------------------------------------------------- k_assemble
@@ -1543,10 +1509,6 @@ void launch_rad(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, c
 void launch_rad_ng(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                    const RunK* runs, Queue Q, cd* rec, int* cnt) {
   hipLaunchKernelGGL(k_rad_ng, g, b, lds, s, gtab, runs, Q, rec, cnt);
-}
-void launch_sph_var(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                    const RunK* runs, Queue Q, cd* rec, int* cnt) {
-  hipLaunchKernelGGL(k_sph_var, g, b, lds, s, gtab, runs, Q, rec, cnt);
 }
 void launch_assemble(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, PairRange PR,
                      cd* rec, cd* cf, const double* tb) {

@@ -17,8 +17,8 @@ namespace ysx {
 struct RunK {
   int lmin, lmax, nw, nr, is, ir, out, cor, npair;
   int nknot, nic, noc, nsl, fic, ocean;
-  int grav, var, ngc;           // gravity switch; spheroidal solves per lane (var) or
-                                // by the gravity-0/1 lockstep march (ngc)
+  int grav, ngc;                // gravity switch; spheroidal solves by the gravity-0/1
+                                // lockstep march (ngc)
   int lpad, jpad, nrpad;        // GEMM padding of this event
   int radg;                     // lanes per radial (l = 0) solve: 1 or 8
   int brow, bl;                 // b-vector as receiver-side rows (1) or source-side
@@ -261,8 +261,6 @@ void launch_rad(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, c
                 Queue Q, cd* rec, int* cnt);
 void launch_rad_ng(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                    const RunK* runs, Queue Q, cd* rec, int* cnt);
-void launch_sph_var(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                    const RunK* runs, Queue Q, cd* rec, int* cnt);
 void launch_assemble(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, PairRange PR,
                      cd* rec, cd* cf, const double* tb);
 void launch_lsum(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q, cd* cf,

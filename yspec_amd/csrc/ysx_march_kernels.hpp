@@ -113,7 +113,7 @@ template <bool DROW>
 __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, int lb,
                                            const StartData& SD, const SphState& S,
                                            int* __restrict__ cnt, double* sb, cd* slot) {
-  if (R.var || R.ngc) return;  // spheroidal solves of this event run in k_sph_var / k_ngc_march
+  if (R.ngc) return;  // spheroidal solves of this event run in k_ngc_march
   int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 1 && l <= R.lmax);
   size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ g
 __device__ __forceinline__ void minor_top_item(const Tab& T, const RunK& R, int j, int lb,
                                                const SphState& S, int* __restrict__ cnt,
                                                double* sb, cd* slot) {
-  if (R.var || R.ngc) return;
+  if (R.ngc) return;
   int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 1 && l <= R.lmax);
   if (wave_min(valid ? 0 : 1)) return;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(64) k_minor_top(const double* __restrict__ gta
 __device__ __forceinline__ void bvec_item(const Tab& T, const RunK& R, int j, int lb,
                                           const SphState& S, cd* __restrict__ rec,
                                           int* __restrict__ cnt, double* sb, cd* slot) {
-  if (R.var || R.ngc) return;
+  if (R.ngc) return;
   int c = threadIdx.x & 3;
   int l = R.lmin + lb * 16 + (int)(threadIdx.x >> 2);
   bool valid = (l >= 1 && l <= R.lmax);
@@ -361,7 +361,7 @@ __device__ __forceinline__ void brow_finish(const RunK& R, int l, int j, int k, 
 __device__ __forceinline__ void brow_item(const Tab& T, const RunK& R, int j, int lb,
                                           const SphState& S, cd* __restrict__ rec,
                                           int* __restrict__ cnt, double* sb, cd* slot) {
-  if (R.var || R.ngc) return;
+  if (R.ngc) return;
   const int rows = 64 / R.bl;  // lanes per degree: row k = lane % rows
   int k = threadIdx.x % rows;
   int l = R.lmin + lb * R.bl + (int)(threadIdx.x / rows);

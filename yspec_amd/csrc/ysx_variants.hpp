@@ -107,92 +107,6 @@ struct VarGeo {
   double rs, rr, alpha_min, fre_norm;
 };
 
-struct VarCount {
-  int sol, flu, bvec;
-};
-
-// sph_solver_fic (grav = 2, fluid inner core) with ss = s6tmp,
-// src/module_solver.f90:1203-1382: yout (6,4) column-major.  Below a
-// source the reference starts a solid minor vector in the (fluid) inner
-// core as sph_solver does (:1323-1331); that is reproduced as written.
-__device__ void sph_solve_fic(const Tab& T, const PS& s, const VarGeo& G, cd* yout,
-                              VarCount& n) {
-  SolidMinorRHS fs;
-  FluidMinorRHS ff;
-  BvecRHS fb;
-  Layers L{G.nknot, G.nic, G.noc, G.nsl, 1, G.ocean};
-  bool up = G.rr >= G.rs;
-  int iup = up ? G.is : G.ir, ibot = up ? G.ir : G.is;
-  double rup = up ? G.rs : G.rr, rbot = up ? G.rr : G.rs;
-  int nk = G.nknot;
-  int sc1 = 0, sc2 = 0, sc3 = 0;
-  cd m1[14], m2[14], m3[14], mf[5];
-  int i1 = sph_start_level(T, L, s.ll, s.rew, iup, G.fre_norm);
-  double rt = T(F_R, i1);
-  if (i1 == 1) {
-    double dr = 1.0 / (4.0 * s.rew / G.alpha_min);
-    rt = fmin(0.1 * dr, 0.1 * T(F_R, 2));
-  }
-  if (i1 >= iup) {
-    i1 = iup - 1;
-    rt = T(F_R, i1);
-  }
-  for (int q = 0; q < 14; q++) m1[q] = C(0.0);
-  if (i1 <= G.nic && !up) {
-    minor_solid_start(T, s, i1, rt, m1);
-    yint(T, s, nk, i1, G.nic, rt, T(F_R, G.nic), m1, fs, &sc1, n.sol);
-    solid_2_fluid(m1, mf);
-    yint(T, s, nk, G.nic + 1, G.noc, T(F_R, G.nic), T(F_R, G.noc), mf, ff, &sc1, n.flu);
-    fluid_2_solid(mf, m1);
-    yint(T, s, nk, G.noc + 1, iup, T(F_R, G.noc + 1), rup, m1, fs, &sc1, n.sol);
-  } else if (i1 <= G.noc) {
-    minor_fluid_start(T, s, i1, rt, m1);
-    for (int q = 0; q < 5; q++) mf[q] = m1[q];
-    yint(T, s, nk, i1, G.noc, rt, T(F_R, G.noc), mf, ff, &sc1, n.flu);
-    fluid_2_solid(mf, m1);
-    yint(T, s, nk, G.noc + 1, iup, T(F_R, G.noc + 1), rup, m1, fs, &sc1, n.sol);
-  } else if (i1 <= G.nsl) {
-    minor_solid_start(T, s, i1, rt, m1);
-    yint(T, s, nk, i1, iup, rt, rup, m1, fs, &sc1, n.sol);
-  }
-  for (int q = 0; q < 14; q++) m3[q] = m1[q];
-  if (up && G.rr > G.rs) yint(T, s, nk, G.is, G.ir, G.rs, G.rr, m3, fs, &sc3, n.sol);
-  if (!up && G.rs > G.rr) yint(T, s, nk, G.ir, G.is, G.rr, G.rs, m3, fs, &sc3, n.sol);
-  if (G.ocean) {
-    cd f2[5] = {C(1.0), C(0.0), C(0.0), C(0.0), C(0.0)};
-    yint(T, s, nk, nk - 1, G.nsl + 1, T(F_R, nk), T(F_R, G.nsl + 1), f2, ff, &sc2, n.flu);
-    fluid_2_solid(f2, m2);
-    yint(T, s, nk, G.nsl, ibot, T(F_R, G.nsl), rbot, m2, fs, &sc2, n.sol);
-  } else {
-    m2[0] = C(1.0);
-    for (int q = 1; q < 14; q++) m2[q] = C(0.0);
-    yint(T, s, nk, up ? nk - 1 : nk, ibot, T(F_R, nk), rbot, m2, fs, &sc2, n.sol);
-  }
-  cd delta = delta_solid(m3, m2);
-  cd mdi = -cinv(delta);
-  const int cols[4] = {0, 1, 3, 4};
-  for (int c = 0; c < 4; c++) {
-    int sc4 = 0, nb = 0;
-    cd b[14];
-    cd* y = yout + 6 * c;
-    if (up) {
-      b_start_unit(m1, cols[c], b);
-      yint(T, s, nk, G.is, G.ir, G.rs, G.rr, b, fb, &sc4, nb);
-      ysol(m2, b, y);
-      double f = ipow_huge(sc3 - sc4);
-      for (int q = 0; q < 6; q++) y[q] = (-(y[q] / delta)) * f;
-    } else {
-      b_start_unit(m2, cols[c], b);
-      for (int q = 0; q < 14; q++) b[q] = b[q] * mdi;
-      yint(T, s, nk, G.is, G.ir, G.rs, G.rr, b, fb, &sc4, nb);
-      ysol(m1, b, y);
-      double f = ipow_huge(sc3 - sc4);
-      for (int q = 0; q < 6; q++) y[q] = y[q] * f;
-    }
-    if (c == 0) n.bvec = nb;
-  }
-}
-
 // rad_ystart_ng, src/module_int.f90:455-489
 __device__ inline void rad_ystart_ng(const Tab& T, const PS& s, int is, double rr, cd* y) {
   SplW w = spl_w(T, is, rr);

@@ -94,6 +94,10 @@ contains
        text(i:i) = msg(i)
     end do
     write(0, '(a)') trim(text)
+    ! input the reference rejects (YSX_ERR_INPUT): it prints its stop text
+    ! and ends with exit status 0 (src/yspec.f90:201-215); errors that have
+    ! no counterpart there (HIP, arguments, unsupported runs) end with 1
+    if (rc == 2) stop
     error stop 1
   end subroutine ysx_check
 
