@@ -225,6 +225,36 @@ int ysx_moment_tensor_vjp(ysx_ctx *ctx, double *mm_bar);
  * terms of the per-pair solves are not included. */
 int ysx_source_model_vjp(ysx_ctx *ctx, double *table_bar);
 
+/* Gradient with respect to the model table through the per-pair solves
+ * (no counterpart in the reference; the core of a misfit gradient): with
+ * the record adjoints of the last ysx_spectra_vjp (after
+ * ysx_seismograms_vjp(seis_bar): the gradient of sum(seis_bar *
+ * seismograms)), table_bar (F_COUNT x nknot, the layout of
+ * ysx_model_table) gets the adjoint of every table entry, through
+ *   - the toroidal, spheroidal minor and b-vector marches (tor_solver,
+ *     sph_solver, src/module_solver.f90:25-142, 640-823; yint and rkck,
+ *     src/module_int.f90:1847-1968, 2016-2064) as discrete adjoints of
+ *     the same fixed step lists, replayed from knot-interval checkpoints;
+ *   - the secular determinant, the b-vector starts and the region
+ *     transforms (src/module_int.f90:1563-1592, 1620-1680, 1705-1725);
+ *   - the starting solutions (src/module_int.f90:213-250, 1192-1270,
+ *     1346-1405, 1451-1534);
+ *   - the radial (l = 0) solves (rad_solver_wg, src/module_solver.f90:
+ *     446-561), every step of both sweeps replayed from a tape;
+ *   - the source vectors (as ysx_source_model_vjp).
+ * Start levels, step counts
+ * and renormalisations are those of the forward (held fixed).  Built for
+ * one event with self-gravitation, a solid inner core, no ocean, receivers
+ * at the surface, no potential/tilt correction and the default b-vector
+ * rows; other runs return YSX_ERR_UNSUPPORTED.  ysx_model_table_vjp takes
+ * table_bar to the deck. */
+int ysx_model_vjp(ysx_ctx *ctx, double *table_bar);
+
+/* Kernel milliseconds of the last ysx_model_vjp: [0] b-vector sweep, [1]
+ * minor sweep, [2] toroidal sweep, [3] starting solutions, [4] radial
+ * solves, [5] source vectors, [6] the sum. */
+int ysx_grad_timings(const ysx_ctx *ctx, double *ms7);
+
 /* Kernel milliseconds (HIP events on the context's stream, host copies
  * excluded) of the last call of [0] ysx_seismograms_vjp, [1] ysx_spectra_vjp,
  * [2] ysx_moment_tensor_vjp or ysx_source_model_vjp, whichever ran last. */

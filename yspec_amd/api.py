@@ -28,7 +28,8 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_model_table_host", "ysx_model_table_vjp",
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts",
            "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
-           "ysx_vjp_timings", "ysx_source_model_vjp", "ysx_model_table_noalloc")
+           "ysx_vjp_timings", "ysx_source_model_vjp", "ysx_model_table_noalloc",
+           "ysx_model_vjp", "ysx_grad_timings")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -92,6 +93,8 @@ def lib():
         L.ysx_moment_tensor_vjp.argtypes = [vp, D]
         L.ysx_vjp_timings.argtypes = [vp, D]
         L.ysx_source_model_vjp.argtypes = [vp, D]
+        L.ysx_model_vjp.argtypes = [vp, D]
+        L.ysx_grad_timings.argtypes = [vp, D]
         L.ysx_model_table_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                               C.c_double, D, C.c_void_p]
         _LIB = L
@@ -309,6 +312,37 @@ class YSpec:
         self.seismograms_vjp(seis_bar)
         self.spectra_vjp(np.zeros(0, np.int32), np.zeros(0, np.int32))
         return self.moment_tensor_vjp()
+
+    def model_vjp(self):
+        """(29, nknot) table adjoint through the per-pair solves, starting
+        solutions and source vectors, from the record adjoints of the last
+        spectra_vjp (input of model_table_vjp)."""
+        out = np.zeros((len(TABLE_FIELDS), self.deck.nknot))
+        self._check(self.L.ysx_model_vjp(self.h, _dp(out)))
+        return out
+
+    def grad_timings(self):
+        """kernel ms of the last model_vjp: b-vector, minor and toroidal
+        sweeps, starting solutions, radial solves, source vectors, sum"""
+        out = np.zeros(7)
+        self._check(self.L.ysx_grad_timings(self.h, _dp(out)))
+        return out
+
+    def misfit_gradient(self, obs, nout=None):
+        """L2 misfit 0.5 * sum((seis - obs)^2) over the first nout samples
+        (the samples the drop-in program writes) and its gradient with
+        respect to the deck columns, (nknot, 9) in the deck's layout, after
+        one forward: seismograms_vjp, spectra_vjp, model_vjp on the device,
+        then the deck -> table adjoint on the host."""
+        nout = self.nout if nout is None else nout
+        seis = self.forward()
+        res = np.zeros_like(seis)
+        res[:nout] = seis[:nout] - obs[:nout]
+        misfit = 0.5 * float(np.sum(res * res))
+        self.seismograms_vjp(res)
+        self.spectra_vjp(np.zeros(0, np.int32), np.zeros(0, np.int32))
+        tb = self.model_vjp()
+        return misfit, model_table_vjp(self.deck, tb)
 
     def model_table(self):
         """(29, nknot) model table the kernels read (built on the device)."""

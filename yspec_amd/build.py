@@ -31,6 +31,7 @@ SOURCES = [("ysx_k_march.hip", f"ysx_k_march_{k}{p}.o", [f"-DYSX_MARCH={k}", f"-
 # loop-carried accumulators are copied between AGPRs and VGPRs every step.
 SOURCES += [("ysx_k_misc.hip", "ysx_k_misc.o", ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]),
             ("ysx_k_ngc.hip", "ysx_k_ngc.o", []),
+            ("ysx_k_adjoint.hip", "ysx_k_adjoint.o", []),
             ("ysx_api.hip", "ysx_api.o", []), ("ysx_host.cpp", "ysx_host.o", [])]
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

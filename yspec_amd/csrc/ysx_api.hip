@@ -40,6 +40,7 @@
 
 #include "../../include/ysx.h"
 #include "ysx_host.hpp"
+#include "ysx_adjoint.hpp"
 #include "ysx_kern.hpp"
 
 using namespace ysx;
@@ -120,6 +121,18 @@ struct ysx_ctx {
   hipEvent_t ev[16] = {};
   double ms[8] = {0};
   double* scratch = nullptr;  // this device's copy of `scratch` (ysx_model_table_noalloc)
+  // model gradient (ysx_model_vjp): per-pair adjoints of the minors at the
+  // source from the b-vector starts (mub), of the starting solutions (m0bar,
+  // y0bar), the table adjoint, the sweeps' per-block checkpoints and tapes
+  cd *d_mub = nullptr, *d_m0bar = nullptr, *d_y0bar = nullptr;
+  double* d_tbar = nullptr;
+  cd *d_ack = nullptr, *d_atape = nullptr;
+  int *d_acks = nullptr, *d_atsc = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
+  int adj_blocks = 0, adj_slots = 0, adj_tmax = 0;
+  void* d_rtape = nullptr;  // radial adjoint: every step of the l = 0 solves
+  long long* d_rtoff = nullptr;
+  size_t rtape_bytes = 0;
+  double ms_grad[7] = {0};  // kernel ms of the last ysx_model_vjp
 };
 
 namespace {
@@ -157,6 +170,11 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
   dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
+  dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_ack);
+  dfree(c->d_atape); dfree(c->d_acks); dfree(c->d_atsc); dfree(c->d_aerr); dfree(c->d_aq);
+  dfree(c->d_rtape); dfree(c->d_rtoff);
+  c->adj_blocks = c->adj_slots = c->adj_tmax = 0;
+  c->rtape_bytes = 0;
   c->have_specb = c->have_recb = false;
   c->sh_order.clear();
   c->n_union = 0;
@@ -1416,6 +1434,159 @@ extern "C" int ysx_source_model_vjp(ysx_ctx* ctx, double* table_bar) {
       y2[1] += b3 * g;
     }
   }
+  return YSX_OK;
+}
+
+namespace {
+
+// The longest step list of any knot interval at the run's highest frequency
+// (the step rule of yint, src/module_int.f90:1903-1905): the tape length of
+// the adjoint sweeps.
+int max_interval_steps(const HostModel& M, const RunK& k) {
+  const double rew = k.wmin + (double)(k.nw - 1) * k.dw;
+  int mx = 1;
+  for (int i = 1; i < M.nknot; i++) {
+    double a = M.at(F_ALPHA, i), b = M.at(F_ALPHA, i + 1);
+    double drr = 1.0 / (4.0 * rew / std::max(a, b));
+    double dr = std::fabs(M.at(F_R, i + 1) - M.at(F_R, i));
+    mx = std::max(mx, 1 + (int)std::floor(dr / drr));
+  }
+  return mx + 1;
+}
+
+}  // namespace
+
+extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
+  if (!ctx || !table_bar) return YSX_ERR_ARG;
+  if (int rc = adjoint_run(ctx)) return rc;
+  if (!ctx->have_recb)
+    return set_err(ctx, YSX_ERR_ARG, "no record adjoints: call ysx_spectra_vjp first");
+  const HostModel& M = ctx->M;
+  if (ctx->rk.size() != 1)
+    return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_model_vjp: one event per context");
+  const RunK& k = ctx->rk[0];
+  const HostRun& R = ctx->R[0];
+  const double rsurf = M.at(F_R, M.nknot);
+  std::string why;
+  if (k.grav != 2) why = "gravity switch 0 or 1";
+  else if (M.fic) why = "a fluid inner core";
+  else if (M.ocean) why = "an ocean";
+  else if (k.rr < k.rs || k.rr != rsurf) why = "a receiver below the surface";
+  else if (k.out == 2 && k.cor != 0) why = "the potential/tilt correction";
+  else if (!k.brow) why = "the source-side b-vector columns";
+  else if (M.nknot > 800) why = "more than 800 knots";
+  if (!why.empty())
+    return set_err(ctx, YSX_ERR_UNSUPPORTED,
+                   "ysx_model_vjp: the solver adjoint is built for self-gravitation, a solid "
+                   "inner core, no ocean and receivers at the surface; this run has " + why);
+  (void)R;
+  const int nk = M.nknot;
+  const size_t P = (size_t)ctx->npair;
+  // buffers (kept for the run)
+  if (!ctx->d_tbar) {
+    ctx->adj_blocks = 4 * ctx->ncu;
+    ctx->adj_slots = k.ir + 1;
+    ctx->adj_tmax = max_interval_steps(M, k);
+    const size_t nb = ctx->adj_blocks, ns = ctx->adj_slots, nt = ctx->adj_tmax;
+    HIPCHK(hipMalloc(&ctx->d_mub, P * 14 * sizeof(cd)));
+    HIPCHK(hipMalloc(&ctx->d_m0bar, P * 14 * sizeof(cd)));
+    HIPCHK(hipMalloc(&ctx->d_y0bar, P * 2 * sizeof(cd)));
+    HIPCHK(hipMalloc(&ctx->d_tbar, (size_t)F_COUNT * nk * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->d_ack, nb * ns * 14 * 64 * sizeof(cd)));
+    HIPCHK(hipMalloc(&ctx->d_acks, nb * ns * 64 * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->d_atape, nb * nt * 6 * 14 * 64 * sizeof(cd)));
+    HIPCHK(hipMalloc(&ctx->d_atsc, nb * nt * 64 * sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->d_aerr, sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->d_aq, 4 * sizeof(int)));
+  }
+  hipStream_t st = ctx->stream;
+  hipEvent_t* ev = ctx->ev;
+  const dim3 b64(64), b256(256);
+  const size_t ntab = (size_t)F_COUNT * nk;
+  launch_fill0(dim3((unsigned)std::min<size_t>(1024, (ntab + 255) / 256)), b256, 0, st,
+               ctx->d_tbar, ntab);
+  launch_reset(dim3(1), b64, 0, st, ctx->d_aerr, 1);
+  launch_reset(dim3(1), b64, 0, st, ctx->d_aq, 4);
+  HIPCHK(hipGetLastError());
+  SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
+  StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
+  AdjWork AW{ctx->d_ack, ctx->d_acks, ctx->d_atape, ctx->d_atsc, ctx->adj_slots, ctx->adj_tmax,
+             ctx->d_aerr};
+  Queue qb = plain_q(ctx, W_BVEC, 0), qt = plain_q(ctx, W_TOR, 0);
+  Queue qm = qt;
+  qb.counter = ctx->d_aq;
+  qm.counter = ctx->d_aq + 1;
+  qt.counter = ctx->d_aq + 2;
+  qb.chunk = qm.chunk = qt.chunk = 1;
+  const dim3 gb(ctx->adj_blocks);
+  HIPCHK(hipEventRecord(ev[0], st));
+  launch_brow_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_recb,
+                  ctx->d_mub, AW, ctx->d_tbar);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[1], st));
+  launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
+                   ctx->d_mub, ctx->d_m0bar, AW, ctx->d_tbar);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[2], st));
+  launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
+                 ctx->d_y0bar, AW, ctx->d_tbar);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[3], st));
+  {
+    PairRange PR = pairs_of(ctx, 0, 1);
+    launch_start_adj(dim3((unsigned)((PR.n + 255) / 256)), b256,
+                     (size_t)SV_N * 2 * nk * sizeof(double), st, ctx->d_tab, ctx->d_runs, PR, SD,
+                     ctx->d_starts, ctx->d_m0bar, ctx->d_y0bar, ctx->d_tbar);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(ev[4], st));
+  if (k.lmin == 0) {
+    // the radial solves: a tape of every step per frequency, at offsets
+    // from the forward's step counts
+    if (!ctx->d_rtoff) {
+      std::vector<int> c(5 * (size_t)k.nw);
+      HIPCHK(hipMemcpy(c.data(), ctx->d_cnt, c.size() * sizeof(int), hipMemcpyDeviceToHost));
+      std::vector<long long> off(k.nw + 1, 0);
+      for (int j = 0; j < k.nw; j++) off[j + 1] = off[j] + c[5 * (size_t)j + 4];
+      HIPCHK(hipMalloc(&ctx->d_rtoff, off.size() * sizeof(long long)));
+      HIPCHK(hipMemcpy(ctx->d_rtoff, off.data(), off.size() * sizeof(long long),
+                       hipMemcpyHostToDevice));
+      ctx->rtape_bytes = std::max<size_t>(1, (size_t)off[k.nw]) * rad_step_bytes();
+      HIPCHK(hipMalloc(&ctx->d_rtape, ctx->rtape_bytes));
+    }
+    launch_rad_adj(dim3((unsigned)((k.nw + 63) / 64)), b64, 0, st, ctx->d_tab, ctx->d_runs,
+                   ctx->d_recb, ctx->d_rtoff, ctx->d_rtape, ctx->d_aerr, ctx->d_tbar);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(ev[5], st));
+  HIPCHK(hipEventSynchronize(ev[5]));
+  for (int q = 0; q < 5; q++)
+    if (int rc = elapsed(ctx, q, q + 1, ctx->ms_grad[q])) return rc;
+  int err = 0;
+  HIPCHK(hipMemcpy(&err, ctx->d_aerr, sizeof(int), hipMemcpyDeviceToHost));
+  if (err & 1)
+    return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a knot interval with more steps than the "
+                                     "sweep's tape (" + std::to_string(ctx->adj_tmax) + ")");
+  if (err & 2)
+    return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a radial solve with more steps than its "
+                                     "forward");
+  if (err & 4)
+    return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_model_vjp: radial solve with the receiver "
+                                             "below the source point");
+  std::vector<double> tb(ntab);
+  HIPCHK(hipMemcpy(tb.data(), ctx->d_tbar, ntab * sizeof(double), hipMemcpyDeviceToHost));
+  // the source vectors' part (ysx_source_model_vjp)
+  if (int rc = ysx_source_model_vjp(ctx, table_bar)) return rc;
+  ctx->ms_grad[5] = ctx->ms_vjp[2];
+  ctx->ms_grad[6] = 0.0;
+  for (int q = 0; q < 6; q++) ctx->ms_grad[6] += ctx->ms_grad[q];
+  for (size_t q = 0; q < ntab; q++) table_bar[q] += tb[q];
+  return YSX_OK;
+}
+
+extern "C" int ysx_grad_timings(const ysx_ctx* ctx, double* ms7) {
+  if (!ctx || !ms7) return YSX_ERR_ARG;
+  std::memcpy(ms7, ctx->ms_grad, sizeof ctx->ms_grad);
   return YSX_OK;
 }
 

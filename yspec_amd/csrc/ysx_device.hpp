@@ -27,19 +27,19 @@ struct cd {
   double x, y;
 };
 __device__ __host__ inline cd C(double x, double y = 0.0) { return {x, y}; }
-__device__ inline cd operator+(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
-__device__ inline cd operator-(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
-__device__ inline cd operator-(cd a) { return {-a.x, -a.y}; }
-__device__ inline cd operator*(cd a, cd b) {
+__host__ __device__ inline cd operator+(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__host__ __device__ inline cd operator-(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
+__host__ __device__ inline cd operator-(cd a) { return {-a.x, -a.y}; }
+__host__ __device__ inline cd operator*(cd a, cd b) {
   return {fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x)};
 }
-__device__ inline cd operator*(double s, cd a) { return {s * a.x, s * a.y}; }
-__device__ inline cd operator*(cd a, double s) { return {s * a.x, s * a.y}; }
-__device__ inline cd operator+(cd a, double s) { return {a.x + s, a.y}; }
-__device__ inline cd operator+(double s, cd a) { return {a.x + s, a.y}; }
-__device__ inline cd operator-(cd a, double s) { return {a.x - s, a.y}; }
-__device__ inline cd operator-(double s, cd a) { return {s - a.x, -a.y}; }
-__device__ inline cd cinv(cd b) {
+__host__ __device__ inline cd operator*(double s, cd a) { return {s * a.x, s * a.y}; }
+__host__ __device__ inline cd operator*(cd a, double s) { return {s * a.x, s * a.y}; }
+__host__ __device__ inline cd operator+(cd a, double s) { return {a.x + s, a.y}; }
+__host__ __device__ inline cd operator+(double s, cd a) { return {a.x + s, a.y}; }
+__host__ __device__ inline cd operator-(cd a, double s) { return {a.x - s, a.y}; }
+__host__ __device__ inline cd operator-(double s, cd a) { return {s - a.x, -a.y}; }
+__host__ __device__ inline cd cinv(cd b) {
   // scaled reciprocal: safe for |b| up to the double range
   double s = fmax(fabs(b.x), fabs(b.y));
   if (s == 0.0) return {1.0 / 0.0, 0.0};
@@ -47,29 +47,29 @@ __device__ inline cd cinv(cd b) {
   double d = 1.0 / (s * (bx * bx + by * by));
   return {bx * d, -by * d};
 }
-__device__ inline cd operator/(cd a, cd b) { return a * cinv(b); }
-__device__ inline cd operator/(cd a, double s) { return {a.x / s, a.y / s}; }
-__device__ inline cd operator/(double s, cd b) { return s * cinv(b); }
-__device__ inline cd cmuli(cd a) { return {-a.y, a.x}; }  // i*a
-__device__ inline double cabs2(cd a) { return a.x * a.x + a.y * a.y; }
-__device__ inline double cabs_(cd a) { return hypot(a.x, a.y); }
-__device__ inline cd cfma(cd a, cd b, cd c) {  // a*b + c
+__host__ __device__ inline cd operator/(cd a, cd b) { return a * cinv(b); }
+__host__ __device__ inline cd operator/(cd a, double s) { return {a.x / s, a.y / s}; }
+__host__ __device__ inline cd operator/(double s, cd b) { return s * cinv(b); }
+__host__ __device__ inline cd cmuli(cd a) { return {-a.y, a.x}; }  // i*a
+__host__ __device__ inline double cabs2(cd a) { return a.x * a.x + a.y * a.y; }
+__host__ __device__ inline double cabs_(cd a) { return hypot(a.x, a.y); }
+__host__ __device__ inline cd cfma(cd a, cd b, cd c) {  // a*b + c
   return {fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y))};
 }
-__device__ inline cd csqrt_(cd z) {
+__host__ __device__ inline cd csqrt_(cd z) {
   // principal branch, Re >= 0
   if (z.x == 0.0 && z.y == 0.0) return {0.0, z.y};
   double t = sqrt(0.5 * (fabs(z.x) + hypot(z.x, z.y)));
   if (z.x >= 0.0) return {t, z.y / (2.0 * t)};
   return {fabs(z.y) / (2.0 * t), copysign(t, z.y)};
 }
-__device__ inline cd clog_(cd z) { return {log(hypot(z.x, z.y)), atan2(z.y, z.x)}; }
+__host__ __device__ inline cd clog_(cd z) { return {log(hypot(z.x, z.y)), atan2(z.y, z.x)}; }
 
 // -------------------------------------------------------------- model table
 struct Tab {
   const double* t;  // field-major: t[f * nk + (i - 1)]
   int nk;
-  __device__ inline double operator()(int f, int i) const { return t[f * nk + i - 1]; }
+  __host__ __device__ inline double operator()(int f, int i) const { return t[f * nk + i - 1]; }
 };
 
 // Natural cubic spline evaluation on knot interval klo (splint_dis_r,
@@ -79,7 +79,7 @@ struct SplW {
   int lo;
   bool deg;
 };
-__device__ inline SplW spl_w(const Tab& T, int klo, double x) {
+__host__ __device__ inline SplW spl_w(const Tab& T, int klo, double x) {
   SplW w;
   w.lo = klo;
   double xl = T(F_R, klo), xh = T(F_R, klo + 1);
@@ -93,7 +93,7 @@ __device__ inline SplW spl_w(const Tab& T, int klo, double x) {
   w.hh = h * h * (1.0 / 6.0);
   return w;
 }
-__device__ inline double spl(const Tab& T, const SplW& w, int f) {
+__host__ __device__ inline double spl(const Tab& T, const SplW& w, int f) {
   double ya = T(f, w.lo);
   if (w.deg) return ya;
   double yb = T(f, w.lo + 1);
@@ -110,7 +110,7 @@ struct PS {
   double rew;
   double pibigg;
 };
-__device__ inline void set_l(PS& s, int l) {
+__host__ __device__ inline void set_l(PS& s, int l) {
   s.ll = l;
   s.lld = (double)l;
   s.llp1 = (double)(l + 1);
@@ -120,7 +120,7 @@ __device__ inline void set_l(PS& s, int l) {
   s.szeta2m2 = s.zeta2m2 > 0.0 ? sqrt(s.zeta2m2) : 0.0;
   s.nu = sqrt((double)(float)(2 * l + 1) / FOURPI_D);
 }
-__device__ inline void set_omega(PS& s, cd w, double tref, double ats) {
+__host__ __device__ inline void set_omega(PS& s, cd w, double tref, double ats) {
   s.omega = w;
   s.omega2 = w * w;
   cd oi = 1.0 / w;
@@ -137,7 +137,7 @@ struct SolidA {
   cd t11, t12, k11, k22, s11, s12, s22;
   double t21, t22, t31, t33, k33, s13, s23;
 };
-__device__ inline void solid_amat(const Tab& T, const PS& s, int il, double rr, SolidA& A) {
+__host__ __device__ inline void solid_amat(const Tab& T, const PS& s, int il, double rr, SolidA& A) {
   SplW w = spl_w(T, il, rr);
   double rrho = spl(T, w, F_RHO), racon = spl(T, w, F_ACON), rccon = spl(T, w, F_CCON),
          rfcon = spl(T, w, F_FCON), rncon = spl(T, w, F_NCON), rlcon = spl(T, w, F_LCON),
@@ -170,7 +170,7 @@ __device__ inline void solid_amat(const Tab& T, const PS& s, int il, double rr, 
 // sph_minors_solid_derivs, src/module_int.f90:818-871 (14 minors)
 struct SolidMinorRHS {
   static constexpr int N = 14;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* m,
+  __host__ __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* m,
                                     cd* d) const {
     SolidA A;
     solid_amat(T, s, il, rr, A);
@@ -206,7 +206,7 @@ struct SolidMinorRHS {
 // sph_bvec_solid_derivs, src/module_int.f90:1016-1057 (14 b-vector terms)
 struct BvecRHS {
   static constexpr int N = 14;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* b,
+  __host__ __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* b,
                                     cd* d) const {
     SolidA A;
     solid_amat(T, s, il, rr, A);
@@ -241,7 +241,7 @@ struct BvecRHS {
 // sph_fluid_amat + sph_minors_fluid_derivs, src/module_int.f90:676-715, 941-967
 struct FluidMinorRHS {
   static constexpr int N = 5;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* m,
+  __host__ __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* m,
                                     cd* d) const {
     SplW w = spl_w(T, il, rr);
     double rrho = spl(T, w, F_RHO), rccon = spl(T, w, F_CCON), gg = spl(T, w, F_GRAV),
@@ -270,7 +270,7 @@ struct FluidMinorRHS {
 // tor_amat + tor_derivs, src/module_int.f90:134-184
 struct TorRHS {
   static constexpr int N = 2;
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* y,
+  __host__ __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* y,
                                     cd* d) const {
     SplW w = spl_w(T, il, rr);
     double rrho = spl(T, w, F_RHO), rlcon = spl(T, w, F_LCON), rncon = spl(T, w, F_NCON),
@@ -294,7 +294,7 @@ struct RadRHS {
   struct Coef {
     cd t, k, sv;
   };
-  __device__ inline Coef coef(const Tab& T, const PS& s, int il, double rr) const {
+  __host__ __device__ inline Coef coef(const Tab& T, const PS& s, int il, double rr) const {
     SplW w = spl_w(T, il, rr);
     double rrho = spl(T, w, F_RHO), racon = spl(T, w, F_ACON), rccon = spl(T, w, F_CCON),
            rfcon = spl(T, w, F_FCON), rncon = spl(T, w, F_NCON), gg = spl(T, w, F_GRAV),
@@ -309,11 +309,11 @@ struct RadRHS {
     cd sv = (4.0 * ri * ri) * (gamma - rrho * gg * rr) - rrho * s.omega2;
     return Coef{t, cccon, sv};
   }
-  __device__ static inline void apply(const Coef& c, const cd* y, cd* d) {
+  __host__ __device__ static inline void apply(const Coef& c, const cd* y, cd* d) {
     d[0] = c.t * y[0] + c.k * y[1];
     d[1] = c.sv * y[0] - c.t * y[1];
   }
-  __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* y,
+  __host__ __device__ inline void operator()(const Tab& T, const PS& s, int il, double rr, const cd* y,
                                     cd* d) const {
     apply(coef(T, s, il, rr), y, d);
   }
@@ -333,7 +333,7 @@ struct RadGroup {
 // partial sums as soon as no later stage needs them separately, so at most
 // six N-vectors are live at once.
 template <class F>
-__device__ inline void rk_step(const Tab& T, const PS& s, int il, double x, double h, cd* y,
+__host__ __device__ inline void rk_step(const Tab& T, const PS& s, int il, double x, double h, cd* y,
                                const F& f) {
   constexpr int N = F::N;
   constexpr double a2 = 0.2, a3 = 0.3, a4 = 0.6, a5 = 1.0, a6 = 0.875, b21 = 0.2,
@@ -380,7 +380,7 @@ __device__ inline void rk_step(const Tab& T, const PS& s, int il, double x, doub
 // no barrier; groups of a wave may be at different points of their solves.
 // G = 1 is the plain rk_step.
 template <class F>
-__device__ inline void rk_step_grp(const Tab& T, const PS& s, int il, double x, double h, cd* y,
+__host__ __device__ inline void rk_step_grp(const Tab& T, const PS& s, int il, double x, double h, cd* y,
                                    const F& f, const RadGroup& g) {
   constexpr int N = F::N;
   constexpr double a2 = 0.2, a3 = 0.3, a4 = 0.6, a5 = 1.0, a6 = 0.875, b21 = 0.2,
@@ -434,7 +434,7 @@ __device__ inline void rk_step_grp(const Tab& T, const PS& s, int il, double x, 
 
 // renormalisation of yint, src/module_int.f90:1910-1919
 template <int N>
-__device__ inline void renorm(cd* y, int& scale) {
+__host__ __device__ inline void renorm(cd* y, int& scale) {
   bool small = true, big = false;
 #pragma unroll
   for (int i = 0; i < N; i++) small = small && (cabs2(y[i]) < TINY2);
@@ -453,16 +453,23 @@ __device__ inline void renorm(cd* y, int& scale) {
 }
 
 // steps, src/module_int.f90:2121-2138 (nw = 4)
-__device__ inline double step_len(const Tab& T, const PS& s, int i, int nknot) {
+__host__ __device__ inline double step_len(const Tab& T, const PS& s, int i, int nknot) {
   double a = T(F_ALPHA, i);
   double tmp = (i < nknot) ? fmax(a, T(F_ALPHA, i + 1)) : a;
   return 1.0 / (4.0 * s.rew / tmp);
 }
 
-// yint, src/module_int.f90:1847-1968: fixed equal steps per knot interval
-template <class F, class G = NoGroup>
-__device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, double r1, double r2,
-                     cd* y, const F& f, int* scale, int& nsteps, const G& grp = G()) {
+// a per-step callback of yint that does nothing
+struct NoStepHook {
+  __host__ __device__ inline void operator()(int, double, double, const cd*) const {}
+};
+
+// yint, src/module_int.f90:1847-1968: fixed equal steps per knot interval;
+// sh(interval, radius, step, y) is called before each step
+template <class F, class G = NoGroup, class SH = NoStepHook>
+__host__ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, double r1, double r2,
+                     cd* y, const F& f, int* scale, int& nsteps, const G& grp = G(),
+                     const SH& sh = SH()) {
   if (r2 > r1) {
     for (int i = i1; i <= i2; i++) {
       double rr1 = (i == i1) ? r1 : T(F_R, i);
@@ -474,6 +481,7 @@ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, doubl
       nsteps += n - 1;
       for (int j = 1; j <= n - 1; j++) {
         double rr = rr1 + (double)(j - 1) * drr;
+        sh(i, rr, drr, y);
         if constexpr (std::is_same<G, RadGroup>::value)
           rk_step_grp(T, s, i, rr, drr, y, f, grp);
         else
@@ -492,6 +500,7 @@ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, doubl
       nsteps += n - 1;
       for (int j = 1; j <= n - 1; j++) {
         double rr = rr1 + (double)(j - 1) * drr;
+        sh(i, rr, drr, y);
         if constexpr (std::is_same<G, RadGroup>::value)
           rk_step_grp(T, s, i, rr, drr, y, f, grp);
         else
@@ -504,7 +513,7 @@ __device__ void yint(const Tab& T, const PS& s, int nknot, int i1, int i2, doubl
 
 // ------------------------------------------------------------ start levels
 // start_level, src/module_start.f90:80-178; wt = 1 uses beta, 2 alpha
-__device__ inline int start_level(const Tab& T, int l, double w, int i1, int i2, int wt) {
+__host__ __device__ inline int start_level(const Tab& T, int l, double w, int i1, int i2, int wt) {
   const int fv = (wt == 1) ? F_BETA : F_ALPHA;
   int isign = (i2 >= i1) ? 1 : -1;
   double pp = (double)(float)(l * (l + 1)) / (w * w);
@@ -543,10 +552,10 @@ struct Layers {
 };
 
 // tor_start_level / sph_start_level (up = 1), src/module_start.f90:14-75
-__device__ inline int tor_start_level(const Tab& T, const Layers& L, int l, double w, int it) {
+__host__ __device__ inline int tor_start_level(const Tab& T, const Layers& L, int l, double w, int it) {
   return start_level(T, l, w, L.noc + 1, it, 1);
 }
-__device__ inline int sph_start_level(const Tab& T, const Layers& L, int l, double w, int it,
+__host__ __device__ inline int sph_start_level(const Tab& T, const Layers& L, int l, double w, int it,
                                       double fre_norm) {
   double wdim = w * fre_norm;
   int is;
@@ -567,7 +576,7 @@ __device__ inline int sph_start_level(const Tab& T, const Layers& L, int l, doub
 
 // ------------------------------------------------------ starting solutions
 // bfs + bfs_dum, src/module_function.f90:497-581
-__device__ inline cd bfs_ratio(int l, cd xsq) {
+__host__ __device__ inline cd bfs_ratio(int l, cd xsq) {
   const double eps = 1.e-9;
   cd fp;
   if (xsq.x > 0.0) {
@@ -611,7 +620,7 @@ __device__ inline cd bfs_ratio(int l, cd xsq) {
 }
 
 // sph_solid_start, src/module_int.f90:1192-1270; y[col][row]
-__device__ inline void solid_start(const Tab& T, const PS& s, int is, double rr, cd y[3][6]) {
+__host__ __device__ inline void solid_start(const Tab& T, const PS& s, int is, double rr, cd y[3][6]) {
   SplW w = spl_w(T, is, rr);
   double rrho = spl(T, w, F_RHO), rmu = spl(T, w, F_MU), rqmu = spl(T, w, F_QMU),
          rkappa = spl(T, w, F_KAPPA), rqkappa = spl(T, w, F_QKAPPA);
@@ -654,7 +663,7 @@ __device__ inline void solid_start(const Tab& T, const PS& s, int is, double rr,
 }
 
 // sph_fluid_start, src/module_int.f90:1346-1405; y[col][row]
-__device__ inline void fluid_start(const Tab& T, const PS& s, int is, double rr, cd y[2][4]) {
+__host__ __device__ inline void fluid_start(const Tab& T, const PS& s, int is, double rr, cd y[2][4]) {
   SplW w = spl_w(T, is, rr);
   double rrho = spl(T, w, F_RHO), rkappa = spl(T, w, F_KAPPA), rqkappa = spl(T, w, F_QKAPPA);
   double pib = s.pibigg, ll = s.lld, llp1 = s.llp1, zeta2 = s.zeta2;
@@ -677,10 +686,10 @@ __device__ inline void fluid_start(const Tab& T, const PS& s, int is, double rr,
   y[1][3] = (0.25 / pib) * ((2.0 * ll + 1.0) * s.omega2 - 8.0 * pib * ll * (ll - 1.0) * rrho / 3.0);
 }
 
-__device__ inline cd det2(cd a1, cd a2, cd b1, cd b2) { return a1 * b2 - a2 * b1; }
+__host__ __device__ inline cd det2(cd a1, cd a2, cd b1, cd b2) { return a1 * b2 - a2 * b1; }
 
 // sph_minor_solid_start, src/module_int.f90:1451-1485 (14 3x3 minors)
-__device__ inline void minor_solid_start(const Tab& T, const PS& s, int is, double rr, cd* m) {
+__host__ __device__ inline void minor_solid_start(const Tab& T, const PS& s, int is, double rr, cd* m) {
   cd y[3][6];
   solid_start(T, s, is, rr, y);
   const int rows[14][3] = {{0, 1, 2}, {0, 1, 3}, {0, 1, 4}, {0, 1, 5}, {0, 2, 3},
@@ -696,7 +705,7 @@ __device__ inline void minor_solid_start(const Tab& T, const PS& s, int is, doub
 }
 
 // sph_minor_fluid_start, src/module_int.f90:1509-1534 (5 2x2 minors)
-__device__ inline void minor_fluid_start(const Tab& T, const PS& s, int is, double rr, cd* m) {
+__host__ __device__ inline void minor_fluid_start(const Tab& T, const PS& s, int is, double rr, cd* m) {
   cd y[2][4];
   fluid_start(T, s, is, rr, y);
   const int rows[5][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {2, 3}};
@@ -708,7 +717,7 @@ __device__ inline void minor_fluid_start(const Tab& T, const PS& s, int is, doub
 }
 
 // tor_ystart with the sbessj2 stub, src/module_int.f90:213-250
-__device__ inline void tor_ystart(const Tab& T, const PS& s, int is, double rr, cd* y) {
+__host__ __device__ inline void tor_ystart(const Tab& T, const PS& s, int is, double rr, cd* y) {
   SplW w = spl_w(T, is, rr);
   double rrho = spl(T, w, F_RHO), rmu = spl(T, w, F_MU), rqmu = spl(T, w, F_QMU);
   cd cmu = rmu * (1.0 + rqmu * s.lno);
@@ -720,7 +729,7 @@ __device__ inline void tor_ystart(const Tab& T, const PS& s, int is, double rr, 
 }
 
 // rad_ystart with the sbessj2 stub, src/module_int.f90:417-452
-__device__ inline void rad_ystart(const Tab& T, const PS& s, int is, double rr, cd* y) {
+__host__ __device__ inline void rad_ystart(const Tab& T, const PS& s, int is, double rr, cd* y) {
   SplW w = spl_w(T, is, rr);
   double rrho = spl(T, w, F_RHO), rmu = spl(T, w, F_MU), rqmu = spl(T, w, F_QMU),
          rkappa = spl(T, w, F_KAPPA), rqkappa = spl(T, w, F_QKAPPA);
@@ -736,17 +745,17 @@ __device__ inline void rad_ystart(const Tab& T, const PS& s, int is, double rr, 
 
 // ----------------------------------------------------------- minor algebra
 // sph_minors_solid_2_fluid / fluid_2_solid, src/module_int.f90:1620-1680
-__device__ inline void solid_2_fluid(const cd* a, cd* f) {
+__host__ __device__ inline void solid_2_fluid(const cd* a, cd* f) {
   f[0] = -a[5]; f[1] = -a[6]; f[2] = a[8]; f[3] = -a[12]; f[4] = a[13];
 }
-__device__ inline void fluid_2_solid(const cd* f, cd* a) {
+__host__ __device__ inline void fluid_2_solid(const cd* f, cd* a) {
 #pragma unroll
   for (int i = 0; i < 14; i++) a[i] = C(0.0);
   a[0] = f[0]; a[1] = f[1]; a[3] = f[2]; a[9] = -f[3]; a[11] = -f[4];
 }
 
 // sph_delta_solid, src/module_int.f90:1705-1725
-__device__ inline cd delta_solid(const cd* a, const cd* b) {
+__host__ __device__ inline cd delta_solid(const cd* a, const cd* b) {
   return (a[0] * b[13] - a[13] * b[0]) + 2.0 * (a[6] * b[1] - a[1] * b[6]) +
          2.0 * (a[10] * b[2] - a[2] * b[10]) + (a[12] * b[3] - a[3] * b[12]) +
          2.0 * (a[7] * b[4] - a[4] * b[7]) + (a[11] * b[5] - a[5] * b[11]) +
@@ -755,7 +764,7 @@ __device__ inline cd delta_solid(const cd* a, const cd* b) {
 
 // b_solid_start for the unit source column e_k (k = 0,1,3,4 for the four
 // columns of src/yspec.f90:234-238), src/module_int.f90:1563-1592
-__device__ inline void b_start_unit(const cd* m, int k, cd* b) {
+__host__ __device__ inline void b_start_unit(const cd* m, int k, cd* b) {
 #pragma unroll
   for (int i = 0; i < 14; i++) b[i] = C(0.0);
   if (k == 0) {
@@ -774,7 +783,7 @@ __device__ inline void b_start_unit(const cd* m, int k, cd* b) {
 }
 
 // sph_ysol, src/module_int.f90:1785-1815
-__device__ inline void ysol(const cd* m, const cd* b, cd* y) {
+__host__ __device__ inline void ysol(const cd* m, const cd* b, cd* y) {
   y[0] = m[2] * b[2] + m[0] * b[5] + m[1] * b[6] + 2.0 * (m[2] * b[7]) + m[3] * b[8] +
          m[4] * b[9] + m[5] * b[10] + m[6] * b[11] + m[7] * b[12] + m[8] * b[13];
   y[1] = m[9] * b[9] + m[10] * b[11] + m[11] * b[12] - m[0] * b[1] -
@@ -789,7 +798,47 @@ __device__ inline void ysol(const cd* m, const cd* b, cd* y) {
          m[8] * b[3] - m[10] * b[9] + m[11] * b[6] + m[13] * b[11];
 }
 
-__device__ inline double ipow_huge(int n) {
+__host__ __device__ inline cd pick3(int k, cd a, cd b, cd c) {
+  return (k == 0) ? a : ((k == 1) ? b : ((k == 2) ? c : C(0.0)));
+}
+// coefficients of b in row k = 0, 1, 2 of sph_ysol (src/module_int.f90:1785-1815)
+__host__ __device__ inline void ysol_row(const cd* m, int k, cd* x) {
+  const cd z = C(0.0);
+  x[0] = pick3(k, z, z, m[0]);
+  x[1] = pick3(k, z, -m[0], z);
+  x[2] = pick3(k, m[2], -2.0 * m[1], -m[4]);
+  x[3] = pick3(k, z, -m[2], -m[5]);
+  x[4] = pick3(k, z, -m[3], m[2]);
+  x[5] = pick3(k, m[0], z, z);
+  x[6] = pick3(k, m[1], z, -m[9]);
+  x[7] = pick3(k, 2.0 * m[2], -m[1], m[4]);
+  x[8] = pick3(k, m[3], z, -m[1]);
+  x[9] = pick3(k, m[4], m[9], z);
+  x[10] = pick3(k, m[5], -m[4], z);
+  x[11] = pick3(k, m[6], m[10], m[12]);
+  x[12] = pick3(k, m[7], m[11], -m[10]);
+  x[13] = pick3(k, m[8], -m[7], m[6]);
+}
+
+__host__ __device__ inline void delta_row(const cd* m2, cd* x) {
+  // coefficients of m1 in sph_delta_solid(m1, m2), src/module_int.f90:1705-1725
+  x[0] = m2[13];
+  x[1] = -2.0 * m2[6];
+  x[2] = -2.0 * m2[10];
+  x[3] = -m2[12];
+  x[4] = -2.0 * m2[7];
+  x[5] = -m2[11];
+  x[6] = 2.0 * m2[1];
+  x[7] = 2.0 * m2[4];
+  x[8] = m2[9];
+  x[9] = -m2[8];
+  x[10] = 2.0 * m2[2];
+  x[11] = m2[5];
+  x[12] = m2[3];
+  x[13] = -m2[0];
+}
+
+__host__ __device__ inline double ipow_huge(int n) {
   double r = 1.0;
   if (n > 0)
     for (int i = 0; i < n; i++) r *= HUGE_;

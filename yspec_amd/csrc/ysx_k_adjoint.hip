@@ -1,0 +1,1009 @@
+// ysx_k_adjoint.hip -- the reverse sweeps of the discrete adjoint: the
+// gradient of Re sum(conj(record adjoint) * record) over all (l, w) pairs
+// with respect to the model table, through the per-pair solves.
+//
+// Every march is a frozen list of Cash-Karp steps (the reference's step
+// rule, yint, src/module_int.f90:1847-1968), so a solve is a product of
+// step matrices and its adjoint pairs, step by step, the column the forward
+// carries (the solution) with a row carried the other way (the adjoint).
+// A wave keeps the forward's lockstep layout (64 lanes at one w, stage
+// radii shared) and for each item:
+//   1. marches the stored side again from its start, writing its state at
+//      every knot interval boundary to the block's checkpoint slot;
+//   2. goes through the intervals in the other direction: per interval the
+//      stored side is recomputed from its checkpoint with every stage's
+//      input (or row) written to the block's tape, then the other side is
+//      stepped through the interval, each stage paired with the tape
+//      (ysx_adjoint.hpp) into adjoints of the stage's base;
+//   3. per chunk of steps, sums those adjoints over the wave (LDS), takes
+//      them through the base computation to the spline values and the
+//      interval's table entries (one lane per stage radius), and adds them
+//      to the table adjoint at the end of the interval (global atomics).
+// Kernels:
+//   k_brow_adj   b-vector system: the receiver-side rows (stored side) and
+//                the source-side columns weighted by the record adjoints
+//   k_minor_adj  minor vectors: the columns from the start level (stored
+//                side) and the row of the secular determinant and b-vector
+//                starts; writes the adjoint of each pair's start solution
+//   k_tor_adj    toroidal system, the same way
+//   k_start_adj  starting solutions: forward-mode tangents with respect to
+//                the spline values at the start radius
+//   k_rad_adj    the radial (l = 0) solves, one lane per frequency: every
+//                step of both sweeps on a per-lane tape, then reversed
+#include "ysx_adjoint.hpp"
+#include "ysx_kern.hpp"
+#include "ysx_radial.hpp"
+
+namespace ysx {
+namespace {
+
+// ------------------------------------------------------------ systems
+struct SysSolidMinor {
+  using F = SolidMinorM;
+  using FT = MinorTM;
+  using Abar = SolidAbar;
+  static constexpr int NOB = SOLID_OB, NV = SOLID_NV;
+  __device__ static void bil(const cd* w, const cd* z, Abar& B) { solid_minor_bil(w, z, B); }
+  __device__ static void ob(const Abar& B, const LS& L, const double* b, double* o) {
+    solid_ob(B, L, b, o);
+  }
+  __device__ static void adj(const Tab& T, const WS& w, int il, double r, double h,
+                             const double* ob, double* vb) {
+    solid_base_adj(T, w, il, r, h, ob, vb);
+  }
+  __device__ static int field(int v) { return solid_field(v); }
+};
+struct SysBvec : SysSolidMinor {
+  using F = BvecM;
+  using FT = BvecTM;
+  __device__ static void bil(const cd* w, const cd* z, Abar& B) { bvec_bil(w, z, B); }
+};
+struct SysFluidMinor {
+  using F = FluidMinorM;
+  using FT = FluidMinorT;
+  using Abar = FluidAbar;
+  static constexpr int NOB = FLUID_OB, NV = FLUID_NV;
+  __device__ static void bil(const cd* w, const cd* z, Abar& B) { fluid_minor_bil(w, z, B); }
+  __device__ static void ob(const Abar& B, const LS& L, const double* b, double* o) {
+    fluid_ob(B, L, b, o);
+  }
+  __device__ static void adj(const Tab& T, const WS& w, int il, double r, double h,
+                             const double* ob, double* vb) {
+    fluid_base_adj(T, w, il, r, h, ob, vb);
+  }
+  __device__ static int field(int v) { return fluid_field(v); }
+};
+struct SysTor {
+  using F = TorM;
+  using FT = TorT;
+  using Abar = TorAbar;
+  static constexpr int NOB = TOR_OB, NV = TOR_NV;
+  __device__ static void bil(const cd* w, const cd* z, Abar& B) { tor_bil(w, z, B); }
+  __device__ static void ob(const Abar& B, const LS& L, const double* b, double* o) {
+    tor_ob(B, L, b, o);
+  }
+  __device__ static void adj(const Tab& T, const WS& w, int il, double r, double h,
+                             const double* ob, double* vb) {
+    tor_base_adj(T, w, il, r, h, ob, vb);
+  }
+  __device__ static int field(int v) { return tor_field(v); }
+};
+
+constexpr int NOB_MAX = SOLID_OB;
+constexpr int NV_MAX = SOLID_NV;
+
+// H^n for the renormalisation counters of a row and a column (the
+// factor between the stored product and the true one); beyond 15 powers
+// one side is zero (a lane outside its solve, whose counter ran on), and
+// the pairing counts nothing
+__device__ inline double scale_pow(int n) {
+  if (n > 15 || n < -15) return 0.0;
+  return ipow_huge(n);
+}
+
+// ------------------------------------------------------ LDS of a block
+// The first pass of an item (the forward's own march, re-run with
+// checkpoints) needs the forward's two stage slots per lane; the second
+// (the interval sweeps) the reduction buffers in the same bytes.
+struct AdjLds {
+  double sb[6 * CH * BASE_MAX];     // stage bases of a chunk of steps
+  double tacc[NV_MAX * 4];          // the interval's table-entry adjoints
+  union {
+    cd slots[2 * 14 * 64];          // rk_step_lds / rk_step_T_lds stage slots
+    struct {
+      double red[NOB_MAX * 64];       // per-lane stage adjoints being summed
+      double bbar[6 * CH * NOB_MAX];  // summed stage adjoints of the chunk
+    } r;
+  } u;
+};
+static_assert(sizeof(AdjLds) <= 40 * 1024, "four blocks per CU");
+
+// bases of steps j0 .. j0 + nst - 1 (1-based) of the interval's step list,
+// one lane per stage radius, as march_interval lays them out
+template <class F>
+__device__ inline void load_bases(const Tab& T, const WS& w, int i, double ra, double drr,
+                                  int j0, int nst, double* sb) {
+  int q = threadIdx.x;
+  if (q < 6 * nst) {
+    int jj = j0 + q / 6, k = q % 6;
+    double x = ra + (double)(jj - 1) * drr;
+    F::base(T, w, i, x + stage_offset(k) * drr, drr * stage_scale<F::N>(k), sb + q * F::BS);
+  }
+}
+
+// dst[q] += sum over the wave's lanes of v[q], q < NOB
+template <int NOB>
+__device__ inline void wave_sum_into(const double* v, double* red, double* dst) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < NOB; q++) red[q * 64 + t] = v[q];
+  __syncthreads();
+  if (t < 2 * NOB) {
+    const double* r = red + (t >> 1) * 64 + (t & 1) * 32;
+    double s = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < 32; k++) s += r[k];
+    s += __shfl_xor(s, 1, 64);
+    if ((t & 1) == 0) dst[t >> 1] += s;
+  }
+  __syncthreads();
+}
+
+// one stage's pairing: row kb with column z, scaled by fs, summed into dst
+template <class Sys>
+__device__ inline void pair_stage(const cd* kb, const cd* z, const LS& L, const double* base,
+                                  double fs, AdjLds& S, double* dst) {
+  typename Sys::Abar B;
+  zero(B);
+  Sys::bil(kb, z, B);
+  double ob[Sys::NOB];
+  Sys::ob(B, L, base, ob);
+#pragma unroll
+  for (int v = 0; v < Sys::NOB; v++) ob[v] *= fs;
+  wave_sum_into<Sys::NOB>(ob, S.u.r.red, dst);
+}
+
+// The chunk's summed stage adjoints through the base computation to the
+// interval's table entries (lane q: stage radius q), into S.tacc.
+template <class Sys>
+__device__ inline void chain_chunk(const Tab& T, const WS& w, int i, double ra, double drr,
+                                   int j0, int nst, AdjLds& S) {
+  using F = typename Sys::F;
+  int q = threadIdx.x;
+  if (q < 6 * nst) {
+    int jj = j0 + q / 6, k = q % 6;
+    double x = ra + (double)(jj - 1) * drr + stage_offset(k) * drr;
+    double vb[Sys::NV];
+    Sys::adj(T, w, i, x, drr * stage_scale<F::N>(k), S.u.r.bbar + q * Sys::NOB, vb);
+    SplW sw = spl_w(T, i, x);
+#pragma unroll
+    for (int v = 0; v < Sys::NV; v++) {
+      const int f0 = Sys::field(v);
+      spl_adj(sw, f0, vb[v], [&](int f, int knot, double val) {
+        atomicAdd(&S.tacc[v * 4 + (f == f0 ? 0 : 2) + (knot - i)], val);
+      });
+    }
+  }
+}
+
+// S.tacc of interval i into the table adjoint (field-major, nk knots)
+template <class Sys>
+__device__ inline void flush_interval(int i, int nk, AdjLds& S, double* tbar) {
+  __syncthreads();
+  int t = threadIdx.x;
+  if (t < Sys::NV * 4) {
+    int v = t / 4, f = Sys::field(v) + ((t / 2) % 2), knot = i + (t % 2);
+    double a = S.tacc[t];
+    if (a != 0.0) atomicAdd(&tbar[(size_t)f * nk + knot - 1], a);
+    S.tacc[t] = 0.0;
+  }
+  __syncthreads();
+}
+
+__device__ inline cd* ck_of(const AdjWork& W, int slot) {
+  return W.ck + ((size_t)blockIdx.x * W.slots + slot) * 14 * 64 + threadIdx.x;
+}
+__device__ inline int* cks_of(const AdjWork& W, int slot) {
+  return W.cks + ((size_t)blockIdx.x * W.slots + slot) * 64 + threadIdx.x;
+}
+__device__ inline cd* tape_of(const AdjWork& W, int s) {
+  return W.tape + ((size_t)blockIdx.x * W.tmax + s) * 6 * 14 * 64 + threadIdx.x;
+}
+__device__ inline int* tsc_of(const AdjWork& W, int s) {
+  return W.tsc + ((size_t)blockIdx.x * W.tmax + s) * 64 + threadIdx.x;
+}
+template <int N>
+__device__ inline void put_ck(const AdjWork& W, int slot, const cd* y, int sc) {
+  cd* c = ck_of(W, slot);
+#pragma unroll
+  for (int q = 0; q < N; q++) c[q * 64] = y[q];
+  *cks_of(W, slot) = sc;
+}
+
+__device__ inline int steps_of(const Tab& T, const WS& w, int i, int nk, double ra, double rb,
+                               double& drr) {
+  double drr0 = step_len_w(T, w.rew, i, nk);
+  int n = 2 + (int)floor(fabs(rb - ra) / drr0);
+  drr = (rb - ra) / (double)(n - 1);
+  return n - 1;
+}
+
+// ------------------------------------------------ one interval, row sweep
+// Interval i of a march whose column went ra -> rb: the column's steps are
+// recomputed from its checkpoint (slot, with its counter) with every stage
+// input on the tape; then the row x (counter sx) is taken back through the
+// steps, last first, each step's stages paired with the tape.
+template <class Sys>
+__device__ void rev_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
+                             double rb, cd* x, int& sx, const AdjWork& W, int slot, AdjLds& S,
+                             double* tbar) {
+  using F = typename Sys::F;
+  constexpr int N = F::N, BS = F::BS;
+  double drr;
+  const int ns = steps_of(T, w, i, nk, ra, rb, drr);
+  if (ns > W.tmax) {
+    if (threadIdx.x == 0) atomicOr(W.err, 1);
+    return;
+  }
+  {
+    cd y[N];
+    const cd* c = ck_of(W, slot);
+#pragma unroll
+    for (int q = 0; q < N; q++) y[q] = c[q * 64];
+    int sc = *cks_of(W, slot);
+    for (int j0 = 1; j0 <= ns; j0 += CH) {
+      int nst = min(CH, ns + 1 - j0);
+      load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+      __syncthreads();
+      for (int jj = 0; jj < nst; jj++) {
+        int s = j0 - 1 + jj;
+        *tsc_of(W, s) = sc;
+        cd* tp = tape_of(W, s);
+        rk_step_hook(S.sb + jj * 6 * BS, L, w, y, F(), [&](int k, const cd* z) {
+#pragma unroll
+          for (int q = 0; q < N; q++) tp[(k * 14 + q) * 64] = z[q];
+        });
+        renorm<N>(y, sc);
+      }
+      __syncthreads();
+    }
+  }
+  const int jlast = 1 + ((ns - 1) / CH) * CH;
+  for (int j0 = jlast; j0 >= 1; j0 -= CH) {
+    int nst = min(CH, ns + 1 - j0);
+    if (j0 != jlast) load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+    for (int q = threadIdx.x; q < 6 * nst * Sys::NOB; q += 64) S.u.r.bbar[q] = 0.0;
+    __syncthreads();
+    for (int jj = nst - 1; jj >= 0; jj--) {
+      int s = j0 - 1 + jj;
+      const cd* tp = tape_of(W, s);
+      const double fs = scale_pow(-(sx + *tsc_of(W, s)));
+      const double* bs = S.sb + jj * 6 * BS;
+      rk_step_T_reg(bs, L, w, x, typename Sys::FT(), [&](int k, const cd* kb) {
+        cd z[N];
+#pragma unroll
+        for (int q = 0; q < N; q++) z[q] = tp[(k * 14 + q) * 64];
+        pair_stage<Sys>(kb, z, L, bs + k * BS, fs, S, S.u.r.bbar + (jj * 6 + k) * Sys::NOB);
+      });
+      renorm<N>(x, sx);
+    }
+    chain_chunk<Sys>(T, w, i, ra, drr, j0, nst, S);
+    __syncthreads();
+  }
+  flush_interval<Sys>(i, nk, S, tbar);
+}
+
+// ------------------------------------------- one interval, column sweep
+// Interval i of a march whose column goes ra -> rb and whose row came
+// rb -> ra: the row's transposed steps are recomputed from its checkpoint
+// at rb (slot) with every stage row on the tape; then the column c
+// (counter sc) is stepped through the interval, each stage paired with the
+// tape.
+template <class Sys>
+__device__ void fwd_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
+                             double rb, cd* c, int& sc, const AdjWork& W, int slot, AdjLds& S,
+                             double* tbar) {
+  using F = typename Sys::F;
+  constexpr int N = F::N, BS = F::BS;
+  double drr;
+  const int ns = steps_of(T, w, i, nk, ra, rb, drr);
+  if (ns > W.tmax) {
+    if (threadIdx.x == 0) atomicOr(W.err, 1);
+    return;
+  }
+  const int jlast = 1 + ((ns - 1) / CH) * CH;
+  {
+    cd x[N];
+    const cd* ck = ck_of(W, slot);
+#pragma unroll
+    for (int q = 0; q < N; q++) x[q] = ck[q * 64];
+    int sl = *cks_of(W, slot);
+    for (int j0 = jlast; j0 >= 1; j0 -= CH) {
+      int nst = min(CH, ns + 1 - j0);
+      load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+      __syncthreads();
+      for (int jj = nst - 1; jj >= 0; jj--) {
+        int s = j0 - 1 + jj;
+        *tsc_of(W, s) = sl;
+        cd* tp = tape_of(W, s);
+        rk_step_T_reg(S.sb + jj * 6 * BS, L, w, x, typename Sys::FT(), [&](int k, const cd* kb) {
+#pragma unroll
+          for (int q = 0; q < N; q++) tp[(k * 14 + q) * 64] = kb[q];
+        });
+        renorm<N>(x, sl);
+      }
+      __syncthreads();
+    }
+  }
+  for (int j0 = 1; j0 <= ns; j0 += CH) {
+    int nst = min(CH, ns + 1 - j0);
+    if (j0 != 1) load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+    for (int q = threadIdx.x; q < 6 * nst * Sys::NOB; q += 64) S.u.r.bbar[q] = 0.0;
+    __syncthreads();
+    for (int jj = 0; jj < nst; jj++) {
+      int s = j0 - 1 + jj;
+      const cd* tp = tape_of(W, s);
+      const double fs = scale_pow(-(sc + *tsc_of(W, s)));
+      const double* bs = S.sb + jj * 6 * BS;
+      rk_step_hook(bs, L, w, c, F(), [&](int k, const cd* z) {
+        cd kb[N];
+#pragma unroll
+        for (int q = 0; q < N; q++) kb[q] = tp[(k * 14 + q) * 64];
+        pair_stage<Sys>(kb, z, L, bs + k * BS, fs, S, S.u.r.bbar + (jj * 6 + k) * Sys::NOB);
+      });
+      renorm<N>(c, sc);
+    }
+    chain_chunk<Sys>(T, w, i, ra, drr, j0, nst, S);
+    __syncthreads();
+  }
+  flush_interval<Sys>(i, nk, S, tbar);
+}
+
+// ------------------------------------------------------- record adjoints
+__device__ inline cd conj_(cd a) { return C(a.x, -a.y); }
+
+// rows b of b_solid_start's map bst = U_k m for the unit source column e_k
+// (b_start_unit): mrow += coef * (x U_k)
+__device__ inline void b_start_unit_T(const cd* x, int k, cd coef, cd* mrow) {
+  auto add = [&](int q, cd v) { mrow[q] = cmad(mrow[q], coef, v); };
+  if (k == 0) {
+    add(13, x[5]); add(6, -x[6]); add(10, -x[7]); add(12, -x[8]); add(7, -x[9]);
+    add(11, -x[10]); add(1, x[11]); add(4, x[12]); add(9, x[13]);
+  } else if (k == 1) {
+    add(13, -x[1]); add(6, x[2]); add(10, x[3]); add(12, x[4]); add(8, -x[9]);
+    add(7, x[10]); add(2, x[11]); add(5, x[12]); add(4, -x[13]);
+  } else if (k == 3) {
+    add(6, -x[0]); add(7, -x[1]); add(1, -x[3]); add(4, -x[4]); add(8, -x[5]);
+    add(2, -x[7]); add(5, -x[8]); add(3, -x[10]); add(0, -x[13]);
+  } else {
+    add(10, -x[0]); add(11, -x[1]); add(1, x[2]); add(9, -x[4]); add(7, x[5]);
+    add(2, x[6]); add(4, x[8]); add(3, x[9]); add(0, x[12]);
+  }
+}
+
+// ---------------------------------------------------------- k_brow_adj
+// The b-vector part of the gradient (sph_solver's b-vector integration,
+// src/module_solver.f90:745-755, as the receiver-side row march of
+// k_brow_march): with the rows x_k (k < rows) marched from the receiver to
+// the source by Q's transposed steps and y_kc = fac H^(sc3 - sl) x_k Q bst_c,
+// the record adjoints a_kc = conj(ybar_kc) give
+//   dJ = Re sum_k x_k dQ beta_k,  beta_k = fac H^sc3 sum_c a_kc bst_c,
+// a column marched from the source up.  The rows are the stored side
+// (checkpoints at each interval's top); the columns sweep up.  Also writes
+// mub[p], the adjoint of the minors at the source through the b-vector
+// starts: sum_kc a_kc fac H^(sc3 - sl) (x_k Q) U_c.
+__device__ __forceinline__ void brow_adj_item(const Tab& T, const RunK& R, int j, int lb,
+                                              const SphState& SS, const cd* __restrict__ rec,
+                                              const cd* __restrict__ recb, cd* __restrict__ mub,
+                                              const AdjWork& W, AdjLds& S, double* tbar) {
+  const int rows = 64 / R.bl;
+  const int k = threadIdx.x % rows;
+  const int l = R.lmin + lb * R.bl + (int)(threadIdx.x / rows);
+  const bool valid = (l >= 1 && l <= R.lmax);
+  if (wave_min(valid ? 0 : 1)) return;
+  const size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  const WS w = wave_omega(R, j);
+  const LS L = l_state(valid ? l : 1);
+  const int nk = R.nknot, is = R.is, ir = R.ir;
+  const double rs = R.rs, rr = R.rr;
+  cd x[14];
+  {
+    cd m[14];
+    for (int q = 0; q < 14; q++) m[q] = valid ? SS.mfin[p * 14 + q] : C(0.0);
+    ysol_row(m, k, x);
+  }
+  // 1. the rows down from the receiver, a checkpoint at each interval's top
+  int sl = 0;
+  for (int i = ir; i >= is; i--) {
+    double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+    put_ck<14>(W, ir - i, x, sl);
+    if (a != b)
+      march_interval_T(T, w, L, nk, i, a, b, x, sl, S.sb, BvecTM(), S.u.slots + threadIdx.x);
+  }
+  __syncthreads();
+  // record adjoints of this lane's row
+  cd acoef[4];
+  cd fac = valid ? SS.fac[p] : C(0.0);
+  const int sc3 = valid ? SS.sc3[p] : 0;
+  for (int c = 0; c < 4; c++) {
+    cd a = (valid && k < 3) ? conj_(recb[p * REC + O_YSPH + 6 * c + k]) : C(0.0);
+    acoef[c] = cmul_(a, fac);
+  }
+  // the minors' adjoint at the source through the b-vector starts
+  {
+    const int cols[4] = {0, 1, 3, 4};
+    cd mrow[14];
+    for (int q = 0; q < 14; q++) mrow[q] = C(0.0);
+    const double hs = scale_pow(sc3 - sl);
+    for (int c = 0; c < 4; c++) b_start_unit_T(x, cols[c], acoef[c] * hs, mrow);
+    for (int o = 1; o < rows; o <<= 1)
+      for (int q = 0; q < 14; q++) {
+        mrow[q].x += __shfl_xor(mrow[q].x, o, 64);
+        mrow[q].y += __shfl_xor(mrow[q].y, o, 64);
+      }
+    if (valid && k == 0)
+      for (int q = 0; q < 14; q++) mub[p * 14 + q] = mrow[q];
+  }
+  // 2. the column beta_k = fac sum_c a_kc bst_c (true value H^sc3 times
+  // this: counter -sc3) up from the source, paired with the rows
+  cd bcol[14];
+  for (int q = 0; q < 14; q++) {
+    cd v = C(0.0);
+    if (valid)
+      for (int c = 0; c < 4; c++) v = cmad(v, acoef[c], SS.bst[p * 56 + c * 14 + q]);
+    bcol[q] = v;
+  }
+  int scb = -sc3;
+  for (int i = is; i <= ir; i++) {
+    double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+    if (a != b) fwd_interval<SysBvec>(T, w, L, nk, i, a, b, bcol, scb, W, ir - i, S, tbar);
+  }
+}
+
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_brow_adj(const double* __restrict__ gtab,
+                                                 const RunK* __restrict__ runs, Queue Q,
+                                                 SphState SS, const cd* __restrict__ rec,
+                                                 const cd* __restrict__ recb, cd* __restrict__ mub,
+                                                 AdjWork W, double* __restrict__ tbar) {
+  __shared__ AdjLds S;
+  if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
+  __syncthreads();
+  RUN_ITEMS(PERSIST, Q,
+            brow_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SS, rec, recb, mub,
+                          W, S, tbar));
+}
+
+// --------------------------------------------------------- k_minor_adj
+// The minor part (sph_solver, grav = 2, receiver above the source,
+// src/module_solver.f90:640-743): the column is the forward's minor march
+// (ms1 from the start level to the source, ms3 on to the receiver); the
+// records depend on it through the secular determinant (fac = -1/delta,
+// delta = d . ms3 with d from the surface minors) and the b-vector starts
+// (bst_c = U_c ms1).  The row starts at the receiver as G d (G = fac sum_kc
+// a_kc y_kc), is carried back to the source, gets mub (k_brow_adj) added,
+// and is carried on down to each lane's start level, where it is the
+// adjoint of the lane's starting minors (m0bar, for k_start_adj).
+__device__ __forceinline__ bool fluid_at(const RunK& R, int i) {
+  return R.fic ? (i < R.noc) : (i > R.nic && i < R.noc);
+}
+
+__device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int j, int lb,
+                                               const StartData& SD, const SphState& SS,
+                                               const cd* __restrict__ rec,
+                                               const cd* __restrict__ recb,
+                                               const cd* __restrict__ mub, cd* __restrict__ m0bar,
+                                               const AdjWork& W, AdjLds& S, double* tbar) {
+  const int l = R.lmin + lb * 64 + (int)threadIdx.x;
+  const bool valid = (l >= 1 && l <= R.lmax);
+  const size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  const int i1 = valid ? SD.sph_i1[p] : NOLANE;
+  const int ib = wave_min(i1);
+  if (ib == NOLANE) return;
+  const WS w = wave_omega(R, j);
+  const LS L = l_state(valid ? l : 1);
+  const int nk = R.nknot, is = R.is, ir = R.ir;
+  const double rs = R.rs, rr = R.rr;
+  const double rt1 = fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2));
+  // 1. the forward's march (minor_item), checkpoint i - 1 at the start of
+  // interval i below the source, i above it
+  cd m[14];
+  for (int q = 0; q < 14; q++) m[q] = C(0.0);
+  int layout = -1, sc1 = 0;
+  for (int i = ib; i <= is; i++) {
+    const bool fl = fluid_at(R, i);
+    if (layout == 0 && fl) {
+      cd mf[5];
+      solid_2_fluid(m, mf);
+      for (int q = 0; q < 14; q++) m[q] = q < 5 ? mf[q] : C(0.0);
+    } else if (layout == 1 && !fl) {
+      cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+      fluid_2_solid(mf, m);
+    }
+    layout = fl ? 1 : 0;
+    if (i == i1)
+      for (int q = 0; q < 14; q++) m[q] = SD.sph_m0[p * 14 + q];
+    const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+    put_ck<14>(W, i - 1, m, sc1);
+    if (a == b) continue;
+    if (fl)
+      march_interval(T, w, L, nk, i, a, b, m, true, sc1, S.sb, FluidMinorM());
+    else
+      march_interval(T, w, L, nk, i, a, b, m, true, sc1, S.sb, SolidMinorM(),
+                     S.u.slots + threadIdx.x);
+  }
+  int sc3 = 0;
+  for (int i = is; i <= ir; i++) {
+    const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+    put_ck<14>(W, i, m, sc3);
+    if (a != b)
+      march_interval(T, w, L, nk, i, a, b, m, true, sc3, S.sb, SolidMinorM(),
+                     S.u.slots + threadIdx.x);
+  }
+  __syncthreads();
+  if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
+  __syncthreads();
+  // the row at the receiver: G d on ms3 (true value H^sc3 times: counter -sc3)
+  cd x[14];
+  {
+    const int rows = (R.out == 2 && R.cor != 0) ? 3 : 2;
+    cd g = C(0.0);
+    if (valid)
+      for (int c = 0; c < 4; c++)
+        for (int k = 0; k < rows; k++) {
+          const size_t f = p * REC + O_YSPH + 6 * c + k;
+          g = cmad(g, conj_(recb[f]), rec[f]);
+        }
+    g = valid ? cmul_(SS.fac[p], g) : C(0.0);
+    cd m2[14];
+    for (int q = 0; q < 14; q++) m2[q] = valid ? SS.mfin[p * 14 + q] : C(0.0);
+    delta_row(m2, x);
+    for (int q = 0; q < 14; q++) x[q] = cmul_(g, x[q]);
+  }
+  int sx = -sc3;
+  // 2. back from the receiver to the source
+  for (int i = ir; i >= is; i--) {
+    const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+    if (a != b) rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, i, S, tbar);
+  }
+  // at the source: the row on ms1, plus the b-vector starts' part
+  {
+    const double h = scale_pow(-sx);
+    for (int q = 0; q < 14; q++)
+      x[q] = valid ? x[q] * h + mub[p * 14 + q] : C(0.0);
+    sx = -sc1;
+  }
+  // 3. back to the start levels
+  for (int i = is; i >= ib; i--) {
+    const bool fl = fluid_at(R, i);
+    const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+    if (a != b) {
+      if (fl)
+        rev_interval<SysFluidMinor>(T, w, L, nk, i, a, b, x, sx, W, i - 1, S, tbar);
+      else
+        rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, i - 1, S, tbar);
+    }
+    if (i == i1) {
+      const double h = scale_pow(-(sx + *cks_of(W, i - 1)));
+      for (int q = 0; q < 14; q++) {
+        m0bar[p * 14 + q] = x[q] * h;
+        x[q] = C(0.0);
+      }
+    }
+    if (i > ib && fluid_at(R, i - 1) != fl) {
+      if (fl) {  // the forward went solid -> fluid here
+        cd xf[5] = {x[0], x[1], x[2], x[3], x[4]};
+        for (int q = 0; q < 14; q++) x[q] = C(0.0);
+        x[5] = -xf[0];
+        x[6] = -xf[1];
+        x[8] = xf[2];
+        x[12] = -xf[3];
+        x[13] = xf[4];
+      } else {  // fluid -> solid
+        cd xf[5] = {x[0], x[1], x[3], -x[9], -x[11]};
+        for (int q = 0; q < 14; q++) x[q] = q < 5 ? xf[q] : C(0.0);
+      }
+    }
+  }
+}
+
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_minor_adj(const double* __restrict__ gtab,
+                                                  const RunK* __restrict__ runs, Queue Q,
+                                                  StartData SD, SphState SS,
+                                                  const cd* __restrict__ rec,
+                                                  const cd* __restrict__ recb,
+                                                  const cd* __restrict__ mub,
+                                                  cd* __restrict__ m0bar, AdjWork W,
+                                                  double* __restrict__ tbar) {
+  __shared__ AdjLds S;
+  RUN_ITEMS(PERSIST, Q,
+            minor_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, SS, rec, recb,
+                           mub, m0bar, W, S, tbar));
+}
+
+// ----------------------------------------------------------- k_tor_adj
+// tor_solver (src/module_solver.f90:25-142), receiver above the source and
+// at the solid surface: ytor(q, c) = -H^sc3 y2_q b_c / D with y2 = (1, 0)
+// at the surface, b = (y1_2, -y1_1) from the march y1 from the start to
+// the source and D = y2_2 y3_1 - y2_1 y3_2 from its continuation y3 to the
+// receiver.  The row starts on y3 as (0, T/D), T = sum a_qc ytor_qc, gets
+// the b part at the source and ends as each lane's start adjoint (y0bar).
+__device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j, int lb,
+                                             const StartData& SD, const cd* __restrict__ rec,
+                                             const cd* __restrict__ recb, cd* __restrict__ y0bar,
+                                             const AdjWork& W, AdjLds& S, double* tbar) {
+  const int l = R.lmin + lb * 64 + (int)threadIdx.x;
+  const bool valid = (l >= 2 && l <= R.lmax);
+  const size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
+  const int i1 = valid ? SD.tor_i1[p] : NOLANE;
+  const int ib = wave_min(i1);
+  if (ib == NOLANE) return;
+  const WS w = wave_omega(R, j);
+  const LS L = l_state(valid ? l : 2);
+  const int nk = R.nknot, is = R.is, ir = R.ir, s0 = R.noc + 1;
+  const double rs = R.rs, rr = R.rr;
+  // 1. the forward's marches (tor_item), checkpoints i - s0 below the
+  // source, is - s0 + 1 + (i - is) above
+  cd y1[2] = {C(0.0), C(0.0)};
+  int sc1 = 0;
+  for (int i = ib; i <= is; i++) {
+    if (i == i1) {
+      y1[0] = SD.tor_y0[2 * p];
+      y1[1] = SD.tor_y0[2 * p + 1];
+    }
+    const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+    put_ck<2>(W, i - s0, y1, sc1);
+    if (a == b) continue;
+    march_interval(T, w, L, nk, i, a, b, y1, true, sc1, S.sb, TorM());
+  }
+  const int up0 = is - s0 + 1;
+  cd y3[2] = {y1[0], y1[1]};
+  int sc3 = 0;
+  for (int i = is; i <= ir; i++) {
+    const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+    put_ck<2>(W, up0 + (i - is), y3, sc3);
+    if (a != b) march_interval(T, w, L, nk, i, a, b, y3, true, sc3, S.sb, TorM());
+  }
+  __syncthreads();
+  if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
+  __syncthreads();
+  cd a00 = C(0.0), a01 = C(0.0), tsum = C(0.0);
+  if (valid) {
+    for (int i = 0; i < 4; i++) {
+      const size_t f = p * REC + O_YTOR + i;
+      tsum = cmad(tsum, conj_(recb[f]), rec[f]);
+    }
+    a00 = conj_(recb[p * REC + O_YTOR]);
+    a01 = conj_(recb[p * REC + O_YTOR + 2]);
+  }
+  const cd dinvv = valid ? cinv(-y3[1]) : C(0.0);  // 1 / D
+  cd x[2] = {C(0.0), cmul_(tsum, dinvv)};
+  int sx = -sc3;
+  for (int i = ir; i >= is; i--) {
+    const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+    if (a != b) rev_interval<SysTor>(T, w, L, nk, i, a, b, x, sx, W, up0 + (i - is), S, tbar);
+  }
+  {
+    const double h = scale_pow(-sx);
+    const cd hd = scale_pow(sc3) * dinvv;
+    x[0] = x[0] * h + cmul_(a01, hd);
+    x[1] = x[1] * h - cmul_(a00, hd);
+    sx = -sc1;
+  }
+  for (int i = is; i >= ib; i--) {
+    const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+    if (a != b) rev_interval<SysTor>(T, w, L, nk, i, a, b, x, sx, W, i - s0, S, tbar);
+    if (i == i1) {
+      const double h = scale_pow(-(sx + *cks_of(W, i - s0)));
+      y0bar[2 * p] = x[0] * h;
+      y0bar[2 * p + 1] = x[1] * h;
+      x[0] = x[1] = C(0.0);
+    }
+  }
+}
+
+template <bool PERSIST>
+__global__ void __launch_bounds__(64) k_tor_adj(const double* __restrict__ gtab,
+                                                const RunK* __restrict__ runs, Queue Q,
+                                                StartData SD, const cd* __restrict__ rec,
+                                                const cd* __restrict__ recb,
+                                                cd* __restrict__ y0bar, AdjWork W,
+                                                double* __restrict__ tbar) {
+  __shared__ AdjLds S;
+  RUN_ITEMS(PERSIST, Q,
+            tor_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, rec, recb,
+                         y0bar, W, S, tbar));
+}
+
+// --------------------------------------------------------- k_start_adj
+// The starting solutions' part: for every pair, Re(m0bar . dm0/dv) and
+// Re(y0bar . dy0/dv) for the spline values v at the start radius (the
+// forward's k_start choices, src/module_solver.f90:64-75, 678-722),
+// spread onto the start interval's table entries.  The block sums into
+// LDS (fields rho, mu, 1/Qmu, kappa, 1/Qkappa and their second
+// derivatives, nknot each), then into the table adjoint.
+__global__ void __launch_bounds__(256) k_start_adj(const double* __restrict__ gtab,
+                                                   const RunK* __restrict__ runs, PairRange PR,
+                                                   StartData SD, const int* __restrict__ starts,
+                                                   const cd* __restrict__ m0bar,
+                                                   const cd* __restrict__ y0bar,
+                                                   double* __restrict__ tbar) {
+  extern __shared__ double acc[];  // [SV_N][2][nknot]
+  const int nk = runs[PR.e0].nknot;
+  for (int q = threadIdx.x; q < SV_N * 2 * nk; q += blockDim.x) acc[q] = 0.0;
+  __syncthreads();
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < PR.n) {
+    const long long g = PR.g0 + t;
+    const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
+    const Tab T{gtab, R.nknot};
+    int l, j;
+    pair_lj(R, (int)(g - R.pbase), l, j);
+    const size_t p = (size_t)g;
+    auto put = [&](const SplW& sw, const double* vb) {
+      for (int v = 0; v < SV_N; v++) {
+        const int f0 = start_field(v);
+        spl_adj(sw, f0, vb[v], [&](int f, int knot, double val) {
+          atomicAdd(&acc[(v * 2 + (f == f0 ? 0 : 1)) * nk + knot - 1], val);
+        });
+      }
+    };
+    if (l >= 1) {
+      PS s;
+      pair_state(R, l, j, s);
+      if (l >= 2) {
+        const int i1 = SD.tor_i1[p];
+        if (i1 != R.noc + 1) {
+          const double rt = T(F_R, i1);
+          DS y[2];
+          tor_ystart_d(T, s, i1, rt, y);
+          double vb[SV_N];
+          for (int v = 0; v < SV_N; v++)
+            vb[v] = cmul_(y0bar[2 * p], y[0].d[v]).x + cmul_(y0bar[2 * p + 1], y[1].d[v]).x;
+          put(spl_w(T, i1, rt), vb);
+        }
+      }
+      const int raw = starts[2 * p + 1], itop = R.is;
+      int i1 = raw;
+      double rt = T(F_R, raw);
+      if (raw == 1) rt = fmin(0.1 * (1.0 / (4.0 * s.rew / R.alpha_min)), 0.1 * T(F_R, 2));
+      if (i1 >= itop) {
+        i1 = itop - 1;
+        rt = T(F_R, i1);
+      }
+      const bool fluid = i1 <= R.noc && (i1 > R.nic || R.fic);
+      DS m[14];
+      minor_start_d(T, s, i1, rt, fluid, m);
+      double vb[SV_N];
+      for (int v = 0; v < SV_N; v++) {
+        double a = 0.0;
+        for (int q = 0; q < 14; q++) a += cmul_(m0bar[p * 14 + q], m[q].d[v]).x;
+        vb[v] = a;
+      }
+      put(spl_w(T, i1, rt), vb);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < SV_N * 2 * nk; q += blockDim.x) {
+    const double a = acc[q];
+    if (a != 0.0) {
+      const int v = q / (2 * nk), which = (q / nk) % 2, knot0 = q % nk;
+      atomicAdd(&tbar[(size_t)(start_field(v) + which) * nk + knot0], a);
+    }
+  }
+}
+
+// ----------------------------------------------------------- k_rad_adj
+// rad_solver_wg (src/module_solver.f90:446-561), receiver at the top grid
+// point: yrad(q, c) = -y2r_q b_c / D, y2r = (1, 0), b = (y1_2, -y1_1),
+// D = y2_2 y1_1 - y2_1 y1_2 with y1 the upward and y2 the downward solution
+// at the source point.  One lane per frequency (the grid depends on w): the
+// forward's steps (rad_sweeps) are recorded with the state before each,
+// then the rows on y2 and y1 are taken back through them; the table-entry
+// adjoints of the current knot interval are kept per lane and added at
+// each change of interval.  No renormalisation on this solve.
+struct RadStep {
+  double x, h;
+  cd y[2];
+  int il, pad;
+};
+
+struct RadAcc {
+  double a[RAD_NV * 4];
+  int il;
+};
+
+__device__ inline void rad_flush(RadAcc& A, int nk, double* tbar) {
+  if (A.il > 0)
+    for (int v = 0; v < RAD_NV; v++)
+      for (int q = 0; q < 4; q++) {
+        double x = A.a[v * 4 + q];
+        if (x != 0.0)
+          atomicAdd(&tbar[(size_t)(rad_field(v) + q / 2) * nk + A.il - 1 + (q % 2)], x);
+        A.a[v * 4 + q] = 0.0;
+      }
+}
+
+// one step of the forward's rk_step (tableau of rkck, src/module_int.f90:
+// 2016-2064, unscaled derivatives, h applied in the combinations), taken
+// back: x <- x S, with the stage pairings into A
+__device__ inline void rad_step_adj(const Tab& T, const PS& s, const RadStep& st, cd* x,
+                                    RadAcc& A, int nk, double* tbar) {
+  using namespace ck;
+  const double h = st.h;
+  const double off[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
+  RadRHS f;
+  RadRHS::Coef c[6];
+  for (int k = 0; k < 6; k++) c[k] = f.coef(T, s, st.il, st.x + off[k] * h);
+  // forward stages: inputs z and derivatives d
+  cd z[6][2], d[6][2];
+  const cd* y = st.y;
+  auto stage = [&](int k) { RadRHS::apply(c[k], z[k], d[k]); };
+  for (int i = 0; i < 2; i++) z[0][i] = y[i];
+  stage(0);
+  for (int i = 0; i < 2; i++) z[1][i] = y[i] + (b21 * h) * d[0][i];
+  stage(1);
+  for (int i = 0; i < 2; i++) z[2][i] = y[i] + h * (b31 * d[0][i] + b32 * d[1][i]);
+  stage(2);
+  for (int i = 0; i < 2; i++)
+    z[3][i] = y[i] + h * (b41 * d[0][i] + b42 * d[1][i] + b43 * d[2][i]);
+  stage(3);
+  for (int i = 0; i < 2; i++)
+    z[4][i] = y[i] + h * (b51 * d[0][i] + b52 * d[1][i] + b53 * d[2][i] + b54 * d[3][i]);
+  stage(4);
+  for (int i = 0; i < 2; i++)
+    z[5][i] = y[i] + h * (b61 * d[0][i] + b62 * d[1][i] + b63 * d[2][i] + b64 * d[3][i] +
+                          b65 * d[4][i]);
+  // transposed stages: kb_k = h (c_k x + sum_{m>k} b_mk zb_m), zb_k = A_k^T kb_k
+  const double cc[6] = {c1, 0.0, c3, c4, 0.0, c6};
+  const double bb[6][6] = {{0, 0, 0, 0, 0, 0},
+                           {b21, 0, 0, 0, 0, 0},
+                           {b31, b32, 0, 0, 0, 0},
+                           {b41, b42, b43, 0, 0, 0},
+                           {b51, b52, b53, b54, 0, 0},
+                           {b61, b62, b63, b64, b65, 0}};
+  cd zb[6][2];
+  cd xn[2] = {x[0], x[1]};
+  for (int k = 5; k >= 0; k--) {
+    cd kb[2];
+    for (int i = 0; i < 2; i++) {
+      cd v = cc[k] * x[i];
+      for (int m = k + 1; m < 6; m++) v = v + bb[m][k] * zb[m][i];
+      kb[i] = h * v;
+    }
+    // zb = A^T kb for A = [[t, k], [sv, -t]]
+    zb[k][0] = c[k].t * kb[0] + c[k].sv * kb[1];
+    zb[k][1] = c[k].k * kb[0] - c[k].t * kb[1];
+    xn[0] = xn[0] + zb[k][0];
+    xn[1] = xn[1] + zb[k][1];
+    // pairing: the entries' adjoints and the chain to the table
+    cd tb = kb[0] * z[k][0] - kb[1] * z[k][1], kbar = kb[0] * z[k][1], svb = kb[1] * z[k][0];
+    double vb[RAD_NV];
+    const double r = st.x + off[k] * h;
+    rad_coef_adj(T, s, st.il, r, tb, kbar, svb, vb);
+    if (st.il != A.il) {
+      rad_flush(A, nk, tbar);
+      A.il = st.il;
+    }
+    SplW sw = spl_w(T, st.il, r);
+    for (int v = 0; v < RAD_NV; v++) {
+      const int f0 = rad_field(v);
+      spl_adj(sw, f0, vb[v], [&](int ff, int knot, double val) {
+        A.a[v * 4 + (ff == f0 ? 0 : 2) + (knot - st.il)] += val;
+      });
+    }
+  }
+  x[0] = xn[0];
+  x[1] = xn[1];
+}
+
+__global__ void __launch_bounds__(64) k_rad_adj(const double* __restrict__ gtab,
+                                                const RunK* __restrict__ runs,
+                                                const cd* __restrict__ recb,
+                                                const long long* __restrict__ toff,
+                                                RadStep* __restrict__ tape, int* __restrict__ err,
+                                                double* __restrict__ tbar) {
+  const RunK& R = runs[0];
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+  if (j > R.nw || R.lmin != 0) return;
+  const Tab T{gtab, R.nknot};
+  const int nk = R.nknot;
+  const size_t p = (size_t)R.pbase + (j - 1);
+  PS s;
+  pair_state(R, 0, j, s);
+  const double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
+  RadStep* tp = tape + toff[j - 1];
+  const long long cap = toff[j] - toff[j - 1];
+  long long n = 0, nu = 0;
+  int nsr = 0;
+  cd y1[2], y2[2], y1r[2], y2r[2];
+  bool rec_below;
+  auto hook = [&](int sweep, int il, double x, double h, const cd* y) {
+    if (n < cap) {
+      RadStep& t = tp[n];
+      t.x = x;
+      t.h = h;
+      t.y[0] = y[0];
+      t.y[1] = y[1];
+      t.il = il;
+    }
+    n++;
+    if (sweep == 0) nu++;
+  };
+  rad_sweeps(T, nk, s, dr, R.rs, R.rr, NoGroup(), hook, y1, y2, y1r, y2r, rec_below, nsr);
+  if (n > cap || rec_below) {
+    atomicOr(err, n > cap ? 2 : 4);
+    return;
+  }
+  // the rows on y1 and y2 at the source point
+  cd a[2][2];
+  for (int q = 0; q < 2; q++)
+    for (int c = 0; c < 2; c++) a[q][c] = conj_(recb[p * REC + O_YRAD + q + 2 * c]);
+  const cd Dv = y2[1] * y1[0] - y2[0] * y1[1], Di = cinv(Dv);
+  const cd b[2] = {y1[1], -y1[0]};
+  cd G[2], E = C(0.0);
+  for (int c = 0; c < 2; c++) {
+    G[c] = C(0.0);
+    for (int q = 0; q < 2; q++) {
+      G[c] = G[c] - a[q][c] * y2r[q] * Di;
+      E = E + a[q][c] * y2r[q] * b[c] * Di * Di;
+    }
+  }
+  cd r1[2] = {-G[1] + E * y2[1], G[0] - E * y2[0]};
+  cd r2[2] = {-E * y1[1], E * y1[0]};
+  RadAcc A;
+  for (int q = 0; q < RAD_NV * 4; q++) A.a[q] = 0.0;
+  A.il = 0;
+  for (long long k = n - 1; k >= nu; k--) rad_step_adj(T, s, tp[k], r2, A, nk, tbar);
+  for (long long k = nu - 1; k >= 0; k--) rad_step_adj(T, s, tp[k], r1, A, nk, tbar);
+  rad_flush(A, nk, tbar);
+  // the starting solution at rt (the first upward step's radius), interval 1
+  if (nu > 0) {
+    const double rt = tp[0].x;
+    DS y[2];
+    rad_ystart_d(T, s, 1, rt, y);
+    SplW sw = spl_w(T, 1, rt);
+    for (int v = 0; v < SV_N; v++) {
+      double vb = cmul_(r1[0], y[0].d[v]).x + cmul_(r1[1], y[1].d[v]).x;
+      const int f0 = start_field(v);
+      spl_adj(sw, f0, vb, [&](int ff, int knot, double val) {
+        atomicAdd(&tbar[(size_t)ff * nk + knot - 1], val);
+      });
+    }
+  }
+}
+
+}  // namespace
+
+void launch_brow_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, Queue Q, SphState SS, const cd* rec, const cd* recb,
+                     cd* mub, AdjWork W, double* tbar) {
+  hipLaunchKernelGGL(k_brow_adj<true>, g, b, lds, s, gtab, runs, Q, SS, rec, recb, mub, W, tbar);
+}
+void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                      const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,
+                      const cd* recb, const cd* mub, cd* m0bar, AdjWork W, double* tbar) {
+  hipLaunchKernelGGL(k_minor_adj<true>, g, b, lds, s, gtab, runs, Q, SD, SS, rec, recb, mub,
+                     m0bar, W, tbar);
+}
+void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                    const RunK* runs, Queue Q, StartData SD, const cd* rec, const cd* recb,
+                    cd* y0bar, AdjWork W, double* tbar) {
+  hipLaunchKernelGGL(k_tor_adj<true>, g, b, lds, s, gtab, runs, Q, SD, rec, recb, y0bar, W,
+                     tbar);
+}
+void launch_start_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                      const RunK* runs, PairRange PR, StartData SD, const int* starts,
+                      const cd* m0bar, const cd* y0bar, double* tbar) {
+  hipLaunchKernelGGL(k_start_adj, g, b, lds, s, gtab, runs, PR, SD, starts, m0bar, y0bar, tbar);
+}
+size_t rad_step_bytes() { return sizeof(RadStep); }
+void launch_rad_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                    const RunK* runs, const cd* recb, const long long* toff, void* tape, int* err,
+                    double* tbar) {
+  hipLaunchKernelGGL(k_rad_adj, g, b, lds, s, gtab, runs, recb, toff,
+                     static_cast<RadStep*>(tape), err, tbar);
+}
+
+}  // namespace ysx

@@ -5,6 +5,7 @@
 #include <cmath>
 
 #include "ysx_kern.hpp"
+#include "ysx_radial.hpp"
 
 namespace ysx {
 namespace {
@@ -149,52 +150,8 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
 
 // ------------------------------------------------------------------- k_rad
 // rad_solver_wg with ss = identity(2), src/module_solver.f90:446-561, on
-// the grid of make_grid, src/module_model.f90:638-787.  Only yout is
-// formed; the caller discards pout (src/yspec.f90:277).
-struct Run3 {
-  double base[3], step[3];
-  int j0[3], j1[3], n;
-  int src, rec;  // run whose last point is the source / receiver (-1: none)
-};
-
-__device__ inline void grid_runs(const Tab& T, int i, double drr, double rs, double rr, Run3& G) {
-  double r1 = T(F_R, i), r2 = T(F_R, i + 1);
-  G.n = 0;
-  G.src = G.rec = -1;
-  if (r1 == r2) return;
-  bool ls = (rs > r1 && rs <= r2), lr = (rr > r1 && rr <= r2);
-  auto add = [&](double a, double b, int j0) {
-    int n = (int)floor((b - a) / drr) + 2;
-    G.base[G.n] = a;
-    G.step[G.n] = (b - a) / (double)(n - 1);
-    G.j0[G.n] = j0;
-    G.j1[G.n] = n;
-    G.n++;
-  };
-  if (ls && lr) {
-    if (rs < rr) {
-      add(r1, rs, 1); G.src = 0;
-      add(rs, rr, 2); G.rec = 1;
-      add(rr, r2, 2);
-    } else if (rr < rs) {
-      add(r1, rr, 1); G.rec = 0;
-      add(rr, rs, 2); G.src = 1;
-      add(rs, r2, 2);
-    } else {
-      add(r1, rs, 1); G.src = 0; G.rec = 0;
-      add(rs, r2, 2);
-    }
-  } else if (ls) {
-    add(r1, rs, 1); G.src = 0;
-    add(rs, r2, 2);
-  } else if (lr) {
-    add(r1, rr, 1); G.rec = 0;
-    add(rr, r2, 2);
-  } else {
-    add(r1, r2, 1);
-  }
-}
-
+// the grid of make_grid, src/module_model.f90:638-787 (ysx_radial.hpp).
+// Only yout is formed; the caller discards pout (src/yspec.f90:277).
 // A group of R.radg lanes per frequency (8, or 1 for runs with many
 // frequencies, where the radial kernel is hidden beside the spheroidal
 // marches and should take few compute units) integrates one solution: the
@@ -213,78 +170,11 @@ __device__ __forceinline__ void rad_item(const Tab& T, const RunK& R, int jb,
   PS s;
   pair_state(R, 0, j, s);
   if (R.grav == 0) return;  // rad_solver_ng: k_rad_ng
-  RadRHS f;
-  int nk = R.nknot;
   double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
-  double rs = R.rs, rr = R.rr;
   int nsr = 0;
-  // --- upward sweep over grid points 1 .. source
-  cd y1[2], y1r[2] = {C(0.0), C(0.0)};
-  bool rec_below = false;  // receiver at or below the source point
-  {
-    int k = 0;
-    double rprev = 0.0;
-    int iprev = 1;
-    bool done = false;
-    for (int i = 1; i <= nk - 1 && !done; i++) {
-      Run3 G;
-      grid_runs(T, i, dr, rs, rr, G);
-      for (int q = 0; q < G.n && !done; q++) {
-        for (int jj = G.j0[q]; jj <= G.j1[q]; jj++) {
-          double rg = G.base[q] + (double)(jj - 1) * G.step[q];
-          k++;
-          if (k == 2) {
-            double rt = fmin(0.1 * dr, 0.1 * rg);
-            rad_ystart(T, s, 1, rt, y1);
-            yint(T, s, nk, 1, i, rt, rg, y1, f, (int*)nullptr, nsr, grp);
-          } else if (k > 2) {
-            yint(T, s, nk, iprev, i, rprev, rg, y1, f, (int*)nullptr, nsr, grp);
-          }
-          rprev = rg;
-          iprev = i;
-          bool last = (jj == G.j1[q]);
-          if (last && q == G.rec) {
-            rec_below = true;
-            y1r[0] = y1[0];
-            y1r[1] = y1[1];
-          }
-          if (last && q == G.src) {
-            done = true;
-            break;
-          }
-        }
-      }
-    }
-  }
-  // --- downward sweep from the top grid point to the source point
-  cd y2[2] = {C(1.0), C(0.0)}, y2r[2] = {C(1.0), C(0.0)};
-  {
-    bool first = true, done = false;
-    double rprev = 0.0;
-    int iprev = nk - 1;
-    for (int i = nk - 1; i >= 1 && !done; i--) {
-      Run3 G;
-      grid_runs(T, i, dr, rs, rr, G);
-      for (int q = G.n - 1; q >= 0 && !done; q--) {
-        for (int jj = G.j1[q]; jj >= G.j0[q]; jj--) {
-          double rg = G.base[q] + (double)(jj - 1) * G.step[q];
-          if (!first) yint(T, s, nk, iprev, i, rprev, rg, y2, f, (int*)nullptr, nsr, grp);
-          first = false;
-          rprev = rg;
-          iprev = i;
-          bool last = (jj == G.j1[q]);
-          if (last && q == G.src) {
-            done = true;
-            break;
-          }
-          if (last && q == G.rec && !rec_below) {
-            y2r[0] = y2[0];
-            y2r[1] = y2[1];
-          }
-        }
-      }
-    }
-  }
+  cd y1[2], y2[2], y1r[2], y2r[2];
+  bool rec_below;
+  rad_sweeps(T, R.nknot, s, dr, R.rs, R.rr, grp, NoRadHook(), y1, y2, y1r, y2r, rec_below, nsr);
   cd delta = cinv(y2[1] * y1[0] - y2[0] * y1[1]);
   cd yo[4];
   if (rec_below) {

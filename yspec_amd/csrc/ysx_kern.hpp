@@ -177,6 +177,19 @@ struct SphState {
   cd* ms1;   // [npair][14] minors at the source (events with the determinant row, drow)
 };
 
+// Per-block scratch of the adjoint sweeps (ysx_k_adjoint.hip) in global
+// memory: checkpoints (a state of up to 14 complex numbers and a
+// renormalisation counter per knot interval and lane) and the tape of one
+// interval's steps (six stage vectors per step and lane, and a counter).
+struct AdjWork {
+  cd* ck;
+  int* cks;
+  cd* tape;
+  int* tsc;
+  int slots, tmax;
+  int* err;  // bit 0: an interval with more steps than the tape holds
+};
+
 // ------------------------------------------------------- march helpers
 // yint's interval loop (src/module_int.f90:1888-1963) over knot intervals,
 // with wave-uniform arguments
@@ -296,6 +309,25 @@ void launch_source_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gt
                      const RunK* runs, int e, const cd* recb, double* part);
 void launch_source_model_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                            const RunK* runs, int e, const cd* recb, double* part);
+// adjoint sweeps (ysx_k_adjoint.hip): persistent blocks over the queue Q
+void launch_brow_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, Queue Q, SphState SS, const cd* rec, const cd* recb,
+                     cd* mub, AdjWork W, double* tbar);
+void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                      const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,
+                      const cd* recb, const cd* mub, cd* m0bar, AdjWork W, double* tbar);
+void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                    const RunK* runs, Queue Q, StartData SD, const cd* rec, const cd* recb,
+                    cd* y0bar, AdjWork W, double* tbar);
+void launch_start_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                      const RunK* runs, PairRange PR, StartData SD, const int* starts,
+                      const cd* m0bar, const cd* y0bar, double* tbar);
+// radial adjoint: per-lane step tapes at offsets toff[j - 1] .. toff[j]
+// (records of rad_step_bytes() bytes) for the frequencies j of event 0
+size_t rad_step_bytes();
+void launch_rad_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                    const RunK* runs, const cd* recb, const long long* toff, void* tape, int* err,
+                    double* tbar);
 #define YSX_MARCH_LAUNCHERS(SUFFIX)                                                              \
   void launch_tor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
                                  const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt);   \

@@ -18,16 +18,16 @@ namespace ysx {
 
 // complex multiply-accumulate chains: acc +- a*b in 4 FP64 FMAs
 // (complex a) or 2 (real a)
-__device__ inline cd cmad(cd c, cd a, cd b) {
+__host__ __device__ inline cd cmad(cd c, cd a, cd b) {
   return {fma(-a.y, b.y, fma(a.x, b.x, c.x)), fma(a.y, b.x, fma(a.x, b.y, c.y))};
 }
-__device__ inline cd cmsb(cd c, cd a, cd b) {
+__host__ __device__ inline cd cmsb(cd c, cd a, cd b) {
   return {fma(a.y, b.y, fma(-a.x, b.x, c.x)), fma(-a.y, b.x, fma(-a.x, b.y, c.y))};
 }
-__device__ inline cd rmad(cd c, double a, cd b) { return {fma(a, b.x, c.x), fma(a, b.y, c.y)}; }
-__device__ inline cd rmsb(cd c, double a, cd b) { return {fma(-a, b.x, c.x), fma(-a, b.y, c.y)}; }
-__device__ inline cd rmul(double a, cd b) { return {a * b.x, a * b.y}; }
-__device__ inline cd cmul_(cd a, cd b) {
+__host__ __device__ inline cd rmad(cd c, double a, cd b) { return {fma(a, b.x, c.x), fma(a, b.y, c.y)}; }
+__host__ __device__ inline cd rmsb(cd c, double a, cd b) { return {fma(-a, b.x, c.x), fma(-a, b.y, c.y)}; }
+__host__ __device__ inline cd rmul(double a, cd b) { return {a * b.x, a * b.y}; }
+__host__ __device__ inline cd cmul_(cd a, cd b) {
   return {fma(-a.y, b.y, a.x * b.x), fma(a.y, b.x, a.x * b.y)};
 }
 
@@ -39,7 +39,7 @@ struct WS {
   cd omega, omega2, omegai2, lno;
   double rew, pibigg;
 };
-__device__ inline WS omega_state(cd w, double tref, double ats, double pibigg) {
+__host__ __device__ inline WS omega_state(cd w, double tref, double ats, double pibigg) {
   PS s;
   set_omega(s, w, tref, ats);
   WS o;
@@ -56,13 +56,13 @@ __device__ inline WS omega_state(cd w, double tref, double ats, double pibigg) {
 struct LS {
   double lld, llp1, zeta, zeta2, zeta2m2;
 };
-__device__ inline LS l_state(int l) {
+__host__ __device__ inline LS l_state(int l) {
   PS s;
   set_l(s, l);
   return LS{s.lld, s.llp1, s.zeta, s.zeta2, s.zeta2m2};
 }
 
-__device__ inline double step_len_w(const Tab& T, double rew, int i, int nknot) {
+__host__ __device__ inline double step_len_w(const Tab& T, double rew, int i, int nknot) {
   double a = T(F_ALPHA, i);
   double tmp = (i < nknot) ? fmax(a, T(F_ALPHA, i + 1)) : a;
   return 1.0 / (4.0 * rew / tmp);
@@ -75,7 +75,7 @@ __device__ inline double step_len_w(const Tab& T, double rew, int i, int nknot) 
 // src/module_int.f90:564-610); the entries used there and with switch 1
 // (sph_solid_amat_cow, :613-672) have the same form as with switch 2.
 template <bool GRAV>
-__device__ inline void solid_base(const Tab& T, const WS& w, int il, double rr, double h,
+__host__ __device__ inline void solid_base(const Tab& T, const WS& w, int il, double rr, double h,
                                 double* o) {
   SplW sw = spl_w(T, il, rr);
   double rrho = spl(T, sw, F_RHO), racon = spl(T, sw, F_ACON), rccon = spl(T, sw, F_CCON),
@@ -107,11 +107,11 @@ __device__ inline void solid_base(const Tab& T, const WS& w, int il, double rr, 
 
 struct SolidB {
   static constexpr int BS = 20;
-  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
+  __host__ __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
     solid_base<true>(T, w, il, rr, h, o);
   }
   // h A from the base of one stage radius
-  __device__ static inline void amat(const double* b, const LS& L, SolidA& A) {
+  __host__ __device__ static inline void amat(const double* b, const LS& L, SolidA& A) {
     double hri = b[0], rrho = b[1];
     A.t11 = C(b[2], b[3]);
     A.k11 = C(b[4], b[5]);
@@ -134,19 +134,19 @@ struct SolidB {
 // product added to an initial value g(i) (a Runge-Kutta combination that
 // takes this stage's derivative once, folded into the chain).
 struct NoInit {
-  __device__ inline cd r(int, double a, cd m) const { return rmul(a, m); }
-  __device__ inline cd c(int, cd a, cd m) const { return cmul_(a, m); }
+  __host__ __device__ inline cd r(int, double a, cd m) const { return rmul(a, m); }
+  __host__ __device__ inline cd c(int, cd a, cd m) const { return cmul_(a, m); }
 };
 template <class G>
 struct AccInit {
   const G& g;
-  __device__ inline cd r(int i, double a, cd m) const { return rmad(g(i), a, m); }
-  __device__ inline cd c(int i, cd a, cd m) const { return cmad(g(i), a, m); }
+  __host__ __device__ inline cd r(int i, double a, cd m) const { return rmad(g(i), a, m); }
+  __host__ __device__ inline cd c(int i, cd a, cd m) const { return cmad(g(i), a, m); }
 };
 
 // sph_minors_solid_derivs, src/module_int.f90:818-871, as FMA chains
 template <class Init, class Sink>
-__device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, const Init& I,
+__host__ __device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, const Init& I,
                                           Sink&& sink) {
   const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
                s23 = A.s23;
@@ -191,7 +191,7 @@ __device__ inline void solid_minor_derivs(const SolidA& A, const cd* m, const In
 
 // sph_bvec_solid_derivs, src/module_int.f90:1016-1057, as FMA chains
 template <class Init, class Sink>
-__device__ inline void bvec_derivs(const SolidA& A, const cd* b, const Init& I, Sink&& sink) {
+__host__ __device__ inline void bvec_derivs(const SolidA& A, const cd* b, const Init& I, Sink&& sink) {
   const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
                s23 = A.s23;
   const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
@@ -234,7 +234,7 @@ __device__ inline void bvec_derivs(const SolidA& A, const cd* b, const Init& I, 
 // e = A^T x for the b-vector system's matrix A, as FMA chains.  It is the
 // derivative of the receiver-side row march (rk_step_T_lds).
 template <class Init, class Sink>
-__device__ inline void bvecT_derivs(const SolidA& A, const cd* x, const Init& I, Sink&& sink) {
+__host__ __device__ inline void bvecT_derivs(const SolidA& A, const cd* x, const Init& I, Sink&& sink) {
   const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
                s23 = A.s23;
   const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
@@ -277,7 +277,7 @@ __device__ inline void bvecT_derivs(const SolidA& A, const cd* x, const Init& I,
 struct SolidMinorM : SolidB {
   static constexpr int N = 14;
   template <class Sink>
-  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
                                 Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
@@ -285,13 +285,13 @@ struct SolidMinorM : SolidB {
   }
   // sink(i, g(i) + h A y) for a base scaled by h
   template <class G, class Sink>
-  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
                                     const G& g, Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
     solid_minor_derivs(A, y, AccInit<G>{g}, sink);
   }
-  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
                                     cd* d) const {
     stream(b, L, w, y, [&](int i, cd v) { d[i] = v; });
   }
@@ -299,20 +299,20 @@ struct SolidMinorM : SolidB {
 struct BvecM : SolidB {
   static constexpr int N = 14;
   template <class Sink>
-  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
                                 Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
     bvec_derivs(A, y, NoInit(), sink);
   }
   template <class G, class Sink>
-  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
                                     const G& g, Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
     bvec_derivs(A, y, AccInit<G>{g}, sink);
   }
-  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
                                     cd* d) const {
     stream(b, L, w, y, [&](int i, cd v) { d[i] = v; });
   }
@@ -323,7 +323,7 @@ struct BvecM : SolidB {
 // receiver back to the source it carries the coefficients of the secular
 // determinant (delta_solid in its first argument) down to the source.
 template <class Init, class Sink>
-__device__ inline void minorT_derivs(const SolidA& A, const cd* x, const Init& I, Sink&& sink) {
+__host__ __device__ inline void minorT_derivs(const SolidA& A, const cd* x, const Init& I, Sink&& sink) {
   const double t21 = A.t21, t22 = A.t22, t31 = A.t31, t33 = A.t33, k33 = A.k33, s13 = A.s13,
                s23 = A.s23;
   const cd t11 = A.t11, t12 = A.t12, k11 = A.k11, k22 = A.k22, s11 = A.s11, s12 = A.s12,
@@ -367,14 +367,14 @@ __device__ inline void minorT_derivs(const SolidA& A, const cd* x, const Init& I
 struct MinorTM : SolidB {
   static constexpr int N = 14;
   template <class Sink>
-  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
                                 Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
     minorT_derivs(A, y, NoInit(), sink);
   }
   template <class G, class Sink>
-  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
                                     const G& g, Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
@@ -386,14 +386,14 @@ struct MinorTM : SolidB {
 struct BvecTM : SolidB {
   static constexpr int N = 14;
   template <class Sink>
-  __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
                                 Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
     bvecT_derivs(A, y, NoInit(), sink);
   }
   template <class G, class Sink>
-  __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void stream_acc(const double* b, const LS& L, const WS& w, const cd* y,
                                     const G& g, Sink&& sink) const {
     SolidA A;
     amat(b, L, A);
@@ -407,7 +407,7 @@ struct BvecTM : SolidB {
 // entries.
 struct FluidMinorM {
   static constexpr int N = 5, BS = 14;
-  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
+  __host__ __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
     SplW sw = spl_w(T, il, rr);
     double rrho = spl(T, sw, F_RHO), rccon = spl(T, sw, F_CCON), gg = spl(T, sw, F_GRAV),
            rxa2 = spl(T, sw, F_XA2);
@@ -424,7 +424,7 @@ struct FluidMinorM {
     o[12] = h * (-4.0 * w.pibigg * rrho); o[13] = h * (4.0 * w.pibigg);
   }
   // h dm/dr
-  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* m,
+  __host__ __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* m,
                                     cd* d) const {
     double ri = b[0], rrho = b[1];  // ri: h / r
     cd fa = C(b[2], b[3]), fb = C(b[4], b[5]), cccon = C(b[6], b[7]);
@@ -445,13 +445,20 @@ struct FluidMinorM {
     d[3] = cmsb(cmsb(rmsb(rmul(2.0 * t21, m[1]), k22, m[4]), s11, m[0]), tm, m[3]);
     d[4] = cmsb(cmsb(cmsb(cmul_(s11, m[2]), 2.0 * s12, m[1]), s22, m[3]), tp, m[4]);
   }
+  template <class Sink>
+  __host__ __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* m,
+                                         Sink&& sink) const {
+    cd d[N];
+    (*this)(b, L, w, m, d);
+    for (int i = 0; i < N; i++) sink(i, d[i]);
+  }
 };
 
 // ---------------------------------------------------------------- toroidal
 // tor_amat + tor_derivs, src/module_int.f90:134-184; base times h
 struct TorM {
   static constexpr int N = 2, BS = 7;
-  __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
+  __host__ __device__ static void base(const Tab& T, const WS& w, int il, double rr, double h, double* o) {
     SplW sw = spl_w(T, il, rr);
     double rrho = spl(T, sw, F_RHO), rlcon = spl(T, sw, F_LCON), rncon = spl(T, sw, F_NCON),
            rqmu = spl(T, sw, F_QMU);
@@ -463,13 +470,20 @@ struct TorM {
     o[0] = h * ri; o[1] = h * k.x; o[2] = h * k.y; o[3] = h * sv0.x; o[4] = h * sv0.y;
     o[5] = h * mw.x; o[6] = h * mw.y;
   }
-  __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
+  __host__ __device__ inline void operator()(const double* b, const LS& L, const WS& w, const cd* y,
                                     cd* d) const {
     double t = 2.0 * b[0];
     cd k = C(b[1], b[2]);
     cd sv = L.zeta2m2 * C(b[3], b[4]) + C(b[5], b[6]);
     d[0] = cmad(rmul(t, y[0]), k, y[1]);
     d[1] = rmsb(cmul_(sv, y[0]), t, y[1]);
+  }
+  template <class Sink>
+  __host__ __device__ inline void stream(const double* b, const LS& L, const WS& w, const cd* y,
+                                         Sink&& sink) const {
+    cd d[N];
+    (*this)(b, L, w, y, d);
+    for (int i = 0; i < N; i++) sink(i, d[i]);
   }
 };
 
@@ -669,7 +683,7 @@ __device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w,
 // 1945-1949), optional renormalisation (:1910-1919).  Every lane of the
 // block must call this with identical (i, ra, rb).  Returns the number of
 // steps taken.
-__device__ inline double stage_offset(int k) {
+__host__ __device__ inline double stage_offset(int k) {
   return k == 0 ? 0.0 : (k == 1 ? 0.2 : (k == 2 ? 0.3 : (k == 3 ? 0.6 : (k == 4 ? 1.0 : 0.875))));
 }
 
@@ -677,7 +691,7 @@ __device__ inline double stage_offset(int k) {
 // c6 (stage 6) into the derivative, since those stages' derivatives enter
 // one combination each.
 template <int N>
-__device__ inline double stage_scale(int k) {
+__host__ __device__ inline double stage_scale(int k) {
   if constexpr (N == 14) return k == 4 ? ck::b65 : (k == 5 ? ck::c6 : 1.0);
   return 1.0;
 }

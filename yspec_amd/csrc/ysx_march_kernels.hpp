@@ -316,28 +316,6 @@ __global__ void __launch_bounds__(64) k_bvec_march(const double* __restrict__ gt
 // with the potential/tilt correction) replace four columns; the results
 // differ from the column march by rounding only.  Rows 3..5 of ysph are not
 // produced (zero).  A wave holds bl degrees x 64 / bl rows at one w.
-__device__ __forceinline__ cd pick3(int k, cd a, cd b, cd c) {
-  return (k == 0) ? a : ((k == 1) ? b : ((k == 2) ? c : C(0.0)));
-}
-// coefficients of b in row k = 0, 1, 2 of sph_ysol (src/module_int.f90:1785-1815)
-__device__ __forceinline__ void ysol_row(const cd* m, int k, cd* x) {
-  const cd z = C(0.0);
-  x[0] = pick3(k, z, z, m[0]);
-  x[1] = pick3(k, z, -m[0], z);
-  x[2] = pick3(k, m[2], -2.0 * m[1], -m[4]);
-  x[3] = pick3(k, z, -m[2], -m[5]);
-  x[4] = pick3(k, z, -m[3], m[2]);
-  x[5] = pick3(k, m[0], z, z);
-  x[6] = pick3(k, m[1], z, -m[9]);
-  x[7] = pick3(k, 2.0 * m[2], -m[1], m[4]);
-  x[8] = pick3(k, m[3], z, -m[1]);
-  x[9] = pick3(k, m[4], m[9], z);
-  x[10] = pick3(k, m[5], -m[4], z);
-  x[11] = pick3(k, m[6], m[10], m[12]);
-  x[12] = pick3(k, m[7], m[11], -m[10]);
-  x[13] = pick3(k, m[8], -m[7], m[6]);
-}
-
 // Rows k of pair (l, j) of event R from the marched row x (scale count sl):
 // y_c[k] = x . bst_c * fac * huge^(sc3 - sl), c = 0..3, into the record;
 // rows the march does not produce are zeroed.
@@ -511,24 +489,6 @@ __global__ void __launch_bounds__(64) k_brow_march(const double* __restrict__ gt
 // transposed steps of the same step sequence.  Writes fac = -1/delta and
 // the row's scale count as sc3 (so y = ysol(...) fac huge^(sc3 - sc4) keeps
 // its form); rounding-level differences from the forward continuation.
-__device__ __forceinline__ void delta_row(const cd* m2, cd* x) {
-  // coefficients of m1 in sph_delta_solid(m1, m2), src/module_int.f90:1705-1725
-  x[0] = m2[13];
-  x[1] = -2.0 * m2[6];
-  x[2] = -2.0 * m2[10];
-  x[3] = -m2[12];
-  x[4] = -2.0 * m2[7];
-  x[5] = -m2[11];
-  x[6] = 2.0 * m2[1];
-  x[7] = 2.0 * m2[4];
-  x[8] = m2[9];
-  x[9] = -m2[8];
-  x[10] = 2.0 * m2[2];
-  x[11] = m2[5];
-  x[12] = m2[3];
-  x[13] = -m2[0];
-}
-
 __device__ __forceinline__ void drow_finish(const RunK& R, int l, int j, const cd* x, int sd,
                                             int nsteps, const SphState& S,
                                             int* __restrict__ cnt) {
