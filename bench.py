@@ -45,7 +45,7 @@ sys.path.insert(0, ROOT)
 
 from yspec_amd import api  # noqa: E402
 from yspec_amd.flops import ode_flops, sum_flops, fft_bytes  # noqa: E402
-from yspec_amd.synth import batch_events, synth_deck, workload  # noqa: E402
+from yspec_amd.synth import batch_events, bump_deck, synth_deck, workload  # noqa: E402
 from yspec_amd.tune import tune  # noqa: E402
 
 METRIC = "(l,ω) solves/s + seismograms/s, batched vs plain GPU; gradient/forward time"
@@ -174,6 +174,59 @@ def batched_compare(deck, steps, warmup, do_tune=True):
     }
 
 
+def gradient_compare(deck, steps, warmup):
+    """The misfit-gradient workload (lmax 300, 20 mHz, 120 min at 2 s, 50 km
+    source, 50 receivers, attenuation + self-gravitation; data from the deck
+    with 2% Gaussian bumps in vp and vs): wall time of one forward against
+    one ysx_misfit_grad (its own forward, the residual, every adjoint stage
+    and the deck -> table adjoint, the 800 knot values' gradient back on the
+    host), both from inputs resident on the device."""
+    p = workload("cfg4")
+    g = api.YSpec(bump_deck(deck), p, device=0)
+    obs = g.forward()
+    g.close()
+    g = api.YSpec(deck, p, device=0)
+    for _ in range(warmup):
+        g.forward(copy=False)
+        g.misfit_grad(obs)
+    tf, tg = [], []
+    fwd_k = np.zeros(8)
+    vjp_k = np.zeros(3)
+    grad_k = np.zeros(7)
+    for k in range(steps):
+        t0 = time.perf_counter()
+        g.forward(copy=False)
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        g.misfit_grad(obs)
+        tg.append(time.perf_counter() - t0)
+        fwd_k += g.timings()
+        vjp_k += g.vjp_timings()
+        grad_k += g.grad_timings()
+        print(f"# gradient step {k + 1}/{steps} done", file=sys.stderr, flush=True)
+    fwd_k /= steps
+    vjp_k /= steps
+    grad_k /= steps
+    out = {
+        "workload": "lmax=300, fmax=20 mHz, 120 min, dt=2 s, 50 km source, 50 receivers, "
+                    "attenuation + self-gravitation; 800 knot parameters",
+        "pairs": g.npair,
+        "forward_ms": 1e3 * float(np.median(tf)),
+        "gradient_ms": 1e3 * float(np.median(tg)),
+        "gradient_over_forward": float(np.median(tg) / np.median(tf)),
+        "forward_kernel_ms": {k: round(float(v), 4) for k, v in zip(PHASES, fwd_k)},
+        "adjoint_kernel_ms": {
+            "seismograms": float(vjp_k[0]), "receiver_sum_and_assembly": float(vjp_k[1]),
+            "bvec_sweep": float(grad_k[0]), "minor_sweep": float(grad_k[1]),
+            "tor_sweep": float(grad_k[2]), "starting_solutions": float(grad_k[3]),
+            "radial": float(grad_k[4]), "source": float(grad_k[5])},
+        "note": "gradient_ms: one ysx_misfit_grad (forward included) over one forward's "
+                "wall time, medians of the steps",
+    }
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +236,10 @@ def main():
     ap.add_argument("--lmax", type=int, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 64-event plain vs batched comparison")
+    ap.add_argument("--no-gradient", action="store_true",
+                    help="skip the misfit-gradient workload")
+    ap.add_argument("--gradient-only", action="store_true",
+                    help="only the misfit-gradient workload, for profiling")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the 64-event comparison (no tuning), for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -203,6 +260,9 @@ def main():
         sys.exit(2)
 
     deck = synth_deck(0)
+    if args.gradient_only:
+        print(json.dumps(gradient_compare(deck, args.steps, args.warmup)), flush=True)
+        return
     if args.batched_only:
         print(json.dumps(batched_compare(deck, args.steps, args.warmup, do_tune=False)),
               flush=True)
@@ -278,9 +338,8 @@ def main():
                              "78.6 TFLOP/s; traffic = HBM bytes per launch from rocprofv3 "
                              "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy"},
         "host_copy_ms_per_forward": pcie_ms,
-        # the metric's third part is the gradient with respect to the knot
-        # parameters (rho, vp, vs, Qmu), which needs the reverse ODE sweeps:
-        # not built, so no ratio is reported for it
+        # the metric's third part: the misfit gradient with respect to the
+        # knot values at its own workload (filled in below)
         "gradient_over_forward": None,
         "moment_tensor_gradient": {
             "kernel_ms": float(mt_k.sum()),
@@ -294,6 +353,10 @@ def main():
         },
     }
     g.close()
+    if not args.no_gradient and args.lmax is None:
+        gr = gradient_compare(deck, args.steps, args.warmup)
+        line["gradient_over_forward"] = gr["gradient_over_forward"]
+        line["gradient"] = gr
     if not args.no_batched and args.lmax is None:
         line["batched"] = batched_compare(deck, args.steps, args.warmup)
     if not args.no_cpu:

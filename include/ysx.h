@@ -250,6 +250,21 @@ int ysx_source_model_vjp(ysx_ctx *ctx, double *table_bar);
  * table_bar to the deck. */
 int ysx_model_vjp(ysx_ctx *ctx, double *table_bar);
 
+/* L2 waveform misfit and its gradient with respect to the deck's knot
+ * values (no counterpart in the reference): one forward of the run, then
+ *   misfit = 0.5 sum (seismogram - obs)^2
+ * over the first nout samples of the 3 nr traces (the samples the output
+ * files hold), and knot_grad (nknot x 4, row-major) = d misfit / d (rho,
+ * vp, vs, Qmu) of each knot in the deck's SI units, where vp (vs) moves vpv
+ * and vph (vsv and vsh) together.  obs: nout x 3 x nr column-major (time
+ * fastest), the first nout samples of each trace in ysx_seismograms'
+ * layout.  The residual is formed on the device; the chain is
+ * ysx_seismograms_vjp, ysx_spectra_vjp, ysx_model_vjp and
+ * ysx_model_table_vjp, with ysx_model_vjp's scope; stage times from
+ * ysx_timings, ysx_vjp_timings and ysx_grad_timings afterwards. */
+int ysx_misfit_grad(ysx_ctx *ctx, const double *obs, int nout, double *misfit,
+                    double *knot_grad);
+
 /* Kernel milliseconds of the last ysx_model_vjp: [0] b-vector sweep, [1]
  * minor sweep, [2] toroidal sweep, [3] starting solutions, [4] radial
  * solves, [5] source vectors, [6] the sum. */

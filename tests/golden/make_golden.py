@@ -22,9 +22,13 @@ one near the antipode).
         per-pair records of the lmax = 600, fmax = 40 mHz workload at
         l = 0, 1, 2, 150, 300, 450, 600 (a spread subset of each degree's
         frequencies)
+    python tests/golden/make_golden.py fd_reference   # fd_reference.npz:
+        central differences of the reference's L2 misfit at 20 seeded knot
+        values of a reduced gradient workload (lmax = 40, 6 mHz, 50 km)
 """
 from __future__ import annotations
 
+import dataclasses
 import os
 import subprocess
 import sys
@@ -37,7 +41,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, HERE]
 
 from yspec_amd.inputs import Deck, read_deck, read_params  # noqa: E402
-from yspec_amd.synth import synth_deck, workload  # noqa: E402
+from yspec_amd.synth import bump_deck, synth_deck, workload  # noqa: E402
 from refdump import read_dump, MODEL_NAMES, PAIR_FIELDS as PAIR_FIELDS_REF  # noqa: E402
 
 REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
@@ -224,10 +228,91 @@ def cfg2_records(outdir, keep_per_l=44):
     return path, os.path.getsize(path)
 
 
+# the reduced gradient workload of the finite-difference fixture: the
+# lmax = 300, 20 mHz, 50 km source workload cut to lmax = 40, 6 mHz, 40 min
+# at 10 s and 4 receivers (attenuation and self-gravitation kept)
+FD_RUN = dict(lmax=40, nr=4, f2=6.0, tout=40.0, dt=10.0, f21=4.5, f22=5.5)
+FD_ENTRIES = 20
+FD_REL = 1e-6
+
+
+def _ref_seis(deck, p, td, tag):
+    """(nout, 3, nr) seismograms of the reference on deck (17 digits)"""
+    d = os.path.join(td, tag)
+    os.makedirs(d)
+    with open(os.path.join(d, "model.deck"), "w") as f:
+        f.write(deck.to_text(digits=17))
+    q = dataclasses.replace(p, model_file="model.deck", prefix="out")
+    with open(os.path.join(d, "in.txt"), "w") as f:
+        f.write(q.to_text())
+    os.makedirs(os.path.join(d, "dump"))
+    subprocess.run([REF_DUMP, "in.txt", "dump"], cwd=d, check=True, timeout=900,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    D = read_dump(os.path.join(d, "dump"))
+    seis = np.stack([D["seis"][c].real for c in range(3)], axis=1)
+    return seis[:int(D["nout"])]
+
+
+def fd_reference(outdir, workers=8):
+    """Central differences (h = 1e-6 relative) of the reference's L2 misfit
+    0.5 sum((seis - obs)^2) over the first nout samples, obs from the deck
+    with Gaussian bumps, at FD_ENTRIES seeded (knot, parameter) entries of
+    rho, vp, vs, Qmu (deck columns 1, 2, 3, 5 of the text deck: vp and vs
+    move both polarisations of an isotropic deck).  Decks are written with
+    17 digits; h is the difference of the values as written."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    deck = synth_deck(0)
+    p = workload("cfg4", **FD_RUN)
+    rng = np.random.default_rng(31)
+    cols = (1, 2, 3, 5)
+    entries = []
+    while len(entries) < FD_ENTRIES:
+        k, q = int(rng.integers(0, deck.nknot)), int(rng.integers(0, 4))
+        if deck.cols[k, cols[q]] != 0.0 and (k, q) not in entries:
+            entries.append((k, q))
+    jobs = {"obs": bump_deck(deck), "base": deck}
+    hs = []
+    for n, (k, q) in enumerate(entries):
+        v = deck.cols[k, cols[q]]
+        vals = [float(f"{v * (1.0 + sg * FD_REL):.17g}") for sg in (1.0, -1.0)]
+        hs.append(vals[0] - vals[1])
+        for tag, val in zip(("p", "m"), vals):
+            c = deck.cols.copy()
+            c[k, cols[q]] = val
+            if deck.ians == 0 and cols[q] in (2, 3):
+                c[k, cols[q] + 4] = val
+            jobs[f"{tag}{n}"] = Deck(deck.title, deck.ians, deck.tref, deck.nknot, deck.nic,
+                                     deck.noc, c)
+    with tempfile.TemporaryDirectory() as td, ThreadPoolExecutor(workers) as ex:
+        futs = {tag: ex.submit(_ref_seis, d, p, td, tag) for tag, d in jobs.items()}
+        seis = {tag: f.result() for tag, f in futs.items()}
+    obs = seis["obs"]
+
+    def J(s):
+        return 0.5 * float(np.sum((s - obs) ** 2))
+
+    fd = np.array([(J(seis[f"p{n}"]) - J(seis[f"m{n}"])) / hs[n] for n in range(FD_ENTRIES)])
+    out = dict(deck_cols=deck.cols,
+               deck_head=np.array([deck.nknot, deck.nic, deck.noc, deck.ians]),
+               deck_tref=np.array(deck.tref), ip=p.int_block(), fp=p.float_block(),
+               lat=p.lat, lon=p.lon, obs=obs, misfit=np.array(J(seis["base"])),
+               entries=np.array(entries), h=np.array(hs), fd=fd)
+    path = os.path.join(outdir, "fd_reference.npz")
+    np.savez_compressed(path, **out)
+    return path, os.path.getsize(path)
+
+
 def main():
     if not os.path.exists(REF_DUMP):
         sys.exit("build the reference first: make -C oracle ref")
     only = set(sys.argv[1:])
+    if "fd_reference" in only:
+        path, size = fd_reference(HERE)
+        print(f"{os.path.basename(path)}: {size / 1024:.1f} KiB")
+        only.discard("fd_reference")
+        if not only:
+            return
     if "cfg2_records" in only:
         path, size = cfg2_records(HERE)
         print(f"{os.path.basename(path)}: {size / 1024:.1f} KiB")

@@ -2,16 +2,20 @@
 vs and Qmu at its knots: ysx_model_vjp (the discrete adjoint of every
 per-pair solve, starting solution and source vector) followed by the host
 adjoint of the deck -> table transform, against central differences
-(h = 1e-6 relative) of GPU forwards on seeded knots.
-
+(h = 1e-6 relative) on seeded knots:
+  - of the reference program's own misfit (tests/golden/fd_reference.npz,
+    written by tests/golden/make_golden.py from the reference modules),
+  - of GPU forwards, at the lmax = 300, 20 mHz, 50 km, 50-receiver gradient
+    workload and at smaller runs with and without attenuation.
 The data are seismograms of the same deck with 2% Gaussian bumps in vp and
-vs.  The reference has no gradient; the check is against the GPU forward's
-own finite differences (the forward matches the reference to 1e-9)."""
+vs.  The reference has no gradient of its own."""
+import os
+
 import numpy as np
 import pytest
 
-from yspec_amd.inputs import Deck
-from yspec_amd.synth import synth_deck, workload
+from yspec_amd.inputs import Deck, Params
+from yspec_amd.synth import bump_deck as bumped, synth_deck, workload
 
 pytestmark = pytest.mark.gpu
 COLS = {1: "rho", 2: "vp", 3: "vs", 5: "Qmu"}
@@ -22,16 +26,6 @@ def api():
     from yspec_amd import api as a
     a.lib()
     return a
-
-
-def bumped(deck, amp=0.02, at=0.8, width=0.05):
-    """2% Gaussian bumps in vp and vs centred at radius `at` (fraction of
-    the Earth's radius)"""
-    cols = deck.cols.copy()
-    x = cols[:, 0] / 6371000.0
-    for c in (2, 3, 6, 7):
-        cols[:, c] *= 1.0 + amp * np.exp(-((x - at) / width) ** 2)
-    return Deck(deck.title, deck.ians, deck.tref, deck.nknot, deck.nic, deck.noc, cols)
 
 
 def seeded_entries(deck, n, seed):
@@ -118,3 +112,124 @@ def test_gradient_unsupported_runs(api):
             g.model_vjp()
         assert ei.value.code == 4
         g.close()
+
+
+def _fixture_run(z):
+    deck = Deck("fd", int(z["deck_head"][3]), float(z["deck_tref"]), int(z["deck_head"][0]),
+                int(z["deck_head"][1]), int(z["deck_head"][2]), z["deck_cols"])
+    ip, fp = z["ip"], z["fp"]
+    p = Params(prefix="yspec.out", model_file="fd", ats=int(ip[2]), grav=int(ip[3]),
+               out=int(ip[4]), cor=int(ip[5]), lmin=int(ip[0]), lmax=int(ip[1]), f1=fp[0],
+               f2=fp[1], tout=fp[2], dt=fp[3], f11=fp[4], f12=fp[5], f21=fp[6], f22=fp[7],
+               src_depth=fp[8], src_lat=fp[9], src_lon=fp[10], mm=fp[11:17].copy(),
+               rec_depth=fp[17], lat=z["lat"], lon=z["lon"])
+    return deck, p
+
+
+def test_gradient_vs_reference_differences(api):
+    """ysx_misfit_grad against central differences of the reference
+    program's misfit (lmax 40, 6 mHz, 50 km source, attenuation, 4
+    receivers; obs from the reference on the bumped deck): the misfit to
+    1e-6 and 20 seeded entries to 1e-5 (relative L2, and each entry on the
+    scale of the largest d misfit / d log(parameter))"""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "fd_reference.npz"))
+    deck, p = _fixture_run(z)
+    g = api.YSpec(deck, p)
+    obs = z["obs"]
+    assert g.nout == obs.shape[0]
+    misfit, grad = g.misfit_grad(obs)
+    g.close()
+    assert abs(misfit - float(z["misfit"])) <= 1e-6 * float(z["misfit"]), (misfit, z["misfit"])
+    ent = z["entries"]
+    got = np.array([grad[k, q] for k, q in ent])
+    fd = z["fd"]
+    val = np.array([deck.cols[k, (1, 2, 3, 5)[q]] for k, q in ent])
+    scale = np.max(np.abs(fd * val))
+    err = np.abs(got - fd) * np.abs(val) / scale
+    rel = float(np.linalg.norm((got - fd) * val) / np.linalg.norm(fd * val))
+    for (k, q), a, b, e in zip(ent, got, fd, err):
+        print(f"knot {k:3d} {api.KNOT_PARAMS[q]:4s} gradient {a: .6e} reference fd {b: .6e} "
+              f"scaled error {e:.1e}", flush=True)
+    print(f"misfit {misfit:.10e} reference {float(z['misfit']):.10e}; relative L2 {rel:.2e}",
+          flush=True)
+    assert rel <= 1e-5 and np.all(err <= 1e-5)
+
+
+def test_gradient_workload_vs_central_differences(api):
+    """The gradient workload itself (lmax 300, 20 mHz, 120 min at 2 s, 50 km
+    source, 50 receivers, attenuation, 98,728 pairs): ysx_misfit_grad against
+    central differences of GPU forwards at 20 seeded (knot, parameter)
+    entries, to 1e-5"""
+    deck = synth_deck(0)
+    p = workload("cfg4")
+    g = api.YSpec(bumped(deck), p)
+    obs = g.forward()
+    g.close()
+    g = api.YSpec(deck, p)
+    nout = g.nout
+    misfit, grad = g.misfit_grad(obs)
+    print(f"pairs {g.npair}, misfit {misfit:.6e}, gradient kernels {g.grad_timings()[-1]:.1f} ms",
+          flush=True)
+    g.close()
+    knots = np.stack([deck.cols[:, c] for c in (1, 2, 3, 5)], axis=1)
+    rng = np.random.default_rng(20)
+    ent = []
+    while len(ent) < 20:
+        k, q = int(rng.integers(0, deck.nknot)), int(rng.integers(0, 4))
+        if knots[k, q] != 0.0 and (k, q) not in ent:
+            ent.append((k, q))
+    h = api.YSpec(deck, p)
+
+    def J(kn):
+        h.set_model(api.deck_with_knots(deck, kn))
+        h.set_run(p)
+        r = h.forward()[:nout] - obs[:nout]
+        return 0.5 * float(np.sum(r * r))
+
+    got, fd, val = [], [], []
+    for k, q in ent:
+        v = knots[k, q]
+        kp, km = knots.copy(), knots.copy()
+        kp[k, q] = v * (1 + 1e-6)
+        km[k, q] = v * (1 - 1e-6)
+        fd.append((J(kp) - J(km)) / (kp[k, q] - km[k, q]))
+        got.append(grad[k, q])
+        val.append(v)
+    h.close()
+    got, fd, val = map(np.array, (got, fd, val))
+    scale = np.max(np.abs(fd * val))
+    err = np.abs(got - fd) * np.abs(val) / scale
+    rel = float(np.linalg.norm((got - fd) * val) / np.linalg.norm(fd * val))
+    for (k, q), a, b, e in zip(ent, got, fd, err):
+        print(f"knot {k:3d} {api.KNOT_PARAMS[q]:4s} gradient {a: .6e} fd {b: .6e} "
+              f"scaled error {e:.1e}", flush=True)
+    print(f"relative L2 over the entries {rel:.2e}", flush=True)
+    assert rel <= 1e-5 and np.all(err <= 1e-5)
+
+
+def test_autograd_knot_gradient(api):
+    """KnotSeismograms (torch.autograd) against ysx_misfit_grad: the same
+    misfit through torch, backward through the library's adjoints"""
+    import torch
+
+    from yspec_amd.autograd import knot_seismograms
+
+    deck = synth_deck(0)
+    p = workload("cfg1", lmin=0, lmax=24, nr=2, ats=1, src_depth=50.0)
+    g = api.YSpec(bumped(deck), p)
+    obs = g.forward()
+    g.set_model(deck)
+    g.set_run(p)
+    nout = g.nout
+    misfit, grad = g.misfit_grad(obs)
+    knots = torch.tensor(np.stack([deck.cols[:, c] for c in (1, 2, 3, 5)], axis=1),
+                         dtype=torch.float64, requires_grad=True)
+    seis = knot_seismograms(knots, g, deck, p)
+    loss = 0.5 * ((seis[:nout] - torch.from_numpy(obs[:nout])) ** 2).sum()
+    loss.backward()
+    g.close()
+    tg = knots.grad.numpy()
+    lv = float(loss.detach())
+    print(f"misfit {misfit:.6e} torch {lv:.6e}", flush=True)
+    assert abs(lv - misfit) <= 1e-12 * misfit
+    assert np.max(np.abs(tg - grad)) <= 1e-12 * np.max(np.abs(grad))

@@ -29,7 +29,7 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts",
            "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
            "ysx_vjp_timings", "ysx_source_model_vjp", "ysx_model_table_noalloc",
-           "ysx_model_vjp", "ysx_grad_timings")
+           "ysx_model_vjp", "ysx_grad_timings", "ysx_misfit_grad")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -95,6 +95,7 @@ def lib():
         L.ysx_source_model_vjp.argtypes = [vp, D]
         L.ysx_model_vjp.argtypes = [vp, D]
         L.ysx_grad_timings.argtypes = [vp, D]
+        L.ysx_misfit_grad.argtypes = [vp, D, C.c_int, D, D]
         L.ysx_model_table_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                               C.c_double, D, C.c_void_p]
         _LIB = L
@@ -127,6 +128,30 @@ def model_table_host(deck):
     return out
 
 
+# deck columns of a knot's rho, vp, vs and Qmu (vp and vs: both polarisations)
+KNOT_PARAMS = ("rho", "vp", "vs", "Qmu")
+_KNOT_COLS = ((1,), (2, 6), (3, 7), (5,))
+
+
+def knot_gradient(deck_bar):
+    """(nknot, 4) gradient with respect to each knot's rho, vp, vs, Qmu from
+    the (nknot, 9) deck-column gradient (vp and vs move vpv and vph, vsv and
+    vsh together), as ysx_misfit_grad folds it."""
+    return np.stack([sum(deck_bar[:, c] for c in cs) for cs in _KNOT_COLS], axis=1)
+
+
+def deck_with_knots(deck, knots):
+    """A copy of `deck` whose rho, vp (vpv = vph), vs (vsv = vsh) and Qmu
+    columns are the (nknot, 4) array knots."""
+    from .inputs import Deck
+
+    cols = np.array(deck.cols, dtype=np.float64)
+    for q, cs in enumerate(_KNOT_COLS):
+        for c in cs:
+            cols[:, c] = knots[:, q]
+    return Deck(deck.title, deck.ians, deck.tref, deck.nknot, deck.nic, deck.noc, cols)
+
+
 def model_table_vjp(deck, table_bar):
     """(nknot, 9) deck_bar = J^T table_bar for the deck -> table transform
     (table_bar: (29, nknot), the layout of model_table_host)."""
@@ -152,12 +177,16 @@ class YSpec:
         if rc:
             msg = self.L.ysx_last_error(self.h).decode() if self.h else ""
             raise YSpecError(rc, msg or "ysx_create failed")
+        self.set_model(deck)
+        if params is not None:
+            self.set_run(params)
+
+    def set_model(self, deck):
+        """A new deck model; the run must be set again afterwards."""
         cols = np.ascontiguousarray(deck.cols, dtype=np.float64)
         self._check(self.L.ysx_set_model(self.h, deck.nknot, deck.nic, deck.noc, deck.ians,
                                          deck.tref, _dp(cols)))
         self.deck = deck
-        if params is not None:
-            self.set_run(params)
 
     def _check(self, rc):
         if rc:
@@ -343,6 +372,21 @@ class YSpec:
         self.spectra_vjp(np.zeros(0, np.int32), np.zeros(0, np.int32))
         tb = self.model_vjp()
         return misfit, model_table_vjp(self.deck, tb)
+
+    def misfit_grad(self, obs, nout=None):
+        """ysx_misfit_grad: one forward, the misfit 0.5 * sum((seis - obs)^2)
+        over the first nout samples (default: the samples the drop-in
+        program writes) and its (nknot, 4) gradient with respect to each
+        knot's rho, vp, vs and Qmu (vp and vs moving both polarisations).
+        obs: (>= nout, 3, nr) seismograms."""
+        nout = self.nout if nout is None else nout
+        o = np.asfortranarray(np.asarray(obs, dtype=np.float64)[:nout])
+        if o.shape != (nout, 3, self.p.nr):
+            raise ValueError(f"obs must hold (nout, 3, nr) = ({nout}, 3, {self.p.nr}) samples")
+        misfit = np.zeros(1)
+        grad = np.zeros((self.deck.nknot, 4))
+        self._check(self.L.ysx_misfit_grad(self.h, _dp(o), nout, _dp(misfit), _dp(grad)))
+        return float(misfit[0]), grad
 
     def model_table(self):
         """(29, nknot) model table the kernels read (built on the device)."""

@@ -133,6 +133,13 @@ struct ysx_ctx {
   long long* d_rtoff = nullptr;
   size_t rtape_bytes = 0;
   double ms_grad[7] = {0};  // kernel ms of the last ysx_model_vjp
+  // ysx_misfit_grad: the observed traces and the residual's per-block sums
+  double *d_obs = nullptr, *d_rpart = nullptr;
+  size_t obs_n = 0;
+  // the deck as set (row-major nknot x 9) for the deck -> table adjoint
+  std::vector<double> deck;
+  int ians = 0;
+  double tref_period = 0.0;
 };
 
 namespace {
@@ -172,7 +179,8 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
   dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_ack);
   dfree(c->d_atape); dfree(c->d_acks); dfree(c->d_atsc); dfree(c->d_aerr); dfree(c->d_aq);
-  dfree(c->d_rtape); dfree(c->d_rtoff);
+  dfree(c->d_rtape); dfree(c->d_rtoff); dfree(c->d_obs); dfree(c->d_rpart);
+  c->obs_n = 0;
   c->adj_blocks = c->adj_slots = c->adj_tmax = 0;
   c->rtape_bytes = 0;
   c->have_specb = c->have_recb = false;
@@ -531,6 +539,9 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
   ctx->have_model = false;
   std::string e = build_model(ctx->N, nknot, nic, noc, ians, tref_period, deck, ctx->M);
   if (!e.empty()) return set_err(ctx, YSX_ERR_INPUT, e);
+  ctx->deck.assign(deck, deck + 9 * (size_t)nknot);
+  ctx->ians = ians;
+  ctx->tref_period = tref_period;
   ctx->lds_bytes = sizeof(double) * F_COUNT * (size_t)nknot;
   // k_source and the radial kernels stage the table in LDS when it fits
   // (with room for their own 2.3 KB); larger decks are read through the cache
@@ -1298,6 +1309,8 @@ int source_sums(ysx_ctx* ctx, bool model, std::vector<double>& out) {
   return YSX_OK;
 }
 
+int seis_vjp_device(ysx_ctx* ctx);
+
 }  // namespace
 
 extern "C" int ysx_seismograms_vjp(ysx_ctx* ctx, const double* seis_bar, double* ur_bar,
@@ -1309,8 +1322,22 @@ extern "C" int ysx_seismograms_vjp(ysx_ctx* ctx, const double* seis_bar, double*
     return set_err(ctx, YSX_ERR_ARG, "several events: the spectra adjoints stay on the device");
   size_t ns = 0;
   for (const HostRun& R : ctx->R) ns += 3 * (size_t)R.nt * R.nr;
-  hipStream_t st = ctx->stream;
   HIPCHK(hipMemcpy(ctx->d_seisb, seis_bar, ns * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = seis_vjp_device(ctx)) return rc;
+  size_t n = (size_t)ctx->R[0].nw * ctx->R[0].nr;
+  double* outs[3] = {ur_bar, ut_bar, up_bar};
+  for (int c = 0; c < 3; c++)
+    if (outs[c])
+      HIPCHK(hipMemcpy(outs[c], ctx->d_specb + c * n, n * sizeof(cd), hipMemcpyDeviceToHost));
+  return YSX_OK;
+}
+
+namespace {
+
+// the seismograms' adjoint from d_seisb to the spectra adjoints d_specb
+int seis_vjp_device(ysx_ctx* ctx) {
+  const int nev = (int)ctx->rk.size();
+  hipStream_t st = ctx->stream;
   HIPCHK(hipEventRecord(ctx->ev[0], st));
   for (int e = 0; e < nev; e++) {
     const HostRun& R = ctx->R[e];
@@ -1332,13 +1359,10 @@ extern "C" int ysx_seismograms_vjp(ysx_ctx* ctx, const double* seis_bar, double*
   HIPCHK(hipEventSynchronize(ctx->ev[1]));
   if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[0])) return rc;
   ctx->have_specb = true;
-  size_t n = (size_t)ctx->R[0].nw * ctx->R[0].nr;
-  double* outs[3] = {ur_bar, ut_bar, up_bar};
-  for (int c = 0; c < 3; c++)
-    if (outs[c])
-      HIPCHK(hipMemcpy(outs[c], ctx->d_specb + c * n, n * sizeof(cd), hipMemcpyDeviceToHost));
   return YSX_OK;
 }
+
+}  // namespace
 
 extern "C" int ysx_spectra_vjp(ysx_ctx* ctx, const double* ur_bar, const double* ut_bar,
                                const double* up_bar, int n, const int* l, const int* j,
@@ -1581,6 +1605,56 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   ctx->ms_grad[6] = 0.0;
   for (int q = 0; q < 6; q++) ctx->ms_grad[6] += ctx->ms_grad[q];
   for (size_t q = 0; q < ntab; q++) table_bar[q] += tb[q];
+  return YSX_OK;
+}
+
+extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double* misfit,
+                               double* knot_grad) {
+  if (!ctx || !obs || !misfit || !knot_grad) return YSX_ERR_ARG;
+  if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
+  if (ctx->rk.size() != 1)
+    return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_misfit_grad: one event per context");
+  const HostRun& R = ctx->R[0];
+  if (nout < 1 || nout > R.nt)
+    return set_err(ctx, YSX_ERR_ARG, "nout outside 1 .. nt (" + std::to_string(R.nt) + ")");
+  if (int rc = ysx_forward(ctx, nullptr)) return rc;
+  if (int rc = adjoint_run(ctx)) return rc;
+  const size_t ntrace = 3 * (size_t)R.nr, nobs = (size_t)nout * ntrace;
+  constexpr int RBLK = 1024;
+  if (ctx->obs_n < nobs) {
+    dfree(ctx->d_obs);
+    HIPCHK(hipMalloc(&ctx->d_obs, nobs * sizeof(double)));
+    ctx->obs_n = nobs;
+  }
+  if (!ctx->d_rpart) HIPCHK(hipMalloc(&ctx->d_rpart, RBLK * sizeof(double)));
+  HIPCHK(hipMemcpy(ctx->d_obs, obs, nobs * sizeof(double), hipMemcpyHostToDevice));
+  const size_t n = (size_t)R.nt * ntrace;
+  const unsigned nb = (unsigned)std::min<size_t>(RBLK, (n + 255) / 256);
+  launch_residual(dim3(nb), dim3(256), 0, ctx->stream, ctx->d_seis, ctx->d_obs, R.nt, nout,
+                  ntrace, ctx->d_seisb, ctx->d_rpart);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part(nb);
+  HIPCHK(hipMemcpy(part.data(), ctx->d_rpart, nb * sizeof(double), hipMemcpyDeviceToHost));
+  double sum = 0.0;
+  for (double v : part) sum += v;
+  *misfit = 0.5 * sum;
+  if (int rc = seis_vjp_device(ctx)) return rc;
+  if (int rc = ysx_spectra_vjp(ctx, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr))
+    return rc;
+  const int nk = ctx->M.nknot;
+  std::vector<double> tb((size_t)F_COUNT * nk, 0.0), db(9 * (size_t)nk, 0.0);
+  if (int rc = ysx_model_vjp(ctx, tb.data())) return rc;
+  std::string e = model_table_vjp(ctx->N, nk, ctx->M.nic, ctx->M.noc, ctx->ians,
+                                  ctx->tref_period, ctx->deck.data(), tb.data(), db.data());
+  if (!e.empty()) return set_err(ctx, YSX_ERR_INPUT, e);
+  for (int k = 0; k < nk; k++) {
+    const double* d = db.data() + 9 * (size_t)k;
+    double* g = knot_grad + 4 * (size_t)k;
+    g[0] = d[1];         // rho
+    g[1] = d[2] + d[6];  // vp: vpv and vph together
+    g[2] = d[3] + d[7];  // vs: vsv and vsh together
+    g[3] = d[5];         // Qmu
+  }
   return YSX_OK;
 }
 

@@ -22,6 +22,36 @@ __global__ void k_fill0(double* __restrict__ p, size_t n) {
     p[i] = 0.0;
 }
 
+// residual seisb = seis - obs on the first nout samples of every trace and 0
+// after them (seismograms column-major, time fastest: nt per trace; obs nout
+// per trace), with per-block sums of the squared residual in part
+__global__ void __launch_bounds__(256) k_residual(const double* __restrict__ seis,
+                                                  const double* __restrict__ obs, int nt,
+                                                  int nout, size_t ntrace,
+                                                  double* __restrict__ seisb,
+                                                  double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  const size_t n = (size_t)nt * ntrace;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const size_t tr = i / (size_t)nt;
+    const int t = (int)(i - tr * (size_t)nt);
+    double r = 0.0;
+    if (t < nout) {
+      r = seis[i] - obs[tr * (size_t)nout + t];
+      acc += r * r;
+    }
+    seisb[i] = r;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // -------------------------------------------------------------- k_source
 // source_vector_tor / _sph / _rad, src/module_solver.f90:145-215,
 // 566-627, 1747-1847.  source_vector_sph keeps only the real part of its
@@ -1381,6 +1411,11 @@ void launch_source_model_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const doub
   hipLaunchKernelGGL(k_source_model_T, g, b, lds, s, gtab, runs, e, recb, part);
 }
 
+void launch_residual(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* seis,
+                     const double* obs, int nt, int nout, size_t ntrace, double* seisb,
+                     double* part) {
+  hipLaunchKernelGGL(k_residual, g, b, lds, s, seis, obs, nt, nout, ntrace, seisb, part);
+}
 void launch_reset(dim3 g, dim3 b, size_t lds, hipStream_t s, int* counters, int n) {
   hipLaunchKernelGGL(k_reset, g, b, lds, s, counters, n);
 }

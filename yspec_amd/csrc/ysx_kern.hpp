@@ -266,6 +266,11 @@ constexpr int FFT_G = 16;
 // with persistent blocks; ysx_k_misc.hip: everything else).  Arguments
 // after the stream are the kernel's.
 void launch_reset(dim3 g, dim3 b, size_t lds, hipStream_t s, int* counters, int n);
+// residual of the first nout samples per trace (block size 256, at most
+// as many blocks as part holds)
+void launch_residual(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* seis,
+                     const double* obs, int nt, int nout, size_t ntrace, double* seisb,
+                     double* part);
 void launch_source(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                    const RunK* runs, PairRange PR, cd* rec, int* cnt);
 void launch_start(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,

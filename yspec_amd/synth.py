@@ -10,6 +10,8 @@
   ICB, CMB and Moho are exact.
 * ``workload``: the parameter blocks of the named workloads, with seeded
   moment tensors (uniform in +-1e20 N m) and receivers uniform on the sphere.
+* ``bump_deck``: the data deck of the misfit-gradient workload (2% Gaussian
+  bumps in vp and vs).
 """
 from __future__ import annotations
 
@@ -69,6 +71,17 @@ def synth_deck(seed: int = 0, knots: int = 200, perturb: float = 0.01) -> Deck:
     return Deck("synth.200", 0, 1.0, len(rows), nic, noc, cols)
 
 
+def bump_deck(deck: Deck, amp: float = 0.02, at: float = 0.8, width: float = 0.05) -> Deck:
+    """The deck with Gaussian bumps of relative amplitude amp in vp and vs
+    (both polarisations) centred at radius `at` (fraction of 6371 km), width
+    `width`: the data deck of the misfit-gradient workload."""
+    cols = deck.cols.copy()
+    x = cols[:, 0] / R_EARTH
+    for c in (2, 3, 6, 7):
+        cols[:, c] *= 1.0 + amp * np.exp(-((x - at) / width) ** 2)
+    return Deck(deck.title, deck.ians, deck.tref, deck.nknot, deck.nic, deck.noc, cols)
+
+
 def _receivers(rng, n):
     u = rng.uniform(0.0, 1.0, (n, 2))
     lat = np.degrees(np.arcsin(2.0 * u[:, 0] - 1.0))
@@ -80,6 +93,8 @@ WORKLOADS = {
     # name: ats, lmax, fmin, fmax, tout (min), dt (s), filters, depth (km), nr
     "cfg1": (0, 150, 0.1, 10.0, 60.0, 10.0, (0.2, 0.5, 8.0, 9.5), 100.0, 10),
     "cfg2": (1, 600, 0.1, 40.0, 360.0, 1.0, (0.2, 0.5, 35.0, 38.0), 600.0, 200),
+    # the gradient workload: attenuation on, so that Qmu enters the forward
+    "cfg4": (1, 300, 0.1, 20.0, 120.0, 2.0, (0.2, 0.5, 17.0, 19.0), 50.0, 50),
 }
 
 
