@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "ysx_adjoint.hpp"
+#include "ysx_radial.hpp"
 
 using namespace ysx;
 
@@ -341,7 +342,55 @@ static double check_start_tangent(Model& M, int is, double frac, int l, int n, F
   return worst;
 }
 
+// the radial segments: enumerated one by one (rad_segment) and put in sweep
+// order, they are rad_sweeps' own steps, bitwise (interval, radius, step);
+// returns the number of differences over frequencies and source and
+// receiver radii (knot radii and inner points)
+static double check_rad_segments(Model& M) {
+  const Tab T = M.T();
+  const int nk = M.nk;
+  double amin = 1e300;
+  for (int k = 1; k <= nk; k++) amin = std::fmin(amin, T(F_ALPHA, k));
+  const double top = T(F_R, nk);
+  int bad = 0, total = 0;
+  for (double w : {0.02, 0.3, 1.7, 7.5, 30.0, 100.0})
+    for (double rs : {0.3 * top, T(F_R, 31), T(F_R, 90), T(F_R, 150), 0.9922 * top, top})
+      for (double rr : {top, 0.95 * top}) {
+        PS s;
+        set_l(s, 0);
+        s.pibigg = 1.0;
+        set_omega(s, C(w, -0.01), 1.0, 1.0);
+        const double dr = 1.0 / (4.0 * s.rew / amin);
+        struct St { int sweep, il; double x, h; };
+        std::vector<St> a, b;
+        cd y1[2], y2[2], y1r[2], y2r[2];
+        bool below;
+        int nsr = 0;
+        rad_sweeps(T, nk, s, dr, rs, rr, NoGroup(),
+                   [&](int sw, int il, double x, double h, const cd*) { a.push_back({sw, il, x, h}); },
+                   y1, y2, y1r, y2r, below, nsr);
+        const int isrc = rad_src_interval(T, nk, rs), itop = rad_top_interval(T, nk);
+        for (int q = 0; q < isrc + (itop - isrc + 1); q++) {
+          const int sw = q < isrc ? 0 : 1, i = q < isrc ? q + 1 : itop - (q - isrc);
+          rad_segment(T, nk, s, dr, rs, rr, sw, i, isrc, itop,
+                      [&](double x, double h) { b.push_back({sw, i, x, h}); });
+        }
+        total += (int)a.size();
+        if (a.size() != b.size()) {
+          bad += 1 + (int)std::abs((long)a.size() - (long)b.size());
+          continue;
+        }
+        for (size_t k = 0; k < a.size(); k++)
+          if (a[k].sweep != b[k].sweep || a[k].il != b[k].il || a[k].x != b[k].x ||
+              a[k].h != b[k].h)
+            bad++;
+      }
+  fprintf(stderr, "rad_segments: %d steps compared\n", total);
+  return (double)bad;
+}
+
 static int main2(Model& M) {
+  printf("rad_segments %.3e\n", check_rad_segments(M));
   printf("rad_coef_adj %.3e\n", std::fmax(check_rad_coef(M, 150), check_rad_coef(M, 60)));
   double ws = 0.0, wf = 0.0, wt = 0.0, wr = 0.0;
   for (double frac : {0.0, 0.37}) {

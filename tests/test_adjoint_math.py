@@ -41,7 +41,7 @@ def results(tmp_path_factory):
 IDENTITIES = ["bilinear_solid_minor", "bilinear_bvec", "transpose_solid_minor",
               "transpose_bvec", "transpose_fluid_minor", "transpose_tor",
               "step_transpose_solid_minor", "step_transpose_bvec",
-              "step_transpose_fluid_minor", "step_transpose_tor"]
+              "step_transpose_fluid_minor", "step_transpose_tor", "rad_segments"]
 DIFFERENCES = ["lane_ob_solid_minor", "lane_ob_bvec", "lane_ob_fluid_minor", "lane_ob_tor",
                "base_adj_solid", "base_adj_fluid", "base_adj_tor",
                "step_gradient_solid_minor", "step_gradient_bvec",

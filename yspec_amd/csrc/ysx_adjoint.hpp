@@ -811,4 +811,107 @@ __host__ __device__ inline void rad_ystart_d(const Tab& T, const PS& s, int is, 
   y[1] = rr * (calpha * gamma) - 4.0 * (cmu * ratio);
 }
 
+// ------------------------------------------------------------ radial steps
+// One Cash-Karp step of the radial system (rkck's tableau, the stage radii
+// of rk_step) applied to the NC columns Y[c], with the six stage
+// coefficient sets evaluated once: the step matrix on NC vectors.
+template <int NC>
+__host__ __device__ inline void rad_step_cols(const Tab& T, const PS& s, int il, double x,
+                                              double h, cd (*Y)[2]) {
+  using namespace ck;
+  const double off[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
+  RadRHS f;
+  RadRHS::Coef c[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) c[k] = f.coef(T, s, il, k == 0 ? x : x + off[k] * h);
+#pragma unroll
+  for (int col = 0; col < NC; col++) {
+    cd* y = Y[col];
+    cd k1[2], k2[2], k3[2], k4[2], k5[2], k6[2], t[2];
+    RadRHS::apply(c[0], y, k1);
+    for (int i = 0; i < 2; i++) t[i] = y[i] + (b21 * h) * k1[i];
+    RadRHS::apply(c[1], t, k2);
+    for (int i = 0; i < 2; i++) t[i] = y[i] + h * (b31 * k1[i] + b32 * k2[i]);
+    RadRHS::apply(c[2], t, k3);
+    for (int i = 0; i < 2; i++) t[i] = y[i] + h * (b41 * k1[i] + b42 * k2[i] + b43 * k3[i]);
+    RadRHS::apply(c[3], t, k4);
+    for (int i = 0; i < 2; i++)
+      t[i] = y[i] + h * (b51 * k1[i] + b52 * k2[i] + b53 * k3[i] + b54 * k4[i]);
+    RadRHS::apply(c[4], t, k5);
+    for (int i = 0; i < 2; i++)
+      t[i] = y[i] + h * (b61 * k1[i] + b62 * k2[i] + b63 * k3[i] + b64 * k4[i] + b65 * k5[i]);
+    RadRHS::apply(c[5], t, k6);
+    for (int i = 0; i < 2; i++) y[i] = y[i] + h * (c1 * k1[i] + c3 * k3[i] + c4 * k4[i] + c6 * k6[i]);
+  }
+}
+
+// One radial step at (il, xr, h) from state y taken back: the row x <- x S
+// (S the step matrix), and every stage's pairing kb_k . (dA_k z_k) through
+// rad_coef_adj and the spline weights into acc[v * 4 + e] (field
+// rad_field(v); e = 0, 1: the value at knots il, il + 1; e = 2, 3: the
+// second derivative at il, il + 1).  Loops unrolled: fixed register
+// indices throughout.
+__host__ __device__ inline void rad_step_back(const Tab& T, const PS& s, int il, double xr,
+                                              double h, const cd* y, cd* x, double* acc) {
+  using namespace ck;
+  const double off[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
+  const double cc[6] = {c1, 0.0, c3, c4, 0.0, c6};
+  const double bb[6][6] = {{0, 0, 0, 0, 0, 0},
+                           {b21, 0, 0, 0, 0, 0},
+                           {b31, b32, 0, 0, 0, 0},
+                           {b41, b42, b43, 0, 0, 0},
+                           {b51, b52, b53, b54, 0, 0},
+                           {b61, b62, b63, b64, b65, 0}};
+  RadRHS f;
+  RadRHS::Coef c[6];
+  cd z[6][2], d[6][2];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    c[k] = f.coef(T, s, il, k == 0 ? xr : xr + off[k] * h);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      cd v = C(0.0);
+#pragma unroll
+      for (int m = 0; m < k; m++) v = v + bb[k][m] * d[m][i];
+      z[k][i] = y[i] + h * v;
+    }
+    RadRHS::apply(c[k], z[k], d[k]);
+  }
+  cd zb[6][2];
+  cd xn[2] = {x[0], x[1]};
+#pragma unroll
+  for (int k = 5; k >= 0; k--) {
+    cd kb[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      cd v = cc[k] * x[i];
+#pragma unroll
+      for (int m = k + 1; m < 6; m++) v = v + bb[m][k] * zb[m][i];
+      kb[i] = h * v;
+    }
+    // zb = A^T kb for A = [[t, k], [sv, -t]]
+    zb[k][0] = c[k].t * kb[0] + c[k].sv * kb[1];
+    zb[k][1] = c[k].k * kb[0] - c[k].t * kb[1];
+    xn[0] = xn[0] + zb[k][0];
+    xn[1] = xn[1] + zb[k][1];
+    const cd tb = kb[0] * z[k][0] - kb[1] * z[k][1], kbar = kb[0] * z[k][1],
+             svb = kb[1] * z[k][0];
+    const double r = k == 0 ? xr : xr + off[k] * h;
+    double vb[RAD_NV];
+    rad_coef_adj(T, s, il, r, tb, kbar, svb, vb);
+    const SplW sw = spl_w(T, il, r);
+    const double wa = sw.deg ? 1.0 : sw.a, wb = sw.deg ? 0.0 : sw.b,
+                 wa3 = sw.deg ? 0.0 : sw.a3 * sw.hh, wb3 = sw.deg ? 0.0 : sw.b3 * sw.hh;
+#pragma unroll
+    for (int v = 0; v < RAD_NV; v++) {
+      acc[v * 4 + 0] += vb[v] * wa;
+      acc[v * 4 + 1] += vb[v] * wb;
+      acc[v * 4 + 2] += vb[v] * wa3;
+      acc[v * 4 + 3] += vb[v] * wb3;
+    }
+  }
+  x[0] = xn[0];
+  x[1] = xn[1];
+}
+
 }  // namespace ysx

@@ -42,6 +42,7 @@
 #include "ysx_host.hpp"
 #include "ysx_adjoint.hpp"
 #include "ysx_kern.hpp"
+#include "ysx_radial.hpp"
 
 using namespace ysx;
 
@@ -125,13 +126,16 @@ struct ysx_ctx {
   // source from the b-vector starts (mub), of the starting solutions (m0bar,
   // y0bar), the table adjoint, the sweeps' per-block checkpoints and tapes
   cd *d_mub = nullptr, *d_m0bar = nullptr, *d_y0bar = nullptr;
-  double* d_tbar = nullptr;
+  double *d_tbar = nullptr, *d_tpart = nullptr;
   cd *d_ack = nullptr, *d_atape = nullptr;
   int *d_acks = nullptr, *d_atsc = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
   int adj_blocks = 0, adj_slots = 0, adj_tmax = 0;
-  void* d_rtape = nullptr;  // radial adjoint: every step of the l = 0 solves
-  long long* d_rtoff = nullptr;
+  // radial adjoint: segment propagators, start states, end rows, tape
+  cd *d_rprop = nullptr, *d_rys = nullptr, *d_rxe = nullptr;
+  int* d_rcap = nullptr;
+  void* d_rtape = nullptr;
   size_t rtape_bytes = 0;
+  int rad_cap = -1;
   double ms_grad[7] = {0};  // kernel ms of the last ysx_model_vjp
   // ysx_misfit_grad: the observed traces and the residual's per-block sums
   double *d_obs = nullptr, *d_rpart = nullptr;
@@ -177,10 +181,12 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
   dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
-  dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_ack);
+  dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_tpart); dfree(c->d_ack);
   dfree(c->d_atape); dfree(c->d_acks); dfree(c->d_atsc); dfree(c->d_aerr); dfree(c->d_aq);
-  dfree(c->d_rtape); dfree(c->d_rtoff); dfree(c->d_obs); dfree(c->d_rpart);
+  dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
+  dfree(c->d_obs); dfree(c->d_rpart);
   c->obs_n = 0;
+  c->rad_cap = -1;
   c->adj_blocks = c->adj_slots = c->adj_tmax = 0;
   c->rtape_bytes = 0;
   c->have_specb = c->have_recb = false;
@@ -573,7 +579,8 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
     if (e1 == hipSuccess) e1 = hipMemcpy(d_lay, lo.data(), nl * sizeof(int), hipMemcpyHostToDevice);
     if (e1 == hipSuccess)
       e1 = hipMemcpy(d_lay + nl, n.data(), nl * sizeof(int), hipMemcpyHostToDevice);
-    if (e1 == hipSuccess) e1 = hipMemset(ctx->d_tab, 0, ctx->lds_bytes);
+    // stream-ordered: the context streams do not wait for the null stream
+    if (e1 == hipSuccess) e1 = hipMemsetAsync(ctx->d_tab, 0, ctx->lds_bytes, ctx->stream);
     if (e1 == hipSuccess) {
       launch_model(dim3(1), dim3(256), 0, ctx->stream, d_deck, nknot, M.ifanis,
                          ctx->N.vel_norm, d_lay, d_lay + nl, nl, ctx->d_tab, d_work, ws,
@@ -757,10 +764,10 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipMalloc(&ctx->d_runs, nev * sizeof(RunK)));
   HIPCHK(hipMemcpy(ctx->d_runs, ctx->rk.data(), nev * sizeof(RunK), hipMemcpyHostToDevice));
   HIPCHK(hipMalloc(&ctx->d_rec, P * REC * sizeof(cd)));
-  HIPCHK(hipMemset(ctx->d_rec, 0, P * REC * sizeof(cd)));
+  HIPCHK(hipMemsetAsync(ctx->d_rec, 0, P * REC * sizeof(cd), ctx->stream));
   HIPCHK(hipMalloc(&ctx->d_starts, P * 2 * sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_cnt, P * 5 * sizeof(int)));
-  HIPCHK(hipMemset(ctx->d_cnt, 0, P * 5 * sizeof(int)));
+  HIPCHK(hipMemsetAsync(ctx->d_cnt, 0, P * 5 * sizeof(int), ctx->stream));
   HIPCHK(hipMalloc(&ctx->d_bst, P * 56 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_mfin, P * 14 * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_ms1, P * 14 * sizeof(cd)));
@@ -777,7 +784,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   // GEMM operands: coefficients (zero where no pair) and the Legendre tables
   // transposed to [X*3 + |m|][lpad][nrpad]
   HIPCHK(hipMalloc(&ctx->d_cf, cfo * sizeof(cd)));
-  HIPCHK(hipMemset(ctx->d_cf, 0, cfo * sizeof(cd)));
+  HIPCHK(hipMemsetAsync(ctx->d_cf, 0, cfo * sizeof(cd), ctx->stream));
   {
     // receiver geometry from the host (O(nr)); the Legendre tables from the
     // device, written straight into the GEMM operand
@@ -796,7 +803,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     }
     double* d_th = nullptr;
     HIPCHK(hipMalloc(&ctx->d_tb, tbo * sizeof(double)));
-    HIPCHK(hipMemset(ctx->d_tb, 0, tbo * sizeof(double)));
+    HIPCHK(hipMemsetAsync(ctx->d_tb, 0, tbo * sizeof(double), ctx->stream));
     HIPCHK(hipMemcpy(ctx->d_azep, az.data(), ro * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_eia, eia.data(), ro * sizeof(cd), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&d_th, th.size() * sizeof(double)));
@@ -1516,11 +1523,12 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     HIPCHK(hipMalloc(&ctx->d_m0bar, P * 14 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_y0bar, P * 2 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_tbar, (size_t)F_COUNT * nk * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->d_tpart, (size_t)F_COUNT * nk * nb * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->d_ack, nb * ns * 14 * 64 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_acks, nb * ns * 64 * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->d_atape, nb * nt * 6 * 14 * 64 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_atsc, nb * nt * 64 * sizeof(int)));
-    HIPCHK(hipMalloc(&ctx->d_aerr, sizeof(int)));
+    HIPCHK(hipMalloc(&ctx->d_aerr, 16 * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->d_aq, 4 * sizeof(int)));
   }
   hipStream_t st = ctx->stream;
@@ -1529,7 +1537,8 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   const size_t ntab = (size_t)F_COUNT * nk;
   launch_fill0(dim3((unsigned)std::min<size_t>(1024, (ntab + 255) / 256)), b256, 0, st,
                ctx->d_tbar, ntab);
-  launch_reset(dim3(1), b64, 0, st, ctx->d_aerr, 1);
+  launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart, ntab * ctx->adj_blocks);
+  launch_reset(dim3(1), b64, 0, st, ctx->d_aerr, 16);
   launch_reset(dim3(1), b64, 0, st, ctx->d_aq, 4);
   HIPCHK(hipGetLastError());
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
@@ -1545,15 +1554,18 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   const dim3 gb(ctx->adj_blocks);
   HIPCHK(hipEventRecord(ev[0], st));
   launch_brow_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_recb,
-                  ctx->d_mub, AW, ctx->d_tbar);
+                  ctx->d_mub, AW, ctx->d_tpart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[1], st));
   launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
-                   ctx->d_mub, ctx->d_m0bar, AW, ctx->d_tbar);
+                   ctx->d_mub, ctx->d_m0bar, AW, ctx->d_tpart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[2], st));
   launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
-                 ctx->d_y0bar, AW, ctx->d_tbar);
+                 ctx->d_y0bar, AW, ctx->d_tpart);
+  HIPCHK(hipGetLastError());
+  launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart, ctx->adj_blocks,
+                  ntab, ctx->d_tbar);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[3], st));
   {
@@ -1565,38 +1577,65 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   }
   HIPCHK(hipEventRecord(ev[4], st));
   if (k.lmin == 0) {
-    // the radial solves: a tape of every step per frequency, at offsets
-    // from the forward's step counts
-    if (!ctx->d_rtoff) {
-      std::vector<int> c(5 * (size_t)k.nw);
-      HIPCHK(hipMemcpy(c.data(), ctx->d_cnt, c.size() * sizeof(int), hipMemcpyDeviceToHost));
-      std::vector<long long> off(k.nw + 1, 0);
-      for (int j = 0; j < k.nw; j++) off[j + 1] = off[j] + c[5 * (size_t)j + 4];
-      HIPCHK(hipMalloc(&ctx->d_rtoff, off.size() * sizeof(long long)));
-      HIPCHK(hipMemcpy(ctx->d_rtoff, off.data(), off.size() * sizeof(long long),
-                       hipMemcpyHostToDevice));
-      ctx->rtape_bytes = std::max<size_t>(1, (size_t)off[k.nw]) * rad_step_bytes();
-      HIPCHK(hipMalloc(&ctx->d_rtape, ctx->rtape_bytes));
+    // the radial solves, by segments (sweep, knot interval)
+    const Tab HT{M.tab.data(), nk};
+    RadSegs G;
+    G.isrc = rad_src_interval(HT, nk, k.rs);
+    G.itop = rad_top_interval(HT, nk);
+    G.nseg = G.isrc + (G.itop - G.isrc + 1);
+    G.nwb = (k.nw + 63) / 64;
+    if (G.isrc < 1 || G.itop < G.isrc || HT(F_R, 1) == HT(F_R, 2))
+      return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_model_vjp: radial grid without a source "
+                                               "interval or a first interval");
+    const size_t ns = (size_t)G.nseg * k.nw;
+    if (!ctx->d_rprop) {
+      HIPCHK(hipMalloc(&ctx->d_rprop, ns * 4 * sizeof(cd)));
+      HIPCHK(hipMalloc(&ctx->d_rys, ns * 2 * sizeof(cd)));
+      HIPCHK(hipMalloc(&ctx->d_rxe, ns * 2 * sizeof(cd)));
+      HIPCHK(hipMalloc(&ctx->d_rcap, sizeof(int)));
     }
-    launch_rad_adj(dim3((unsigned)((k.nw + 63) / 64)), b64, 0, st, ctx->d_tab, ctx->d_runs,
-                   ctx->d_recb, ctx->d_rtoff, ctx->d_rtape, ctx->d_aerr, ctx->d_tbar);
+    const dim3 gs(G.nseg, G.nwb);
+    launch_reset(dim3(1), b64, 0, st, ctx->d_rcap, 1);
+    G.cap = 0;
+    launch_rad_prop(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, G, ctx->d_rprop, ctx->d_rcap);
+    launch_rad_scan(dim3(G.nwb), b64, 0, st, ctx->d_tab, ctx->d_runs, G, ctx->d_rprop,
+                    ctx->d_recb, ctx->d_rys, ctx->d_rxe, ctx->d_tbar);
+    HIPCHK(hipGetLastError());
+    if (ctx->rad_cap < 0) {  // the tape's length: the most steps of a segment (once per run)
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipMemcpy(&ctx->rad_cap, ctx->d_rcap, sizeof(int), hipMemcpyDeviceToHost));
+      const size_t bytes =
+          std::max<size_t>(1, (size_t)G.nseg * G.nwb * ctx->rad_cap * 64) * rad_step_bytes();
+      if (bytes > ctx->rtape_bytes) {
+        dfree(ctx->d_rtape);
+        HIPCHK(hipMalloc(&ctx->d_rtape, bytes));
+        ctx->rtape_bytes = bytes;
+      }
+    }
+    G.cap = ctx->rad_cap;
+    launch_rad_seg_adj(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, G, ctx->d_rys, ctx->d_rxe,
+                       ctx->d_rtape, ctx->d_aerr, ctx->d_tbar);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipEventSynchronize(ev[5]));
   for (int q = 0; q < 5; q++)
     if (int rc = elapsed(ctx, q, q + 1, ctx->ms_grad[q])) return rc;
-  int err = 0;
-  HIPCHK(hipMemcpy(&err, ctx->d_aerr, sizeof(int), hipMemcpyDeviceToHost));
-  if (err & 1)
-    return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a knot interval with more steps than the "
-                                     "sweep's tape (" + std::to_string(ctx->adj_tmax) + ")");
+  int errw[16];
+  HIPCHK(hipMemcpy(errw, ctx->d_aerr, sizeof errw, hipMemcpyDeviceToHost));
+  const int err = errw[0];
+  if (err & 1) {
+    double info[5];
+    std::memcpy(info, errw + 2, sizeof info);
+    char msg[256];
+    std::snprintf(msg, sizeof msg, "ysx_model_vjp: a knot interval with more steps than the "
+                  "sweep's tape (%d): interval %d, %d steps from %.17g to %.17g at w %.17g",
+                  ctx->adj_tmax, errw[1], (int)info[0], info[1], info[2], info[3]);
+    return set_err(ctx, YSX_ERR_HIP, msg);
+  }
   if (err & 2)
-    return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a radial solve with more steps than its "
-                                     "forward");
-  if (err & 4)
-    return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_model_vjp: radial solve with the receiver "
-                                             "below the source point");
+    return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a radial segment with more steps than "
+                                     "its tape");
   std::vector<double> tb(ntab);
   HIPCHK(hipMemcpy(tb.data(), ctx->d_tbar, ntab * sizeof(double), hipMemcpyDeviceToHost));
   // the source vectors' part (ysx_source_model_vjp)
@@ -1633,6 +1672,9 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
   launch_residual(dim3(nb), dim3(256), 0, ctx->stream, ctx->d_seis, ctx->d_obs, R.nt, nout,
                   ntrace, ctx->d_seisb, ctx->d_rpart);
   HIPCHK(hipGetLastError());
+  // the context's streams do not block on the null stream: wait for the
+  // residual before copying its sums
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   std::vector<double> part(nb);
   HIPCHK(hipMemcpy(part.data(), ctx->d_rpart, nb * sizeof(double), hipMemcpyDeviceToHost));
   double sum = 0.0;

@@ -28,8 +28,9 @@
 //   k_tor_adj    toroidal system, the same way
 //   k_start_adj  starting solutions: forward-mode tangents with respect to
 //                the spline values at the start radius
-//   k_rad_adj    the radial (l = 0) solves, one lane per frequency: every
-//                step of both sweeps on a per-lane tape, then reversed
+//   k_rad_prop, k_rad_scan, k_rad_seg_adj  the radial (l = 0) solves, cut
+//                into per-interval segments (2 x 2 propagators), each
+//                recomputed and reversed by its own lane
 #include "ysx_adjoint.hpp"
 #include "ysx_kern.hpp"
 #include "ysx_radial.hpp"
@@ -131,20 +132,32 @@ __device__ inline void load_bases(const Tab& T, const WS& w, int i, double ra, d
   }
 }
 
-// dst[q] += sum over the wave's lanes of v[q], q < NOB
+// lanes summing one value in wave_sum_into: the largest power of two P
+// with NOB P <= 64
+template <int NOB>
+constexpr int sum_lanes() {
+  int p = 1;
+  while (2 * p * NOB <= 64) p *= 2;
+  return p;
+}
+
+// dst[q] += sum over the wave's lanes of v[q], q < NOB: through LDS (red,
+// NOB x 64), P lanes per value, then P-lane shuffles
 template <int NOB>
 __device__ inline void wave_sum_into(const double* v, double* red, double* dst) {
+  constexpr int P = sum_lanes<NOB>(), LEN = 64 / P;
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < NOB; q++) red[q * 64 + t] = v[q];
   __syncthreads();
-  if (t < 2 * NOB) {
-    const double* r = red + (t >> 1) * 64 + (t & 1) * 32;
+  if (t < NOB * P) {
+    const double* r = red + (t / P) * 64 + (t % P) * LEN;
     double s = 0.0;
 #pragma unroll 8
-    for (int k = 0; k < 32; k++) s += r[k];
-    s += __shfl_xor(s, 1, 64);
-    if ((t & 1) == 0) dst[t >> 1] += s;
+    for (int k = 0; k < LEN; k++) s += r[k];
+#pragma unroll
+    for (int o = 1; o < P; o <<= 1) s += __shfl_xor(s, o, 64);
+    if (t % P == 0) dst[t / P] += s;
   }
   __syncthreads();
 }
@@ -186,18 +199,39 @@ __device__ inline void chain_chunk(const Tab& T, const WS& w, int i, double ra, 
   }
 }
 
-// S.tacc of interval i into the table adjoint (field-major, nk knots)
+// S.tacc of interval i into the block's own column of the partial table
+// adjoints, tpart[(f nk + knot - 1) gridDim.x + blockIdx.x] (plain
+// read-modify-write: no other block writes that column; k_part_sum adds
+// the columns into the table adjoint afterwards)
 template <class Sys>
-__device__ inline void flush_interval(int i, int nk, AdjLds& S, double* tbar) {
+__device__ inline void flush_interval(int i, int nk, AdjLds& S, double* tpart) {
   __syncthreads();
   int t = threadIdx.x;
   if (t < Sys::NV * 4) {
     int v = t / 4, f = Sys::field(v) + ((t / 2) % 2), knot = i + (t % 2);
     double a = S.tacc[t];
-    if (a != 0.0) atomicAdd(&tbar[(size_t)f * nk + knot - 1], a);
+    if (a != 0.0) {
+      double* e = tpart + ((size_t)f * nk + knot - 1) * gridDim.x + blockIdx.x;
+      *e += a;
+    }
     S.tacc[t] = 0.0;
   }
   __syncthreads();
+}
+
+// tbar[e] += sum over the nb block columns of tpart[e nb ..], one wave per
+// entry e < n
+__global__ void __launch_bounds__(256) k_part_sum(const double* __restrict__ tpart, int nb,
+                                                  size_t n, double* __restrict__ tbar) {
+  const size_t e = (size_t)blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (e >= n) return;
+  const double* p = tpart + e * nb;
+  double a = 0.0;
+  for (int b = lane; b < nb; b += 64) a += p[b];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) a += __shfl_xor(a, o, 64);
+  if (lane == 0) tbar[e] += a;
 }
 
 __device__ inline cd* ck_of(const AdjWork& W, int slot) {
@@ -220,6 +254,19 @@ __device__ inline void put_ck(const AdjWork& W, int slot, const cd* y, int sc) {
   *cks_of(W, slot) = sc;
 }
 
+// the tape overflow: flag it and describe the first one
+__device__ inline void tape_overflow(const AdjWork& W, int i, int ns, double ra, double rb,
+                                     double rew) {
+  if (threadIdx.x == 0 && !(atomicOr(W.err, 1) & 1)) {
+    W.err[1] = i;
+    double* d = reinterpret_cast<double*>(W.err + 2);
+    d[0] = ns;
+    d[1] = ra;
+    d[2] = rb;
+    d[3] = rew;
+  }
+}
+
 __device__ inline int steps_of(const Tab& T, const WS& w, int i, int nk, double ra, double rb,
                                double& drr) {
   double drr0 = step_len_w(T, w.rew, i, nk);
@@ -234,7 +281,7 @@ __device__ inline int steps_of(const Tab& T, const WS& w, int i, int nk, double 
 // input on the tape; then the row x (counter sx) is taken back through the
 // steps, last first, each step's stages paired with the tape.
 template <class Sys>
-__device__ void rev_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
+__device__ __forceinline__ void rev_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
                              double rb, cd* x, int& sx, const AdjWork& W, int slot, AdjLds& S,
                              double* tbar) {
   using F = typename Sys::F;
@@ -242,7 +289,7 @@ __device__ void rev_interval(const Tab& T, const WS& w, const LS& L, int nk, int
   double drr;
   const int ns = steps_of(T, w, i, nk, ra, rb, drr);
   if (ns > W.tmax) {
-    if (threadIdx.x == 0) atomicOr(W.err, 1);
+    tape_overflow(W, i, ns, ra, rb, w.rew);
     return;
   }
   {
@@ -300,7 +347,7 @@ __device__ void rev_interval(const Tab& T, const WS& w, const LS& L, int nk, int
 // (counter sc) is stepped through the interval, each stage paired with the
 // tape.
 template <class Sys>
-__device__ void fwd_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
+__device__ __forceinline__ void fwd_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
                              double rb, cd* c, int& sc, const AdjWork& W, int slot, AdjLds& S,
                              double* tbar) {
   using F = typename Sys::F;
@@ -308,7 +355,7 @@ __device__ void fwd_interval(const Tab& T, const WS& w, const LS& L, int nk, int
   double drr;
   const int ns = steps_of(T, w, i, nk, ra, rb, drr);
   if (ns > W.tmax) {
-    if (threadIdx.x == 0) atomicOr(W.err, 1);
+    tape_overflow(W, i, ns, ra, rb, w.rew);
     return;
   }
   const int jlast = 1 + ((ns - 1) / CH) * CH;
@@ -794,183 +841,204 @@ __global__ void __launch_bounds__(256) k_start_adj(const double* __restrict__ gt
   }
 }
 
-// ----------------------------------------------------------- k_rad_adj
+// ------------------------------------------------------- radial adjoint
 // rad_solver_wg (src/module_solver.f90:446-561), receiver at the top grid
 // point: yrad(q, c) = -y2r_q b_c / D, y2r = (1, 0), b = (y1_2, -y1_1),
 // D = y2_2 y1_1 - y2_1 y1_2 with y1 the upward and y2 the downward solution
-// at the source point.  One lane per frequency (the grid depends on w): the
-// forward's steps (rad_sweeps) are recorded with the state before each,
-// then the rows on y2 and y1 are taken back through them; the table-entry
-// adjoints of the current knot interval are kept per lane and added at
-// each change of interval.  No renormalisation on this solve.
-struct RadStep {
-  double x, h;
-  cd y[2];
-  int il, pad;
-};
+// at the source point.  The solve is linear and 2 x 2 with no
+// renormalisation, so it is cut into its per-interval segments
+// (rad_segment), each a 2 x 2 propagator:
+//   k_rad_prop     every (segment, frequency): the segment's propagator;
+//   k_rad_scan     per frequency: the states at the segment starts, the
+//                  rows of the record adjoints at the source point, the
+//                  rows at the segment ends, and the start solution's part;
+//   k_rad_seg_adj  every (segment, frequency): the segment's steps again
+//                  from its start state (states on a tape), the row from
+//                  its end back through them with every stage paired; per
+//                  wave (64 frequencies, one interval) summed into the
+//                  interval's table entries.
+// Segment q of a frequency, in sweep order: q < isrc upward in interval
+// q + 1, then downward in intervals itop .. isrc.
 
-struct RadAcc {
-  double a[RAD_NV * 4];
-  int il;
-};
-
-__device__ inline void rad_flush(RadAcc& A, int nk, double* tbar) {
-  if (A.il > 0)
-    for (int v = 0; v < RAD_NV; v++)
-      for (int q = 0; q < 4; q++) {
-        double x = A.a[v * 4 + q];
-        if (x != 0.0)
-          atomicAdd(&tbar[(size_t)(rad_field(v) + q / 2) * nk + A.il - 1 + (q % 2)], x);
-        A.a[v * 4 + q] = 0.0;
-      }
-}
-
-// one step of the forward's rk_step (tableau of rkck, src/module_int.f90:
-// 2016-2064, unscaled derivatives, h applied in the combinations), taken
-// back: x <- x S, with the stage pairings into A
-__device__ inline void rad_step_adj(const Tab& T, const PS& s, const RadStep& st, cd* x,
-                                    RadAcc& A, int nk, double* tbar) {
-  using namespace ck;
-  const double h = st.h;
-  const double off[6] = {0.0, 0.2, 0.3, 0.6, 1.0, 0.875};
-  RadRHS f;
-  RadRHS::Coef c[6];
-  for (int k = 0; k < 6; k++) c[k] = f.coef(T, s, st.il, st.x + off[k] * h);
-  // forward stages: inputs z and derivatives d
-  cd z[6][2], d[6][2];
-  const cd* y = st.y;
-  auto stage = [&](int k) { RadRHS::apply(c[k], z[k], d[k]); };
-  for (int i = 0; i < 2; i++) z[0][i] = y[i];
-  stage(0);
-  for (int i = 0; i < 2; i++) z[1][i] = y[i] + (b21 * h) * d[0][i];
-  stage(1);
-  for (int i = 0; i < 2; i++) z[2][i] = y[i] + h * (b31 * d[0][i] + b32 * d[1][i]);
-  stage(2);
-  for (int i = 0; i < 2; i++)
-    z[3][i] = y[i] + h * (b41 * d[0][i] + b42 * d[1][i] + b43 * d[2][i]);
-  stage(3);
-  for (int i = 0; i < 2; i++)
-    z[4][i] = y[i] + h * (b51 * d[0][i] + b52 * d[1][i] + b53 * d[2][i] + b54 * d[3][i]);
-  stage(4);
-  for (int i = 0; i < 2; i++)
-    z[5][i] = y[i] + h * (b61 * d[0][i] + b62 * d[1][i] + b63 * d[2][i] + b64 * d[3][i] +
-                          b65 * d[4][i]);
-  // transposed stages: kb_k = h (c_k x + sum_{m>k} b_mk zb_m), zb_k = A_k^T kb_k
-  const double cc[6] = {c1, 0.0, c3, c4, 0.0, c6};
-  const double bb[6][6] = {{0, 0, 0, 0, 0, 0},
-                           {b21, 0, 0, 0, 0, 0},
-                           {b31, b32, 0, 0, 0, 0},
-                           {b41, b42, b43, 0, 0, 0},
-                           {b51, b52, b53, b54, 0, 0},
-                           {b61, b62, b63, b64, b65, 0}};
-  cd zb[6][2];
-  cd xn[2] = {x[0], x[1]};
-  for (int k = 5; k >= 0; k--) {
-    cd kb[2];
-    for (int i = 0; i < 2; i++) {
-      cd v = cc[k] * x[i];
-      for (int m = k + 1; m < 6; m++) v = v + bb[m][k] * zb[m][i];
-      kb[i] = h * v;
-    }
-    // zb = A^T kb for A = [[t, k], [sv, -t]]
-    zb[k][0] = c[k].t * kb[0] + c[k].sv * kb[1];
-    zb[k][1] = c[k].k * kb[0] - c[k].t * kb[1];
-    xn[0] = xn[0] + zb[k][0];
-    xn[1] = xn[1] + zb[k][1];
-    // pairing: the entries' adjoints and the chain to the table
-    cd tb = kb[0] * z[k][0] - kb[1] * z[k][1], kbar = kb[0] * z[k][1], svb = kb[1] * z[k][0];
-    double vb[RAD_NV];
-    const double r = st.x + off[k] * h;
-    rad_coef_adj(T, s, st.il, r, tb, kbar, svb, vb);
-    if (st.il != A.il) {
-      rad_flush(A, nk, tbar);
-      A.il = st.il;
-    }
-    SplW sw = spl_w(T, st.il, r);
-    for (int v = 0; v < RAD_NV; v++) {
-      const int f0 = rad_field(v);
-      spl_adj(sw, f0, vb[v], [&](int ff, int knot, double val) {
-        A.a[v * 4 + (ff == f0 ? 0 : 2) + (knot - st.il)] += val;
-      });
-    }
+__device__ inline void seg_of(const RadSegs& G, int q, int& sweep, int& i) {
+  if (q < G.isrc) {
+    sweep = 0;
+    i = q + 1;
+  } else {
+    sweep = 1;
+    i = G.itop - (q - G.isrc);
   }
-  x[0] = xn[0];
-  x[1] = xn[1];
 }
 
-__global__ void __launch_bounds__(64) k_rad_adj(const double* __restrict__ gtab,
-                                                const RunK* __restrict__ runs,
-                                                const cd* __restrict__ recb,
-                                                const long long* __restrict__ toff,
-                                                RadStep* __restrict__ tape, int* __restrict__ err,
-                                                double* __restrict__ tbar) {
+__device__ inline double rad_dr(const RunK& R, const PS& s) {
+  return 1.0 / (4.0 * s.rew / R.alpha_min);
+}
+
+// propagators, column-major: P[2 c + r] (row r, column c)
+__global__ void __launch_bounds__(64) k_rad_prop(const double* __restrict__ gtab,
+                                                 const RunK* __restrict__ runs, RadSegs G,
+                                                 cd* __restrict__ prop, int* __restrict__ cap) {
   const RunK& R = runs[0];
-  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+  const int q = blockIdx.x, j = (int)(blockIdx.y * 64 + threadIdx.x) + 1;
+  if (j > R.nw) return;
+  const Tab T{gtab, R.nknot};
+  PS s;
+  pair_state(R, 0, j, s);
+  int sweep, i;
+  seg_of(G, q, sweep, i);
+  cd Y[2][2] = {{C(1.0), C(0.0)}, {C(0.0), C(1.0)}};
+  const int n = rad_segment(T, R.nknot, s, rad_dr(R, s), R.rs, R.rr, sweep, i, G.isrc, G.itop,
+                            [&](double x, double h) { rad_step_cols<2>(T, s, i, x, h, Y); });
+  cd* P = prop + ((size_t)q * R.nw + (j - 1)) * 4;
+  P[0] = Y[0][0];
+  P[1] = Y[0][1];
+  P[2] = Y[1][0];
+  P[3] = Y[1][1];
+  atomicMax(cap, n);
+}
+
+__device__ inline void prop_apply(const cd* P, cd* y) {
+  const cd a = P[0] * y[0] + P[2] * y[1], b = P[1] * y[0] + P[3] * y[1];
+  y[0] = a;
+  y[1] = b;
+}
+__device__ inline void prop_row(const cd* P, cd* x) {
+  const cd a = x[0] * P[0] + x[1] * P[1], b = x[0] * P[2] + x[1] * P[3];
+  x[0] = a;
+  x[1] = b;
+}
+
+__global__ void __launch_bounds__(64) k_rad_scan(const double* __restrict__ gtab,
+                                                 const RunK* __restrict__ runs, RadSegs G,
+                                                 const cd* __restrict__ prop,
+                                                 const cd* __restrict__ recb,
+                                                 cd* __restrict__ ys, cd* __restrict__ xe,
+                                                 double* __restrict__ tbar) {
+  const RunK& R = runs[0];
+  const int j = (int)(blockIdx.x * 64 + threadIdx.x) + 1;
   if (j > R.nw || R.lmin != 0) return;
   const Tab T{gtab, R.nknot};
   const int nk = R.nknot;
   const size_t p = (size_t)R.pbase + (j - 1);
   PS s;
   pair_state(R, 0, j, s);
-  const double dr = 1.0 / (4.0 * s.rew / R.alpha_min);
-  RadStep* tp = tape + toff[j - 1];
-  const long long cap = toff[j] - toff[j - 1];
-  long long n = 0, nu = 0;
-  int nsr = 0;
-  cd y1[2], y2[2], y1r[2], y2r[2];
-  bool rec_below;
-  auto hook = [&](int sweep, int il, double x, double h, const cd* y) {
-    if (n < cap) {
-      RadStep& t = tp[n];
-      t.x = x;
-      t.h = h;
-      t.y[0] = y[0];
-      t.y[1] = y[1];
-      t.il = il;
-    }
-    n++;
-    if (sweep == 0) nu++;
-  };
-  rad_sweeps(T, nk, s, dr, R.rs, R.rr, NoGroup(), hook, y1, y2, y1r, y2r, rec_below, nsr);
-  if (n > cap || rec_below) {
-    atomicOr(err, n > cap ? 2 : 4);
-    return;
+  auto at = [&](int q) { return (size_t)q * R.nw + (j - 1); };
+  // the start radius: the first step of segment (upward, interval 1)
+  double rt = 0.0;
+  bool got = false;
+  rad_segment(T, nk, s, rad_dr(R, s), R.rs, R.rr, 0, 1, G.isrc, G.itop, [&](double x, double) {
+    if (!got) rt = x;
+    got = true;
+  });
+  cd y[2];
+  rad_ystart(T, s, 1, rt, y);
+  for (int q = 0; q < G.isrc; q++) {
+    ys[2 * at(q)] = y[0];
+    ys[2 * at(q) + 1] = y[1];
+    prop_apply(prop + 4 * at(q), y);
   }
+  const cd y1[2] = {y[0], y[1]};
+  y[0] = C(1.0);
+  y[1] = C(0.0);
+  for (int q = G.isrc; q < G.nseg; q++) {
+    ys[2 * at(q)] = y[0];
+    ys[2 * at(q) + 1] = y[1];
+    prop_apply(prop + 4 * at(q), y);
+  }
+  const cd y2[2] = {y[0], y[1]}, y2r[2] = {C(1.0), C(0.0)};
   // the rows on y1 and y2 at the source point
   cd a[2][2];
   for (int q = 0; q < 2; q++)
     for (int c = 0; c < 2; c++) a[q][c] = conj_(recb[p * REC + O_YRAD + q + 2 * c]);
   const cd Dv = y2[1] * y1[0] - y2[0] * y1[1], Di = cinv(Dv);
   const cd b[2] = {y1[1], -y1[0]};
-  cd G[2], E = C(0.0);
+  cd Gc[2], E = C(0.0);
   for (int c = 0; c < 2; c++) {
-    G[c] = C(0.0);
+    Gc[c] = C(0.0);
     for (int q = 0; q < 2; q++) {
-      G[c] = G[c] - a[q][c] * y2r[q] * Di;
+      Gc[c] = Gc[c] - a[q][c] * y2r[q] * Di;
       E = E + a[q][c] * y2r[q] * b[c] * Di * Di;
     }
   }
-  cd r1[2] = {-G[1] + E * y2[1], G[0] - E * y2[0]};
-  cd r2[2] = {-E * y1[1], E * y1[0]};
-  RadAcc A;
-  for (int q = 0; q < RAD_NV * 4; q++) A.a[q] = 0.0;
-  A.il = 0;
-  for (long long k = n - 1; k >= nu; k--) rad_step_adj(T, s, tp[k], r2, A, nk, tbar);
-  for (long long k = nu - 1; k >= 0; k--) rad_step_adj(T, s, tp[k], r1, A, nk, tbar);
-  rad_flush(A, nk, tbar);
-  // the starting solution at rt (the first upward step's radius), interval 1
-  if (nu > 0) {
-    const double rt = tp[0].x;
-    DS y[2];
-    rad_ystart_d(T, s, 1, rt, y);
-    SplW sw = spl_w(T, 1, rt);
-    for (int v = 0; v < SV_N; v++) {
-      double vb = cmul_(r1[0], y[0].d[v]).x + cmul_(r1[1], y[1].d[v]).x;
-      const int f0 = start_field(v);
-      spl_adj(sw, f0, vb, [&](int ff, int knot, double val) {
-        atomicAdd(&tbar[(size_t)ff * nk + knot - 1], val);
-      });
+  cd x[2] = {-E * y1[1], E * y1[0]};  // on y2
+  for (int q = G.nseg - 1; q >= G.isrc; q--) {
+    xe[2 * at(q)] = x[0];
+    xe[2 * at(q) + 1] = x[1];
+    prop_row(prop + 4 * at(q), x);
+  }
+  x[0] = -Gc[1] + E * y2[1];  // on y1
+  x[1] = Gc[0] - E * y2[0];
+  for (int q = G.isrc - 1; q >= 0; q--) {
+    xe[2 * at(q)] = x[0];
+    xe[2 * at(q) + 1] = x[1];
+    prop_row(prop + 4 * at(q), x);
+  }
+  // the starting solution at rt, interval 1
+  DS yd[2];
+  rad_ystart_d(T, s, 1, rt, yd);
+  const SplW sw = spl_w(T, 1, rt);
+  for (int v = 0; v < SV_N; v++) {
+    const double vb = cmul_(x[0], yd[0].d[v]).x + cmul_(x[1], yd[1].d[v]).x;
+    const int f0 = start_field(v);
+    spl_adj(sw, f0, vb, [&](int ff, int knot, double val) {
+      atomicAdd(&tbar[(size_t)ff * nk + knot - 1], val);
+    });
+  }
+}
+
+__global__ void __launch_bounds__(64) k_rad_seg_adj(const double* __restrict__ gtab,
+                                                    const RunK* __restrict__ runs, RadSegs G,
+                                                    const cd* __restrict__ ys,
+                                                    const cd* __restrict__ xe,
+                                                    RadSt* __restrict__ tape,
+                                                    int* __restrict__ err,
+                                                    double* __restrict__ tbar) {
+  __shared__ double red[RAD_NV * 4 * 64];
+  const RunK& R = runs[0];
+  const int q = blockIdx.x, lane = threadIdx.x, j = (int)(blockIdx.y * 64 + lane) + 1;
+  const Tab T{gtab, R.nknot};
+  int sweep, i;
+  seg_of(G, q, sweep, i);
+  double acc[RAD_NV * 4];
+#pragma unroll
+  for (int v = 0; v < RAD_NV * 4; v++) acc[v] = 0.0;
+  if (j <= R.nw) {
+    PS s;
+    pair_state(R, 0, j, s);
+    const size_t at = (size_t)q * R.nw + (j - 1);
+    cd Y[1][2] = {{ys[2 * at], ys[2 * at + 1]}};
+    cd x[2] = {xe[2 * at], xe[2 * at + 1]};
+    RadSt* tp = tape + ((size_t)q * G.nwb + blockIdx.y) * (size_t)G.cap * 64 + lane;
+    int n = 0;
+    rad_segment(T, R.nknot, s, rad_dr(R, s), R.rs, R.rr, sweep, i, G.isrc, G.itop,
+                [&](double xr, double h) {
+                  if (n < G.cap) {
+                    RadSt& t = tp[(size_t)n * 64];
+                    t.x = xr;
+                    t.h = h;
+                    t.y[0] = Y[0][0];
+                    t.y[1] = Y[0][1];
+                  }
+                  n++;
+                  rad_step_cols<1>(T, s, i, xr, h, Y);
+                });
+    if (n > G.cap) {
+      atomicOr(err, 2);
+    } else {
+      for (int m = n - 1; m >= 0; m--) {
+        const RadSt t = tp[(size_t)m * 64];
+        rad_step_back(T, s, i, t.x, t.h, t.y, x, acc);
+      }
     }
+  }
+#pragma unroll
+  for (int v = 0; v < RAD_NV * 4; v++) red[v * 64 + lane] = acc[v];
+  __syncthreads();
+  if (lane < RAD_NV * 4) {
+    double sum = 0.0;
+    for (int k = 0; k < 64; k++) sum += red[lane * 64 + k];
+    const int f = rad_field(lane / 4) + (lane % 4) / 2;
+    if (sum != 0.0) atomicAdd(&tbar[(size_t)f * R.nknot + i - 1 + (lane % 2)], sum);
   }
 }
 
@@ -998,12 +1066,25 @@ void launch_start_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* g
                       const cd* m0bar, const cd* y0bar, double* tbar) {
   hipLaunchKernelGGL(k_start_adj, g, b, lds, s, gtab, runs, PR, SD, starts, m0bar, y0bar, tbar);
 }
-size_t rad_step_bytes() { return sizeof(RadStep); }
-void launch_rad_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                    const RunK* runs, const cd* recb, const long long* toff, void* tape, int* err,
-                    double* tbar) {
-  hipLaunchKernelGGL(k_rad_adj, g, b, lds, s, gtab, runs, recb, toff,
-                     static_cast<RadStep*>(tape), err, tbar);
+void launch_part_sum(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* tpart, int nb,
+                     size_t n, double* tbar) {
+  hipLaunchKernelGGL(k_part_sum, g, b, lds, s, tpart, nb, n, tbar);
+}
+size_t rad_step_bytes() { return sizeof(RadSt); }
+void launch_rad_prop(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, RadSegs G, cd* prop, int* cap) {
+  hipLaunchKernelGGL(k_rad_prop, g, b, lds, s, gtab, runs, G, prop, cap);
+}
+void launch_rad_scan(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, RadSegs G, const cd* prop, const cd* recb, cd* ys, cd* xe,
+                     double* tbar) {
+  hipLaunchKernelGGL(k_rad_scan, g, b, lds, s, gtab, runs, G, prop, recb, ys, xe, tbar);
+}
+void launch_rad_seg_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                        const RunK* runs, RadSegs G, const cd* ys, const cd* xe, void* tape,
+                        int* err, double* tbar) {
+  hipLaunchKernelGGL(k_rad_seg_adj, g, b, lds, s, gtab, runs, G, ys, xe,
+                     static_cast<RadSt*>(tape), err, tbar);
 }
 
 }  // namespace ysx

@@ -187,7 +187,10 @@ struct AdjWork {
   cd* tape;
   int* tsc;
   int slots, tmax;
-  int* err;  // bit 0: an interval with more steps than the tape holds
+  // err[0] bit 0: an interval with more steps than the tape holds; the
+  // first such interval is described in err[1] (interval) and the doubles
+  // at err + 2: steps, its two radii, Re w
+  int* err;
 };
 
 // ------------------------------------------------------- march helpers
@@ -314,7 +317,13 @@ void launch_source_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gt
                      const RunK* runs, int e, const cd* recb, double* part);
 void launch_source_model_T(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                            const RunK* runs, int e, const cd* recb, double* part);
-// adjoint sweeps (ysx_k_adjoint.hip): persistent blocks over the queue Q
+// adjoint sweeps (ysx_k_adjoint.hip): persistent blocks over the queue Q,
+// each adding into its own column of the partial table adjoints tbar
+// ((F_COUNT nknot) x gridDim.x, block fastest); launch_part_sum (4 entries
+// per 256-thread block) adds the nb columns of n entries into the table
+// adjoint
+void launch_part_sum(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* tpart, int nb,
+                     size_t n, double* tbar);
 void launch_brow_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                      const RunK* runs, Queue Q, SphState SS, const cd* rec, const cd* recb,
                      cd* mub, AdjWork W, double* tbar);
@@ -327,12 +336,30 @@ void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gta
 void launch_start_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                       const RunK* runs, PairRange PR, StartData SD, const int* starts,
                       const cd* m0bar, const cd* y0bar, double* tbar);
-// radial adjoint: per-lane step tapes at offsets toff[j - 1] .. toff[j]
-// (records of rad_step_bytes() bytes) for the frequencies j of event 0
+// radial adjoint of event 0, cut into segments (sweep, knot interval):
+// isrc upward segments (intervals 1 .. isrc), then downward ones (itop ..
+// isrc); nwb blocks of 64 frequencies; cap: the most steps of a segment
+// (k_rad_prop's atomic maximum), the tape's length per lane
+struct RadSegs {
+  int isrc, itop, nseg, nwb, cap;
+};
+struct RadSt {  // a step of the tape: radius, step, state before it
+  double x, h;
+  cd y[2];
+};
 size_t rad_step_bytes();
-void launch_rad_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                    const RunK* runs, const cd* recb, const long long* toff, void* tape, int* err,
-                    double* tbar);
+// grid (nseg, nwb) x 64: propagators prop[(q nw + j - 1) 4 ..], cap
+void launch_rad_prop(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, RadSegs G, cd* prop, int* cap);
+// grid nwb x 64: states ys and rows xe at the segment ends, start part
+void launch_rad_scan(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                     const RunK* runs, RadSegs G, const cd* prop, const cd* recb, cd* ys, cd* xe,
+                     double* tbar);
+// grid (nseg, nwb) x 64: the segments reversed, tape of nseg nwb cap 64
+// steps
+void launch_rad_seg_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                        const RunK* runs, RadSegs G, const cd* ys, const cd* xe, void* tape,
+                        int* err, double* tbar);
 #define YSX_MARCH_LAUNCHERS(SUFFIX)                                                              \
   void launch_tor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
                                  const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt);   \

@@ -60,7 +60,7 @@ __host__ __device__ inline void grid_runs(const Tab& T, int i, double drr, doubl
 // else y2r).  hook(sweep, interval, radius, step, y) is called before every
 // Cash-Karp step (sweep 0: upward, 1: downward); nsr counts the steps.
 template <class G, class Hook>
-__device__ inline void rad_sweeps(const Tab& T, int nk, const PS& s, double dr, double rs,
+__host__ __device__ inline void rad_sweeps(const Tab& T, int nk, const PS& s, double dr, double rs,
                                   double rr, const G& grp, const Hook& hook, cd* y1, cd* y2,
                                   cd* y1r, cd* y2r, bool& rec_below, int& nsr) {
   RadRHS f;
@@ -136,9 +136,94 @@ __device__ inline void rad_sweeps(const Tab& T, int nk, const PS& s, double dr, 
   }
 }
 
+// ------------------------------------------------ per-interval segments
+// The steps of rad_sweeps split into independent segments, one per (sweep,
+// knot interval): segment (sweep, i) holds the steps the sweep takes inside
+// interval i, in its order, and can be enumerated without the others.
+// Consecutive grid points of an interval are joined by yint's equal steps
+// in either direction; where a sweep passes into the next interval the
+// join runs from the last point of one interval to the knot radius that
+// starts the next, and yint takes it only in the sweep's own direction:
+// upward as a (rounding-length) step in the interval left, downward in the
+// interval entered.
+
+// knot interval of the source point (grid_runs' rule r1 < rs <= r2) and the
+// topmost interval of nonzero thickness (0: none)
+__host__ __device__ inline int rad_src_interval(const Tab& T, int nk, double rs) {
+  for (int i = 1; i <= nk - 1; i++) {
+    const double r1 = T(F_R, i), r2 = T(F_R, i + 1);
+    if (r1 != r2 && rs > r1 && rs <= r2) return i;
+  }
+  return 0;
+}
+__host__ __device__ inline int rad_top_interval(const Tab& T, int nk) {
+  for (int i = nk - 1; i >= 1; i--)
+    if (T(F_R, i) != T(F_R, i + 1)) return i;
+  return 0;
+}
+
+// yint's equal steps between radii p0 and p1 of interval i: f(x, h) per step
+template <class Fn>
+__host__ __device__ inline int rad_cell(const Tab& T, const PS& s, int nk, int i, double p0,
+                                        double p1, const Fn& f) {
+  if (p1 == p0) return 0;
+  double drr = step_len(T, s, i, nk);
+  const int n = 2 + (int)floor(fabs(p1 - p0) / drr);
+  drr = (p1 - p0) / (double)(n - 1);
+  for (int j = 1; j <= n - 1; j++) f(p0 + (double)(j - 1) * drr, drr);
+  return n - 1;
+}
+
+// The steps of segment (sweep, i) (sweep 0 upward from the centre to the
+// source point, 1 downward from the top point), f(x, h) per step; returns
+// their number.  isrc = rad_src_interval, itop = rad_top_interval; interval
+// 1 must have nonzero thickness (the upward sweep starts in it).
+template <class Fn>
+__host__ __device__ inline int rad_segment(const Tab& T, int nk, const PS& s, double dr,
+                                           double rs, double rr, int sweep, int i, int isrc,
+                                           int itop, const Fn& f) {
+  Run3 G;
+  grid_runs(T, i, dr, rs, rr, G);
+  if (G.n == 0) return 0;
+  int n = 0;
+  if (sweep == 0) {
+    if (i > isrc) return 0;
+    int k = 0;
+    double prev = 0.0;
+    for (int q = 0; q < G.n; q++)
+      for (int jj = G.j0[q]; jj <= G.j1[q]; jj++) {
+        const double rg = G.base[q] + (double)(jj - 1) * G.step[q];
+        k++;
+        if (k == 2 && i == 1)  // the start radius below the second grid point
+          n += rad_cell(T, s, nk, i, fmin(0.1 * dr, 0.1 * rg), rg, f);
+        else if (k >= 2)
+          n += rad_cell(T, s, nk, i, prev, rg, f);
+        prev = rg;
+        if (jj == G.j1[q] && q == G.src) return n;
+      }
+    if (prev < T(F_R, i + 1)) n += rad_cell(T, s, nk, i, prev, T(F_R, i + 1), f);
+    return n;
+  }
+  if (i < isrc) return 0;
+  bool first = true;
+  double prev = T(F_R, i + 1);
+  for (int q = G.n - 1; q >= 0; q--)
+    for (int jj = G.j1[q]; jj >= G.j0[q]; jj--) {
+      const double rg = G.base[q] + (double)(jj - 1) * G.step[q];
+      if (!first)
+        n += rad_cell(T, s, nk, i, prev, rg, f);
+      else if (i != itop && rg < prev)
+        n += rad_cell(T, s, nk, i, prev, rg, f);
+      first = false;
+      prev = rg;
+      if (jj == G.j1[q] && q == G.src) return n;
+    }
+  return n;
+}
+
 // a hook of rad_sweeps that does nothing
 struct NoRadHook {
-  __device__ inline void operator()(int, int, double, double, const cd*) const {}
+  __host__ __device__ inline void operator()(int, int, double, double, const cd*) const {}
 };
 
 }  // namespace ysx
