@@ -45,7 +45,8 @@ sys.path.insert(0, ROOT)
 
 from yspec_amd import api  # noqa: E402
 from yspec_amd.flops import ode_flops, sum_flops, fft_bytes  # noqa: E402
-from yspec_amd.synth import batch_events, bump_deck, synth_deck, workload  # noqa: E402
+from yspec_amd.synth import (batch_events, bump_deck, inversion_events, synth_deck,  # noqa: E402
+                             workload)
 from yspec_amd.tune import tune  # noqa: E402
 
 METRIC = "(l,ω) solves/s + seismograms/s, batched vs plain GPU; gradient/forward time"
@@ -227,6 +228,57 @@ def gradient_compare(deck, steps, warmup):
     return out
 
 
+def inversion_compare(deck, steps, warmup):
+    """The batched inversion step: 32 events (depths U[0,700] km, 100
+    receivers each, lmax 600, 40 mHz, 180 min at 1 s, attenuation +
+    self-gravitation; data from the deck with 2% Gaussian bumps in vp and
+    vs): one batched forward launch against one ysx_misfit_grad over all
+    events (its own batched forward, the residuals, the adjoint sweeps as
+    one persistent launch each over every event's pairs, the per-event
+    gradients summed on the device)."""
+    events = inversion_events(32)
+    g = api.YSpec(bump_deck(deck), device=0)
+    g.set_events(events)
+    g.forward_events(1)
+    obs = [g.event_seismograms(e)[:g.info[e]["nout"]] for e in range(len(events))]
+    g.close()
+    g = api.YSpec(deck, device=0)
+    g.set_events(events)
+    for _ in range(warmup):
+        g.forward_events(1)
+        g.misfit_grad(obs)
+    tf, tg, gk = [], [], np.zeros(7)
+    for k in range(steps):
+        t0 = time.perf_counter()
+        g.forward_events(1)
+        tf.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        g.misfit_grad(obs)
+        tg.append(time.perf_counter() - t0)
+        gk += g.grad_timings()
+        print(f"# inversion step {k + 1}/{steps} done", file=sys.stderr, flush=True)
+    gk /= steps
+    mem = g.grad_memory()
+    out = {
+        "workload": "32 events, depths U[0,700] km, 100 receivers each, lmax=600, "
+                    "fmax=40 mHz, 180 min, dt=1 s, attenuation + self-gravitation",
+        "pairs": g.npair,
+        "forward_batched_ms": 1e3 * float(np.median(tf)),
+        "gradient_ms": 1e3 * float(np.median(tg)),
+        "gradient_over_forward": float(np.median(tg) / np.median(tf)),
+        "adjoint_kernel_ms": {"bvec_sweep": float(gk[0]), "minor_sweep": float(gk[1]),
+                              "tor_sweep": float(gk[2]), "starting_solutions": float(gk[3]),
+                              "radial": float(gk[4]), "source": float(gk[5])},
+        "gradient_device_bytes": {"interval_checkpoints": float(mem[0]),
+                                  "tapes": float(mem[1]), "per_pair_adjoints": float(mem[2])},
+        "note": "gradient_ms: one ysx_misfit_grad over all events (batched forward "
+                "included); checkpoints and tapes are per persistent block, so they do "
+                "not grow with the number of pairs",
+    }
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +292,10 @@ def main():
                     help="skip the misfit-gradient workload")
     ap.add_argument("--gradient-only", action="store_true",
                     help="only the misfit-gradient workload, for profiling")
+    ap.add_argument("--no-inversion", action="store_true",
+                    help="skip the 32-event batched forward + adjoint workload")
+    ap.add_argument("--inversion-only", action="store_true",
+                    help="only the 32-event batched forward + adjoint workload")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the 64-event comparison (no tuning), for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -262,6 +318,9 @@ def main():
     deck = synth_deck(0)
     if args.gradient_only:
         print(json.dumps(gradient_compare(deck, args.steps, args.warmup)), flush=True)
+        return
+    if args.inversion_only:
+        print(json.dumps(inversion_compare(deck, args.steps, args.warmup)), flush=True)
         return
     if args.batched_only:
         print(json.dumps(batched_compare(deck, args.steps, args.warmup, do_tune=False)),
@@ -357,6 +416,8 @@ def main():
         gr = gradient_compare(deck, args.steps, args.warmup)
         line["gradient_over_forward"] = gr["gradient_over_forward"]
         line["gradient"] = gr
+    if not args.no_inversion and args.lmax is None:
+        line["inversion"] = inversion_compare(deck, 1, 1)
     if not args.no_batched and args.lmax is None:
         line["batched"] = batched_compare(deck, args.steps, args.warmup)
     if not args.no_cpu:

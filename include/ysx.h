@@ -243,27 +243,39 @@ int ysx_source_model_vjp(ysx_ctx *ctx, double *table_bar);
  *     446-561), every step of both sweeps replayed from a tape;
  *   - the source vectors (as ysx_source_model_vjp).
  * Start levels, step counts
- * and renormalisations are those of the forward (held fixed).  Built for
- * one event with self-gravitation, a solid inner core, no ocean, receivers
- * at the surface, no potential/tilt correction and the default b-vector
- * rows; other runs return YSX_ERR_UNSUPPORTED.  ysx_model_table_vjp takes
+ * and renormalisations are those of the forward (held fixed).  With
+ * several events (ysx_set_events) their gradients are summed on the
+ * device: each sweep is one persistent launch over every event's pairs.
+ * Built for self-gravitation, a solid inner core, no ocean, receivers at
+ * the surface, no potential/tilt correction and the default b-vector rows;
+ * other runs return YSX_ERR_UNSUPPORTED.  ysx_model_table_vjp takes
  * table_bar to the deck. */
 int ysx_model_vjp(ysx_ctx *ctx, double *table_bar);
 
 /* L2 waveform misfit and its gradient with respect to the deck's knot
- * values (no counterpart in the reference): one forward of the run, then
+ * values (no counterpart in the reference): one forward of the run (the
+ * batched launch of ysx_forward_events when several events are set), then
  *   misfit = 0.5 sum (seismogram - obs)^2
- * over the first nout samples of the 3 nr traces (the samples the output
- * files hold), and knot_grad (nknot x 4, row-major) = d misfit / d (rho,
- * vp, vs, Qmu) of each knot in the deck's SI units, where vp (vs) moves vpv
- * and vph (vsv and vsh) together.  obs: nout x 3 x nr column-major (time
- * fastest), the first nout samples of each trace in ysx_seismograms'
- * layout.  The residual is formed on the device; the chain is
- * ysx_seismograms_vjp, ysx_spectra_vjp, ysx_model_vjp and
- * ysx_model_table_vjp, with ysx_model_vjp's scope; stage times from
+ * over the first n samples of every trace of every event, n = nout, or for
+ * nout <= 0 each event's own output length (the samples the output files
+ * hold), and knot_grad (nknot x 4, row-major) = d misfit / d (rho, vp, vs,
+ * Qmu) of each knot in the deck's SI units, summed over the events, where
+ * vp (vs) moves vpv and vph (vsv and vsh) together.  obs: per event in
+ * order, n x 3 x nr column-major (time fastest), the first n samples of
+ * each trace in ysx_seismograms' layout.  The residuals are formed on the
+ * device; the chain is ysx_seismograms_vjp, ysx_spectra_vjp, ysx_model_vjp
+ * and ysx_model_table_vjp, with ysx_model_vjp's scope; stage times from
  * ysx_timings, ysx_vjp_timings and ysx_grad_timings afterwards. */
 int ysx_misfit_grad(ysx_ctx *ctx, const double *obs, int nout, double *misfit,
                     double *knot_grad);
+
+/* Device bytes the model gradient holds for the run (after a
+ * ysx_model_vjp; no counterpart in the reference): [0] the sweeps' knot-
+ * interval checkpoints (per persistent block, independent of the number of
+ * pairs), [1] their per-interval tapes and the radial tape, [2] the
+ * per-pair adjoints (record adjoints, minors at the source and start) and
+ * the per-block partial table adjoints. */
+int ysx_grad_memory(const ysx_ctx *ctx, double *bytes3);
 
 /* Kernel milliseconds of the last ysx_model_vjp: [0] b-vector sweep, [1]
  * minor sweep, [2] toroidal sweep, [3] starting solutions, [4] radial

@@ -233,3 +233,36 @@ def test_autograd_knot_gradient(api):
     print(f"misfit {misfit:.6e} torch {lv:.6e}", flush=True)
     assert abs(lv - misfit) <= 1e-12 * misfit
     assert np.max(np.abs(tg - grad)) <= 1e-12 * np.max(np.abs(grad))
+
+
+def test_batched_events_gradient_is_the_sum(api):
+    """Several events on one context (one batched forward launch, each
+    adjoint sweep one persistent launch over every event's pairs): the
+    misfit and the knot gradient are the sums of the single-event ones
+    (events with different depths, frequency grids, receivers, all from
+    degree 0)"""
+    deck = synth_deck(0)
+    events = [workload("cfg1", seed=s, lmin=0, lmax=20, nr=2, ats=1, src_depth=d, f2=f,
+                       f21=0.8 * f, f22=0.9 * f)
+              for s, d, f in ((1, 50.0, 5.0), (2, 300.0, 6.0), (3, 650.0, 4.0))]
+    obs = []
+    for p in events:
+        h = api.YSpec(bumped(deck), p)
+        obs.append(h.forward())
+        h.close()
+    total, gsum = 0.0, np.zeros((deck.nknot, 4))
+    for p, o in zip(events, obs):
+        h = api.YSpec(deck, p)
+        m, gr = h.misfit_grad(o)
+        h.close()
+        total += m
+        gsum += gr
+    g = api.YSpec(deck)
+    g.set_events(events)
+    mb, gb = g.misfit_grad(obs)
+    g.close()
+    print(f"misfit batched {mb:.12e} summed {total:.12e}; gradient max difference "
+          f"{np.max(np.abs(gb - gsum)) / np.max(np.abs(gsum)):.2e} of its largest entry",
+          flush=True)
+    assert abs(mb - total) <= 1e-10 * total
+    assert np.max(np.abs(gb - gsum)) <= 1e-9 * np.max(np.abs(gsum))

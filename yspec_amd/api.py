@@ -29,7 +29,7 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts",
            "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
            "ysx_vjp_timings", "ysx_source_model_vjp", "ysx_model_table_noalloc",
-           "ysx_model_vjp", "ysx_grad_timings", "ysx_misfit_grad")
+           "ysx_model_vjp", "ysx_grad_timings", "ysx_misfit_grad", "ysx_grad_memory")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -96,6 +96,7 @@ def lib():
         L.ysx_model_vjp.argtypes = [vp, D]
         L.ysx_grad_timings.argtypes = [vp, D]
         L.ysx_misfit_grad.argtypes = [vp, D, C.c_int, D, D]
+        L.ysx_grad_memory.argtypes = [vp, D]
         L.ysx_model_table_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                               C.c_double, D, C.c_void_p]
         _LIB = L
@@ -350,6 +351,13 @@ class YSpec:
         self._check(self.L.ysx_model_vjp(self.h, _dp(out)))
         return out
 
+    def grad_memory(self):
+        """device bytes of the model gradient: interval checkpoints, tapes,
+        per-pair adjoints (ysx_grad_memory)"""
+        out = np.zeros(3)
+        self._check(self.L.ysx_grad_memory(self.h, _dp(out)))
+        return out
+
     def grad_timings(self):
         """kernel ms of the last model_vjp: b-vector, minor and toroidal
         sweeps, starting solutions, radial solves, source vectors, sum"""
@@ -374,18 +382,31 @@ class YSpec:
         return misfit, model_table_vjp(self.deck, tb)
 
     def misfit_grad(self, obs, nout=None):
-        """ysx_misfit_grad: one forward, the misfit 0.5 * sum((seis - obs)^2)
-        over the first nout samples (default: the samples the drop-in
-        program writes) and its (nknot, 4) gradient with respect to each
-        knot's rho, vp, vs and Qmu (vp and vs moving both polarisations).
-        obs: (>= nout, 3, nr) seismograms."""
-        nout = self.nout if nout is None else nout
-        o = np.asfortranarray(np.asarray(obs, dtype=np.float64)[:nout])
-        if o.shape != (nout, 3, self.p.nr):
-            raise ValueError(f"obs must hold (nout, 3, nr) = ({nout}, 3, {self.p.nr}) samples")
+        """ysx_misfit_grad: one forward (the batched launch when several
+        events are set), the misfit 0.5 * sum((seis - obs)^2) over the first
+        nout samples of every trace (default: each event's own output
+        length, the samples the drop-in program writes) and its (nknot, 4)
+        gradient with respect to each knot's rho, vp, vs and Qmu (vp and vs
+        moving both polarisations), summed over the events.  obs: the
+        (>= n, 3, nr) seismograms of the run, or a list with one per event."""
+        if not isinstance(obs, (list, tuple)):
+            obs = [obs]
+        if len(obs) != len(self.events):
+            raise ValueError(f"obs must hold one array per event ({len(self.events)})")
+        parts = []
+        for e, (o, p) in enumerate(zip(obs, self.events)):
+            own = self.nout if len(self.events) == 1 else self.info[e]["nout"]
+            n = own if nout is None else nout
+            a = np.asarray(o, dtype=np.float64)[:n]
+            if a.shape != (n, 3, p.nr):
+                raise ValueError(f"event {e}: obs must hold (nout, 3, nr) = ({n}, 3, {p.nr}) "
+                                 "samples")
+            parts.append(a.ravel(order="F"))
+        flat = np.ascontiguousarray(np.concatenate(parts))
         misfit = np.zeros(1)
         grad = np.zeros((self.deck.nknot, 4))
-        self._check(self.L.ysx_misfit_grad(self.h, _dp(o), nout, _dp(misfit), _dp(grad)))
+        self._check(self.L.ysx_misfit_grad(self.h, _dp(flat), 0 if nout is None else nout,
+                                           _dp(misfit), _dp(grad)))
         return float(misfit[0]), grad
 
     def model_table(self):

@@ -135,7 +135,7 @@ struct ysx_ctx {
   int* d_rcap = nullptr;
   void* d_rtape = nullptr;
   size_t rtape_bytes = 0;
-  int rad_cap = -1;
+  std::vector<int> rad_caps;  // per event from degree 0: the tape's length
   double ms_grad[7] = {0};  // kernel ms of the last ysx_model_vjp
   // ysx_misfit_grad: the observed traces and the residual's per-block sums
   double *d_obs = nullptr, *d_rpart = nullptr;
@@ -186,7 +186,7 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
   dfree(c->d_obs); dfree(c->d_rpart);
   c->obs_n = 0;
-  c->rad_cap = -1;
+  c->rad_caps.clear();
   c->adj_blocks = c->adj_slots = c->adj_tmax = 0;
   c->rtape_bytes = 0;
   c->have_specb = c->have_recb = false;
@@ -1493,31 +1493,35 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   if (!ctx->have_recb)
     return set_err(ctx, YSX_ERR_ARG, "no record adjoints: call ysx_spectra_vjp first");
   const HostModel& M = ctx->M;
-  if (ctx->rk.size() != 1)
-    return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_model_vjp: one event per context");
-  const RunK& k = ctx->rk[0];
-  const HostRun& R = ctx->R[0];
+  const int nev = (int)ctx->rk.size();
   const double rsurf = M.at(F_R, M.nknot);
-  std::string why;
-  if (k.grav != 2) why = "gravity switch 0 or 1";
-  else if (M.fic) why = "a fluid inner core";
-  else if (M.ocean) why = "an ocean";
-  else if (k.rr < k.rs || k.rr != rsurf) why = "a receiver below the surface";
-  else if (k.out == 2 && k.cor != 0) why = "the potential/tilt correction";
-  else if (!k.brow) why = "the source-side b-vector columns";
-  else if (M.nknot > 800) why = "more than 800 knots";
-  if (!why.empty())
-    return set_err(ctx, YSX_ERR_UNSUPPORTED,
-                   "ysx_model_vjp: the solver adjoint is built for self-gravitation, a solid "
-                   "inner core, no ocean and receivers at the surface; this run has " + why);
-  (void)R;
+  for (int e = 0; e < nev; e++) {
+    const RunK& k = ctx->rk[e];
+    std::string why;
+    if (k.grav != 2) why = "gravity switch 0 or 1";
+    else if (M.fic) why = "a fluid inner core";
+    else if (M.ocean) why = "an ocean";
+    else if (k.rr < k.rs || k.rr != rsurf) why = "a receiver below the surface";
+    else if (k.out == 2 && k.cor != 0) why = "the potential/tilt correction";
+    else if (!k.brow) why = "the source-side b-vector columns";
+    else if (M.nknot > 800) why = "more than 800 knots";
+    if (!why.empty())
+      return set_err(ctx, YSX_ERR_UNSUPPORTED,
+                     "ysx_model_vjp: the solver adjoint is built for self-gravitation, a solid "
+                     "inner core, no ocean and receivers at the surface; event " +
+                         std::to_string(e) + " has " + why);
+  }
   const int nk = M.nknot;
   const size_t P = (size_t)ctx->npair;
   // buffers (kept for the run)
   if (!ctx->d_tbar) {
     ctx->adj_blocks = 4 * ctx->ncu;
-    ctx->adj_slots = k.ir + 1;
-    ctx->adj_tmax = max_interval_steps(M, k);
+    ctx->adj_slots = 1;
+    ctx->adj_tmax = 1;
+    for (const RunK& k : ctx->rk) {
+      ctx->adj_slots = std::max(ctx->adj_slots, k.ir + 1);
+      ctx->adj_tmax = std::max(ctx->adj_tmax, max_interval_steps(M, k));
+    }
     const size_t nb = ctx->adj_blocks, ns = ctx->adj_slots, nt = ctx->adj_tmax;
     HIPCHK(hipMalloc(&ctx->d_mub, P * 14 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_m0bar, P * 14 * sizeof(cd)));
@@ -1545,7 +1549,13 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
   AdjWork AW{ctx->d_ack, ctx->d_acks, ctx->d_atape, ctx->d_atsc, ctx->adj_slots, ctx->adj_tmax,
              ctx->d_aerr};
-  Queue qb = plain_q(ctx, W_BVEC, 0), qt = plain_q(ctx, W_TOR, 0);
+  // every event's items in one persistent launch per sweep: the plain
+  // lists of all events are contiguous
+  auto all_q = [&](int kind) {
+    const WorkList& w = ctx->W[kind];
+    return Queue{w.d_plain, w.start[nev] - w.start[0], nullptr, 1};
+  };
+  Queue qb = all_q(W_BVEC), qt = all_q(W_TOR);
   Queue qm = qt;
   qb.counter = ctx->d_aq;
   qm.counter = ctx->d_aq + 1;
@@ -1569,53 +1579,76 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[3], st));
   {
-    PairRange PR = pairs_of(ctx, 0, 1);
+    PairRange PR = pairs_of(ctx, 0, nev);
     launch_start_adj(dim3((unsigned)((PR.n + 255) / 256)), b256,
                      (size_t)SV_N * 2 * nk * sizeof(double), st, ctx->d_tab, ctx->d_runs, PR, SD,
                      ctx->d_starts, ctx->d_m0bar, ctx->d_y0bar, ctx->d_tbar);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ev[4], st));
-  if (k.lmin == 0) {
-    // the radial solves, by segments (sweep, knot interval)
+  // the radial solves of the events from degree 0, by segments (sweep,
+  // knot interval), one event after another on shared buffers
+  {
     const Tab HT{M.tab.data(), nk};
-    RadSegs G;
-    G.isrc = rad_src_interval(HT, nk, k.rs);
-    G.itop = rad_top_interval(HT, nk);
-    G.nseg = G.isrc + (G.itop - G.isrc + 1);
-    G.nwb = (k.nw + 63) / 64;
-    if (G.isrc < 1 || G.itop < G.isrc || HT(F_R, 1) == HT(F_R, 2))
-      return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_model_vjp: radial grid without a source "
-                                               "interval or a first interval");
-    const size_t ns = (size_t)G.nseg * k.nw;
-    if (!ctx->d_rprop) {
-      HIPCHK(hipMalloc(&ctx->d_rprop, ns * 4 * sizeof(cd)));
-      HIPCHK(hipMalloc(&ctx->d_rys, ns * 2 * sizeof(cd)));
-      HIPCHK(hipMalloc(&ctx->d_rxe, ns * 2 * sizeof(cd)));
+    std::vector<int> rev;
+    std::vector<RadSegs> segs;
+    size_t nsmax = 0;
+    for (int e = 0; e < nev; e++) {
+      const RunK& k = ctx->rk[e];
+      if (k.lmin != 0) continue;
+      RadSegs G;
+      G.isrc = rad_src_interval(HT, nk, k.rs);
+      G.itop = rad_top_interval(HT, nk);
+      G.nseg = G.isrc + (G.itop - G.isrc + 1);
+      G.nwb = (k.nw + 63) / 64;
+      G.cap = 0;
+      if (G.isrc < 1 || G.itop < G.isrc || HT(F_R, 1) == HT(F_R, 2))
+        return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_model_vjp: radial grid without a source "
+                                                 "interval or a first interval");
+      nsmax = std::max(nsmax, (size_t)G.nseg * k.nw);
+      rev.push_back(e);
+      segs.push_back(G);
+    }
+    if (!rev.empty() && !ctx->d_rprop) {
+      HIPCHK(hipMalloc(&ctx->d_rprop, nsmax * 4 * sizeof(cd)));
+      HIPCHK(hipMalloc(&ctx->d_rys, nsmax * 2 * sizeof(cd)));
+      HIPCHK(hipMalloc(&ctx->d_rxe, nsmax * 2 * sizeof(cd)));
       HIPCHK(hipMalloc(&ctx->d_rcap, sizeof(int)));
     }
-    const dim3 gs(G.nseg, G.nwb);
-    launch_reset(dim3(1), b64, 0, st, ctx->d_rcap, 1);
-    G.cap = 0;
-    launch_rad_prop(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, G, ctx->d_rprop, ctx->d_rcap);
-    launch_rad_scan(dim3(G.nwb), b64, 0, st, ctx->d_tab, ctx->d_runs, G, ctx->d_rprop,
-                    ctx->d_recb, ctx->d_rys, ctx->d_rxe, ctx->d_tbar);
-    HIPCHK(hipGetLastError());
-    if (ctx->rad_cap < 0) {  // the tape's length: the most steps of a segment (once per run)
-      HIPCHK(hipStreamSynchronize(st));
-      HIPCHK(hipMemcpy(&ctx->rad_cap, ctx->d_rcap, sizeof(int), hipMemcpyDeviceToHost));
-      const size_t bytes =
-          std::max<size_t>(1, (size_t)G.nseg * G.nwb * ctx->rad_cap * 64) * rad_step_bytes();
+    if (!rev.empty() && ctx->rad_caps.empty()) {
+      // the tape's length per event: the most steps of a segment (once per run)
+      size_t bytes = 1;
+      for (size_t q = 0; q < rev.size(); q++) {
+        const RadSegs& G = segs[q];
+        launch_reset(dim3(1), b64, 0, st, ctx->d_rcap, 1);
+        launch_rad_prop(dim3(G.nseg, G.nwb), b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G,
+                        ctx->d_rprop, ctx->d_rcap);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        int cap = 0;
+        HIPCHK(hipMemcpy(&cap, ctx->d_rcap, sizeof(int), hipMemcpyDeviceToHost));
+        ctx->rad_caps.push_back(cap);
+        bytes = std::max(bytes, (size_t)G.nseg * G.nwb * cap * 64 * rad_step_bytes());
+      }
       if (bytes > ctx->rtape_bytes) {
         dfree(ctx->d_rtape);
         HIPCHK(hipMalloc(&ctx->d_rtape, bytes));
         ctx->rtape_bytes = bytes;
       }
     }
-    G.cap = ctx->rad_cap;
-    launch_rad_seg_adj(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, G, ctx->d_rys, ctx->d_rxe,
-                       ctx->d_rtape, ctx->d_aerr, ctx->d_tbar);
-    HIPCHK(hipGetLastError());
+    for (size_t q = 0; q < rev.size(); q++) {
+      RadSegs G = segs[q];
+      G.cap = ctx->rad_caps[q];
+      const dim3 gs(G.nseg, G.nwb);
+      launch_reset(dim3(1), b64, 0, st, ctx->d_rcap, 1);
+      launch_rad_prop(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G, ctx->d_rprop,
+                      ctx->d_rcap);
+      launch_rad_scan(dim3(G.nwb), b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G,
+                      ctx->d_rprop, ctx->d_recb, ctx->d_rys, ctx->d_rxe, ctx->d_tbar);
+      launch_rad_seg_adj(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G, ctx->d_rys,
+                         ctx->d_rxe, ctx->d_rtape, ctx->d_aerr, ctx->d_tbar);
+      HIPCHK(hipGetLastError());
+    }
   }
   HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipEventSynchronize(ev[5]));
@@ -1651,34 +1684,52 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
                                double* knot_grad) {
   if (!ctx || !obs || !misfit || !knot_grad) return YSX_ERR_ARG;
   if (!ctx->have_run) return set_err(ctx, YSX_ERR_ARG, "no run");
-  if (ctx->rk.size() != 1)
-    return set_err(ctx, YSX_ERR_UNSUPPORTED, "ysx_misfit_grad: one event per context");
-  const HostRun& R = ctx->R[0];
-  if (nout < 1 || nout > R.nt)
-    return set_err(ctx, YSX_ERR_ARG, "nout outside 1 .. nt (" + std::to_string(R.nt) + ")");
-  if (int rc = ysx_forward(ctx, nullptr)) return rc;
+  const int nev = (int)ctx->rk.size();
+  // samples per event: nout, or each event's own output length
+  std::vector<int> ne(nev);
+  std::vector<size_t> ooff(nev + 1, 0);
+  for (int e = 0; e < nev; e++) {
+    const HostRun& R = ctx->R[e];
+    ne[e] = nout > 0 ? nout : R.nout;
+    if (ne[e] < 1 || ne[e] > R.nt)
+      return set_err(ctx, YSX_ERR_ARG, "nout outside 1 .. nt (" + std::to_string(R.nt) +
+                                           ") for event " + std::to_string(e));
+    ooff[e + 1] = ooff[e] + (size_t)ne[e] * 3 * R.nr;
+  }
+  // one forward: the plain path for one event, the batched launch for several
+  if (int rc = nev == 1 ? ysx_forward(ctx, nullptr) : ysx_forward_events(ctx, 1, nullptr))
+    return rc;
   if (int rc = adjoint_run(ctx)) return rc;
-  const size_t ntrace = 3 * (size_t)R.nr, nobs = (size_t)nout * ntrace;
+  const size_t nobs = ooff[nev];
   constexpr int RBLK = 1024;
   if (ctx->obs_n < nobs) {
     dfree(ctx->d_obs);
     HIPCHK(hipMalloc(&ctx->d_obs, nobs * sizeof(double)));
     ctx->obs_n = nobs;
   }
-  if (!ctx->d_rpart) HIPCHK(hipMalloc(&ctx->d_rpart, RBLK * sizeof(double)));
+  dfree(ctx->d_rpart);
+  HIPCHK(hipMalloc(&ctx->d_rpart, (size_t)RBLK * nev * sizeof(double)));
   HIPCHK(hipMemcpy(ctx->d_obs, obs, nobs * sizeof(double), hipMemcpyHostToDevice));
-  const size_t n = (size_t)R.nt * ntrace;
-  const unsigned nb = (unsigned)std::min<size_t>(RBLK, (n + 255) / 256);
-  launch_residual(dim3(nb), dim3(256), 0, ctx->stream, ctx->d_seis, ctx->d_obs, R.nt, nout,
-                  ntrace, ctx->d_seisb, ctx->d_rpart);
+  std::vector<unsigned> nb(nev);
+  for (int e = 0; e < nev; e++) {
+    const HostRun& R = ctx->R[e];
+    const RunK& k = ctx->rk[e];
+    const size_t ntrace = 3 * (size_t)R.nr, n = (size_t)R.nt * ntrace;
+    nb[e] = (unsigned)std::min<size_t>(RBLK, (n + 255) / 256);
+    launch_residual(dim3(nb[e]), dim3(256), 0, ctx->stream, ctx->d_seis + k.seis_off,
+                    ctx->d_obs + ooff[e], R.nt, ne[e], ntrace, ctx->d_seisb + k.seis_off,
+                    ctx->d_rpart + (size_t)RBLK * e);
+  }
   HIPCHK(hipGetLastError());
   // the context's streams do not block on the null stream: wait for the
-  // residual before copying its sums
+  // residuals before copying their sums
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  std::vector<double> part(nb);
-  HIPCHK(hipMemcpy(part.data(), ctx->d_rpart, nb * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> part((size_t)RBLK * nev);
+  HIPCHK(hipMemcpy(part.data(), ctx->d_rpart, part.size() * sizeof(double),
+                   hipMemcpyDeviceToHost));
   double sum = 0.0;
-  for (double v : part) sum += v;
+  for (int e = 0; e < nev; e++)
+    for (unsigned q = 0; q < nb[e]; q++) sum += part[(size_t)RBLK * e + q];
   *misfit = 0.5 * sum;
   if (int rc = seis_vjp_device(ctx)) return rc;
   if (int rc = ysx_spectra_vjp(ctx, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr))
@@ -1697,6 +1748,18 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
     g[2] = d[3] + d[7];  // vs: vsv and vsh together
     g[3] = d[5];         // Qmu
   }
+  return YSX_OK;
+}
+
+extern "C" int ysx_grad_memory(const ysx_ctx* ctx, double* bytes3) {
+  if (!ctx || !bytes3) return YSX_ERR_ARG;
+  const double nb = ctx->adj_blocks, ns = ctx->adj_slots, nt = ctx->adj_tmax;
+  const double P = (double)ctx->npair, nk = ctx->M.nknot;
+  bytes3[0] = nb * ns * 64 * (14 * sizeof(cd) + sizeof(int));         // interval checkpoints
+  bytes3[1] = nb * nt * 64 * (6 * 14 * sizeof(cd) + sizeof(int)) +    // per-interval tapes
+              (double)ctx->rtape_bytes;
+  bytes3[2] = ctx->d_tbar ? P * (2 * 14 + 2 + REC) * sizeof(cd) + nb * F_COUNT * nk * 8.0
+                          : 0.0;                                       // per-pair adjoints
   return YSX_OK;
 }
 

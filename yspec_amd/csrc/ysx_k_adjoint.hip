@@ -601,7 +601,10 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
           const size_t f = p * REC + O_YSPH + 6 * c + k;
           g = cmad(g, conj_(recb[f]), rec[f]);
         }
-    g = valid ? cmul_(SS.fac[p], g) : C(0.0);
+    // fac H^SS.sc3 is the same for every forward (the batched path's
+    // k_drow_march stores fac with its own row's count in sc3); here it
+    // multiplies the continuation re-marched above, counter sc3
+    g = valid ? cmul_(SS.fac[p] * scale_pow(SS.sc3[p] - sc3), g) : C(0.0);
     cd m2[14];
     for (int q = 0; q < 14; q++) m2[q] = valid ? SS.mfin[p * 14 + q] : C(0.0);
     delta_row(m2, x);
@@ -876,9 +879,9 @@ __device__ inline double rad_dr(const RunK& R, const PS& s) {
 
 // propagators, column-major: P[2 c + r] (row r, column c)
 __global__ void __launch_bounds__(64) k_rad_prop(const double* __restrict__ gtab,
-                                                 const RunK* __restrict__ runs, RadSegs G,
+                                                 const RunK* __restrict__ runs, int e, RadSegs G,
                                                  cd* __restrict__ prop, int* __restrict__ cap) {
-  const RunK& R = runs[0];
+  const RunK& R = runs[e];
   const int q = blockIdx.x, j = (int)(blockIdx.y * 64 + threadIdx.x) + 1;
   if (j > R.nw) return;
   const Tab T{gtab, R.nknot};
@@ -909,12 +912,12 @@ __device__ inline void prop_row(const cd* P, cd* x) {
 }
 
 __global__ void __launch_bounds__(64) k_rad_scan(const double* __restrict__ gtab,
-                                                 const RunK* __restrict__ runs, RadSegs G,
+                                                 const RunK* __restrict__ runs, int e, RadSegs G,
                                                  const cd* __restrict__ prop,
                                                  const cd* __restrict__ recb,
                                                  cd* __restrict__ ys, cd* __restrict__ xe,
                                                  double* __restrict__ tbar) {
-  const RunK& R = runs[0];
+  const RunK& R = runs[e];
   const int j = (int)(blockIdx.x * 64 + threadIdx.x) + 1;
   if (j > R.nw || R.lmin != 0) return;
   const Tab T{gtab, R.nknot};
@@ -987,14 +990,15 @@ __global__ void __launch_bounds__(64) k_rad_scan(const double* __restrict__ gtab
 }
 
 __global__ void __launch_bounds__(64) k_rad_seg_adj(const double* __restrict__ gtab,
-                                                    const RunK* __restrict__ runs, RadSegs G,
+                                                    const RunK* __restrict__ runs, int e,
+                                                    RadSegs G,
                                                     const cd* __restrict__ ys,
                                                     const cd* __restrict__ xe,
                                                     RadSt* __restrict__ tape,
                                                     int* __restrict__ err,
                                                     double* __restrict__ tbar) {
   __shared__ double red[RAD_NV * 4 * 64];
-  const RunK& R = runs[0];
+  const RunK& R = runs[e];
   const int q = blockIdx.x, lane = threadIdx.x, j = (int)(blockIdx.y * 64 + lane) + 1;
   const Tab T{gtab, R.nknot};
   int sweep, i;
@@ -1072,18 +1076,18 @@ void launch_part_sum(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* tp
 }
 size_t rad_step_bytes() { return sizeof(RadSt); }
 void launch_rad_prop(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                     const RunK* runs, RadSegs G, cd* prop, int* cap) {
-  hipLaunchKernelGGL(k_rad_prop, g, b, lds, s, gtab, runs, G, prop, cap);
+                     const RunK* runs, int e, RadSegs G, cd* prop, int* cap) {
+  hipLaunchKernelGGL(k_rad_prop, g, b, lds, s, gtab, runs, e, G, prop, cap);
 }
 void launch_rad_scan(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                     const RunK* runs, RadSegs G, const cd* prop, const cd* recb, cd* ys, cd* xe,
-                     double* tbar) {
-  hipLaunchKernelGGL(k_rad_scan, g, b, lds, s, gtab, runs, G, prop, recb, ys, xe, tbar);
+                     const RunK* runs, int e, RadSegs G, const cd* prop, const cd* recb, cd* ys,
+                     cd* xe, double* tbar) {
+  hipLaunchKernelGGL(k_rad_scan, g, b, lds, s, gtab, runs, e, G, prop, recb, ys, xe, tbar);
 }
 void launch_rad_seg_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                        const RunK* runs, RadSegs G, const cd* ys, const cd* xe, void* tape,
-                        int* err, double* tbar) {
-  hipLaunchKernelGGL(k_rad_seg_adj, g, b, lds, s, gtab, runs, G, ys, xe,
+                        const RunK* runs, int e, RadSegs G, const cd* ys, const cd* xe,
+                        void* tape, int* err, double* tbar) {
+  hipLaunchKernelGGL(k_rad_seg_adj, g, b, lds, s, gtab, runs, e, G, ys, xe,
                      static_cast<RadSt*>(tape), err, tbar);
 }
 

@@ -336,7 +336,7 @@ void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gta
 void launch_start_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                       const RunK* runs, PairRange PR, StartData SD, const int* starts,
                       const cd* m0bar, const cd* y0bar, double* tbar);
-// radial adjoint of event 0, cut into segments (sweep, knot interval):
+// radial adjoint of event e, cut into segments (sweep, knot interval):
 // isrc upward segments (intervals 1 .. isrc), then downward ones (itop ..
 // isrc); nwb blocks of 64 frequencies; cap: the most steps of a segment
 // (k_rad_prop's atomic maximum), the tape's length per lane
@@ -350,16 +350,16 @@ struct RadSt {  // a step of the tape: radius, step, state before it
 size_t rad_step_bytes();
 // grid (nseg, nwb) x 64: propagators prop[(q nw + j - 1) 4 ..], cap
 void launch_rad_prop(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                     const RunK* runs, RadSegs G, cd* prop, int* cap);
+                     const RunK* runs, int e, RadSegs G, cd* prop, int* cap);
 // grid nwb x 64: states ys and rows xe at the segment ends, start part
 void launch_rad_scan(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                     const RunK* runs, RadSegs G, const cd* prop, const cd* recb, cd* ys, cd* xe,
-                     double* tbar);
+                     const RunK* runs, int e, RadSegs G, const cd* prop, const cd* recb, cd* ys,
+                     cd* xe, double* tbar);
 // grid (nseg, nwb) x 64: the segments reversed, tape of nseg nwb cap 64
 // steps
 void launch_rad_seg_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                        const RunK* runs, RadSegs G, const cd* ys, const cd* xe, void* tape,
-                        int* err, double* tbar);
+                        const RunK* runs, int e, RadSegs G, const cd* ys, const cd* xe,
+                        void* tape, int* err, double* tbar);
 #define YSX_MARCH_LAUNCHERS(SUFFIX)                                                              \
   void launch_tor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
                                  const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt);   \

@@ -12,6 +12,8 @@
   moment tensors (uniform in +-1e20 N m) and receivers uniform on the sphere.
 * ``bump_deck``: the data deck of the misfit-gradient workload (2% Gaussian
   bumps in vp and vs).
+* ``batch_events`` / ``inversion_events``: the seeded multi-event workloads
+  (64 events of uneven frequency grids; 32 events at lmax 600, 40 mHz).
 """
 from __future__ import annotations
 
@@ -139,4 +141,25 @@ def batch_events(nev: int = 64, seed: int = 0, nr: int = 16, fmax_range=(10.0, 4
                           tout=tout, dt=dt, f11=0.2, f12=0.5, f21=0.85 * fmax,
                           f22=0.95 * fmax, src_depth=depth, src_lat=float(slat[0]),
                           src_lon=float(slon[0]), mm=mm, rec_depth=0.0, lat=lat, lon=lon))
+    return out
+
+
+def inversion_events(nev: int = 32, seed: int = 0, nr: int = 100, depth_range=(0.0, 700.0),
+                     model_file: str = "synth.200") -> list:
+    """Seeded batched inversion workload: per event a source depth uniform in
+    ``depth_range`` km, ``nr`` receivers at the surface and a moment tensor
+    uniform in +-1e20 N m; every event lmax = 600, fmax = 40 mHz, 180 min at
+    1 s, filters 0.2/0.5/35/38 mHz, attenuation and full self-gravitation."""
+    rng = np.random.default_rng(7000 + seed)
+    out = []
+    for _ in range(nev):
+        depth = float(rng.uniform(*depth_range))
+        mm = rng.uniform(-1e20, 1e20, 6)
+        slat, slon = _receivers(rng, 1)
+        lat, lon = _receivers(rng, nr)
+        out.append(Params(prefix="yspec.out", model_file=model_file, ats=1, grav=2, out=0,
+                          cor=0, lmin=0, lmax=600, f1=0.1, f2=40.0, tout=180.0, dt=1.0,
+                          f11=0.2, f12=0.5, f21=35.0, f22=38.0, src_depth=depth,
+                          src_lat=float(slat[0]), src_lon=float(slon[0]), mm=mm,
+                          rec_depth=0.0, lat=lat, lon=lon))
     return out
