@@ -82,9 +82,18 @@ int ysx_model_table_noalloc(ysx_ctx *ctx, int nknot, int nic, int noc, int ians,
  * the deck's layout) = J^T table_bar for table = ysx_model_table_host(deck)
  * and J its Jacobian with respect to the deck columns rho, vpv, vsv, qkappa,
  * qmu, vph, vsh, eta.  Knot radii are not differentiated; at a zero shear
- * modulus (fluid) beta = sqrt(mu / rho) contributes nothing.  Host only. */
+ * modulus (fluid) beta = sqrt(mu / rho) contributes nothing.  Host function;
+ * ysx_model_vjp_deck is the same map on the device. */
 int ysx_model_table_vjp(int nknot, int nic, int noc, int ians, double tref_period,
                         const double *deck, const double *table_bar, double *deck_bar);
+
+/* ysx_model_table_vjp on the device for the context's model (the deck of
+ * the last ysx_set_model; src/module_model.f90:253-378 transposed): one
+ * block of k_model_vjp, the same functions as the host's, so deck_bar
+ * equals ysx_model_table_vjp's bit for bit.  table_bar: host, F_COUNT x
+ * nknot (the layout of ysx_model_table); deck_bar: host, nknot x 9.
+ * ysx_misfit_grad uses it for its last stage. */
+int ysx_model_vjp_deck(ysx_ctx *ctx, const double *table_bar, double *deck_bar);
 
 /* Set the run: ip = lmin lmax ats grav out cor nr; fp = f1 f2 tout dt f11
  * f12 f21 f22 src_depth src_lat src_lon Mrr Mrt Mrp Mtt Mtp Mpp rec_depth;

@@ -29,7 +29,8 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_pair_records", "ysx_pair_steps", "ysx_timings", "ysx_step_counts",
            "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
            "ysx_vjp_timings", "ysx_source_model_vjp", "ysx_model_table_noalloc",
-           "ysx_model_vjp", "ysx_grad_timings", "ysx_misfit_grad", "ysx_grad_memory")
+           "ysx_model_vjp", "ysx_grad_timings", "ysx_misfit_grad", "ysx_grad_memory",
+           "ysx_model_vjp_deck")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -97,6 +98,7 @@ def lib():
         L.ysx_grad_timings.argtypes = [vp, D]
         L.ysx_misfit_grad.argtypes = [vp, D, C.c_int, D, D]
         L.ysx_grad_memory.argtypes = [vp, D]
+        L.ysx_model_vjp_deck.argtypes = [vp, D, D]
         L.ysx_model_table_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                               C.c_double, D, C.c_void_p]
         _LIB = L
@@ -369,8 +371,8 @@ class YSpec:
         """L2 misfit 0.5 * sum((seis - obs)^2) over the first nout samples
         (the samples the drop-in program writes) and its gradient with
         respect to the deck columns, (nknot, 9) in the deck's layout, after
-        one forward: seismograms_vjp, spectra_vjp, model_vjp on the device,
-        then the deck -> table adjoint on the host."""
+        one forward: seismograms_vjp, spectra_vjp, model_vjp and the deck ->
+        table adjoint (model_vjp_deck), each on the device."""
         nout = self.nout if nout is None else nout
         seis = self.forward()
         res = np.zeros_like(seis)
@@ -379,7 +381,7 @@ class YSpec:
         self.seismograms_vjp(res)
         self.spectra_vjp(np.zeros(0, np.int32), np.zeros(0, np.int32))
         tb = self.model_vjp()
-        return misfit, model_table_vjp(self.deck, tb)
+        return misfit, self.model_vjp_deck(tb)
 
     def misfit_grad(self, obs, nout=None):
         """ysx_misfit_grad: one forward (the batched launch when several
@@ -408,6 +410,16 @@ class YSpec:
         self._check(self.L.ysx_misfit_grad(self.h, _dp(flat), 0 if nout is None else nout,
                                            _dp(misfit), _dp(grad)))
         return float(misfit[0]), grad
+
+    def model_vjp_deck(self, table_bar):
+        """ysx_model_vjp_deck: (nknot, 9) deck_bar = J^T table_bar for this
+        context's deck, computed on the device (table_bar: (29, nknot))."""
+        tb = np.ascontiguousarray(table_bar, dtype=np.float64)
+        if tb.shape != (len(TABLE_FIELDS), self.deck.nknot):
+            raise ValueError("table_bar must be (29, nknot)")
+        out = np.zeros((self.deck.nknot, 9))
+        self._check(self.L.ysx_model_vjp_deck(self.h, _dp(tb), _dp(out)))
+        return out
 
     def model_table(self):
         """(29, nknot) model table the kernels read (built on the device)."""

@@ -5,7 +5,7 @@
   in it);
 * KnotSeismograms: with respect to each deck knot's rho, vp, vs and Qmu,
   through the discrete adjoint of every per-pair solve (ysx_model_vjp) and
-  the deck -> table transform (ysx_model_table_vjp).
+  the deck -> table transform (ysx_model_vjp_deck), all on the device.
 
 Thin on purpose: the forward and the backward are the library's own entry
 points; torch only carries the (host) tensors, so finite-difference checks
@@ -15,7 +15,7 @@ import dataclasses
 import numpy as np
 import torch
 
-from .api import deck_with_knots, knot_gradient, model_table_vjp
+from .api import deck_with_knots, knot_gradient
 
 
 class MomentTensorSeismograms(torch.autograd.Function):
@@ -70,7 +70,7 @@ class KnotSeismograms(torch.autograd.Function):
             raise RuntimeError("another run was set on this context after the forward")
         g.seismograms_vjp(seis_bar.detach().cpu().numpy())
         g.spectra_vjp(np.zeros(0, np.int32), np.zeros(0, np.int32))
-        grad = knot_gradient(model_table_vjp(ctx.deck, g.model_vjp()))
+        grad = knot_gradient(g.model_vjp_deck(g.model_vjp()))
         return torch.from_numpy(grad), None, None, None
 
 

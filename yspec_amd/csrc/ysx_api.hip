@@ -140,8 +140,14 @@ struct ysx_ctx {
   // ysx_misfit_grad: the observed traces and the residual's per-block sums
   double *d_obs = nullptr, *d_rpart = nullptr;
   size_t obs_n = 0;
-  // the deck as set (row-major nknot x 9) for the deck -> table adjoint
+  // the deck as set (row-major nknot x 9) for the deck -> table adjoint,
+  // and on the device: the staged deck and layer lists k_model read
+  // (layer_lo, layer_n), and k_model_vjp's buffers (allocated by its first
+  // call for the model)
   std::vector<double> deck;
+  double* d_mdeck = nullptr;
+  int* d_mlay = nullptr;
+  double* d_mvjp = nullptr;
   int ians = 0;
   double tref_period = 0.0;
 };
@@ -525,6 +531,9 @@ extern "C" void ysx_destroy(ysx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   free_run(ctx);
   dfree(ctx->d_tab);
+  dfree(ctx->d_mdeck);
+  dfree(ctx->d_mlay);
+  dfree(ctx->d_mvjp);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -542,6 +551,9 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
   HIPCHK(hipSetDevice(ctx->device));
   free_run(ctx);
   dfree(ctx->d_tab);
+  dfree(ctx->d_mdeck);
+  dfree(ctx->d_mlay);
+  dfree(ctx->d_mvjp);
   ctx->have_model = false;
   std::string e = build_model(ctx->N, nknot, nic, noc, ians, tref_period, deck, ctx->M);
   if (!e.empty()) return set_err(ctx, YSX_ERR_INPUT, e);
@@ -571,6 +583,13 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
       dfree(d_work);
       dfree(d_lay);
     };
+    // the staged deck and layer lists stay for k_model_vjp
+    auto keep = [&]() {
+      ctx->d_mdeck = d_deck;
+      ctx->d_mlay = d_lay;
+      d_deck = nullptr;
+      d_lay = nullptr;
+    };
     hipError_t e1 = hipMalloc(&d_deck, 9 * (size_t)nknot * sizeof(double));
     if (e1 == hipSuccess) e1 = hipMalloc(&d_work, (nwork + 3 * (size_t)nknot) * sizeof(double));
     if (e1 == hipSuccess) e1 = hipMalloc(&d_lay, 2 * (size_t)nl * sizeof(int));
@@ -588,6 +607,7 @@ extern "C" int ysx_set_model(ysx_ctx* ctx, int nknot, int nic, int noc, int ians
       e1 = hipGetLastError();
     }
     if (e1 == hipSuccess) e1 = hipStreamSynchronize(ctx->stream);
+    if (e1 == hipSuccess) keep();
     cleanup();
     if (e1 != hipSuccess)
       return set_err(ctx, YSX_ERR_HIP, std::string("model transform: ") + hipGetErrorString(e1));
@@ -1680,6 +1700,61 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   return YSX_OK;
 }
 
+namespace {
+// k_model_vjp's buffers for the context's model (doubles): table adjoint
+// input [F_COUNT nk] | bar [F_COUNT nk] | aux [6 nk] | dbar [9 nk] |
+// work [12 nlayer ws]
+struct MvjpLayout {
+  size_t ntab, ws, o_bar, o_aux, o_dbar, o_work, total;
+};
+MvjpLayout mvjp_layout(const HostModel& M) {
+  MvjpLayout L;
+  const size_t nk = M.nknot, nl = M.nlayer;
+  int nmax = 0;
+  for (size_t k = 0; k < nl; k++) nmax = std::max(nmax, M.layer_hi[k] - M.layer_lo[k] + 1);
+  L.ntab = (size_t)F_COUNT * nk;
+  L.ws = 3 * ((size_t)nmax + 2);
+  L.o_bar = L.ntab;
+  L.o_aux = 2 * L.ntab;
+  L.o_dbar = L.o_aux + 6 * nk;
+  L.o_work = L.o_dbar + 9 * nk;
+  L.total = L.o_work + 12 * nl * L.ws;
+  return L;
+}
+
+// k_model_vjp on the context's model: the table adjoint at d_in (device,
+// F_COUNT x nknot; nullptr: the input part of the buffers) to the deck
+// adjoint, left at *d_out (device, nknot x 9 row-major)
+int model_vjp_deck_device(ysx_ctx* ctx, const double* d_in, double** d_out) {
+  const HostModel& M = ctx->M;
+  const MvjpLayout L = mvjp_layout(M);
+  if (!ctx->d_mvjp) HIPCHK(hipMalloc(&ctx->d_mvjp, L.total * sizeof(double)));
+  double* b = ctx->d_mvjp;
+  launch_model_vjp(dim3(1), dim3(256), 0, ctx->stream, ctx->d_mdeck, M.nknot, M.ifanis,
+                   ctx->N.vel_norm, ctx->d_mlay, ctx->d_mlay + M.nlayer, M.nlayer, ctx->d_tab,
+                   d_in ? d_in : b, b + L.o_bar, b + L.o_work, L.ws, b + L.o_aux, b + L.o_dbar);
+  HIPCHK(hipGetLastError());
+  *d_out = b + L.o_dbar;
+  return YSX_OK;
+}
+}  // namespace
+
+extern "C" int ysx_model_vjp_deck(ysx_ctx* ctx, const double* table_bar, double* deck_bar) {
+  if (!ctx || !table_bar || !deck_bar) return YSX_ERR_ARG;
+  if (!ctx->have_model || !ctx->d_mdeck) return set_err(ctx, YSX_ERR_ARG, "no model");
+  HIPCHK(hipSetDevice(ctx->device));
+  const MvjpLayout L = mvjp_layout(ctx->M);
+  if (!ctx->d_mvjp) HIPCHK(hipMalloc(&ctx->d_mvjp, L.total * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(ctx->d_mvjp, table_bar, L.ntab * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
+  double* d_out = nullptr;
+  if (int rc = model_vjp_deck_device(ctx, nullptr, &d_out)) return rc;
+  HIPCHK(hipMemcpyAsync(deck_bar, d_out, 9 * (size_t)ctx->M.nknot * sizeof(double),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return YSX_OK;
+}
+
 extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double* misfit,
                                double* knot_grad) {
   if (!ctx || !obs || !misfit || !knot_grad) return YSX_ERR_ARG;
@@ -1737,9 +1812,15 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
   const int nk = ctx->M.nknot;
   std::vector<double> tb((size_t)F_COUNT * nk, 0.0), db(9 * (size_t)nk, 0.0);
   if (int rc = ysx_model_vjp(ctx, tb.data())) return rc;
-  std::string e = model_table_vjp(ctx->N, nk, ctx->M.nic, ctx->M.noc, ctx->ians,
-                                  ctx->tref_period, ctx->deck.data(), tb.data(), db.data());
-  if (!e.empty()) return set_err(ctx, YSX_ERR_INPUT, e);
+  // the whole table adjoint (solves + source vectors) to the deck, on the device
+  if (!ctx->d_mvjp) HIPCHK(hipMalloc(&ctx->d_mvjp, mvjp_layout(ctx->M).total * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(ctx->d_mvjp, tb.data(), tb.size() * sizeof(double),
+                        hipMemcpyHostToDevice, ctx->stream));
+  double* d_db = nullptr;
+  if (int rc = model_vjp_deck_device(ctx, nullptr, &d_db)) return rc;
+  HIPCHK(hipMemcpyAsync(db.data(), d_db, db.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   for (int k = 0; k < nk; k++) {
     const double* d = db.data() + 9 * (size_t)k;
     double* g = knot_grad + 4 * (size_t)k;

@@ -217,57 +217,9 @@ std::string build_run(const Norms& N, const HostModel& M, const int* ip, const d
 
 
 // ----------------------------------------------------- model table adjoint
-// Transpose of the natural-spline map y2 = S y on one layer (natural_spline:
-// u = T^-1 B y with the tridiagonal T and the second-difference operator B
-// of spline_r / tridag_ser_r, src/module_spline.f90:40-73, 426-449):
-// ybar += B^T T^-T y2bar.
-static void natural_spline_T(const double* x, const double* y2bar, double* ybar, int n) {
-  if (n < 3) return;  // u_1 = u_n = 0: no dependence on y
-  // 1-based: h[i] = x[i] - x[i-1] (0-based x), rows 1..n of T
-  std::vector<double> h(n + 1, 0.0), sub(n + 2, 0.0), dia(n + 2, 1.0), spr(n + 2, 0.0);
-  for (int i = 1; i <= n - 1; i++) h[i] = x[i] - x[i - 1];
-  for (int i = 2; i <= n - 1; i++) {
-    sub[i] = h[i - 1];
-    dia[i] = 2.0 * (h[i] + h[i - 1]);
-    spr[i] = h[i];
-  }
-  // T^T z = y2bar: T^T has sub'(j) = spr(j-1), dia(j), spr'(j) = sub(j+1)
-  std::vector<double> c(n + 2, 0.0), z(n + 2, 0.0);
-  double bet = dia[1];
-  z[1] = y2bar[0] / bet;
-  for (int j = 2; j <= n; j++) {
-    double sb = spr[j - 1];  // sub-diagonal of T^T in row j
-    c[j] = sub[j] / bet;     // super-diagonal of T^T in row j-1, divided by the pivot
-    bet = dia[j] - sb * c[j];
-    z[j] = (y2bar[j - 1] - sb * z[j - 1]) / bet;
-  }
-  for (int j = n - 1; j >= 1; j--) z[j] -= c[j + 1] * z[j + 1];
-  // rhs[i] = 6 (D_i - D_{i-1}) for i = 2..n-1, D_i = (y[i] - y[i-1]) / h_i
-  for (int i = 2; i <= n - 1; i++) {
-    double g = 6.0 * z[i];
-    ybar[i] += g / h[i];
-    ybar[i - 1] -= g / h[i];
-    ybar[i - 1] -= g / h[i - 1];
-    ybar[i - 2] += g / h[i - 1];
-  }
-}
-
-// ybar (4 knot entries of interval i) of splint_dis_r at x:
-// s = a y[i] + b y[i+1] + ((a^3 - a) y2[i] + (b^3 - b) y2[i+1]) h^2 / 6
-static void splint_T(const double* xa, double x, int i, double sbar, double* ybar,
-                     double* y2bar) {
-  double h = xa[i + 1] - xa[i];
-  if (h == 0.0) {
-    ybar[i] += sbar;
-    return;
-  }
-  double a = (xa[i + 1] - x) / h, b = (x - xa[i]) / h, hh = h * h / 6.0;
-  ybar[i] += a * sbar;
-  ybar[i + 1] += b * sbar;
-  y2bar[i] += (a * a * a - a) * hh * sbar;
-  y2bar[i + 1] += (b * b * b - b) * hh * sbar;
-}
-
+// The transposes of build_model's steps in reverse order, through the
+// shared functions of ysx_model.hpp; k_model_vjp runs the same sequence on
+// the device.
 std::string model_table_vjp(const Norms& N, int nknot, int nic, int noc, int ians,
                             double tref_period, const double* deck, const double* tbar,
                             double* dbar) {
@@ -275,16 +227,22 @@ std::string model_table_vjp(const Norms& N, int nknot, int nic, int noc, int ian
   std::string err = build_model(N, nknot, nic, noc, ians, tref_period, deck, M);
   if (!err.empty()) return err;
   const double* tab = M.tab.data();
-  auto col = [&](int f) { return tab + (size_t)f * nknot; };
-  const double* r = col(F_R);
+  const double* r = tab + (size_t)F_R * nknot;
+  const double* rho = tab + (size_t)F_RHO * nknot;
   // adjoints of the knot values of every table field
   std::vector<double> fb(tbar, tbar + (size_t)F_COUNT * nknot);
   auto bar = [&](int f) { return fb.data() + (size_t)f * nknot; };
+  std::vector<double> work(3 * ((size_t)nknot + 2));
+  auto layer = [&](int k, int& lo, int& n) {
+    lo = M.layer_lo[k] - 1;
+    n = M.layer_hi[k] - M.layer_lo[k] + 1;
+  };
+  int lo, n;
   // gravity: grav2 = S(grav) per layer; grav[i] = 4 G[i] / r^2 (r > 0) with
   // G the running sum of the interval integrals gi of the spline of r^2 rho
   for (int k = 0; k < M.nlayer; k++) {
-    int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
-    natural_spline_T(r + lo, bar(F_GRAV2) + lo, bar(F_GRAV) + lo, n);
+    layer(k, lo, n);
+    natural_spline_T(r + lo, bar(F_GRAV2) + lo, bar(F_GRAV) + lo, n, work.data());
   }
   std::vector<double> gbar(nknot, 0.0);  // adjoint of the running sum G
   for (int i = 0; i < nknot; i++)
@@ -296,117 +254,28 @@ std::string model_table_vjp(const Norms& N, int nknot, int nic, int noc, int ian
     gib[i - 1] = acc;
   }
   std::vector<double> intg(nknot), qint(nknot), intgb(nknot, 0.0), qintb(nknot, 0.0);
-  std::vector<double> work(3 * ((size_t)nknot + 1));
-  for (int i = 0; i < nknot; i++) intg[i] = r[i] * r[i] * col(F_RHO)[i];
+  for (int i = 0; i < nknot; i++) intg[i] = r[i] * r[i] * rho[i];
   for (int k = 0; k < M.nlayer; k++) {
-    int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
+    layer(k, lo, n);
     natural_spline(r + lo, intg.data() + lo, qint.data() + lo, n, work.data());
   }
-  // grav_interval: 5-point Gauss-Legendre of the spline on interval i
-  const double gw0 = .478628670499366, gw1 = .236926885056189;
-  const double gx0 = .538469310105683, gx1 = .906179845938664, gc = .568888888888889;
-  for (int i = 0; i < nknot - 1; i++) {
-    double y1 = 0.5 * (r[i + 1] + r[i]), y2 = 0.5 * (r[i + 1] - r[i]), sb = y2 * gib[i];
-    splint_T(r, y1, i, gc * sb, intgb.data(), qintb.data());
-    splint_T(r, y1 + gx0 * y2, i, gw0 * sb, intgb.data(), qintb.data());
-    splint_T(r, y1 - gx0 * y2, i, gw0 * sb, intgb.data(), qintb.data());
-    splint_T(r, y1 + gx1 * y2, i, gw1 * sb, intgb.data(), qintb.data());
-    splint_T(r, y1 - gx1 * y2, i, gw1 * sb, intgb.data(), qintb.data());
-  }
+  for (int i = 0; i < nknot - 1; i++) grav_interval_T(r, i, gib[i], intgb.data(), qintb.data());
   for (int k = 0; k < M.nlayer; k++) {
-    int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
-    natural_spline_T(r + lo, qintb.data() + lo, intgb.data() + lo, n);
+    layer(k, lo, n);
+    natural_spline_T(r + lo, qintb.data() + lo, intgb.data() + lo, n, work.data());
   }
   for (int i = 0; i < nknot; i++) bar(F_RHO)[i] += r[i] * r[i] * intgb[i];
   // the 12 splined fields: y2 = S(y) per layer
   for (int k = 0; k < M.nlayer; k++) {
-    int lo = M.layer_lo[k] - 1, n = M.layer_hi[k] - M.layer_lo[k] + 1;
+    layer(k, lo, n);
     for (int q = 0; q < 12; q++) {
       int f = spline_field(q);
-      natural_spline_T(r + lo, bar(f + 1) + lo, bar(f) + lo, n);
+      natural_spline_T(r + lo, bar(f + 1) + lo, bar(f) + lo, n, work.data());
     }
   }
-  // knot_fields, per knot (radii are not differentiated)
-  const bool ifanis = (ians == 1);
-  for (int i = 0; i < nknot; i++) {
-    const double* d = deck + 9 * (size_t)i;
-    double* db = dbar + 9 * (size_t)i;
-    for (int c = 0; c < 9; c++) db[c] = 0.0;
-    double rho = d[1] / RHO_NORM_M;
-    double Av = ifanis ? d[6] : d[2], Cv = d[2], Lv = d[3], Nv = ifanis ? d[7] : d[3];
-    double F0 = ifanis ? d[8] : 1.0, qk = d[4], qm = d[5], vn = N.vel_norm;
-    double vA = Av / vn, vC = Cv / vn, vL = Lv / vn, vN = Nv / vn;
-    double A = rho * (vA * vA), Cc = rho * (vC * vC), L = rho * (vL * vL), Nn = rho * (vN * vN);
-    double F = (A - 2.0 * L) * F0;
-    double kap = (Cc + 4.0 * A - 4.0 * Nn + 4.0 * F) / 9.0;
-    double mu = (Cc + A + 6.0 * L + 5.0 * Nn - 2.0 * F) / 15.0;
-    double lam = kap - 2.0 * mu / 3.0;
-    double qk1 = (qk > 0.0) ? 1.0 / qk : qk, qm1 = (qm > 0.0) ? 1.0 / qm : qm;
-    double D = lam + 2.0 * mu, rat = 4.0 * mu / (3.0 * D);
-    double num = (1.0 - rat) * qk1 - 0.5 * rat * qm1, den = 1.0 - 1.5 * rat;
-    double rhob = bar(F_RHO)[i], Ab = bar(F_ACON)[i], Cb = bar(F_CCON)[i], Lb = bar(F_LCON)[i];
-    double Nb = bar(F_NCON)[i], Fb = bar(F_FCON)[i], kb = bar(F_KAPPA)[i], mb = bar(F_MU)[i];
-    double qkb = bar(F_QKAPPA)[i], qmb = bar(F_QMU)[i], xlb = bar(F_XLAM)[i];
-    double xab = bar(F_XA2)[i], bb = bar(F_BETA)[i], ab = bar(F_ALPHA)[i];
-    // beta = sqrt(mu / rho), alpha = sqrt((kappa + 4/3 mu) / rho); a zero
-    // shear modulus (fluid) has no finite derivative through mu
-    double beta = col(F_BETA)[i], alpha = col(F_ALPHA)[i];
-    if (beta > 0.0) {
-      mb += bb / (2.0 * beta * rho);
-      rhob -= bb * beta / (2.0 * rho);
-    }
-    if (alpha > 0.0) {
-      kb += ab / (2.0 * alpha * rho);
-      mb += 1.3333333333 * ab / (2.0 * alpha * rho);
-      rhob -= ab * alpha / (2.0 * rho);
-    }
-    // xa2 = (1 - rat) qk1 + rat qm1; xlam = num / den
-    double ratb = (qm1 - qk1) * xab;
-    qkb += (1.0 - rat) * xab;
-    qmb += rat * xab;
-    qkb += (1.0 - rat) / den * xlb;
-    qmb += -0.5 * rat / den * xlb;
-    ratb += ((-qk1 - 0.5 * qm1) * den + 1.5 * num) / (den * den) * xlb;
-    // rat = 4 mu / (3 D), D = lam + 2 mu, lam = kappa - 2 mu / 3
-    mb += ratb * 4.0 / (3.0 * D);
-    double Db = -ratb * 4.0 * mu / (3.0 * D * D);
-    double lb = Db;
-    mb += 2.0 * Db;
-    kb += lb;
-    mb += -2.0 / 3.0 * lb;
-    // mu and kappa from the moduli
-    Cb += mb / 15.0;
-    Ab += mb / 15.0;
-    Lb += 6.0 * mb / 15.0;
-    Nb += 5.0 * mb / 15.0;
-    Fb += -2.0 * mb / 15.0;
-    Cb += kb / 9.0;
-    Ab += 4.0 * kb / 9.0;
-    Nb += -4.0 * kb / 9.0;
-    Fb += 4.0 * kb / 9.0;
-    // F = (A - 2 L) F0
-    Ab += F0 * Fb;
-    Lb += -2.0 * F0 * Fb;
-    double F0b = (A - 2.0 * L) * Fb;
-    // X = rho v^2
-    rhob += vA * vA * Ab + vC * vC * Cb + vL * vL * Lb + vN * vN * Nb;
-    double vAb = 2.0 * rho * vA * Ab, vCb = 2.0 * rho * vC * Cb;
-    double vLb = 2.0 * rho * vL * Lb, vNb = 2.0 * rho * vN * Nb;
-    db[1] = rhob / RHO_NORM_M;
-    db[2] = vCb / vn;
-    db[3] = vLb / vn;
-    if (ifanis) {
-      db[6] = vAb / vn;
-      db[7] = vNb / vn;
-      db[8] = F0b;
-    } else {
-      db[2] += vAb / vn;
-      db[3] += vNb / vn;
-    }
-    // 1/Q (zero Q stays zero)
-    db[4] = (qk > 0.0) ? -qkb / (qk * qk) : qkb;
-    db[5] = (qm > 0.0) ? -qmb / (qm * qm) : qmb;
-  }
+  // knot_fields, per knot
+  for (int i = 0; i < nknot; i++)
+    knot_fields_T(deck, nknot, i, ians == 1, N.vel_norm, tab, fb.data(), dbar + 9 * (size_t)i);
   return "";
 }
 

@@ -593,6 +593,74 @@ __global__ void __launch_bounds__(256) k_model(const double* __restrict__ deck, 
                    layer_n[k], work + k * ws);
 }
 
+// ------------------------------------------------------------- k_model_vjp
+// Adjoint of k_model (one block): the table adjoint tbar (F_COUNT x nknot)
+// to the adjoints of the deck's 9 columns per knot, dbar (9 x nknot), by
+// the transposes of k_model's steps in reverse order (the sequence of the
+// host's model_table_vjp, through the same functions).  bar holds F_COUNT
+// x nknot doubles, aux 6 nknot (intg, qint, intgb, qintb, gbar, gib), work
+// ws >= 3 (largest layer_n + 2) doubles per (layer, field) solve, 12 nlayer
+// solves.  Per-layer solves run one thread each (layers hold disjoint knot
+// ranges); the running sums and the quadrature transposes, which add into
+// neighbouring knots, run on one thread in the host's order.
+__global__ void __launch_bounds__(256) k_model_vjp(const double* __restrict__ deck, int nknot,
+                                                   int ifanis, double vel_norm,
+                                                   const int* __restrict__ layer_lo,
+                                                   const int* __restrict__ layer_n, int nlayer,
+                                                   const double* __restrict__ tab,
+                                                   const double* __restrict__ tbar,
+                                                   double* __restrict__ bar,
+                                                   double* __restrict__ work, size_t ws,
+                                                   double* __restrict__ aux,
+                                                   double* __restrict__ dbar) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const size_t nk = nknot;
+  const double* r = tab + (size_t)F_R * nk;
+  const double* rho = tab + (size_t)F_RHO * nk;
+  double *intg = aux, *qint = aux + nk, *intgb = aux + 2 * nk, *qintb = aux + 3 * nk;
+  double *gbar = aux + 4 * nk, *gib = aux + 5 * nk;
+  for (size_t e = tid; e < (size_t)F_COUNT * nk; e += nt) bar[e] = tbar[e];
+  for (int i = tid; i < nknot; i += nt) {
+    intgb[i] = qintb[i] = gib[i] = 0.0;
+    intg[i] = r[i] * r[i] * rho[i];
+  }
+  __syncthreads();
+  double* bgrav = bar + (size_t)F_GRAV * nk;
+  for (int k = tid; k < nlayer; k += nt) {
+    const int lo = layer_lo[k], n = layer_n[k];
+    natural_spline_T(r + lo, bar + (size_t)F_GRAV2 * nk + lo, bgrav + lo, n, work + k * ws);
+    natural_spline(r + lo, intg + lo, qint + lo, n, work + (nlayer + k) * ws);
+  }
+  __syncthreads();
+  for (int i = tid; i < nknot; i += nt)
+    gbar[i] = (r[i] > 0.0) ? bgrav[i] * 4.0 / (r[i] * r[i]) : bgrav[i];
+  __syncthreads();
+  if (tid == 0) {
+    double acc = 0.0;
+    for (int i = nknot - 1; i >= 1; i--) {
+      acc += gbar[i];
+      gib[i - 1] = acc;
+    }
+    for (int i = 0; i < nknot - 1; i++) grav_interval_T(r, i, gib[i], intgb, qintb);
+  }
+  __syncthreads();
+  for (int k = tid; k < nlayer; k += nt)
+    natural_spline_T(r + layer_lo[k], qintb + layer_lo[k], intgb + layer_lo[k], layer_n[k],
+                     work + k * ws);
+  __syncthreads();
+  for (int i = tid; i < nknot; i += nt) bar[(size_t)F_RHO * nk + i] += r[i] * r[i] * intgb[i];
+  __syncthreads();
+  for (int t = tid; t < 12 * nlayer; t += nt) {
+    const int k = t / 12, f = spline_field(t % 12), lo = layer_lo[k];
+    natural_spline_T(r + lo, bar + (size_t)(f + 1) * nk + lo, bar + (size_t)f * nk + lo,
+                     layer_n[k], work + t * ws);
+  }
+  __syncthreads();
+  for (int i = tid; i < nknot; i += nt)
+    knot_fields_T(deck, nknot, i, ifanis, vel_norm, tab, bar, dbar + 9 * (size_t)i);
+}
+
 // ------------------------------------------------------------------ k_seis
 // ifft (src/module_fourier.f90:55-145) and src/yspec.f90:439-473 for one
 // receiver per block: pack bins i1..i2, cosine filters, Hermitian fill,
@@ -1453,6 +1521,13 @@ void launch_model(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck,
                   double* gi) {
   hipLaunchKernelGGL(k_model, g, b, lds, s, deck, nknot, ifanis, vel_norm, layer_lo, layer_n,
                      nlayer, tab, work, ws, intg, qint, gi);
+}
+void launch_model_vjp(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck, int nknot,
+                      int ifanis, double vel_norm, const int* layer_lo, const int* layer_n,
+                      int nlayer, const double* tab, const double* tbar, double* bar,
+                      double* work, size_t ws, double* aux, double* dbar) {
+  hipLaunchKernelGGL(k_model_vjp, g, b, lds, s, deck, nknot, ifanis, vel_norm, layer_lo, layer_n,
+                     nlayer, tab, tbar, bar, work, ws, aux, dbar);
 }
 void launch_fill0(dim3 g, dim3 b, size_t lds, hipStream_t s, double* p, size_t n) {
   hipLaunchKernelGGL(k_fill0, g, b, lds, s, p, n);

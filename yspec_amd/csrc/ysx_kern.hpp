@@ -292,6 +292,10 @@ void launch_model(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck,
                   int ifanis, double vel_norm, const int* layer_lo, const int* layer_n,
                   int nlayer, double* tab, double* work, size_t ws, double* intg, double* qint,
                   double* gi);
+void launch_model_vjp(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* deck, int nknot,
+                      int ifanis, double vel_norm, const int* layer_lo, const int* layer_n,
+                      int nlayer, const double* tab, const double* tbar, double* bar,
+                      double* work, size_t ws, double* aux, double* dbar);
 void launch_fill0(dim3 g, dim3 b, size_t lds, hipStream_t s, double* p, size_t n);
 void launch_seis(dim3 g, dim3 b, size_t lds, hipStream_t s, const RunK* runs, Queue Q,
                  const cd* spec, const cd* tw, cd* work, const double* azep, double* seis);

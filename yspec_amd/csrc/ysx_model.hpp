@@ -123,6 +123,167 @@ __host__ __device__ inline double grav_interval(const double* r, const double* i
   return y2 * sum;
 }
 
+// ------------------------------------------------ adjoint of the transform
+// The transposes of the steps above, shared by the host function
+// (model_table_vjp) and the device kernel (k_model_vjp) so that both
+// round alike.  The radii are not differentiated.
+
+// Transpose of natural_spline on one layer: y2 = T^-1 B y with the
+// tridiagonal T and the second-difference operator B of spline_r /
+// tridag_ser_r (src/module_spline.f90:40-73, 426-449): ybar += B^T T^-T y2bar.
+// work holds 2 (n + 2) doubles.
+__host__ __device__ inline void natural_spline_T(const double* x, const double* y2bar,
+                                                 double* ybar, int n, double* work) {
+#pragma clang fp contract(off)
+  if (n < 3) return;  // u_1 = u_n = 0: no dependence on y
+  // 1-based rows 1..n of T; h(i) = x[i] - x[i-1] (0-based x)
+  auto h = [&](int i) { return x[i] - x[i - 1]; };
+  auto sub = [&](int i) { return (i >= 2 && i <= n - 1) ? h(i - 1) : 0.0; };
+  auto dia = [&](int i) { return (i >= 2 && i <= n - 1) ? 2.0 * (h(i) + h(i - 1)) : 1.0; };
+  auto spr = [&](int i) { return (i >= 2 && i <= n - 1) ? h(i) : 0.0; };
+  double* c = work;          // [n + 2]
+  double* z = work + n + 2;  // [n + 2]
+  for (int j = 0; j < n + 2; j++) c[j] = z[j] = 0.0;
+  // T^T z = y2bar: row j of T^T has sub-diagonal spr(j-1) and
+  // super-diagonal sub(j+1)
+  double bet = dia(1);
+  z[1] = y2bar[0] / bet;
+  for (int j = 2; j <= n; j++) {
+    double sb = spr(j - 1);
+    c[j] = sub(j) / bet;
+    bet = dia(j) - sb * c[j];
+    z[j] = (y2bar[j - 1] - sb * z[j - 1]) / bet;
+  }
+  for (int j = n - 1; j >= 1; j--) z[j] -= c[j + 1] * z[j + 1];
+  // rhs[i] = 6 (D_i - D_{i-1}) for i = 2..n-1, D_i = (y[i] - y[i-1]) / h_i
+  for (int i = 2; i <= n - 1; i++) {
+    double g = 6.0 * z[i];
+    ybar[i] += g / h(i);
+    ybar[i - 1] -= g / h(i);
+    ybar[i - 1] -= g / h(i - 1);
+    ybar[i - 2] += g / h(i - 1);
+  }
+}
+
+// Transpose of splint on interval i (0-based) at x: the adjoint sbar of
+// s = a y[i] + b y[i+1] + ((a^3 - a) y2[i] + (b^3 - b) y2[i+1]) h^2 / 6
+// into ybar and y2bar
+__host__ __device__ inline void splint_T(const double* xa, double x, int i, double sbar,
+                                         double* ybar, double* y2bar) {
+#pragma clang fp contract(off)
+  double h = xa[i + 1] - xa[i];
+  if (h == 0.0) {
+    ybar[i] += sbar;
+    return;
+  }
+  double a = (xa[i + 1] - x) / h, b = (x - xa[i]) / h, hh = h * h / 6.0;
+  ybar[i] += a * sbar;
+  ybar[i + 1] += b * sbar;
+  y2bar[i] += (a * a * a - a) * hh * sbar;
+  y2bar[i + 1] += (b * b * b - b) * hh * sbar;
+}
+
+// Transpose of grav_interval on interval i: gbar (the adjoint of the
+// interval's integral) into the adjoints of the spline of r^2 rho
+__host__ __device__ inline void grav_interval_T(const double* r, int i, double gbar,
+                                                double* intgb, double* qintb) {
+#pragma clang fp contract(off)
+  const double gw0 = .478628670499366, gw1 = .236926885056189;
+  const double gx0 = .538469310105683, gx1 = .906179845938664, gc = .568888888888889;
+  double y1 = 0.5 * (r[i + 1] + r[i]), y2 = 0.5 * (r[i + 1] - r[i]), sb = y2 * gbar;
+  splint_T(r, y1, i, gc * sb, intgb, qintb);
+  splint_T(r, y1 + gx0 * y2, i, gw0 * sb, intgb, qintb);
+  splint_T(r, y1 - gx0 * y2, i, gw0 * sb, intgb, qintb);
+  splint_T(r, y1 + gx1 * y2, i, gw1 * sb, intgb, qintb);
+  splint_T(r, y1 - gx1 * y2, i, gw1 * sb, intgb, qintb);
+}
+
+// Transpose of knot_fields for knot i: the table-field adjoints bar
+// (field-major, F_COUNT x nknot, gravity and spline terms already folded
+// in) into the adjoints of the knot's 9 deck columns db[9]; tab is the
+// forward table (for alpha and beta).  A zero shear modulus (fluid) has no
+// finite derivative through beta; zero Q stays zero.
+__host__ __device__ inline void knot_fields_T(const double* deck, int nknot, int i, int ifanis,
+                                              double vel_norm, const double* tab,
+                                              const double* bar, double* db) {
+#pragma clang fp contract(off)
+  const double* d = deck + 9 * (size_t)i;
+  auto fb = [&](int f) { return bar[(size_t)f * nknot + i]; };
+  for (int c = 0; c < 9; c++) db[c] = 0.0;
+  double rho = d[1] / RHO_NORM_M;
+  double Av = ifanis ? d[6] : d[2], Cv = d[2], Lv = d[3], Nv = ifanis ? d[7] : d[3];
+  double F0 = ifanis ? d[8] : 1.0, qk = d[4], qm = d[5], vn = vel_norm;
+  double vA = Av / vn, vC = Cv / vn, vL = Lv / vn, vN = Nv / vn;
+  double A = rho * (vA * vA), Cc = rho * (vC * vC), L = rho * (vL * vL), Nn = rho * (vN * vN);
+  double F = (A - 2.0 * L) * F0;
+  double kap = (Cc + 4.0 * A - 4.0 * Nn + 4.0 * F) / 9.0;
+  double mu = (Cc + A + 6.0 * L + 5.0 * Nn - 2.0 * F) / 15.0;
+  double lam = kap - 2.0 * mu / 3.0;
+  double qk1 = (qk > 0.0) ? 1.0 / qk : qk, qm1 = (qm > 0.0) ? 1.0 / qm : qm;
+  double D = lam + 2.0 * mu, rat = 4.0 * mu / (3.0 * D);
+  double num = (1.0 - rat) * qk1 - 0.5 * rat * qm1, den = 1.0 - 1.5 * rat;
+  double rhob = fb(F_RHO), Ab = fb(F_ACON), Cb = fb(F_CCON), Lb = fb(F_LCON);
+  double Nb = fb(F_NCON), Fb = fb(F_FCON), kb = fb(F_KAPPA), mb = fb(F_MU);
+  double qkb = fb(F_QKAPPA), qmb = fb(F_QMU), xlb = fb(F_XLAM);
+  double xab = fb(F_XA2), bb = fb(F_BETA), ab = fb(F_ALPHA);
+  // beta = sqrt(mu / rho), alpha = sqrt((kappa + 4/3 mu) / rho)
+  double beta = tab[(size_t)F_BETA * nknot + i], alpha = tab[(size_t)F_ALPHA * nknot + i];
+  if (beta > 0.0) {
+    mb += bb / (2.0 * beta * rho);
+    rhob -= bb * beta / (2.0 * rho);
+  }
+  if (alpha > 0.0) {
+    kb += ab / (2.0 * alpha * rho);
+    mb += 1.3333333333 * ab / (2.0 * alpha * rho);
+    rhob -= ab * alpha / (2.0 * rho);
+  }
+  // xa2 = (1 - rat) qk1 + rat qm1; xlam = num / den
+  double ratb = (qm1 - qk1) * xab;
+  qkb += (1.0 - rat) * xab;
+  qmb += rat * xab;
+  qkb += (1.0 - rat) / den * xlb;
+  qmb += -0.5 * rat / den * xlb;
+  ratb += ((-qk1 - 0.5 * qm1) * den + 1.5 * num) / (den * den) * xlb;
+  // rat = 4 mu / (3 D), D = lam + 2 mu, lam = kappa - 2 mu / 3
+  mb += ratb * 4.0 / (3.0 * D);
+  double Db = -ratb * 4.0 * mu / (3.0 * D * D);
+  double lb = Db;
+  mb += 2.0 * Db;
+  kb += lb;
+  mb += -2.0 / 3.0 * lb;
+  // mu and kappa from the moduli
+  Cb += mb / 15.0;
+  Ab += mb / 15.0;
+  Lb += 6.0 * mb / 15.0;
+  Nb += 5.0 * mb / 15.0;
+  Fb += -2.0 * mb / 15.0;
+  Cb += kb / 9.0;
+  Ab += 4.0 * kb / 9.0;
+  Nb += -4.0 * kb / 9.0;
+  Fb += 4.0 * kb / 9.0;
+  // F = (A - 2 L) F0
+  Ab += F0 * Fb;
+  Lb += -2.0 * F0 * Fb;
+  double F0b = (A - 2.0 * L) * Fb;
+  // X = rho v^2
+  rhob += vA * vA * Ab + vC * vC * Cb + vL * vL * Lb + vN * vN * Nb;
+  double vAb = 2.0 * rho * vA * Ab, vCb = 2.0 * rho * vC * Cb;
+  double vLb = 2.0 * rho * vL * Lb, vNb = 2.0 * rho * vN * Nb;
+  db[1] = rhob / RHO_NORM_M;
+  db[2] = vCb / vn;
+  db[3] = vLb / vn;
+  if (ifanis) {
+    db[6] = vAb / vn;
+    db[7] = vNb / vn;
+    db[8] = F0b;
+  } else {
+    db[2] += vAb / vn;
+    db[3] += vNb / vn;
+  }
+  db[4] = (qk > 0.0) ? -qkb / (qk * qk) : qkb;
+  db[5] = (qm > 0.0) ? -qmb / (qm * qm) : qmb;
+}
+
 // legendre, src/module_function.f90:6-132.  X_lm, dX_lm/dtheta and
 // X_lm/sin(theta) for orders 0..m at colatitude theta (ct = cos theta,
 // st = sin theta); the reference takes sfl3, f1, f2 and the second
