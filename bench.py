@@ -114,6 +114,57 @@ def cpu_baseline_all_cores(deck, sample, nproc):
     return pairs / wall, pairs, wall, len(procs)
 
 
+def cpu_model():
+    """the host CPU's model name"""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+REF_YSPEC = os.path.join(ROOT, "oracle", "_ref", "yspec")
+
+
+def cfg1_compare(deck, steps, warmup, cpu=True):
+    """The lmax = 150, fmax = 10 mHz workload (60 min at 10 s, 100 km
+    source, 10 receivers, no attenuation) in full: the reference program
+    (oracle/_ref/yspec, one core, its own parameter file and deck, output
+    files included) against full GPU forwards."""
+    p = workload("cfg1")
+    g = api.YSpec(deck, p, device=0)
+    for _ in range(warmup):
+        g.forward(copy=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.forward(copy=False)
+    sec = (time.perf_counter() - t0) / steps
+    out = {"workload": "lmax=150, fmax=10 mHz, 60 min, dt=10 s, 100 km source, 10 receivers, "
+                       "self-gravitation, no attenuation",
+           "pairs": g.npair, "gpu_ms_per_forward": sec * 1e3,
+           "gpu_solves_per_s": g.npair / sec, "gpu_seismograms_per_s": p.nr / sec}
+    g.close()
+    if cpu and os.path.exists(REF_YSPEC):
+        with tempfile.TemporaryDirectory() as td:
+            with open(os.path.join(td, "synth.200"), "w") as f:
+                f.write(deck.to_text())
+            with open(os.path.join(td, "in.txt"), "w") as f:
+                f.write(workload("cfg1", model_file="synth.200").to_text())
+            t0 = time.perf_counter()
+            subprocess.run([REF_YSPEC, "in.txt"], cwd=td, check=True, capture_output=True)
+            wall = time.perf_counter() - t0
+        out["cpu_reference_s"] = wall
+        out["cpu_solves_per_s"] = out["pairs"] / wall
+        out["cpu_seismograms_per_s"] = p.nr / wall
+        out["cpu_cores"] = 1
+        out["cpu_note"] = ("reference yspec (amdflang -O3) run once in full on one core, wall "
+                           "time including its parameter-file, deck and output-file I/O")
+    return out
+
+
 def batched_compare(deck, steps, warmup, do_tune=True):
     """The 64-event workload through the plain path (events one after
     another, each with its own launches) and the batched path (all events
@@ -138,10 +189,6 @@ def batched_compare(deck, steps, warmup, do_tune=True):
               flush=True)
         res[name] = {"ms_per_step": sec * 1e3, "solves_per_s": P / sec,
                      "seismograms_per_s": nrec / sec}
-    counts = g.step_counts()
-    rows = {brow_rows(e) for e in ev}
-    assert len(rows) == 1, "events with and without the potential/tilt correction"
-    ode = ode_flops(counts, brows=rows.pop())
     # per-event moment-tensor gradients after the batched forward (the
     # records are that forward's): kernel time of the three adjoint stages
     rng = np.random.default_rng(11)
@@ -159,12 +206,6 @@ def batched_compare(deck, steps, warmup, do_tune=True):
         "events": len(ev), "pairs": P, "receivers": nrec,
         "plain": res["plain"], "batched": res["batched"],
         "speedup": res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"],
-        # the events' own ODE work (what the plain path computes) over the
-        # batched forward's time: an effective rate, since the batched path
-        # marches the b-vector rows of events with a common receiver depth
-        # and frequency grid once for all of them
-        "batched_ode_tflops_effective": sum(ode.values()) / (res["batched"]["ms_per_step"]
-                                                             * 1e-3) / 1e12,
         "tuning": tuning["best"] if tuning else None,
         "moment_tensor_gradients": {
             "kernel_ms": grad_ms,
@@ -389,7 +430,7 @@ def main():
         "ode_roofline_frac": ode_flop / (ode_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
         "step_counts": {k: int(v) for k, v in zip(("tor", "solid", "fluid", "bvec", "rad"),
                                                   counts)},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved,
+        "roofline": {"bound": "fp64_vector", "kernel": dom, "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC_BYTES.get(dom) if args.lmax is None else None,
@@ -420,6 +461,8 @@ def main():
         line["inversion"] = inversion_compare(deck, 1, 1)
     if not args.no_batched and args.lmax is None:
         line["batched"] = batched_compare(deck, args.steps, args.warmup)
+    if args.lmax is None:
+        line["cfg1"] = cfg1_compare(deck, args.steps, args.warmup, cpu=not args.no_cpu)
     if not args.no_cpu:
         sample = list(range(0, p.lmax + 1, 30))
         cb = cpu_baseline(deck, p, sample)
@@ -427,8 +470,12 @@ def main():
             rate, pairs, secs, wall = cb
             line["cpu_baseline"] = {
                 "value": rate, "unit": "(l,ω) solves/s", "cores": 1, "kind": "reference",
-                "sample": f"reference yspec (amdflang -O2) on l in {sample}, all {g.nw} w, "
+                "sample": f"reference yspec (amdflang -O3) on l in {sample}, all {g.nw} w, "
                           f"{nr} receivers: {pairs} pairs in {secs:.1f} s",
+                "cpu_model": cpu_model(),
+                # the full forward's solves at this rate (the sum over l and
+                # the IFFT take well under 1% of the reference's time)
+                "seismograms_per_s_extrapolated": nr / (P / rate),
             }
         nproc = max(1, min(16, os.cpu_count() or 1))
         sample_all = list(range(0, p.lmax + 1, 5))
@@ -437,7 +484,7 @@ def main():
             rate, pairs, wall, used = ca
             line["cpu_baseline_all_cores"] = {
                 "value": rate, "unit": "(l,ω) solves/s", "cores": used, "kind": "reference",
-                "sample": f"reference yspec (amdflang -O2), {used} processes, l = 0, 5, ..., "
+                "sample": f"reference yspec (amdflang -O3), {used} processes, l = 0, 5, ..., "
                           f"{p.lmax} dealt out in turn, all {g.nw} w, {nr} receivers: "
                           f"{pairs} pairs in {wall:.1f} s wall",
             }

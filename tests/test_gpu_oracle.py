@@ -3,8 +3,9 @@ properties at the full lmax = 600, fmax = 40 mHz workload.
 
 * the whole lmax = 150, fmax = 10 mHz workload (7,852 pairs, 10 receivers):
   spectra <= 1e-9, seismograms <= 1e-8 relative L2 against the oracle;
-* the full 787,310-pair workload: per-pair solver outputs of a spread
-  sample of (l, w), highest l and w included, against the oracle's;
+* the full 787,310-pair workload: spectra and seismograms against the
+  oracle's (16 host threads); per-pair solver outputs of a spread sample of
+  (l, w), highest l and w included, against the oracle's;
   linearity of the spectra in the moment tensor; run-to-run identity."""
 import numpy as np
 import pytest
@@ -68,6 +69,29 @@ def test_cfg2_pairs_vs_oracle(cfg2, oracle_mod):
         e = rel_l2(a, b)
         print("cfg2 pairs", k, f"{e:.2e}", flush=True)
         assert e <= 1e-9, k
+
+
+@pytest.mark.timeout(600)
+def test_cfg2_full_vs_oracle(cfg2, oracle_mod):
+    """The whole lmax = 600, fmax = 40 mHz workload (787,310 pairs, 200
+    receivers) against the oracle run on 16 host threads: spectra <= 1e-9,
+    seismograms <= 1e-8 relative L2."""
+    import time
+
+    deck, p, g = cfg2
+    gs = g.spectra()
+    seis = g.seismograms()
+    o = oracle_mod.Oracle(deck, p)
+    t0 = time.perf_counter()
+    os_ = o.spectra(nthreads=16)
+    print(f"cfg2 oracle spectra: {time.perf_counter() - t0:.1f} s on 16 threads", flush=True)
+    for a, b, key in zip(gs, os_, ("ur", "ut", "up")):
+        e = rel_l2(a, b)
+        print("cfg2 full", key, f"{e:.2e}", flush=True)
+        assert e <= 1e-9, key
+    e = rel_l2(seis, o.seismograms(*os_))
+    print("cfg2 full seis", f"{e:.2e}", flush=True)
+    assert e <= 1e-8
 
 
 def test_cfg2_pairs_vs_reference(cfg2):
