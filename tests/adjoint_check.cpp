@@ -417,6 +417,32 @@ static int main2(Model& M) {
   return 0;
 }
 
+// The z-streamed bilinear forms (solid_minor_bil_s, ...) give the same
+// per-lane base adjoints as the full forms: ob(full) against ob(streamed)
+// for random rows, columns, bases and degrees.
+template <class Abar, int N, int NOB, class Full, class Stream, class Ob>
+static double check_streamed(Full full, Stream stream, Ob ob) {
+  double worst = 0.0;
+  for (int trial = 0; trial < 50; trial++) {
+    cd w[14], z[14];
+    double base[20];
+    for (int i = 0; i < 14; i++) { w[i] = CU(); z[i] = CU(); }
+    for (int i = 0; i < 20; i++) base[i] = U();
+    LS L = l_state(2 + trial * 13);
+    Abar B1, B2;
+    zero(B1);
+    zero(B2);
+    full(w, z, B1);
+    stream(w, [&](int q) { return z[q]; }, B2);
+    double o1[NOB], o2[NOB], scale = 0.0;
+    ob(B1, L, base, o1);
+    ob(B2, L, base, o2);
+    for (int v = 0; v < NOB; v++) scale = std::fmax(scale, std::fabs(o1[v]));
+    for (int v = 0; v < NOB; v++) worst = std::fmax(worst, rel(o1[v], o2[v], scale));
+  }
+  return worst;
+}
+
 int main(int argc, char** argv) {
   if (argc < 3) return 2;
   Model M;
@@ -434,6 +460,23 @@ int main(int argc, char** argv) {
          check_bilinear([](const SolidA& A, const cd* z, cd* d) {
            bvec_derivs(A, z, NoInit(), [&](int i, cd v) { d[i] = v; });
          }, [](const cd* w, const cd* z, SolidAbar& B) { bvec_bil(w, z, B); }));
+
+  printf("streamed_solid_minor %.3e\n", check_streamed<SolidAbar, 14, SOLID_OB>(
+      [](const cd* w, const cd* z, SolidAbar& B) { solid_minor_bil(w, z, B); },
+      [](const cd* w, auto zl, SolidAbar& B) { solid_minor_bil_s(w, zl, B); },
+      [](const SolidAbar& B, const LS& L, const double* b, double* o) { solid_ob(B, L, b, o); }));
+  printf("streamed_bvec %.3e\n", check_streamed<SolidAbar, 14, SOLID_OB>(
+      [](const cd* w, const cd* z, SolidAbar& B) { bvec_bil(w, z, B); },
+      [](const cd* w, auto zl, SolidAbar& B) { bvec_bil_s(w, zl, B); },
+      [](const SolidAbar& B, const LS& L, const double* b, double* o) { solid_ob(B, L, b, o); }));
+  printf("streamed_fluid_minor %.3e\n", check_streamed<FluidAbar, 5, FLUID_OB>(
+      [](const cd* w, const cd* z, FluidAbar& B) { fluid_minor_bil(w, z, B); },
+      [](const cd* w, auto zl, FluidAbar& B) { fluid_minor_bil_s(w, zl, B); },
+      [](const FluidAbar& B, const LS& L, const double* b, double* o) { fluid_ob(B, L, b, o); }));
+  printf("streamed_tor %.3e\n", check_streamed<TorAbar, 2, TOR_OB>(
+      [](const cd* w, const cd* z, TorAbar& B) { tor_bil(w, z, B); },
+      [](const cd* w, auto zl, TorAbar& B) { tor_bil_s(w, zl, B); },
+      [](const TorAbar& B, const LS& L, const double* b, double* o) { tor_ob(B, L, b, o); }));
 
   printf("transpose_solid_minor %.3e\n", check_transpose<SolidMinorM, MinorTM>(14, 20));
   printf("transpose_bvec %.3e\n", check_transpose<BvecM, BvecTM>(14, 20));

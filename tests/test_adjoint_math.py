@@ -38,7 +38,8 @@ def results(tmp_path_factory):
     return {k: float(v) for k, v in (line.split() for line in out.splitlines())}
 
 
-IDENTITIES = ["bilinear_solid_minor", "bilinear_bvec", "transpose_solid_minor",
+IDENTITIES = ["bilinear_solid_minor", "bilinear_bvec", "streamed_solid_minor", "streamed_bvec",
+              "streamed_fluid_minor", "streamed_tor", "transpose_solid_minor",
               "transpose_bvec", "transpose_fluid_minor", "transpose_tor",
               "step_transpose_solid_minor", "step_transpose_bvec",
               "step_transpose_fluid_minor", "step_transpose_tor", "rad_segments"]

@@ -87,6 +87,224 @@ __host__ __device__ inline void bvec_bil(const cd* w, const cd* z, SolidAbar& B)
   B.k33 = B.k33 + ((W(12, 9) - W(4, 1)) + (W(13, 10) - W(8, 5)));
 }
 
+// The same forms with z streamed: zl(q) returns component q of z (a tape
+// read), and only the entries solid_ob reads are formed (t21, t22, t33 and
+// k33 do not depend on the base; s13, s23 and t31 enter through their real
+// parts).  Each distinct product w_p z_q is taken once.  tests/
+// adjoint_check.cpp checks them against the full forms.
+template <class ZL>
+__host__ __device__ inline void solid_minor_bil_s(const cd* w, ZL&& zl, SolidAbar& B) {
+  {
+    const cd z = zl(0);
+    { const cd P = cmul_(w[0], z); B.t11 = B.t11 + P; }
+    { const double P = w[1].x * z.x - w[1].y * z.y; B.s13.x = B.s13.x + P; }
+    { const double P = w[2].x * z.x - w[2].y * z.y; B.s23.x = B.s23.x + P; }
+    { const cd P = cmul_(w[4], z); B.s12 = B.s12 - P; }
+    { const cd P = cmul_(w[5], z); B.s22 = B.s22 - P; }
+    { const cd P = cmul_(w[9], z); B.s11 = B.s11 + P; }
+  }
+  {
+    const cd z = zl(1);
+    { const cd P = cmul_(w[2], z); B.t12 = B.t12 - P; }
+    { const cd P = cmul_(w[6], z); B.s22 = B.s22 - P; }
+    { const double P = w[7].x * z.x - w[7].y * z.y; B.s23.x = B.s23.x - P; }
+    { const double P = w[9].x * z.x - w[9].y * z.y; B.t31.x = B.t31.x + -2.0 * P; }
+    { const cd P = cmul_(w[10], z); B.s12 = B.s12 + P; }
+    { const double P = w[11].x * z.x - w[11].y * z.y; B.s13.x = B.s13.x + 2.0 * P; }
+  }
+  {
+    const cd z = zl(2);
+    { const cd P = cmul_(w[2], z); B.t11 = B.t11 + P; }
+    { const double P = w[4].x * z.x - w[4].y * z.y; B.t31.x = B.t31.x + P; }
+    { const cd P = cmul_(w[6], z); B.s12 = B.s12 + P; }
+    { const double P = w[7].x * z.x - w[7].y * z.y; B.s13.x = B.s13.x - P; }
+    { const double P = w[8].x * z.x - w[8].y * z.y; B.s23.x = B.s23.x + -2.0 * P; }
+    { const cd P = cmul_(w[10], z); B.s11 = B.s11 - P; }
+  }
+  {
+    const cd z = zl(3);
+    { const double P = w[1].x * z.x - w[1].y * z.y; B.t31.x = B.t31.x - P; }
+    { const cd P = cmul_(w[3], z); B.t11 = B.t11 + P; }
+    { const cd P = cmul_(w[7], z); B.s12 = B.s12 + P; }
+    { const cd P = cmul_(w[8], z); B.s22 = B.s22 + P; }
+    { const cd P = cmul_(w[11], z); B.s11 = B.s11 - P; }
+  }
+  {
+    const cd z = zl(4);
+    { const cd P = cmul_(w[5], z); B.t12 = B.t12 + -2.0 * P; }
+    { const double P = w[6].x * z.x - w[6].y * z.y; B.s23.x = B.s23.x - P; }
+    { const double P = w[10].x * z.x - w[10].y * z.y; B.s13.x = B.s13.x - P; }
+    { const cd P = cmul_(w[12], z); B.s12 = B.s12 + 2.0 * P; }
+  }
+  {
+    const cd z = zl(5);
+    { const cd P = cmul_(w[0], z); B.k22 = B.k22 - P; }
+    { const cd P = cmul_(w[5], z); B.t11 = B.t11 + P; }
+    { const double P = w[6].x * z.x - w[6].y * z.y; B.s13.x = B.s13.x + P; }
+    { const cd P = cmul_(w[12], z); B.s11 = B.s11 - P; }
+  }
+  {
+    const cd z = zl(6);
+    { const cd P = cmul_(w[1], z); B.k22 = B.k22 - P; }
+    { const double P = w[12].x * z.x - w[12].y * z.y; B.t31.x = B.t31.x + 2.0 * P; }
+    { const double P = w[13].x * z.x - w[13].y * z.y; B.s13.x = B.s13.x + 2.0 * P; }
+  }
+  {
+    const cd z = zl(7);
+    { const cd P = cmul_(w[8], z); B.t12 = B.t12 + -2.0 * P; }
+    { const double P = w[10].x * z.x - w[10].y * z.y; B.t31.x = B.t31.x - P; }
+    { const cd P = cmul_(w[13], z); B.s12 = B.s12 + -2.0 * P; }
+  }
+  {
+    const cd z = zl(8);
+    { const cd P = cmul_(w[3], z); B.k22 = B.k22 + P; }
+    { const double P = w[6].x * z.x - w[6].y * z.y; B.t31.x = B.t31.x + P; }
+    { const cd P = cmul_(w[8], z); B.t11 = B.t11 + P; }
+    { const cd P = cmul_(w[13], z); B.s11 = B.s11 + P; }
+  }
+  {
+    const cd z = zl(9);
+    { const cd P = cmul_(w[0], z); B.k11 = B.k11 + P; }
+    { const cd P = cmul_(w[4], z); B.t12 = B.t12 + P; }
+    { const cd P = cmul_(w[9], z); B.t11 = B.t11 - P; }
+    { const double P = w[10].x * z.x - w[10].y * z.y; B.s23.x = B.s23.x - P; }
+    { const cd P = cmul_(w[12], z); B.s22 = B.s22 + P; }
+  }
+  {
+    const cd z = zl(10);
+    { const cd P = cmul_(w[2], z); B.k11 = B.k11 - P; }
+    { const cd P = cmul_(w[6], z); B.t12 = B.t12 + P; }
+    { const cd P = cmul_(w[10], z); B.t11 = B.t11 - P; }
+    { const double P = w[13].x * z.x - w[13].y * z.y; B.s23.x = B.s23.x + 2.0 * P; }
+  }
+  {
+    const cd z = zl(11);
+    { const cd P = cmul_(w[3], z); B.k11 = B.k11 - P; }
+    { const cd P = cmul_(w[7], z); B.t12 = B.t12 + P; }
+    { const cd P = cmul_(w[11], z); B.t11 = B.t11 - P; }
+    { const cd P = cmul_(w[13], z); B.s22 = B.s22 - P; }
+  }
+  {
+    const cd z = zl(12);
+    { const cd P = cmul_(w[5], z); B.k11 = B.k11 - P; }
+    { const cd P = cmul_(w[9], z); B.k22 = B.k22 + P; }
+    { const cd P = cmul_(w[12], z); B.t11 = B.t11 - P; }
+  }
+  {
+    const cd z = zl(13);
+    { const cd P = cmul_(w[8], z); B.k11 = B.k11 + P; }
+    { const cd P = cmul_(w[11], z); B.k22 = B.k22 - P; }
+    { const cd P = cmul_(w[13], z); B.t11 = B.t11 - P; }
+  }
+}
+
+template <class ZL>
+__host__ __device__ inline void bvec_bil_s(const cd* w, ZL&& zl, SolidAbar& B) {
+  {
+    const cd z = zl(0);
+    { const cd P = cmul_(w[0], z); B.t11 = B.t11 - P; }
+    { const cd P = cmul_(w[3], z); B.k22 = B.k22 - P; }
+    { const cd P = cmul_(w[6], z); B.k11 = B.k11 + P; }
+  }
+  {
+    const cd z = zl(1);
+    { const cd P = cmul_(w[1], z); B.t11 = B.t11 - P; }
+    { const cd P = cmul_(w[5], z); B.t12 = B.t12 - P; }
+    { const cd P = cmul_(w[9], z); B.k11 = B.k11 + P; }
+  }
+  {
+    const cd z = zl(2);
+    { const cd P = cmul_(w[0], z); B.s12 = B.s12 - P; }
+    { const double P = w[1].x * z.x - w[1].y * z.y; B.s13.x = B.s13.x + -2.0 * P; }
+    { const double P = w[4].x * z.x - w[4].y * z.y; B.t31.x = B.t31.x + 2.0 * P; }
+    { const double P = w[5].x * z.x - w[5].y * z.y; B.s23.x = B.s23.x - P; }
+    { const cd P = cmul_(w[6], z); B.t12 = B.t12 - P; }
+  }
+  {
+    const cd z = zl(3);
+    { const cd P = cmul_(w[0], z); B.s22 = B.s22 - P; }
+    { const double P = w[1].x * z.x - w[1].y * z.y; B.s23.x = B.s23.x - P; }
+    { const cd P = cmul_(w[2], z); B.t12 = B.t12 + P; }
+    { const cd P = cmul_(w[3], z); B.t11 = B.t11 - P; }
+    { const cd P = cmul_(w[7], z); B.t12 = B.t12 - P; }
+    { const cd P = cmul_(w[11], z); B.k11 = B.k11 - P; }
+  }
+  {
+    const cd z = zl(4);
+    { const double P = w[0].x * z.x - w[0].y * z.y; B.s23.x = B.s23.x - P; }
+    { const cd P = cmul_(w[4], z); B.t11 = B.t11 - P; }
+    { const cd P = cmul_(w[8], z); B.t12 = B.t12 - P; }
+    { const cd P = cmul_(w[12], z); B.k11 = B.k11 - P; }
+  }
+  {
+    const cd z = zl(5);
+    { const double P = w[0].x * z.x - w[0].y * z.y; B.t31.x = B.t31.x + P; }
+    { const cd P = cmul_(w[10], z); B.k22 = B.k22 + P; }
+  }
+  {
+    const cd z = zl(6);
+    { const cd P = cmul_(w[0], z); B.s11 = B.s11 + P; }
+    { const double P = w[5].x * z.x - w[5].y * z.y; B.s13.x = B.s13.x - P; }
+    { const cd P = cmul_(w[6], z); B.t11 = B.t11 + P; }
+    { const double P = w[8].x * z.x - w[8].y * z.y; B.t31.x = B.t31.x + P; }
+    { const cd P = cmul_(w[11], z); B.k22 = B.k22 + P; }
+  }
+  {
+    const cd z = zl(7);
+    { const cd P = cmul_(w[0], z); B.s12 = B.s12 + P; }
+    { const double P = w[1].x * z.x - w[1].y * z.y; B.s13.x = B.s13.x - P; }
+    { const double P = w[4].x * z.x - w[4].y * z.y; B.t31.x = B.t31.x + P; }
+    { const double P = w[5].x * z.x - w[5].y * z.y; B.s23.x = B.s23.x + -2.0 * P; }
+    { const cd P = cmul_(w[6], z); B.t12 = B.t12 + P; }
+  }
+  {
+    const cd z = zl(8);
+    { const double P = w[0].x * z.x - w[0].y * z.y; B.s13.x = B.s13.x + P; }
+    { const cd P = cmul_(w[13], z); B.k22 = B.k22 - P; }
+  }
+  {
+    const cd z = zl(9);
+    { const cd P = cmul_(w[1], z); B.s11 = B.s11 + P; }
+    { const double P = w[2].x * z.x - w[2].y * z.y; B.t31.x = B.t31.x - P; }
+    { const cd P = cmul_(w[5], z); B.s12 = B.s12 + P; }
+    { const cd P = cmul_(w[9], z); B.t11 = B.t11 + P; }
+  }
+  {
+    const cd z = zl(10);
+    { const cd P = cmul_(w[1], z); B.s12 = B.s12 + P; }
+    { const double P = w[3].x * z.x - w[3].y * z.y; B.t31.x = B.t31.x - P; }
+    { const cd P = cmul_(w[5], z); B.s22 = B.s22 + P; }
+    { const cd P = cmul_(w[9], z); B.t12 = B.t12 + P; }
+  }
+  {
+    const cd z = zl(11);
+    { const cd P = cmul_(w[2], z); B.s12 = B.s12 + P; }
+    { const cd P = cmul_(w[3], z); B.s11 = B.s11 - P; }
+    { const cd P = cmul_(w[6], z); B.s22 = B.s22 + P; }
+    { const cd P = cmul_(w[7], z); B.s12 = B.s12 - P; }
+    { const double P = w[9].x * z.x - w[9].y * z.y; B.s23.x = B.s23.x + P; }
+    { const double P = w[10].x * z.x - w[10].y * z.y; B.s13.x = B.s13.x - P; }
+    { const cd P = cmul_(w[11], z); B.t11 = B.t11 + P; }
+    { const double P = w[13].x * z.x - w[13].y * z.y; B.t31.x = B.t31.x - P; }
+  }
+  {
+    const cd z = zl(12);
+    { const double P = w[2].x * z.x - w[2].y * z.y; B.s13.x = B.s13.x + P; }
+    { const cd P = cmul_(w[4], z); B.s11 = B.s11 - P; }
+    { const double P = w[6].x * z.x - w[6].y * z.y; B.s23.x = B.s23.x + P; }
+    { const cd P = cmul_(w[8], z); B.s12 = B.s12 - P; }
+    { const cd P = cmul_(w[12], z); B.t11 = B.t11 + P; }
+  }
+  {
+    const cd z = zl(13);
+    { const double P = w[3].x * z.x - w[3].y * z.y; B.s13.x = B.s13.x + P; }
+    { const cd P = cmul_(w[4], z); B.s12 = B.s12 - P; }
+    { const double P = w[7].x * z.x - w[7].y * z.y; B.s23.x = B.s23.x + P; }
+    { const cd P = cmul_(w[8], z); B.s22 = B.s22 - P; }
+    { const cd P = cmul_(w[12], z); B.t12 = B.t12 + P; }
+  }
+}
+
 // Per-lane map from the solid entry adjoints to adjoints of the stored
 // base doubles that depend on the model (SolidB::amat builds A from the
 // base b and the lane's degree): ob[0..15] the holomorphic adjoints of the
@@ -129,6 +347,41 @@ __host__ __device__ inline void fluid_minor_bil(const cd* w, const cd* z, FluidA
   B.s12 = B.s12 + (W(1, 0) - 2.0 * W(4, 1));
   B.s22 = B.s22 + (W(2, 0) - W(4, 3));
 }
+// fluid_minor_bil with z streamed, forming the entries fluid_ob reads
+// (t21 through its real part)
+template <class ZL>
+__host__ __device__ inline void fluid_minor_bil_s(const cd* w, ZL&& zl, FluidAbar& B) {
+  {
+    const cd z = zl(0);
+    { const cd P = cmul_(w[0], z); B.t11 = B.t11 + P; }
+    { const cd P = cmul_(w[1], z); B.s12 = B.s12 + P; }
+    { const cd P = cmul_(w[2], z); B.s22 = B.s22 + P; }
+    { const cd P = cmul_(w[3], z); B.s11 = B.s11 - P; }
+  }
+  {
+    const cd z = zl(1);
+    { const double P = w[3].x * z.x - w[3].y * z.y; B.t21.x = B.t21.x + 2.0 * P; }
+    { const cd P = cmul_(w[4], z); B.s12 = B.s12 + -2.0 * P; }
+  }
+  {
+    const cd z = zl(2);
+    { const double P = w[1].x * z.x - w[1].y * z.y; B.t21.x = B.t21.x - P; }
+    { const cd P = cmul_(w[2], z); B.t11 = B.t11 + P; }
+    { const cd P = cmul_(w[4], z); B.s11 = B.s11 + P; }
+  }
+  {
+    const cd z = zl(3);
+    { const cd P = cmul_(w[0], z); B.k11 = B.k11 - P; }
+    { const cd P = cmul_(w[3], z); B.t11 = B.t11 - P; }
+    { const cd P = cmul_(w[4], z); B.s22 = B.s22 - P; }
+  }
+  {
+    const cd z = zl(4);
+    { const cd P = cmul_(w[2], z); B.k11 = B.k11 + P; }
+    { const cd P = cmul_(w[4], z); B.t11 = B.t11 - P; }
+  }
+}
+
 // fluid base adjoints per lane: ob[0..1] fa, [2..3] cccon, [4..5] s11a,
 // [6..7] s11b (holomorphic), ob[8] rho, ob[9] t21 (base 12); fb = h/(r w)^2
 // does not depend on the model
@@ -157,6 +410,13 @@ __host__ __device__ inline void tor_bil(const cd* w, const cd* z, TorAbar& B) {
   B.t = B.t + (cmul_(w[0], z[0]) - cmul_(w[1], z[1]));
   B.k = B.k + cmul_(w[0], z[1]);
   B.sv = B.sv + cmul_(w[1], z[0]);
+}
+// tor_bil with z streamed (the entries tor_ob reads)
+template <class ZL>
+__host__ __device__ inline void tor_bil_s(const cd* w, ZL&& zl, TorAbar& B) {
+  const cd z0 = zl(0), z1 = zl(1);
+  B.k = B.k + cmul_(w[0], z1);
+  B.sv = B.sv + cmul_(w[1], z0);
 }
 // toroidal base adjoints per lane: ob[0..1] k, [2..3] sv0, [4..5] mw
 constexpr int TOR_OB = 6;

@@ -8,17 +8,22 @@
 // carries (the solution) with a row carried the other way (the adjoint).
 // A wave keeps the forward's lockstep layout (64 lanes at one w, stage
 // radii shared) and for each item:
-//   1. marches the stored side again from its start, writing its state at
-//      every knot interval boundary to the block's checkpoint slot;
-//   2. goes through the intervals in the other direction: per interval the
-//      stored side is recomputed from its checkpoint with every stage's
-//      input (or row) written to the block's tape, then the other side is
-//      stepped through the interval, each stage paired with the tape
-//      (ysx_adjoint.hpp) into adjoints of the stage's base;
+//   1. marches the stored side again from its start, writing its state
+//      before every step to the block's per-step storage;
+//   2. goes through the steps in the other direction: per step the stored
+//      side's step is replayed from its stored state with every stage
+//      input written to the block's step scratch, the other side takes its
+//      step with every stage row written beside them, and each stage's
+//      row is paired with its input (ysx_adjoint.hpp) into adjoints of the
+//      stage's base (the next stage's vectors load while one is paired);
 //   3. per chunk of steps, sums those adjoints over the wave (LDS), takes
 //      them through the base computation to the spline values and the
-//      interval's table entries (one lane per stage radius), and adds them
-//      to the table adjoint at the end of the interval (global atomics).
+//      interval's table entries (one lane per stage radius), and at the end
+//      of the interval adds them to the block's column of partial table
+//      adjoints.
+// The step scratch is read and written through a buffer resource (scalar
+// and immediate offsets), so the replay keeps the registers of the
+// forward's step; only the stored side's 14-vectors are per step.
 // Kernels:
 //   k_brow_adj   b-vector system: the receiver-side rows (stored side) and
 //                the source-side columns weighted by the record adjoints
@@ -45,6 +50,8 @@ struct SysSolidMinor {
   using Abar = SolidAbar;
   static constexpr int NOB = SOLID_OB, NV = SOLID_NV;
   __device__ static void bil(const cd* w, const cd* z, Abar& B) { solid_minor_bil(w, z, B); }
+  template <class ZL>
+  __device__ static void bil_s(const cd* w, ZL&& zl, Abar& B) { solid_minor_bil_s(w, zl, B); }
   __device__ static void ob(const Abar& B, const LS& L, const double* b, double* o) {
     solid_ob(B, L, b, o);
   }
@@ -58,6 +65,8 @@ struct SysBvec : SysSolidMinor {
   using F = BvecM;
   using FT = BvecTM;
   __device__ static void bil(const cd* w, const cd* z, Abar& B) { bvec_bil(w, z, B); }
+  template <class ZL>
+  __device__ static void bil_s(const cd* w, ZL&& zl, Abar& B) { bvec_bil_s(w, zl, B); }
 };
 struct SysFluidMinor {
   using F = FluidMinorM;
@@ -65,6 +74,8 @@ struct SysFluidMinor {
   using Abar = FluidAbar;
   static constexpr int NOB = FLUID_OB, NV = FLUID_NV;
   __device__ static void bil(const cd* w, const cd* z, Abar& B) { fluid_minor_bil(w, z, B); }
+  template <class ZL>
+  __device__ static void bil_s(const cd* w, ZL&& zl, Abar& B) { fluid_minor_bil_s(w, zl, B); }
   __device__ static void ob(const Abar& B, const LS& L, const double* b, double* o) {
     fluid_ob(B, L, b, o);
   }
@@ -80,6 +91,8 @@ struct SysTor {
   using Abar = TorAbar;
   static constexpr int NOB = TOR_OB, NV = TOR_NV;
   __device__ static void bil(const cd* w, const cd* z, Abar& B) { tor_bil(w, z, B); }
+  template <class ZL>
+  __device__ static void bil_s(const cd* w, ZL&& zl, Abar& B) { tor_bil_s(w, zl, B); }
   __device__ static void ob(const Abar& B, const LS& L, const double* b, double* o) {
     tor_ob(B, L, b, o);
   }
@@ -103,21 +116,48 @@ __device__ inline double scale_pow(int n) {
 }
 
 // ------------------------------------------------------ LDS of a block
-// The first pass of an item (the forward's own march, re-run with
-// checkpoints) needs the forward's two stage slots per lane; the second
-// (the interval sweeps) the reduction buffers in the same bytes.
+// Steps per chunk of shared stage bases in the sweeps (the forward marches
+// use CH): with the reduction buffer and the two stage slots it keeps a
+// block within 40 KB, four blocks per CU.
+constexpr int CHA = 3;
+// values summed over the wave per round through red (wave_sum)
+constexpr int RED_MAX = 9;
+
 struct AdjLds {
-  double sb[6 * CH * BASE_MAX];     // stage bases of a chunk of steps
-  double tacc[NV_MAX * 4];          // the interval's table-entry adjoints
-  union {
-    cd slots[2 * 14 * 64];          // rk_step_lds / rk_step_T_lds stage slots
-    struct {
-      double red[NOB_MAX * 64];       // per-lane stage adjoints being summed
-      double bbar[6 * CH * NOB_MAX];  // summed stage adjoints of the chunk
-    } r;
-  } u;
+  double sb[6 * CHA * BASE_MAX];   // stage bases of a chunk of steps
+  double bbar[6 * CHA * NOB_MAX];  // the chunk's wave-summed stage adjoints
+  double tacc[NV_MAX * 4];         // the interval's table-entry adjoints
+  double red[RED_MAX * 64];        // per-lane values being summed
+  cd slots[2 * 14 * 64];           // rk_step_lds / rk_step_T_lds stage slots
 };
 static_assert(sizeof(AdjLds) <= 40 * 1024, "four blocks per CU");
+
+// vectors of a block's step scratch (global memory, 14 x 64 complex each):
+// the column's six stage inputs (0..5), the row's six stage rows (6..11),
+// a parked 14-vector (12)
+constexpr int SCR_ROWS = 6, SCR_PARK = 12, SCR_VECS = 13;
+// Layout of one vector: groups of 16 lanes, component q of lane g 16 + t
+// at byte g 3584 + q 256 + t 16 (256 contiguous bytes per component and
+// group).  Accessed through a buffer resource: the vector's offset is a
+// scalar, the component's an instruction immediate, the lane's one VGPR,
+// so no address stays live across a step.
+constexpr int SCR_VEC_BYTES = 4 * 3584;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+struct Scr {
+  __amdgpu_buffer_rsrc_t r;
+  int voff;
+  __device__ explicit Scr(cd* block)
+      : r(__builtin_amdgcn_make_buffer_rsrc(block, 0, SCR_VECS * SCR_VEC_BYTES, 0x00020000)),
+        voff((int)(threadIdx.x >> 4) * 3584 + (int)(threadIdx.x & 15) * 16) {}
+  __device__ __attribute__((always_inline)) void st(int vec, int q, cd v) const {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, voff + q * 256,
+                                           vec * SCR_VEC_BYTES, 0);
+  }
+  __device__ __attribute__((always_inline)) cd ld(int vec, int q) const {
+    return __builtin_bit_cast(
+        cd, __builtin_amdgcn_raw_buffer_load_b128(r, voff + q * 256, vec * SCR_VEC_BYTES, 0));
+  }
+};
 
 // bases of steps j0 .. j0 + nst - 1 (1-based) of the interval's step list,
 // one lane per stage radius, as march_interval lays them out
@@ -132,7 +172,7 @@ __device__ inline void load_bases(const Tab& T, const WS& w, int i, double ra, d
   }
 }
 
-// lanes summing one value in wave_sum_into: the largest power of two P
+// lanes summing one value in wave_sum_round: the largest power of two P
 // with NOB P <= 64
 template <int NOB>
 constexpr int sum_lanes() {
@@ -141,15 +181,24 @@ constexpr int sum_lanes() {
   return p;
 }
 
-// dst[q] += sum over the wave's lanes of v[q], q < NOB: through LDS (red,
-// NOB x 64), P lanes per value, then P-lane shuffles
+// Orders one wave's LDS accesses around an exchange between its lanes (a
+// block of the sweeps is one wave, whose LDS instructions execute in order):
+// a compiler barrier, no workgroup barrier.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// dst[q] = sum over the wave's lanes of v[q], q < NOB <= RED_MAX: through
+// LDS (red, NOB x 64), P lanes per value, then P-lane shuffles
 template <int NOB>
-__device__ inline void wave_sum_into(const double* v, double* red, double* dst) {
+__device__ inline void wave_sum_round(const double* v, double* red, double* dst) {
   constexpr int P = sum_lanes<NOB>(), LEN = 64 / P;
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < NOB; q++) red[q * 64 + t] = v[q];
-  __syncthreads();
+  wave_sync();
   if (t < NOB * P) {
     const double* r = red + (t / P) * 64 + (t % P) * LEN;
     double s = 0.0;
@@ -157,23 +206,83 @@ __device__ inline void wave_sum_into(const double* v, double* red, double* dst) 
     for (int k = 0; k < LEN; k++) s += r[k];
 #pragma unroll
     for (int o = 1; o < P; o <<= 1) s += __shfl_xor(s, o, 64);
-    if (t % P == 0) dst[t / P] += s;
+    if (t % P == 0) dst[t / P] = s;
   }
-  __syncthreads();
+  wave_sync();
 }
 
-// one stage's pairing: row kb with column z, scaled by fs, summed into dst
+// dst[q] = sum over the wave's lanes of v[q], q < NOB, in rounds of at most
+// RED_MAX values
+template <int NOB>
+__device__ inline void wave_sum(const double* v, double* red, double* dst) {
+  if constexpr (NOB <= RED_MAX) {
+    wave_sum_round<NOB>(v, red, dst);
+  } else {
+    wave_sum_round<RED_MAX>(v, red, dst);
+    wave_sum<NOB - RED_MAX>(v + RED_MAX, red, dst + RED_MAX);
+  }
+}
+
+// The pairings of one step: stage k's row (scratch vector SCR_ROWS + k,
+// loaded) with its column stage input (scratch vector k, streamed), scaled
+// by fs, summed over the wave into dst[k NOB .. (k + 1) NOB).
 template <class Sys>
-__device__ inline void pair_stage(const cd* kb, const cd* z, const LS& L, const double* base,
-                                  double fs, AdjLds& S, double* dst) {
-  typename Sys::Abar B;
-  zero(B);
-  Sys::bil(kb, z, B);
-  double ob[Sys::NOB];
-  Sys::ob(B, L, base, ob);
+__device__ inline void pair_step(const Scr& scr, const LS& L, const double* bs, double fs,
+                                 AdjLds& S, double* dst) {
+  constexpr int N = Sys::F::N, BS = Sys::F::BS;
+  // stage k + 1's row and column are loaded while stage k is paired and
+  // summed, so the scratch latency overlaps the arithmetic
+  cd kb[2][N], zs[2][N];
 #pragma unroll
-  for (int v = 0; v < Sys::NOB; v++) ob[v] *= fs;
-  wave_sum_into<Sys::NOB>(ob, S.u.r.red, dst);
+  for (int q = 0; q < N; q++) {
+    kb[0][q] = scr.ld(SCR_ROWS, q);
+    zs[0][q] = scr.ld(0, q);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const int c = k & 1;
+    if (k + 1 < 6) {
+#pragma unroll
+      for (int q = 0; q < N; q++) {
+        kb[c ^ 1][q] = scr.ld(SCR_ROWS + k + 1, q);
+        zs[c ^ 1][q] = scr.ld(k + 1, q);
+      }
+    }
+    typename Sys::Abar B;
+    zero(B);
+    Sys::bil_s(kb[c], [&](int q) { return zs[c][q]; }, B);
+    double ob[Sys::NOB];
+    Sys::ob(B, L, bs + k * BS, ob);
+#pragma unroll
+    for (int v = 0; v < Sys::NOB; v++) ob[v] *= fs;
+    wave_sum<Sys::NOB>(ob, S.red, dst + k * Sys::NOB);
+  }
+}
+
+// scratch writers: hook(k, v) of a step stores v as scratch vector base + k
+template <int N>
+struct ScrWrite {
+  const Scr& scr;
+  int base;
+  __device__ __attribute__((always_inline)) void operator()(int k, const cd* v) const {
+#pragma unroll
+    for (int q = 0; q < N; q++) scr.st(base + k, q, v[q]);
+  }
+};
+
+template <int N>
+__device__ inline void park(const Scr& scr, const cd* v) {
+#pragma unroll
+  for (int q = 0; q < N; q++) scr.st(SCR_PARK, q, v[q]);
+  // the compiler reloads the vector from the scratch afterwards instead
+  // of keeping it in registers across the replay
+  asm volatile("" ::: "memory");
+}
+template <int N>
+__device__ inline void unpark(const Scr& scr, cd* v) {
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int q = 0; q < N; q++) v[q] = scr.ld(SCR_PARK, q);
 }
 
 // The chunk's summed stage adjoints through the base computation to the
@@ -187,7 +296,7 @@ __device__ inline void chain_chunk(const Tab& T, const WS& w, int i, double ra, 
     int jj = j0 + q / 6, k = q % 6;
     double x = ra + (double)(jj - 1) * drr + stage_offset(k) * drr;
     double vb[Sys::NV];
-    Sys::adj(T, w, i, x, drr * stage_scale<F::N>(k), S.u.r.bbar + q * Sys::NOB, vb);
+    Sys::adj(T, w, i, x, drr * stage_scale<F::N>(k), S.bbar + q * Sys::NOB, vb);
     SplW sw = spl_w(T, i, x);
 #pragma unroll
     for (int v = 0; v < Sys::NV; v++) {
@@ -234,17 +343,25 @@ __global__ void __launch_bounds__(256) k_part_sum(const double* __restrict__ tpa
   if (lane == 0) tbar[e] += a;
 }
 
+// ---------------------------------------------- per-block device storage
+// knot-interval checkpoints (slot per interval), the stored side's state
+// before every step of an item (index s in the order the first pass takes
+// them), and the step scratch: the six stage vectors of one step and a
+// parked 14-vector
 __device__ inline cd* ck_of(const AdjWork& W, int slot) {
   return W.ck + ((size_t)blockIdx.x * W.slots + slot) * 14 * 64 + threadIdx.x;
 }
 __device__ inline int* cks_of(const AdjWork& W, int slot) {
   return W.cks + ((size_t)blockIdx.x * W.slots + slot) * 64 + threadIdx.x;
 }
-__device__ inline cd* tape_of(const AdjWork& W, int s) {
-  return W.tape + ((size_t)blockIdx.x * W.tmax + s) * 6 * 14 * 64 + threadIdx.x;
+__device__ inline cd* ys_of(const AdjWork& W, int s) {
+  return W.ys + ((size_t)blockIdx.x * W.smax + s) * 14 * 64 + threadIdx.x;
 }
-__device__ inline int* tsc_of(const AdjWork& W, int s) {
-  return W.tsc + ((size_t)blockIdx.x * W.tmax + s) * 64 + threadIdx.x;
+__device__ inline int* yss_of(const AdjWork& W, int s) {
+  return W.yss + ((size_t)blockIdx.x * W.smax + s) * 64 + threadIdx.x;
+}
+__device__ inline Scr scr_of(const AdjWork& W) {
+  return Scr(W.scr + (size_t)blockIdx.x * (SCR_VECS * SCR_VEC_BYTES / sizeof(cd)));
 }
 template <int N>
 __device__ inline void put_ck(const AdjWork& W, int slot, const cd* y, int sc) {
@@ -254,13 +371,14 @@ __device__ inline void put_ck(const AdjWork& W, int slot, const cd* y, int sc) {
   *cks_of(W, slot) = sc;
 }
 
-// the tape overflow: flag it and describe the first one
-__device__ inline void tape_overflow(const AdjWork& W, int i, int ns, double ra, double rb,
-                                     double rew) {
+// more steps in an item than the per-step storage holds: flag it and
+// describe the first one (interval, step index, radii, Re w)
+__device__ inline void steps_overflow(const AdjWork& W, int i, int s, double ra, double rb,
+                                      double rew) {
   if (threadIdx.x == 0 && !(atomicOr(W.err, 1) & 1)) {
     W.err[1] = i;
     double* d = reinterpret_cast<double*>(W.err + 2);
-    d[0] = ns;
+    d[0] = s;
     d[1] = ra;
     d[2] = rb;
     d[3] = rew;
@@ -275,135 +393,202 @@ __device__ inline int steps_of(const Tab& T, const WS& w, int i, int nk, double 
   return n - 1;
 }
 
-// ------------------------------------------------ one interval, row sweep
-// Interval i of a march whose column went ra -> rb: the column's steps are
-// recomputed from its checkpoint (slot, with its counter) with every stage
-// input on the tape; then the row x (counter sx) is taken back through the
-// steps, last first, each step's stages paired with the tape.
-template <class Sys>
-__device__ __forceinline__ void rev_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
-                             double rb, cd* x, int& sx, const AdjWork& W, int slot, AdjLds& S,
-                             double* tbar) {
-  using F = typename Sys::F;
+// the stored side's state y (N components, counter sc) before step s
+template <int N>
+__device__ inline void put_step(const AdjWork& W, int s, const cd* y, int sc) {
+  cd* c = ys_of(W, s);
+#pragma unroll
+  for (int q = 0; q < N; q++) c[q * 64] = y[q];
+  *yss_of(W, s) = sc;
+}
+
+template <int N>
+__device__ inline void load_step(const AdjWork& W, int s, cd* y, int& sc) {
+  const cd* c = ys_of(W, s);
+#pragma unroll
+  for (int q = 0; q < N; q++) y[q] = c[q * 64];
+  sc = *yss_of(W, s);
+}
+
+// ------------------------------------------------------ first pass
+// Interval i of the stored side's march, ra -> rb, as march_interval
+// (forward steps, renormalised), its state stored before every step at
+// s, s + 1, ...; s advances by the step count.
+template <class F>
+__device__ inline void adj_march(const Tab& T, const WS& w, const LS& L, int nk, int i,
+                                 double ra, double rb, cd* y, int& sc, AdjLds& S, const F& f,
+                                 const AdjWork& W, int& s) {
   constexpr int N = F::N, BS = F::BS;
   double drr;
   const int ns = steps_of(T, w, i, nk, ra, rb, drr);
-  if (ns > W.tmax) {
-    tape_overflow(W, i, ns, ra, rb, w.rew);
+  if (s + ns > W.smax) {
+    steps_overflow(W, i, s + ns, ra, rb, w.rew);
+    s += ns;
     return;
   }
-  {
-    cd y[N];
-    const cd* c = ck_of(W, slot);
-#pragma unroll
-    for (int q = 0; q < N; q++) y[q] = c[q * 64];
-    int sc = *cks_of(W, slot);
-    for (int j0 = 1; j0 <= ns; j0 += CH) {
-      int nst = min(CH, ns + 1 - j0);
-      load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
-      __syncthreads();
-      for (int jj = 0; jj < nst; jj++) {
-        int s = j0 - 1 + jj;
-        *tsc_of(W, s) = sc;
-        cd* tp = tape_of(W, s);
-        rk_step_hook(S.sb + jj * 6 * BS, L, w, y, F(), [&](int k, const cd* z) {
-#pragma unroll
-          for (int q = 0; q < N; q++) tp[(k * 14 + q) * 64] = z[q];
-        });
-        renorm<N>(y, sc);
+  for (int j0 = 1; j0 <= ns; j0 += CHA) {
+    const int nst = min(CHA, ns + 1 - j0);
+    load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+    __syncthreads();
+    for (int jj = 0; jj < nst; jj++) {
+      put_step<N>(W, s++, y, sc);
+      if constexpr (N == 14) {
+        rk_step_lds(S.sb + jj * 6 * BS, L, w, y, f, S.slots + threadIdx.x);
+      } else {
+        rk_step_b(S.sb + jj * 6 * BS, L, w, y, f);
       }
-      __syncthreads();
+      renorm<N>(y, sc);
     }
+    __syncthreads();
   }
-  const int jlast = 1 + ((ns - 1) / CH) * CH;
-  for (int j0 = jlast; j0 >= 1; j0 -= CH) {
-    int nst = min(CH, ns + 1 - j0);
-    if (j0 != jlast) load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
-    for (int q = threadIdx.x; q < 6 * nst * Sys::NOB; q += 64) S.u.r.bbar[q] = 0.0;
+}
+
+// The same for a row marched rb -> ra by transposed steps (march_interval_T:
+// steps ns, ns - 1, ..., 1), its state stored before every step.
+template <class Sys>
+__device__ inline void adj_march_T(const Tab& T, const WS& w, const LS& L, int nk, int i,
+                                   double ra, double rb, cd* x, int& sl, AdjLds& S,
+                                   const AdjWork& W, int& s) {
+  using F = typename Sys::F;
+  constexpr int BS = F::BS;
+  double drr;
+  const int ns = steps_of(T, w, i, nk, ra, rb, drr);
+  if (s + ns > W.smax) {
+    steps_overflow(W, i, s + ns, ra, rb, w.rew);
+    s += ns;
+    return;
+  }
+  const int jlast = 1 + ((ns - 1) / CHA) * CHA;
+  for (int j0 = jlast; j0 >= 1; j0 -= CHA) {
+    const int nst = min(CHA, ns + 1 - j0);
+    load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
     __syncthreads();
     for (int jj = nst - 1; jj >= 0; jj--) {
-      int s = j0 - 1 + jj;
-      const cd* tp = tape_of(W, s);
-      const double fs = scale_pow(-(sx + *tsc_of(W, s)));
+      put_step<14>(W, s++, x, sl);
+      rk_step_T_lds(S.sb + jj * 6 * BS, L, w, x, typename Sys::FT(), S.slots + threadIdx.x);
+      renorm<14>(x, sl);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------ one interval, row sweep
+// Interval i of a march whose column went ra -> rb, its states before each
+// step stored at s_end - ns .. s_end - 1: the row x (counter sx) is taken
+// back through the steps, last first.  Per step the column's step is
+// replayed from its stored state with every stage input written to the
+// step scratch, then the row's transposed step pairs each stage's row with
+// the scratch.  Returns s_end - ns.
+template <class Sys>
+__device__ __forceinline__ int rev_interval(const Tab& T, const WS& w, const LS& L, int nk, int i,
+                                            double ra, double rb, cd* x, int& sx,
+                                            const AdjWork& W, int s_end, AdjLds& S,
+                                            double* tbar) {
+  using F = typename Sys::F;
+  constexpr int N = F::N, BS = F::BS, NOB = Sys::NOB;
+  double drr;
+  const int ns = steps_of(T, w, i, nk, ra, rb, drr);
+  const int s0 = s_end - ns;
+  if (s_end > W.smax) return s0;  // not stored (flagged by the first pass)
+  const Scr scr = scr_of(W);
+  cd yn[N];  // the stored state of the next step to replay, and its counter
+  int ysc;
+  load_step<N>(W, s_end - 1, yn, ysc);
+  const int jlast = 1 + ((ns - 1) / CHA) * CHA;
+  for (int j0 = jlast; j0 >= 1; j0 -= CHA) {
+    const int nst = min(CHA, ns + 1 - j0);
+    load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+    __syncthreads();
+    for (int jj = nst - 1; jj >= 0; jj--) {
+      const int s = s0 + j0 - 1 + jj;
       const double* bs = S.sb + jj * 6 * BS;
-      rk_step_T_reg(bs, L, w, x, typename Sys::FT(), [&](int k, const cd* kb) {
-        cd z[N];
+      // the column's step again from its stored state (loaded during the
+      // previous step's pairings), stage inputs into the scratch
+      int sc = ysc;
+      {
+        cd y[N];
 #pragma unroll
-        for (int q = 0; q < N; q++) z[q] = tp[(k * 14 + q) * 64];
-        pair_stage<Sys>(kb, z, L, bs + k * BS, fs, S, S.u.r.bbar + (jj * 6 + k) * Sys::NOB);
-      });
+        for (int q = 0; q < N; q++) y[q] = yn[q];
+        if constexpr (N == 14) {
+          // the row waits in the scratch: the replay needs the registers
+          park<N>(scr, x);
+          rk_step_lds(bs, L, w, y, F(), S.slots + threadIdx.x, ScrWrite<N>{scr, 0});
+          unpark<N>(scr, x);
+        } else {
+          rk_step_hook(bs, L, w, y, F(), ScrWrite<N>{scr, 0});
+        }
+      }
+      // the row's transposed step, stage rows into the scratch
+      if constexpr (N == 14) {
+        rk_step_T_lds(bs, L, w, x, typename Sys::FT(), S.slots + threadIdx.x,
+                      ScrWrite<N>{scr, SCR_ROWS});
+      } else {
+        rk_step_T_reg(bs, L, w, x, typename Sys::FT(), ScrWrite<N>{scr, SCR_ROWS});
+      }
+      const double fs = scale_pow(-(sx + sc));
       renorm<N>(x, sx);
+      // the next step's stored state is loaded while this step is paired
+      if (s > s0) load_step<N>(W, s - 1, yn, ysc);
+      pair_step<Sys>(scr, L, bs, fs, S, S.bbar + jj * 6 * NOB);
     }
     chain_chunk<Sys>(T, w, i, ra, drr, j0, nst, S);
     __syncthreads();
   }
   flush_interval<Sys>(i, nk, S, tbar);
+  return s0;
 }
 
 // ------------------------------------------- one interval, column sweep
 // Interval i of a march whose column goes ra -> rb and whose row came
-// rb -> ra: the row's transposed steps are recomputed from its checkpoint
-// at rb (slot) with every stage row on the tape; then the column c
-// (counter sc) is stepped through the interval, each stage paired with the
-// tape.
+// rb -> ra by transposed steps, the row's states before each of its steps
+// stored at s_end - ns .. s_end - 1 (step ns first): the column c (counter
+// sc) is stepped through the interval with every stage input written to
+// the step scratch; per step the row's transposed step is then replayed
+// from its stored state, each stage's row paired with the scratch.
+// Returns s_end - ns.
 template <class Sys>
-__device__ __forceinline__ void fwd_interval(const Tab& T, const WS& w, const LS& L, int nk, int i, double ra,
-                             double rb, cd* c, int& sc, const AdjWork& W, int slot, AdjLds& S,
-                             double* tbar) {
+__device__ __forceinline__ int fwd_interval(const Tab& T, const WS& w, const LS& L, int nk, int i,
+                                            double ra, double rb, cd* c, int& sc,
+                                            const AdjWork& W, int s_end, AdjLds& S,
+                                            double* tbar) {
   using F = typename Sys::F;
-  constexpr int N = F::N, BS = F::BS;
+  constexpr int BS = F::BS, NOB = Sys::NOB;
+  static_assert(F::N == 14, "the column sweep is the b-vector system's");
   double drr;
   const int ns = steps_of(T, w, i, nk, ra, rb, drr);
-  if (ns > W.tmax) {
-    tape_overflow(W, i, ns, ra, rb, w.rew);
-    return;
-  }
-  const int jlast = 1 + ((ns - 1) / CH) * CH;
-  {
-    cd x[N];
-    const cd* ck = ck_of(W, slot);
-#pragma unroll
-    for (int q = 0; q < N; q++) x[q] = ck[q * 64];
-    int sl = *cks_of(W, slot);
-    for (int j0 = jlast; j0 >= 1; j0 -= CH) {
-      int nst = min(CH, ns + 1 - j0);
-      load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
-      __syncthreads();
-      for (int jj = nst - 1; jj >= 0; jj--) {
-        int s = j0 - 1 + jj;
-        *tsc_of(W, s) = sl;
-        cd* tp = tape_of(W, s);
-        rk_step_T_reg(S.sb + jj * 6 * BS, L, w, x, typename Sys::FT(), [&](int k, const cd* kb) {
-#pragma unroll
-          for (int q = 0; q < N; q++) tp[(k * 14 + q) * 64] = kb[q];
-        });
-        renorm<N>(x, sl);
-      }
-      __syncthreads();
-    }
-  }
-  for (int j0 = 1; j0 <= ns; j0 += CH) {
-    int nst = min(CH, ns + 1 - j0);
-    if (j0 != 1) load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
-    for (int q = threadIdx.x; q < 6 * nst * Sys::NOB; q += 64) S.u.r.bbar[q] = 0.0;
+  const int s0 = s_end - ns;
+  if (s_end > W.smax) return s0;  // not stored (flagged by the first pass)
+  const Scr scr = scr_of(W);
+  for (int j0 = 1; j0 <= ns; j0 += CHA) {
+    const int nst = min(CHA, ns + 1 - j0);
+    load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
     __syncthreads();
     for (int jj = 0; jj < nst; jj++) {
-      int s = j0 - 1 + jj;
-      const cd* tp = tape_of(W, s);
-      const double fs = scale_pow(-(sc + *tsc_of(W, s)));
+      // the row's state before its transposed step j = j0 + jj
+      const int s = s0 + (ns - (j0 + jj));
       const double* bs = S.sb + jj * 6 * BS;
-      rk_step_hook(bs, L, w, c, F(), [&](int k, const cd* z) {
-        cd kb[N];
-#pragma unroll
-        for (int q = 0; q < N; q++) kb[q] = tp[(k * 14 + q) * 64];
-        pair_stage<Sys>(kb, z, L, bs + k * BS, fs, S, S.u.r.bbar + (jj * 6 + k) * Sys::NOB);
-      });
-      renorm<N>(c, sc);
+      const double fs = scale_pow(-(sc + *yss_of(W, s)));
+      // the column's step, stage inputs into the scratch
+      rk_step_lds(bs, L, w, c, F(), S.slots + threadIdx.x, ScrWrite<14>{scr, 0});
+      renorm<14>(c, sc);
+      park<14>(scr, c);
+      // the row's transposed step again from its stored state, stage rows
+      // into the scratch
+      {
+        cd xr[14];
+        int xsc;
+        load_step<14>(W, s, xr, xsc);
+        rk_step_T_lds(bs, L, w, xr, typename Sys::FT(), S.slots + threadIdx.x,
+                      ScrWrite<14>{scr, SCR_ROWS});
+      }
+      pair_step<Sys>(scr, L, bs, fs, S, S.bbar + jj * 6 * NOB);
+      unpark<14>(scr, c);
     }
     chain_chunk<Sys>(T, w, i, ra, drr, j0, nst, S);
     __syncthreads();
   }
   flush_interval<Sys>(i, nk, S, tbar);
+  return s0;
 }
 
 // ------------------------------------------------------- record adjoints
@@ -459,13 +644,11 @@ __device__ __forceinline__ void brow_adj_item(const Tab& T, const RunK& R, int j
     for (int q = 0; q < 14; q++) m[q] = valid ? SS.mfin[p * 14 + q] : C(0.0);
     ysol_row(m, k, x);
   }
-  // 1. the rows down from the receiver, a checkpoint at each interval's top
-  int sl = 0;
+  // 1. the rows down from the receiver, their state stored before every step
+  int sl = 0, s = 0;
   for (int i = ir; i >= is; i--) {
     double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    put_ck<14>(W, ir - i, x, sl);
-    if (a != b)
-      march_interval_T(T, w, L, nk, i, a, b, x, sl, S.sb, BvecTM(), S.u.slots + threadIdx.x);
+    if (a != b) adj_march_T<SysBvec>(T, w, L, nk, i, a, b, x, sl, S, W, s);
   }
   __syncthreads();
   // record adjoints of this lane's row
@@ -492,7 +675,8 @@ __device__ __forceinline__ void brow_adj_item(const Tab& T, const RunK& R, int j
       for (int q = 0; q < 14; q++) mub[p * 14 + q] = mrow[q];
   }
   // 2. the column beta_k = fac sum_c a_kc bst_c (true value H^sc3 times
-  // this: counter -sc3) up from the source, paired with the rows
+  // this: counter -sc3) up from the source, paired with the rows; the
+  // intervals come in the reverse of the first pass's order
   cd bcol[14];
   for (int q = 0; q < 14; q++) {
     cd v = C(0.0);
@@ -503,7 +687,7 @@ __device__ __forceinline__ void brow_adj_item(const Tab& T, const RunK& R, int j
   int scb = -sc3;
   for (int i = is; i <= ir; i++) {
     double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    if (a != b) fwd_interval<SysBvec>(T, w, L, nk, i, a, b, bcol, scb, W, ir - i, S, tbar);
+    if (a != b) s = fwd_interval<SysBvec>(T, w, L, nk, i, a, b, bcol, scb, W, s, S, tbar);
   }
 }
 
@@ -556,7 +740,7 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
   // interval i below the source, i above it
   cd m[14];
   for (int q = 0; q < 14; q++) m[q] = C(0.0);
-  int layout = -1, sc1 = 0;
+  int layout = -1, sc1 = 0, s = 0;
   for (int i = ib; i <= is; i++) {
     const bool fl = fluid_at(R, i);
     if (layout == 0 && fl) {
@@ -574,18 +758,14 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
     put_ck<14>(W, i - 1, m, sc1);
     if (a == b) continue;
     if (fl)
-      march_interval(T, w, L, nk, i, a, b, m, true, sc1, S.sb, FluidMinorM());
+      adj_march(T, w, L, nk, i, a, b, m, sc1, S, FluidMinorM(), W, s);
     else
-      march_interval(T, w, L, nk, i, a, b, m, true, sc1, S.sb, SolidMinorM(),
-                     S.u.slots + threadIdx.x);
+      adj_march(T, w, L, nk, i, a, b, m, sc1, S, SolidMinorM(), W, s);
   }
   int sc3 = 0;
   for (int i = is; i <= ir; i++) {
     const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    put_ck<14>(W, i, m, sc3);
-    if (a != b)
-      march_interval(T, w, L, nk, i, a, b, m, true, sc3, S.sb, SolidMinorM(),
-                     S.u.slots + threadIdx.x);
+    if (a != b) adj_march(T, w, L, nk, i, a, b, m, sc3, S, SolidMinorM(), W, s);
   }
   __syncthreads();
   if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
@@ -614,7 +794,7 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
   // 2. back from the receiver to the source
   for (int i = ir; i >= is; i--) {
     const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    if (a != b) rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, i, S, tbar);
+    if (a != b) s = rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
   }
   // at the source: the row on ms1, plus the b-vector starts' part
   {
@@ -629,9 +809,9 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
     const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
     if (a != b) {
       if (fl)
-        rev_interval<SysFluidMinor>(T, w, L, nk, i, a, b, x, sx, W, i - 1, S, tbar);
+        s = rev_interval<SysFluidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
       else
-        rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, i - 1, S, tbar);
+        s = rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
     }
     if (i == i1) {
       const double h = scale_pow(-(sx + *cks_of(W, i - 1)));
@@ -693,10 +873,10 @@ __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j,
   const LS L = l_state(valid ? l : 2);
   const int nk = R.nknot, is = R.is, ir = R.ir, s0 = R.noc + 1;
   const double rs = R.rs, rr = R.rr;
-  // 1. the forward's marches (tor_item), checkpoints i - s0 below the
-  // source, is - s0 + 1 + (i - is) above
+  // 1. the forward's marches (tor_item), every step's state stored, the
+  // counter at the bottom of each interval below the source in slot i - s0
   cd y1[2] = {C(0.0), C(0.0)};
-  int sc1 = 0;
+  int sc1 = 0, s = 0;
   for (int i = ib; i <= is; i++) {
     if (i == i1) {
       y1[0] = SD.tor_y0[2 * p];
@@ -705,15 +885,13 @@ __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j,
     const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
     put_ck<2>(W, i - s0, y1, sc1);
     if (a == b) continue;
-    march_interval(T, w, L, nk, i, a, b, y1, true, sc1, S.sb, TorM());
+    adj_march(T, w, L, nk, i, a, b, y1, sc1, S, TorM(), W, s);
   }
-  const int up0 = is - s0 + 1;
   cd y3[2] = {y1[0], y1[1]};
   int sc3 = 0;
   for (int i = is; i <= ir; i++) {
     const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    put_ck<2>(W, up0 + (i - is), y3, sc3);
-    if (a != b) march_interval(T, w, L, nk, i, a, b, y3, true, sc3, S.sb, TorM());
+    if (a != b) adj_march(T, w, L, nk, i, a, b, y3, sc3, S, TorM(), W, s);
   }
   __syncthreads();
   if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
@@ -732,7 +910,7 @@ __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j,
   int sx = -sc3;
   for (int i = ir; i >= is; i--) {
     const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    if (a != b) rev_interval<SysTor>(T, w, L, nk, i, a, b, x, sx, W, up0 + (i - is), S, tbar);
+    if (a != b) s = rev_interval<SysTor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
   }
   {
     const double h = scale_pow(-sx);
@@ -743,7 +921,7 @@ __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j,
   }
   for (int i = is; i >= ib; i--) {
     const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
-    if (a != b) rev_interval<SysTor>(T, w, L, nk, i, a, b, x, sx, W, i - s0, S, tbar);
+    if (a != b) s = rev_interval<SysTor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
     if (i == i1) {
       const double h = scale_pow(-(sx + *cks_of(W, i - s0)));
       y0bar[2 * p] = x[0] * h;

@@ -544,6 +544,11 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd*
   for (int i = 0; i < N; i++) y[i] = rmad(k1[i], c6, k3[i]);
 }
 
+// The default stage hook of rk_step_lds / rk_step_T_lds: nothing.
+struct NoHook {
+  __device__ void operator()(int, const cd*) const {}
+};
+
 // Cash-Karp step with stage vectors kept in per-lane LDS slots.
 // slot points at this lane's first slot entry; entry (s, i) is at
 // slot[(s * N + i) * 64] (lane-contiguous, conflict-free).  Three register
@@ -552,10 +557,11 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd*
 // as soon as it is produced.  After stage 3, y is dead and its registers
 // accumulate the stage-5 input; the stage-6 input is formed in t and the
 // result in y, so each component takes 9 LDS accesses per step.  Same
-// arithmetic as rk_step_b.
-template <class F>
+// arithmetic as rk_step_b.  hook(k, z) sees stage k's input z (registers)
+// just before its derivative is taken (the adjoint's replay).
+template <class F, class Hook = NoHook>
 __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, cd* y, const F& f,
-                                   cd* slot) {
+                                   cd* slot, Hook&& hook = Hook()) {
   using namespace ck;
   constexpr int N = F::N, BS = F::BS;
   cd* S1 = slot;
@@ -565,12 +571,14 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
   // that LDS latency overlaps the arithmetic of the previous component.
   cd p1, p2;  // stages 2 and 3
   // stage 1: k1 -> S1, t = y + b21 k1
+  hook(0, y);
   f.stream(sb, L, w, y, [&](int i, cd d) {
     S1[i * 64] = d;
     t[i] = rmad(y[i], b21, d);
   });
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 2: k2 -> u
+  hook(1, t);
   f.stream(sb + BS, L, w, t, [&](int i, cd d) { u[i] = d; });
   p1 = S1[0];
 #pragma unroll
@@ -581,6 +589,7 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
   }
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 3: k3 -> S2; then t = t4, y = y + A5, S2 = y + A6, u = y + Aout
+  hook(2, t);
   f.stream(sb + 2 * BS, L, w, t, [&](int i, cd d) { S2[i * 64] = d; });
   p1 = S1[0];
   p2 = S2[0];
@@ -599,6 +608,7 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 4: fold k4 into the three accumulators; y is now the stage-5 input
   // (reads two components ahead: q[i & 1])
+  hook(3, t);
   cd q[2] = {S2[0], S2[64]};
   f.stream(sb + 3 * BS, L, w, t, [&](int i, cd d) {
     cd a6 = q[i & 1];
@@ -610,10 +620,12 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 5: t = stage-6 input (the stage-5 base is scaled by b65 h, so the
   // chains started from S2 give y + A6 + b65 k5 directly)
+  hook(4, y);
   f.stream_acc(sb + 4 * BS, L, w, y, [&](int i) { return S2[i * 64]; },
                [&](int i, cd d) { t[i] = d; });
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 6: y = y + Aout + c6 k6 (base scaled by c6 h, chains started from u)
+  hook(5, t);
   f.stream_acc(sb + 5 * BS, L, w, t, [&](int i) { return u[i]; },
                [&](int i, cd d) { y[i] = d; });
 }
@@ -628,18 +640,21 @@ __device__ inline void rk_step_lds(const double* sb, const LS& L, const WS& w, c
 //   x' = x + sum_i zb_i.
 // Registers y, z and the slots S1, S2 hold the partial sums
 // (A3 -> w3, A2 -> w2, A1 -> w1, L -> x'), so the step uses the same
-// storage as the forward one.
-template <class F>
+// storage as the forward one.  stage(k, kb) sees the row kb of stage k (the
+// input of its transposed derivative, registers) just before it is used.
+template <class F, class Hook = NoHook>
 __device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w, cd* x,
-                                     const F& f, cd* slot) {
+                                     const F& f, cd* slot, Hook&& stage = Hook()) {
   using namespace ck;
   constexpr int N = F::N, BS = F::BS;
   cd* S1 = slot;
   cd* S2 = slot + N * 64;
   cd y[N], z[N];
   // stage 6 (base scaled by c6 h): y = zb6; stage 5 (by b65 h): z = zb5
+  stage(5, x);
   f.stream(sb + 5 * BS, L, w, x, [&](int i, cd d) { y[i] = d; });
   __builtin_amdgcn_sched_barrier(0x100);
+  stage(4, y);
   f.stream(sb + 4 * BS, L, w, y, [&](int i, cd d) { z[i] = d; });
   __builtin_amdgcn_sched_barrier(0x100);
   // x = w4, y = A3, z = A1, S1 = A2, S2 = L
@@ -654,6 +669,7 @@ __device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w,
   }
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 4: zb4 from w4; y = w3
+  stage(3, x);
   f.stream(sb + 3 * BS, L, w, x, [&](int i, cd d) {
     y[i] = rmad(y[i], b43, d);
     z[i] = rmad(z[i], b41, d);
@@ -662,6 +678,7 @@ __device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w,
   });
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 3: zb3 from w3; x = w2
+  stage(2, y);
   f.stream(sb + 2 * BS, L, w, y, [&](int i, cd d) {
     x[i] = rmad(S1[i * 64], b32, d);
     z[i] = rmad(z[i], b31, d);
@@ -669,12 +686,14 @@ __device__ inline void rk_step_T_lds(const double* sb, const LS& L, const WS& w,
   });
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 2: zb2 from w2; z = w1; L moves to y (free once stage 3 has run)
+  stage(1, x);
   f.stream(sb + BS, L, w, x, [&](int i, cd d) {
     z[i] = rmad(z[i], b21, d);
     y[i] = S2[i * 64] + d;
   });
   __builtin_amdgcn_sched_barrier(0x100);
   // stage 1: x' = L + zb1 (the chains start from y)
+  stage(0, z);
   f.stream_acc(sb, L, w, z, [&](int i) { return y[i]; }, [&](int i, cd d) { x[i] = d; });
 }
 
