@@ -286,26 +286,40 @@ __device__ inline void unpark(const Scr& scr, cd* v) {
 }
 
 // The chunk's summed stage adjoints through the base computation to the
-// interval's table entries (lane q: stage radius q), into S.tacc.
+// interval's table entries (lane q: stage radius q), added to S.tacc: each
+// lane's contributions to the NV x 4 entries (value and second derivative
+// at the interval's two knots, spl_adj's weights) go through the stage
+// slots (free between steps) as rows of a table that lane e sums down.
 template <class Sys>
 __device__ inline void chain_chunk(const Tab& T, const WS& w, int i, double ra, double drr,
                                    int j0, int nst, AdjLds& S) {
   using F = typename Sys::F;
-  int q = threadIdx.x;
-  if (q < 6 * nst) {
+  constexpr int NE = Sys::NV * 4, LD = NE + 1;
+  static_assert(6 * CHA * LD * sizeof(double) <= sizeof(S.slots), "chain table in the slots");
+  double* ct = reinterpret_cast<double*>(S.slots);
+  const int q = threadIdx.x, nq = 6 * nst;
+  if (q < nq) {
     int jj = j0 + q / 6, k = q % 6;
     double x = ra + (double)(jj - 1) * drr + stage_offset(k) * drr;
     double vb[Sys::NV];
     Sys::adj(T, w, i, x, drr * stage_scale<F::N>(k), S.bbar + q * Sys::NOB, vb);
-    SplW sw = spl_w(T, i, x);
+    const SplW sw = spl_w(T, i, x);
+    double* row = ct + q * LD;
 #pragma unroll
     for (int v = 0; v < Sys::NV; v++) {
-      const int f0 = Sys::field(v);
-      spl_adj(sw, f0, vb[v], [&](int f, int knot, double val) {
-        atomicAdd(&S.tacc[v * 4 + (f == f0 ? 0 : 2) + (knot - i)], val);
-      });
+      row[4 * v + 0] = sw.deg ? vb[v] : vb[v] * sw.a;
+      row[4 * v + 1] = sw.deg ? 0.0 : vb[v] * sw.b;
+      row[4 * v + 2] = sw.deg ? 0.0 : vb[v] * sw.a3 * sw.hh;
+      row[4 * v + 3] = sw.deg ? 0.0 : vb[v] * sw.b3 * sw.hh;
     }
   }
+  wave_sync();
+  if (q < NE) {
+    double a = 0.0;
+    for (int r = 0; r < nq; r++) a += ct[r * LD + q];
+    S.tacc[q] += a;
+  }
+  wave_sync();
 }
 
 // S.tacc of interval i into the block's own column of the partial table
@@ -504,32 +518,56 @@ __device__ __forceinline__ int rev_interval(const Tab& T, const WS& w, const LS&
       const double* bs = S.sb + jj * 6 * BS;
       // the column's step again from its stored state (loaded during the
       // previous step's pairings), stage inputs into the scratch
-      int sc = ysc;
-      {
-        cd y[N];
+      const int sc = ysc;
+      if constexpr (N == 14) {
+        {
+          cd y[N];
 #pragma unroll
-        for (int q = 0; q < N; q++) y[q] = yn[q];
-        if constexpr (N == 14) {
+          for (int q = 0; q < N; q++) y[q] = yn[q];
           // the row waits in the scratch: the replay needs the registers
           park<N>(scr, x);
           rk_step_lds(bs, L, w, y, F(), S.slots + threadIdx.x, ScrWrite<N>{scr, 0});
           unpark<N>(scr, x);
-        } else {
-          rk_step_hook(bs, L, w, y, F(), ScrWrite<N>{scr, 0});
         }
-      }
-      // the row's transposed step, stage rows into the scratch
-      if constexpr (N == 14) {
+        // the row's transposed step, stage rows into the scratch
         rk_step_T_lds(bs, L, w, x, typename Sys::FT(), S.slots + threadIdx.x,
                       ScrWrite<N>{scr, SCR_ROWS});
+        const double fs = scale_pow(-(sx + sc));
+        renorm<N>(x, sx);
+        // the next step's stored state is loaded while this step is paired
+        if (s > s0) load_step<N>(W, s - 1, yn, ysc);
+        pair_step<Sys>(scr, L, bs, fs, S, S.bbar + jj * 6 * NOB);
       } else {
-        rk_step_T_reg(bs, L, w, x, typename Sys::FT(), ScrWrite<N>{scr, SCR_ROWS});
+        // small systems: the six stage vectors of both sides in registers
+        cd zst[6][N], kst[6][N];
+        {
+          cd y[N];
+#pragma unroll
+          for (int q = 0; q < N; q++) y[q] = yn[q];
+          rk_step_hook(bs, L, w, y, F(), [&](int k, const cd* z) {
+#pragma unroll
+            for (int q = 0; q < N; q++) zst[k][q] = z[q];
+          });
+        }
+        rk_step_T_reg(bs, L, w, x, typename Sys::FT(), [&](int k, const cd* kb) {
+#pragma unroll
+          for (int q = 0; q < N; q++) kst[k][q] = kb[q];
+        });
+        const double fs = scale_pow(-(sx + sc));
+        renorm<N>(x, sx);
+        if (s > s0) load_step<N>(W, s - 1, yn, ysc);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          typename Sys::Abar B;
+          zero(B);
+          Sys::bil_s(kst[k], [&](int q) { return zst[k][q]; }, B);
+          double ob[NOB];
+          Sys::ob(B, L, bs + k * BS, ob);
+#pragma unroll
+          for (int v = 0; v < NOB; v++) ob[v] *= fs;
+          wave_sum<NOB>(ob, S.red, S.bbar + (jj * 6 + k) * NOB);
+        }
       }
-      const double fs = scale_pow(-(sx + sc));
-      renorm<N>(x, sx);
-      // the next step's stored state is loaded while this step is paired
-      if (s > s0) load_step<N>(W, s - 1, yn, ysc);
-      pair_step<Sys>(scr, L, bs, fs, S, S.bbar + jj * 6 * NOB);
     }
     chain_chunk<Sys>(T, w, i, ra, drr, j0, nst, S);
     __syncthreads();
