@@ -117,17 +117,14 @@ __device__ inline double scale_pow(int n) {
 
 // ------------------------------------------------------ LDS of a block
 // Steps per chunk of shared stage bases in the sweeps (the forward marches
-// use CH): with the reduction buffer and the two stage slots it keeps a
-// block within 40 KB, four blocks per CU.
-constexpr int CHA = 3;
-// values summed over the wave per round through red (wave_sum)
-constexpr int RED_MAX = 9;
+// use CH): with the two stage slots it keeps a block within 40 KB, four
+// blocks per CU.
+constexpr int CHA = 6;
 
 struct AdjLds {
   double sb[6 * CHA * BASE_MAX];   // stage bases of a chunk of steps
   double bbar[6 * CHA * NOB_MAX];  // the chunk's wave-summed stage adjoints
   double tacc[NV_MAX * 4];         // the interval's table-entry adjoints
-  double red[RED_MAX * 64];        // per-lane values being summed
   cd slots[2 * 14 * 64];           // rk_step_lds / rk_step_T_lds stage slots
 };
 static_assert(sizeof(AdjLds) <= 40 * 1024, "four blocks per CU");
@@ -172,15 +169,6 @@ __device__ inline void load_bases(const Tab& T, const WS& w, int i, double ra, d
   }
 }
 
-// lanes summing one value in wave_sum_round: the largest power of two P
-// with NOB P <= 64
-template <int NOB>
-constexpr int sum_lanes() {
-  int p = 1;
-  while (2 * p * NOB <= 64) p *= 2;
-  return p;
-}
-
 // Orders one wave's LDS accesses around an exchange between its lanes (a
 // block of the sweeps is one wave, whose LDS instructions execute in order):
 // a compiler barrier, no workgroup barrier.
@@ -190,38 +178,42 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// dst[q] = sum over the wave's lanes of v[q], q < NOB <= RED_MAX: through
-// LDS (red, NOB x 64), P lanes per value, then P-lane shuffles
+// Sums over the wave of the per-lane stage adjoints of one step, batched:
+// the values of up to SUM_ROWS / NOB stages go as rows into the stage slots
+// (free once the step is taken; row length 65 doubles, so that one lane per
+// row reads without bank conflicts), then lane t sums row t with eight
+// partial sums.  put(k, ob) files stage k's values; the batch is summed
+// into dst + k0 NOB when its last stage is filed.
+constexpr int SUM_ROWS = 54, SUM_LD = 65;
+static_assert(SUM_ROWS * SUM_LD * sizeof(double) <= 2 * 14 * 64 * sizeof(cd),
+              "stage sums fit in the slots");
 template <int NOB>
-__device__ inline void wave_sum_round(const double* v, double* red, double* dst) {
-  constexpr int P = sum_lanes<NOB>(), LEN = 64 / P;
-  const int t = threadIdx.x;
+struct StageSums {
+  static constexpr int SB = SUM_ROWS / NOB;  // stages per batch
+  double* red;
+  double* dst;
+  __device__ void put(int k, const double* ob) const {
+    const int row0 = (k % SB) * NOB, t = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < NOB; q++) red[q * 64 + t] = v[q];
-  wave_sync();
-  if (t < NOB * P) {
-    const double* r = red + (t / P) * 64 + (t % P) * LEN;
-    double s = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < LEN; k++) s += r[k];
+    for (int v = 0; v < NOB; v++) red[(row0 + v) * SUM_LD + t] = ob[v];
+    if (k % SB == SB - 1 || k == 5) {
+      const int k0 = k - k % SB, rows = (k - k0 + 1) * NOB;
+      wave_sync();
+      if (t < rows) {
+        const double* r = red + t * SUM_LD;
+        double a[8];
 #pragma unroll
-    for (int o = 1; o < P; o <<= 1) s += __shfl_xor(s, o, 64);
-    if (t % P == 0) dst[t / P] = s;
+        for (int u = 0; u < 8; u++) a[u] = r[u];
+#pragma unroll
+        for (int m = 8; m < 64; m += 8)
+#pragma unroll
+          for (int u = 0; u < 8; u++) a[u] += r[m + u];
+        dst[k0 * NOB + t] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+      }
+      wave_sync();
+    }
   }
-  wave_sync();
-}
-
-// dst[q] = sum over the wave's lanes of v[q], q < NOB, in rounds of at most
-// RED_MAX values
-template <int NOB>
-__device__ inline void wave_sum(const double* v, double* red, double* dst) {
-  if constexpr (NOB <= RED_MAX) {
-    wave_sum_round<NOB>(v, red, dst);
-  } else {
-    wave_sum_round<RED_MAX>(v, red, dst);
-    wave_sum<NOB - RED_MAX>(v + RED_MAX, red, dst + RED_MAX);
-  }
-}
+};
 
 // The pairings of one step: stage k's row (scratch vector SCR_ROWS + k,
 // loaded) with its column stage input (scratch vector k, streamed), scaled
@@ -232,6 +224,7 @@ __device__ inline void pair_step(const Scr& scr, const LS& L, const double* bs, 
   constexpr int N = Sys::F::N, BS = Sys::F::BS;
   // stage k + 1's row and column are loaded while stage k is paired and
   // summed, so the scratch latency overlaps the arithmetic
+  const StageSums<Sys::NOB> sums{reinterpret_cast<double*>(S.slots), dst};
   cd kb[2][N], zs[2][N];
 #pragma unroll
   for (int q = 0; q < N; q++) {
@@ -255,7 +248,7 @@ __device__ inline void pair_step(const Scr& scr, const LS& L, const double* bs, 
     Sys::ob(B, L, bs + k * BS, ob);
 #pragma unroll
     for (int v = 0; v < Sys::NOB; v++) ob[v] *= fs;
-    wave_sum<Sys::NOB>(ob, S.red, dst + k * Sys::NOB);
+    sums.put(k, ob);
   }
 }
 
@@ -556,6 +549,7 @@ __device__ __forceinline__ int rev_interval(const Tab& T, const WS& w, const LS&
         const double fs = scale_pow(-(sx + sc));
         renorm<N>(x, sx);
         if (s > s0) load_step<N>(W, s - 1, yn, ysc);
+        const StageSums<NOB> sums{reinterpret_cast<double*>(S.slots), S.bbar + jj * 6 * NOB};
 #pragma unroll
         for (int k = 0; k < 6; k++) {
           typename Sys::Abar B;
@@ -565,7 +559,7 @@ __device__ __forceinline__ int rev_interval(const Tab& T, const WS& w, const LS&
           Sys::ob(B, L, bs + k * BS, ob);
 #pragma unroll
           for (int v = 0; v < NOB; v++) ob[v] *= fs;
-          wave_sum<NOB>(ob, S.red, S.bbar + (jj * 6 + k) * NOB);
+          sums.put(k, ob);
         }
       }
     }
