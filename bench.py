@@ -179,6 +179,9 @@ def batched_compare(deck, steps, warmup, do_tune=True):
         else None
     res = {}
     for mode, name in ((0, "plain"), (1, "batched")):
+        # the plain path with its default lanes per radial solve, the
+        # batched path with the tuned ones
+        g.set_lanes(tuning["best"]["rad_lanes"] if (tuning and mode == 1) else 0)
         for _ in range(warmup):
             g.forward_events(mode)
         t0 = time.perf_counter()

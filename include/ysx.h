@@ -153,6 +153,15 @@ int ysx_forward_events(ysx_ctx *ctx, int mode, double *seis);
  * queue access; persist = 0 launches one block per item. */
 int ysx_set_batch_tuning(ysx_ctx *ctx, int persist, int chunk, int blocks_per_cu);
 
+/* Lanes per (l, w) pair where the kernels offer a choice: rad_lanes lanes
+ * integrate each l = 0 radial solve (rad_solver_wg / rad_solver_ng,
+ * src/module_solver.f90:335-561) together, sharing its stage coefficients:
+ * 8, 1, or 0 for the default (8, or 1 for runs with more than 640
+ * frequencies).  Applies to the runs set and those set later (their work
+ * lists are rebuilt); results agree to rounding.  The tuner of the batched
+ * path sets it.  No counterpart in the reference. */
+int ysx_set_lanes(ysx_ctx *ctx, int rad_lanes);
+
 /* Diagnostic switch, no counterpart in the reference: full = 1 makes runs
  * set afterwards integrate the spheroidal b-vector system as the
  * reference's four source-side columns (sph_solver,

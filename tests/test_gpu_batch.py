@@ -85,6 +85,24 @@ def test_plain_and_batched_equal_single(api, single):
             assert all(np.array_equal(x, y) for x, y in zip(sa, sb)) and np.array_equal(ka, kb)
 
 
+def test_radial_lanes_knob(api, single):
+    """ysx_set_lanes: one lane or eight per l = 0 radial solve, set before
+    or after the events (the work lists are rebuilt), on the plain and the
+    batched path: every event agrees with its single-event result to
+    rounding; other values are refused."""
+    g = api.YSpec(synth_deck(0))
+    g.set_lanes(1)
+    g.set_events(small_events())
+    for n in (1, 8, 0, 1):
+        g.set_lanes(n)
+        for mode in (0, 1):
+            g.forward_events(mode)
+            _check_events(g, single, tol=1e-12)
+    with pytest.raises(api.YSpecError):
+        g.set_lanes(4)
+    g.close()
+
+
 def test_events_vs_oracle(api, oracle_mod, single):
     deck = synth_deck(0)
     for e, p in enumerate(small_events()):

@@ -30,7 +30,7 @@ EXPORTS = ("ysx_create", "ysx_destroy", "ysx_last_error", "ysx_set_model", "ysx_
            "ysx_seismograms_vjp", "ysx_spectra_vjp", "ysx_moment_tensor_vjp",
            "ysx_vjp_timings", "ysx_source_model_vjp", "ysx_model_table_noalloc",
            "ysx_model_vjp", "ysx_grad_timings", "ysx_misfit_grad", "ysx_grad_memory",
-           "ysx_model_vjp_deck")
+           "ysx_model_vjp_deck", "ysx_set_lanes")
 
 PAIR_FIELDS = (("svt", (5, 2)), ("ytor", (2, 2)), ("svs", (5, 6)),
                ("ysph", (6, 4)), ("sr0", (2,)), ("yrad", (2, 2)),
@@ -99,6 +99,7 @@ def lib():
         L.ysx_misfit_grad.argtypes = [vp, D, C.c_int, D, D]
         L.ysx_grad_memory.argtypes = [vp, D]
         L.ysx_model_vjp_deck.argtypes = [vp, D, D]
+        L.ysx_set_lanes.argtypes = [vp, C.c_int]
         L.ysx_model_table_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                               C.c_double, D, C.c_void_p]
         _LIB = L
@@ -474,6 +475,10 @@ class YSpec:
 
     def set_batch_tuning(self, persist=0, chunk=1, blocks_per_cu=4):
         self._check(self.L.ysx_set_batch_tuning(self.h, persist, chunk, blocks_per_cu))
+
+    def set_lanes(self, rad_lanes=0):
+        """ysx_set_lanes: lanes per l = 0 radial solve (0 default, 1 or 8)."""
+        self._check(self.L.ysx_set_lanes(self.h, rad_lanes))
 
     def forward_events(self, mode=1):
         """mode 0: one event after another (plain); 1: batched.  Returns ms."""

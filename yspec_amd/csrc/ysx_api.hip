@@ -117,6 +117,8 @@ struct ysx_ctx {
   // batched mode: persistent blocks or one block per item, items taken per
   // queue access, persistent blocks per CU, and the captured forward
   int persist = 0, chunk = 1, blocks_per_cu = 4;
+  // lanes per radial (l = 0) solve: 0 automatic, 1 or 8 (ysx_set_lanes)
+  int rad_lanes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev[16] = {};
@@ -700,6 +702,43 @@ extern "C" int ysx_model_table_vjp(int nknot, int nic, int noc, int ians, double
   return YSX_OK;
 }
 
+namespace {
+// The work lists and run array of the events set: built on the host
+// (build_work, which also marks the events that share marches in
+// RunK::drow) and uploaded; replaces lists uploaded before.
+int upload_work(ysx_ctx* ctx) {
+  for (auto& w : ctx->W) {
+    dfree(w.d_plain);
+    dfree(w.d_batched);
+    w = WorkList();
+  }
+  build_work(ctx);
+  const size_t nev = ctx->rk.size();
+  HIPCHK(hipMemcpy(ctx->d_runs, ctx->rk.data(), nev * sizeof(RunK), hipMemcpyHostToDevice));
+  for (auto& w : ctx->W) {
+    size_t n = std::max<size_t>(1, std::max(w.plain.size(), w.batched.size()));
+    HIPCHK(hipMalloc(&w.d_plain, n * sizeof(int4)));
+    HIPCHK(hipMalloc(&w.d_batched, n * sizeof(int4)));
+    if (!w.plain.empty())
+      HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+    if (!w.batched.empty())
+      HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
+                       hipMemcpyHostToDevice));
+  }
+  return YSX_OK;
+}
+
+// lanes per radial (l = 0) solve: the context's setting, or by default 8
+// lanes, one for runs with many frequencies (the radial kernel then runs
+// hidden beside the spheroidal marches and one lane each keeps it to few
+// compute units)
+int rad_lanes_of(const ysx_ctx* ctx, int nw) {
+  if (ctx->rad_lanes) return ctx->rad_lanes;
+  return (nw <= 640) ? 8 : 1;
+}
+}  // namespace
+
 extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double* fp,
                               const double* lat, const double* lon) {
   if (!ctx || nev < 1 || !ip || !fp || !lat || !lon) return YSX_ERR_ARG;
@@ -748,10 +787,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
     }
     k.ngc = (R.grav != 2) ? 1 : 0;
     k.lpad = (R.lmax + 1 + 3) / 4 * 4;
-    // radial solves: 8 lanes per frequency, or one when there are many
-    // frequencies (then the radial kernel runs hidden beside the marches and
-    // one lane each keeps it to few compute units)
-    k.radg = (R.nw <= 640) ? 8 : 1;
+    k.radg = rad_lanes_of(ctx, R.nw);
     k.tablds = ctx->tab_lds ? 1 : 0;
     // b-vector: 2 receiver-side rows per degree (4 lanes, rows 0..2, with the
     // potential/tilt correction), or the 4 source-side columns
@@ -844,20 +880,7 @@ extern "C" int ysx_set_events(ysx_ctx* ctx, int nev, const int* ip, const double
   HIPCHK(hipMalloc(&ctx->d_work, sio * sizeof(cd)));
   HIPCHK(hipMalloc(&ctx->d_seis, sio * sizeof(double)));
   HIPCHK(hipMalloc(&ctx->d_queue, Q_N * sizeof(int)));
-  build_work(ctx);
-  // build_work marks the events that share marches (RunK::drow)
-  HIPCHK(hipMemcpy(ctx->d_runs, ctx->rk.data(), nev * sizeof(RunK), hipMemcpyHostToDevice));
-  for (auto& w : ctx->W) {
-    size_t n = std::max<size_t>(1, std::max(w.plain.size(), w.batched.size()));
-    HIPCHK(hipMalloc(&w.d_plain, n * sizeof(int4)));
-    HIPCHK(hipMalloc(&w.d_batched, n * sizeof(int4)));
-    if (!w.plain.empty())
-      HIPCHK(hipMemcpy(w.d_plain, w.plain.data(), w.plain.size() * sizeof(int4),
-                       hipMemcpyHostToDevice));
-    if (!w.batched.empty())
-      HIPCHK(hipMemcpy(w.d_batched, w.batched.data(), w.batched.size() * sizeof(int4),
-                       hipMemcpyHostToDevice));
-  }
+  if (int rc = upload_work(ctx)) return rc;
   if (ctx->n_union > 0) {
     HIPCHK(hipMalloc(&ctx->d_shorder, ctx->sh_order.size() * sizeof(int)));
     HIPCHK(hipMemcpy(ctx->d_shorder, ctx->sh_order.data(), ctx->sh_order.size() * sizeof(int),
@@ -1191,6 +1214,21 @@ extern "C" int ysx_set_batch_tuning(ysx_ctx* ctx, int persist, int chunk, int bl
   ctx->persist = persist;
   ctx->chunk = chunk;
   ctx->blocks_per_cu = blocks_per_cu;
+  return YSX_OK;
+}
+
+extern "C" int ysx_set_lanes(ysx_ctx* ctx, int rad_lanes) {
+  if (!ctx || (rad_lanes != 0 && rad_lanes != 1 && rad_lanes != 8)) return YSX_ERR_ARG;
+  if (rad_lanes == ctx->rad_lanes) return YSX_OK;
+  ctx->rad_lanes = rad_lanes;
+  if (!ctx->have_run) return YSX_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream2));
+  for (size_t e = 0; e < ctx->rk.size(); e++) ctx->rk[e].radg = rad_lanes_of(ctx, ctx->R[e].nw);
+  drop_graph(ctx);
+  if (int rc = upload_work(ctx)) return rc;
+  ctx->refined = false;  // the wave lists are re-sorted by measured cost again
   return YSX_OK;
 }
 
