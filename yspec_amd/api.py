@@ -221,6 +221,8 @@ class YSpec:
         self.nw, self.nt, self.i1, self.i2, self.is_, self.ir, self.nout, self.npair = map(int, io)
         (self.rs, self.rr, self.dt, self.tout, self.ep, self.df, self.wmin,
          self.dw) = map(float, fo)
+        self.info = [dict(zip(("nw", "nt", "i1", "i2", "is", "ir", "nout", "npair"),
+                              map(int, io)))]
 
     def set_records(self, full):
         """full=True: ysph of the pair records holds all six rows (the
@@ -492,7 +494,14 @@ class YSpec:
         self._check(self.L.ysx_event_spectra(self.h, e, *[a.ctypes.data_as(D) for a in out]))
         return tuple(out)
 
-    def event_seismograms(self, e):
-        seis = np.zeros((self.info[e]["nt"], 3, self.events[e].nr), order="F")
-        self._check(self.L.ysx_event_seismograms(self.h, e, _dp(seis)))
-        return seis
+    def event_seismograms(self, e, out=None):
+        """event e's (nt, 3, nr) seismograms of the last forward (a device
+        to host copy; into `out` when given, a Fortran-ordered float64 array
+        of that shape)."""
+        shape = (self.info[e]["nt"], 3, self.events[e].nr)
+        if out is None:
+            out = np.zeros(shape, order="F")
+        elif out.shape != shape or out.dtype != np.float64 or not out.flags.f_contiguous:
+            raise ValueError(f"out must be a Fortran-ordered float64 array of shape {shape}")
+        self._check(self.L.ysx_event_seismograms(self.h, e, _dp(out)))
+        return out

@@ -36,6 +36,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ysx.h"
@@ -139,6 +140,11 @@ struct ysx_ctx {
   size_t rtape_bytes = 0;
   std::vector<int> rad_caps;  // per event from degree 0: the tape's length
   double ms_grad[7] = {0};  // kernel ms of the last ysx_model_vjp
+  // pinned staging of large host copies (copy_d2h / copy_h2d) and the
+  // events that order its two halves
+  void* pin = nullptr;
+  size_t pin_bytes = 0;
+  hipEvent_t pev[2] = {};
   // ysx_misfit_grad: the observed traces and the residual's per-block sums
   double *d_obs = nullptr, *d_rpart = nullptr;
   size_t obs_n = 0;
@@ -537,6 +543,9 @@ extern "C" void ysx_destroy(ysx_ctx* ctx) {
   dfree(ctx->d_mdeck);
   dfree(ctx->d_mlay);
   dfree(ctx->d_mvjp);
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
+  for (auto& e : ctx->pev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1178,6 +1187,89 @@ extern "C" int ysx_spectra(ysx_ctx* ctx, double* ur, double* ut, double* up) {
   return YSX_OK;
 }
 
+namespace {
+// Host copies of large arrays through pinned staging.  hipMemcpy to or
+// from pageable memory stages through the runtime's own small buffers at a
+// few GB/s; here the array moves in chunks of PIN_CHUNK bytes between the
+// device and a pinned buffer of two chunks (hipHostMalloc, kept by the
+// context) by DMA on the context's stream, while up to PIN_THREADS host
+// threads copy the previous chunk between the pinned buffer and the
+// caller's memory.  Small arrays take hipMemcpy.
+constexpr size_t PIN_CHUNK = 32u << 20, PIN_SMALL = 4u << 20;
+constexpr unsigned PIN_THREADS = 8;
+
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  const unsigned nt = std::max(1u, std::min(PIN_THREADS, std::thread::hardware_concurrency()));
+  const size_t per = (bytes + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++) {
+    const size_t a = t * per, b = std::min(bytes, a + per);
+    if (a >= b) break;
+    th.emplace_back([=]() {
+      std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+    });
+  }
+  for (auto& t : th) t.join();
+}
+
+int pin_setup(ysx_ctx* ctx) {
+  if (!ctx->pin) {
+    HIPCHK(hipHostMalloc(&ctx->pin, 2 * PIN_CHUNK, hipHostMallocDefault));
+    ctx->pin_bytes = 2 * PIN_CHUNK;
+  }
+  for (auto& e : ctx->pev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return YSX_OK;
+}
+
+int copy_d2h(ysx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes < PIN_SMALL) {
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return YSX_OK;
+  }
+  if (int rc = pin_setup(ctx)) return rc;
+  char* pin = static_cast<char*>(ctx->pin);
+  const char* s = static_cast<const char*>(src);
+  const size_t nch = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+  auto len = [&](size_t i) { return std::min(PIN_CHUNK, bytes - i * PIN_CHUNK); };
+  HIPCHK(hipMemcpyAsync(pin, s, len(0), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->pev[0], ctx->stream));
+  for (size_t i = 0; i < nch; i++) {
+    const int h = (int)(i % 2);
+    if (i + 1 < nch) {  // the other half's previous chunk was copied out last round
+      HIPCHK(hipMemcpyAsync(pin + (1 - h) * PIN_CHUNK, s + (i + 1) * PIN_CHUNK, len(i + 1),
+                            hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipEventRecord(ctx->pev[1 - h], ctx->stream));
+    }
+    HIPCHK(hipEventSynchronize(ctx->pev[h]));
+    parallel_copy(static_cast<char*>(dst) + i * PIN_CHUNK, pin + h * PIN_CHUNK, len(i));
+  }
+  return YSX_OK;
+}
+
+int copy_h2d(ysx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes < PIN_SMALL) {
+    HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return YSX_OK;
+  }
+  if (int rc = pin_setup(ctx)) return rc;
+  char* pin = static_cast<char*>(ctx->pin);
+  char* d = static_cast<char*>(dst);
+  const size_t nch = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+  auto len = [&](size_t i) { return std::min(PIN_CHUNK, bytes - i * PIN_CHUNK); };
+  for (size_t i = 0; i < nch; i++) {
+    const int h = (int)(i % 2);
+    if (i >= 2) HIPCHK(hipEventSynchronize(ctx->pev[h]));  // this half's DMA is done
+    parallel_copy(pin + h * PIN_CHUNK, static_cast<const char*>(src) + i * PIN_CHUNK, len(i));
+    HIPCHK(hipMemcpyAsync(d + i * PIN_CHUNK, pin + h * PIN_CHUNK, len(i), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipEventRecord(ctx->pev[h], ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return YSX_OK;
+}
+}  // namespace
+
 extern "C" int ysx_seismograms(ysx_ctx* ctx, double* seis) {
   if (!ctx) return YSX_ERR_ARG;
   if (!ctx->have_spec) return set_err(ctx, YSX_ERR_ARG, "no spectra");
@@ -1186,8 +1278,8 @@ extern "C" int ysx_seismograms(ysx_ctx* ctx, double* seis) {
   int rc = run_seis(ctx);
   if (rc) return rc;
   if (seis)
-    HIPCHK(hipMemcpy(seis, ctx->d_seis, 3 * (size_t)ctx->R[0].nt * ctx->R[0].nr * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    return copy_d2h(ctx, seis, ctx->d_seis,
+                    3 * (size_t)ctx->R[0].nt * ctx->R[0].nr * sizeof(double));
   return YSX_OK;
 }
 
@@ -1273,7 +1365,7 @@ extern "C" int ysx_forward_events(ysx_ctx* ctx, int mode, double* seis) {
   if (seis) {
     size_t n = 0;
     for (const HostRun& R : ctx->R) n += 3 * (size_t)R.nt * R.nr;
-    HIPCHK(hipMemcpy(seis, ctx->d_seis, n * sizeof(double), hipMemcpyDeviceToHost));
+    return copy_d2h(ctx, seis, ctx->d_seis, n * sizeof(double));
   }
   return YSX_OK;
 }
@@ -1296,9 +1388,7 @@ extern "C" int ysx_event_seismograms(ysx_ctx* ctx, int e, double* seis) {
   if (!ctx->have_seis) return set_err(ctx, YSX_ERR_ARG, "no seismograms computed");
   HIPCHK(hipSetDevice(ctx->device));
   size_t n = 3 * (size_t)ctx->R[e].nt * ctx->R[e].nr;
-  HIPCHK(hipMemcpy(seis, ctx->d_seis + ctx->rk[e].seis_off, n * sizeof(double),
-                   hipMemcpyDeviceToHost));
-  return YSX_OK;
+  return copy_d2h(ctx, seis, ctx->d_seis + ctx->rk[e].seis_off, n * sizeof(double));
 }
 
 extern "C" int ysx_pair_records(ysx_ctx* ctx, int n, const int* l, const int* j, double* rec) {
@@ -1388,7 +1478,7 @@ extern "C" int ysx_seismograms_vjp(ysx_ctx* ctx, const double* seis_bar, double*
     return set_err(ctx, YSX_ERR_ARG, "several events: the spectra adjoints stay on the device");
   size_t ns = 0;
   for (const HostRun& R : ctx->R) ns += 3 * (size_t)R.nt * R.nr;
-  HIPCHK(hipMemcpy(ctx->d_seisb, seis_bar, ns * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = copy_h2d(ctx, ctx->d_seisb, seis_bar, ns * sizeof(double))) return rc;
   if (int rc = seis_vjp_device(ctx)) return rc;
   size_t n = (size_t)ctx->R[0].nw * ctx->R[0].nr;
   double* outs[3] = {ur_bar, ut_bar, up_bar};
@@ -1829,7 +1919,7 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
   }
   dfree(ctx->d_rpart);
   HIPCHK(hipMalloc(&ctx->d_rpart, (size_t)RBLK * nev * sizeof(double)));
-  HIPCHK(hipMemcpy(ctx->d_obs, obs, nobs * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = copy_h2d(ctx, ctx->d_obs, obs, nobs * sizeof(double))) return rc;
   std::vector<unsigned> nb(nev);
   for (int e = 0; e < nev; e++) {
     const HostRun& R = ctx->R[e];
