@@ -400,21 +400,27 @@ __device__ inline int steps_of(const Tab& T, const WS& w, int i, int nk, double 
   return n - 1;
 }
 
-// the stored side's state y (N components, counter sc) before step s
+// the stored side's state y (N components, counter sc) before step s; the
+// states stream through once each way, so they are written and read
+// non-temporally and leave the caches to the step scratch
 template <int N>
 __device__ inline void put_step(const AdjWork& W, int s, const cd* y, int sc) {
-  cd* c = ys_of(W, s);
+  double* c = reinterpret_cast<double*>(ys_of(W, s));
 #pragma unroll
-  for (int q = 0; q < N; q++) c[q * 64] = y[q];
-  *yss_of(W, s) = sc;
+  for (int q = 0; q < N; q++) {
+    __builtin_nontemporal_store(y[q].x, c + q * 128);
+    __builtin_nontemporal_store(y[q].y, c + q * 128 + 1);
+  }
+  __builtin_nontemporal_store(sc, yss_of(W, s));
 }
 
 template <int N>
 __device__ inline void load_step(const AdjWork& W, int s, cd* y, int& sc) {
-  const cd* c = ys_of(W, s);
+  const double* c = reinterpret_cast<const double*>(ys_of(W, s));
 #pragma unroll
-  for (int q = 0; q < N; q++) y[q] = c[q * 64];
-  sc = *yss_of(W, s);
+  for (int q = 0; q < N; q++)
+    y[q] = C(__builtin_nontemporal_load(c + q * 128), __builtin_nontemporal_load(c + q * 128 + 1));
+  sc = __builtin_nontemporal_load(yss_of(W, s));
 }
 
 // ------------------------------------------------------ first pass
