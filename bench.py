@@ -313,11 +313,12 @@ def inversion_compare(deck, steps, warmup):
         "adjoint_kernel_ms": {"bvec_sweep": float(gk[0]), "minor_sweep": float(gk[1]),
                               "tor_sweep": float(gk[2]), "starting_solutions": float(gk[3]),
                               "radial": float(gk[4]), "source": float(gk[5])},
-        "gradient_device_bytes": {"interval_checkpoints": float(mem[0]),
-                                  "tapes": float(mem[1]), "per_pair_adjoints": float(mem[2])},
+        "gradient_device_bytes": {"per_step_states": float(mem[0]),
+                                  "step_scratch_and_radial_tape": float(mem[1]),
+                                  "per_pair_adjoints": float(mem[2])},
         "note": "gradient_ms: one ysx_misfit_grad over all events (batched forward "
-                "included); checkpoints and tapes are per persistent block, so they do "
-                "not grow with the number of pairs",
+                "included); per-step states and step scratch are per persistent block, "
+                "so they do not grow with the number of pairs",
     }
     g.close()
     return out

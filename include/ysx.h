@@ -290,11 +290,12 @@ int ysx_misfit_grad(ysx_ctx *ctx, const double *obs, int nout, double *misfit,
                     double *knot_grad);
 
 /* Device bytes the model gradient holds for the run (after a
- * ysx_model_vjp; no counterpart in the reference): [0] the sweeps' knot-
- * interval checkpoints (per persistent block, independent of the number of
- * pairs), [1] their per-interval tapes and the radial tape, [2] the
- * per-pair adjoints (record adjoints, minors at the source and start) and
- * the per-block partial table adjoints. */
+ * ysx_model_vjp; no counterpart in the reference): [0] the sweeps'
+ * per-step states (the stored side's state before every step of the
+ * longest item, per persistent block), [1] the per-block step scratch and
+ * the radial tape — [0] and [1] do not grow with the number of pairs — and
+ * [2] the per-pair adjoints (record adjoints, minors at the source and
+ * start) and the per-block partial table adjoints. */
 int ysx_grad_memory(const ysx_ctx *ctx, double *bytes3);
 
 /* Kernel milliseconds of the last ysx_model_vjp: [0] b-vector sweep, [1]

@@ -357,8 +357,8 @@ class YSpec:
         return out
 
     def grad_memory(self):
-        """device bytes of the model gradient: interval checkpoints, tapes,
-        per-pair adjoints (ysx_grad_memory)"""
+        """device bytes of the model gradient: per-step states, step scratch
+        with the radial tape, per-pair adjoints (ysx_grad_memory)"""
         out = np.zeros(3)
         self._check(self.L.ysx_grad_memory(self.h, _dp(out)))
         return out

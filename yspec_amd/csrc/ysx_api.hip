@@ -130,9 +130,9 @@ struct ysx_ctx {
   // y0bar), the table adjoint, the sweeps' per-block checkpoints and tapes
   cd *d_mub = nullptr, *d_m0bar = nullptr, *d_y0bar = nullptr;
   double *d_tbar = nullptr, *d_tpart = nullptr;
-  cd *d_ack = nullptr, *d_ays = nullptr, *d_ascr = nullptr;
-  int *d_acks = nullptr, *d_ayss = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
-  int adj_blocks = 0, adj_slots = 0, adj_smax = 0;
+  cd *d_ays = nullptr, *d_ascr = nullptr;
+  int *d_ayss = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
+  int adj_blocks = 0, adj_smax = 0;
   // radial adjoint: segment propagators, start states, end rows, tape
   cd *d_rprop = nullptr, *d_rys = nullptr, *d_rxe = nullptr;
   int* d_rcap = nullptr;
@@ -195,14 +195,14 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_spec); dfree(c->d_cf); dfree(c->d_tb); dfree(c->d_tw); dfree(c->d_work);
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
   dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
-  dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_tpart); dfree(c->d_ack);
-  dfree(c->d_ays); dfree(c->d_acks); dfree(c->d_ayss); dfree(c->d_ascr); dfree(c->d_aerr);
+  dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_tpart);
+  dfree(c->d_ays); dfree(c->d_ayss); dfree(c->d_ascr); dfree(c->d_aerr);
   dfree(c->d_aq);
   dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
   dfree(c->d_obs); dfree(c->d_rpart);
   c->obs_n = 0;
   c->rad_caps.clear();
-  c->adj_blocks = c->adj_slots = c->adj_smax = 0;
+  c->adj_blocks = c->adj_smax = 0;
   c->rtape_bytes = 0;
   c->have_specb = c->have_recb = false;
   c->sh_order.clear();
@@ -1670,20 +1670,14 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   // buffers (kept for the run)
   if (!ctx->d_tbar) {
     ctx->adj_blocks = 4 * ctx->ncu;
-    ctx->adj_slots = 1;
     ctx->adj_smax = 1;
-    for (const RunK& k : ctx->rk) {
-      ctx->adj_slots = std::max(ctx->adj_slots, k.ir + 1);
-      ctx->adj_smax = std::max(ctx->adj_smax, max_item_steps(M, k));
-    }
-    const size_t nb = ctx->adj_blocks, ns = ctx->adj_slots, sm = ctx->adj_smax;
+    for (const RunK& k : ctx->rk) ctx->adj_smax = std::max(ctx->adj_smax, max_item_steps(M, k));
+    const size_t nb = ctx->adj_blocks, sm = ctx->adj_smax;
     HIPCHK(hipMalloc(&ctx->d_mub, P * 14 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_m0bar, P * 14 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_y0bar, P * 2 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_tbar, (size_t)F_COUNT * nk * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->d_tpart, (size_t)F_COUNT * nk * nb * sizeof(double)));
-    HIPCHK(hipMalloc(&ctx->d_ack, nb * ns * 14 * 64 * sizeof(cd)));
-    HIPCHK(hipMalloc(&ctx->d_acks, nb * ns * 64 * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->d_ays, nb * sm * 14 * 64 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_ayss, nb * sm * 64 * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->d_ascr, nb * 13 * 14 * 64 * sizeof(cd)));
@@ -1702,8 +1696,7 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   HIPCHK(hipGetLastError());
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
-  AdjWork AW{ctx->d_ack,   ctx->d_acks,      ctx->d_ays,      ctx->d_ayss,
-             ctx->d_ascr,  ctx->adj_slots,  ctx->adj_smax,  ctx->d_aerr};
+  AdjWork AW{ctx->d_ays, ctx->d_ayss, ctx->d_ascr, ctx->adj_smax, ctx->d_aerr};
   // every event's items in one persistent launch per sweep: the plain
   // lists of all events are contiguous
   auto all_q = [&](int kind) {
@@ -1966,11 +1959,10 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
 
 extern "C" int ysx_grad_memory(const ysx_ctx* ctx, double* bytes3) {
   if (!ctx || !bytes3) return YSX_ERR_ARG;
-  const double nb = ctx->adj_blocks, ns = ctx->adj_slots, sm = ctx->adj_smax;
+  const double nb = ctx->adj_blocks, sm = ctx->adj_smax;
   const double P = (double)ctx->npair, nk = ctx->M.nknot;
-  bytes3[0] = nb * ns * 64 * (14 * sizeof(cd) + sizeof(int));         // interval checkpoints
-  bytes3[1] = nb * sm * 64 * (14 * sizeof(cd) + sizeof(int)) +        // per-step states,
-              nb * 13 * 14 * 64 * sizeof(cd) +                         // step scratch,
+  bytes3[0] = nb * sm * 64 * (14 * sizeof(cd) + sizeof(int));         // per-step states
+  bytes3[1] = nb * 13 * 14 * 64 * sizeof(cd) +                         // step scratch,
               (double)ctx->rtape_bytes;                                // radial tape
   bytes3[2] = ctx->d_tbar ? P * (2 * 14 + 2 + REC) * sizeof(cd) + nb * F_COUNT * nk * 8.0
                           : 0.0;                                       // per-pair adjoints

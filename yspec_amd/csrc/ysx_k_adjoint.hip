@@ -351,16 +351,9 @@ __global__ void __launch_bounds__(256) k_part_sum(const double* __restrict__ tpa
 }
 
 // ---------------------------------------------- per-block device storage
-// knot-interval checkpoints (slot per interval), the stored side's state
-// before every step of an item (index s in the order the first pass takes
-// them), and the step scratch: the six stage vectors of one step and a
-// parked 14-vector
-__device__ inline cd* ck_of(const AdjWork& W, int slot) {
-  return W.ck + ((size_t)blockIdx.x * W.slots + slot) * 14 * 64 + threadIdx.x;
-}
-__device__ inline int* cks_of(const AdjWork& W, int slot) {
-  return W.cks + ((size_t)blockIdx.x * W.slots + slot) * 64 + threadIdx.x;
-}
+// the stored side's state before every step of an item (index s in the
+// order the first pass takes them), and the step scratch: the six stage
+// vectors of one step and a parked 14-vector
 __device__ inline cd* ys_of(const AdjWork& W, int s) {
   return W.ys + ((size_t)blockIdx.x * W.smax + s) * 14 * 64 + threadIdx.x;
 }
@@ -370,14 +363,6 @@ __device__ inline int* yss_of(const AdjWork& W, int s) {
 __device__ inline Scr scr_of(const AdjWork& W) {
   return Scr(W.scr + (size_t)blockIdx.x * (SCR_VECS * SCR_VEC_BYTES / sizeof(cd)));
 }
-template <int N>
-__device__ inline void put_ck(const AdjWork& W, int slot, const cd* y, int sc) {
-  cd* c = ck_of(W, slot);
-#pragma unroll
-  for (int q = 0; q < N; q++) c[q * 64] = y[q];
-  *cks_of(W, slot) = sc;
-}
-
 // more steps in an item than the per-step storage holds: flag it and
 // describe the first one (interval, step index, radii, Re w)
 __device__ inline void steps_overflow(const AdjWork& W, int i, int s, double ra, double rb,
@@ -774,11 +759,11 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
   const int nk = R.nknot, is = R.is, ir = R.ir;
   const double rs = R.rs, rr = R.rr;
   const double rt1 = fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2));
-  // 1. the forward's march (minor_item), checkpoint i - 1 at the start of
-  // interval i below the source, i above it
+  // 1. the forward's march (minor_item), every step's state stored, the
+  // counter at the lane's start kept for the start adjoint
   cd m[14];
   for (int q = 0; q < 14; q++) m[q] = C(0.0);
-  int layout = -1, sc1 = 0, s = 0;
+  int layout = -1, sc1 = 0, s = 0, sci1 = 0;
   for (int i = ib; i <= is; i++) {
     const bool fl = fluid_at(R, i);
     if (layout == 0 && fl) {
@@ -790,10 +775,11 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
       fluid_2_solid(mf, m);
     }
     layout = fl ? 1 : 0;
-    if (i == i1)
+    if (i == i1) {
       for (int q = 0; q < 14; q++) m[q] = SD.sph_m0[p * 14 + q];
+      sci1 = sc1;
+    }
     const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
-    put_ck<14>(W, i - 1, m, sc1);
     if (a == b) continue;
     if (fl)
       adj_march(T, w, L, nk, i, a, b, m, sc1, S, FluidMinorM(), W, s);
@@ -852,7 +838,7 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
         s = rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
     }
     if (i == i1) {
-      const double h = scale_pow(-(sx + *cks_of(W, i - 1)));
+      const double h = scale_pow(-(sx + sci1));
       for (int q = 0; q < 14; q++) {
         m0bar[p * 14 + q] = x[q] * h;
         x[q] = C(0.0);
@@ -909,19 +895,19 @@ __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j,
   if (ib == NOLANE) return;
   const WS w = wave_omega(R, j);
   const LS L = l_state(valid ? l : 2);
-  const int nk = R.nknot, is = R.is, ir = R.ir, s0 = R.noc + 1;
+  const int nk = R.nknot, is = R.is, ir = R.ir;
   const double rs = R.rs, rr = R.rr;
   // 1. the forward's marches (tor_item), every step's state stored, the
-  // counter at the bottom of each interval below the source in slot i - s0
+  // counter at the lane's start kept for the start adjoint
   cd y1[2] = {C(0.0), C(0.0)};
-  int sc1 = 0, s = 0;
+  int sc1 = 0, s = 0, sci1 = 0;
   for (int i = ib; i <= is; i++) {
     if (i == i1) {
       y1[0] = SD.tor_y0[2 * p];
       y1[1] = SD.tor_y0[2 * p + 1];
+      sci1 = sc1;
     }
     const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
-    put_ck<2>(W, i - s0, y1, sc1);
     if (a == b) continue;
     adj_march(T, w, L, nk, i, a, b, y1, sc1, S, TorM(), W, s);
   }
@@ -961,7 +947,7 @@ __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j,
     const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
     if (a != b) s = rev_interval<SysTor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
     if (i == i1) {
-      const double h = scale_pow(-(sx + *cks_of(W, i - s0)));
+      const double h = scale_pow(-(sx + sci1));
       y0bar[2 * p] = x[0] * h;
       y0bar[2 * p + 1] = x[1] * h;
       x[0] = x[1] = C(0.0);

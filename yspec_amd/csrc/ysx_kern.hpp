@@ -182,12 +182,10 @@ struct SphState {
 // renormalisation counter per knot interval and lane) and the tape of one
 // interval's steps (six stage vectors per step and lane, and a counter).
 struct AdjWork {
-  cd* ck;    // knot-interval checkpoints, slots per block
-  int* cks;
   cd* ys;    // the stored side's state before every step, smax per block
   int* yss;
   cd* scr;   // the step scratch of each block (6 stage vectors + 1)
-  int slots, smax;
+  int smax;
   // err[0] bit 0: an item with more steps than smax; the first is
   // described in err[1] (interval) and the doubles at err + 2: the step
   // count reached, the interval's two radii, Re w
