@@ -133,6 +133,16 @@ struct ysx_ctx {
   cd *d_ays = nullptr, *d_ascr = nullptr;
   int *d_ayss = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
   int adj_blocks = 0, adj_smax = 0;
+  // the minor march's per-step states stored by the single-event forward
+  // of ysx_misfit_grad for the minor sweep (StepStore): states, counters,
+  // per-w item offsets and capacities, per-pair counters, overflow flag;
+  // fs_valid while the records of that forward are the context's
+  cd* d_fys = nullptr;
+  int* d_fyss = nullptr;
+  long long* d_foff = nullptr;
+  int *d_fcap = nullptr, *d_fsc = nullptr, *d_ferr = nullptr;
+  size_t fs_steps = 0;
+  bool fs_want = false, fs_valid = false;
   // radial adjoint: segment propagators, start states, end rows, tape
   cd *d_rprop = nullptr, *d_rys = nullptr, *d_rxe = nullptr;
   int* d_rcap = nullptr;
@@ -200,6 +210,10 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_aq);
   dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
   dfree(c->d_obs); dfree(c->d_rpart);
+  dfree(c->d_fys); dfree(c->d_fyss); dfree(c->d_foff); dfree(c->d_fcap); dfree(c->d_fsc);
+  dfree(c->d_ferr);
+  c->fs_steps = 0;
+  c->fs_want = c->fs_valid = false;
   c->obs_n = 0;
   c->rad_caps.clear();
   c->adj_blocks = c->adj_smax = 0;
@@ -960,6 +974,12 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
   dim3 b256(256), b64(64);
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
+  // a single event's minor march also stores its per-step states when
+  // ysx_misfit_grad asks for them (prepare_step_store)
+  const StepStore FS{ctx->d_fys, ctx->d_fyss, ctx->d_foff, ctx->d_fcap, ctx->d_fsc, ctx->d_ferr};
+  const StepStore* fsp =
+      (ctx->fs_want && ctx->d_fys && e0 == 0 && e1 == 1 && ctx->rk.size() == 1) ? &FS : nullptr;
+  ctx->fs_valid = fsp != nullptr;
   for (int e = e0; e < e1; e++) {
     PairRange PR = pairs_of(ctx, e, e + 1);
     int nb = (int)((PR.n + 255) / 256);
@@ -995,7 +1015,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
       launch_minor_top_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                          ctx->d_runs, qt, S, ctx->d_cnt);
       launch_minor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
-                         ctx->d_runs, qt, SD, S, ctx->d_cnt, 0);
+                         ctx->d_runs, qt, SD, S, ctx->d_cnt, 0, fsp);
       if (timing) HIPCHK(hipEventRecord(ev[4], st));
       if (ctx->rk[0].brow)
         launch_brow_march_item(dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
@@ -1330,6 +1350,7 @@ extern "C" int ysx_forward_events(ysx_ctx* ctx, int mode, double* seis) {
   HIPCHK(hipSetDevice(ctx->device));
   int nev = (int)ctx->rk.size();
   hipStream_t st = ctx->stream;
+  ctx->fs_valid = false;  // the batched forward stores no per-step states
   HIPCHK(hipEventRecord(ctx->ev[14], st));
   if (mode == 0) {
     int rc = enqueue_plain(ctx, 0, nev, false);
@@ -1624,8 +1645,10 @@ namespace {
 // interval's full step list once, the longest twice (a march splits the
 // source interval in two, each part taking at most the full list): the
 // per-step storage of the adjoint sweeps.
-int max_item_steps(const HostModel& M, const RunK& k) {
-  const double rew = k.wmin + (double)(k.nw - 1) * k.dw;
+// the most steps one minor-march item at Re w = rew can take: the step
+// rule's count over every knot interval, plus the splits of the source and
+// receiver intervals
+int item_steps_bound(const HostModel& M, double rew) {
   long long sum = 0;
   int mx = 1;
   for (int i = 1; i < M.nknot; i++) {
@@ -1637,6 +1660,73 @@ int max_item_steps(const HostModel& M, const RunK& k) {
     mx = std::max(mx, n);
   }
   return (int)std::min<long long>(sum + mx + 2, 1 << 30);
+}
+
+// the most steps of any item of event k (its largest frequency)
+int max_item_steps(const HostModel& M, const RunK& k) {
+  return item_steps_bound(M, k.wmin + (double)(k.nw - 1) * k.dw);
+}
+
+// Device memory for the minor march's per-step states of a single-event
+// forward (StepStore), so that the minor sweep need not march again:
+// item (j, lb) gets item_steps_bound at w_j.  Returns whether the forward
+// should store them: one event in the solver adjoint's scope, and at most
+// 32 GiB or half the free device memory.
+bool prepare_step_store(ysx_ctx* ctx) {
+  if (ctx->rk.size() != 1) return false;
+  const RunK& k = ctx->rk[0];
+  const HostModel& M = ctx->M;
+  if (k.grav != 2 || k.ngc || M.fic || M.ocean || k.rr < k.rs || k.rr != M.at(F_R, M.nknot) ||
+      !k.brow || (k.out == 2 && k.cor != 0) || M.nknot > 800)
+    return false;
+  const int nlb = (k.lmax - k.lmin + 1 + 63) / 64;
+  std::vector<long long> off(k.nw);
+  std::vector<int> cap(k.nw);
+  long long steps = 0;
+  for (int j = 0; j < k.nw; j++) {
+    cap[j] = item_steps_bound(M, k.wmin + (double)j * k.dw);
+    off[j] = steps;
+    steps += (long long)nlb * cap[j];
+  }
+  const size_t per_step = 14 * 64 * sizeof(cd) + 64 * sizeof(int);
+  size_t freeb = 0, totalb = 0;
+  if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return false;
+  const size_t have = ctx->fs_steps * per_step;
+  const size_t limit = std::min<size_t>(size_t(32) << 30, (freeb + have) / 2);
+  if ((size_t)steps * per_step > limit) return false;
+  if ((size_t)steps > ctx->fs_steps) {
+    dfree(ctx->d_fys);
+    dfree(ctx->d_fyss);
+    ctx->fs_steps = 0;
+    if (hipMalloc(&ctx->d_fys, (size_t)steps * 14 * 64 * sizeof(cd)) != hipSuccess ||
+        hipMalloc(&ctx->d_fyss, (size_t)steps * 64 * sizeof(int)) != hipSuccess) {
+      (void)hipGetLastError();
+      dfree(ctx->d_fys);
+      dfree(ctx->d_fyss);
+      return false;
+    }
+    ctx->fs_steps = (size_t)steps;
+  }
+  if (!ctx->d_foff) {
+    if (hipMalloc(&ctx->d_foff, k.nw * sizeof(long long)) != hipSuccess ||
+        hipMalloc(&ctx->d_fcap, k.nw * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_fsc, (size_t)ctx->npair * 3 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_ferr, sizeof(int)) != hipSuccess) {
+      (void)hipGetLastError();
+      dfree(ctx->d_foff);
+      dfree(ctx->d_fcap);
+      dfree(ctx->d_fsc);
+      dfree(ctx->d_ferr);
+      return false;
+    }
+    if (hipMemcpy(ctx->d_foff, off.data(), k.nw * sizeof(long long), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(ctx->d_fcap, cap.data(), k.nw * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess)
+      return false;
+  }
+  launch_reset(dim3(1), dim3(64), 0, ctx->stream, ctx->d_ferr, 1);
+  return hipGetLastError() == hipSuccess;
 }
 
 }  // namespace
@@ -1696,7 +1786,7 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   HIPCHK(hipGetLastError());
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
-  AdjWork AW{ctx->d_ays, ctx->d_ayss, ctx->d_ascr, ctx->adj_smax, ctx->d_aerr};
+  AdjWork AW{ctx->d_ays, ctx->d_ayss, 0, ctx->d_ascr, ctx->adj_smax, ctx->d_aerr};
   // every event's items in one persistent launch per sweep: the plain
   // lists of all events are contiguous
   auto all_q = [&](int kind) {
@@ -1715,8 +1805,11 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
                   ctx->d_mub, AW, ctx->d_tpart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[1], st));
+  const StepStore FS = ctx->fs_valid ? StepStore{ctx->d_fys, ctx->d_fyss, ctx->d_foff,
+                                                 ctx->d_fcap, ctx->d_fsc, ctx->d_ferr}
+                                       : StepStore{};
   launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
-                   ctx->d_mub, ctx->d_m0bar, AW, ctx->d_tpart);
+                   ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[2], st));
   launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
@@ -1817,6 +1910,13 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   if (err & 2)
     return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a radial segment with more steps than "
                                      "its tape");
+  if (FS.ys) {
+    int ferr = 0;
+    HIPCHK(hipMemcpy(&ferr, ctx->d_ferr, sizeof ferr, hipMemcpyDeviceToHost));
+    if (ferr)
+      return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a minor-march item of the forward with "
+                                       "more steps than its per-step storage");
+  }
   std::vector<double> tb(ntab);
   HIPCHK(hipMemcpy(tb.data(), ctx->d_tbar, ntab * sizeof(double), hipMemcpyDeviceToHost));
   // the source vectors' part (ysx_source_model_vjp)
@@ -1899,9 +1999,13 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
                                            ") for event " + std::to_string(e));
     ooff[e + 1] = ooff[e] + (size_t)ne[e] * 3 * R.nr;
   }
-  // one forward: the plain path for one event, the batched launch for several
-  if (int rc = nev == 1 ? ysx_forward(ctx, nullptr) : ysx_forward_events(ctx, 1, nullptr))
-    return rc;
+  // one forward: the plain path for one event, the batched launch for several;
+  // a single event's forward also stores the minor march's per-step states
+  // for the minor sweep when they fit
+  ctx->fs_want = nev == 1 && prepare_step_store(ctx);
+  const int frc = nev == 1 ? ysx_forward(ctx, nullptr) : ysx_forward_events(ctx, 1, nullptr);
+  ctx->fs_want = false;
+  if (frc) return frc;
   if (int rc = adjoint_run(ctx)) return rc;
   const size_t nobs = ooff[nev];
   constexpr int RBLK = 1024;

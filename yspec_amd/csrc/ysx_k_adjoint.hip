@@ -352,13 +352,18 @@ __global__ void __launch_bounds__(256) k_part_sum(const double* __restrict__ tpa
 
 // ---------------------------------------------- per-block device storage
 // the stored side's state before every step of an item (index s in the
-// order the first pass takes them), and the step scratch: the six stage
-// vectors of one step and a parked 14-vector
+// order the first pass takes them, from step W.base on), and the step
+// scratch: the six stage vectors of one step and a parked 14-vector
 __device__ inline cd* ys_of(const AdjWork& W, int s) {
-  return W.ys + ((size_t)blockIdx.x * W.smax + s) * 14 * 64 + threadIdx.x;
+  return W.ys + (W.base + s) * (14 * 64) + threadIdx.x;
 }
 __device__ inline int* yss_of(const AdjWork& W, int s) {
-  return W.yss + ((size_t)blockIdx.x * W.smax + s) * 64 + threadIdx.x;
+  return W.yss + (W.base + s) * 64 + threadIdx.x;
+}
+// a block's own per-step storage: smax steps from blockIdx.x smax on
+__device__ inline AdjWork block_work(AdjWork W) {
+  W.base = (long long)blockIdx.x * W.smax;
+  return W;
 }
 __device__ inline Scr scr_of(const AdjWork& W) {
   return Scr(W.scr + (size_t)blockIdx.x * (SCR_VECS * SCR_VEC_BYTES / sizeof(cd)));
@@ -723,9 +728,10 @@ __global__ void __launch_bounds__(64) k_brow_adj(const double* __restrict__ gtab
   __shared__ AdjLds S;
   if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
   __syncthreads();
+  const AdjWork WB = block_work(W);
   RUN_ITEMS(PERSIST, Q,
             brow_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SS, rec, recb, mub,
-                          W, S, tbar));
+                          WB, S, tbar));
 }
 
 // --------------------------------------------------------- k_minor_adj
@@ -747,7 +753,8 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
                                                const cd* __restrict__ rec,
                                                const cd* __restrict__ recb,
                                                const cd* __restrict__ mub, cd* __restrict__ m0bar,
-                                               const AdjWork& W, AdjLds& S, double* tbar) {
+                                               AdjWork W, const StepStore& FS, AdjLds& S,
+                                               double* tbar) {
   const int l = R.lmin + lb * 64 + (int)threadIdx.x;
   const bool valid = (l >= 1 && l <= R.lmax);
   const size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
@@ -759,37 +766,60 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
   const int nk = R.nknot, is = R.is, ir = R.ir;
   const double rs = R.rs, rr = R.rr;
   const double rt1 = fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2));
-  // 1. the forward's march (minor_item), every step's state stored, the
-  // counter at the lane's start kept for the start adjoint
-  cd m[14];
-  for (int q = 0; q < 14; q++) m[q] = C(0.0);
-  int layout = -1, sc1 = 0, s = 0, sci1 = 0;
-  for (int i = ib; i <= is; i++) {
-    const bool fl = fluid_at(R, i);
-    if (layout == 0 && fl) {
-      cd mf[5];
-      solid_2_fluid(m, mf);
-      for (int q = 0; q < 14; q++) m[q] = q < 5 ? mf[q] : C(0.0);
-    } else if (layout == 1 && !fl) {
-      cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
-      fluid_2_solid(mf, m);
+  int sc1 = 0, sc3 = 0, s = 0, sci1 = 0;
+  if (FS.ys) {
+    // 1. the forward stored every step's state (k_minor_march, STORE) and
+    // the counters: the item's steps are counted, not marched again
+    W.ys = FS.ys;
+    W.yss = FS.yss;
+    W.smax = FS.cap[j - 1];
+    W.base = FS.off[j - 1] + (long long)lb * W.smax;
+    double drr;
+    for (int i = ib; i <= is; i++) {
+      const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+      if (a != b) s += steps_of(T, w, i, nk, a, b, drr);
     }
-    layout = fl ? 1 : 0;
-    if (i == i1) {
-      for (int q = 0; q < 14; q++) m[q] = SD.sph_m0[p * 14 + q];
-      sci1 = sc1;
+    for (int i = is; i <= ir; i++) {
+      const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+      if (a != b) s += steps_of(T, w, i, nk, a, b, drr);
     }
-    const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
-    if (a == b) continue;
-    if (fl)
-      adj_march(T, w, L, nk, i, a, b, m, sc1, S, FluidMinorM(), W, s);
-    else
-      adj_march(T, w, L, nk, i, a, b, m, sc1, S, SolidMinorM(), W, s);
-  }
-  int sc3 = 0;
-  for (int i = is; i <= ir; i++) {
-    const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    if (a != b) adj_march(T, w, L, nk, i, a, b, m, sc3, S, SolidMinorM(), W, s);
+    if (valid) {
+      sc1 = FS.sc[3 * p];
+      sc3 = FS.sc[3 * p + 1];
+      sci1 = FS.sc[3 * p + 2];
+    }
+  } else {
+    // 1. the forward's march (minor_item), every step's state stored, the
+    // counter at the lane's start kept for the start adjoint
+    cd m[14];
+    for (int q = 0; q < 14; q++) m[q] = C(0.0);
+    int layout = -1;
+    for (int i = ib; i <= is; i++) {
+      const bool fl = fluid_at(R, i);
+      if (layout == 0 && fl) {
+        cd mf[5];
+        solid_2_fluid(m, mf);
+        for (int q = 0; q < 14; q++) m[q] = q < 5 ? mf[q] : C(0.0);
+      } else if (layout == 1 && !fl) {
+        cd mf[5] = {m[0], m[1], m[2], m[3], m[4]};
+        fluid_2_solid(mf, m);
+      }
+      layout = fl ? 1 : 0;
+      if (i == i1) {
+        for (int q = 0; q < 14; q++) m[q] = SD.sph_m0[p * 14 + q];
+        sci1 = sc1;
+      }
+      const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+      if (a == b) continue;
+      if (fl)
+        adj_march(T, w, L, nk, i, a, b, m, sc1, S, FluidMinorM(), W, s);
+      else
+        adj_march(T, w, L, nk, i, a, b, m, sc1, S, SolidMinorM(), W, s);
+    }
+    for (int i = is; i <= ir; i++) {
+      const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+      if (a != b) adj_march(T, w, L, nk, i, a, b, m, sc3, S, SolidMinorM(), W, s);
+    }
   }
   __syncthreads();
   if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
@@ -869,11 +899,12 @@ __global__ void __launch_bounds__(64) k_minor_adj(const double* __restrict__ gta
                                                   const cd* __restrict__ recb,
                                                   const cd* __restrict__ mub,
                                                   cd* __restrict__ m0bar, AdjWork W,
-                                                  double* __restrict__ tbar) {
+                                                  StepStore FS, double* __restrict__ tbar) {
   __shared__ AdjLds S;
+  const AdjWork WB = block_work(W);
   RUN_ITEMS(PERSIST, Q,
             minor_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, SS, rec, recb,
-                           mub, m0bar, W, S, tbar));
+                           mub, m0bar, WB, FS, S, tbar));
 }
 
 // ----------------------------------------------------------- k_tor_adj
@@ -963,9 +994,10 @@ __global__ void __launch_bounds__(64) k_tor_adj(const double* __restrict__ gtab,
                                                 cd* __restrict__ y0bar, AdjWork W,
                                                 double* __restrict__ tbar) {
   __shared__ AdjLds S;
+  const AdjWork WB = block_work(W);
   RUN_ITEMS(PERSIST, Q,
             tor_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, rec, recb,
-                         y0bar, W, S, tbar));
+                         y0bar, WB, S, tbar));
 }
 
 // --------------------------------------------------------- k_start_adj
@@ -1257,9 +1289,10 @@ void launch_brow_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gt
 }
 void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                       const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,
-                      const cd* recb, const cd* mub, cd* m0bar, AdjWork W, double* tbar) {
+                      const cd* recb, const cd* mub, cd* m0bar, AdjWork W, StepStore FS,
+                      double* tbar) {
   hipLaunchKernelGGL(k_minor_adj<true>, g, b, lds, s, gtab, runs, Q, SD, SS, rec, recb, mub,
-                     m0bar, W, tbar);
+                     m0bar, W, FS, tbar);
 }
 void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                     const RunK* runs, Queue Q, StartData SD, const cd* rec, const cd* recb,

@@ -32,12 +32,16 @@ void YSX_LAUNCH(minor_top)(dim3 g, dim3 b, size_t lds, hipStream_t s, const doub
 #elif YSX_MARCH == 2
 void YSX_LAUNCH(minor_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                              const RunK* runs, Queue Q, StartData SD, SphState S, int* cnt,
-                             int drow_on) {
+                             int drow_on, const StepStore* fs) {
   if (drow_on)
-    hipLaunchKernelGGL((k_minor_march<YSX_PERSIST, true>), g, b, lds, s, gtab, runs, Q, SD, S, cnt);
+    hipLaunchKernelGGL((k_minor_march<YSX_PERSIST, true>), g, b, lds, s, gtab, runs, Q, SD, S, cnt,
+                       StepStore{});
+  else if (fs && fs->ys)
+    hipLaunchKernelGGL((k_minor_march<YSX_PERSIST, false, true>), g, b, lds, s, gtab, runs, Q, SD,
+                       S, cnt, *fs);
   else
     hipLaunchKernelGGL((k_minor_march<YSX_PERSIST, false>), g, b, lds, s, gtab, runs, Q, SD, S,
-                       cnt);
+                       cnt, StepStore{});
 }
 #elif YSX_MARCH == 3
 void YSX_LAUNCH(bvec_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,

@@ -182,9 +182,10 @@ struct SphState {
 // renormalisation counter per knot interval and lane) and the tape of one
 // interval's steps (six stage vectors per step and lane, and a counter).
 struct AdjWork {
-  cd* ys;    // the stored side's state before every step, smax per block
-  int* yss;
-  cd* scr;   // the step scratch of each block (6 stage vectors + 1)
+  cd* ys;     // the stored side's state before every step: steps base ..
+  int* yss;   // base + smax - 1 of the current item (per block, or per item
+  long long base;  // when the forward stored them)
+  cd* scr;    // the step scratch of each block (6 stage vectors + 1)
   int smax;
   // err[0] bit 0: an item with more steps than smax; the first is
   // described in err[1] (interval) and the doubles at err + 2: the step
@@ -192,13 +193,28 @@ struct AdjWork {
   int* err;
 };
 
+// Per-step states of the minor march stored by a single-event forward for
+// the gradient that follows it: item (j, lb) holds cap[j - 1] steps from
+// step off[j - 1] + lb cap[j - 1] on (state [step][14][64] complex,
+// counter [step][64]); sc[3 p ..] gets pair p's renormalisation counters
+// at the source, at the receiver and at its start level; err[0] is set when
+// an item has more steps than its capacity.  ys == nullptr: not stored.
+struct StepStore {
+  cd* ys;
+  int* yss;
+  const long long* off;
+  const int* cap;
+  int* sc;
+  int* err;
+};
+
 // ------------------------------------------------------- march helpers
 // yint's interval loop (src/module_int.f90:1888-1963) over knot intervals,
 // with wave-uniform arguments
-template <class F>
+template <class F, class Pre = NoPre>
 __device__ int march_yint(const Tab& T, const WS& w, const LS& L, int nk, int i1, int i2,
                           double r1, double r2, cd* y, int& sc, double* sb, const F& f,
-                          cd* slot = nullptr) {
+                          cd* slot = nullptr, Pre&& pre = Pre()) {
   // one call site of march_interval for both directions
   int n = 0;
   if (r2 == r1) return 0;
@@ -207,7 +223,7 @@ __device__ int march_yint(const Tab& T, const WS& w, const LS& L, int nk, int i1
     double a = (i == i1) ? r1 : T(F_R, i + lo);
     double b = (i == i2) ? r2 : T(F_R, i + 1 - lo);
     if (a == b) continue;
-    n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot);
+    n += march_interval(T, w, L, nk, i, a, b, y, true, sc, sb, f, slot, pre);
   }
   return n;
 }
@@ -330,9 +346,12 @@ void launch_part_sum(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* tp
 void launch_brow_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                      const RunK* runs, Queue Q, SphState SS, const cd* rec, const cd* recb,
                      cd* mub, AdjWork W, double* tbar);
+// FS.ys != nullptr: the minor march's per-step states from the forward
+// (k_minor_march with STORE) replace the sweep's first pass
 void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                       const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,
-                      const cd* recb, const cd* mub, cd* m0bar, AdjWork W, double* tbar);
+                      const cd* recb, const cd* mub, cd* m0bar, AdjWork W, StepStore FS,
+                      double* tbar);
 void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                     const RunK* runs, Queue Q, StartData SD, const cd* rec, const cd* recb,
                     cd* y0bar, AdjWork W, double* tbar);
@@ -370,7 +389,8 @@ void launch_rad_seg_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double*
                                  const RunK* runs, Queue Q, SphState S, int* cnt);              \
   void launch_minor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s,                   \
                                    const double* gtab, const RunK* runs, Queue Q, StartData SD, \
-                                   SphState S, int* cnt, int drow_on);                          \
+                                   SphState S, int* cnt, int drow_on,                           \
+                                   const StepStore* fs = nullptr);                              \
   void launch_bvec_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\
                                   const RunK* runs, Queue Q, SphState S, cd* rec, int* cnt);  \
   void launch_brow_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,\

@@ -548,6 +548,11 @@ __device__ inline void rk_step_b(const double* sb, const LS& L, const WS& w, cd*
 struct NoHook {
   __device__ void operator()(int, const cd*) const {}
 };
+// march_interval's hook before every step: pre(y, scale) sees the state
+// and renormalisation counter the step starts from
+struct NoPre {
+  __device__ void operator()(const cd*, int) const {}
+};
 
 // Cash-Karp step with stage vectors kept in per-lane LDS slots.
 // slot points at this lane's first slot entry; entry (s, i) is at
@@ -715,10 +720,11 @@ __host__ __device__ inline double stage_scale(int k) {
   return 1.0;
 }
 
-template <class F>
+template <class F, class Pre = NoPre>
 __device__ int march_interval(const Tab& T, const WS& w, const LS& L, int nknot, int i,
                               double ra, double rb, cd* y, bool scaled, int& scale,
-                              double* sbase, const F& f, cd* slot = nullptr) {
+                              double* sbase, const F& f, cd* slot = nullptr,
+                              Pre&& pre = Pre()) {
   double drr0 = step_len_w(T, w.rew, i, nknot);
   int n = 2 + (int)floor(fabs(rb - ra) / drr0);
   double drr = (rb - ra) / (double)(n - 1);
@@ -732,6 +738,7 @@ __device__ int march_interval(const Tab& T, const WS& w, const LS& L, int nknot,
     }
     __syncthreads();
     for (int jj = 0; jj < nst; jj++) {
+      pre(y, scale);
       if constexpr (F::N == 14) {
         rk_step_lds(sbase + jj * 6 * F::BS, L, w, y, f, slot);
       } else {

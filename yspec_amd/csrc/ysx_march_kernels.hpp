@@ -109,10 +109,34 @@ __device__ __forceinline__ int minors_from_top(const Tab& T, const WS& w, const 
   return sc2;
 }
 
-template <bool DROW>
+// the per-step states of one item of the minor march into a StepStore:
+// the state before each step, step s of the item at base + s
+struct StorePre {
+  const StepStore& FS;
+  long long base;
+  int cap;
+  int& s;
+  __device__ void operator()(const cd* y, int sc) const {
+    if (s < cap) {
+      double* c = reinterpret_cast<double*>(FS.ys + (base + s) * (14 * 64) + threadIdx.x);
+#pragma unroll
+      for (int q = 0; q < 14; q++) {
+        __builtin_nontemporal_store(y[q].x, c + q * 128);
+        __builtin_nontemporal_store(y[q].y, c + q * 128 + 1);
+      }
+      __builtin_nontemporal_store(sc, FS.yss + (base + s) * 64 + threadIdx.x);
+    }
+    s++;
+  }
+};
+
+// STORE: also the state before every step into FS, for the gradient
+// (receiver at or above the source only)
+template <bool DROW, bool STORE = false>
 __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, int lb,
                                            const StartData& SD, const SphState& S,
-                                           int* __restrict__ cnt, double* sb, cd* slot) {
+                                           int* __restrict__ cnt, double* sb, cd* slot,
+                                           const StepStore& FS) {
   if (R.ngc) return;  // spheroidal solves of this event run in k_ngc_march
   int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 1 && l <= R.lmax);
@@ -128,6 +152,15 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
   int itop = up ? is : ir;
   double rtop = up ? rs : rr;
   int nsol = 0, nflu = 0;
+  int fs = 0, sci1 = 0;
+  auto pre = [&]() {
+    if constexpr (STORE) {
+      const int cap = FS.cap[j - 1];
+      return StorePre{FS, FS.off[j - 1] + (long long)lb * cap, cap, fs};
+    } else {
+      return NoPre();
+    }
+  }();
   cd m[14];
   // (2) ms1 from each lane's start level up to the lower point
   double rt1 = up ? fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2)) : T(F_R, 1);
@@ -147,16 +180,18 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
       fluid_2_solid(mf, m);
     }
     layout = region;
-    if (i == i1)
+    if (i == i1) {
       for (int q = 0; q < 14; q++) m[q] = SD.sph_m0[p * 14 + q];
+      if constexpr (STORE) sci1 = sc1;
+    }
     double a = (i == 1) ? rt1 : T(F_R, i);
     double b = (i == itop) ? rtop : T(F_R, i + 1);
     if (a == b) continue;
     int n;
     if (fluid_i)
-      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, FluidMinorM());
+      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, FluidMinorM(), nullptr, pre);
     else
-      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, SolidMinorM(), slot);
+      n = march_interval(T, w, L, nk, i, a, b, m, true, sc1, sb, SolidMinorM(), slot, pre);
     if (i >= i1) (fluid_i ? nflu : nsol) += n;
   }
   const int cols[4] = {0, 1, 3, 4};
@@ -186,7 +221,15 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
   }
   // on from the lower point to the upper one (one call site for both cases)
   nsol += march_yint(T, w, L, nk, up ? is : ir, up ? ir : is, rtop, up ? rr : rs, m, sc3, sb,
-                     SolidMinorM(), slot);
+                     SolidMinorM(), slot, pre);
+  if constexpr (STORE) {
+    if (threadIdx.x == 0 && fs > FS.cap[j - 1]) atomicOr(FS.err, 1);
+    if (valid) {
+      FS.sc[3 * p] = sc1;
+      FS.sc[3 * p + 1] = sc3;
+      FS.sc[3 * p + 2] = sci1;
+    }
+  }
   if (!valid) return;
   cd m2[14];
   for (int q = 0; q < 14; q++) m2[q] = S.m2[p * 14 + q];
@@ -210,17 +253,17 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
 
 // items (event, j, l-block of 64).  DROW: events marked drow stop at the
 // source and leave the determinant to k_drow_march (batched path)
-template <bool PERSIST, bool DROW>
+template <bool PERSIST, bool DROW, bool STORE = false>
 __global__ void __launch_bounds__(64) k_minor_march(const double* __restrict__ gtab,
                                                     const RunK* __restrict__ runs, Queue Q,
                                                     StartData SD, SphState S,
-                                                    int* __restrict__ cnt) {
+                                                    int* __restrict__ cnt, StepStore FS) {
   __shared__ double sb[6 * CH * BASE_MAX];
   __shared__ cd slots[2 * 14 * 64];
   cd* slot = slots + threadIdx.x;
   RUN_ITEMS(PERSIST, Q,
-            minor_item<DROW>(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, S, cnt, sb,
-                             slot));
+            (minor_item<DROW, STORE>(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, S,
+                                     cnt, sb, slot, FS)));
 }
 
 // The surface minor vector ms2 of every pair (src/module_solver.f90:730-741,
