@@ -386,6 +386,37 @@ class YSpec:
         tb = self.model_vjp()
         return misfit, self.model_vjp_deck(tb)
 
+    def _obs_sizes(self, nout=None):
+        out = []
+        for e, p in enumerate(self.events):
+            own = self.nout if len(self.events) == 1 else self.info[e]["nout"]
+            out.append((own if nout is None else nout) * 3 * p.nr)
+        return out
+
+    def pack_obs(self, obs, nout=None):
+        """The observed seismograms in ysx_misfit_grad's layout: per event,
+        the first nout samples of every trace, time fastest, events one
+        after another, as one float64 array (one copy).  obs: the (>= n, 3,
+        nr) seismograms of the run, or a list with one per event.  A caller
+        that reuses the same data (an inversion's iterations) packs them
+        once and hands the packed array to misfit_grad."""
+        if not isinstance(obs, (list, tuple)):
+            obs = [obs]
+        if len(obs) != len(self.events):
+            raise ValueError(f"obs must hold one array per event ({len(self.events)})")
+        sizes = self._obs_sizes(nout)
+        flat = np.empty(sum(sizes))
+        off = 0
+        for e, (o, p, m) in enumerate(zip(obs, self.events, sizes)):
+            n = m // (3 * p.nr)
+            a = np.asarray(o, dtype=np.float64)
+            if a.ndim != 3 or a.shape[0] < n or a.shape[1:] != (3, p.nr):
+                raise ValueError(f"event {e}: obs must hold (nout, 3, nr) = ({n}, 3, {p.nr}) "
+                                 "samples")
+            flat[off:off + m].reshape((n, 3, p.nr), order="F")[...] = a[:n]
+            off += m
+        return flat
+
     def misfit_grad(self, obs, nout=None):
         """ysx_misfit_grad: one forward (the batched launch when several
         events are set), the misfit 0.5 * sum((seis - obs)^2) over the first
@@ -393,21 +424,15 @@ class YSpec:
         length, the samples the drop-in program writes) and its (nknot, 4)
         gradient with respect to each knot's rho, vp, vs and Qmu (vp and vs
         moving both polarisations), summed over the events.  obs: the
-        (>= n, 3, nr) seismograms of the run, or a list with one per event."""
-        if not isinstance(obs, (list, tuple)):
-            obs = [obs]
-        if len(obs) != len(self.events):
-            raise ValueError(f"obs must hold one array per event ({len(self.events)})")
-        parts = []
-        for e, (o, p) in enumerate(zip(obs, self.events)):
-            own = self.nout if len(self.events) == 1 else self.info[e]["nout"]
-            n = own if nout is None else nout
-            a = np.asarray(o, dtype=np.float64)[:n]
-            if a.shape != (n, 3, p.nr):
-                raise ValueError(f"event {e}: obs must hold (nout, 3, nr) = ({n}, 3, {p.nr}) "
-                                 "samples")
-            parts.append(a.ravel(order="F"))
-        flat = np.ascontiguousarray(np.concatenate(parts))
+        (>= n, 3, nr) seismograms of the run, a list with one per event, or
+        their pack_obs array."""
+        if isinstance(obs, np.ndarray) and obs.ndim == 1:
+            flat = obs  # already in the library's layout (pack_obs)
+            if flat.dtype != np.float64 or not flat.flags.c_contiguous or \
+                    flat.size != sum(self._obs_sizes(nout)):
+                raise ValueError("a packed obs must be pack_obs's float64 array for this nout")
+        else:
+            flat = self.pack_obs(obs, nout)
         misfit = np.zeros(1)
         grad = np.zeros((self.deck.nknot, 4))
         self._check(self.L.ysx_misfit_grad(self.h, _dp(flat), 0 if nout is None else nout,

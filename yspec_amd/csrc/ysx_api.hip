@@ -154,6 +154,8 @@ struct ysx_ctx {
   // events that order its two halves
   void* pin = nullptr;
   size_t pin_bytes = 0;
+  void* opin = nullptr;  // ysx_misfit_grad's observed seismograms, staged
+  size_t opin_bytes = 0;
   hipEvent_t pev[2] = {};
   // ysx_misfit_grad: the observed traces and the residual's per-block sums
   double *d_obs = nullptr, *d_rpart = nullptr;
@@ -558,6 +560,7 @@ extern "C" void ysx_destroy(ysx_ctx* ctx) {
   dfree(ctx->d_mlay);
   dfree(ctx->d_mvjp);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
+  if (ctx->opin) (void)hipHostFree(ctx->opin);
   for (auto& e : ctx->pev)
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev)
@@ -1217,6 +1220,8 @@ namespace {
 // caller's memory.  Small arrays take hipMemcpy.
 constexpr size_t PIN_CHUNK = 32u << 20, PIN_SMALL = 4u << 20;
 constexpr unsigned PIN_THREADS = 8;
+// observed seismograms of ysx_misfit_grad staged in pinned memory during its forward
+constexpr size_t OBS_PIN = 64u << 20;
 
 void parallel_copy(void* dst, const void* src, size_t bytes) {
   const unsigned nt = std::max(1u, std::min(PIN_THREADS, std::thread::hardware_concurrency()));
@@ -1268,6 +1273,7 @@ int copy_d2h(ysx_ctx* ctx, void* dst, const void* src, size_t bytes) {
 }
 
 int copy_h2d(ysx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  hipStream_t st = ctx->stream;
   if (bytes < PIN_SMALL) {
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return YSX_OK;
@@ -1282,10 +1288,10 @@ int copy_h2d(ysx_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (i >= 2) HIPCHK(hipEventSynchronize(ctx->pev[h]));  // this half's DMA is done
     parallel_copy(pin + h * PIN_CHUNK, static_cast<const char*>(src) + i * PIN_CHUNK, len(i));
     HIPCHK(hipMemcpyAsync(d + i * PIN_CHUNK, pin + h * PIN_CHUNK, len(i), hipMemcpyHostToDevice,
-                          ctx->stream));
-    HIPCHK(hipEventRecord(ctx->pev[h], ctx->stream));
+                          st));
+    HIPCHK(hipEventRecord(ctx->pev[h], st));
   }
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipStreamSynchronize(st));
   return YSX_OK;
 }
 }  // namespace
@@ -1999,24 +2005,42 @@ extern "C" int ysx_misfit_grad(ysx_ctx* ctx, const double* obs, int nout, double
                                            ") for event " + std::to_string(e));
     ooff[e + 1] = ooff[e] + (size_t)ne[e] * 3 * R.nr;
   }
-  // one forward: the plain path for one event, the batched launch for several;
-  // a single event's forward also stores the minor march's per-step states
-  // for the minor sweep when they fit
-  ctx->fs_want = nev == 1 && prepare_step_store(ctx);
-  const int frc = nev == 1 ? ysx_forward(ctx, nullptr) : ysx_forward_events(ctx, 1, nullptr);
-  ctx->fs_want = false;
-  if (frc) return frc;
-  if (int rc = adjoint_run(ctx)) return rc;
   const size_t nobs = ooff[nev];
   constexpr int RBLK = 1024;
   if (ctx->obs_n < nobs) {
     dfree(ctx->d_obs);
+    dfree(ctx->d_rpart);
     HIPCHK(hipMalloc(&ctx->d_obs, nobs * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->d_rpart, (size_t)RBLK * nev * sizeof(double)));
     ctx->obs_n = nobs;
   }
-  dfree(ctx->d_rpart);
-  HIPCHK(hipMalloc(&ctx->d_rpart, (size_t)RBLK * nev * sizeof(double)));
-  if (int rc = copy_h2d(ctx, ctx->d_obs, obs, nobs * sizeof(double))) return rc;
+  // up to OBS_PIN bytes of observed seismograms are staged into pinned
+  // memory by host threads while the forward runs (no HIP calls there) and
+  // go up after it; more go up after the forward through the double buffer
+  const size_t obytes = nobs * sizeof(double);
+  const bool staged = obytes <= OBS_PIN;
+  if (staged && ctx->opin_bytes < obytes) {
+    if (ctx->opin) HIPCHK(hipHostFree(ctx->opin));
+    ctx->opin = nullptr;
+    ctx->opin_bytes = 0;
+    HIPCHK(hipHostMalloc(&ctx->opin, obytes, hipHostMallocDefault));
+    ctx->opin_bytes = obytes;
+  }
+  std::thread stage;
+  if (staged) stage = std::thread([&]() { parallel_copy(ctx->opin, obs, obytes); });
+  // one forward: the plain path for one event, the batched launch for
+  // several; a single event's forward also stores the minor march's
+  // per-step states for the minor sweep when they fit
+  ctx->fs_want = nev == 1 && prepare_step_store(ctx);
+  const int frc = nev == 1 ? ysx_forward(ctx, nullptr) : ysx_forward_events(ctx, 1, nullptr);
+  ctx->fs_want = false;
+  if (staged) stage.join();
+  if (frc) return frc;
+  if (int rc = adjoint_run(ctx)) return rc;
+  if (staged)
+    HIPCHK(hipMemcpyAsync(ctx->d_obs, ctx->opin, obytes, hipMemcpyHostToDevice, ctx->stream));
+  else if (int rc = copy_h2d(ctx, ctx->d_obs, obs, obytes))
+    return rc;
   std::vector<unsigned> nb(nev);
   for (int e = 0; e < nev; e++) {
     const HostRun& R = ctx->R[e];
