@@ -133,14 +133,15 @@ struct ysx_ctx {
   cd *d_ays = nullptr, *d_ascr = nullptr;
   int *d_ayss = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
   int adj_blocks = 0, adj_smax = 0;
-  // the minor march's per-step states stored by the single-event forward
-  // of ysx_misfit_grad for the minor sweep (StepStore): states, counters,
-  // per-w item offsets and capacities, per-pair counters, overflow flag;
+  // the minor and toroidal marches' per-step states stored by the
+  // single-event forward of ysx_misfit_grad for the sweeps (StepStore):
+  // states, counters, per-pair counters (and the toroidal continuation's
+  // end), shared per-w item offsets and capacities and overflow flag;
   // fs_valid while the records of that forward are the context's
-  cd* d_fys = nullptr;
-  int* d_fyss = nullptr;
+  cd *d_fys = nullptr, *d_tys = nullptr, *d_tyend = nullptr;
+  int *d_fyss = nullptr, *d_tyss = nullptr;
   long long* d_foff = nullptr;
-  int *d_fcap = nullptr, *d_fsc = nullptr, *d_ferr = nullptr;
+  int *d_fcap = nullptr, *d_fsc = nullptr, *d_tsc = nullptr, *d_ferr = nullptr;
   size_t fs_steps = 0;
   bool fs_want = false, fs_valid = false;
   // radial adjoint: segment propagators, start states, end rows, tape
@@ -213,7 +214,7 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
   dfree(c->d_obs); dfree(c->d_rpart);
   dfree(c->d_fys); dfree(c->d_fyss); dfree(c->d_foff); dfree(c->d_fcap); dfree(c->d_fsc);
-  dfree(c->d_ferr);
+  dfree(c->d_ferr); dfree(c->d_tys); dfree(c->d_tyss); dfree(c->d_tsc); dfree(c->d_tyend);
   c->fs_steps = 0;
   c->fs_want = c->fs_valid = false;
   c->obs_n = 0;
@@ -979,10 +980,13 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
   // a single event's minor march also stores its per-step states when
   // ysx_misfit_grad asks for them (prepare_step_store)
-  const StepStore FS{ctx->d_fys, ctx->d_fyss, ctx->d_foff, ctx->d_fcap, ctx->d_fsc, ctx->d_ferr};
-  const StepStore* fsp =
-      (ctx->fs_want && ctx->d_fys && e0 == 0 && e1 == 1 && ctx->rk.size() == 1) ? &FS : nullptr;
-  ctx->fs_valid = fsp != nullptr;
+  const StepStore FS{ctx->d_fys, ctx->d_fyss, ctx->d_foff, ctx->d_fcap,
+                     ctx->d_fsc, ctx->d_ferr, nullptr},
+      FT{ctx->d_tys, ctx->d_tyss, ctx->d_foff, ctx->d_fcap, ctx->d_tsc, ctx->d_ferr, ctx->d_tyend};
+  const bool store = ctx->fs_want && ctx->d_fys && e0 == 0 && e1 == 1 && ctx->rk.size() == 1;
+  const StepStore* fsp = store ? &FS : nullptr;
+  const StepStore* ftp = store ? &FT : nullptr;
+  ctx->fs_valid = store;
   for (int e = e0; e < e1; e++) {
     PairRange PR = pairs_of(ctx, e, e + 1);
     int nb = (int)((PR.n + 255) / 256);
@@ -1008,7 +1012,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
                        ctx->d_starts);
     if (timing) HIPCHK(hipEventRecord(ev[2], st));
     launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
-                       SD, ctx->d_rec, ctx->d_cnt);
+                       SD, ctx->d_rec, ctx->d_cnt, ftp);
     if (timing) HIPCHK(hipEventRecord(ev[3], st));
     if (ctx->rk[e].ngc) {
       launch_ngc_march(ctx->rk[e].grav == 1, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
@@ -1694,7 +1698,8 @@ bool prepare_step_store(ysx_ctx* ctx) {
     off[j] = steps;
     steps += (long long)nlb * cap[j];
   }
-  const size_t per_step = 14 * 64 * sizeof(cd) + 64 * sizeof(int);
+  // the minor march's 14-component states and the toroidal march's 2
+  const size_t per_step = 16 * 64 * sizeof(cd) + 2 * 64 * sizeof(int);
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return false;
   const size_t have = ctx->fs_steps * per_step;
@@ -1703,12 +1708,18 @@ bool prepare_step_store(ysx_ctx* ctx) {
   if ((size_t)steps > ctx->fs_steps) {
     dfree(ctx->d_fys);
     dfree(ctx->d_fyss);
+    dfree(ctx->d_tys);
+    dfree(ctx->d_tyss);
     ctx->fs_steps = 0;
     if (hipMalloc(&ctx->d_fys, (size_t)steps * 14 * 64 * sizeof(cd)) != hipSuccess ||
-        hipMalloc(&ctx->d_fyss, (size_t)steps * 64 * sizeof(int)) != hipSuccess) {
+        hipMalloc(&ctx->d_fyss, (size_t)steps * 64 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_tys, (size_t)steps * 2 * 64 * sizeof(cd)) != hipSuccess ||
+        hipMalloc(&ctx->d_tyss, (size_t)steps * 64 * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       dfree(ctx->d_fys);
       dfree(ctx->d_fyss);
+      dfree(ctx->d_tys);
+      dfree(ctx->d_tyss);
       return false;
     }
     ctx->fs_steps = (size_t)steps;
@@ -1717,11 +1728,15 @@ bool prepare_step_store(ysx_ctx* ctx) {
     if (hipMalloc(&ctx->d_foff, k.nw * sizeof(long long)) != hipSuccess ||
         hipMalloc(&ctx->d_fcap, k.nw * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->d_fsc, (size_t)ctx->npair * 3 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_tsc, (size_t)ctx->npair * 3 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_tyend, (size_t)ctx->npair * 2 * sizeof(cd)) != hipSuccess ||
         hipMalloc(&ctx->d_ferr, sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       dfree(ctx->d_foff);
       dfree(ctx->d_fcap);
       dfree(ctx->d_fsc);
+      dfree(ctx->d_tsc);
+      dfree(ctx->d_tyend);
       dfree(ctx->d_ferr);
       return false;
     }
@@ -1792,7 +1807,7 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   HIPCHK(hipGetLastError());
   SphState S{ctx->d_bst, ctx->d_mfin, ctx->d_m2, ctx->d_fac, ctx->d_sc3, ctx->d_ms1};
   StartData SD{ctx->d_tor_i1, ctx->d_tor_y0, ctx->d_sph_i1, ctx->d_sph_m0};
-  AdjWork AW{ctx->d_ays, ctx->d_ayss, 0, ctx->d_ascr, ctx->adj_smax, ctx->d_aerr};
+  AdjWork AW{ctx->d_ays, ctx->d_ayss, 0, ctx->d_ascr, ctx->adj_smax, ctx->d_aerr, 14 * 64};
   // every event's items in one persistent launch per sweep: the plain
   // lists of all events are contiguous
   auto all_q = [&](int kind) {
@@ -1812,14 +1827,18 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[1], st));
   const StepStore FS = ctx->fs_valid ? StepStore{ctx->d_fys, ctx->d_fyss, ctx->d_foff,
-                                                 ctx->d_fcap, ctx->d_fsc, ctx->d_ferr}
+                                                 ctx->d_fcap, ctx->d_fsc, ctx->d_ferr, nullptr}
+                                       : StepStore{};
+  const StepStore FT = ctx->fs_valid ? StepStore{ctx->d_tys, ctx->d_tyss, ctx->d_foff,
+                                                 ctx->d_fcap, ctx->d_tsc, ctx->d_ferr,
+                                                 ctx->d_tyend}
                                        : StepStore{};
   launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
                    ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[2], st));
   launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
-                 ctx->d_y0bar, AW, ctx->d_tpart);
+                 ctx->d_y0bar, AW, FT, ctx->d_tpart);
   HIPCHK(hipGetLastError());
   launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart, ctx->adj_blocks,
                   ntab, ctx->d_tbar);
@@ -1920,8 +1939,8 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     int ferr = 0;
     HIPCHK(hipMemcpy(&ferr, ctx->d_ferr, sizeof ferr, hipMemcpyDeviceToHost));
     if (ferr)
-      return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a minor-march item of the forward with "
-                                       "more steps than its per-step storage");
+      return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a minor or toroidal march item of the "
+                                       "forward with more steps than its per-step storage");
   }
   std::vector<double> tb(ntab);
   HIPCHK(hipMemcpy(tb.data(), ctx->d_tbar, ntab * sizeof(double), hipMemcpyDeviceToHost));

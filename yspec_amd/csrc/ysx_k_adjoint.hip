@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(256) k_part_sum(const double* __restrict__ tpa
 // order the first pass takes them, from step W.base on), and the step
 // scratch: the six stage vectors of one step and a parked 14-vector
 __device__ inline cd* ys_of(const AdjWork& W, int s) {
-  return W.ys + (W.base + s) * (14 * 64) + threadIdx.x;
+  return W.ys + (W.base + s) * W.ystride + threadIdx.x;
 }
 __device__ inline int* yss_of(const AdjWork& W, int s) {
   return W.yss + (W.base + s) * 64 + threadIdx.x;
@@ -774,6 +774,7 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
     W.yss = FS.yss;
     W.smax = FS.cap[j - 1];
     W.base = FS.off[j - 1] + (long long)lb * W.smax;
+    W.ystride = 14 * 64;
     double drr;
     for (int i = ib; i <= is; i++) {
       const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
@@ -917,7 +918,8 @@ __global__ void __launch_bounds__(64) k_minor_adj(const double* __restrict__ gta
 __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j, int lb,
                                              const StartData& SD, const cd* __restrict__ rec,
                                              const cd* __restrict__ recb, cd* __restrict__ y0bar,
-                                             const AdjWork& W, AdjLds& S, double* tbar) {
+                                             AdjWork W, const StepStore& FS, AdjLds& S,
+                                             double* tbar) {
   const int l = R.lmin + lb * 64 + (int)threadIdx.x;
   const bool valid = (l >= 2 && l <= R.lmax);
   const size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
@@ -928,25 +930,52 @@ __device__ __forceinline__ void tor_adj_item(const Tab& T, const RunK& R, int j,
   const LS L = l_state(valid ? l : 2);
   const int nk = R.nknot, is = R.is, ir = R.ir;
   const double rs = R.rs, rr = R.rr;
-  // 1. the forward's marches (tor_item), every step's state stored, the
-  // counter at the lane's start kept for the start adjoint
-  cd y1[2] = {C(0.0), C(0.0)};
-  int sc1 = 0, s = 0, sci1 = 0;
-  for (int i = ib; i <= is; i++) {
-    if (i == i1) {
-      y1[0] = SD.tor_y0[2 * p];
-      y1[1] = SD.tor_y0[2 * p + 1];
-      sci1 = sc1;
+  int sc1 = 0, sc3 = 0, s = 0, sci1 = 0;
+  cd y3[2] = {C(0.0), C(0.0)};
+  if (FS.ys) {
+    // 1. the forward stored every step's state (k_tor_march, STORE), the
+    // counters and the continuation's end: the steps are counted only
+    W.ys = FS.ys;
+    W.yss = FS.yss;
+    W.smax = FS.cap[j - 1];
+    W.base = FS.off[j - 1] + (long long)lb * W.smax;
+    W.ystride = 2 * 64;
+    double drr;
+    for (int i = ib; i <= is; i++) {
+      const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+      if (a != b) s += steps_of(T, w, i, nk, a, b, drr);
     }
-    const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
-    if (a == b) continue;
-    adj_march(T, w, L, nk, i, a, b, y1, sc1, S, TorM(), W, s);
-  }
-  cd y3[2] = {y1[0], y1[1]};
-  int sc3 = 0;
-  for (int i = is; i <= ir; i++) {
-    const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    if (a != b) adj_march(T, w, L, nk, i, a, b, y3, sc3, S, TorM(), W, s);
+    for (int i = is; i <= ir; i++) {
+      const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+      if (a != b) s += steps_of(T, w, i, nk, a, b, drr);
+    }
+    if (valid) {
+      sc1 = FS.sc[3 * p];
+      sc3 = FS.sc[3 * p + 1];
+      sci1 = FS.sc[3 * p + 2];
+      y3[0] = FS.yend[2 * p];
+      y3[1] = FS.yend[2 * p + 1];
+    }
+  } else {
+    // 1. the forward's marches (tor_item), every step's state stored, the
+    // counter at the lane's start kept for the start adjoint
+    cd y1[2] = {C(0.0), C(0.0)};
+    for (int i = ib; i <= is; i++) {
+      if (i == i1) {
+        y1[0] = SD.tor_y0[2 * p];
+        y1[1] = SD.tor_y0[2 * p + 1];
+        sci1 = sc1;
+      }
+      const double a = T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
+      if (a == b) continue;
+      adj_march(T, w, L, nk, i, a, b, y1, sc1, S, TorM(), W, s);
+    }
+    y3[0] = y1[0];
+    y3[1] = y1[1];
+    for (int i = is; i <= ir; i++) {
+      const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
+      if (a != b) adj_march(T, w, L, nk, i, a, b, y3, sc3, S, TorM(), W, s);
+    }
   }
   __syncthreads();
   if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
@@ -992,12 +1021,12 @@ __global__ void __launch_bounds__(64) k_tor_adj(const double* __restrict__ gtab,
                                                 StartData SD, const cd* __restrict__ rec,
                                                 const cd* __restrict__ recb,
                                                 cd* __restrict__ y0bar, AdjWork W,
-                                                double* __restrict__ tbar) {
+                                                StepStore FS, double* __restrict__ tbar) {
   __shared__ AdjLds S;
   const AdjWork WB = block_work(W);
   RUN_ITEMS(PERSIST, Q,
             tor_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, rec, recb,
-                         y0bar, WB, S, tbar));
+                         y0bar, WB, FS, S, tbar));
 }
 
 // --------------------------------------------------------- k_start_adj
@@ -1296,8 +1325,8 @@ void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* g
 }
 void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                     const RunK* runs, Queue Q, StartData SD, const cd* rec, const cd* recb,
-                    cd* y0bar, AdjWork W, double* tbar) {
-  hipLaunchKernelGGL(k_tor_adj<true>, g, b, lds, s, gtab, runs, Q, SD, rec, recb, y0bar, W,
+                    cd* y0bar, AdjWork W, StepStore FS, double* tbar) {
+  hipLaunchKernelGGL(k_tor_adj<true>, g, b, lds, s, gtab, runs, Q, SD, rec, recb, y0bar, W, FS,
                      tbar);
 }
 void launch_start_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,

@@ -21,8 +21,14 @@ namespace ysx {
 
 #if YSX_MARCH == 0
 void YSX_LAUNCH(tor_march)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                           const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt) {
-  hipLaunchKernelGGL(k_tor_march<YSX_PERSIST>, g, b, lds, s, gtab, runs, Q, SD, rec, cnt);
+                           const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt,
+                           const StepStore* fs) {
+  if (fs && fs->ys)
+    hipLaunchKernelGGL((k_tor_march<YSX_PERSIST, true>), g, b, lds, s, gtab, runs, Q, SD, rec,
+                       cnt, *fs);
+  else
+    hipLaunchKernelGGL((k_tor_march<YSX_PERSIST, false>), g, b, lds, s, gtab, runs, Q, SD, rec,
+                       cnt, StepStore{});
 }
 #elif YSX_MARCH == 1
 void YSX_LAUNCH(minor_top)(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,

@@ -184,21 +184,24 @@ struct SphState {
 struct AdjWork {
   cd* ys;     // the stored side's state before every step: steps base ..
   int* yss;   // base + smax - 1 of the current item (per block, or per item
-  long long base;  // when the forward stored them)
+  long long base;  // when the forward stored them), ystride complex per step
   cd* scr;    // the step scratch of each block (6 stage vectors + 1)
   int smax;
   // err[0] bit 0: an item with more steps than smax; the first is
   // described in err[1] (interval) and the doubles at err + 2: the step
   // count reached, the interval's two radii, Re w
   int* err;
+  int ystride;
 };
 
-// Per-step states of the minor march stored by a single-event forward for
-// the gradient that follows it: item (j, lb) holds cap[j - 1] steps from
-// step off[j - 1] + lb cap[j - 1] on (state [step][14][64] complex,
-// counter [step][64]); sc[3 p ..] gets pair p's renormalisation counters
-// at the source, at the receiver and at its start level; err[0] is set when
-// an item has more steps than its capacity.  ys == nullptr: not stored.
+// Per-step states of a march (the minor march, or the toroidal one) stored
+// by a single-event forward for the gradient that follows it: item (j, lb)
+// holds cap[j - 1] steps from step off[j - 1] + lb cap[j - 1] on (state
+// [step][N][64] complex, counter [step][64]); sc[3 p ..] gets pair p's
+// renormalisation counters at the source, at the receiver and at its start
+// level, and yend[N p ..] (when given) its state at the receiver; err[0] is
+// set when an item has more steps than its capacity.  ys == nullptr: not
+// stored.
 struct StepStore {
   cd* ys;
   int* yss;
@@ -206,6 +209,7 @@ struct StepStore {
   const int* cap;
   int* sc;
   int* err;
+  cd* yend;
 };
 
 // ------------------------------------------------------- march helpers
@@ -352,9 +356,11 @@ void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* g
                       const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,
                       const cd* recb, const cd* mub, cd* m0bar, AdjWork W, StepStore FS,
                       double* tbar);
+// FS.ys != nullptr: the toroidal march's per-step states from the forward
+// (k_tor_march with STORE) replace the sweep's first pass
 void launch_tor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                     const RunK* runs, Queue Q, StartData SD, const cd* rec, const cd* recb,
-                    cd* y0bar, AdjWork W, double* tbar);
+                    cd* y0bar, AdjWork W, StepStore FS, double* tbar);
 void launch_start_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                       const RunK* runs, PairRange PR, StartData SD, const int* starts,
                       const cd* m0bar, const cd* y0bar, double* tbar);
@@ -384,7 +390,8 @@ void launch_rad_seg_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double*
                         void* tape, int* err, double* tbar);
 #define YSX_MARCH_LAUNCHERS(SUFFIX)                                                              \
   void launch_tor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
-                                 const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt);   \
+                                 const RunK* runs, Queue Q, StartData SD, cd* rec, int* cnt,    \
+                                 const StepStore* fs = nullptr);                                \
   void launch_minor_top_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, \
                                  const RunK* runs, Queue Q, SphState S, int* cnt);              \
   void launch_minor_march_##SUFFIX(dim3 g, dim3 b, size_t lds, hipStream_t s,                   \

@@ -8,12 +8,50 @@
 namespace ysx {
 namespace {
 
+// the per-step states of one item of a march into a StepStore: the state
+// (N components) before each step, step s of the item at base + s
+template <int N>
+struct StorePre {
+  const StepStore& FS;
+  long long base;
+  int cap;
+  int& s;
+  __device__ void operator()(const cd* y, int sc) const {
+    if (s < cap) {
+      double* c = reinterpret_cast<double*>(FS.ys + (base + s) * (N * 64) + threadIdx.x);
+#pragma unroll
+      for (int q = 0; q < N; q++) {
+        __builtin_nontemporal_store(y[q].x, c + q * 128);
+        __builtin_nontemporal_store(y[q].y, c + q * 128 + 1);
+      }
+      __builtin_nontemporal_store(sc, FS.yss + (base + s) * 64 + threadIdx.x);
+    }
+    s++;
+  }
+};
+
+// the hook of item (j, lb): StorePre when the forward stores, else nothing
+template <bool STORE, int N>
+__device__ __forceinline__ auto store_pre(const StepStore& FS, int j, int lb, int& s) {
+  if constexpr (STORE) {
+    const int cap = FS.cap[j - 1];
+    return StorePre<N>{FS, FS.off[j - 1] + (long long)lb * cap, cap, s};
+  } else {
+    return NoPre();
+  }
+}
+
 // ------------------------------------------------------------- k_tor_march
 // tor_solver with ss = identity(2), src/module_solver.f90:25-142; a wave
 // holds 64 consecutive l at one w, largest w first.
+// STORE: also the state before every step of the marches from the start
+// level to the source and on to the receiver into FS, for the gradient
+// (receiver at or above the source only)
+template <bool STORE = false>
 __device__ __forceinline__ void tor_item(const Tab& T, const RunK& R, int j, int lb,
                                          const StartData& SD, cd* __restrict__ rec,
-                                         int* __restrict__ cnt, double* sb) {
+                                         int* __restrict__ cnt, double* sb,
+                                         const StepStore& FS) {
   int l = R.lmin + lb * 64 + (int)threadIdx.x;
   bool valid = (l >= 2 && l <= R.lmax);
   size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
@@ -30,14 +68,17 @@ __device__ __forceinline__ void tor_item(const Tab& T, const RunK& R, int j, int
   double rtop = up ? rs : rr;
   cd y1[2] = {C(0.0), C(0.0)};
   int sc1 = 0, sc2 = 0, sc3 = 0, ns = 0;
+  int fs = 0, sci1 = 0;
+  auto pre = store_pre<STORE, 2>(FS, j, lb, fs);
   for (int i = ib; i <= itop; i++) {
     if (i == i1) {
       y1[0] = SD.tor_y0[2 * p];
       y1[1] = SD.tor_y0[2 * p + 1];
+      if constexpr (STORE) sci1 = sc1;
     }
     double a = T(F_R, i), b = (i == itop) ? rtop : T(F_R, i + 1);
     if (a == b) continue;
-    int n = march_interval(T, w, L, nk, i, a, b, y1, true, sc1, sb, f);
+    int n = march_interval(T, w, L, nk, i, a, b, y1, true, sc1, sb, f, nullptr, pre);
     if (i >= i1) ns += n;
   }
   double rsl = T(F_R, R.nsl);
@@ -45,7 +86,17 @@ __device__ __forceinline__ void tor_item(const Tab& T, const RunK& R, int j, int
   if (up) {
     y3[0] = y1[0];
     y3[1] = y1[1];
-    if (rr != rs) ns += march_yint(T, w, L, nk, is, ir, rs, rr, y3, sc3, sb, f);
+    if (rr != rs) ns += march_yint(T, w, L, nk, is, ir, rs, rr, y3, sc3, sb, f, nullptr, pre);
+    if constexpr (STORE) {
+      if (threadIdx.x == 0 && fs > FS.cap[j - 1]) atomicOr(FS.err, 1);
+      if (valid) {
+        FS.sc[3 * p] = sc1;
+        FS.sc[3 * p + 1] = sc3;
+        FS.sc[3 * p + 2] = sci1;
+        FS.yend[2 * p] = y3[0];
+        FS.yend[2 * p + 1] = y3[1];
+      }
+    }
     if (rr != rsl) ns += march_yint(T, w, L, nk, R.nsl - 1, ir, rsl, rr, y2, sc2, sb, f);
     cd delta = cinv(y2[1] * y3[0] - y2[0] * y3[1]);
     cd b1 = y1[1], b2 = -y1[0];
@@ -73,14 +124,15 @@ __device__ __forceinline__ void tor_item(const Tab& T, const RunK& R, int j, int
 }
 
 // items (event, j, l-block of 64)
-template <bool PERSIST>
+template <bool PERSIST, bool STORE = false>
 __global__ void __launch_bounds__(64) k_tor_march(const double* __restrict__ gtab,
                                                   const RunK* __restrict__ runs, Queue Q,
                                                   StartData SD, cd* __restrict__ rec,
-                                                  int* __restrict__ cnt) {
+                                                  int* __restrict__ cnt, StepStore FS) {
   __shared__ double sb[6 * CH * BASE_MAX];
   RUN_ITEMS(PERSIST, Q,
-            tor_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, rec, cnt, sb));
+            tor_item<STORE>(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, rec, cnt,
+                            sb, FS));
 }
 
 // ------------------------------------------------ k_minor_top, k_minor_march
@@ -109,27 +161,6 @@ __device__ __forceinline__ int minors_from_top(const Tab& T, const WS& w, const 
   return sc2;
 }
 
-// the per-step states of one item of the minor march into a StepStore:
-// the state before each step, step s of the item at base + s
-struct StorePre {
-  const StepStore& FS;
-  long long base;
-  int cap;
-  int& s;
-  __device__ void operator()(const cd* y, int sc) const {
-    if (s < cap) {
-      double* c = reinterpret_cast<double*>(FS.ys + (base + s) * (14 * 64) + threadIdx.x);
-#pragma unroll
-      for (int q = 0; q < 14; q++) {
-        __builtin_nontemporal_store(y[q].x, c + q * 128);
-        __builtin_nontemporal_store(y[q].y, c + q * 128 + 1);
-      }
-      __builtin_nontemporal_store(sc, FS.yss + (base + s) * 64 + threadIdx.x);
-    }
-    s++;
-  }
-};
-
 // STORE: also the state before every step into FS, for the gradient
 // (receiver at or above the source only)
 template <bool DROW, bool STORE = false>
@@ -153,14 +184,7 @@ __device__ __forceinline__ void minor_item(const Tab& T, const RunK& R, int j, i
   double rtop = up ? rs : rr;
   int nsol = 0, nflu = 0;
   int fs = 0, sci1 = 0;
-  auto pre = [&]() {
-    if constexpr (STORE) {
-      const int cap = FS.cap[j - 1];
-      return StorePre{FS, FS.off[j - 1] + (long long)lb * cap, cap, fs};
-    } else {
-      return NoPre();
-    }
-  }();
+  auto pre = store_pre<STORE, 14>(FS, j, lb, fs);
   cd m[14];
   // (2) ms1 from each lane's start level up to the lower point
   double rt1 = up ? fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2)) : T(F_R, 1);
