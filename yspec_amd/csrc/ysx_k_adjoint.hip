@@ -354,8 +354,12 @@ __global__ void __launch_bounds__(256) k_part_sum(const double* __restrict__ tpa
 // the stored side's state before every step of an item (index s in the
 // order the first pass takes them, from step W.base on), and the step
 // scratch: the six stage vectors of one step and a parked 14-vector
+// (the 2-component toroidal states are packed when the forward stored
+// them: W.ystride; the others always take 14 x 64 per step)
+template <int N>
 __device__ inline cd* ys_of(const AdjWork& W, int s) {
-  return W.ys + (W.base + s) * W.ystride + threadIdx.x;
+  const long long stride = N == 2 ? W.ystride : 14 * 64;
+  return W.ys + (W.base + s) * stride + threadIdx.x;
 }
 __device__ inline int* yss_of(const AdjWork& W, int s) {
   return W.yss + (W.base + s) * 64 + threadIdx.x;
@@ -395,7 +399,7 @@ __device__ inline int steps_of(const Tab& T, const WS& w, int i, int nk, double 
 // non-temporally and leave the caches to the step scratch
 template <int N>
 __device__ inline void put_step(const AdjWork& W, int s, const cd* y, int sc) {
-  double* c = reinterpret_cast<double*>(ys_of(W, s));
+  double* c = reinterpret_cast<double*>(ys_of<N>(W, s));
 #pragma unroll
   for (int q = 0; q < N; q++) {
     __builtin_nontemporal_store(y[q].x, c + q * 128);
@@ -406,7 +410,7 @@ __device__ inline void put_step(const AdjWork& W, int s, const cd* y, int sc) {
 
 template <int N>
 __device__ inline void load_step(const AdjWork& W, int s, cd* y, int& sc) {
-  const double* c = reinterpret_cast<const double*>(ys_of(W, s));
+  const double* c = reinterpret_cast<const double*>(ys_of<N>(W, s));
 #pragma unroll
   for (int q = 0; q < N; q++)
     y[q] = C(__builtin_nontemporal_load(c + q * 128), __builtin_nontemporal_load(c + q * 128 + 1));
@@ -774,7 +778,6 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
     W.yss = FS.yss;
     W.smax = FS.cap[j - 1];
     W.base = FS.off[j - 1] + (long long)lb * W.smax;
-    W.ystride = 14 * 64;
     double drr;
     for (int i = ib; i <= is; i++) {
       const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
