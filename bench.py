@@ -268,9 +268,13 @@ def gradient_compare(deck, steps, warmup):
             "bvec_sweep": float(grad_k[0]), "minor_sweep": float(grad_k[1]),
             "tor_sweep": float(grad_k[2]), "starting_solutions": float(grad_k[3]),
             "radial": float(grad_k[4]), "source": float(grad_k[5])},
+        "gradient_device_bytes": dict(zip(("per_step_states", "step_scratch_and_radial_tape",
+                                           "per_pair_adjoints"),
+                                          map(float, g.grad_memory()))),
         "note": "gradient_ms: one ysx_misfit_grad (forward included; the observed data "
                 "packed once beforehand, as an inversion reuses them) over one forward's "
-                "wall time, medians of the steps",
+                "wall time, medians of the steps; per_step_states include the minor and "
+                "toroidal states the single-event forward stores for the sweeps",
     }
     g.close()
     return out

@@ -285,16 +285,22 @@ int ysx_model_vjp(ysx_ctx *ctx, double *table_bar);
  * each trace in ysx_seismograms' layout.  The residuals are formed on the
  * device; the chain is ysx_seismograms_vjp, ysx_spectra_vjp, ysx_model_vjp
  * and ysx_model_table_vjp, with ysx_model_vjp's scope; stage times from
- * ysx_timings, ysx_vjp_timings and ysx_grad_timings afterwards. */
+ * ysx_timings, ysx_vjp_timings and ysx_grad_timings afterwards.  With one
+ * event the forward also stores the minor and toroidal marches' state
+ * before every step (when it fits in 32 GiB and half the free device
+ * memory; kept with the run, ysx_grad_memory), and their sweeps start from
+ * those states instead of marching again.  obs up to 64 MiB is staged into
+ * pinned host memory while the forward runs. */
 int ysx_misfit_grad(ysx_ctx *ctx, const double *obs, int nout, double *misfit,
                     double *knot_grad);
 
 /* Device bytes the model gradient holds for the run (after a
  * ysx_model_vjp; no counterpart in the reference): [0] the sweeps'
  * per-step states (the stored side's state before every step of the
- * longest item, per persistent block), [1] the per-block step scratch and
- * the radial tape — [0] and [1] do not grow with the number of pairs — and
- * [2] the per-pair adjoints (record adjoints, minors at the source and
+ * longest item, per persistent block) and those that a single-event
+ * ysx_misfit_grad's forward stored, [1] the per-block step scratch and the
+ * radial tape — the per-block parts do not grow with the number of pairs —
+ * and [2] the per-pair adjoints (record adjoints, minors at the source and
  * start) and the per-block partial table adjoints. */
 int ysx_grad_memory(const ysx_ctx *ctx, double *bytes3);
 

@@ -2108,7 +2108,8 @@ extern "C" int ysx_grad_memory(const ysx_ctx* ctx, double* bytes3) {
   if (!ctx || !bytes3) return YSX_ERR_ARG;
   const double nb = ctx->adj_blocks, sm = ctx->adj_smax;
   const double P = (double)ctx->npair, nk = ctx->M.nknot;
-  bytes3[0] = nb * sm * 64 * (14 * sizeof(cd) + sizeof(int));         // per-step states
+  bytes3[0] = nb * sm * 64 * (14 * sizeof(cd) + sizeof(int)) +        // per-step states, and
+              (double)ctx->fs_steps * 64 * (16 * sizeof(cd) + 2 * sizeof(int));  // the forward's
   bytes3[1] = nb * 13 * 14 * 64 * sizeof(cd) +                         // step scratch,
               (double)ctx->rtape_bytes;                                // radial tape
   bytes3[2] = ctx->d_tbar ? P * (2 * 14 + 2 + REC) * sizeof(cd) + nb * F_COUNT * nk * 8.0
