@@ -1008,7 +1008,7 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
                            ctx->d_rec, ctx->d_cnt);
     }
     HIPCHK(hipEventRecord(ev[11], st2));
-    launch_start(dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
+    launch_start(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, SD,
                        ctx->d_starts);
     if (timing) HIPCHK(hipEventRecord(ev[2], st));
     launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
@@ -1106,7 +1106,7 @@ int enqueue_batched(ysx_ctx* ctx) {
                          ctx->d_rec, ctx->d_cnt);
   }
   HIPCHK(hipEventRecord(ev[11], st2));
-  launch_start(dim3(nb), b256, 0, st, ctx->d_tab, ctx->d_runs, PR, SD,
+  launch_start(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, SD,
                      ctx->d_starts);
   if (pe) {
     launch_tor_march_pers(grid(qt), b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD,

@@ -127,11 +127,13 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
 __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
                                                const RunK* __restrict__ runs, PairRange PR,
                                                StartData SD, int* __restrict__ starts) {
+  // the start-level walks read the table knot after knot: from LDS when it
+  // fits there (all events share the deck)
+  const Tab T = load_tab(gtab, runs[PR.e0].nknot, runs[PR.e0].tablds);
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= PR.n) return;
   long long g = PR.g0 + t;
   const RunK& R = runs[find_event(runs, PR.e0, PR.e1, g)];
-  Tab T{gtab, R.nknot};
   int l, j;
   pair_lj(R, (int)(g - R.pbase), l, j);
   size_t p = (size_t)g;
