@@ -120,6 +120,8 @@ struct ysx_ctx {
   int persist = 0, chunk = 1, blocks_per_cu = 4;
   // lanes per radial (l = 0) solve: 0 automatic, 1 or 8 (ysx_set_lanes)
   int rad_lanes = 0;
+  // the last plain forward ran its toroidal chain on the second stream
+  bool tor_side = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t ev[16] = {};
@@ -129,7 +131,7 @@ struct ysx_ctx {
   // source from the b-vector starts (mub), of the starting solutions (m0bar,
   // y0bar), the table adjoint, the sweeps' per-block checkpoints and tapes
   cd *d_mub = nullptr, *d_m0bar = nullptr, *d_y0bar = nullptr;
-  double *d_tbar = nullptr, *d_tpart = nullptr;
+  double *d_tbar = nullptr, *d_tpart = nullptr, *d_tpart2 = nullptr;
   cd *d_ays = nullptr, *d_ascr = nullptr;
   int *d_ayss = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
   int adj_blocks = 0, adj_smax = 0;
@@ -209,6 +211,7 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
   dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
   dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_tpart);
+  dfree(c->d_tpart2);
   dfree(c->d_ays); dfree(c->d_ayss); dfree(c->d_ascr); dfree(c->d_aerr);
   dfree(c->d_aq);
   dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
@@ -996,8 +999,21 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
     launch_source(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR,
                        ctx->d_rec, ctx->d_cnt);
     HIPCHK(hipEventRecord(ev[1], st));
-    // the radial (l = 0) solves run beside the marches on a second stream
+    // the radial (l = 0) solves run beside the marches on a second stream.
+    // An event too small to fill the GPU with its spheroidal chain (fewer
+    // than four waves per SIMD) runs its toroidal chain (starts and march)
+    // there too, ahead of the radial solves: none of them feeds the others
+    // before the assembly, and the spheroidal chain is the longer one
+    const bool side = qt.n < 16 * ctx->ncu;
+    ctx->tor_side = side;
     HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+    if (side) {
+      if (timing) HIPCHK(hipEventRecord(ev[9], st2));
+      launch_start(dim3(nb), b256, lds, st2, ctx->d_tab, ctx->d_runs, PR, SD, ctx->d_starts, 1);
+      launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st2, ctx->d_tab, ctx->d_runs, qt,
+                            SD, ctx->d_rec, ctx->d_cnt, ftp);
+      if (timing) HIPCHK(hipEventRecord(ev[3], st2));
+    }
     if (timing) HIPCHK(hipEventRecord(ev[10], st2));
     if (qr.n > 0) {
       if (ctx->rk[e].grav == 0)
@@ -1008,12 +1024,14 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
                            ctx->d_rec, ctx->d_cnt);
     }
     HIPCHK(hipEventRecord(ev[11], st2));
-    launch_start(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, SD,
-                       ctx->d_starts);
+    launch_start(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, SD, ctx->d_starts,
+                 side ? 2 : 3);
     if (timing) HIPCHK(hipEventRecord(ev[2], st));
-    launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
-                       SD, ctx->d_rec, ctx->d_cnt, ftp);
-    if (timing) HIPCHK(hipEventRecord(ev[3], st));
+    if (!side) {
+      launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                            SD, ctx->d_rec, ctx->d_cnt, ftp);
+      if (timing) HIPCHK(hipEventRecord(ev[3], st));
+    }
     if (ctx->rk[e].ngc) {
       launch_ngc_march(ctx->rk[e].grav == 1, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                        ctx->d_runs, qt, ctx->d_rec, ctx->d_cnt);
@@ -1174,8 +1192,10 @@ int run_spectra(ysx_ctx* ctx) {
   HIPCHK(hipEventSynchronize(ctx->ev[8]));
   double a, b;
   if ((rc = elapsed(ctx, 0, 1, a)) || (rc = elapsed(ctx, 1, 2, b))) return rc;
-  ctx->ms[0] = a + b;  // source vectors + starts
-  if ((rc = elapsed(ctx, 2, 3, ctx->ms[1])) || (rc = elapsed(ctx, 3, 4, ctx->ms[2])) ||
+  ctx->ms[0] = a + b;  // source vectors + start levels (the spheroidal ones
+                       // alone when the toroidal chain ran beside)
+  if ((rc = ctx->tor_side ? elapsed(ctx, 9, 3, ctx->ms[1]) : elapsed(ctx, 2, 3, ctx->ms[1])) ||
+      (rc = elapsed(ctx, ctx->tor_side ? 2 : 3, 4, ctx->ms[2])) ||
       (rc = elapsed(ctx, 4, 5, ctx->ms[3])) || (rc = elapsed(ctx, 10, 11, ctx->ms[4])) ||
       (rc = elapsed(ctx, 6, 7, ctx->ms[5])) || (rc = elapsed(ctx, 7, 8, ctx->ms[6])))
     return rc;
@@ -1789,6 +1809,7 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     HIPCHK(hipMalloc(&ctx->d_y0bar, P * 2 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_tbar, (size_t)F_COUNT * nk * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->d_tpart, (size_t)F_COUNT * nk * nb * sizeof(double)));
+    HIPCHK(hipMalloc(&ctx->d_tpart2, (size_t)F_COUNT * nk * nb * sizeof(double)));
     HIPCHK(hipMalloc(&ctx->d_ays, nb * sm * 14 * 64 * sizeof(cd)));
     HIPCHK(hipMalloc(&ctx->d_ayss, nb * sm * 64 * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->d_ascr, nb * 13 * 14 * 64 * sizeof(cd)));
@@ -1802,6 +1823,7 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   launch_fill0(dim3((unsigned)std::min<size_t>(1024, (ntab + 255) / 256)), b256, 0, st,
                ctx->d_tbar, ntab);
   launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart, ntab * ctx->adj_blocks);
+  launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart2, ntab * ctx->adj_blocks);
   launch_reset(dim3(1), b64, 0, st, ctx->d_aerr, 16);
   launch_reset(dim3(1), b64, 0, st, ctx->d_aq, 4);
   HIPCHK(hipGetLastError());
@@ -1821,11 +1843,6 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   qt.counter = ctx->d_aq + 2;
   qb.chunk = qm.chunk = qt.chunk = 1;
   const dim3 gb(ctx->adj_blocks);
-  HIPCHK(hipEventRecord(ev[0], st));
-  launch_brow_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_recb,
-                  ctx->d_mub, AW, ctx->d_tpart);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[1], st));
   const StepStore FS = ctx->fs_valid ? StepStore{ctx->d_fys, ctx->d_fyss, ctx->d_foff,
                                                  ctx->d_fcap, ctx->d_fsc, ctx->d_ferr, nullptr}
                                        : StepStore{};
@@ -1833,15 +1850,43 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
                                                  ctx->d_fcap, ctx->d_tsc, ctx->d_ferr,
                                                  ctx->d_tyend}
                                        : StepStore{};
+  // With the toroidal states from the forward, the toroidal sweep uses none
+  // of the blocks' per-step storage or scratch: it runs on the second
+  // stream beside the minor sweep, whose persistent blocks hold every
+  // compute unit until their queue is empty and then leave them one by one
+  // as their last items end, into its own partial table columns.
+  // Otherwise the three sweeps share that storage and run one after another.
+  const bool side = FT.ys != nullptr;
+  hipStream_t st2 = ctx->stream2;
+  HIPCHK(hipEventRecord(ev[0], st));
+  launch_brow_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_recb,
+                  ctx->d_mub, AW, ctx->d_tpart);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[1], st));
+  if (side) {
+    HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+    HIPCHK(hipEventRecord(ev[12], st2));
+    launch_tor_adj(gb, b64, 0, st2, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
+                   ctx->d_y0bar, AW, FT, ctx->d_tpart2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev[13], st2));
+  }
   launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
                    ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[2], st));
-  launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
-                 ctx->d_y0bar, AW, FT, ctx->d_tpart);
-  HIPCHK(hipGetLastError());
+  if (side) {
+    HIPCHK(hipStreamWaitEvent(st, ev[13], 0));
+  } else {
+    launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
+                   ctx->d_y0bar, AW, FT, ctx->d_tpart);
+    HIPCHK(hipGetLastError());
+  }
   launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart, ctx->adj_blocks,
                   ntab, ctx->d_tbar);
+  if (side)
+    launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart2,
+                    ctx->adj_blocks, ntab, ctx->d_tbar);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[3], st));
   {
@@ -1920,6 +1965,9 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   HIPCHK(hipEventSynchronize(ev[5]));
   for (int q = 0; q < 5; q++)
     if (int rc = elapsed(ctx, q, q + 1, ctx->ms_grad[q])) return rc;
+  // the toroidal sweep on the second stream (its own events)
+  if (side)
+    if (int rc = elapsed(ctx, 12, 13, ctx->ms_grad[2])) return rc;
   int errw[16];
   HIPCHK(hipMemcpy(errw, ctx->d_aerr, sizeof errw, hipMemcpyDeviceToHost));
   const int err = errw[0];

@@ -123,10 +123,13 @@ __global__ void __launch_bounds__(256) k_source(const double* __restrict__ gtab,
 // sph_solver's start (:678-722, 758-785); with a fluid inner core
 // (receiver not below the source) sph_solver_fic's (:1240-1265), which
 // starts the fluid minors in the inner core.  The marches pick these up
-// when they reach each lane's start knot.
+// when they reach each lane's start knot.  parts: bit 0 the toroidal
+// starts, bit 1 the spheroidal ones (separate launches let the toroidal
+// chain run on another stream beside the spheroidal one).
 __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
                                                const RunK* __restrict__ runs, PairRange PR,
-                                               StartData SD, int* __restrict__ starts) {
+                                               StartData SD, int* __restrict__ starts,
+                                               int parts) {
   // the start-level walks read the table knot after knot: from LDS when it
   // fits there (all events share the deck)
   const Tab T = load_tab(gtab, runs[PR.e0].nknot, runs[PR.e0].tablds);
@@ -143,7 +146,7 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
   Layers L = layers(R);
   bool up = R.rr >= R.rs;
   int itop = up ? R.is : R.ir;
-  if (l >= 2) {
+  if ((parts & 1) && l >= 2) {
     int i1 = tor_start_level(T, L, l, s.rew, itop);
     starts[2 * p] = i1;
     if (i1 == itop && i1 > L.noc + 1) i1--;
@@ -158,7 +161,8 @@ __global__ void __launch_bounds__(256) k_start(const double* __restrict__ gtab,
     SD.tor_y0[2 * p] = y[0];
     SD.tor_y0[2 * p + 1] = y[1];
   }
-  if (R.ngc) return;  // the gravity-0/1 march finds its own start
+  // the gravity-0/1 march finds its own start
+  if (!(parts & 2) || R.ngc) return;
   int i1 = sph_start_level(T, L, l, s.rew, itop, R.fre_norm);
   starts[2 * p + 1] = i1;
   double rt = T(F_R, i1);
@@ -1494,8 +1498,8 @@ void launch_source(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab
   hipLaunchKernelGGL(k_source, g, b, lds, s, gtab, runs, PR, rec, cnt);
 }
 void launch_start(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                  const RunK* runs, PairRange PR, StartData SD, int* starts) {
-  hipLaunchKernelGGL(k_start, g, b, lds, s, gtab, runs, PR, SD, starts);
+                  const RunK* runs, PairRange PR, StartData SD, int* starts, int parts) {
+  hipLaunchKernelGGL(k_start, g, b, lds, s, gtab, runs, PR, SD, starts, parts);
 }
 void launch_rad(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, const RunK* runs,
                 Queue Q, cd* rec, int* cnt) {

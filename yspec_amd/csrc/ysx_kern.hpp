@@ -295,8 +295,9 @@ void launch_residual(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* se
                      double* part);
 void launch_source(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                    const RunK* runs, PairRange PR, cd* rec, int* cnt);
+// parts: bit 0 the toroidal starts, bit 1 the spheroidal ones
 void launch_start(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
-                  const RunK* runs, PairRange PR, StartData SD, int* starts);
+                  const RunK* runs, PairRange PR, StartData SD, int* starts, int parts = 3);
 void launch_rad(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab, const RunK* runs,
                 Queue Q, cd* rec, int* cnt);
 void launch_rad_ng(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
