@@ -1842,64 +1842,9 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   qm.counter = ctx->d_aq + 1;
   qt.counter = ctx->d_aq + 2;
   qb.chunk = qm.chunk = qt.chunk = 1;
-  const dim3 gb(ctx->adj_blocks);
-  const StepStore FS = ctx->fs_valid ? StepStore{ctx->d_fys, ctx->d_fyss, ctx->d_foff,
-                                                 ctx->d_fcap, ctx->d_fsc, ctx->d_ferr, nullptr}
-                                       : StepStore{};
-  const StepStore FT = ctx->fs_valid ? StepStore{ctx->d_tys, ctx->d_tyss, ctx->d_foff,
-                                                 ctx->d_fcap, ctx->d_tsc, ctx->d_ferr,
-                                                 ctx->d_tyend}
-                                       : StepStore{};
-  // With the toroidal states from the forward, the toroidal sweep uses none
-  // of the blocks' per-step storage or scratch: it runs on the second
-  // stream beside the minor sweep, whose persistent blocks hold every
-  // compute unit until their queue is empty and then leave them one by one
-  // as their last items end, into its own partial table columns.
-  // Otherwise the three sweeps share that storage and run one after another.
-  const bool side = FT.ys != nullptr;
-  hipStream_t st2 = ctx->stream2;
-  HIPCHK(hipEventRecord(ev[0], st));
-  launch_brow_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_recb,
-                  ctx->d_mub, AW, ctx->d_tpart);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[1], st));
-  if (side) {
-    HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
-    HIPCHK(hipEventRecord(ev[12], st2));
-    launch_tor_adj(gb, b64, 0, st2, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
-                   ctx->d_y0bar, AW, FT, ctx->d_tpart2);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev[13], st2));
-  }
-  launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
-                   ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[2], st));
-  if (side) {
-    HIPCHK(hipStreamWaitEvent(st, ev[13], 0));
-  } else {
-    launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
-                   ctx->d_y0bar, AW, FT, ctx->d_tpart);
-    HIPCHK(hipGetLastError());
-  }
-  launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart, ctx->adj_blocks,
-                  ntab, ctx->d_tbar);
-  if (side)
-    launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart2,
-                    ctx->adj_blocks, ntab, ctx->d_tbar);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ev[3], st));
-  {
-    PairRange PR = pairs_of(ctx, 0, nev);
-    launch_start_adj(dim3((unsigned)((PR.n + 255) / 256)), b256,
-                     (size_t)SV_N * 2 * nk * sizeof(double), st, ctx->d_tab, ctx->d_runs, PR, SD,
-                     ctx->d_starts, ctx->d_m0bar, ctx->d_y0bar, ctx->d_tbar);
-    HIPCHK(hipGetLastError());
-  }
-  HIPCHK(hipEventRecord(ev[4], st));
   // the radial solves of the events from degree 0, by segments (sweep,
-  // knot interval), one event after another on shared buffers
-  {
+  // knot interval), one event after another on shared buffers, on stream rs
+  auto radial = [&](hipStream_t rs) -> int {
     const Tab HT{M.tab.data(), nk};
     std::vector<int> rev;
     std::vector<RadSegs> segs;
@@ -1931,11 +1876,11 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
       size_t bytes = 1;
       for (size_t q = 0; q < rev.size(); q++) {
         const RadSegs& G = segs[q];
-        launch_reset(dim3(1), b64, 0, st, ctx->d_rcap, 1);
-        launch_rad_prop(dim3(G.nseg, G.nwb), b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G,
+        launch_reset(dim3(1), b64, 0, rs, ctx->d_rcap, 1);
+        launch_rad_prop(dim3(G.nseg, G.nwb), b64, 0, rs, ctx->d_tab, ctx->d_runs, rev[q], G,
                         ctx->d_rprop, ctx->d_rcap);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipStreamSynchronize(rs));
         int cap = 0;
         HIPCHK(hipMemcpy(&cap, ctx->d_rcap, sizeof(int), hipMemcpyDeviceToHost));
         ctx->rad_caps.push_back(cap);
@@ -1951,23 +1896,86 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
       RadSegs G = segs[q];
       G.cap = ctx->rad_caps[q];
       const dim3 gs(G.nseg, G.nwb);
-      launch_reset(dim3(1), b64, 0, st, ctx->d_rcap, 1);
-      launch_rad_prop(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G, ctx->d_rprop,
+      launch_reset(dim3(1), b64, 0, rs, ctx->d_rcap, 1);
+      launch_rad_prop(gs, b64, 0, rs, ctx->d_tab, ctx->d_runs, rev[q], G, ctx->d_rprop,
                       ctx->d_rcap);
-      launch_rad_scan(dim3(G.nwb), b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G,
+      launch_rad_scan(dim3(G.nwb), b64, 0, rs, ctx->d_tab, ctx->d_runs, rev[q], G,
                       ctx->d_rprop, ctx->d_recb, ctx->d_rys, ctx->d_rxe, ctx->d_tbar);
-      launch_rad_seg_adj(gs, b64, 0, st, ctx->d_tab, ctx->d_runs, rev[q], G, ctx->d_rys,
+      launch_rad_seg_adj(gs, b64, 0, rs, ctx->d_tab, ctx->d_runs, rev[q], G, ctx->d_rys,
                          ctx->d_rxe, ctx->d_rtape, ctx->d_aerr, ctx->d_tbar);
       HIPCHK(hipGetLastError());
     }
+      return YSX_OK;
+  };
+  const dim3 gb(ctx->adj_blocks);
+  const StepStore FS = ctx->fs_valid ? StepStore{ctx->d_fys, ctx->d_fyss, ctx->d_foff,
+                                                 ctx->d_fcap, ctx->d_fsc, ctx->d_ferr, nullptr}
+                                       : StepStore{};
+  const StepStore FT = ctx->fs_valid ? StepStore{ctx->d_tys, ctx->d_tyss, ctx->d_foff,
+                                                 ctx->d_fcap, ctx->d_tsc, ctx->d_ferr,
+                                                 ctx->d_tyend}
+                                       : StepStore{};
+  // With the toroidal states from the forward, the toroidal sweep uses none
+  // of the blocks' per-step storage or scratch: it runs on the second
+  // stream beside the minor sweep, whose persistent blocks hold every
+  // compute unit until their queue is empty and then leave them one by one
+  // as their last items end, into its own partial table columns.
+  // Otherwise the three sweeps share that storage and run one after another.
+  const bool side = FT.ys != nullptr;
+  hipStream_t st2 = ctx->stream2;
+  HIPCHK(hipEventRecord(ev[0], st));
+  launch_brow_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S, ctx->d_rec, ctx->d_recb,
+                  ctx->d_mub, AW, ctx->d_tpart);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[1], st));
+  if (side) {
+    HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
+    HIPCHK(hipEventRecord(ev[12], st2));
+    launch_tor_adj(gb, b64, 0, st2, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
+                   ctx->d_y0bar, AW, FT, ctx->d_tpart2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev[14], st2));
+    // the radial solves' adjoint after it, also beside the minor sweep
+    if (int rc = radial(st2)) return rc;
+    HIPCHK(hipEventRecord(ev[15], st2));
   }
+  launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
+                   ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[2], st));
+  if (side) {
+    HIPCHK(hipStreamWaitEvent(st, ev[15], 0));
+  } else {
+    launch_tor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qt, SD, ctx->d_rec, ctx->d_recb,
+                   ctx->d_y0bar, AW, FT, ctx->d_tpart);
+    HIPCHK(hipGetLastError());
+  }
+  launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart, ctx->adj_blocks,
+                  ntab, ctx->d_tbar);
+  if (side)
+    launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart2,
+                    ctx->adj_blocks, ntab, ctx->d_tbar);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ev[3], st));
+  {
+    PairRange PR = pairs_of(ctx, 0, nev);
+    launch_start_adj(dim3((unsigned)((PR.n + 255) / 256)), b256,
+                     (size_t)SV_N * 2 * nk * sizeof(double), st, ctx->d_tab, ctx->d_runs, PR, SD,
+                     ctx->d_starts, ctx->d_m0bar, ctx->d_y0bar, ctx->d_tbar);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(ev[4], st));
+  if (!side)
+    if (int rc = radial(st)) return rc;
   HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipEventSynchronize(ev[5]));
   for (int q = 0; q < 5; q++)
     if (int rc = elapsed(ctx, q, q + 1, ctx->ms_grad[q])) return rc;
-  // the toroidal sweep on the second stream (its own events)
+  // the toroidal sweep and the radial adjoint on the second stream (their
+  // own events)
   if (side)
-    if (int rc = elapsed(ctx, 12, 13, ctx->ms_grad[2])) return rc;
+    if (int rc = elapsed(ctx, 12, 14, ctx->ms_grad[2]) || elapsed(ctx, 14, 15, ctx->ms_grad[4]))
+      return rc;
   int errw[16];
   HIPCHK(hipMemcpy(errw, ctx->d_aerr, sizeof errw, hipMemcpyDeviceToHost));
   const int err = errw[0];
