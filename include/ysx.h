@@ -306,7 +306,11 @@ int ysx_grad_memory(const ysx_ctx *ctx, double *bytes3);
 
 /* Kernel milliseconds of the last ysx_model_vjp: [0] b-vector sweep, [1]
  * minor sweep, [2] toroidal sweep, [3] starting solutions, [4] radial
- * solves, [5] source vectors, [6] the sum. */
+ * solves, [5] source vectors, [6] the sum.  When the forward stored the
+ * toroidal states (a single-event ysx_misfit_grad), the toroidal sweep and
+ * then the radial solves run on a second stream beside the minor sweep:
+ * [2] and [4] are their elapsed times there, overlapped with [1], and [6]
+ * exceeds the wall time. */
 int ysx_grad_timings(const ysx_ctx *ctx, double *ms7);
 
 /* Kernel milliseconds (HIP events on the context's stream, host copies
