@@ -1489,24 +1489,31 @@ int adjoint_run(ysx_ctx* ctx) {
 
 // The per-block sums of k_source_T / k_source_model_T of every event, added
 // per event in block order: out[6 e + i].
-int source_sums(ysx_ctx* ctx, bool model, std::vector<double>& out) {
+// the per-block partial sums of k_source_T / k_source_model_T of every
+// event into d_mmpart on stream st, between events ev[a] and ev[b]
+int enqueue_source_sums(ysx_ctx* ctx, bool model, hipStream_t st, int a, int b) {
   const int nev = (int)ctx->rk.size();
-  hipStream_t st = ctx->stream;
-  std::vector<size_t> b0(nev + 1, 0);
-  for (int e = 0; e < nev; e++) b0[e + 1] = b0[e] + src_blocks(ctx->rk[e]);
-  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  size_t b0 = 0;
+  HIPCHK(hipEventRecord(ctx->ev[a], st));
   for (int e = 0; e < nev; e++) {
-    double* part = ctx->d_mmpart + 6 * b0[e];
+    double* part = ctx->d_mmpart + 6 * b0;
     dim3 g(src_blocks(ctx->rk[e]));
     if (model)
       launch_source_model_T(g, dim3(256), 0, st, ctx->d_tab, ctx->d_runs, e, ctx->d_recb, part);
     else
       launch_source_T(g, dim3(256), 0, st, ctx->d_tab, ctx->d_runs, e, ctx->d_recb, part);
+    b0 += src_blocks(ctx->rk[e]);
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ctx->ev[1], st));
-  HIPCHK(hipEventSynchronize(ctx->ev[1]));
-  if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[2])) return rc;
+  HIPCHK(hipEventRecord(ctx->ev[b], st));
+  return YSX_OK;
+}
+
+// their sums per event (6 each), once the launches are complete
+int collect_source_sums(ysx_ctx* ctx, std::vector<double>& out) {
+  const int nev = (int)ctx->rk.size();
+  std::vector<size_t> b0(nev + 1, 0);
+  for (int e = 0; e < nev; e++) b0[e + 1] = b0[e] + src_blocks(ctx->rk[e]);
   std::vector<double> part(6 * b0[nev]);
   HIPCHK(hipMemcpy(part.data(), ctx->d_mmpart, part.size() * sizeof(double), hipMemcpyDeviceToHost));
   out.assign(6 * (size_t)nev, 0.0);
@@ -1514,6 +1521,42 @@ int source_sums(ysx_ctx* ctx, bool model, std::vector<double>& out) {
     for (int i = 0; i < 6; i++)
       for (size_t b = b0[e]; b < b0[e + 1]; b++) out[6 * (size_t)e + i] += part[6 * b + i];
   return YSX_OK;
+}
+
+int source_sums(ysx_ctx* ctx, bool model, std::vector<double>& out) {
+  if (int rc = enqueue_source_sums(ctx, model, ctx->stream, 0, 1)) return rc;
+  HIPCHK(hipEventSynchronize(ctx->ev[1]));
+  if (int rc = elapsed(ctx, 0, 1, ctx->ms_vjp[2])) return rc;
+  return collect_source_sums(ctx, out);
+}
+
+// the source-vector term of the table adjoint from the per-event sums G
+// (source_model_T's six moduli fields at each event's source radius, on its
+// knot interval's spline weights, as spl_w computes them), into table_bar
+void source_model_fold(const ysx_ctx* ctx, const std::vector<double>& G, double* table_bar) {
+  const int nk = ctx->M.nknot;
+  const int fields[6] = {F_LCON, F_CCON, F_FCON, F_QMU, F_XLAM, F_XA2};
+  for (size_t e = 0; e < ctx->rk.size(); e++) {
+    const RunK& k = ctx->rk[e];
+    const double r0 = ctx->M.at(F_R, k.is), r1 = ctx->M.at(F_R, k.is + 1);
+    const double h = r1 - r0;
+    const bool deg = (h == 0.0);
+    const double ih = deg ? 0.0 : 1.0 / h, a = (r1 - k.rs) * ih, bw = (k.rs - r0) * ih;
+    const double hh = h * h / 6.0, a3 = (a * a * a - a) * hh, b3 = (bw * bw * bw - bw) * hh;
+    for (int i = 0; i < 6; i++) {
+      const double g = G[6 * e + i];
+      double* y = table_bar + (size_t)fields[i] * nk + (k.is - 1);
+      double* y2 = table_bar + (size_t)(fields[i] + 1) * nk + (k.is - 1);
+      if (deg) {
+        y[0] += g;
+        continue;
+      }
+      y[0] += a * g;
+      y[1] += bw * g;
+      y2[0] += a3 * g;
+      y2[1] += b3 * g;
+    }
+  }
 }
 
 int seis_vjp_device(ysx_ctx* ctx);
@@ -1637,47 +1680,17 @@ extern "C" int ysx_source_model_vjp(ysx_ctx* ctx, double* table_bar) {
   if (!ctx->have_recb) return set_err(ctx, YSX_ERR_ARG, "no record adjoints: call ysx_spectra_vjp first");
   std::vector<double> G;
   if (int rc = source_sums(ctx, true, G)) return rc;
-  const int nk = ctx->rk[0].nknot;
-  std::fill(table_bar, table_bar + (size_t)F_COUNT * nk, 0.0);
-  const int fields[6] = {F_LCON, F_CCON, F_FCON, F_QMU, F_XLAM, F_XA2};
-  for (size_t e = 0; e < ctx->rk.size(); e++) {
-    const RunK& k = ctx->rk[e];
-    // the spline weights of the event's source radius on its knot interval
-    // is (1-based), as spl_w computes them; every pair reads the same values
-    double r[2];
-    HIPCHK(hipMemcpy(r, ctx->d_tab + (size_t)F_R * nk + (k.is - 1), 2 * sizeof(double),
-                     hipMemcpyDeviceToHost));
-    const double h = r[1] - r[0];
-    const bool deg = (h == 0.0);
-    const double ih = deg ? 0.0 : 1.0 / h, a = (r[1] - k.rs) * ih, bw = (k.rs - r[0]) * ih;
-    const double hh = h * h / 6.0, a3 = (a * a * a - a) * hh, b3 = (bw * bw * bw - bw) * hh;
-    for (int i = 0; i < 6; i++) {
-      const double g = G[6 * e + i];
-      double* y = table_bar + (size_t)fields[i] * nk + (k.is - 1);
-      double* y2 = table_bar + (size_t)(fields[i] + 1) * nk + (k.is - 1);
-      if (deg) {
-        y[0] += g;
-        continue;
-      }
-      y[0] += a * g;
-      y[1] += bw * g;
-      y2[0] += a3 * g;
-      y2[1] += b3 * g;
-    }
-  }
+  std::fill(table_bar, table_bar + (size_t)F_COUNT * ctx->M.nknot, 0.0);
+  source_model_fold(ctx, G, table_bar);
   return YSX_OK;
 }
 
 namespace {
 
-// A bound on the steps of one sweep item at the run's highest frequency
-// (the step rule of yint, src/module_int.f90:1903-1905): every knot
-// interval's full step list once, the longest twice (a march splits the
-// source interval in two, each part taking at most the full list): the
-// per-step storage of the adjoint sweeps.
-// the most steps one minor-march item at Re w = rew can take: the step
-// rule's count over every knot interval, plus the splits of the source and
-// receiver intervals
+// A bound on the steps of one sweep item at Re w = rew (the step rule of
+// yint, src/module_int.f90:1903-1905): every knot interval's full step list
+// once, the longest twice (a march splits the source interval in two, each
+// part taking at most the full list), and two more.
 int item_steps_bound(const HostModel& M, double rew) {
   long long sum = 0;
   int mx = 1;
@@ -1692,7 +1705,8 @@ int item_steps_bound(const HostModel& M, double rew) {
   return (int)std::min<long long>(sum + mx + 2, 1 << 30);
 }
 
-// the most steps of any item of event k (its largest frequency)
+// the most steps of any item of event k (its largest frequency): the
+// blocks' per-step storage of the adjoint sweeps
 int max_item_steps(const HostModel& M, const RunK& k) {
   return item_steps_bound(M, k.wmin + (double)(k.nw - 1) * k.dw);
 }
@@ -1968,7 +1982,9 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   if (!side)
     if (int rc = radial(st)) return rc;
   HIPCHK(hipEventRecord(ev[5], st));
-  HIPCHK(hipEventSynchronize(ev[5]));
+  // the source vectors' part (as ysx_source_model_vjp)
+  if (int rc = enqueue_source_sums(ctx, true, st, 6, 7)) return rc;
+  HIPCHK(hipEventSynchronize(ev[7]));
   for (int q = 0; q < 5; q++)
     if (int rc = elapsed(ctx, q, q + 1, ctx->ms_grad[q])) return rc;
   // the toroidal sweep and the radial adjoint on the second stream (their
@@ -2000,9 +2016,11 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   }
   std::vector<double> tb(ntab);
   HIPCHK(hipMemcpy(tb.data(), ctx->d_tbar, ntab * sizeof(double), hipMemcpyDeviceToHost));
-  // the source vectors' part (ysx_source_model_vjp)
-  if (int rc = ysx_source_model_vjp(ctx, table_bar)) return rc;
-  ctx->ms_grad[5] = ctx->ms_vjp[2];
+  std::vector<double> G;
+  if (int rc = collect_source_sums(ctx, G)) return rc;
+  std::fill(table_bar, table_bar + ntab, 0.0);
+  source_model_fold(ctx, G, table_bar);
+  if (int rc = elapsed(ctx, 6, 7, ctx->ms_grad[5])) return rc;
   ctx->ms_grad[6] = 0.0;
   for (int q = 0; q < 6; q++) ctx->ms_grad[6] += ctx->ms_grad[q];
   for (size_t q = 0; q < ntab; q++) table_bar[q] += tb[q];
