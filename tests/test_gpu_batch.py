@@ -103,6 +103,30 @@ def test_radial_lanes_knob(api, single):
     g.close()
 
 
+def test_tuner_applies_its_fastest_setting(api, single):
+    """yspec_amd.tune: times the given launch shapes and both lane counts per
+    radial solve on the loaded events, reports every candidate, and leaves
+    the fastest applied; the results under it agree with the single-event
+    ones to rounding."""
+    from yspec_amd.tune import tune
+
+    g = api.YSpec(synth_deck(0))
+    g.set_events(small_events())
+    shapes = [(0, 1, 4), (1, 2, 2)]
+    out = tune(g, reps=1, candidates=shapes)
+    best = out["best"]
+    assert [(c["persist"], c["chunk"], c["blocks_per_cu"]) for c in out["candidates"]] == shapes
+    assert sorted(r["rad_lanes"] for r in out["rad_lanes"]) == [1, 8]
+    assert all(c["ms"] > 0.0 for c in out["candidates"] + out["rad_lanes"])
+    times = [c["ms"] for c in out["candidates"]] + [r["ms"] for r in out["rad_lanes"]]
+    assert best["ms"] == min(times)
+    assert (best["persist"], best["chunk"], best["blocks_per_cu"]) in shapes
+    assert best["rad_lanes"] in (0, 1, 8)
+    g.forward_events(1)
+    _check_events(g, single, tol=1e-12)
+    g.close()
+
+
 def test_events_vs_oracle(api, oracle_mod, single):
     deck = synth_deck(0)
     for e, p in enumerate(small_events()):
