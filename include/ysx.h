@@ -319,12 +319,12 @@ int ysx_grad_timings(const ysx_ctx *ctx, double *ms7);
 int ysx_vjp_timings(const ysx_ctx *ctx, double *ms3);
 
 /* Timings of the last ysx_spectra / ysx_seismograms, milliseconds, from
- * HIP events on the context's streams: [0] source vectors and toroidal
- * start levels (second stream) + spheroidal start levels, [1] toroidal
- * march, [2] spheroidal minors, [3] b-vector, [4] radial (second stream),
- * [5] coefficient assembly, [6] Legendre sum, [7] IFFT + post.  An event
- * too small to fill the GPU runs its toroidal march on the second stream,
- * ahead of the radial solves. */
+ * HIP events on the context's streams: [0] source vectors + start levels,
+ * [1] toroidal, [2] spheroidal minors, [3] b-vector, [4] radial (second
+ * stream), [5] coefficient assembly, [6] Legendre sum, [7] IFFT + post.
+ * An event too small to fill the GPU runs its toroidal start levels and
+ * march on the second stream ahead of the radial solves; [1] then covers
+ * both and [0] holds the spheroidal start levels only. */
 int ysx_timings(const ysx_ctx *ctx, double *ms8);
 
 /* Exact Cash-Karp step counts of the last ysx_spectra, summed over pairs:

@@ -995,24 +995,21 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
     int nb = (int)((PR.n + 255) / 256);
     Queue qt = plain_q(ctx, W_TOR, e), qb = plain_q(ctx, W_BVEC, e), qr = plain_q(ctx, W_RAD, e),
           ql = plain_q(ctx, W_LSUM, e);
-    // The second stream takes what feeds only the assembly: the source
-    // vectors, the toroidal start levels and the radial (l = 0) solves. An
-    // event too small to fill the GPU with its spheroidal chain (fewer than
-    // four waves per SIMD) runs its toroidal march there too, ahead of the
-    // radial solves; a larger one runs it on the main stream after the
-    // minors (the spheroidal chain is the longer one), once its start levels
-    // are in (ev[12]).  The main stream starts with the spheroidal start
-    // levels.
+    if (timing) HIPCHK(hipEventRecord(ev[0], st));
+    launch_source(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR,
+                       ctx->d_rec, ctx->d_cnt);
+    HIPCHK(hipEventRecord(ev[1], st));
+    // the radial (l = 0) solves run beside the marches on a second stream.
+    // An event too small to fill the GPU with its spheroidal chain (fewer
+    // than four waves per SIMD) runs its toroidal chain (starts and march)
+    // there too, ahead of the radial solves: none of them feeds the others
+    // before the assembly, and the spheroidal chain is the longer one
     const bool side = qt.n < 16 * ctx->ncu;
     ctx->tor_side = side;
-    HIPCHK(hipEventRecord(ev[0], st));
-    HIPCHK(hipStreamWaitEvent(st2, ev[0], 0));
-    if (timing) HIPCHK(hipEventRecord(ev[9], st2));
-    launch_source(dim3(nb), b256, lds, st2, ctx->d_tab, ctx->d_runs, PR, ctx->d_rec, ctx->d_cnt);
-    if (timing) HIPCHK(hipEventRecord(ev[1], st2));
-    launch_start(dim3(nb), b256, lds, st2, ctx->d_tab, ctx->d_runs, PR, SD, ctx->d_starts, 1);
-    HIPCHK(hipEventRecord(ev[12], st2));
+    HIPCHK(hipStreamWaitEvent(st2, ev[1], 0));
     if (side) {
+      if (timing) HIPCHK(hipEventRecord(ev[9], st2));
+      launch_start(dim3(nb), b256, lds, st2, ctx->d_tab, ctx->d_runs, PR, SD, ctx->d_starts, 1);
       launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st2, ctx->d_tab, ctx->d_runs, qt,
                             SD, ctx->d_rec, ctx->d_cnt, ftp);
       if (timing) HIPCHK(hipEventRecord(ev[3], st2));
@@ -1027,8 +1024,14 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
                            ctx->d_rec, ctx->d_cnt);
     }
     HIPCHK(hipEventRecord(ev[11], st2));
-    launch_start(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, SD, ctx->d_starts, 2);
+    launch_start(dim3(nb), b256, lds, st, ctx->d_tab, ctx->d_runs, PR, SD, ctx->d_starts,
+                 side ? 2 : 3);
     if (timing) HIPCHK(hipEventRecord(ev[2], st));
+    if (!side) {
+      launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
+                            SD, ctx->d_rec, ctx->d_cnt, ftp);
+      if (timing) HIPCHK(hipEventRecord(ev[3], st));
+    }
     if (ctx->rk[e].ngc) {
       launch_ngc_march(ctx->rk[e].grav == 1, dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                        ctx->d_runs, qt, ctx->d_rec, ctx->d_cnt);
@@ -1039,15 +1042,6 @@ int enqueue_plain(ysx_ctx* ctx, int e0, int e1, bool timing) {
       launch_minor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab,
                          ctx->d_runs, qt, SD, S, ctx->d_cnt, 0, fsp);
       if (timing) HIPCHK(hipEventRecord(ev[4], st));
-    }
-    if (!side) {
-      HIPCHK(hipStreamWaitEvent(st, ev[12], 0));
-      if (timing) HIPCHK(hipEventRecord(ev[13], st));
-      launch_tor_march_item(dim3(grid_of(qt)), b64, 0, st, ctx->d_tab, ctx->d_runs, qt,
-                            SD, ctx->d_rec, ctx->d_cnt, ftp);
-      if (timing) HIPCHK(hipEventRecord(ev[3], st));
-    }
-    if (!ctx->rk[e].ngc) {
       if (ctx->rk[0].brow)
         launch_brow_march_item(dim3(grid_of(qb)), b64, 0, st, ctx->d_tab, ctx->d_runs, qb, S,
                                ShareK{}, ctx->d_rec, ctx->d_cnt);
@@ -1197,12 +1191,12 @@ int run_spectra(ysx_ctx* ctx) {
   if (rc) return rc;
   HIPCHK(hipEventSynchronize(ctx->ev[8]));
   double a, b;
-  if ((rc = elapsed(ctx, 9, 12, a)) || (rc = elapsed(ctx, 0, 2, b))) return rc;
-  ctx->ms[0] = a + b;  // source vectors and toroidal start levels (second
-                       // stream) + spheroidal start levels (main stream)
-  const bool ts = ctx->tor_side;
-  if ((rc = elapsed(ctx, ts ? 12 : 13, 3, ctx->ms[1])) || (rc = elapsed(ctx, 2, 4, ctx->ms[2])) ||
-      (rc = elapsed(ctx, ts ? 4 : 3, 5, ctx->ms[3])) || (rc = elapsed(ctx, 10, 11, ctx->ms[4])) ||
+  if ((rc = elapsed(ctx, 0, 1, a)) || (rc = elapsed(ctx, 1, 2, b))) return rc;
+  ctx->ms[0] = a + b;  // source vectors + start levels (the spheroidal ones
+                       // alone when the toroidal chain ran beside)
+  if ((rc = ctx->tor_side ? elapsed(ctx, 9, 3, ctx->ms[1]) : elapsed(ctx, 2, 3, ctx->ms[1])) ||
+      (rc = elapsed(ctx, ctx->tor_side ? 2 : 3, 4, ctx->ms[2])) ||
+      (rc = elapsed(ctx, 4, 5, ctx->ms[3])) || (rc = elapsed(ctx, 10, 11, ctx->ms[4])) ||
       (rc = elapsed(ctx, 6, 7, ctx->ms[5])) || (rc = elapsed(ctx, 7, 8, ctx->ms[6])))
     return rc;
   ctx->have_spec = true;
