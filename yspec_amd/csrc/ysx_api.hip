@@ -142,6 +142,11 @@ struct ysx_ctx {
   // fs_valid while the records of that forward are the context's
   cd *d_fys = nullptr, *d_tys = nullptr, *d_tyend = nullptr;
   int *d_fyss = nullptr, *d_tyss = nullptr;
+  // the minor sweep's row states and pairing units (RowStore) beside them
+  cd* d_fxs = nullptr;
+  int *d_fxss = nullptr, *d_pq = nullptr;
+  PairUnit* d_units = nullptr;
+  int units_cap = 0;
   long long* d_foff = nullptr;
   int *d_fcap = nullptr, *d_fsc = nullptr, *d_tsc = nullptr, *d_ferr = nullptr;
   size_t fs_steps = 0;
@@ -218,6 +223,8 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_obs); dfree(c->d_rpart);
   dfree(c->d_fys); dfree(c->d_fyss); dfree(c->d_foff); dfree(c->d_fcap); dfree(c->d_fsc);
   dfree(c->d_ferr); dfree(c->d_tys); dfree(c->d_tyss); dfree(c->d_tsc); dfree(c->d_tyend);
+  dfree(c->d_fxs); dfree(c->d_fxss); dfree(c->d_pq); dfree(c->d_units);
+  c->units_cap = 0;
   c->fs_steps = 0;
   c->fs_want = c->fs_valid = false;
   c->obs_n = 0;
@@ -1732,8 +1739,9 @@ bool prepare_step_store(ysx_ctx* ctx) {
     off[j] = steps;
     steps += (long long)nlb * cap[j];
   }
-  // the minor march's 14-component states and the toroidal march's 2
-  const size_t per_step = 16 * 64 * sizeof(cd) + 2 * 64 * sizeof(int);
+  // per step of an item: the minor march's 14-component states, the
+  // minor sweep's rows of 14 and the toroidal march's 2
+  const size_t per_step = 30 * 64 * sizeof(cd) + 3 * 64 * sizeof(int);
   size_t freeb = 0, totalb = 0;
   if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return false;
   const size_t have = ctx->fs_steps * per_step;
@@ -1744,19 +1752,42 @@ bool prepare_step_store(ysx_ctx* ctx) {
     dfree(ctx->d_fyss);
     dfree(ctx->d_tys);
     dfree(ctx->d_tyss);
+    dfree(ctx->d_fxs);
+    dfree(ctx->d_fxss);
     ctx->fs_steps = 0;
     if (hipMalloc(&ctx->d_fys, (size_t)steps * 14 * 64 * sizeof(cd)) != hipSuccess ||
         hipMalloc(&ctx->d_fyss, (size_t)steps * 64 * sizeof(int)) != hipSuccess ||
         hipMalloc(&ctx->d_tys, (size_t)steps * 2 * 64 * sizeof(cd)) != hipSuccess ||
-        hipMalloc(&ctx->d_tyss, (size_t)steps * 64 * sizeof(int)) != hipSuccess) {
+        hipMalloc(&ctx->d_tyss, (size_t)steps * 64 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&ctx->d_fxs, (size_t)steps * 14 * 64 * sizeof(cd)) != hipSuccess ||
+        hipMalloc(&ctx->d_fxss, (size_t)steps * 64 * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();
       dfree(ctx->d_fys);
       dfree(ctx->d_fyss);
       dfree(ctx->d_tys);
       dfree(ctx->d_tyss);
+      dfree(ctx->d_fxs);
+      dfree(ctx->d_fxss);
       return false;
     }
     ctx->fs_steps = (size_t)steps;
+  }
+  // one pairing unit per item and interval segment: at most two per knot
+  // interval (the source's is split)
+  const long long ucap = (long long)k.nw * nlb * (2 * (long long)M.nknot + 2);
+  if (ucap > ctx->units_cap) {
+    dfree(ctx->d_units);
+    ctx->units_cap = 0;
+    if (ucap > (1ll << 30) ||
+        hipMalloc(&ctx->d_units, (size_t)ucap * sizeof(PairUnit)) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    ctx->units_cap = (int)ucap;
+  }
+  if (!ctx->d_pq && hipMalloc(&ctx->d_pq, 2 * sizeof(int)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
   }
   if (!ctx->d_foff) {
     if (hipMalloc(&ctx->d_foff, k.nw * sizeof(long long)) != hipSuccess ||
@@ -1953,8 +1984,18 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     if (int rc = radial(st2)) return rc;
     HIPCHK(hipEventRecord(ev[15], st2));
   }
-  launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec, ctx->d_recb,
-                   ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
+  if (FS.ys) {
+    // the row chains, then every interval's pairings on all compute units
+    const RowStore RS{ctx->d_fxs, ctx->d_fxss, ctx->d_units, ctx->d_pq, ctx->d_pq + 1,
+                      ctx->units_cap};
+    launch_reset(dim3(1), b64, 0, st, ctx->d_pq, 2);
+    launch_minor_rows(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec,
+                      ctx->d_recb, ctx->d_mub, ctx->d_m0bar, AW, FS, RS, ctx->d_tpart);
+    launch_minor_pairs(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, AW, FS, RS, ctx->d_tpart);
+  } else {
+    launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec,
+                     ctx->d_recb, ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[2], st));
   if (side) {
@@ -2007,6 +2048,9 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   if (err & 2)
     return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: a radial segment with more steps than "
                                      "its tape");
+  if (err & 4)
+    return set_err(ctx, YSX_ERR_HIP, "ysx_model_vjp: more minor-sweep pairing units than their "
+                                     "queue holds");
   if (FS.ys) {
     int ferr = 0;
     HIPCHK(hipMemcpy(&ferr, ctx->d_ferr, sizeof ferr, hipMemcpyDeviceToHost));
@@ -2183,7 +2227,7 @@ extern "C" int ysx_grad_memory(const ysx_ctx* ctx, double* bytes3) {
   const double nb = ctx->adj_blocks, sm = ctx->adj_smax;
   const double P = (double)ctx->npair, nk = ctx->M.nknot;
   bytes3[0] = nb * sm * 64 * (14 * sizeof(cd) + sizeof(int)) +        // per-step states, and
-              (double)ctx->fs_steps * 64 * (16 * sizeof(cd) + 2 * sizeof(int));  // the forward's
+              (double)ctx->fs_steps * 64 * (30 * sizeof(cd) + 3 * sizeof(int));  // the forward's
   bytes3[1] = nb * 13 * 14 * 64 * sizeof(cd) +                         // step scratch,
               (double)ctx->rtape_bytes;                                // radial tape
   bytes3[2] = ctx->d_tbar ? P * (2 * 14 + 2 + REC) * sizeof(cd) + nb * F_COUNT * nk * 8.0

@@ -570,6 +570,127 @@ __device__ __forceinline__ int rev_interval(const Tab& T, const WS& w, const LS&
   return s0;
 }
 
+// ------------------------------------ one interval, row chain alone
+// rev_interval's sequential part for the two-kernel minor sweep: the row x
+// (counter sx) taken back through the interval's steps, its state before
+// every step stored in X at the step's index; the interval's pairings are
+// left to k_minor_pairs as one PairUnit (event e, frequency j, degree
+// block lb, interval i, part: 1 the march below the source, 0 the
+// continuation to the receiver).  Returns s_end - ns.
+template <class Sys>
+__device__ __forceinline__ int rows_interval(const Tab& T, const WS& w, const LS& L, int nk, int i,
+                                             double ra, double rb, cd* x, int& sx,
+                                             const AdjWork& X, int s_end, AdjLds& S,
+                                             const RowStore& RS, int e, int j, int lb, int part) {
+  using F = typename Sys::F;
+  constexpr int N = F::N, BS = F::BS;
+  double drr;
+  const int ns = steps_of(T, w, i, nk, ra, rb, drr);
+  const int s0 = s_end - ns;
+  if (s_end > X.smax) return s0;  // not stored (flagged by the forward)
+  const int jlast = 1 + ((ns - 1) / CHA) * CHA;
+  for (int j0 = jlast; j0 >= 1; j0 -= CHA) {
+    const int nst = min(CHA, ns + 1 - j0);
+    load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+    __syncthreads();
+    for (int jj = nst - 1; jj >= 0; jj--) {
+      put_step<N>(X, s0 + j0 - 1 + jj, x, sx);
+      if constexpr (N == 14) {
+        rk_step_T_lds(S.sb + jj * 6 * BS, L, w, x, typename Sys::FT(), S.slots + threadIdx.x);
+      } else {
+        rk_step_T_reg(S.sb + jj * 6 * BS, L, w, x, typename Sys::FT(), NoHook());
+      }
+      renorm<N>(x, sx);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int k = atomicAdd(RS.count, 1);
+    if (k < RS.cap)
+      RS.units[k] = PairUnit{e, j, lb, i | (part << 16), s0, ns, 0, 0};
+    else
+      atomicOr(X.err, 4);
+  }
+  return s0;
+}
+
+// ------------------------------- one interval segment, pairings alone
+// The other part of rev_interval for the two-kernel minor sweep: steps s0
+// .. s0 + ns - 1 of an item, whose column states (W) and row states (X)
+// are stored; per step the column's step and the row's transposed step are
+// replayed from them and the stages paired, summed over the wave and taken
+// to the interval's table entries.  The steps are independent.
+template <class Sys>
+__device__ __forceinline__ void pairs_interval(const Tab& T, const WS& w, const LS& L, int nk, int i,
+                                               double ra, double rb, const AdjWork& W,
+                                               const AdjWork& X, int s0, AdjLds& S,
+                                               double* tbar) {
+  using F = typename Sys::F;
+  constexpr int N = F::N, BS = F::BS, NOB = Sys::NOB;
+  double drr;
+  const int ns = steps_of(T, w, i, nk, ra, rb, drr);
+  if (s0 + ns > W.smax) return;
+  const Scr scr = scr_of(W);
+  for (int j0 = 1; j0 <= ns; j0 += CHA) {
+    const int nst = min(CHA, ns + 1 - j0);
+    load_bases<F>(T, w, i, ra, drr, j0, nst, S.sb);
+    __syncthreads();
+    for (int jj = 0; jj < nst; jj++) {
+      const int s = s0 + j0 - 1 + jj;
+      const double* bs = S.sb + jj * 6 * BS;
+      int ysc, xsc;
+      if constexpr (N == 14) {
+        {
+          cd y[N];
+          load_step<N>(W, s, y, ysc);
+          rk_step_lds(bs, L, w, y, F(), S.slots + threadIdx.x, ScrWrite<N>{scr, 0});
+        }
+        {
+          cd xr[N];
+          load_step<N>(X, s, xr, xsc);
+          rk_step_T_lds(bs, L, w, xr, typename Sys::FT(), S.slots + threadIdx.x,
+                        ScrWrite<N>{scr, SCR_ROWS});
+        }
+        pair_step<Sys>(scr, L, bs, scale_pow(-(xsc + ysc)), S, S.bbar + jj * 6 * NOB);
+      } else {
+        cd zst[6][N], kst[6][N];
+        {
+          cd y[N];
+          load_step<N>(W, s, y, ysc);
+          rk_step_hook(bs, L, w, y, F(), [&](int k, const cd* z) {
+#pragma unroll
+            for (int q = 0; q < N; q++) zst[k][q] = z[q];
+          });
+        }
+        {
+          cd xr[N];
+          load_step<N>(X, s, xr, xsc);
+          rk_step_T_reg(bs, L, w, xr, typename Sys::FT(), [&](int k, const cd* kb) {
+#pragma unroll
+            for (int q = 0; q < N; q++) kst[k][q] = kb[q];
+          });
+        }
+        const double fs = scale_pow(-(xsc + ysc));
+        const StageSums<NOB> sums{reinterpret_cast<double*>(S.slots), S.bbar + jj * 6 * NOB};
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          typename Sys::Abar B;
+          zero(B);
+          Sys::bil_s(kst[k], [&](int q) { return zst[k][q]; }, B);
+          double ob[NOB];
+          Sys::ob(B, L, bs + k * BS, ob);
+#pragma unroll
+          for (int v = 0; v < NOB; v++) ob[v] *= fs;
+          sums.put(k, ob);
+        }
+      }
+    }
+    chain_chunk<Sys>(T, w, i, ra, drr, j0, nst, S);
+    __syncthreads();
+  }
+  flush_interval<Sys>(i, nk, S, tbar);
+}
+
 // ------------------------------------------- one interval, column sweep
 // Interval i of a march whose column goes ra -> rb and whose row came
 // rb -> ra by transposed steps, the row's states before each of its steps
@@ -752,13 +873,16 @@ __device__ __forceinline__ bool fluid_at(const RunK& R, int i) {
   return R.fic ? (i < R.noc) : (i > R.nic && i < R.noc);
 }
 
-__device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int j, int lb,
+// ROWS (with the forward's stored states only): the row chain alone
+// (rows_interval), the pairings left to k_minor_pairs through RS
+template <bool ROWS = false>
+__device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int e, int j, int lb,
                                                const StartData& SD, const SphState& SS,
                                                const cd* __restrict__ rec,
                                                const cd* __restrict__ recb,
                                                const cd* __restrict__ mub, cd* __restrict__ m0bar,
                                                AdjWork W, const StepStore& FS, AdjLds& S,
-                                               double* tbar) {
+                                               double* tbar, const RowStore& RS = RowStore{}) {
   const int l = R.lmin + lb * 64 + (int)threadIdx.x;
   const bool valid = (l >= 1 && l <= R.lmax);
   const size_t p = (size_t)R.pbase + (size_t)(l - R.lmin) * R.nw + (j - 1);
@@ -771,7 +895,7 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
   const double rs = R.rs, rr = R.rr;
   const double rt1 = fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2));
   int sc1 = 0, sc3 = 0, s = 0, sci1 = 0;
-  if (FS.ys) {
+  if (ROWS || FS.ys) {
     // 1. the forward stored every step's state (k_minor_march, STORE) and
     // the counters: the item's steps are counted, not marched again
     W.ys = FS.ys;
@@ -792,7 +916,7 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
       sc3 = FS.sc[3 * p + 1];
       sci1 = FS.sc[3 * p + 2];
     }
-  } else {
+  } else if constexpr (!ROWS) {
     // 1. the forward's march (minor_item), every step's state stored, the
     // counter at the lane's start kept for the start adjoint
     cd m[14];
@@ -849,10 +973,20 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
     for (int q = 0; q < 14; q++) x[q] = cmul_(g, x[q]);
   }
   int sx = -sc3;
+  // the rows' storage (ROWS): the item's steps as the columns'
+  AdjWork X = W;
+  if constexpr (ROWS) {
+    X.ys = RS.xs;
+    X.yss = RS.xss;
+  }
   // 2. back from the receiver to the source
   for (int i = ir; i >= is; i--) {
     const double a = (i == is) ? rs : T(F_R, i), b = (i == ir) ? rr : T(F_R, i + 1);
-    if (a != b) s = rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
+    if (a == b) continue;
+    if constexpr (ROWS)
+      s = rows_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, X, s, S, RS, e, j, lb, 0);
+    else
+      s = rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
   }
   // at the source: the row on ms1, plus the b-vector starts' part
   {
@@ -866,10 +1000,17 @@ __device__ __forceinline__ void minor_adj_item(const Tab& T, const RunK& R, int 
     const bool fl = fluid_at(R, i);
     const double a = (i == 1) ? rt1 : T(F_R, i), b = (i == is) ? rs : T(F_R, i + 1);
     if (a != b) {
-      if (fl)
-        s = rev_interval<SysFluidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
-      else
-        s = rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
+      if constexpr (ROWS) {
+        if (fl)
+          s = rows_interval<SysFluidMinor>(T, w, L, nk, i, a, b, x, sx, X, s, S, RS, e, j, lb, 1);
+        else
+          s = rows_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, X, s, S, RS, e, j, lb, 1);
+      } else {
+        if (fl)
+          s = rev_interval<SysFluidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
+        else
+          s = rev_interval<SysSolidMinor>(T, w, L, nk, i, a, b, x, sx, W, s, S, tbar);
+      }
     }
     if (i == i1) {
       const double h = scale_pow(-(sx + sci1));
@@ -907,8 +1048,77 @@ __global__ void __launch_bounds__(64) k_minor_adj(const double* __restrict__ gta
   __shared__ AdjLds S;
   const AdjWork WB = block_work(W);
   RUN_ITEMS(PERSIST, Q,
-            minor_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.y, it.z, SD, SS, rec, recb,
-                           mub, m0bar, WB, FS, S, tbar));
+            minor_adj_item(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.x, it.y, it.z, SD, SS, rec,
+                           recb, mub, m0bar, WB, FS, S, tbar));
+}
+
+// The minor sweep's row chains with the forward's stored states (FS): per
+// item as k_minor_adj, the pairings of each interval appended to RS
+__global__ void __launch_bounds__(64) k_minor_rows(const double* __restrict__ gtab,
+                                                   const RunK* __restrict__ runs, Queue Q,
+                                                   StartData SD, SphState SS,
+                                                   const cd* __restrict__ rec,
+                                                   const cd* __restrict__ recb,
+                                                   const cd* __restrict__ mub,
+                                                   cd* __restrict__ m0bar, AdjWork W,
+                                                   StepStore FS, RowStore RS,
+                                                   double* __restrict__ tbar) {
+  __shared__ AdjLds S;
+  const AdjWork WB = block_work(W);
+  RUN_ITEMS(true, Q,
+            minor_adj_item<true>(Tab{gtab, runs[it.x].nknot}, runs[it.x], it.x, it.y, it.z, SD,
+                                 SS, rec, recb, mub, m0bar, WB, FS, S, tbar, RS));
+}
+
+// The pairings of every unit k_minor_rows appended (persistent blocks take
+// them one at a time from RS.next until RS.count), each into the block's
+// own partial table column
+__global__ void __launch_bounds__(64) k_minor_pairs(const double* __restrict__ gtab,
+                                                    const RunK* __restrict__ runs, AdjWork W,
+                                                    StepStore FS, RowStore RS,
+                                                    double* __restrict__ tbar) {
+  __shared__ AdjLds S;
+  if (threadIdx.x < NV_MAX * 4) S.tacc[threadIdx.x] = 0.0;
+  __syncthreads();
+  const int n = min(*RS.count, RS.cap);
+  while (true) {
+    int k = 0;
+    if (threadIdx.x == 0) k = atomicAdd(RS.next, 1);
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, 64));
+    if (k >= n) break;
+    const PairUnit u = RS.units[k];
+    const RunK& R = runs[u.e];
+    const Tab T{gtab, R.nknot};
+    const int j = u.j, lb = u.lb, i = u.ik & 0xffff, part = u.ik >> 16;
+    const int l = R.lmin + lb * 64 + (int)threadIdx.x;
+    const bool valid = (l >= 1 && l <= R.lmax);
+    const WS w = wave_omega(R, j);
+    const LS L = l_state(valid ? l : 1);
+    const int nk = R.nknot, is = R.is, ir = R.ir;
+    AdjWork CW = W;  // the item's column states, and its row states
+    CW.ys = FS.ys;
+    CW.yss = FS.yss;
+    CW.smax = FS.cap[j - 1];
+    CW.base = FS.off[j - 1] + (long long)lb * CW.smax;
+    AdjWork XW = CW;
+    XW.ys = RS.xs;
+    XW.yss = RS.xss;
+    double a, b;
+    bool fl = false;
+    if (part == 1) {
+      const double rt1 = fmin(0.1 * (1.0 / (4.0 * w.rew / R.alpha_min)), 0.1 * T(F_R, 2));
+      a = (i == 1) ? rt1 : T(F_R, i);
+      b = (i == is) ? R.rs : T(F_R, i + 1);
+      fl = fluid_at(R, i);
+    } else {
+      a = (i == is) ? R.rs : T(F_R, i);
+      b = (i == ir) ? R.rr : T(F_R, i + 1);
+    }
+    if (fl)
+      pairs_interval<SysFluidMinor>(T, w, L, nk, i, a, b, CW, XW, u.s0, S, tbar);
+    else
+      pairs_interval<SysSolidMinor>(T, w, L, nk, i, a, b, CW, XW, u.s0, S, tbar);
+  }
 }
 
 // ----------------------------------------------------------- k_tor_adj
@@ -1318,6 +1528,17 @@ void launch_brow_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gt
                      const RunK* runs, Queue Q, SphState SS, const cd* rec, const cd* recb,
                      cd* mub, AdjWork W, double* tbar) {
   hipLaunchKernelGGL(k_brow_adj<true>, g, b, lds, s, gtab, runs, Q, SS, rec, recb, mub, W, tbar);
+}
+void launch_minor_rows(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                       const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,
+                       const cd* recb, const cd* mub, cd* m0bar, AdjWork W, StepStore FS,
+                       RowStore RS, double* tbar) {
+  hipLaunchKernelGGL(k_minor_rows, g, b, lds, s, gtab, runs, Q, SD, SS, rec, recb, mub, m0bar,
+                     W, FS, RS, tbar);
+}
+void launch_minor_pairs(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                        const RunK* runs, AdjWork W, StepStore FS, RowStore RS, double* tbar) {
+  hipLaunchKernelGGL(k_minor_pairs, g, b, lds, s, gtab, runs, W, FS, RS, tbar);
 }
 void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                       const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,

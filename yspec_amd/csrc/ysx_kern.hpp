@@ -212,6 +212,26 @@ struct StepStore {
   cd* yend;
 };
 
+// The minor sweep in two kernels when the forward stored the minor states:
+// k_minor_rows carries each item's row through its steps (the sequential
+// part), storing the row's state before every step in xs / xss (as the
+// StepStore layout, same offsets) and appending one PairUnit per interval
+// segment; k_minor_pairs then takes the units in any order (the pairings of
+// every step are independent once both states are stored).  count: units
+// appended; next: units taken; cap: room in units.
+struct PairUnit {
+  int e, j, lb, ik;  // event, frequency, degree block, interval | part << 16
+  int s0, ns, pad0, pad1;  // first step of the segment in the item's storage, steps
+};
+struct RowStore {
+  cd* xs;
+  int* xss;
+  PairUnit* units;
+  int* count;
+  int* next;
+  int cap;
+};
+
 // ------------------------------------------------------- march helpers
 // yint's interval loop (src/module_int.f90:1888-1963) over knot intervals,
 // with wave-uniform arguments
@@ -351,6 +371,14 @@ void launch_part_sum(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* tp
 void launch_brow_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
                      const RunK* runs, Queue Q, SphState SS, const cd* rec, const cd* recb,
                      cd* mub, AdjWork W, double* tbar);
+// the two kernels of the minor sweep when the forward stored the states
+// (FS.ys != nullptr): the rows, then the pairings (grid: persistent blocks)
+void launch_minor_rows(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                       const RunK* runs, Queue Q, StartData SD, SphState SS, const cd* rec,
+                       const cd* recb, const cd* mub, cd* m0bar, AdjWork W, StepStore FS,
+                       RowStore RS, double* tbar);
+void launch_minor_pairs(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
+                        const RunK* runs, AdjWork W, StepStore FS, RowStore RS, double* tbar);
 // FS.ys != nullptr: the minor march's per-step states from the forward
 // (k_minor_march with STORE) replace the sweep's first pass
 void launch_minor_adj(dim3 g, dim3 b, size_t lds, hipStream_t s, const double* gtab,
