@@ -298,8 +298,10 @@ int ysx_misfit_grad(ysx_ctx *ctx, const double *obs, int nout, double *misfit,
  * ysx_model_vjp; no counterpart in the reference): [0] the sweeps'
  * per-step states (the stored side's state before every step of the
  * longest item, per persistent block) and those that a single-event
- * ysx_misfit_grad's forward stored, [1] the per-block step scratch and the
- * radial tape — the per-block parts do not grow with the number of pairs —
+ * ysx_misfit_grad's forward stored, [1] the per-block step scratch (twice
+ * with those states: the minor sweep's two item groups run side by side)
+ * and the radial tape — the per-block parts do not grow with the number of
+ * pairs —
  * and [2] the per-pair adjoints (record adjoints, minors at the source and
  * start) and the per-block partial table adjoints. */
 int ysx_grad_memory(const ysx_ctx *ctx, double *bytes3);

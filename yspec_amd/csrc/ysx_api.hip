@@ -66,7 +66,7 @@ struct WorkList {
 
 struct ysx_ctx {
   int device = 0, ncu = 256;
-  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr, stream3 = nullptr;
   std::string err;
   Norms N;
   HostModel M;
@@ -124,15 +124,15 @@ struct ysx_ctx {
   bool tor_side = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  hipEvent_t ev[16] = {};
+  hipEvent_t ev[18] = {};
   double ms[8] = {0};
   double* scratch = nullptr;  // this device's copy of `scratch` (ysx_model_table_noalloc)
   // model gradient (ysx_model_vjp): per-pair adjoints of the minors at the
   // source from the b-vector starts (mub), of the starting solutions (m0bar,
   // y0bar), the table adjoint, the sweeps' per-block checkpoints and tapes
   cd *d_mub = nullptr, *d_m0bar = nullptr, *d_y0bar = nullptr;
-  double *d_tbar = nullptr, *d_tpart = nullptr, *d_tpart2 = nullptr;
-  cd *d_ays = nullptr, *d_ascr = nullptr;
+  double *d_tbar = nullptr, *d_tpart = nullptr, *d_tpart2 = nullptr, *d_tpart3 = nullptr;
+  cd *d_ays = nullptr, *d_ascr = nullptr, *d_ascr2 = nullptr;
   int *d_ayss = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
   int adj_blocks = 0, adj_smax = 0;
   // the minor and toroidal marches' per-step states stored by the
@@ -216,8 +216,8 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
   dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
   dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_tpart);
-  dfree(c->d_tpart2);
-  dfree(c->d_ays); dfree(c->d_ayss); dfree(c->d_ascr); dfree(c->d_aerr);
+  dfree(c->d_tpart2); dfree(c->d_tpart3);
+  dfree(c->d_ays); dfree(c->d_ayss); dfree(c->d_ascr); dfree(c->d_ascr2); dfree(c->d_aerr);
   dfree(c->d_aq);
   dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
   dfree(c->d_obs); dfree(c->d_rpart);
@@ -553,6 +553,7 @@ extern "C" int ysx_create(ysx_ctx** out, int device) {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -578,6 +579,7 @@ extern "C" void ysx_destroy(ysx_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+  if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
   delete ctx;
 }
 
@@ -1785,7 +1787,7 @@ bool prepare_step_store(ysx_ctx* ctx) {
     }
     ctx->units_cap = (int)ucap;
   }
-  if (!ctx->d_pq && hipMalloc(&ctx->d_pq, 2 * sizeof(int)) != hipSuccess) {
+  if (!ctx->d_pq && hipMalloc(&ctx->d_pq, 4 * sizeof(int)) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
@@ -1861,6 +1863,13 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     HIPCHK(hipMalloc(&ctx->d_aerr, 16 * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->d_aq, 4 * sizeof(int)));
   }
+  // the step scratch and partial table columns of the minor sweep's second
+  // group of items (with the forward's states: its own stream)
+  if (ctx->fs_valid && !ctx->d_ascr2) {
+    const size_t nb = ctx->adj_blocks;
+    HIPCHK(hipMalloc(&ctx->d_ascr2, nb * 13 * 14 * 64 * sizeof(cd)));
+    HIPCHK(hipMalloc(&ctx->d_tpart3, (size_t)F_COUNT * nk * nb * sizeof(double)));
+  }
   hipStream_t st = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   const dim3 b64(64), b256(256);
@@ -1869,6 +1878,8 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
                ctx->d_tbar, ntab);
   launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart, ntab * ctx->adj_blocks);
   launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart2, ntab * ctx->adj_blocks);
+  if (ctx->d_tpart3)
+    launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart3, ntab * ctx->adj_blocks);
   launch_reset(dim3(1), b64, 0, st, ctx->d_aerr, 16);
   launch_reset(dim3(1), b64, 0, st, ctx->d_aq, 4);
   HIPCHK(hipGetLastError());
@@ -1984,14 +1995,46 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     if (int rc = radial(st2)) return rc;
     HIPCHK(hipEventRecord(ev[15], st2));
   }
-  if (FS.ys) {
-    // the row chains, then every interval's pairings on all compute units
-    const RowStore RS{ctx->d_fxs, ctx->d_fxss, ctx->d_units, ctx->d_pq, ctx->d_pq + 1,
-                      ctx->units_cap};
-    launch_reset(dim3(1), b64, 0, st, ctx->d_pq, 2);
-    launch_minor_rows(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec,
-                      ctx->d_recb, ctx->d_mub, ctx->d_m0bar, AW, FS, RS, ctx->d_tpart);
-    launch_minor_pairs(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, AW, FS, RS, ctx->d_tpart);
+  const bool split = FS.ys != nullptr;
+  if (split) {
+    // the row chains, then every interval's pairings on all compute units.
+    // The items are cost-sorted: the first half (the longer chains) run
+    // here, the others' chains and then their pairings on the third stream
+    // in the compute units the long chains leave idle, with their own step
+    // scratch, units and partial columns; the long chains' pairings follow
+    // them here.  (A chain is one wave's sequential march, so the chains
+    // alone keep few compute units busy; half and half measured best of
+    // 1/16 .. 1/2 on the lmax 300, 20 mHz gradient.)
+    const int per = 2 * nk + 2;  // pairing units of an item, at most
+    const int nl = (qm.n + 1) / 2;
+    Queue qL = qm, qS = qm;
+    qL.n = nl;
+    qS.items = qm.items + nl;
+    qS.n = qm.n - nl;
+    qS.counter = ctx->d_aq + 3;
+    const RowStore RL{ctx->d_fxs, ctx->d_fxss, ctx->d_units, ctx->d_pq, ctx->d_pq + 1,
+                      std::min(ctx->units_cap, nl * per)};
+    const RowStore RR{ctx->d_fxs, ctx->d_fxss, ctx->d_units + RL.cap, ctx->d_pq + 2,
+                      ctx->d_pq + 3, ctx->units_cap - RL.cap};
+    AdjWork AW2 = AW;
+    AW2.scr = ctx->d_ascr2;
+    hipStream_t st3 = ctx->stream3;
+    launch_reset(dim3(1), b64, 0, st, ctx->d_pq, 4);
+    HIPCHK(hipEventRecord(ev[16], st));
+    HIPCHK(hipStreamWaitEvent(st3, ev[16], 0));
+    if (nl > 0)
+      launch_minor_rows(dim3((unsigned)std::min(nl, ctx->adj_blocks)), b64, 0, st, ctx->d_tab,
+                        ctx->d_runs, qL, SD, S, ctx->d_rec, ctx->d_recb, ctx->d_mub,
+                        ctx->d_m0bar, AW, FS, RL, ctx->d_tpart);
+    if (qS.n > 0) {
+      launch_minor_rows(gb, b64, 0, st3, ctx->d_tab, ctx->d_runs, qS, SD, S, ctx->d_rec,
+                        ctx->d_recb, ctx->d_mub, ctx->d_m0bar, AW2, FS, RR, ctx->d_tpart3);
+      launch_minor_pairs(gb, b64, 0, st3, ctx->d_tab, ctx->d_runs, AW2, FS, RR, ctx->d_tpart3);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ev[17], st3));
+    launch_minor_pairs(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, AW, FS, RL, ctx->d_tpart);
+    HIPCHK(hipStreamWaitEvent(st, ev[17], 0));
   } else {
     launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec,
                      ctx->d_recb, ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
@@ -2009,6 +2052,9 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
                   ntab, ctx->d_tbar);
   if (side)
     launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart2,
+                    ctx->adj_blocks, ntab, ctx->d_tbar);
+  if (split)
+    launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart3,
                     ctx->adj_blocks, ntab, ctx->d_tbar);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[3], st));
@@ -2228,10 +2274,12 @@ extern "C" int ysx_grad_memory(const ysx_ctx* ctx, double* bytes3) {
   const double P = (double)ctx->npair, nk = ctx->M.nknot;
   bytes3[0] = nb * sm * 64 * (14 * sizeof(cd) + sizeof(int)) +        // per-step states, and
               (double)ctx->fs_steps * 64 * (30 * sizeof(cd) + 3 * sizeof(int));  // the forward's
-  bytes3[1] = nb * 13 * 14 * 64 * sizeof(cd) +                         // step scratch,
+  const double groups = ctx->d_ascr2 ? 2.0 : 1.0;  // the minor sweep's second item group
+  bytes3[1] = groups * nb * 13 * 14 * 64 * sizeof(cd) +                // step scratch,
               (double)ctx->rtape_bytes;                                // radial tape
-  bytes3[2] = ctx->d_tbar ? P * (2 * 14 + 2 + REC) * sizeof(cd) + nb * F_COUNT * nk * 8.0
-                          : 0.0;                                       // per-pair adjoints
+  bytes3[2] = ctx->d_tbar ? P * (2 * 14 + 2 + REC) * sizeof(cd) +      // per-pair adjoints,
+                                (1.0 + groups) * nb * F_COUNT * nk * 8.0  // partial columns
+                          : 0.0;
   return YSX_OK;
 }
 
