@@ -298,8 +298,9 @@ int ysx_misfit_grad(ysx_ctx *ctx, const double *obs, int nout, double *misfit,
  * ysx_model_vjp; no counterpart in the reference): [0] the sweeps'
  * per-step states (the stored side's state before every step of the
  * longest item, per persistent block) and those that a single-event
- * ysx_misfit_grad's forward stored, [1] the per-block step scratch (twice
- * with those states: the minor sweep's two item groups run side by side)
+ * ysx_misfit_grad's forward stored, [1] the per-block step scratch (three
+ * times with those states: the minor sweep's three item groups run side by
+ * side)
  * and the radial tape — the per-block parts do not grow with the number of
  * pairs —
  * and [2] the per-pair adjoints (record adjoints, minors at the source and

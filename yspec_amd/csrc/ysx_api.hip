@@ -66,7 +66,8 @@ struct WorkList {
 
 struct ysx_ctx {
   int device = 0, ncu = 256;
-  hipStream_t stream = nullptr, stream2 = nullptr, stream3 = nullptr;
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipStream_t gstream[2] = {};  // the minor sweep's further item groups
   std::string err;
   Norms N;
   HostModel M;
@@ -124,15 +125,17 @@ struct ysx_ctx {
   bool tor_side = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  hipEvent_t ev[18] = {};
+  hipEvent_t ev[20] = {};
   double ms[8] = {0};
   double* scratch = nullptr;  // this device's copy of `scratch` (ysx_model_table_noalloc)
   // model gradient (ysx_model_vjp): per-pair adjoints of the minors at the
   // source from the b-vector starts (mub), of the starting solutions (m0bar,
   // y0bar), the table adjoint, the sweeps' per-block checkpoints and tapes
   cd *d_mub = nullptr, *d_m0bar = nullptr, *d_y0bar = nullptr;
-  double *d_tbar = nullptr, *d_tpart = nullptr, *d_tpart2 = nullptr, *d_tpart3 = nullptr;
-  cd *d_ays = nullptr, *d_ascr = nullptr, *d_ascr2 = nullptr;
+  double *d_tbar = nullptr, *d_tpart = nullptr, *d_tpart2 = nullptr;
+  double* d_gtpart[2] = {};
+  cd *d_ays = nullptr, *d_ascr = nullptr;
+  cd* d_gscr[2] = {};
   int *d_ayss = nullptr, *d_aerr = nullptr, *d_aq = nullptr;
   int adj_blocks = 0, adj_smax = 0;
   // the minor and toroidal marches' per-step states stored by the
@@ -216,8 +219,12 @@ void free_run(ysx_ctx* c) {
   dfree(c->d_seis); dfree(c->d_queue); dfree(c->d_shorder); dfree(c->d_shx); dfree(c->d_shs);
   dfree(c->d_seisb); dfree(c->d_specb); dfree(c->d_cfb); dfree(c->d_recb); dfree(c->d_mmpart);
   dfree(c->d_mub); dfree(c->d_m0bar); dfree(c->d_y0bar); dfree(c->d_tbar); dfree(c->d_tpart);
-  dfree(c->d_tpart2); dfree(c->d_tpart3);
-  dfree(c->d_ays); dfree(c->d_ayss); dfree(c->d_ascr); dfree(c->d_ascr2); dfree(c->d_aerr);
+  dfree(c->d_tpart2);
+  for (int g = 0; g < 2; g++) {
+    dfree(c->d_gtpart[g]);
+    dfree(c->d_gscr[g]);
+  }
+  dfree(c->d_ays); dfree(c->d_ayss); dfree(c->d_ascr); dfree(c->d_aerr);
   dfree(c->d_aq);
   dfree(c->d_rtape); dfree(c->d_rprop); dfree(c->d_rys); dfree(c->d_rxe); dfree(c->d_rcap);
   dfree(c->d_obs); dfree(c->d_rpart);
@@ -553,7 +560,7 @@ extern "C" int ysx_create(ysx_ctx** out, int device) {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking));
+  for (auto& g : ctx->gstream) HIPCHK(hipStreamCreateWithFlags(&g, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIPCHK(hipEventCreate(&e));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -579,7 +586,8 @@ extern "C" void ysx_destroy(ysx_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
-  if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
+  for (auto& g : ctx->gstream)
+    if (g) (void)hipStreamDestroy(g);
   delete ctx;
 }
 
@@ -1787,7 +1795,7 @@ bool prepare_step_store(ysx_ctx* ctx) {
     }
     ctx->units_cap = (int)ucap;
   }
-  if (!ctx->d_pq && hipMalloc(&ctx->d_pq, 4 * sizeof(int)) != hipSuccess) {
+  if (!ctx->d_pq && hipMalloc(&ctx->d_pq, 8 * sizeof(int)) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
@@ -1865,11 +1873,18 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   }
   // the step scratch and partial table columns of the minor sweep's second
   // group of items (with the forward's states: its own stream)
-  if (ctx->fs_valid && !ctx->d_ascr2) {
-    const size_t nb = ctx->adj_blocks;
-    HIPCHK(hipMalloc(&ctx->d_ascr2, nb * 13 * 14 * 64 * sizeof(cd)));
-    HIPCHK(hipMalloc(&ctx->d_tpart3, (size_t)F_COUNT * nk * nb * sizeof(double)));
-  }
+  // the minor sweep's item groups with the forward's states: the first
+  // eighth of the cost-sorted items, the rest of the first half, the second
+  // half (the ends of the first two)
+  constexpr int ngroups = 3;
+  const double gcut[ngroups - 1] = {0.125, 0.5};
+  if (ctx->fs_valid)
+    for (int g = 0; g + 1 < ngroups; g++)
+      if (!ctx->d_gscr[g]) {
+        const size_t nb = ctx->adj_blocks;
+        HIPCHK(hipMalloc(&ctx->d_gscr[g], nb * 13 * 14 * 64 * sizeof(cd)));
+        HIPCHK(hipMalloc(&ctx->d_gtpart[g], (size_t)F_COUNT * nk * nb * sizeof(double)));
+      }
   hipStream_t st = ctx->stream;
   hipEvent_t* ev = ctx->ev;
   const dim3 b64(64), b256(256);
@@ -1878,8 +1893,8 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
                ctx->d_tbar, ntab);
   launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart, ntab * ctx->adj_blocks);
   launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart2, ntab * ctx->adj_blocks);
-  if (ctx->d_tpart3)
-    launch_fill0(dim3(2048), b256, 0, st, ctx->d_tpart3, ntab * ctx->adj_blocks);
+  for (double* t : ctx->d_gtpart)
+    if (t) launch_fill0(dim3(2048), b256, 0, st, t, ntab * ctx->adj_blocks);
   launch_reset(dim3(1), b64, 0, st, ctx->d_aerr, 16);
   launch_reset(dim3(1), b64, 0, st, ctx->d_aq, 4);
   HIPCHK(hipGetLastError());
@@ -1998,43 +2013,49 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
   const bool split = FS.ys != nullptr;
   if (split) {
     // the row chains, then every interval's pairings on all compute units.
-    // The items are cost-sorted: the first half (the longer chains) run
-    // here, the others' chains and then their pairings on the third stream
-    // in the compute units the long chains leave idle, with their own step
-    // scratch, units and partial columns; the long chains' pairings follow
-    // them here.  (A chain is one wave's sequential march, so the chains
-    // alone keep few compute units busy; half and half measured best of
-    // 1/16 .. 1/2 on the lmax 300, 20 mHz gradient.)
+    // A chain is one wave's sequential march, so the chains alone keep few
+    // compute units busy, and an item's pairings wait for its chain.  The
+    // items are cost-sorted and cut into groups, each on its own stream with
+    // its own step scratch, units, counters and partial columns: the longest
+    // chains here, the others beside them, each group's pairings as soon as
+    // its own chains end, in the compute units the longer chains leave
+    // idle.  (On the lmax 300, 20 mHz gradient: one group 10.1 ms, two
+    // 9.4 ms, three 8.9 - 9.1 ms over several cuts.)
     const int per = 2 * nk + 2;  // pairing units of an item, at most
-    const int nl = (qm.n + 1) / 2;
-    Queue qL = qm, qS = qm;
-    qL.n = nl;
-    qS.items = qm.items + nl;
-    qS.n = qm.n - nl;
-    qS.counter = ctx->d_aq + 3;
-    const RowStore RL{ctx->d_fxs, ctx->d_fxss, ctx->d_units, ctx->d_pq, ctx->d_pq + 1,
-                      std::min(ctx->units_cap, nl * per)};
-    const RowStore RR{ctx->d_fxs, ctx->d_fxss, ctx->d_units + RL.cap, ctx->d_pq + 2,
-                      ctx->d_pq + 3, ctx->units_cap - RL.cap};
-    AdjWork AW2 = AW;
-    AW2.scr = ctx->d_ascr2;
-    hipStream_t st3 = ctx->stream3;
-    launch_reset(dim3(1), b64, 0, st, ctx->d_pq, 4);
+    launch_reset(dim3(1), b64, 0, st, ctx->d_pq, 8);
     HIPCHK(hipEventRecord(ev[16], st));
-    HIPCHK(hipStreamWaitEvent(st3, ev[16], 0));
-    if (nl > 0)
-      launch_minor_rows(dim3((unsigned)std::min(nl, ctx->adj_blocks)), b64, 0, st, ctx->d_tab,
-                        ctx->d_runs, qL, SD, S, ctx->d_rec, ctx->d_recb, ctx->d_mub,
-                        ctx->d_m0bar, AW, FS, RL, ctx->d_tpart);
-    if (qS.n > 0) {
-      launch_minor_rows(gb, b64, 0, st3, ctx->d_tab, ctx->d_runs, qS, SD, S, ctx->d_rec,
-                        ctx->d_recb, ctx->d_mub, ctx->d_m0bar, AW2, FS, RR, ctx->d_tpart3);
-      launch_minor_pairs(gb, b64, 0, st3, ctx->d_tab, ctx->d_runs, AW2, FS, RR, ctx->d_tpart3);
+    int i0 = 0;
+    for (int g = 0; g < ngroups; g++) {
+      const int i1 = g + 1 < ngroups ? std::max(i0, std::min(qm.n, (int)(gcut[g] * qm.n))) : qm.n;
+      Queue q = qm;
+      q.items = qm.items + i0;
+      q.n = i1 - i0;
+      q.counter = g == 0 ? qm.counter : ctx->d_pq + 5 + g;
+      const RowStore RG{ctx->d_fxs, ctx->d_fxss, ctx->d_units + (size_t)i0 * per,
+                        ctx->d_pq + 2 * g, ctx->d_pq + 2 * g + 1,
+                        std::max(0, std::min(ctx->units_cap - i0 * per, q.n * per))};
+      AdjWork AG = AW;
+      hipStream_t sg = st;
+      double* tg = ctx->d_tpart;
+      if (g > 0) {
+        AG.scr = ctx->d_gscr[g - 1];
+        sg = ctx->gstream[g - 1];
+        tg = ctx->d_gtpart[g - 1];
+        HIPCHK(hipStreamWaitEvent(sg, ev[16], 0));
+      }
+      if (q.n > 0) {
+        launch_minor_rows(dim3((unsigned)std::min(q.n, ctx->adj_blocks)), b64, 0, sg, ctx->d_tab,
+                          ctx->d_runs, q, SD, S, ctx->d_rec, ctx->d_recb, ctx->d_mub,
+                          ctx->d_m0bar, AG, FS, RG, tg);
+        launch_minor_pairs(gb, b64, 0, sg, ctx->d_tab, ctx->d_runs, AG, FS, RG, tg);
+      }
+      HIPCHK(hipGetLastError());
+      if (g > 0) {
+        HIPCHK(hipEventRecord(ev[16 + g], sg));
+      }
+      i0 = i1;
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ev[17], st3));
-    launch_minor_pairs(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, AW, FS, RL, ctx->d_tpart);
-    HIPCHK(hipStreamWaitEvent(st, ev[17], 0));
+    for (int g = 1; g < ngroups; g++) HIPCHK(hipStreamWaitEvent(st, ev[16 + g], 0));
   } else {
     launch_minor_adj(gb, b64, 0, st, ctx->d_tab, ctx->d_runs, qm, SD, S, ctx->d_rec,
                      ctx->d_recb, ctx->d_mub, ctx->d_m0bar, AW, FS, ctx->d_tpart);
@@ -2054,8 +2075,9 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart2,
                     ctx->adj_blocks, ntab, ctx->d_tbar);
   if (split)
-    launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_tpart3,
-                    ctx->adj_blocks, ntab, ctx->d_tbar);
+    for (int g = 0; g + 1 < ngroups; g++)
+      launch_part_sum(dim3((unsigned)((ntab + 3) / 4)), b256, 0, st, ctx->d_gtpart[g],
+                      ctx->adj_blocks, ntab, ctx->d_tbar);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ev[3], st));
   {
@@ -2274,7 +2296,7 @@ extern "C" int ysx_grad_memory(const ysx_ctx* ctx, double* bytes3) {
   const double P = (double)ctx->npair, nk = ctx->M.nknot;
   bytes3[0] = nb * sm * 64 * (14 * sizeof(cd) + sizeof(int)) +        // per-step states, and
               (double)ctx->fs_steps * 64 * (30 * sizeof(cd) + 3 * sizeof(int));  // the forward's
-  const double groups = ctx->d_ascr2 ? 2.0 : 1.0;  // the minor sweep's second item group
+  const double groups = 1.0 + (ctx->d_gscr[0] ? 1 : 0) + (ctx->d_gscr[1] ? 1 : 0);  // minor sweep item groups
   bytes3[1] = groups * nb * 13 * 14 * 64 * sizeof(cd) +                // step scratch,
               (double)ctx->rtape_bytes;                                // radial tape
   bytes3[2] = ctx->d_tbar ? P * (2 * 14 + 2 + REC) * sizeof(cd) +      // per-pair adjoints,
