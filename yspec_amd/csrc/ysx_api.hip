@@ -1871,11 +1871,10 @@ extern "C" int ysx_model_vjp(ysx_ctx* ctx, double* table_bar) {
     HIPCHK(hipMalloc(&ctx->d_aerr, 16 * sizeof(int)));
     HIPCHK(hipMalloc(&ctx->d_aq, 4 * sizeof(int)));
   }
-  // the step scratch and partial table columns of the minor sweep's second
-  // group of items (with the forward's states: its own stream)
   // the minor sweep's item groups with the forward's states: the first
   // eighth of the cost-sorted items, the rest of the first half, the second
-  // half (the ends of the first two)
+  // half (gcut: the ends of the first two); the groups after the first have
+  // their own step scratch and partial table columns
   constexpr int ngroups = 3;
   const double gcut[ngroups - 1] = {0.125, 0.5};
   if (ctx->fs_valid)
