@@ -30,6 +30,10 @@
 //   k_minor_adj  minor vectors: the columns from the start level (stored
 //                side) and the row of the secular determinant and b-vector
 //                starts; writes the adjoint of each pair's start solution
+//   k_minor_rows, k_minor_pairs  the same when the forward stored the minor
+//                states: the rows' sequential chains (row states stored, one
+//                pairing unit per interval), then every unit's step 2 on all
+//                compute units
 //   k_tor_adj    toroidal system, the same way
 //   k_start_adj  starting solutions: forward-mode tangents with respect to
 //                the spline values at the start radius
