@@ -236,6 +236,31 @@ def test_forward_stored_states_match_the_sweep_march(api):
     g.close()
 
 
+def test_stored_states_with_few_sweep_items(api):
+    """The stored-state sweeps cut the minor items into three cost-sorted
+    groups (the first eighth, up to half, the rest); with seven items (lmax
+    24, 7 frequencies) the first group is empty.  The gradient still equals
+    the one whose sweeps march the states themselves, to 1e-12 of the
+    largest entry."""
+    deck = synth_deck(0)
+    p = workload("cfg1", lmax=24, nr=2, ats=1, src_depth=50.0, f2=1.2, f21=0.9, f22=1.1)
+    g = api.YSpec(bumped(deck), p)
+    assert g.nw * ((p.lmax - p.lmin + 1 + 63) // 64) < 8
+    obs = g.forward()
+    g.set_model(deck)
+    g.set_run(p)
+    misfit, grad = g.misfit_grad(obs)
+    m2, db = g.misfit_gradient(obs)
+    grad2 = api.knot_gradient(db)
+    d = float(np.max(np.abs(grad2 - grad)))
+    print(f"nw {g.nw}: misfit {misfit:.9e} / {m2:.9e}, max |stored - marched| {d:.2e} of "
+          f"{np.max(np.abs(grad)):.2e}", flush=True)
+    assert np.max(np.abs(grad)) > 0.0
+    assert abs(m2 - misfit) <= 1e-12 * misfit
+    assert d <= 1e-12 * np.max(np.abs(grad))
+    g.close()
+
+
 def test_autograd_knot_gradient(api):
     """KnotSeismograms (torch.autograd) against ysx_misfit_grad: the same
     misfit through torch, backward through the library's adjoints"""
